@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark: poses/sec of DRR + patch gradient-NCC at the 384^2 detector, batched vs plain.
+
+Workload (CMA-ES population batching, single view): seeded 256x256x200 float32 ellipsoid
+phantom at 1.0 mm, 384x384 detector at 0.776 mm, focal 1020 mm, ray step 0.5 mm, lambda = 100
+poses per generation (sigma 10 deg / 15 mm about the ground truth), patch gradient-NCC
+(11x11 patches, Sobel after the default 5x5 smoothing, mean over patches).
+
+One timed step = one generation: the lambda poses go to the device, one batched submission
+(graph-replayed) casts all DRRs and scores them, lambda similarity values come back. The plain
+leg evaluates poses one at a time (DRR launch, similarity launch, synchronous read-back).
+
+Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits: this build runs on one
+GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--lam", type=int, default=100)
+    ap.add_argument("--plain-poses", type=int, default=30)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--geom", default="fp64", choices=["fp32", "fp64"])
+    ap.add_argument("--smooth", type=int, default=5)
+    return ap.parse_args()
+
+
+def generation_poses(w, lam, n_gen, seed=11):
+    from xreg_amd import synth
+
+    out = []
+    for g in range(n_gen):
+        out.append(synth.random_poses(w.gt, w.center, lam, 10.0, 15.0, seed=seed + g, normal=True))
+    return out
+
+
+def cpu_baseline(w, fixed, poses, budget_s, smooth):
+    """The CPU restatement of RayCasterLineIntCPU + patch gradient-NCC (oracle), all host
+    threads, on as many poses of the same workload as fit in about budget_s seconds."""
+    from oracle import oracle as O
+
+    O.lib()
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    done = 0
+    t_drr = t_sim = 0.0
+    for p in poses:
+        a = time.perf_counter()
+        drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, np.asarray(p, np.float32)[None], step=w.step,
+                           threads=threads)
+        b = time.perf_counter()
+        O.patch_grad_ncc_f32(fixed, drr, smooth_ksize=smooth, threads=threads)
+        c = time.perf_counter()
+        t_drr += b - a
+        t_sim += c - b
+        done += 1
+        if c - t0 >= budget_s and done >= 2:
+            break
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "poses/s", "cores": threads, "kind": "port",
+            "sample": f"{done} poses of the same 384^2 patch-grad-NCC workload, DRR {t_drr:.2f}s + "
+                      f"similarity {t_sim:.2f}s, oracle FP32 restatement, {threads} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    storage = {"f32": X.XRH_F32, "f16": X.XRH_F16, "bf16": X.XRH_BF16}[args.storage]
+    geom = X.XRH_GEOM_FP64 if args.geom == "fp64" else X.XRH_GEOM_FP32
+    lam = args.lam
+    w = synth.config("c2", n_poses=1)
+    cam = w.cams[0]
+    P = cam.num_det_rows * cam.num_det_cols
+
+    ctx = X.Context(0)
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys, storage=storage)
+    # fixed image: DRR at the ground truth + 1% noise, computed on the device
+    buf = ctx.alloc(4 * P)
+    X.line_int(ctx, vol, w.cams, w.gt.astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
+    fixed = synth.noisy_fixed(buf.download(np.float32).reshape(cam.num_det_rows, cam.num_det_cols))
+    fx = X.Fixed(ctx, fixed, X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize=args.smooth, patch_radius=5)
+    batch = X.Batch(ctx, [X.make_view(vol, fx, cam, w.step)], max_items=lam, geom=geom)
+    gens = generation_poses(w, lam, args.warmup + args.steps)
+    iv = np.zeros(lam, np.int32)
+
+    # ---- batched: warmup, then exactly K generations between synchronisations ----
+    for g in range(args.warmup):
+        batch.eval(iv, gens[g])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sims = None
+    for g in range(args.warmup, args.warmup + args.steps):
+        sims = batch.eval(iv, gens[g])
+    ctx.synchronize()
+    t_batched = time.perf_counter() - t0
+    value = args.steps * lam / t_batched
+
+    # ---- DRR kernel duration (HIP events on the library stream, eager submissions) ----
+    batch.set_options(use_graph=False, time_kernels=True)
+    drr_ms = []
+    for g in range(min(5, len(gens))):
+        batch.eval(iv, gens[g])
+        drr_ms.append(batch.last_timing()[0])
+    batch.set_options(use_graph=True, time_kernels=False)
+    drr_ms_avg = float(np.mean(drr_ms[1:])) if len(drr_ms) > 1 else float(drr_ms[0])
+    samples = [batch.sample_count(iv, gens[g]) for g in range(min(5, len(gens)))]
+    S = float(np.mean(samples[1:])) if len(samples) > 1 else float(samples[0])
+    bv = {X.XRH_F32: 4, X.XRH_F16: 2, X.XRH_BF16: 2}[storage]
+    bytes_launch = 8.0 * bv * S + 4.0 * P * lam  # gathers + DRR write
+    achieved = bytes_launch / (drr_ms_avg * 1e-3) / 1e9
+
+    # ---- plain: one pose per launch, synchronous read-back ----
+    n_plain = min(args.plain_poses, lam)
+    pl = gens[0][:n_plain]
+    for i in range(2):
+        X.line_int(ctx, vol, w.cams, pl[i].astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
+        fx.compute(buf, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_plain):
+        X.line_int(ctx, vol, w.cams, pl[i].astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
+        fx.compute(buf, 1)
+    ctx.synchronize()
+    plain_value = n_plain / (time.perf_counter() - t0)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, fixed, gens[0], args.cpu_seconds, args.smooth)
+
+    line = {
+        "metric": "poses/sec (DRR+grad-NCC, 384^2 det) batched vs plain; p50/p99 served regi latency",
+        "value": round(value, 2),
+        "unit": "poses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_batched / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.storage.replace("f32", "fp32"),
+        "data": "synthetic seeded ellipsoid phantom",
+        "config": {"workload": "cmaes-lambda100-384sq-patch-grad-ncc", "volume": "256x256x200@1.0mm",
+                   "detector": "384x384@0.776mm", "step_mm": w.step, "lambda": lam,
+                   "similarity": f"patch-grad-ncc r5 smooth{args.smooth}", "geometry": args.geom,
+                   "storage": args.storage, "parallelism": "single-gpu"},
+        "plain_poses_per_s": round(plain_value, 2),
+        "batched_over_plain": round(value / plain_value, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4),
+                     "samples_per_launch": int(S), "algorithmic_bytes_per_launch": int(bytes_launch)},
+        "cpu_baseline": cpu,
+        "sim_checksum": float(np.sum(sims)),
+    }
+    print(json.dumps(line))
+    for o in (batch, fx, vol):
+        o.close()
+    buf.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

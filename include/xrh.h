@@ -1,0 +1,248 @@
+/*
+ * xrh.h -- C-ABI of the MI355X (gfx950) DRR + image-similarity hot path.
+ *
+ * This is the device layer that replaces xReg's lib/opencl backend (Boost.Compute queues,
+ * OpenCL-C kernel strings, ViennaCL GEMV reductions) underneath the RayCaster and
+ * ImgSimMetric2D interfaces. Conventions:
+ *   - extern "C", plain pointers and sizes; no C++ or torch types cross this boundary;
+ *   - every entry point returns an xrh_status; on failure xrh_last_error() (thread-local)
+ *     holds a message; no exception ever escapes;
+ *   - host arrays are caller-owned and only read during the call; device buffers passed
+ *     in ("_dev") are caller-owned device memory (xrh_device_alloc or any HIP allocation);
+ *   - all device work is ordered on the context's stream; entry points that return host
+ *     results synchronise that stream before returning (the reference's compute() calls
+ *     are synchronous: xregRayCastLineIntOCL.cpp:192-197).
+ * Matrices are row-major 4x4 (affine, last row 0 0 0 1) or row-major 3x3.
+ */
+#ifndef XRH_H
+#define XRH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XRH_ABI_VERSION 1
+
+typedef int32_t xrh_status;
+enum {
+  XRH_OK = 0,
+  XRH_ERR_INVALID = 1,     /* bad argument (xregASSERT / xregThrow in the reference)   */
+  XRH_ERR_HIP = 2,         /* a HIP runtime call failed                                 */
+  XRH_ERR_OOM = 3,         /* device allocation failed                                  */
+  XRH_ERR_BUDGET = 4,      /* request exceeds the context's HBM budget                  */
+  XRH_ERR_UNSUPPORTED = 5, /* UnsupportedOperationException (xregRayCastInterface.h:59) */
+  XRH_ERR_STATE = 6        /* call made in the wrong order (e.g. before allocation)     */
+};
+
+/* Storage type of a resident volume. FP16/BF16 halve the gather bytes; interpolation and
+ * accumulation stay FP32. */
+enum { XRH_F32 = 0, XRH_F16 = 1, XRH_BF16 = 2 };
+
+/* RayCastLineIntKernel (lib/ray_cast/xregRayCastInterface.h:575-579). */
+enum { XRH_LINE_INT_SUM = 0, XRH_LINE_INT_MAX = 1 };
+
+/* RayCaster::ProjPixelStoreMethod (lib/ray_cast/xregRayCastInterface.h:71-75). */
+enum { XRH_STORE_REPLACE = 0, XRH_STORE_ACCUM = 1 };
+
+/* Ray set-up arithmetic.
+ * XRH_GEOM_FP32: RayCasterLineIntCPU's float arithmetic in its operation order with no
+ *   fused multiply-add (xregRayCastLineIntCPU.cpp:204-277), so step counts and sample
+ *   positions match the CPU reference bit for bit; samples advance by repeated addition.
+ * XRH_GEOM_FP64: pose composition and per-ray set-up (entry/exit, chord length, step
+ *   count) in double; samples are placed at start + i*step in float. */
+enum { XRH_GEOM_FP32 = 0, XRH_GEOM_FP64 = 1 };
+
+/* ImgSimMetric2D subclasses on the hot path (lib/regi/sim_metrics_2d/). */
+enum {
+  XRH_SIM_NCC = 0,            /* ImgSimMetric2DNCCCPU            */
+  XRH_SIM_GRAD_NCC = 1,       /* ImgSimMetric2DGradNCCCPU        */
+  XRH_SIM_PATCH_NCC = 2,      /* ImgSimMetric2DPatchNCCCPU       */
+  XRH_SIM_PATCH_GRAD_NCC = 3  /* ImgSimMetric2DPatchGradNCCCPU   */
+};
+
+typedef struct xrh_ctx xrh_ctx;
+typedef struct xrh_volume xrh_volume;
+typedef struct xrh_fixed xrh_fixed;
+typedef struct xrh_batch xrh_batch;
+
+/* Pinhole camera exactly as xReg's CameraModel stores it
+ * (lib/transforms/xregPerspectiveXform.h:108-273): coord_frame is CameraCoordFrame
+ * (0 = origin at focal point, detector on +z; 1 = detector on -z, the default;
+ * 2 = origin on the detector). intrins_inv / extrins_inv / pinhole are the float members
+ * CameraModel::setup computes (xregPerspectiveXform.cpp:230-254). */
+typedef struct xrh_camera {
+  int32_t num_det_rows;
+  int32_t num_det_cols;
+  int32_t coord_frame;
+  float focal_len;
+  float intrins_inv[9];
+  float extrins_inv[16];
+  float pinhole[3];
+  float det_row_spacing;
+  float det_col_spacing;
+} xrh_camera;
+
+/* Similarity-metric parameters. smooth_ksize is
+ * ImgSimMetric2DGradImgParamInterface::smooth_img_before_sobel_kernel_radius (an odd
+ * GaussianBlur aperture 1,3,5,7 or 0 = no smoothing; xregImgSimMetric2DGradImgCPU.cpp:48-55);
+ * patch_radius / patch_stride are ImgSimMetric2DPatchCommon's (defaults 5 and 1,
+ * xregImgSimMetric2DPatchCommon.h:141-143). */
+typedef struct xrh_sim_params {
+  int32_t metric;
+  int32_t smooth_ksize;
+  int32_t patch_radius;
+  int32_t patch_stride;
+} xrh_sim_params;
+
+/* ---------------------------------------------------------------------------------- */
+/* errors, version                                                                      */
+
+/* Message of the last failed call on this thread (empty string if none). */
+const char* xrh_last_error(void);
+int32_t xrh_abi_version(void);
+
+/* ---------------------------------------------------------------------------------- */
+/* context: one HIP device, one stream, an HBM budget for everything it allocates.
+ * Replaces the OpenCL context/queue pair the OCL classes take in their constructors
+ * (xregRayCastBaseOCL.h:38-60) and the device selection of xregOpenCLSys.cpp:31-83. */
+
+xrh_status xrh_ctx_create(int32_t device, uint64_t hbm_budget_bytes, xrh_ctx** out);
+xrh_status xrh_ctx_destroy(xrh_ctx* ctx);
+/* Waits for all work queued on the context's stream. */
+xrh_status xrh_ctx_synchronize(xrh_ctx* ctx);
+/* The context's hipStream_t, for callers that order their own work against it. */
+void* xrh_ctx_stream(xrh_ctx* ctx);
+/* Bytes currently charged to the budget. */
+uint64_t xrh_ctx_bytes_in_use(const xrh_ctx* ctx);
+
+/* Device memory charged to the context budget (DRR buffers, background images...). */
+xrh_status xrh_device_alloc(xrh_ctx* ctx, uint64_t bytes, void** dptr);
+xrh_status xrh_device_free(xrh_ctx* ctx, void* dptr);
+/* Stream-ordered copies; _sync variants wait for completion. */
+xrh_status xrh_copy_h2d(xrh_ctx* ctx, void* dst_dev, const void* src_host, uint64_t bytes);
+xrh_status xrh_copy_d2h(xrh_ctx* ctx, void* dst_host, const void* src_dev, uint64_t bytes);
+
+/* ---------------------------------------------------------------------------------- */
+/* volumes: RayCaster::set_volume / vols_changed (xregRayCastInterface.h:87-90, 425;
+ * the OCL image3d upload of xregRayCastBaseOCL.cpp:446-465).
+ *   vox          x-fastest float voxels (ITK buffer order), size[0]*size[1]*size[2]
+ *   idx_to_phys  ITKImagePhysicalPointTransformsAsEigen (lib/itk/xregITKBasicImageUtils.h:
+ *                131-168) as row-major 4x4 float
+ *   storage      XRH_F32 / XRH_F16 / XRH_BF16
+ * The volume is copied into a device-resident, edge-padded layout; the host array is not
+ * retained. */
+xrh_status xrh_volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
+                             const float idx_to_phys[16], int32_t storage, xrh_volume** out);
+xrh_status xrh_volume_destroy(xrh_volume* vol);
+/* Same, with an explicit HBM layout (0 = padded linear, 1 = quad-packed FP32 corners,
+ * 2 = oct-packed 16-bit corners); xrh_volume_create picks 1 for FP32 and 2 otherwise
+ * (XRH_VOLUME_LAYOUT=linear in the environment selects 0). Used by the layout tuner. */
+xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_t size[3],
+                                    const float idx_to_phys[16], int32_t storage,
+                                    int32_t layout, xrh_volume** out);
+/* Device bytes held by the volume. */
+uint64_t xrh_volume_bytes(const xrh_volume* vol);
+
+/* ---------------------------------------------------------------------------------- */
+/* line-integral ray casting: RayCasterLineIntCPU::compute(vol_idx)
+ * (lib/ray_cast/xregRayCastLineIntCPU.cpp:294-349) including RayCasterCPU::pre_compute
+ * (xregRayCastBaseCPU.cpp:128-158).
+ *   cams, num_cams          camera models (host); all must share num_det_rows/cols
+ *                           (xregRayCastBaseCPU.cpp:54-70)
+ *   xforms_cam_to_phys      num_projs row-major 4x4 poses (host), xforms_cam_to_itk_phys_
+ *   cam_for_proj            camera index of each projection (host), cam_model_for_proj_
+ *   step                    ray_step_size_ (physical units)
+ *   kernel                  XRH_LINE_INT_SUM / _MAX
+ *   store                   XRH_STORE_REPLACE (pre-fill) / XRH_STORE_ACCUM (add into drr_dev)
+ *   bg_dev                  NULL, or num_cams background images on the device, copied into
+ *                           each projection before casting (use_bg_projs_)
+ *   default_bg              default_bg_pixel_val_ used by REPLACE without bg images
+ *   geom                    XRH_GEOM_FP32 / XRH_GEOM_FP64
+ *   drr_dev                 [num_projs][rows][cols] float on the device
+ * Returns once the work is queued on the context stream (not synchronised). */
+xrh_status xrh_line_int(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* cams,
+                        int32_t num_cams, const float* xforms_cam_to_phys,
+                        const int64_t* cam_for_proj, int64_t num_projs, float step,
+                        int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
+                        int32_t geom, float* drr_dev);
+
+/* Exact number of trilinear samples the same call would take (sum over rays of n+1,
+ * the S_pose of the roofline byte model). Synchronous. */
+xrh_status xrh_line_int_sample_count(xrh_ctx* ctx, const xrh_volume* vol,
+                                     const xrh_camera* cams, int32_t num_cams,
+                                     const float* xforms_cam_to_phys,
+                                     const int64_t* cam_for_proj, int64_t num_projs,
+                                     float step, int32_t geom, uint64_t* samples_out);
+
+/* ---------------------------------------------------------------------------------- */
+/* fixed images + similarity: ImgSimMetric2D::set_fixed_image + allocate_resources
+ * (xregImgSimMetric2D.h:69-81) process the fixed image once (zero-mean and std for NCC;
+ * Sobel gradients for Grad-NCC; per-patch statistics for the patch metrics); compute()
+ * (xregImgSimMetric2D.h:88) evaluates num_mov moving images stored contiguously on the
+ * device. mask: optional host uint8 image (non-zero = use), NCC only
+ * (xregImgSimMetric2DNCCCPU.cpp:186-198); NULL otherwise. */
+xrh_status xrh_fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, int32_t cols,
+                            const xrh_sim_params* params, const uint8_t* mask_host,
+                            xrh_fixed** out);
+xrh_status xrh_fixed_destroy(xrh_fixed* fixed);
+
+/* sim_host_out: num_mov floats (synchronises); sim_dev_out: optional device destination
+ * (stream-ordered). At least one must be non-NULL. The moving images are not modified
+ * (unlike the reference NCC, which zero-means them in place: xregImgSimMetric2DNCCCPU.h). */
+xrh_status xrh_sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev,
+                           int64_t num_mov, float* sim_dev_out, float* sim_host_out);
+
+/* ---------------------------------------------------------------------------------- */
+/* batched evaluation: one optimiser generation (all poses x all views, any mix of
+ * volumes, detector sizes and fixed images) as one fused DRR + similarity pass, replayed
+ * from a cached hipGraph when the shape repeats. This is what Intensity2D3DRegi::obj_fn
+ * (xregIntensity2D3DRegi.cpp:571-696) does per call through ray_caster_->compute() and
+ * sim_metrics_[v]->compute(), collapsed into one submission.
+ *
+ * A view binds a resident volume, a camera, a processed fixed image, the ray step and the
+ * line-integral kernel. An item is (view index, FP64 row-major cam->phys pose). */
+typedef struct xrh_view {
+  const xrh_volume* vol;
+  const xrh_fixed* fixed;
+  xrh_camera cam;
+  float step;
+  int32_t kernel;
+} xrh_view;
+
+xrh_status xrh_batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views,
+                            int64_t max_items, int32_t geom, xrh_batch** out);
+xrh_status xrh_batch_destroy(xrh_batch* batch);
+
+/* use_graph: capture / replay hipGraphs per item->view signature (default 1).
+ * time_kernels: submit eagerly with HIP events around the DRR kernel and the whole
+ * submission, readable through xrh_batch_last_timing (default 0). */
+xrh_status xrh_batch_set_options(xrh_batch* batch, int32_t use_graph, int32_t time_kernels);
+
+/* Evaluates n items. sim_host_out receives n similarity values (synchronises).
+ * drr_dev_out: optional device buffer receiving every item's DRR back to back (in item
+ * order, each rows*cols of its view); NULL skips the copy-out when the metric allows. */
+xrh_status xrh_batch_eval(xrh_batch* batch, const int32_t* item_view, const double* item_pose,
+                          int64_t n, float* sim_host_out, float* drr_dev_out);
+
+/* Asynchronous form for the serving loop: queues the same work and returns; the values
+ * land in sim_host_out (pinned by the batch) once xrh_batch_wait returns. */
+xrh_status xrh_batch_launch(xrh_batch* batch, const int32_t* item_view,
+                            const double* item_pose, int64_t n);
+xrh_status xrh_batch_wait(xrh_batch* batch, float* sim_host_out);
+
+/* Per-launch timing of the last xrh_batch_eval / launch on the context stream, measured
+ * with HIP events around the DRR kernel and around the whole submission (milliseconds). */
+xrh_status xrh_batch_last_timing(const xrh_batch* batch, float* drr_kernel_ms, float* total_ms);
+
+/* Exact trilinear sample count of the last evaluated batch (computed on request). */
+xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view,
+                                  const double* item_pose, int64_t n, uint64_t* samples_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XRH_H */

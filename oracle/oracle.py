@@ -1,0 +1,284 @@
+"""TEST INFRASTRUCTURE ONLY: numpy wrappers of liboracle.so (the CPU restatement of the
+reference's DRR + similarity path, see xrh_oracle.h). Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg load this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+class OrcCamF(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("frame", C.c_int32), ("focal", C.c_float),
+                ("kinv", C.c_float * 9), ("einv", C.c_float * 16), ("pinhole", C.c_float * 3)]
+
+
+class OrcCamD(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("frame", C.c_int32), ("pad_", C.c_int32),
+                ("focal", C.c_double), ("kinv", C.c_double * 9), ("einv", C.c_double * 16),
+                ("pinhole", C.c_double * 3)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def cam_f(cam) -> OrcCamF:
+    """From an xrh Camera (same float members)."""
+    o = OrcCamF()
+    o.rows, o.cols, o.frame, o.focal = cam.num_det_rows, cam.num_det_cols, cam.coord_frame, cam.focal_len
+    for i in range(9):
+        o.kinv[i] = cam.intrins_inv[i]
+    for i in range(16):
+        o.einv[i] = cam.extrins_inv[i]
+    for i in range(3):
+        o.pinhole[i] = cam.pinhole[i]
+    return o
+
+
+def cam_d(cam) -> OrcCamD:
+    o = OrcCamD()
+    o.rows, o.cols, o.frame, o.focal = cam.num_det_rows, cam.num_det_cols, cam.coord_frame, cam.focal_len
+    for i in range(9):
+        o.kinv[i] = cam.intrins_inv[i]
+    for i in range(16):
+        o.einv[i] = cam.extrins_inv[i]
+    for i in range(3):
+        o.pinhole[i] = cam.pinhole[i]
+    return o
+
+
+def cam_naive(focal, rows, cols, row_sp, col_sp) -> OrcCamF:
+    o = OrcCamF()
+    lib().orc_cam_naive_f32(C.c_float(focal), rows, cols, C.c_float(row_sp), C.c_float(col_sp), C.byref(o))
+    return o
+
+
+def p2i_f32(idx_to_phys: np.ndarray) -> np.ndarray:
+    m = np.ascontiguousarray(idx_to_phys, dtype=np.float32).reshape(16)
+    out = np.zeros(16, dtype=np.float32)
+    lib().orc_affine_inverse_f32(_f(m), _f(out))
+    return out
+
+
+def p2i_f64(idx_to_phys: np.ndarray) -> np.ndarray:
+    m = np.ascontiguousarray(idx_to_phys, dtype=np.float64).reshape(16)
+    out = np.zeros(16, dtype=np.float64)
+    lib().orc_affine_inverse_f64(_d(m), _d(out))
+    return out
+
+
+def drr_f32(vol_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, kernel=0, store=0, bg=None,
+            default_bg=0.0, out=None, threads=0):
+    """RayCasterLineIntCPU restatement. Returns (drr [n, rows, cols] float32, samples)."""
+    vol = np.ascontiguousarray(vol_zyx, dtype=np.float32)
+    dims = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], dtype=np.int64)
+    x = np.ascontiguousarray(np.asarray(xforms, dtype=np.float32).reshape(-1, 16))
+    n = x.shape[0]
+    c4p = np.zeros(n, dtype=np.int64) if cam_for_proj is None else np.ascontiguousarray(cam_for_proj, np.int64)
+    ca = (OrcCamF * len(cams))(*[cam_f(c) for c in cams])
+    rows, cols = cams[0].num_det_rows, cams[0].num_det_cols
+    if out is None:
+        out = np.zeros((n, rows, cols), dtype=np.float32)
+    bgp = None
+    if bg is not None:
+        bga = np.ascontiguousarray(bg, dtype=np.float32)
+        bgp = _f(bga)
+    s = C.c_int64(0)
+    p2i = p2i_f32(idx_to_phys)
+    lib().orc_drr_f32(_f(vol), _i64(dims), _f(p2i), ca, len(cams), _f(x), _i64(c4p), C.c_int64(n),
+                      C.c_float(step), kernel, store, bgp, C.c_float(default_bg), _f(out), threads, C.byref(s))
+    return out, int(s.value)
+
+
+def drr_f64(vol_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, kernel=0, store=0, bg=None,
+            default_bg=0.0, out=None, threads=0):
+    """FP64 oracle (FP64 volume, geometry and accumulation). xforms may be float64."""
+    vol = np.ascontiguousarray(vol_zyx, dtype=np.float64)
+    dims = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], dtype=np.int64)
+    x = np.ascontiguousarray(np.asarray(xforms, dtype=np.float64).reshape(-1, 16))
+    n = x.shape[0]
+    c4p = np.zeros(n, dtype=np.int64) if cam_for_proj is None else np.ascontiguousarray(cam_for_proj, np.int64)
+    ca = (OrcCamD * len(cams))(*[cam_d(c) for c in cams])
+    rows, cols = cams[0].num_det_rows, cams[0].num_det_cols
+    if out is None:
+        out = np.zeros((n, rows, cols), dtype=np.float64)
+    bgp = None
+    if bg is not None:
+        bga = np.ascontiguousarray(bg, dtype=np.float64)
+        bgp = _d(bga)
+    s = C.c_int64(0)
+    p2i = p2i_f64(np.asarray(idx_to_phys, dtype=np.float64))
+    lib().orc_drr_f64(_d(vol), _i64(dims), _d(p2i), ca, len(cams), _d(x), _i64(c4p), C.c_int64(n),
+                      C.c_double(step), kernel, store, bgp, C.c_double(default_bg), _d(out), threads, C.byref(s))
+    return out, int(s.value)
+
+
+def ncc_f32(fixed, mov, mask=None):
+    f = np.ascontiguousarray(fixed, dtype=np.float32)
+    m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, f.size)
+    out = np.zeros(m.shape[0], dtype=np.float32)
+    mk = None
+    if mask is not None:
+        mka = np.ascontiguousarray(mask, dtype=np.uint8)
+        mk = mka.ctypes.data_as(C.POINTER(C.c_uint8))
+    lib().orc_ncc_f32(_f(f), _f(m), C.c_int64(f.size), C.c_int64(m.shape[0]), mk, _f(out))
+    return out
+
+
+def ncc_f64(fixed, mov, mask=None):
+    f = np.ascontiguousarray(fixed, dtype=np.float64)
+    m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, f.size)
+    out = np.zeros(m.shape[0], dtype=np.float64)
+    mk = None
+    if mask is not None:
+        mka = np.ascontiguousarray(mask, dtype=np.uint8)
+        mk = mka.ctypes.data_as(C.POINTER(C.c_uint8))
+    lib().orc_ncc_f64(_d(f), _d(m), C.c_int64(f.size), C.c_int64(m.shape[0]), mk, _d(out))
+    return out
+
+
+def sobel_f32(img, smooth_ksize=0):
+    im = np.ascontiguousarray(img, dtype=np.float32)
+    gx = np.zeros_like(im)
+    gy = np.zeros_like(im)
+    lib().orc_sobel_f32(_f(im), im.shape[0], im.shape[1], smooth_ksize, _f(gx), _f(gy))
+    return gx, gy
+
+
+def sobel_f64(img, smooth_ksize=0):
+    im = np.ascontiguousarray(img, dtype=np.float64)
+    gx = np.zeros_like(im)
+    gy = np.zeros_like(im)
+    lib().orc_sobel_f64(_d(im), im.shape[0], im.shape[1], smooth_ksize, _d(gx), _d(gy))
+    return gx, gy
+
+
+def grad_ncc_f32(fixed, mov, smooth_ksize=5):
+    f = np.ascontiguousarray(fixed, dtype=np.float32)
+    m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float32)
+    lib().orc_grad_ncc_f32(_f(f), _f(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), smooth_ksize, _f(out))
+    return out
+
+
+def grad_ncc_f64(fixed, mov, smooth_ksize=5):
+    f = np.ascontiguousarray(fixed, dtype=np.float64)
+    m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float64)
+    lib().orc_grad_ncc_f64(_d(f), _d(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), smooth_ksize, _d(out))
+    return out
+
+
+def patch_ncc_f32(fixed, mov, radius=5, stride=1, threads=0):
+    f = np.ascontiguousarray(fixed, dtype=np.float32)
+    m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float32)
+    lib().orc_patch_ncc_f32(_f(f), _f(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius, stride,
+                            _f(out), threads)
+    return out
+
+
+def patch_ncc_f64(fixed, mov, radius=5, stride=1, threads=0):
+    f = np.ascontiguousarray(fixed, dtype=np.float64)
+    m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float64)
+    lib().orc_patch_ncc_f64(_d(f), _d(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius, stride,
+                            _d(out), threads)
+    return out
+
+
+def patch_grad_ncc_f32(fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):
+    f = np.ascontiguousarray(fixed, dtype=np.float32)
+    m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float32)
+    lib().orc_patch_grad_ncc_f32(_f(f), _f(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), smooth_ksize,
+                                 radius, stride, _f(out), threads)
+    return out
+
+
+def patch_grad_ncc_f64(fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):
+    f = np.ascontiguousarray(fixed, dtype=np.float64)
+    m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float64)
+    lib().orc_patch_grad_ncc_f64(_d(f), _d(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), smooth_ksize,
+                                 radius, stride, _d(out), threads)
+    return out
+
+
+def sim_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):
+    if metric == 0:
+        return ncc_f32(fixed, mov)
+    if metric == 1:
+        return grad_ncc_f32(fixed, mov, smooth_ksize)
+    if metric == 2:
+        return patch_ncc_f32(fixed, mov, radius, stride, threads)
+    return patch_grad_ncc_f32(fixed, mov, smooth_ksize, radius, stride, threads)
+
+
+def sim_f64(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):
+    if metric == 0:
+        return ncc_f64(fixed, mov)
+    if metric == 1:
+        return grad_ncc_f64(fixed, mov, smooth_ksize)
+    if metric == 2:
+        return patch_ncc_f64(fixed, mov, radius, stride, threads)
+    return patch_grad_ncc_f64(fixed, mov, smooth_ksize, radius, stride, threads)
+
+
+def itk_interp(vol_zyx, idx):
+    vol = np.ascontiguousarray(vol_zyx, dtype=np.float32)
+    dims = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], dtype=np.int64)
+    p = np.ascontiguousarray(idx, dtype=np.float32)
+    lib().orc_itk_linear_interp.restype = C.c_double
+    return float(lib().orc_itk_linear_interp(_f(vol), _i64(dims), _f(p)))
+
+
+def ray_rect_f32(mn, mx, p0, d, limit=True):
+    a = [np.ascontiguousarray(v, dtype=np.float32) for v in (mn, mx, p0, d)]
+    t0, t1 = C.c_float(0), C.c_float(0)
+    hit = lib().orc_ray_rect_f32(_f(a[0]), _f(a[1]), _f(a[2]), _f(a[3]), int(limit), C.byref(t0), C.byref(t1))
+    return bool(hit), float(t0.value), float(t1.value)
+
+
+def eigen_sum_f32(x):
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    lib().orc_eigen_sum_f32.restype = C.c_float
+    return float(lib().orc_eigen_sum_f32(_f(a), C.c_int64(a.size)))
+
+
+def cam_ind_to_det_f32(ocam: OrcCamF, col, row):
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_cam_ind_to_det_f32(C.byref(ocam), C.c_float(col), C.c_float(row), _f(out))
+    return out

@@ -1,0 +1,1309 @@
+/*
+ * xrh_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * CPU restatement of the reference's DRR + 2-D similarity hot path. Every function
+ * cites the reference file:line it follows. Build: see oracle/Makefile (-O2
+ * -ffp-contract=off so that no multiply-add is fused, matching the reference's
+ * x86-64 SSE2 build where Eigen emits separate mul/add instructions).
+ *
+ * Where the reference's arithmetic lives in third-party code that is absent here
+ * (ITK 5.1.1, Eigen 3.3.4, OpenCV 3.4.12), the published algorithm of that pinned
+ * version is restated and the spot is marked "parity unpinned" -- its exact
+ * rounding cannot be checked against the real library in this environment.
+ */
+#include "xrh_oracle.h"
+
+#include <math.h>
+#include <float.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+/* ------------------------------------------------------------------------------------ */
+/* threading: blocked ranges handed out by an atomic counter (TBB parallel_for analogue,
+ * lib/common/xregTBBUtils.h:93-101). Per-item arithmetic is sequential, so results do not
+ * depend on the partition. */
+
+typedef void (*orc_range_fn)(void* ctx, int64_t begin, int64_t end);
+
+typedef struct {
+  orc_range_fn fn;
+  void* ctx;
+  int64_t n;
+  int64_t grain;
+  int64_t next;
+} orc_pool_job;
+
+static void* orc_pool_worker(void* arg)
+{
+  orc_pool_job* job = (orc_pool_job*)arg;
+  for (;;) {
+    const int64_t b = __atomic_fetch_add(&job->next, job->grain, __ATOMIC_RELAXED);
+    if (b >= job->n) break;
+    const int64_t e = (b + job->grain < job->n) ? (b + job->grain) : job->n;
+    job->fn(job->ctx, b, e);
+  }
+  return NULL;
+}
+
+static void orc_parallel_for(orc_range_fn fn, void* ctx, int64_t n, int64_t grain, int32_t threads)
+{
+  if (n <= 0) return;
+  if (threads <= 0) {
+    long hw = sysconf(_SC_NPROCESSORS_ONLN);
+    threads = (hw > 0) ? (int32_t)hw : 1;
+  }
+  if (grain < 1) grain = 1;
+  orc_pool_job job = {fn, ctx, n, grain, 0};
+  if (threads == 1 || n <= grain) {
+    orc_pool_worker(&job);
+    return;
+  }
+  pthread_t* tids = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+  int32_t started = 0;
+  for (int32_t t = 1; t < threads; ++t) {
+    if (pthread_create(&tids[t], NULL, orc_pool_worker, &job) == 0) {
+      ++started;
+    } else {
+      tids[t] = 0;
+    }
+  }
+  orc_pool_worker(&job);
+  for (int32_t t = 1; t < threads; ++t) {
+    if (tids[t]) pthread_join(tids[t], NULL);
+  }
+  (void)started;
+  free(tids);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Eigen 3.3.4 small fixed-size arithmetic as the reference's float types use it
+ * (lib/common/xregCommon.h:44-65: CoordScalar = float, FrameTransform = Affine3f).
+ * Fixed-size 3x3 products are coefficient-based: res(i) = (a_i0*b0 + a_i1*b1) + a_i2*b2.
+ * parity unpinned: Eigen is absent, the order is the library's documented lazy product. */
+
+static inline float dot3f(const float* row, int stride, const float v[3])
+{
+  return (row[0] * v[0] + row[stride] * v[1]) + row[2 * stride] * v[2];
+}
+
+void orc_affine_pt_f32(const float m[16], const float p[3], float out[3])
+{
+  /* Transform<float,3,Affine> * Vector3f = linear()*p + translation()
+   * (Eigen/src/Geometry/Transform.h, transform_right_product_impl). */
+  float r[3];
+  for (int i = 0; i < 3; ++i) {
+    r[i] = dot3f(&m[4 * i], 1, p) + m[4 * i + 3];
+  }
+  out[0] = r[0];
+  out[1] = r[1];
+  out[2] = r[2];
+}
+
+static inline void affine_pt_f64(const double m[16], const double p[3], double out[3])
+{
+  double r[3];
+  for (int i = 0; i < 3; ++i) {
+    r[i] = (m[4 * i] * p[0] + m[4 * i + 1] * p[1]) + m[4 * i + 2] * p[2] + m[4 * i + 3];
+  }
+  out[0] = r[0];
+  out[1] = r[1];
+  out[2] = r[2];
+}
+
+static inline void linear_vec_f32(const float m[16], const float v[3], float out[3])
+{
+  float r[3];
+  for (int i = 0; i < 3; ++i) r[i] = dot3f(&m[4 * i], 1, v);
+  out[0] = r[0];
+  out[1] = r[1];
+  out[2] = r[2];
+}
+
+void orc_affine_mul_f32(const float a[16], const float b[16], float out[16])
+{
+  /* Affine * Affine: linear = La*Lb, translation = La*tb + ta, last row [0 0 0 1]
+   * (Eigen transform_transform_product_impl). */
+  float r[16];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      r[4 * i + j] = (a[4 * i] * b[j] + a[4 * i + 1] * b[4 + j]) + a[4 * i + 2] * b[8 + j];
+    }
+    r[4 * i + 3] = ((a[4 * i] * b[3] + a[4 * i + 1] * b[7]) + a[4 * i + 2] * b[11]) + a[4 * i + 3];
+  }
+  r[12] = 0.0f;
+  r[13] = 0.0f;
+  r[14] = 0.0f;
+  r[15] = 1.0f;
+  memcpy(out, r, sizeof(r));
+}
+
+static void affine_mul_f64(const double a[16], const double b[16], double out[16])
+{
+  double r[16];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      r[4 * i + j] = (a[4 * i] * b[j] + a[4 * i + 1] * b[4 + j]) + a[4 * i + 2] * b[8 + j];
+    }
+    r[4 * i + 3] = ((a[4 * i] * b[3] + a[4 * i + 1] * b[7]) + a[4 * i + 2] * b[11]) + a[4 * i + 3];
+  }
+  r[12] = 0.0;
+  r[13] = 0.0;
+  r[14] = 0.0;
+  r[15] = 1.0;
+  memcpy(out, r, sizeof(r));
+}
+
+void orc_mat3_inverse_f32(const float m[9], float out[9])
+{
+  /* Eigen compute_inverse<...,3>: cofactors of column 0, det = sum(cof .* col0),
+   * invdet = 1/det, every entry = cofactor * invdet (Eigen/src/LU/InverseImpl.h).
+   * cofactor(i,j) uses rows i+1, i+2 and cols j+1, j+2 (mod 3); entry (j,i) = cof(i,j). */
+#define M3(r, c) m[3 * (r) + (c)]
+#define COF(i, j) (M3(((i) + 1) % 3, ((j) + 1) % 3) * M3(((i) + 2) % 3, ((j) + 2) % 3) - \
+                   M3(((i) + 1) % 3, ((j) + 2) % 3) * M3(((i) + 2) % 3, ((j) + 1) % 3))
+  const float c00 = COF(0, 0);
+  const float c10 = COF(1, 0);
+  const float c20 = COF(2, 0);
+  /* (cofactors_col0.cwiseProduct(matrix.col(0))).sum(): 3 elements, linear redux */
+  const float det = (c00 * M3(0, 0) + c10 * M3(1, 0)) + c20 * M3(2, 0);
+  const float invdet = 1.0f / det;
+  float r[9];
+  r[0] = c00 * invdet;
+  r[1] = c10 * invdet;
+  r[2] = c20 * invdet;
+  r[3] = COF(0, 1) * invdet;
+  r[4] = COF(1, 1) * invdet;
+  r[5] = COF(2, 1) * invdet;
+  r[6] = COF(0, 2) * invdet;
+  r[7] = COF(1, 2) * invdet;
+  r[8] = COF(2, 2) * invdet;
+#undef COF
+#undef M3
+  memcpy(out, r, sizeof(r));
+}
+
+static void mat3_inverse_f64(const double m[9], double out[9])
+{
+#define M3(r, c) m[3 * (r) + (c)]
+#define COF(i, j) (M3(((i) + 1) % 3, ((j) + 1) % 3) * M3(((i) + 2) % 3, ((j) + 2) % 3) - \
+                   M3(((i) + 1) % 3, ((j) + 2) % 3) * M3(((i) + 2) % 3, ((j) + 1) % 3))
+  const double c00 = COF(0, 0), c10 = COF(1, 0), c20 = COF(2, 0);
+  const double det = (c00 * M3(0, 0) + c10 * M3(1, 0)) + c20 * M3(2, 0);
+  const double invdet = 1.0 / det;
+  double r[9] = {c00 * invdet, c10 * invdet, c20 * invdet,
+                 COF(0, 1) * invdet, COF(1, 1) * invdet, COF(2, 1) * invdet,
+                 COF(0, 2) * invdet, COF(1, 2) * invdet, COF(2, 2) * invdet};
+#undef COF
+#undef M3
+  memcpy(out, r, sizeof(r));
+}
+
+void orc_affine_inverse_f64(const double m[16], double out[16])
+{
+  double lin[9] = {m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]};
+  double inv[9];
+  mat3_inverse_f64(lin, inv);
+  double r[16];
+  for (int i = 0; i < 3; ++i) {
+    r[4 * i + 0] = inv[3 * i + 0];
+    r[4 * i + 1] = inv[3 * i + 1];
+    r[4 * i + 2] = inv[3 * i + 2];
+    r[4 * i + 3] = -((inv[3 * i] * m[3] + inv[3 * i + 1] * m[7]) + inv[3 * i + 2] * m[11]);
+  }
+  r[12] = 0.0;
+  r[13] = 0.0;
+  r[14] = 0.0;
+  r[15] = 1.0;
+  memcpy(out, r, sizeof(r));
+}
+
+void orc_affine_inverse_f32(const float m[16], float out[16])
+{
+  /* Transform<float,3,Affine>::inverse(): linear part via the 3x3 inverse, translation
+   * = -(Linv * t) (Eigen/src/Geometry/Transform.h, inverse()). */
+  float lin[9] = {m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]};
+  float inv[9];
+  orc_mat3_inverse_f32(lin, inv);
+  float r[16];
+  for (int i = 0; i < 3; ++i) {
+    r[4 * i + 0] = inv[3 * i + 0];
+    r[4 * i + 1] = inv[3 * i + 1];
+    r[4 * i + 2] = inv[3 * i + 2];
+    const float t = (inv[3 * i] * m[3] + inv[3 * i + 1] * m[7]) + inv[3 * i + 2] * m[11];
+    r[4 * i + 3] = -t;
+  }
+  r[12] = 0.0f;
+  r[13] = 0.0f;
+  r[14] = 0.0f;
+  r[15] = 1.0f;
+  memcpy(out, r, sizeof(r));
+}
+
+void orc_cam_naive_f32(float focal, int32_t rows, int32_t cols, float row_sp, float col_sp,
+                       orc_cam_f* out)
+{
+  /* CameraModel::setup(f, nr, nc, rs, cs) (lib/transforms/xregPerspectiveXform.cpp:230-254)
+   * with MakeNaiveIntrins (:200-228), NEG_Z frame (default, xregPerspectiveXform.h:173). */
+  float K[9] = {0};
+  K[0] = focal / col_sp;
+  K[4] = focal / row_sp;
+  K[8] = 1.0f;
+  K[0] *= -1.0f;
+  K[4] *= -1.0f;
+  K[2] = (float)((double)(cols - 1) * 0.5);
+  K[5] = (float)((double)(rows - 1) * 0.5);
+  memset(out, 0, sizeof(*out));
+  out->rows = rows;
+  out->cols = cols;
+  out->frame = 1;
+  out->focal = focal;
+  orc_mat3_inverse_f32(K, out->kinv);
+  out->einv[0] = out->einv[5] = out->einv[10] = out->einv[15] = 1.0f;
+  out->pinhole[0] = out->pinhole[1] = out->pinhole[2] = 0.0f;
+}
+
+void orc_cam_ind_to_det_f32(const orc_cam_f* cam, float col, float row, float out[3])
+{
+  /* CameraModel::ind_pt_to_phys_det_pt (xregPerspectiveXform.cpp:391-414):
+   *   extrins_inv * ((intrins_inv * (det_z * [c, r, 1])) + z_adjust) */
+  const float det_z = ((cam->frame == 1) ? -1.0f : 1.0f) * cam->focal;
+  const float q[3] = {det_z * col, det_z * row, det_z * 1.0f};
+  float r[3];
+  for (int i = 0; i < 3; ++i) r[i] = dot3f(&cam->kinv[3 * i], 1, q);
+  const float zadj = (cam->frame == 2) ? -cam->focal : 0.0f;
+  r[0] = r[0] + 0.0f;
+  r[1] = r[1] + 0.0f;
+  r[2] = r[2] + zadj;
+  orc_affine_pt_f32(cam->einv, r, out);
+}
+
+void orc_cam_ind_to_det_f64(const orc_cam_d* cam, double col, double row, double out[3])
+{
+  const double det_z = ((cam->frame == 1) ? -1.0 : 1.0) * cam->focal;
+  const double q[3] = {det_z * col, det_z * row, det_z};
+  double r[3];
+  for (int i = 0; i < 3; ++i) {
+    r[i] = (cam->kinv[3 * i] * q[0] + cam->kinv[3 * i + 1] * q[1]) + cam->kinv[3 * i + 2] * q[2];
+  }
+  if (cam->frame == 2) r[2] -= cam->focal;
+  affine_pt_f64(cam->einv, r, out);
+}
+
+int32_t orc_ray_rect_f32(const float mn[3], const float mx[3], const float p0[3],
+                         const float d[3], int32_t limit_to_segment, float* t0_out, float* t1_out)
+{
+  /* RayRectIntersect (lib/spatial/xregSpatialPrimitives.cpp:175-222), slab method. */
+  int32_t hit = 1;
+  float t_start = 0.0f;
+  float t_stop = limit_to_segment ? 1.0f : INFINITY;
+  for (int k = 0; k < 3; ++k) {
+    if ((double)fabsf(d[k]) > 1.0e-8) { /* float component vs the double literal 1.0e-8 */
+      const float inv = 1.0f / d[k];
+      float ta = (mn[k] - p0[k]) * inv;
+      float tb = (mx[k] - p0[k]) * inv;
+      if (tb < ta) {
+        const float tmp = ta;
+        ta = tb;
+        tb = tmp;
+      }
+      t_start = (t_start > ta) ? t_start : ta; /* std::max(t_start, t[0]) */
+      t_stop = (tb < t_stop) ? tb : t_stop;    /* std::min(t_stop, t[1]) */
+      if (t_start > t_stop) {
+        hit = 0;
+        break;
+      }
+    } else if ((p0[k] < mn[k]) || (p0[k] > mx[k])) {
+      hit = 0;
+      break;
+    }
+  }
+  *t0_out = t_start;
+  *t1_out = t_stop;
+  return hit;
+}
+
+int32_t orc_ray_rect_f64(const double mn[3], const double mx[3], const double p0[3],
+                         const double d[3], int32_t limit_to_segment, double* t0_out, double* t1_out)
+{
+  int32_t hit = 1;
+  double t_start = 0.0;
+  double t_stop = limit_to_segment ? 1.0 : INFINITY;
+  for (int k = 0; k < 3; ++k) {
+    if (fabs(d[k]) > 1.0e-8) {
+      const double inv = 1.0 / d[k];
+      double ta = (mn[k] - p0[k]) * inv;
+      double tb = (mx[k] - p0[k]) * inv;
+      if (tb < ta) {
+        const double tmp = ta;
+        ta = tb;
+        tb = tmp;
+      }
+      t_start = (t_start > ta) ? t_start : ta;
+      t_stop = (tb < t_stop) ? tb : t_stop;
+      if (t_start > t_stop) {
+        hit = 0;
+        break;
+      }
+    } else if ((p0[k] < mn[k]) || (p0[k] > mx[k])) {
+      hit = 0;
+      break;
+    }
+  }
+  *t0_out = t_start;
+  *t1_out = t_stop;
+  return hit;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Trilinear interpolation as itk::LinearInterpolateImageFunction<Image<float,3>, float>
+ * evaluates it (called at lib/ray_cast/xregRayCastLineIntCPU.cpp:273-274):
+ * base = floor(index) clamped below at 0, distance = index - base in float (the
+ * coordinate type), values and lerps in double (RealType of float), x then y then z;
+ * a dimension whose distance is <= 0, or whose upper neighbour is past the end, is not
+ * interpolated (which equals clamping the neighbour to the base voxel). The result is
+ * rounded to float when the reference passes it to the line-integral kernel.
+ * parity unpinned: ITK is absent; this follows the published 5.1 EvaluateOptimized. */
+double orc_itk_linear_interp(const float* vol, const int64_t dims[3], const float idx[3])
+{
+  int64_t b[3], u[3];
+  double w[3];
+  for (int k = 0; k < 3; ++k) {
+    int64_t bk = (int64_t)floorf(idx[k]);
+    if (bk < 0) bk = 0;
+    if (bk > dims[k] - 1) bk = dims[k] - 1; /* guard; the 1e-3 nudge keeps this unreached */
+    const float dk = idx[k] - (float)bk;
+    b[k] = bk;
+    u[k] = (bk + 1 > dims[k] - 1) ? bk : bk + 1;
+    w[k] = (dk > 0.0f) ? (double)dk : 0.0;
+  }
+  const int64_t X = dims[0], XY = dims[0] * dims[1];
+#define V(i, j, k) ((double)vol[(i) + X * (j) + XY * (k)])
+  const double v000 = V(b[0], b[1], b[2]), v100 = V(u[0], b[1], b[2]);
+  const double v010 = V(b[0], u[1], b[2]), v110 = V(u[0], u[1], b[2]);
+  const double v001 = V(b[0], b[1], u[2]), v101 = V(u[0], b[1], u[2]);
+  const double v011 = V(b[0], u[1], u[2]), v111 = V(u[0], u[1], u[2]);
+#undef V
+  const double x00 = v000 + (v100 - v000) * w[0];
+  const double x10 = v010 + (v110 - v010) * w[0];
+  const double x01 = v001 + (v101 - v001) * w[0];
+  const double x11 = v011 + (v111 - v011) * w[0];
+  const double xx0 = x00 + (x10 - x00) * w[1];
+  const double xx1 = x01 + (x11 - x01) * w[1];
+  return xx0 + (xx1 - xx0) * w[2];
+}
+
+double orc_linear_interp_f64(const double* vol, const int64_t dims[3], const double idx[3])
+{
+  int64_t b[3], u[3];
+  double w[3];
+  for (int k = 0; k < 3; ++k) {
+    int64_t bk = (int64_t)floor(idx[k]);
+    if (bk < 0) bk = 0;
+    if (bk > dims[k] - 1) bk = dims[k] - 1;
+    const double dk = idx[k] - (double)bk;
+    b[k] = bk;
+    u[k] = (bk + 1 > dims[k] - 1) ? bk : bk + 1;
+    w[k] = (dk > 0.0) ? dk : 0.0;
+  }
+  const int64_t X = dims[0], XY = dims[0] * dims[1];
+#define V(i, j, k) (vol[(i) + X * (j) + XY * (k)])
+  const double x00 = V(b[0], b[1], b[2]) + (V(u[0], b[1], b[2]) - V(b[0], b[1], b[2])) * w[0];
+  const double x10 = V(b[0], u[1], b[2]) + (V(u[0], u[1], b[2]) - V(b[0], u[1], b[2])) * w[0];
+  const double x01 = V(b[0], b[1], u[2]) + (V(u[0], b[1], u[2]) - V(b[0], b[1], u[2])) * w[0];
+  const double x11 = V(b[0], u[1], u[2]) + (V(u[0], u[1], u[2]) - V(b[0], u[1], u[2])) * w[0];
+#undef V
+  const double xx0 = x00 + (x10 - x00) * w[1];
+  const double xx1 = x01 + (x11 - x01) * w[1];
+  return xx0 + (xx1 - xx0) * w[2];
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* DRR, FP32 restatement of ComputeLineInts<Kernel> (xregRayCastLineIntCPU.cpp:105-290). */
+
+typedef struct {
+  const float* vol;
+  const int64_t* dims;
+  float aabb_max[3];
+  const float* p2i;
+  const orc_cam_f* cams;
+  const float* xforms;
+  const int64_t* cam_for_proj;
+  float step;
+  int32_t kernel;
+  int64_t npx;
+  int32_t cols0;
+  float* out;
+  int64_t samples;
+} drr_f32_ctx;
+
+static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
+{
+  drr_f32_ctx* c = (drr_f32_ctx*)vctx;
+  const float aabb_min[3] = {0.0f, 0.0f, 0.0f};
+  const float kTol = 1.0e-3f; /* RayCasterCPU::kVOL_BB_STEP_INC_TOL (xregRayCastBaseCPU.h:37) */
+  int64_t samples = 0;
+  for (int64_t r = begin; r < end; ++r) {
+    /* index decomposition uses camera 0's size (:151-152, 179-183) */
+    const int64_t proj = r / c->npx;
+    const int64_t off = r - c->npx * proj;
+    const int64_t row = off / c->cols0;
+    const int64_t col = off - (int64_t)c->cols0 * row;
+    const orc_cam_f* cam = &c->cams[c->cam_for_proj[proj]];
+
+    float det[3];
+    orc_cam_ind_to_det_f32(cam, (float)col, (float)row, det);                 /* :204 */
+    float M[16];
+    orc_affine_mul_f32(c->p2i, &c->xforms[16 * proj], M);                      /* :207-208 */
+    float ph[3], dp[3], d[3];
+    orc_affine_pt_f32(M, cam->pinhole, ph);                                    /* :211 */
+    orc_affine_pt_f32(M, det, dp);                                             /* :215-216 */
+    d[0] = dp[0] - ph[0];
+    d[1] = dp[1] - ph[1];
+    d[2] = dp[2] - ph[2];
+
+    float acc = (c->kernel == 0) ? 0.0f : -FLT_MAX;                           /* :221 */
+    float t0 = 0.0f, t1 = 0.0f;
+    const int32_t hit = orc_ray_rect_f32(aabb_min, c->aabb_max, ph, d, 1, &t0, &t1); /* :227 */
+    if (hit && ((t1 - t0) > (float)(2.0f * kTol))) {                            /* :233 */
+      t0 += kTol;
+      t1 -= kTol;
+      const float start[3] = {ph[0] + t0 * d[0], ph[1] + t0 * d[1], ph[2] + t0 * d[2]};
+      /* Eigen norm(): sqrt of the linear 3-element redux of squares */
+      const float dlen = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);   /* :243-244 */
+      const float L = (t1 - t0) * dlen;                                        /* :245-246 */
+      /* step length in index space: ||M3x3 * (normalized(det - pinhole) * step)|| (:249-251) */
+      float v[3] = {det[0] - cam->pinhole[0], det[1] - cam->pinhole[1], det[2] - cam->pinhole[2]};
+      const float vz = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+      if (vz > 0.0f) {
+        const float vn = sqrtf(vz);
+        v[0] = v[0] / vn;
+        v[1] = v[1] / vn;
+        v[2] = v[2] / vn;
+      }
+      v[0] = v[0] * c->step;
+      v[1] = v[1] * c->step;
+      v[2] = v[2] * c->step;
+      float sv[3];
+      linear_vec_f32(M, v, sv);
+      const float s = sqrtf((sv[0] * sv[0] + sv[1] * sv[1]) + sv[2] * sv[2]);
+      const int64_t n = (int64_t)(L / s);                                      /* :253-254 */
+      const float scale = s / dlen;                                            /* :263-264 */
+      const float dx = d[0] * scale, dy = d[1] * scale, dz = d[2] * scale;
+      float p[3] = {start[0], start[1], start[2]};
+      for (int64_t i = 0; i <= n; ++i) {                                       /* :270-277 */
+        const float val = (float)orc_itk_linear_interp(c->vol, c->dims, p);
+        if (c->kernel == 0) {
+          acc = acc + val;
+        } else {
+          acc = (acc < val) ? val : acc; /* std::max(a, b) = (a < b) ? b : a */
+        }
+        p[0] = p[0] + dx;
+        p[1] = p[1] + dy;
+        p[2] = p[2] + dz;
+      }
+      samples += n + 1;
+      acc = acc * c->step;                                                     /* :279 */
+    }
+    /* aa_sum = acc * 1 (:282), store with the same kernel (:285) */
+    const float aa = 0.0f + acc * 1.0f;
+    if (c->kernel == 0) {
+      c->out[r] = c->out[r] + aa;
+    } else {
+      c->out[r] = (c->out[r] < aa) ? aa : c->out[r];
+    }
+  }
+  __atomic_fetch_add(&c->samples, samples, __ATOMIC_RELAXED);
+}
+
+static void pre_compute_f32(float* out, const orc_cam_f* cams, const int64_t* cam_for_proj,
+                            int64_t num_projs, int64_t npx, int32_t store, const float* bg,
+                            float default_bg)
+{
+  /* RayCasterCPU::pre_compute (lib/ray_cast/xregRayCastBaseCPU.cpp:128-158) */
+  (void)cams;
+  if (bg) {
+    for (int64_t p = 0; p < num_projs; ++p) {
+      memcpy(out + p * npx, bg + cam_for_proj[p] * npx, sizeof(float) * (size_t)npx);
+    }
+  }
+  if (store == 0 && !bg) {
+    for (int64_t i = 0; i < num_projs * npx; ++i) out[i] = default_bg;
+  }
+}
+
+void orc_drr_f32(const float* vol, const int64_t dims[3], const float p2i[16],
+                 const orc_cam_f* cams, int32_t num_cams,
+                 const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                 float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
+                 float* out, int32_t threads, int64_t* samples_out)
+{
+  (void)num_cams;
+  drr_f32_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.vol = vol;
+  c.dims = dims;
+  /* ITKImageIndexBoundsAsEigen: max index = size - 1 (lib/itk/xregITKBasicImageUtils.h:56-75) */
+  c.aabb_max[0] = (float)(dims[0] - 1);
+  c.aabb_max[1] = (float)(dims[1] - 1);
+  c.aabb_max[2] = (float)(dims[2] - 1);
+  c.p2i = p2i;
+  c.cams = cams;
+  c.xforms = xforms;
+  c.cam_for_proj = cam_for_proj;
+  c.step = step;
+  c.kernel = kernel;
+  c.cols0 = cams[0].cols;
+  c.npx = (int64_t)cams[0].rows * cams[0].cols;
+  c.out = out;
+  pre_compute_f32(out, cams, cam_for_proj, num_projs, c.npx, store, bg, default_bg);
+  orc_parallel_for(drr_f32_range, &c, num_projs * c.npx, 2048, threads);
+  if (samples_out) *samples_out = c.samples;
+}
+
+/* DRR, FP64 oracle: identical algorithm, every quantity in double. */
+
+typedef struct {
+  const double* vol;
+  const int64_t* dims;
+  double aabb_max[3];
+  const double* p2i;
+  const orc_cam_d* cams;
+  const double* xforms;
+  const int64_t* cam_for_proj;
+  double step;
+  int32_t kernel;
+  int64_t npx;
+  int32_t cols0;
+  double* out;
+  int64_t samples;
+} drr_f64_ctx;
+
+static void drr_f64_range(void* vctx, int64_t begin, int64_t end)
+{
+  drr_f64_ctx* c = (drr_f64_ctx*)vctx;
+  const double aabb_min[3] = {0.0, 0.0, 0.0};
+  const double kTol = 1.0e-3;
+  int64_t samples = 0;
+  for (int64_t r = begin; r < end; ++r) {
+    const int64_t proj = r / c->npx;
+    const int64_t off = r - c->npx * proj;
+    const int64_t row = off / c->cols0;
+    const int64_t col = off - (int64_t)c->cols0 * row;
+    const orc_cam_d* cam = &c->cams[c->cam_for_proj[proj]];
+    double det[3];
+    orc_cam_ind_to_det_f64(cam, (double)col, (double)row, det);
+    double M[16];
+    affine_mul_f64(c->p2i, &c->xforms[16 * proj], M);
+    double ph[3], dp[3], d[3];
+    affine_pt_f64(M, cam->pinhole, ph);
+    affine_pt_f64(M, det, dp);
+    d[0] = dp[0] - ph[0];
+    d[1] = dp[1] - ph[1];
+    d[2] = dp[2] - ph[2];
+    double acc = (c->kernel == 0) ? 0.0 : -DBL_MAX;
+    double t0 = 0.0, t1 = 0.0;
+    const int32_t hit = orc_ray_rect_f64(aabb_min, c->aabb_max, ph, d, 1, &t0, &t1);
+    if (hit && ((t1 - t0) > 2.0 * kTol)) {
+      t0 += kTol;
+      t1 -= kTol;
+      const double start[3] = {ph[0] + t0 * d[0], ph[1] + t0 * d[1], ph[2] + t0 * d[2]};
+      const double dlen = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const double L = (t1 - t0) * dlen;
+      double v[3] = {det[0] - cam->pinhole[0], det[1] - cam->pinhole[1], det[2] - cam->pinhole[2]};
+      const double vn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      if (vn > 0.0) {
+        v[0] /= vn;
+        v[1] /= vn;
+        v[2] /= vn;
+      }
+      v[0] *= c->step;
+      v[1] *= c->step;
+      v[2] *= c->step;
+      double sv[3];
+      for (int i = 0; i < 3; ++i) sv[i] = M[4 * i] * v[0] + M[4 * i + 1] * v[1] + M[4 * i + 2] * v[2];
+      const double s = sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]);
+      const int64_t n = (int64_t)(L / s);
+      const double scale = s / dlen;
+      const double dx = d[0] * scale, dy = d[1] * scale, dz = d[2] * scale;
+      for (int64_t i = 0; i <= n; ++i) {
+        /* positions evaluated directly (no accumulated drift) */
+        const double p[3] = {start[0] + (double)i * dx, start[1] + (double)i * dy,
+                             start[2] + (double)i * dz};
+        const double val = orc_linear_interp_f64(c->vol, c->dims, p);
+        if (c->kernel == 0) {
+          acc += val;
+        } else if (acc < val) {
+          acc = val;
+        }
+      }
+      samples += n + 1;
+      acc *= c->step;
+    }
+    if (c->kernel == 0) {
+      c->out[r] += acc;
+    } else if (c->out[r] < acc) {
+      c->out[r] = acc;
+    }
+  }
+  __atomic_fetch_add(&c->samples, samples, __ATOMIC_RELAXED);
+}
+
+void orc_drr_f64(const double* vol, const int64_t dims[3], const double p2i[16],
+                 const orc_cam_d* cams, int32_t num_cams,
+                 const double* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                 double step, int32_t kernel, int32_t store, const double* bg, double default_bg,
+                 double* out, int32_t threads, int64_t* samples_out)
+{
+  (void)num_cams;
+  drr_f64_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.vol = vol;
+  c.dims = dims;
+  c.aabb_max[0] = (double)(dims[0] - 1);
+  c.aabb_max[1] = (double)(dims[1] - 1);
+  c.aabb_max[2] = (double)(dims[2] - 1);
+  c.p2i = p2i;
+  c.cams = cams;
+  c.xforms = xforms;
+  c.cam_for_proj = cam_for_proj;
+  c.step = step;
+  c.kernel = kernel;
+  c.cols0 = cams[0].cols;
+  c.npx = (int64_t)cams[0].rows * cams[0].cols;
+  c.out = out;
+  if (bg) {
+    for (int64_t p = 0; p < num_projs; ++p) {
+      memcpy(out + p * c.npx, bg + cam_for_proj[p] * c.npx, sizeof(double) * (size_t)c.npx);
+    }
+  }
+  if (store == 0 && !bg) {
+    for (int64_t i = 0; i < num_projs * c.npx; ++i) out[i] = default_bg;
+  }
+  orc_parallel_for(drr_f64_range, &c, num_projs * c.npx, 2048, threads);
+  if (samples_out) *samples_out = c.samples;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Eigen 3.3.4 redux_impl<sum, LinearVectorizedTraversal, NoUnrolling> with SSE Packet4f
+ * (Eigen/src/Core/Redux.h): two 4-lane accumulators over 8-element blocks, one trailing
+ * packet, predux = (a0 + a2) + (a1 + a3) (Eigen/src/Core/arch/SSE/PacketMath.h), then the
+ * scalar tail. Assumes a 16-byte aligned start. parity unpinned (Eigen absent). */
+float orc_eigen_sum_f32(const float* x, int64_t n)
+{
+  const int64_t ps = 4;
+  const int64_t aligned2 = (n / (2 * ps)) * (2 * ps);
+  const int64_t aligned = (n / ps) * ps;
+  float res;
+  if (aligned) {
+    float p0[4] = {x[0], x[1], x[2], x[3]};
+    if (aligned > ps) {
+      float p1[4] = {x[4], x[5], x[6], x[7]};
+      for (int64_t i = 2 * ps; i < aligned2; i += 2 * ps) {
+        for (int l = 0; l < 4; ++l) {
+          p0[l] = p0[l] + x[i + l];
+          p1[l] = p1[l] + x[i + ps + l];
+        }
+      }
+      for (int l = 0; l < 4; ++l) p0[l] = p0[l] + p1[l];
+      if (aligned > aligned2) {
+        for (int l = 0; l < 4; ++l) p0[l] = p0[l] + x[aligned2 + l];
+      }
+    }
+    res = (p0[0] + p0[2]) + (p0[1] + p0[3]);
+    for (int64_t i = aligned; i < n; ++i) res = res + x[i];
+  } else {
+    res = x[0];
+    for (int64_t i = 1; i < n; ++i) res = res + x[i];
+  }
+  return res;
+}
+
+static double sum_f64(const double* x, int64_t n)
+{
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += x[i];
+  return s;
+}
+
+/* ComputeImage2DMeanStdDev (xregImgSimMetric2DNCCCPU.cpp:52-64): mean = sum/n,
+ * std = max(1e-6, sqrt(sum((x-mean)^2)/(n-1))). */
+static void mean_std_f32(const float* x, int64_t n, float* tmp, float* mean_out, float* std_out)
+{
+  const float mean = orc_eigen_sum_f32(x, n) / (float)n;
+  for (int64_t i = 0; i < n; ++i) {
+    const float t = x[i] - mean;
+    tmp[i] = t * t;
+  }
+  const float var_sum = orc_eigen_sum_f32(tmp, n);
+  const float s = sqrtf(var_sum / (float)(n - 1));
+  *mean_out = mean;
+  *std_out = (s > 1.0e-6f) ? s : 1.0e-6f;
+}
+
+static void mean_std_masked_f32(const float* x, const uint8_t* m, int64_t n, int64_t len,
+                                float* mean_out, float* std_out)
+{
+  /* ComputeImage2DMeanStdDevWithMask (xregImgSimMetric2DNCCCPU.cpp:77-115) */
+  float mean = 0.0f;
+  for (int64_t i = 0; i < n; ++i) {
+    if (m[i]) mean += x[i];
+  }
+  mean /= (float)len;
+  float sd = 0.0f;
+  for (int64_t i = 0; i < n; ++i) {
+    if (m[i]) {
+      const float t = x[i] - mean;
+      sd += t * t;
+    }
+  }
+  sd = sqrtf(sd / (float)(len - 1));
+  *mean_out = mean;
+  *std_out = (sd > 1.0e-6f) ? sd : 1.0e-6f;
+}
+
+void orc_zero_mean_inplace_f32(float* img, int64_t npix)
+{
+  float* tmp = (float*)malloc(sizeof(float) * (size_t)npix);
+  float mean, sd;
+  mean_std_f32(img, npix, tmp, &mean, &sd);
+  for (int64_t i = 0; i < npix; ++i) img[i] = img[i] - mean;
+  free(tmp);
+}
+
+void orc_ncc_f32(const float* fixed, const float* mov, int64_t npix, int64_t num_mov,
+                 const uint8_t* mask, float* sim_out)
+{
+  /* ImgSimMetric2DNCCCPU::process_mask + compute (xregImgSimMetric2DNCCCPU.cpp:164-236) */
+  float* zf = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* zm = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* tmp = (float*)malloc(sizeof(float) * (size_t)npix);
+  float fmean, fsd;
+  int64_t len = 0;
+  if (!mask) {
+    mean_std_f32(fixed, npix, tmp, &fmean, &fsd);
+  } else {
+    for (int64_t i = 0; i < npix; ++i) len += mask[i] ? 1 : 0;
+    mean_std_masked_f32(fixed, mask, npix, len, &fmean, &fsd);
+  }
+  for (int64_t i = 0; i < npix; ++i) zf[i] = fixed[i] - fmean;
+  for (int64_t m = 0; m < num_mov; ++m) {
+    const float* x = mov + m * npix;
+    float mmean, msd, sim;
+    if (!mask) {
+      mean_std_f32(x, npix, tmp, &mmean, &msd);
+      for (int64_t i = 0; i < npix; ++i) zm[i] = x[i] - mmean;
+      for (int64_t i = 0; i < npix; ++i) tmp[i] = zf[i] * zm[i];
+      const float dot = orc_eigen_sum_f32(tmp, npix);
+      sim = dot / (((float)npix * fsd) * msd);
+    } else {
+      mean_std_masked_f32(x, mask, npix, len, &mmean, &msd);
+      sim = 0.0f;
+      for (int64_t i = 0; i < npix; ++i) {
+        if (mask[i]) sim += zf[i] * (x[i] - mmean);
+      }
+      sim /= ((float)len * fsd) * msd;
+    }
+    sim_out[m] = (1.0f - sim) * 0.5f;                                          /* :204 */
+  }
+  free(zf);
+  free(zm);
+  free(tmp);
+}
+
+void orc_ncc_f64(const double* fixed, const double* mov, int64_t npix, int64_t num_mov,
+                 const uint8_t* mask, double* sim_out)
+{
+  int64_t len = npix;
+  if (mask) {
+    len = 0;
+    for (int64_t i = 0; i < npix; ++i) len += mask[i] ? 1 : 0;
+  }
+  double fm = 0.0;
+  for (int64_t i = 0; i < npix; ++i) {
+    if (!mask || mask[i]) fm += fixed[i];
+  }
+  fm /= (double)len;
+  double fv = 0.0;
+  for (int64_t i = 0; i < npix; ++i) {
+    if (!mask || mask[i]) fv += (fixed[i] - fm) * (fixed[i] - fm);
+  }
+  double fsd = sqrt(fv / (double)(len - 1));
+  if (fsd < 1.0e-6) fsd = 1.0e-6;
+  for (int64_t m = 0; m < num_mov; ++m) {
+    const double* x = mov + m * npix;
+    double mm = 0.0;
+    for (int64_t i = 0; i < npix; ++i) {
+      if (!mask || mask[i]) mm += x[i];
+    }
+    mm /= (double)len;
+    double mv = 0.0, dot = 0.0;
+    for (int64_t i = 0; i < npix; ++i) {
+      if (!mask || mask[i]) {
+        mv += (x[i] - mm) * (x[i] - mm);
+        dot += (fixed[i] - fm) * (x[i] - mm);
+      }
+    }
+    double msd = sqrt(mv / (double)(len - 1));
+    if (msd < 1.0e-6) msd = 1.0e-6;
+    sim_out[m] = (1.0 - dot / ((double)len * fsd * msd)) * 0.5;
+  }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Gradient images: ImgSimMetric2DGradImgCPU (xregImgSimMetric2DGradImgCPU.cpp:32-102)
+ * calls cv::GaussianBlur(img, Size(k,k), 0, 0) then cv::Sobel(img, -1, 1,0) / (0,1).
+ * OpenCV 3.4: Sobel ksize 3 = separable [-1,0,1] x [1,2,1]; GaussianBlur with sigma 0 and
+ * odd ksize <= 7 uses the fixed table (ksize 5: [1,4,6,4,1]/16, ksize 3: [1,2,1]/4);
+ * the default border is BORDER_REFLECT_101; the row filter runs before the column filter.
+ * parity unpinned: OpenCV is absent (its IPP path for float blur is not restated). */
+
+static inline int32_t reflect101(int32_t p, int32_t n)
+{
+  if (n == 1) return 0;
+  while (p < 0 || p >= n) {
+    if (p < 0) p = -p;
+    if (p >= n) p = 2 * (n - 1) - p;
+  }
+  return p;
+}
+
+static const float* gauss_table_f32(int32_t ksize)
+{
+  static const float k1[1] = {1.0f};
+  static const float k3[3] = {0.25f, 0.5f, 0.25f};
+  static const float k5[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+  static const float k7[7] = {0.03125f, 0.109375f, 0.21875f, 0.28125f, 0.21875f, 0.109375f, 0.03125f};
+  switch (ksize) {
+    case 1: return k1;
+    case 3: return k3;
+    case 5: return k5;
+    case 7: return k7;
+    default: return NULL;
+  }
+}
+
+static void blur_f32(const float* src, int32_t rows, int32_t cols, int32_t ksize, float* dst)
+{
+  const float* k = gauss_table_f32(ksize);
+  const int32_t r = ksize / 2;
+  float* tmp = (float*)malloc(sizeof(float) * (size_t)rows * cols);
+  /* row pass: symmetric kernel, k[r]*s0 + sum_j k[r+j]*(s[-j] + s[+j]) */
+  for (int32_t y = 0; y < rows; ++y) {
+    const float* s = src + (int64_t)y * cols;
+    for (int32_t x = 0; x < cols; ++x) {
+      float acc = k[r] * s[x];
+      for (int32_t j = 1; j <= r; ++j) {
+        acc = acc + k[r + j] * (s[reflect101(x - j, cols)] + s[reflect101(x + j, cols)]);
+      }
+      tmp[(int64_t)y * cols + x] = acc;
+    }
+  }
+  for (int32_t y = 0; y < rows; ++y) {
+    for (int32_t x = 0; x < cols; ++x) {
+      float acc = k[r] * tmp[(int64_t)y * cols + x];
+      for (int32_t j = 1; j <= r; ++j) {
+        acc = acc + k[r + j] * (tmp[(int64_t)reflect101(y - j, rows) * cols + x] +
+                                tmp[(int64_t)reflect101(y + j, rows) * cols + x]);
+      }
+      dst[(int64_t)y * cols + x] = acc;
+    }
+  }
+  free(tmp);
+}
+
+void orc_sobel_f32(const float* img, int32_t rows, int32_t cols, int32_t smooth_ksize,
+                   float* gx, float* gy)
+{
+  const float* src = img;
+  float* sm = NULL;
+  if (smooth_ksize > 1) {
+    sm = (float*)malloc(sizeof(float) * (size_t)rows * cols);
+    blur_f32(img, rows, cols, smooth_ksize, sm);
+    src = sm;
+  }
+  float* tmp = (float*)malloc(sizeof(float) * (size_t)rows * cols);
+  /* gx: row derivative s[x+1] - s[x-1], then column smoothing (t[-1] + t[+1]) + 2 t[0] */
+  for (int32_t y = 0; y < rows; ++y) {
+    const float* s = src + (int64_t)y * cols;
+    for (int32_t x = 0; x < cols; ++x) {
+      tmp[(int64_t)y * cols + x] = s[reflect101(x + 1, cols)] - s[reflect101(x - 1, cols)];
+    }
+  }
+  for (int32_t y = 0; y < rows; ++y) {
+    const float* a = tmp + (int64_t)reflect101(y - 1, rows) * cols;
+    const float* b = tmp + (int64_t)y * cols;
+    const float* c = tmp + (int64_t)reflect101(y + 1, rows) * cols;
+    for (int32_t x = 0; x < cols; ++x) gx[(int64_t)y * cols + x] = (a[x] + c[x]) + b[x] * 2.0f;
+  }
+  /* gy: row smoothing (s[-1] + s[+1]) + 2 s[0], then column derivative t[+1] - t[-1] */
+  for (int32_t y = 0; y < rows; ++y) {
+    const float* s = src + (int64_t)y * cols;
+    for (int32_t x = 0; x < cols; ++x) {
+      tmp[(int64_t)y * cols + x] = (s[reflect101(x - 1, cols)] + s[reflect101(x + 1, cols)]) + s[x] * 2.0f;
+    }
+  }
+  for (int32_t y = 0; y < rows; ++y) {
+    const float* a = tmp + (int64_t)reflect101(y - 1, rows) * cols;
+    const float* c = tmp + (int64_t)reflect101(y + 1, rows) * cols;
+    for (int32_t x = 0; x < cols; ++x) gy[(int64_t)y * cols + x] = c[x] - a[x];
+  }
+  free(tmp);
+  free(sm);
+}
+
+static void blur_f64(const double* src, int32_t rows, int32_t cols, int32_t ksize, double* dst)
+{
+  const float* kf = gauss_table_f32(ksize);
+  const int32_t r = ksize / 2;
+  double k[7];
+  for (int i = 0; i < ksize; ++i) k[i] = (double)kf[i];
+  double* tmp = (double*)malloc(sizeof(double) * (size_t)rows * cols);
+  for (int32_t y = 0; y < rows; ++y) {
+    for (int32_t x = 0; x < cols; ++x) {
+      double acc = 0.0;
+      for (int32_t j = -r; j <= r; ++j) acc += k[r + j] * src[(int64_t)y * cols + reflect101(x + j, cols)];
+      tmp[(int64_t)y * cols + x] = acc;
+    }
+  }
+  for (int32_t y = 0; y < rows; ++y) {
+    for (int32_t x = 0; x < cols; ++x) {
+      double acc = 0.0;
+      for (int32_t j = -r; j <= r; ++j) acc += k[r + j] * tmp[(int64_t)reflect101(y + j, rows) * cols + x];
+      dst[(int64_t)y * cols + x] = acc;
+    }
+  }
+  free(tmp);
+}
+
+void orc_sobel_f64(const double* img, int32_t rows, int32_t cols, int32_t smooth_ksize,
+                   double* gx, double* gy)
+{
+  const double* src = img;
+  double* sm = NULL;
+  if (smooth_ksize > 1) {
+    sm = (double*)malloc(sizeof(double) * (size_t)rows * cols);
+    blur_f64(img, rows, cols, smooth_ksize, sm);
+    src = sm;
+  }
+  for (int32_t y = 0; y < rows; ++y) {
+    const int32_t ym = reflect101(y - 1, rows), yp = reflect101(y + 1, rows);
+    for (int32_t x = 0; x < cols; ++x) {
+      const int32_t xm = reflect101(x - 1, cols), xp = reflect101(x + 1, cols);
+#define S(yy, xx) src[(int64_t)(yy) * cols + (xx)]
+      gx[(int64_t)y * cols + x] = (S(ym, xp) - S(ym, xm)) + 2.0 * (S(y, xp) - S(y, xm)) + (S(yp, xp) - S(yp, xm));
+      gy[(int64_t)y * cols + x] = (S(yp, xm) - S(ym, xm)) + 2.0 * (S(yp, x) - S(ym, x)) + (S(yp, xp) - S(ym, xp));
+#undef S
+    }
+  }
+  free(sm);
+}
+
+void orc_grad_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32_t cols,
+                      int64_t num_mov, int32_t smooth_ksize, float* sim_out)
+{
+  /* ImgSimMetric2DGradNCCCPU::compute (xregImgSimMetric2DGradNCCCPU.cpp:51-65):
+   * NCC on gx and on gy, sim = 0.5 * (ncc_x + ncc_y). */
+  const int64_t npix = (int64_t)rows * cols;
+  float* fgx = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* fgy = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* mgx = (float*)malloc(sizeof(float) * (size_t)(npix * num_mov));
+  float* mgy = (float*)malloc(sizeof(float) * (size_t)(npix * num_mov));
+  float* sx = (float*)malloc(sizeof(float) * (size_t)num_mov);
+  float* sy = (float*)malloc(sizeof(float) * (size_t)num_mov);
+  orc_sobel_f32(fixed, rows, cols, smooth_ksize, fgx, fgy);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    orc_sobel_f32(mov + m * npix, rows, cols, smooth_ksize, mgx + m * npix, mgy + m * npix);
+  }
+  orc_ncc_f32(fgx, mgx, npix, num_mov, NULL, sx);
+  orc_ncc_f32(fgy, mgy, npix, num_mov, NULL, sy);
+  for (int64_t m = 0; m < num_mov; ++m) sim_out[m] = (float)(0.5 * (double)(sx[m] + sy[m]));
+  free(fgx);
+  free(fgy);
+  free(mgx);
+  free(mgy);
+  free(sx);
+  free(sy);
+}
+
+void orc_grad_ncc_f64(const double* fixed, const double* mov, int32_t rows, int32_t cols,
+                      int64_t num_mov, int32_t smooth_ksize, double* sim_out)
+{
+  const int64_t npix = (int64_t)rows * cols;
+  double* fgx = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* fgy = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* mgx = (double*)malloc(sizeof(double) * (size_t)(npix * num_mov));
+  double* mgy = (double*)malloc(sizeof(double) * (size_t)(npix * num_mov));
+  double* sx = (double*)malloc(sizeof(double) * (size_t)num_mov);
+  double* sy = (double*)malloc(sizeof(double) * (size_t)num_mov);
+  orc_sobel_f64(fixed, rows, cols, smooth_ksize, fgx, fgy);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    orc_sobel_f64(mov + m * npix, rows, cols, smooth_ksize, mgx + m * npix, mgy + m * npix);
+  }
+  orc_ncc_f64(fgx, mgx, npix, num_mov, NULL, sx);
+  orc_ncc_f64(fgy, mgy, npix, num_mov, NULL, sy);
+  for (int64_t m = 0; m < num_mov; ++m) sim_out[m] = 0.5 * (sx[m] + sy[m]);
+  free(fgx);
+  free(fgy);
+  free(mgx);
+  free(mgy);
+  free(sx);
+  free(sy);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Patch NCC: ImgSimMetric2DPatchCommon::setup_patches (xregImgSimMetric2DPatchCommon.cpp:
+ * 256-297: centres r .. size-1-r, row-major, stride s), detail::ComputePatchMeanStdDev
+ * (xregImgSimMetric2DPatchNCCCPU.cpp:558-619), fixed preprocessing (:398-409) and the
+ * per-patch score + combine (:190-287) with unit weights. */
+
+typedef struct {
+  const float* fixed;
+  const float* mov;
+  int32_t cols;
+  int32_t r;
+  int32_t diam;
+  int32_t ppr; /* patches per row */
+  int32_t stride;
+  float* fhat; /* num_patches * diam * diam */
+  float* vals;
+} patch_f32_ctx;
+
+static void patch_stats_f32(const float* img, int32_t cols, int32_t r0, int32_t c0, int32_t diam,
+                            float* mean_out, float* std_out)
+{
+  float mean = 0.0f;
+  for (int32_t y = 0; y < diam; ++y) {
+    for (int32_t x = 0; x < diam; ++x) mean += img[(int64_t)(r0 + y) * cols + c0 + x];
+  }
+  const int64_t n = (int64_t)diam * diam;
+  float var = 0.0f;
+  if (n > 1) {
+    mean /= (float)n;
+    for (int32_t y = 0; y < diam; ++y) {
+      for (int32_t x = 0; x < diam; ++x) {
+        const float t = img[(int64_t)(r0 + y) * cols + c0 + x] - mean;
+        var += t * t;
+      }
+    }
+    var /= (float)n - 1.0f;
+  }
+  const float s = sqrtf(var);
+  *mean_out = mean;
+  *std_out = (s > 1.0e-6f) ? s : 1.0e-6f;
+}
+
+static void patch_fixed_range(void* vctx, int64_t begin, int64_t end)
+{
+  patch_f32_ctx* c = (patch_f32_ctx*)vctx;
+  const int32_t d = c->diam;
+  for (int64_t p = begin; p < end; ++p) {
+    const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
+    const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
+    float mean, sd;
+    patch_stats_f32(c->fixed, c->cols, r0, c0, d, &mean, &sd);
+    const float denom = sd * (float)((int64_t)d * d);
+    float* dst = c->fhat + p * d * d;
+    for (int32_t y = 0; y < d; ++y) {
+      for (int32_t x = 0; x < d; ++x) {
+        dst[y * d + x] = (c->fixed[(int64_t)(r0 + y) * c->cols + c0 + x] - mean) / denom;
+      }
+    }
+  }
+}
+
+static void patch_mov_range(void* vctx, int64_t begin, int64_t end)
+{
+  patch_f32_ctx* c = (patch_f32_ctx*)vctx;
+  const int32_t d = c->diam;
+  for (int64_t p = begin; p < end; ++p) {
+    const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
+    const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
+    float mean, sd;
+    patch_stats_f32(c->mov, c->cols, r0, c0, d, &mean, &sd);
+    const float* fh = c->fhat + p * d * d;
+    float accum = 0.0f;
+    for (int32_t y = 0; y < d; ++y) {
+      for (int32_t x = 0; x < d; ++x) {
+        accum += ((c->mov[(int64_t)(r0 + y) * c->cols + c0 + x] - mean) / sd) * fh[y * d + x];
+      }
+    }
+    /* weight 1 (weight_patch_sims_in_combine_ with unit weights) */
+    c->vals[p] = 1.0f * (1.0f - accum);
+  }
+}
+
+void orc_patch_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32_t cols,
+                       int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                       float* sim_out, int32_t threads)
+{
+  const int32_t diam = 2 * patch_radius + 1;
+  const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int64_t np = (int64_t)ppr * ppc;
+  const int64_t npix = (int64_t)rows * cols;
+  patch_f32_ctx c;
+  c.fixed = fixed;
+  c.cols = cols;
+  c.r = patch_radius;
+  c.diam = diam;
+  c.ppr = ppr;
+  c.stride = patch_stride;
+  c.fhat = (float*)malloc(sizeof(float) * (size_t)(np * diam * diam));
+  c.vals = (float*)malloc(sizeof(float) * (size_t)np);
+  orc_parallel_for(patch_fixed_range, &c, np, 256, threads);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    c.mov = mov + m * npix;
+    orc_parallel_for(patch_mov_range, &c, np, 256, threads);
+    float sum = 0.0f;
+    for (int64_t p = 0; p < np; ++p) sum += c.vals[p];
+    /* compute_mean_of_patch_sims_ is false by default; weighted combine divides by the
+     * sum of the (unit) weights, accumulated in float (:272-285) */
+    float tot_wgt = 0.0f;
+    for (int64_t p = 0; p < np; ++p) tot_wgt += 1.0f;
+    sim_out[m] = sum / tot_wgt;
+  }
+  free(c.fhat);
+  free(c.vals);
+}
+
+typedef struct {
+  const double* fixed;
+  const double* mov;
+  int32_t cols;
+  int32_t diam;
+  int32_t ppr;
+  int32_t stride;
+  double* fmean;
+  double* fsd;
+  double* vals;
+} patch_f64_ctx;
+
+static void patch_stats_f64(const double* img, int32_t cols, int32_t r0, int32_t c0, int32_t d,
+                            double* mean_out, double* sd_out)
+{
+  const double n = (double)d * d;
+  double mean = 0.0;
+  for (int32_t y = 0; y < d; ++y) {
+    for (int32_t x = 0; x < d; ++x) mean += img[(int64_t)(r0 + y) * cols + c0 + x];
+  }
+  mean /= n;
+  double var = 0.0;
+  for (int32_t y = 0; y < d; ++y) {
+    for (int32_t x = 0; x < d; ++x) {
+      const double t = img[(int64_t)(r0 + y) * cols + c0 + x] - mean;
+      var += t * t;
+    }
+  }
+  double sd = sqrt(var / (n - 1.0));
+  *mean_out = mean;
+  *sd_out = (sd > 1.0e-6) ? sd : 1.0e-6;
+}
+
+static void patch_fixed_range_f64(void* vctx, int64_t begin, int64_t end)
+{
+  patch_f64_ctx* c = (patch_f64_ctx*)vctx;
+  for (int64_t p = begin; p < end; ++p) {
+    const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
+    const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
+    patch_stats_f64(c->fixed, c->cols, r0, c0, c->diam, &c->fmean[p], &c->fsd[p]);
+  }
+}
+
+static void patch_mov_range_f64(void* vctx, int64_t begin, int64_t end)
+{
+  patch_f64_ctx* c = (patch_f64_ctx*)vctx;
+  const int32_t d = c->diam;
+  const double n = (double)d * d;
+  for (int64_t p = begin; p < end; ++p) {
+    const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
+    const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
+    double mean, sd;
+    patch_stats_f64(c->mov, c->cols, r0, c0, d, &mean, &sd);
+    double acc = 0.0;
+    for (int32_t y = 0; y < d; ++y) {
+      for (int32_t x = 0; x < d; ++x) {
+        const int64_t o = (int64_t)(r0 + y) * c->cols + c0 + x;
+        acc += ((c->mov[o] - mean) / sd) * ((c->fixed[o] - c->fmean[p]) / (c->fsd[p] * n));
+      }
+    }
+    c->vals[p] = 1.0 - acc;
+  }
+}
+
+void orc_patch_ncc_f64(const double* fixed, const double* mov, int32_t rows, int32_t cols,
+                       int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                       double* sim_out, int32_t threads)
+{
+  const int32_t diam = 2 * patch_radius + 1;
+  const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int64_t np = (int64_t)ppr * ppc;
+  const int64_t npix = (int64_t)rows * cols;
+  patch_f64_ctx c;
+  c.fixed = fixed;
+  c.cols = cols;
+  c.diam = diam;
+  c.ppr = ppr;
+  c.stride = patch_stride;
+  c.fmean = (double*)malloc(sizeof(double) * (size_t)np);
+  c.fsd = (double*)malloc(sizeof(double) * (size_t)np);
+  c.vals = (double*)malloc(sizeof(double) * (size_t)np);
+  orc_parallel_for(patch_fixed_range_f64, &c, np, 256, threads);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    c.mov = mov + m * npix;
+    orc_parallel_for(patch_mov_range_f64, &c, np, 256, threads);
+    sim_out[m] = sum_f64(c.vals, np) / (double)np;
+  }
+  free(c.fmean);
+  free(c.fsd);
+  free(c.vals);
+}
+
+void orc_patch_grad_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32_t cols,
+                            int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                            int32_t patch_stride, float* sim_out, int32_t threads)
+{
+  /* ImgSimMetric2DPatchGradNCCCPU::compute (xregImgSimMetric2DPatchGradNCCCPU.cpp:104-224) */
+  const int64_t npix = (int64_t)rows * cols;
+  float* fgx = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* fgy = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* mgx = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* mgy = (float*)malloc(sizeof(float) * (size_t)npix);
+  orc_sobel_f32(fixed, rows, cols, smooth_ksize, fgx, fgy);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    float sx, sy;
+    orc_sobel_f32(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
+    orc_patch_ncc_f32(fgx, mgx, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
+    orc_patch_ncc_f32(fgy, mgy, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
+    sim_out[m] = (float)(0.5 * (double)(sx + sy));
+  }
+  free(fgx);
+  free(fgy);
+  free(mgx);
+  free(mgy);
+}
+
+void orc_patch_grad_ncc_f64(const double* fixed, const double* mov, int32_t rows, int32_t cols,
+                            int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                            int32_t patch_stride, double* sim_out, int32_t threads)
+{
+  const int64_t npix = (int64_t)rows * cols;
+  double* fgx = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* fgy = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* mgx = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* mgy = (double*)malloc(sizeof(double) * (size_t)npix);
+  orc_sobel_f64(fixed, rows, cols, smooth_ksize, fgx, fgy);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    double sx, sy;
+    orc_sobel_f64(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
+    orc_patch_ncc_f64(fgx, mgx, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
+    orc_patch_ncc_f64(fgy, mgy, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
+    sim_out[m] = 0.5 * (sx + sy);
+  }
+  free(fgx);
+  free(fgy);
+  free(mgx);
+  free(mgy);
+}

@@ -1,0 +1,70 @@
+"""GPU parity of the similarity metrics (xrh_sim_compute) against the CPU restatements of
+ImgSimMetric2D{NCC,GradNCC,PatchNCC,PatchGradNCC}CPU, through the C-ABI."""
+import numpy as np
+import pytest
+
+from tests.helpers import SIM_ABS_TOL, poses_f32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def images(oracle):
+    from xreg_amd import synth
+
+    w = synth.config("c2", n_poses=5, small=True)
+    poses = np.concatenate([w.gt[None], w.poses])
+    drr, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(poses), step=w.step)
+    fixed = synth.noisy_fixed(drr[0])
+    return fixed, drr
+
+
+@pytest.mark.parametrize("metric,ksize", [(0, 0), (1, 0), (1, 5), (2, 0), (3, 0), (3, 5), (3, 3)])
+def test_sim_matches_cpu_reference(ctx, oracle, images, metric, ksize):
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    print(f"case: metric={metric} smooth={ksize} shape={fixed.shape} n={mov.shape[0]}", flush=True)
+    ref = oracle.sim_f32(metric, fixed, mov, smooth_ksize=ksize)
+    ref64 = oracle.sim_f64(metric, fixed.astype(np.float64), mov.astype(np.float64), smooth_ksize=ksize)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, metric, smooth_ksize=ksize)
+    got = f.compute(buf, mov.shape[0])
+    f.close()
+    buf.free()
+    print(f"max |gpu-cpu| {np.abs(got - ref).max():.3e}; |gpu-fp64| {np.abs(got - ref64).max():.3e}; "
+          f"|cpu-fp64| {np.abs(ref - ref64).max():.3e}")
+    assert np.abs(got - ref).max() <= SIM_ABS_TOL
+    assert np.abs(got - ref64).max() <= SIM_ABS_TOL
+
+
+def test_ncc_mask(ctx, oracle, images):
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    mask = np.zeros(fixed.shape, np.uint8)
+    mask[5:-7, 3:-4] = 1
+    ref = oracle.ncc_f32(fixed, mov, mask=mask)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, 0, mask=mask)
+    got = f.compute(buf, mov.shape[0])
+    f.close()
+    buf.free()
+    assert np.abs(got - ref).max() <= SIM_ABS_TOL
+
+
+def test_sim_does_not_modify_moving_images(ctx, images):
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    for metric in range(4):
+        f = X.Fixed(ctx, fixed, metric)
+        f.compute(buf, mov.shape[0])
+        f.close()
+    after = buf.download(np.float32).reshape(mov.shape)
+    buf.free()
+    assert np.array_equal(after, mov)

@@ -1,0 +1,354 @@
+"""ctypes binding of libxrh.so (include/xrh.h).
+
+Loading fails loudly when the library is missing or a symbol is absent: there is no CPU
+fallback behind these classes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libxrh.so"
+
+XRH_OK = 0
+XRH_ERR_INVALID = 1
+XRH_ERR_HIP = 2
+XRH_ERR_OOM = 3
+XRH_ERR_BUDGET = 4
+XRH_ERR_UNSUPPORTED = 5
+XRH_ERR_STATE = 6
+
+XRH_F32, XRH_F16, XRH_BF16 = 0, 1, 2
+XRH_LINE_INT_SUM, XRH_LINE_INT_MAX = 0, 1
+XRH_STORE_REPLACE, XRH_STORE_ACCUM = 0, 1
+XRH_GEOM_FP32, XRH_GEOM_FP64 = 0, 1
+XRH_SIM_NCC, XRH_SIM_GRAD_NCC, XRH_SIM_PATCH_NCC, XRH_SIM_PATCH_GRAD_NCC = 0, 1, 2, 3
+LAYOUT_LINEAR, LAYOUT_QUAD, LAYOUT_OCT = 0, 1, 2
+
+
+class XrhError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"xrh status {status}: {msg}")
+        self.status = status
+
+
+class Camera(C.Structure):
+    """xrh_camera: CameraModel's float members (xregPerspectiveXform.h:108-273)."""
+
+    _fields_ = [
+        ("num_det_rows", C.c_int32),
+        ("num_det_cols", C.c_int32),
+        ("coord_frame", C.c_int32),
+        ("focal_len", C.c_float),
+        ("intrins_inv", C.c_float * 9),
+        ("extrins_inv", C.c_float * 16),
+        ("pinhole", C.c_float * 3),
+        ("det_row_spacing", C.c_float),
+        ("det_col_spacing", C.c_float),
+    ]
+
+
+class SimParams(C.Structure):
+    _fields_ = [
+        ("metric", C.c_int32),
+        ("smooth_ksize", C.c_int32),
+        ("patch_radius", C.c_int32),
+        ("patch_stride", C.c_int32),
+    ]
+
+
+class View(C.Structure):
+    _fields_ = [
+        ("vol", C.c_void_p),
+        ("fixed", C.c_void_p),
+        ("cam", Camera),
+        ("step", C.c_float),
+        ("kernel", C.c_int32),
+    ]
+
+
+_lib = None
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_U64 = C.c_uint64
+_F = C.c_float
+_PF = C.POINTER(C.c_float)
+_PD = C.POINTER(C.c_double)
+_PI64 = C.POINTER(C.c_int64)
+_PI32 = C.POINTER(C.c_int32)
+_PU64 = C.POINTER(C.c_uint64)
+_PU8 = C.POINTER(C.c_uint8)
+
+# name -> (restype, argtypes); the full exported surface of include/xrh.h
+SIGNATURES = {
+    "xrh_last_error": (C.c_char_p, []),
+    "xrh_abi_version": (_I32, []),
+    "xrh_ctx_create": (_I32, [_I32, _U64, C.POINTER(_P)]),
+    "xrh_ctx_destroy": (_I32, [_P]),
+    "xrh_ctx_synchronize": (_I32, [_P]),
+    "xrh_ctx_stream": (_P, [_P]),
+    "xrh_ctx_bytes_in_use": (_U64, [_P]),
+    "xrh_device_alloc": (_I32, [_P, _U64, C.POINTER(_P)]),
+    "xrh_device_free": (_I32, [_P, _P]),
+    "xrh_copy_h2d": (_I32, [_P, _P, _P, _U64]),
+    "xrh_copy_d2h": (_I32, [_P, _P, _P, _U64]),
+    "xrh_volume_create": (_I32, [_P, _PF, _PI64, _PF, _I32, C.POINTER(_P)]),
+    "xrh_volume_create_layout": (_I32, [_P, _PF, _PI64, _PF, _I32, _I32, C.POINTER(_P)]),
+    "xrh_volume_destroy": (_I32, [_P]),
+    "xrh_volume_bytes": (_U64, [_P]),
+    "xrh_line_int": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
+                            _I32, _P]),
+    "xrh_line_int_sample_count": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32,
+                                         _PU64]),
+    "xrh_fixed_create": (_I32, [_P, _PF, _I32, _I32, C.POINTER(SimParams), _PU8, C.POINTER(_P)]),
+    "xrh_fixed_destroy": (_I32, [_P]),
+    "xrh_sim_compute": (_I32, [_P, _P, _P, _I64, _P, _PF]),
+    "xrh_batch_create": (_I32, [_P, C.POINTER(View), _I32, _I64, _I32, C.POINTER(_P)]),
+    "xrh_batch_destroy": (_I32, [_P]),
+    "xrh_batch_set_options": (_I32, [_P, _I32, _I32]),
+    "xrh_batch_eval": (_I32, [_P, _PI32, _PD, _I64, _PF, _P]),
+    "xrh_batch_launch": (_I32, [_P, _PI32, _PD, _I64]),
+    "xrh_batch_wait": (_I32, [_P, _PF]),
+    "xrh_batch_last_timing": (_I32, [_P, _PF, _PF]),
+    "xrh_batch_sample_count": (_I32, [_P, _PI32, _PD, _I64, _PU64]),
+}
+
+
+def lib() -> C.CDLL:
+    """Loads libxrh.so once; raises if it is missing (build with xreg_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} not found: build it with `python -m xreg_amd.build`")
+        l = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)  # AttributeError if not exported
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != XRH_OK:
+        msg = lib().xrh_last_error().decode(errors="replace")
+        raise XrhError(status, msg)
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(_PF)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_PD)
+
+
+class DeviceBuffer:
+    """Device memory owned by a Context (charged to its HBM budget)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _P()
+        check(lib().xrh_device_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().xrh_copy_h2d(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, dtype=np.float32, count: int | None = None, offset_bytes: int = 0) -> np.ndarray:
+        itemsize = np.dtype(dtype).itemsize
+        if count is None:
+            count = (self.nbytes - offset_bytes) // itemsize
+        out = np.empty(count, dtype=dtype)
+        check(lib().xrh_copy_d2h(self.ctx.h, out.ctypes.data, self.ptr + offset_bytes, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            check(lib().xrh_device_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                lib().xrh_device_free(self.ctx.h, self.ptr)
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        p = _P()
+        check(lib().xrh_ctx_create(int(device), int(hbm_budget_bytes), C.byref(p)))
+        self.h = p.value
+
+    def synchronize(self) -> None:
+        check(lib().xrh_ctx_synchronize(self.h))
+
+    def bytes_in_use(self) -> int:
+        return int(lib().xrh_ctx_bytes_in_use(self.h))
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def close(self) -> None:
+        if self.h:
+            check(lib().xrh_ctx_destroy(self.h))
+            self.h = None
+
+
+class Volume:
+    def __init__(self, ctx: Context, vox_zyx: np.ndarray, idx_to_phys: np.ndarray, storage: int = XRH_F32,
+                 layout: int | None = None):
+        vox = np.ascontiguousarray(vox_zyx, dtype=np.float32)
+        assert vox.ndim == 3
+        size = np.array([vox.shape[2], vox.shape[1], vox.shape[0]], dtype=np.int64)
+        m = np.ascontiguousarray(idx_to_phys, dtype=np.float32).reshape(16)
+        p = _P()
+        if layout is None:
+            check(lib().xrh_volume_create(ctx.h, _fp(vox), size.ctypes.data_as(_PI64), _fp(m), int(storage),
+                                          C.byref(p)))
+        else:
+            check(lib().xrh_volume_create_layout(ctx.h, _fp(vox), size.ctypes.data_as(_PI64), _fp(m),
+                                                 int(storage), int(layout), C.byref(p)))
+        self.ctx = ctx
+        self.h = p.value
+        self.size = tuple(int(s) for s in size)
+
+    @property
+    def nbytes(self) -> int:
+        return int(lib().xrh_volume_bytes(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            check(lib().xrh_volume_destroy(self.h))
+            self.h = None
+
+
+def _cams_array(cams) -> C.Array:
+    arr = (Camera * len(cams))()
+    for i, c in enumerate(cams):
+        arr[i] = c
+    return arr
+
+
+def line_int(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_for_proj: np.ndarray, step: float,
+             drr: DeviceBuffer, kernel: int = XRH_LINE_INT_SUM, store: int = XRH_STORE_REPLACE,
+             bg: DeviceBuffer | None = None, default_bg: float = 0.0, geom: int = XRH_GEOM_FP32) -> None:
+    x = np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16)
+    c4p = np.ascontiguousarray(cam_for_proj, dtype=np.int64)
+    ca = _cams_array(cams)
+    check(lib().xrh_line_int(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64), x.shape[0],
+                             float(step), int(kernel), int(store), bg.ptr if bg else None, float(default_bg),
+                             int(geom), drr.ptr))
+
+
+def line_int_sample_count(ctx: Context, vol: Volume, cams, xforms, cam_for_proj, step: float,
+                          geom: int = XRH_GEOM_FP32) -> int:
+    x = np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16)
+    c4p = np.ascontiguousarray(cam_for_proj, dtype=np.int64)
+    ca = _cams_array(cams)
+    out = C.c_uint64(0)
+    check(lib().xrh_line_int_sample_count(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64),
+                                          x.shape[0], float(step), int(geom), C.byref(out)))
+    return int(out.value)
+
+
+class Fixed:
+    def __init__(self, ctx: Context, img: np.ndarray, metric: int, smooth_ksize: int = 5, patch_radius: int = 5,
+                 patch_stride: int = 1, mask: np.ndarray | None = None):
+        im = np.ascontiguousarray(img, dtype=np.float32)
+        prm = SimParams(int(metric), int(smooth_ksize), int(patch_radius), int(patch_stride))
+        mk = None
+        if mask is not None:
+            self._mask = np.ascontiguousarray(mask, dtype=np.uint8)
+            mk = self._mask.ctypes.data_as(_PU8)
+        p = _P()
+        check(lib().xrh_fixed_create(ctx.h, _fp(im), im.shape[0], im.shape[1], C.byref(prm), mk, C.byref(p)))
+        self.ctx = ctx
+        self.h = p.value
+        self.shape = im.shape
+        self.metric = metric
+
+    def compute(self, mov: DeviceBuffer, num_mov: int, offset_imgs: int = 0) -> np.ndarray:
+        out = np.empty(num_mov, dtype=np.float32)
+        npix = self.shape[0] * self.shape[1]
+        check(lib().xrh_sim_compute(self.ctx.h, self.h, mov.ptr + 4 * npix * offset_imgs, int(num_mov), None,
+                                    _fp(out)))
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            check(lib().xrh_fixed_destroy(self.h))
+            self.h = None
+
+
+class Batch:
+    def __init__(self, ctx: Context, views, max_items: int, geom: int = XRH_GEOM_FP32):
+        arr = (View * len(views))()
+        for i, v in enumerate(views):
+            arr[i] = v
+        p = _P()
+        check(lib().xrh_batch_create(ctx.h, arr, len(views), int(max_items), int(geom), C.byref(p)))
+        self.ctx = ctx
+        self.h = p.value
+        self._views = arr
+
+    def set_options(self, use_graph: bool = True, time_kernels: bool = False) -> None:
+        check(lib().xrh_batch_set_options(self.h, int(use_graph), int(time_kernels)))
+
+    def eval(self, item_view: np.ndarray, item_pose: np.ndarray, drr: DeviceBuffer | None = None) -> np.ndarray:
+        iv = np.ascontiguousarray(item_view, dtype=np.int32)
+        ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
+        out = np.empty(iv.shape[0], dtype=np.float32)
+        check(lib().xrh_batch_eval(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0], _fp(out),
+                                   drr.ptr if drr else None))
+        return out
+
+    def launch(self, item_view: np.ndarray, item_pose: np.ndarray) -> None:
+        self._iv = np.ascontiguousarray(item_view, dtype=np.int32)
+        self._ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
+        check(lib().xrh_batch_launch(self.h, self._iv.ctypes.data_as(_PI32), _dp(self._ip), self._iv.shape[0]))
+
+    def wait(self) -> np.ndarray:
+        out = np.empty(self._iv.shape[0], dtype=np.float32)
+        check(lib().xrh_batch_wait(self.h, _fp(out)))
+        return out
+
+    def last_timing(self) -> tuple[float, float]:
+        a, t = C.c_float(0), C.c_float(0)
+        check(lib().xrh_batch_last_timing(self.h, C.byref(a), C.byref(t)))
+        return float(a.value), float(t.value)
+
+    def sample_count(self, item_view: np.ndarray, item_pose: np.ndarray) -> int:
+        iv = np.ascontiguousarray(item_view, dtype=np.int32)
+        ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
+        out = C.c_uint64(0)
+        check(lib().xrh_batch_sample_count(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0],
+                                           C.byref(out)))
+        return int(out.value)
+
+    def close(self) -> None:
+        if self.h:
+            check(lib().xrh_batch_destroy(self.h))
+            self.h = None
+
+
+def make_view(vol: Volume, fixed: Fixed, cam: Camera, step: float, kernel: int = XRH_LINE_INT_SUM) -> View:
+    return View(vol.h, fixed.h, cam, float(step), int(kernel))
+
+
+def gpu_available() -> bool:
+    """True when a HIP device is visible (checked without initialising the GPU runtime in this
+    process more than the library itself would)."""
+    if os.environ.get("HIP_VISIBLE_DEVICES", None) == "":
+        return False
+    return Path("/dev/kfd").exists()

@@ -1,0 +1,412 @@
+// xrh_batch.hip -- one optimiser generation (all poses x all views) as one submission.
+//
+// Per evaluation: one pinned upload of the pose records / tile plan / similarity items, the
+// per-item set-up kernel (FP64 pose composition), one DRR launch over every tile of every
+// item (uneven detector sizes allowed; NCC fused into the DRR epilogue), the similarity
+// kernels grouped by metric, the FP64 finish, one download of the similarity values. The
+// whole sequence is captured in a hipGraph the first time an item->view signature is seen
+// and replayed afterwards (poses change between generations, shapes do not).
+#include "xrh_internal.h"
+#include "xrh_sim.h"
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <map>
+
+using namespace xrh;
+
+struct xrh_batch {
+  xrh_ctx* ctx = nullptr;
+  int32_t geom = XRH_GEOM_FP32;
+  std::vector<xrh_view> views;
+  std::vector<const xrh_volume*> vols;  // distinct volumes
+  std::vector<int32_t> view_vol;        // view -> index into vols
+  int32_t storage = XRH_F32, layout = kLayoutQuad;
+  int64_t max_items = 0;
+  int64_t max_npix = 0;
+  int max_img_tiles = 0, max_patch_tiles = 0, max_lds = 0;
+  bool any_patch_grad = false;
+  bool fuse_ncc = true;            // NCC sums in the DRR epilogue (off if any NCC view is masked)
+  bool use_graph = true;
+  bool time_kernels = false;
+  // device
+  VolDesc* vols_dev = nullptr;
+  CamDev* cams_dev = nullptr;
+  char* blob_dev = nullptr;        // poses | band plan | sim items
+  uint64_t blob_bytes = 0;
+  DrrItem* items_dev = nullptr;
+  float* drr_dev = nullptr;
+  float* grad_dev = nullptr;
+  double* partials_dev = nullptr;
+  double* ncc_partials_dev = nullptr;
+  float* sims_dev = nullptr;
+  // host
+  char* blob_host = nullptr;       // pinned
+  float* sims_host = nullptr;      // pinned
+  hipEvent_t ev_drr0 = nullptr, ev_drr1 = nullptr, ev_all0 = nullptr, ev_all1 = nullptr;
+  bool timed_last = false;
+  std::map<std::string, hipGraphExec_t> graphs;
+  std::vector<hipGraph_t> graph_objs;
+  int64_t last_n = 0;
+  bool pending = false;
+};
+
+namespace {
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+struct BlobLayout {
+  uint64_t off_pose, off_band, off_sim, total;
+  uint64_t band_words;
+};
+
+BlobLayout blob_layout(int64_t max_items)
+{
+  BlobLayout L;
+  L.band_words = (uint64_t)8 * (max_items + 1);
+  L.off_pose = 0;
+  L.off_band = align256(sizeof(PoseIn) * max_items);
+  L.off_sim = align256(L.off_band + sizeof(uint32_t) * L.band_words);
+  L.total = align256(L.off_sim + sizeof(SimItem) * max_items);
+  return L;
+}
+
+void batch_free(xrh_batch* b)
+{
+  if (!b) return;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto g : b->graph_objs) (void)hipGraphDestroy(g);
+  for (void* p : {(void*)b->vols_dev, (void*)b->cams_dev, (void*)b->blob_dev, (void*)b->items_dev,
+                  (void*)b->drr_dev, (void*)b->grad_dev, (void*)b->partials_dev, (void*)b->ncc_partials_dev,
+                  (void*)b->sims_dev}) {
+    if (p) ctx_free(b->ctx, p);
+  }
+  if (b->blob_host) (void)hipHostFree(b->blob_host);
+  if (b->sims_host) (void)hipHostFree(b->sims_host);
+  for (hipEvent_t e : {b->ev_drr0, b->ev_drr1, b->ev_all0, b->ev_all1}) {
+    if (e) (void)hipEventDestroy(e);
+  }
+  delete b;
+}
+
+void batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views, int64_t max_items, int32_t geom,
+                  xrh_batch** out)
+{
+  XRH_REQUIRE(ctx && views && out, "null argument");
+  XRH_REQUIRE(num_views >= 1, "need at least one view");
+  XRH_REQUIRE(max_items >= 1 && max_items < 65536, "max_items must be in [1, 65535]");
+  XRH_REQUIRE(geom == XRH_GEOM_FP32 || geom == XRH_GEOM_FP64, "bad geometry mode");
+  auto* b = new xrh_batch();
+  b->ctx = ctx;
+  b->geom = geom;
+  b->max_items = max_items;
+  try {
+    std::vector<CamDev> cams;
+    std::vector<VolDesc> vds;
+    for (int32_t v = 0; v < num_views; ++v) {
+      const xrh_view& w = views[v];
+      XRH_REQUIRE(w.vol && w.fixed, "view needs a volume and a fixed image");
+      XRH_REQUIRE(w.vol->ctx == ctx && w.fixed->ctx == ctx, "view objects belong to another context");
+      XRH_REQUIRE(w.kernel == XRH_LINE_INT_SUM || w.kernel == XRH_LINE_INT_MAX, "bad line-integral kernel");
+      XRH_REQUIRE(w.step > 0.0f, "ray step must be positive");
+      check_step(w.step, w.vol->desc);
+      XRH_REQUIRE(w.cam.num_det_rows == w.fixed->host.rows && w.cam.num_det_cols == w.fixed->host.cols,
+                  "fixed image size differs from the view's detector");
+      b->views.push_back(w);
+      cams.push_back(make_cam_dev(w.cam));
+      auto it = std::find(b->vols.begin(), b->vols.end(), w.vol);
+      if (it == b->vols.end()) {
+        b->view_vol.push_back((int32_t)b->vols.size());
+        b->vols.push_back(w.vol);
+        vds.push_back(w.vol->desc);
+      } else {
+        b->view_vol.push_back((int32_t)(it - b->vols.begin()));
+      }
+      const FixedDev& f = w.fixed->host;
+      b->max_npix = std::max<int64_t>(b->max_npix, (int64_t)f.rows * f.cols);
+      b->max_img_tiles = std::max(b->max_img_tiles, sim_img_tiles(f));
+      b->max_patch_tiles = std::max(b->max_patch_tiles, sim_tiles(f));
+      b->max_lds = std::max(b->max_lds, sim_patch_lds_bytes(f));
+      if (f.metric == XRH_SIM_PATCH_GRAD_NCC) b->any_patch_grad = true;
+      if (f.metric == XRH_SIM_NCC && f.mask) b->fuse_ncc = false;
+    }
+    b->storage = vds[0].storage;
+    b->layout = vds[0].layout;
+    for (const VolDesc& d : vds) {
+      XRH_REQUIRE(d.storage == b->storage && d.layout == b->layout,
+                  "all volumes of one batch must share storage type and layout");
+    }
+    b->vols_dev = (VolDesc*)ctx_alloc(ctx, sizeof(VolDesc) * vds.size());
+    b->cams_dev = (CamDev*)ctx_alloc(ctx, sizeof(CamDev) * cams.size());
+    XRH_HIP_CHECK(hipMemcpyAsync(b->vols_dev, vds.data(), sizeof(VolDesc) * vds.size(), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    XRH_HIP_CHECK(hipMemcpyAsync(b->cams_dev, cams.data(), sizeof(CamDev) * cams.size(), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    const BlobLayout L = blob_layout(max_items);
+    b->blob_bytes = L.total;
+    b->blob_dev = (char*)ctx_alloc(ctx, L.total);
+    b->items_dev = (DrrItem*)ctx_alloc(ctx, sizeof(DrrItem) * max_items);
+    b->drr_dev = (float*)ctx_alloc(ctx, sizeof(float) * max_items * b->max_npix);
+    if (b->any_patch_grad) b->grad_dev = (float*)ctx_alloc(ctx, sizeof(float) * 2 * max_items * b->max_npix);
+    const int ptiles = std::max(b->max_img_tiles, b->max_patch_tiles);
+    b->partials_dev = (double*)ctx_alloc(ctx, sizeof(double) * 8 * ptiles * max_items);
+    b->ncc_partials_dev = (double*)ctx_alloc(ctx, sizeof(double) * 4 * b->max_img_tiles * max_items);
+    b->sims_dev = (float*)ctx_alloc(ctx, sizeof(float) * max_items);
+    XRH_HIP_CHECK(hipHostMalloc(&b->blob_host, L.total, hipHostMallocDefault));
+    XRH_HIP_CHECK(hipHostMalloc(&b->sims_host, sizeof(float) * max_items, hipHostMallocDefault));
+    XRH_HIP_CHECK(hipEventCreate(&b->ev_drr0));
+    XRH_HIP_CHECK(hipEventCreate(&b->ev_drr1));
+    XRH_HIP_CHECK(hipEventCreate(&b->ev_all0));
+    XRH_HIP_CHECK(hipEventCreate(&b->ev_all1));
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    batch_free(b);
+    throw;
+  }
+  *out = b;
+}
+
+// Fills the pinned staging for n items; returns the DRR launch description.
+DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, float* drr_out,
+                std::vector<std::pair<int, std::vector<int64_t>>>& groups)
+{
+  const BlobLayout L = blob_layout(b->max_items);
+  PoseIn* pin = (PoseIn*)(b->blob_host + L.off_pose);
+  SimItem* sin = (SimItem*)(b->blob_host + L.off_sim);
+  float* drr = drr_out ? drr_out : b->drr_dev;
+  std::vector<int32_t> rows((size_t)n), cols((size_t)n);
+  int64_t off = 0;
+  groups.clear();
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t v = item_view[i];
+    XRH_REQUIRE(v >= 0 && v < (int32_t)b->views.size(), "item view index out of range");
+    const xrh_view& w = b->views[v];
+    const FixedDev& f = w.fixed->host;
+    PoseIn& p = pin[i];
+    std::memcpy(p.pose, item_pose + 16 * i, sizeof(double) * 16);
+    p.cam = v;
+    p.vol = b->view_vol[v];
+    p.out_off = off;
+    p.ncc_zf = (f.metric == XRH_SIM_NCC && b->fuse_ncc) ? f.zf : nullptr;
+    p.step = w.step;
+    p.pad_ = 0;
+    rows[i] = w.cam.num_det_rows;
+    cols[i] = w.cam.num_det_cols;
+    SimItem& s = sin[i];
+    s.mov = drr + off;
+    s.fixed = w.fixed->dev;
+    s.gx = b->grad_dev ? b->grad_dev + 2 * i * b->max_npix : nullptr;
+    s.gy = b->grad_dev ? b->grad_dev + (2 * i + 1) * b->max_npix : nullptr;
+    s.out_index = i;
+    off += (int64_t)rows[i] * cols[i];
+  }
+  // items grouped by metric, in item order (sim items are re-ordered per group below)
+  for (int m = XRH_SIM_NCC; m <= XRH_SIM_PATCH_GRAD_NCC; ++m) {
+    std::vector<int64_t> idx;
+    for (int64_t i = 0; i < n; ++i) {
+      if (b->views[item_view[i]].fixed->host.metric == m) idx.push_back(i);
+    }
+    if (!idx.empty()) groups.emplace_back(m, std::move(idx));
+  }
+  // SimItem table laid out group by group
+  std::vector<SimItem> tmp(sin, sin + n);
+  int64_t k = 0;
+  for (auto& g : groups) {
+    for (int64_t i : g.second) sin[k++] = tmp[(size_t)i];
+  }
+  const TilePlan plan = build_tiles(rows, cols, 8);
+  std::memcpy(b->blob_host + L.off_band, plan.band_start.data(), sizeof(uint32_t) * plan.band_start.size());
+
+  DrrLaunch D;
+  std::memset(&D, 0, sizeof(D));
+  D.poses_dev = (const PoseIn*)(b->blob_dev + L.off_pose);
+  D.vols_dev = b->vols_dev;
+  D.cams_dev = b->cams_dev;
+  D.items_dev = b->items_dev;
+  D.band_start_dev = (const uint32_t*)(b->blob_dev + L.off_band);
+  D.num_items = n;
+  D.bands = plan.bands;
+  D.max_band_tiles = plan.max_band_tiles;
+  D.geom = b->geom;
+  D.kernel = b->views[item_view[0]].kernel;
+  D.store = XRH_STORE_REPLACE;
+  D.fill = 0.0f;
+  D.storage = b->storage;
+  D.layout = b->layout;
+  D.out = drr;
+  D.ncc_partials = b->ncc_partials_dev;
+  D.ncc_tiles_per_item = b->max_img_tiles;
+  return D;
+}
+
+void enqueue(xrh_batch* b, const DrrLaunch& D, const std::vector<std::pair<int, std::vector<int64_t>>>& groups,
+             int64_t n, bool timed)
+{
+  hipStream_t s = b->ctx->stream;
+  const BlobLayout L = blob_layout(b->max_items);
+  const uint64_t up = L.off_sim + sizeof(SimItem) * n;
+  XRH_HIP_CHECK(hipMemcpyAsync(b->blob_dev, b->blob_host, up, hipMemcpyHostToDevice, s));
+  launch_drr_setup(D, s);
+  if (timed) XRH_HIP_CHECK(hipEventRecord(b->ev_drr0, s));
+  launch_drr(D, s);
+  if (timed) XRH_HIP_CHECK(hipEventRecord(b->ev_drr1, s));
+  const SimItem* sim_items = (const SimItem*)(b->blob_dev + L.off_sim);
+  int64_t k = 0;
+  for (auto& g : groups) {
+    const int64_t cnt = (int64_t)g.second.size();
+    // NCC sums were written by the DRR epilogue (records keyed by item index)
+    const bool fused = (g.first == XRH_SIM_NCC) && b->fuse_ncc;
+    launch_sim(g.first, sim_items + k, cnt, b->max_img_tiles, b->max_patch_tiles,
+               fused ? b->ncc_partials_dev : b->partials_dev, b->sims_dev, fused, b->max_lds, s);
+    k += cnt;
+  }
+  XRH_HIP_CHECK(hipMemcpyAsync(b->sims_host, b->sims_dev, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+}
+
+std::string signature(const int32_t* item_view, int64_t n, const float* drr_out)
+{
+  std::string key((const char*)item_view, (size_t)n * sizeof(int32_t));
+  key.append((const char*)&drr_out, sizeof(drr_out));
+  return key;
+}
+
+void launch(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, float* drr_out)
+{
+  XRH_REQUIRE(b && item_view && item_pose, "null argument");
+  XRH_REQUIRE(n >= 1 && n <= b->max_items, "item count must be in [1, max_items]");
+  XRH_REQUIRE(!b->pending, "previous launch not yet waited for");
+  hipStream_t s = b->ctx->stream;
+  XRH_HIP_CHECK(hipStreamSynchronize(s));  // pinned staging is reused
+  std::vector<std::pair<int, std::vector<int64_t>>> groups;
+  const DrrLaunch D = stage(b, item_view, item_pose, n, drr_out, groups);
+  const bool timed = b->time_kernels || !b->use_graph;
+  b->timed_last = timed;
+  if (timed) XRH_HIP_CHECK(hipEventRecord(b->ev_all0, s));
+  if (b->use_graph && !b->time_kernels) {
+    const std::string key = signature(item_view, n, drr_out);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+      hipGraph_t g = nullptr;
+      XRH_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(b, D, groups, n, false);
+      } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(s, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+      }
+      XRH_HIP_CHECK(hipStreamEndCapture(s, &g));
+      hipGraphExec_t ge = nullptr;
+      XRH_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      b->graph_objs.push_back(g);
+      it = b->graphs.emplace(key, ge).first;
+    }
+    XRH_HIP_CHECK(hipGraphLaunch(it->second, s));
+  } else {
+    enqueue(b, D, groups, n, timed);
+  }
+  if (timed) XRH_HIP_CHECK(hipEventRecord(b->ev_all1, s));
+  b->last_n = n;
+  b->pending = true;
+}
+
+void wait(xrh_batch* b, float* sim_host_out)
+{
+  XRH_REQUIRE(b, "null batch");
+  XRH_REQUIRE(b->pending, "no launch to wait for");
+  XRH_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  b->pending = false;
+  if (sim_host_out) std::memcpy(sim_host_out, b->sims_host, sizeof(float) * b->last_n);
+}
+
+}  // namespace
+
+extern "C" {
+
+xrh_status xrh_batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views, int64_t max_items,
+                            int32_t geom, xrh_batch** out)
+{
+  return guarded([&] { batch_create(ctx, views, num_views, max_items, geom, out); });
+}
+
+xrh_status xrh_batch_destroy(xrh_batch* batch)
+{
+  return guarded([&] { batch_free(batch); });
+}
+
+xrh_status xrh_batch_set_options(xrh_batch* batch, int32_t use_graph, int32_t time_kernels)
+{
+  return guarded([&] {
+    XRH_REQUIRE(batch, "null batch");
+    XRH_REQUIRE(!batch->pending, "batch has a launch in flight");
+    batch->use_graph = use_graph != 0;
+    batch->time_kernels = time_kernels != 0;
+  });
+}
+
+xrh_status xrh_batch_eval(xrh_batch* batch, const int32_t* item_view, const double* item_pose, int64_t n,
+                          float* sim_host_out, float* drr_dev_out)
+{
+  return guarded([&] {
+    launch(batch, item_view, item_pose, n, drr_dev_out);
+    wait(batch, sim_host_out);
+  });
+}
+
+xrh_status xrh_batch_launch(xrh_batch* batch, const int32_t* item_view, const double* item_pose, int64_t n)
+{
+  return guarded([&] { launch(batch, item_view, item_pose, n, nullptr); });
+}
+
+xrh_status xrh_batch_wait(xrh_batch* batch, float* sim_host_out)
+{
+  return guarded([&] { wait(batch, sim_host_out); });
+}
+
+xrh_status xrh_batch_last_timing(const xrh_batch* batch, float* drr_kernel_ms, float* total_ms)
+{
+  return guarded([&] {
+    XRH_REQUIRE(batch, "null batch");
+    XRH_REQUIRE(!batch->pending, "launch still in flight");
+    XRH_REQUIRE(batch->timed_last, "last launch was not timed (enable time_kernels)");
+    float a = 0.0f, t = 0.0f;
+    if (drr_kernel_ms) {
+      XRH_HIP_CHECK(hipEventElapsedTime(&a, batch->ev_drr0, batch->ev_drr1));
+      *drr_kernel_ms = a;
+    }
+    if (total_ms) {
+      XRH_HIP_CHECK(hipEventElapsedTime(&t, batch->ev_all0, batch->ev_all1));
+      *total_ms = t;
+    }
+  });
+}
+
+xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view, const double* item_pose,
+                                  int64_t n, uint64_t* samples_out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(batch && samples_out, "null argument");
+    XRH_REQUIRE(!batch->pending, "batch has a launch in flight");
+    XRH_REQUIRE(n >= 1 && n <= batch->max_items, "item count must be in [1, max_items]");
+    hipStream_t s = batch->ctx->stream;
+    XRH_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<std::pair<int, std::vector<int64_t>>> groups;
+    DrrLaunch D = stage(batch, item_view, item_pose, n, nullptr, groups);
+    const BlobLayout L = blob_layout(batch->max_items);
+    XRH_HIP_CHECK(hipMemcpyAsync(batch->blob_dev, batch->blob_host, L.off_sim, hipMemcpyHostToDevice, s));
+    uint64_t* cnt = (uint64_t*)batch->sims_dev;  // reuse: >= 8 bytes when max_items >= 2
+    XRH_REQUIRE(batch->max_items >= 2, "sample count needs max_items >= 2");
+    const uint64_t zero = 0;
+    XRH_HIP_CHECK(hipMemcpyAsync(cnt, &zero, 8, hipMemcpyHostToDevice, s));
+    D.sample_counter = cnt;
+    launch_drr_setup(D, s);
+    launch_drr(D, s);
+    XRH_HIP_CHECK(hipMemcpyAsync(samples_out, cnt, 8, hipMemcpyDeviceToHost, s));
+    XRH_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,601 @@
+// xrh_drr.hip -- line-integral DRR kernels for gfx950.
+//
+// Semantics follow RayCasterLineIntCPU::compute / ComputeLineInts
+// (lib/ray_cast/xregRayCastLineIntCPU.cpp:105-349):
+//   per ray: detector point (CameraModel::ind_pt_to_phys_det_pt), pinhole and direction in
+//   volume index space, slab intersection against [0, N-1] limited to the source->detector
+//   segment, 1e-3 parametric nudge, n = floor(chord / step_len), n+1 trilinear samples
+//   accumulated with SUM or MAX, times the physical step; stored with the same operator.
+//
+// Mapping onto CDNA4:
+//   - one lane = one ray; a 256-lane workgroup owns a 16x16 detector tile and each of its
+//     four wave64s an 8x8 sub-tile, so neighbouring lanes march neighbouring pixels and
+//     their gathers hit the same cache lines;
+//   - workgroups are ordered band-major and swizzled so that the workgroups that the
+//     dispatcher sends to one XCD (blockIdx mod 8) all work on the same band of detector
+//     rows across every projection: each XCD's L2 then holds one slab of the volume;
+//   - the volume sits in HBM in an edge-padded layout read through buffer loads
+//     (hardware range check: a stray offset returns 0, it can never fault); the quad (f32)
+//     and oct (f16/bf16) layouts pre-pack interpolation corners so a sample costs 2 or 1
+//     16-byte loads instead of 8 scalar gathers;
+//   - trilinear weights and lerps are FP32 in software (hardware texture filtering has
+//     8-bit weights, far outside the 1e-4 per-pixel tolerance).
+#include "xrh_internal.h"
+
+#include <hip/hip_fp16.h>
+
+namespace xrh {
+namespace {
+
+constexpr int kTile = 16;          // detector tile edge (pixels)
+constexpr int kThreads = kTile * kTile;
+
+// ---------------------------------------------------------------------------------------
+// float geometry in the reference's operation order, no contraction (XRH_GEOM_FP32)
+
+__device__ inline float dot3(const float* r, const float v0, const float v1, const float v2)
+{
+#pragma clang fp contract(off)
+  return (r[0] * v0 + r[1] * v1) + r[2] * v2;
+}
+
+__device__ inline void affine_pt(const float* m, const float p[3], float out[3])
+{
+#pragma clang fp contract(off)
+  const float a = dot3(&m[0], p[0], p[1], p[2]) + m[3];
+  const float b = dot3(&m[4], p[0], p[1], p[2]) + m[7];
+  const float c = dot3(&m[8], p[0], p[1], p[2]) + m[11];
+  out[0] = a;
+  out[1] = b;
+  out[2] = c;
+}
+
+__device__ inline void affine_pt_d(const double* m, const double p[3], double out[3])
+{
+  const double a = (m[0] * p[0] + m[1] * p[1]) + m[2] * p[2] + m[3];
+  const double b = (m[4] * p[0] + m[5] * p[1]) + m[6] * p[2] + m[7];
+  const double c = (m[8] * p[0] + m[9] * p[1]) + m[10] * p[2] + m[11];
+  out[0] = a;
+  out[1] = b;
+  out[2] = c;
+}
+
+// CameraModel::ind_pt_to_phys_det_pt (xregPerspectiveXform.cpp:401-414)
+__device__ inline void ind_to_det(const CamDev& c, float col, float row, float out[3])
+{
+#pragma clang fp contract(off)
+  const float q0 = c.det_z * col;
+  const float q1 = c.det_z * row;
+  const float q2 = c.det_z * 1.0f;
+  float r[3];
+  r[0] = dot3(&c.kinv[0], q0, q1, q2) + 0.0f;
+  r[1] = dot3(&c.kinv[3], q0, q1, q2) + 0.0f;
+  r[2] = dot3(&c.kinv[6], q0, q1, q2) + c.z_adj;
+  affine_pt(c.einv, r, out);
+}
+
+__device__ inline void ind_to_det_d(const CamDev& c, double col, double row, double out[3])
+{
+  const double q0 = c.det_z_d * col, q1 = c.det_z_d * row, q2 = c.det_z_d;
+  double r[3];
+  for (int i = 0; i < 3; ++i) {
+    r[i] = (c.kinv_d[3 * i] * q0 + c.kinv_d[3 * i + 1] * q1) + c.kinv_d[3 * i + 2] * q2;
+  }
+  r[2] += c.z_adj_d;
+  affine_pt_d(c.einv_d, r, out);
+}
+
+// RayRectIntersect with limit_to_segment (lib/spatial/xregSpatialPrimitives.cpp:175-222)
+template <class T>
+__device__ inline bool ray_rect(const T mx[3], const T p0[3], const T d[3], T& t0, T& t1)
+{
+#pragma clang fp contract(off)
+  T ts = T(0), te = T(1);
+  for (int k = 0; k < 3; ++k) {
+    const T dk = d[k];
+    // the reference compares the float component against the double literal 1.0e-8
+    if ((double)fabs(dk) > 1.0e-8) {
+      const T inv = T(1) / dk;
+      T a = (T(0) - p0[k]) * inv;
+      T b = (mx[k] - p0[k]) * inv;
+      if (b < a) {
+        const T t = a;
+        a = b;
+        b = t;
+      }
+      ts = (ts > a) ? ts : a;
+      te = (b < te) ? b : te;
+      if (ts > te) {
+        t0 = ts;
+        t1 = te;
+        return false;
+      }
+    } else if ((p0[k] < T(0)) || (p0[k] > mx[k])) {
+      t0 = ts;
+      t1 = te;
+      return false;
+    }
+  }
+  t0 = ts;
+  t1 = te;
+  return true;
+}
+
+struct Ray {
+  float sx, sy, sz;  // first sample (continuous index)
+  float dx, dy, dz;  // step vector (index units)
+  int n;             // last sample index (n+1 samples), -1 = ray misses
+};
+
+// Per-ray set-up in float, reference order (xregRayCastLineIntCPU.cpp:204-268).
+__device__ inline Ray setup_ray_f32(const DrrItem& it, int col, int row)
+{
+#pragma clang fp contract(off)
+  Ray ray;
+  ray.n = -1;
+  float det[3];
+  ind_to_det(it.cam, (float)col, (float)row, det);
+  float ph[3], dp[3];
+  affine_pt(it.M, it.cam.pinhole, ph);
+  affine_pt(it.M, det, dp);
+  const float d[3] = {dp[0] - ph[0], dp[1] - ph[1], dp[2] - ph[2]};
+  const float mx[3] = {(float)(it.X - 1), (float)(it.Y - 1), (float)(it.Z - 1)};
+  float t0, t1;
+  const bool hit = ray_rect<float>(mx, ph, d, t0, t1);
+  const float kTol = 1.0e-3f;
+  if (hit && ((t1 - t0) > 2.0f * kTol)) {
+    t0 = t0 + kTol;
+    t1 = t1 - kTol;
+    ray.sx = ph[0] + t0 * d[0];
+    ray.sy = ph[1] + t0 * d[1];
+    ray.sz = ph[2] + t0 * d[2];
+    const float dlen = __fsqrt_rn((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    const float L = (t1 - t0) * dlen;
+    float v0 = det[0] - it.cam.pinhole[0];
+    float v1 = det[1] - it.cam.pinhole[1];
+    float v2 = det[2] - it.cam.pinhole[2];
+    const float vz = (v0 * v0 + v1 * v1) + v2 * v2;
+    if (vz > 0.0f) {
+      const float vn = __fsqrt_rn(vz);
+      v0 = __fdiv_rn(v0, vn);
+      v1 = __fdiv_rn(v1, vn);
+      v2 = __fdiv_rn(v2, vn);
+    }
+    v0 = v0 * it.step;
+    v1 = v1 * it.step;
+    v2 = v2 * it.step;
+    const float s0 = dot3(&it.M[0], v0, v1, v2);
+    const float s1 = dot3(&it.M[4], v0, v1, v2);
+    const float s2 = dot3(&it.M[8], v0, v1, v2);
+    const float s = __fsqrt_rn((s0 * s0 + s1 * s1) + s2 * s2);
+    ray.n = (int)__fdiv_rn(L, s);
+    const float scale = __fdiv_rn(s, dlen);
+    ray.dx = d[0] * scale;
+    ray.dy = d[1] * scale;
+    ray.dz = d[2] * scale;
+  }
+  return ray;
+}
+
+// Per-ray set-up in double; samples later placed at start + i*step in float.
+__device__ inline Ray setup_ray_f64(const DrrItem& it, int col, int row)
+{
+  Ray ray;
+  ray.n = -1;
+  double det[3];
+  ind_to_det_d(it.cam, (double)col, (double)row, det);
+  double ph[3], dp[3];
+  affine_pt_d(it.Md, it.cam.pinhole_d, ph);
+  affine_pt_d(it.Md, det, dp);
+  const double d[3] = {dp[0] - ph[0], dp[1] - ph[1], dp[2] - ph[2]};
+  const double mx[3] = {(double)(it.X - 1), (double)(it.Y - 1), (double)(it.Z - 1)};
+  double t0, t1;
+  const bool hit = ray_rect<double>(mx, ph, d, t0, t1);
+  if (hit && ((t1 - t0) > 2.0e-3)) {
+    t0 += 1.0e-3;
+    t1 -= 1.0e-3;
+    const double dlen = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double L = (t1 - t0) * dlen;
+    double v0 = det[0] - it.cam.pinhole_d[0];
+    double v1 = det[1] - it.cam.pinhole_d[1];
+    double v2 = det[2] - it.cam.pinhole_d[2];
+    const double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    if (vn > 0.0) {
+      v0 /= vn;
+      v1 /= vn;
+      v2 /= vn;
+    }
+    const double st = (double)it.step;
+    v0 *= st;
+    v1 *= st;
+    v2 *= st;
+    const double s0 = it.Md[0] * v0 + it.Md[1] * v1 + it.Md[2] * v2;
+    const double s1 = it.Md[4] * v0 + it.Md[5] * v1 + it.Md[6] * v2;
+    const double s2 = it.Md[8] * v0 + it.Md[9] * v1 + it.Md[10] * v2;
+    const double s = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+    ray.n = (int)(L / s);
+    const double scale = s / dlen;
+    ray.sx = (float)(ph[0] + t0 * d[0]);
+    ray.sy = (float)(ph[1] + t0 * d[1]);
+    ray.sz = (float)(ph[2] + t0 * d[2]);
+    ray.dx = (float)(d[0] * scale);
+    ray.dy = (float)(d[1] * scale);
+    ray.dz = (float)(d[2] * scale);
+  }
+  return ray;
+}
+
+// ---------------------------------------------------------------------------------------
+// trilinear sampling per layout. Formats: 0 f32 linear, 1 f32 quad, 2 f16 linear, 3 f16 oct,
+// 4 bf16 linear, 5 bf16 oct.
+
+struct VolView {
+  __amdgpu_buffer_rsrc_t rsrc;
+  uint32_t py;   // row pitch (elements)
+  uint32_t pz;   // slice pitch (elements)
+};
+
+__device__ inline float lerp(float a, float b, float w) { return fmaf(b - a, w, a); }
+
+__device__ inline float tri(float v000, float v100, float v010, float v110, float v001,
+                            float v101, float v011, float v111, float wx, float wy, float wz)
+{
+  // x, then y, then z -- the order of ITK's LinearInterpolateImageFunction
+  const float x00 = lerp(v000, v100, wx);
+  const float x10 = lerp(v010, v110, wx);
+  const float x01 = lerp(v001, v101, wx);
+  const float x11 = lerp(v011, v111, wx);
+  const float xx0 = lerp(x00, x10, wy);
+  const float xx1 = lerp(x01, x11, wy);
+  return lerp(xx0, xx1, wz);
+}
+
+__device__ inline float h2f(uint32_t bits16) { return __half2float(__ushort_as_half((unsigned short)bits16)); }
+__device__ inline float bf2f(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+
+template <int kFmt>
+__device__ inline float sample(const VolView& v, float x, float y, float z)
+{
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const float wx = x - fx, wy = y - fy, wz = z - fz;
+  // +1: the apron occupies padded index 0
+  const uint32_t e = (uint32_t)((int)fx + 1) + v.py * (uint32_t)((int)fy + 1) +
+                     v.pz * (uint32_t)((int)fz + 1);
+  if constexpr (kFmt == 0) {
+    const int o = (int)(e * 4u);
+    const int oy = (int)(v.py * 4u);
+    const int sz = (int)(v.pz * 4u);
+    const float v000 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o, 0, 0));
+    const float v100 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o + 4, 0, 0));
+    const float v010 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o + oy, 0, 0));
+    const float v110 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o + oy + 4, 0, 0));
+    const float v001 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o, sz, 0));
+    const float v101 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o + 4, sz, 0));
+    const float v011 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o + oy, sz, 0));
+    const float v111 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v.rsrc, o + oy + 4, sz, 0));
+    return tri(v000, v100, v010, v110, v001, v101, v011, v111, wx, wy, wz);
+  } else if constexpr (kFmt == 1) {
+    const int o = (int)(e * 16u);
+    const int sz = (int)(v.pz * 16u);
+    const auto q0 = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, o, 0, 0);
+    const auto q1 = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, o, sz, 0);
+    return tri(__uint_as_float(q0[0]), __uint_as_float(q0[1]), __uint_as_float(q0[2]),
+               __uint_as_float(q0[3]), __uint_as_float(q1[0]), __uint_as_float(q1[1]),
+               __uint_as_float(q1[2]), __uint_as_float(q1[3]), wx, wy, wz);
+  } else if constexpr (kFmt == 2 || kFmt == 4) {
+    const int o = (int)(e * 2u);
+    const int oy = (int)(v.py * 2u);
+    const int sz = (int)(v.pz * 2u);
+    uint32_t r[8];
+    r[0] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o, 0, 0);
+    r[1] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o + 2, 0, 0);
+    r[2] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o + oy, 0, 0);
+    r[3] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o + oy + 2, 0, 0);
+    r[4] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o, sz, 0);
+    r[5] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o + 2, sz, 0);
+    r[6] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o + oy, sz, 0);
+    r[7] = __builtin_amdgcn_raw_buffer_load_b16(v.rsrc, o + oy + 2, sz, 0);
+    float f[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) f[i] = (kFmt == 2) ? h2f(r[i]) : bf2f(r[i]);
+    return tri(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], wx, wy, wz);
+  } else {
+    const int o = (int)(e * 16u);
+    const auto q = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, o, 0, 0);
+    float f[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t lo = q[i] & 0xffffu, hi = q[i] >> 16;
+      f[2 * i] = (kFmt == 3) ? h2f(lo) : bf2f(lo);
+      f[2 * i + 1] = (kFmt == 3) ? h2f(hi) : bf2f(hi);
+    }
+    return tri(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], wx, wy, wz);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// tile decode: blockIdx -> (item, tile_x, tile_y) through the band plan.
+
+struct TileRef {
+  int64_t item;
+  int tx, ty;
+  bool valid;
+};
+
+__device__ inline TileRef decode_tile(const uint32_t* band_start, int64_t num_items, int bands,
+                                      const DrrItem* items)
+{
+  TileRef t;
+  const uint32_t b = blockIdx.x;
+  const int band = (int)(b % (uint32_t)bands);
+  const uint32_t j = b / (uint32_t)bands;
+  const uint32_t* bs = band_start + (int64_t)band * (num_items + 1);
+  t.valid = j < bs[num_items];
+  if (!t.valid) return t;
+  // largest item with bs[item] <= j (uniform binary search on scalar registers)
+  int64_t lo = 0, hi = num_items - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (bs[mid] <= j) lo = mid; else hi = mid - 1;
+  }
+  t.item = lo;
+  const uint32_t local = j - bs[lo];
+  const int TX = items[lo].tiles_x, TY = items[lo].tiles_y;
+  const int row0 = (int)(((int64_t)band * TY) / bands);
+  t.ty = row0 + (int)(local / (uint32_t)TX);
+  t.tx = (int)(local % (uint32_t)TX);
+  return t;
+}
+
+__device__ inline void lane_pixel(int tx, int ty, int& col, int& row)
+{
+  const int t = threadIdx.x;
+  const int w = t >> 6, l = t & 63;
+  col = tx * kTile + (w & 1) * 8 + (l & 7);
+  row = ty * kTile + (w >> 1) * 8 + (l >> 3);
+}
+
+__device__ inline double wave_sum_d(double v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------------------
+// kernels
+
+__global__ void drr_setup_kernel(const PoseIn* __restrict__ poses, const VolDesc* __restrict__ vols,
+                                 const CamDev* __restrict__ cams, DrrItem* __restrict__ items,
+                                 int64_t n)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PoseIn& p = poses[i];
+  const VolDesc& v = vols[p.vol];
+  DrrItem it;
+  // phys->idx * cam->phys: Affine*Affine product (xregRayCastLineIntCPU.cpp:207-208), float
+  float T[12];
+  for (int k = 0; k < 12; ++k) T[k] = (float)p.pose[k];
+  for (int r = 0; r < 3; ++r) {
+    const float* a = &v.p2i[4 * r];
+    for (int c = 0; c < 3; ++c) it.M[4 * r + c] = (a[0] * T[c] + a[1] * T[4 + c]) + a[2] * T[8 + c];
+    it.M[4 * r + 3] = ((a[0] * T[3] + a[1] * T[7]) + a[2] * T[11]) + a[3];
+  }
+  // the same in double from the FP64 pose
+  for (int r = 0; r < 3; ++r) {
+    const double* a = &v.p2i_d[4 * r];
+    for (int c = 0; c < 3; ++c) {
+      it.Md[4 * r + c] = a[0] * p.pose[c] + a[1] * p.pose[4 + c] + a[2] * p.pose[8 + c];
+    }
+    it.Md[4 * r + 3] = a[0] * p.pose[3] + a[1] * p.pose[7] + a[2] * p.pose[11] + a[3];
+  }
+  it.cam = cams[p.cam];
+  it.vol = v.data;
+  it.vol_bytes = v.bytes;
+  it.X = v.X;
+  it.Y = v.Y;
+  it.Z = v.Z;
+  it.pitch_y = v.pitch_y;
+  it.pitch_z = v.pitch_z;
+  it.step = p.step;
+  it.out_off = p.out_off;
+  it.ncc_zf = p.ncc_zf;
+  it.tiles_x = (it.cam.cols + kTile - 1) / kTile;
+  it.tiles_y = (it.cam.rows + kTile - 1) / kTile;
+  it.pad_[0] = it.pad_[1] = 0;
+  items[i] = it;
+}
+
+template <int kGeom, int kOp, int kFmt>
+__global__ __launch_bounds__(kThreads) void drr_kernel(
+    const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
+    int bands, float* __restrict__ out, int accumulate, float fill,
+    double* __restrict__ ncc_partials, int ncc_tiles)
+{
+  const TileRef tr = decode_tile(band_start, num_items, bands, items);
+  if (!tr.valid) return;
+  const DrrItem& it = items[tr.item];
+  int col, row;
+  lane_pixel(tr.tx, tr.ty, col, row);
+  const bool inside = (col < it.cam.cols) && (row < it.cam.rows);
+
+  float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
+  if (inside) {
+    const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
+    if (ray.n >= 0) {
+      VolView vv;
+      vv.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(it.vol), (short)0,
+                                                  (int)it.vol_bytes, 0x00020000);
+      vv.py = it.pitch_y;
+      vv.pz = it.pitch_z;
+      if constexpr (kGeom == XRH_GEOM_FP32) {
+        // positions advance by repeated float addition, as the reference does (:276)
+        float px = ray.sx, py = ray.sy, pz = ray.sz;
+#pragma unroll 2
+        for (int i = 0; i <= ray.n; ++i) {
+          const float s = sample<kFmt>(vv, px, py, pz);
+          if constexpr (kOp == XRH_LINE_INT_SUM) acc = acc + s; else acc = (acc < s) ? s : acc;
+          px = px + ray.dx;
+          py = py + ray.dy;
+          pz = pz + ray.dz;
+        }
+      } else {
+#pragma unroll 2
+        for (int i = 0; i <= ray.n; ++i) {
+          const float fi = (float)i;
+          const float s = sample<kFmt>(vv, fmaf(fi, ray.dx, ray.sx), fmaf(fi, ray.dy, ray.sy),
+                                       fmaf(fi, ray.dz, ray.sz));
+          if constexpr (kOp == XRH_LINE_INT_SUM) acc = acc + s; else acc = (acc < s) ? s : acc;
+        }
+      }
+      acc = acc * it.step;
+    }
+  }
+  float pix = 0.0f;
+  if (inside) {
+    const int64_t o = it.out_off + (int64_t)row * it.cam.cols + col;
+    const float prev = accumulate ? out[o] : fill;
+    const float aa = 0.0f + acc * 1.0f;
+    if constexpr (kOp == XRH_LINE_INT_SUM) pix = prev + aa; else pix = (prev < aa) ? aa : prev;
+    out[o] = pix;
+  }
+  if (ncc_partials && it.ncc_zf) {
+    // NCC epilogue: sums of m, m^2 and m * zero-mean fixed, FP64, one record per tile
+    double s1 = 0.0, s2 = 0.0, sf = 0.0;
+    if (inside) {
+      const double m = (double)pix;
+      const double f = (double)it.ncc_zf[(int64_t)row * it.cam.cols + col];
+      s1 = m;
+      s2 = m * m;
+      sf = m * f;
+    }
+    s1 = wave_sum_d(s1);
+    s2 = wave_sum_d(s2);
+    sf = wave_sum_d(sf);
+    __shared__ double red[4][3];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      red[w][0] = s1;
+      red[w][1] = s2;
+      red[w][2] = sf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double* dst = ncc_partials + ((int64_t)tr.item * ncc_tiles + (int64_t)tr.ty * it.tiles_x + tr.tx) * 4;
+      dst[0] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+      dst[1] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+      dst[2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
+      dst[3] = 0.0;
+    }
+  }
+}
+
+template <int kGeom>
+__global__ __launch_bounds__(kThreads) void drr_count_kernel(
+    const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
+    int bands, unsigned long long* __restrict__ counter)
+{
+  const TileRef tr = decode_tile(band_start, num_items, bands, items);
+  if (!tr.valid) return;
+  const DrrItem& it = items[tr.item];
+  int col, row;
+  lane_pixel(tr.tx, tr.ty, col, row);
+  double cnt = 0.0;
+  if ((col < it.cam.cols) && (row < it.cam.rows)) {
+    const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
+    cnt = (double)(ray.n + 1);
+  }
+  cnt = wave_sum_d(cnt);
+  if ((threadIdx.x & 63) == 0) atomicAdd(counter, (unsigned long long)cnt);
+}
+
+template <int kGeom, int kOp>
+void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, float fill)
+{
+#define XRH_LAUNCH_FMT(F)                                                                    \
+  hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev,   \
+                     L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,        \
+                     L.ncc_partials, L.ncc_tiles_per_item)
+  const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutQuad ? 1 : 0)
+                : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
+                                         : (L.layout == kLayoutOct ? 5 : 4);
+  switch (fmt) {
+    case 0: XRH_LAUNCH_FMT(0); break;
+    case 1: XRH_LAUNCH_FMT(1); break;
+    case 2: XRH_LAUNCH_FMT(2); break;
+    case 3: XRH_LAUNCH_FMT(3); break;
+    case 4: XRH_LAUNCH_FMT(4); break;
+    default: XRH_LAUNCH_FMT(5); break;
+  }
+#undef XRH_LAUNCH_FMT
+}
+
+}  // namespace
+
+TilePlan build_tiles(const std::vector<int32_t>& rows, const std::vector<int32_t>& cols, int32_t bands)
+{
+  TilePlan p;
+  p.bands = bands < 1 ? 1 : bands;
+  const int64_t n = (int64_t)rows.size();
+  p.band_start.assign((size_t)p.bands * (n + 1), 0);
+  p.max_band_tiles = 0;
+  for (int b = 0; b < p.bands; ++b) {
+    uint32_t acc = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      const int TX = (cols[i] + kTile - 1) / kTile;
+      const int TY = (rows[i] + kTile - 1) / kTile;
+      const int r0 = (int)(((int64_t)b * TY) / p.bands);
+      const int r1 = (int)(((int64_t)(b + 1) * TY) / p.bands);
+      p.band_start[(size_t)b * (n + 1) + i] = acc;
+      acc += (uint32_t)((r1 - r0) * TX);
+    }
+    p.band_start[(size_t)b * (n + 1) + n] = acc;
+    if ((int64_t)acc > p.max_band_tiles) p.max_band_tiles = acc;
+  }
+  return p;
+}
+
+void launch_drr_setup(const DrrLaunch& L, hipStream_t s)
+{
+  if (L.num_items <= 0) return;
+  const int threads = 64;
+  const int blocks = (int)((L.num_items + threads - 1) / threads);
+  hipLaunchKernelGGL(drr_setup_kernel, dim3(blocks), dim3(threads), 0, s, L.poses_dev, L.vols_dev,
+                     L.cams_dev, L.items_dev, L.num_items);
+  XRH_HIP_CHECK(hipGetLastError());
+}
+
+void launch_drr(const DrrLaunch& L, hipStream_t s)
+{
+  if (L.num_items <= 0 || L.max_band_tiles <= 0) return;
+  const int64_t nblocks = L.max_band_tiles * L.bands;
+  XRH_REQUIRE(nblocks < (int64_t)INT32_MAX, "too many DRR tiles in one launch");
+  const dim3 grid((unsigned)nblocks);
+  if (L.sample_counter) {
+    if (L.geom == XRH_GEOM_FP32) {
+      hipLaunchKernelGGL((drr_count_kernel<XRH_GEOM_FP32>), grid, dim3(kThreads), 0, s, L.items_dev,
+                         L.band_start_dev, L.num_items, L.bands,
+                         (unsigned long long*)L.sample_counter);
+    } else {
+      hipLaunchKernelGGL((drr_count_kernel<XRH_GEOM_FP64>), grid, dim3(kThreads), 0, s, L.items_dev,
+                         L.band_start_dev, L.num_items, L.bands,
+                         (unsigned long long*)L.sample_counter);
+    }
+    XRH_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const int accumulate = (L.store == XRH_STORE_ACCUM) ? 1 : 0;
+  const float fill = L.fill;
+  if (L.geom == XRH_GEOM_FP32) {
+    if (L.kernel == XRH_LINE_INT_SUM) launch_fmt<XRH_GEOM_FP32, XRH_LINE_INT_SUM>(L, grid, s, accumulate, fill);
+    else launch_fmt<XRH_GEOM_FP32, XRH_LINE_INT_MAX>(L, grid, s, accumulate, fill);
+  } else {
+    if (L.kernel == XRH_LINE_INT_SUM) launch_fmt<XRH_GEOM_FP64, XRH_LINE_INT_SUM>(L, grid, s, accumulate, fill);
+    else launch_fmt<XRH_GEOM_FP64, XRH_LINE_INT_MAX>(L, grid, s, accumulate, fill);
+  }
+  XRH_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace xrh

@@ -1,0 +1,547 @@
+// xrh_runtime.hip -- context, budgeted device memory, resident volumes and xrh_line_int.
+#include "xrh_internal.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <cstdlib>
+
+namespace xrh {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+[[noreturn]] void fail(xrh_status s, const std::string& msg) { throw Error{s, msg}; }
+
+void* ctx_alloc(xrh_ctx* ctx, uint64_t bytes)
+{
+  if (bytes == 0) bytes = 256;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->budget && ctx->in_use + bytes > ctx->budget) {
+      fail(XRH_ERR_BUDGET, "HBM budget exceeded: " + std::to_string(ctx->in_use) + " + " +
+                               std::to_string(bytes) + " > " + std::to_string(ctx->budget));
+    }
+    ctx->in_use += bytes;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->in_use -= bytes;
+    (void)hipGetLastError();
+    fail(XRH_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->allocs[p] = bytes;
+  return p;
+}
+
+void ctx_free(xrh_ctx* ctx, void* p)
+{
+  if (!p) return;
+  uint64_t bytes = 0;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->allocs.find(p);
+    XRH_REQUIRE(it != ctx->allocs.end(), "device pointer was not allocated by this context");
+    bytes = it->second;
+    ctx->allocs.erase(it);
+    ctx->in_use -= bytes;
+  }
+  XRH_HIP_CHECK(hipFree(p));
+}
+
+void* ctx_scratch(xrh_ctx* ctx, uint64_t bytes)
+{
+  if (ctx->scratch_bytes >= bytes) return ctx->scratch;
+  XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (ctx->scratch) ctx_free(ctx, ctx->scratch);
+  ctx->scratch = nullptr;
+  ctx->scratch_bytes = 0;
+  const uint64_t want = std::max<uint64_t>(bytes, 1ull << 20);
+  ctx->scratch = ctx_alloc(ctx, want);
+  ctx->scratch_bytes = want;
+  return ctx->scratch;
+}
+
+void* ctx_pinned(xrh_ctx* ctx, uint64_t bytes)
+{
+  // the caller has synchronised the stream before reusing the staging area
+  if (ctx->pinned_bytes >= bytes) return ctx->pinned;
+  if (ctx->pinned) XRH_HIP_CHECK(hipHostFree(ctx->pinned));
+  ctx->pinned = nullptr;
+  ctx->pinned_bytes = 0;
+  const uint64_t want = std::max<uint64_t>(bytes, 1ull << 20);
+  XRH_HIP_CHECK(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+  ctx->pinned_bytes = want;
+  return ctx->pinned;
+}
+
+void check_camera(const xrh_camera& c)
+{
+  XRH_REQUIRE(c.num_det_rows > 0 && c.num_det_cols > 0, "camera detector size must be positive");
+  XRH_REQUIRE(c.num_det_rows <= 32768 && c.num_det_cols <= 32768, "camera detector too large");
+  XRH_REQUIRE(c.coord_frame >= 0 && c.coord_frame <= 2, "camera coord_frame must be 0, 1 or 2");
+  XRH_REQUIRE(c.focal_len > 1.0e-8f, "camera focal length must be positive");
+}
+
+CamDev make_cam_dev(const xrh_camera& c)
+{
+  check_camera(c);
+  CamDev d;
+  std::memset(&d, 0, sizeof(d));
+  for (int i = 0; i < 9; ++i) {
+    d.kinv[i] = c.intrins_inv[i];
+    d.kinv_d[i] = (double)c.intrins_inv[i];
+  }
+  for (int i = 0; i < 12; ++i) {
+    d.einv[i] = c.extrins_inv[i];
+    d.einv_d[i] = (double)c.extrins_inv[i];
+  }
+  for (int i = 0; i < 3; ++i) {
+    d.pinhole[i] = c.pinhole[i];
+    d.pinhole_d[i] = (double)c.pinhole[i];
+  }
+  // det_z_val = ((frame == NEG_Z) ? -1 : 1) * focal_len (xregPerspectiveXform.cpp:411)
+  d.det_z = ((c.coord_frame == 1) ? -1.0f : 1.0f) * c.focal_len;
+  d.det_z_d = (double)d.det_z;
+  d.z_adj = (c.coord_frame == 2) ? -c.focal_len : 0.0f;
+  d.z_adj_d = (double)d.z_adj;
+  d.rows = c.num_det_rows;
+  d.cols = c.num_det_cols;
+  return d;
+}
+
+// float 3x3 inverse in Eigen's cofactor order (Eigen/src/LU/InverseImpl.h), used for the
+// phys->idx transform exactly as RayCasterLineIntCPU::compute obtains it via
+// FrameTransform::inverse() (xregRayCastLineIntCPU.cpp:307-309).
+static void affine_inverse_f32(const float m[16], float out[12])
+{
+#define M3(r, c) lin[3 * (r) + (c)]
+#define COF(i, j) (M3(((i) + 1) % 3, ((j) + 1) % 3) * M3(((i) + 2) % 3, ((j) + 2) % 3) - \
+                   M3(((i) + 1) % 3, ((j) + 2) % 3) * M3(((i) + 2) % 3, ((j) + 1) % 3))
+  const float lin[9] = {m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]};
+  float c00 = COF(0, 0), c10 = COF(1, 0), c20 = COF(2, 0);
+  const float det = (c00 * M3(0, 0) + c10 * M3(1, 0)) + c20 * M3(2, 0);
+  const float invdet = 1.0f / det;
+  const float inv[9] = {c00 * invdet,       c10 * invdet,       c20 * invdet,
+                        COF(0, 1) * invdet, COF(1, 1) * invdet, COF(2, 1) * invdet,
+                        COF(0, 2) * invdet, COF(1, 2) * invdet, COF(2, 2) * invdet};
+#undef COF
+#undef M3
+  for (int i = 0; i < 3; ++i) {
+    out[4 * i + 0] = inv[3 * i + 0];
+    out[4 * i + 1] = inv[3 * i + 1];
+    out[4 * i + 2] = inv[3 * i + 2];
+    const float t = (inv[3 * i] * m[3] + inv[3 * i + 1] * m[7]) + inv[3 * i + 2] * m[11];
+    out[4 * i + 3] = -t;
+  }
+}
+
+static void affine_inverse_f64(const float mf[16], double out[12])
+{
+  double m[16];
+  for (int i = 0; i < 16; ++i) m[i] = (double)mf[i];
+  const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], k = m[10];
+  const double c00 = e * k - f * h, c01 = f * g - d * k, c02 = d * h - e * g;
+  const double det = a * c00 + b * c01 + c * c02;
+  const double inv[9] = {c00 / det, (c * h - b * k) / det, (b * f - c * e) / det,
+                         c01 / det, (a * k - c * g) / det, (c * d - a * f) / det,
+                         c02 / det, (b * g - a * h) / det, (a * e - b * d) / det};
+  for (int i = 0; i < 3; ++i) {
+    out[4 * i + 0] = inv[3 * i + 0];
+    out[4 * i + 1] = inv[3 * i + 1];
+    out[4 * i + 2] = inv[3 * i + 2];
+    out[4 * i + 3] = -(inv[3 * i] * m[3] + inv[3 * i + 1] * m[7] + inv[3 * i + 2] * m[11]);
+  }
+}
+
+namespace {
+
+__device__ inline uint16_t f2h_bits(float v) { return __half_as_ushort(__float2half_rn(v)); }
+
+__device__ inline uint16_t f2bf_bits(float v)
+{
+  const uint32_t u = __float_as_uint(v);
+  const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite input)
+  return (uint16_t)(r >> 16);
+}
+
+// Builds the padded layout from the raw x-fastest volume. One thread per padded voxel.
+__global__ void build_layout_kernel(const float* __restrict__ src, int X, int Y, int Z,
+                                    void* __restrict__ dst, int storage, int layout)
+{
+  const int64_t Xp = X + 2, Yp = Y + 2, Zp = Z + 2;
+  const int64_t n = Xp * Yp * Zp;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = (int)(i % Xp);
+  const int b = (int)((i / Xp) % Yp);
+  const int c = (int)(i / (Xp * Yp));
+  auto V = [&](int aa, int bb, int cc) -> float {
+    // padded index -> source index with edge replication
+    int x = min(max(aa - 1, 0), X - 1);
+    int y = min(max(bb - 1, 0), Y - 1);
+    int z = min(max(cc - 1, 0), Z - 1);
+    return src[(int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * z)];
+  };
+  if (layout == kLayoutLinear) {
+    const float v = V(a, b, c);
+    if (storage == XRH_F32) ((float*)dst)[i] = v;
+    else if (storage == XRH_F16) ((uint16_t*)dst)[i] = f2h_bits(v);
+    else ((uint16_t*)dst)[i] = f2bf_bits(v);
+  } else if (layout == kLayoutQuad) {
+    float4 q;
+    q.x = V(a, b, c);
+    q.y = V(a + 1, b, c);
+    q.z = V(a, b + 1, c);
+    q.w = V(a + 1, b + 1, c);
+    ((float4*)dst)[i] = q;
+  } else {
+    const float f[8] = {V(a, b, c),     V(a + 1, b, c),     V(a, b + 1, c),     V(a + 1, b + 1, c),
+                        V(a, b, c + 1), V(a + 1, b, c + 1), V(a, b + 1, c + 1), V(a + 1, b + 1, c + 1)};
+    uint32_t w[4];
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t lo = (storage == XRH_F16) ? f2h_bits(f[2 * k]) : f2bf_bits(f[2 * k]);
+      const uint32_t hi = (storage == XRH_F16) ? f2h_bits(f[2 * k + 1]) : f2bf_bits(f[2 * k + 1]);
+      w[k] = lo | (hi << 16);
+    }
+    ((uint4*)dst)[i] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+__global__ void copy_bg_kernel(const float* __restrict__ bg, const int64_t* __restrict__ cam_for_proj,
+                               int64_t npx, int64_t num_projs, float* __restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npx * num_projs) return;
+  const int64_t p = i / npx;
+  out[i] = bg[cam_for_proj[p] * npx + (i - p * npx)];
+}
+
+}  // namespace
+
+static int default_layout(int storage)
+{
+  const char* env = std::getenv("XRH_VOLUME_LAYOUT");
+  if (env && std::strcmp(env, "linear") == 0) return kLayoutLinear;
+  return (storage == XRH_F32) ? kLayoutQuad : kLayoutOct;
+}
+
+static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
+                          const float idx_to_phys[16], int32_t storage, int32_t layout,
+                          xrh_volume** out)
+{
+  XRH_REQUIRE(ctx && vox && size && idx_to_phys && out, "null argument");
+  XRH_REQUIRE(storage == XRH_F32 || storage == XRH_F16 || storage == XRH_BF16, "bad storage type");
+  for (int k = 0; k < 3; ++k) XRH_REQUIRE(size[k] >= 1 && size[k] < (1 << 20), "bad volume size");
+  if (layout < 0) layout = default_layout(storage);
+  if (layout == kLayoutQuad) XRH_REQUIRE(storage == XRH_F32, "quad layout needs FP32 storage");
+  if (layout == kLayoutOct) XRH_REQUIRE(storage != XRH_F32, "oct layout needs 16-bit storage");
+  const int64_t X = size[0], Y = size[1], Z = size[2];
+  const int64_t padded = (X + 2) * (Y + 2) * (Z + 2);
+  const int64_t esize = (layout == kLayoutLinear) ? (storage == XRH_F32 ? 4 : 2) : 16;
+  const uint64_t bytes = (uint64_t)(padded * esize);
+  // buffer loads address with 32-bit byte offsets
+  XRH_REQUIRE(bytes < 0xFFFFF000ull, "volume layout exceeds 4 GiB; use linear layout or split");
+  auto* v = new xrh_volume();
+  v->ctx = ctx;
+  v->size[0] = X;
+  v->size[1] = Y;
+  v->size[2] = Z;
+  void* raw = nullptr;
+  try {
+    v->data = ctx_alloc(ctx, bytes);
+    v->bytes = bytes;
+    raw = ctx_alloc(ctx, (uint64_t)(X * Y * Z) * 4);
+    XRH_HIP_CHECK(hipMemcpyAsync(raw, vox, (size_t)(X * Y * Z) * 4, hipMemcpyHostToDevice, ctx->stream));
+    const int threads = 256;
+    const int64_t blocks = (padded + threads - 1) / threads;
+    hipLaunchKernelGGL(build_layout_kernel, dim3((unsigned)blocks), dim3(threads), 0, ctx->stream,
+                       (const float*)raw, (int)X, (int)Y, (int)Z, v->data, storage, layout);
+    XRH_HIP_CHECK(hipGetLastError());
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx_free(ctx, raw);
+    raw = nullptr;
+  } catch (...) {
+    if (raw) ctx_free(ctx, raw);
+    if (v->data) ctx_free(ctx, v->data);
+    delete v;
+    throw;
+  }
+  VolDesc& d = v->desc;
+  std::memset(&d, 0, sizeof(d));
+  affine_inverse_f32(idx_to_phys, d.p2i);
+  affine_inverse_f64(idx_to_phys, d.p2i_d);
+  d.data = v->data;
+  d.bytes = (uint32_t)bytes;
+  d.X = (int32_t)X;
+  d.Y = (int32_t)Y;
+  d.Z = (int32_t)Z;
+  d.pitch_y = (uint32_t)(X + 2);
+  d.pitch_z = (uint32_t)((X + 2) * (Y + 2));
+  d.storage = storage;
+  d.layout = layout;
+  float ms = INFINITY;
+  for (int c = 0; c < 3; ++c) {
+    const float n = std::sqrt(idx_to_phys[c] * idx_to_phys[c] + idx_to_phys[4 + c] * idx_to_phys[4 + c] +
+                              idx_to_phys[8 + c] * idx_to_phys[8 + c]);
+    ms = std::min(ms, n);
+  }
+  XRH_REQUIRE(ms > 0.0f && std::isfinite(ms), "idx_to_phys has a zero or non-finite column");
+  d.min_spacing = ms;
+  *out = v;
+}
+
+// Packs PoseIn/CamDev/VolDesc/band tables into one pinned blob, uploads it and launches
+// the set-up + DRR kernels. Shared by xrh_line_int and its sample counter.
+static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* cams,
+                            int32_t num_cams, const float* xforms, const int64_t* cam_for_proj,
+                            int64_t num_projs, float step, int32_t kernel, int32_t store,
+                            const float* bg_dev, float default_bg, int32_t geom, float* drr_dev,
+                            uint64_t* samples_out)
+{
+  XRH_REQUIRE(ctx && vol && cams && xforms && cam_for_proj, "null argument");
+  XRH_REQUIRE(num_cams >= 1 && num_projs >= 1, "need at least one camera and one projection");
+  XRH_REQUIRE(kernel == XRH_LINE_INT_SUM || kernel == XRH_LINE_INT_MAX, "bad line-integral kernel");
+  XRH_REQUIRE(store == XRH_STORE_REPLACE || store == XRH_STORE_ACCUM, "bad store method");
+  XRH_REQUIRE(geom == XRH_GEOM_FP32 || geom == XRH_GEOM_FP64, "bad geometry mode");
+  XRH_REQUIRE(std::isfinite(step) && step > 0.0f, "ray step must be positive");
+  check_step(step, vol->desc);
+  XRH_REQUIRE(samples_out || drr_dev, "null DRR buffer");
+  const int32_t rows = cams[0].num_det_rows, cols = cams[0].num_det_cols;
+  for (int32_t c = 0; c < num_cams; ++c) {
+    check_camera(cams[c]);
+    // xregRayCastBaseCPU.cpp:54-70
+    XRH_REQUIRE(cams[c].num_det_rows == rows, "Inconsistent number rows in camera model images!");
+    XRH_REQUIRE(cams[c].num_det_cols == cols, "Inconsistent number columns in camera model images!");
+  }
+  for (int64_t p = 0; p < num_projs; ++p) {
+    XRH_REQUIRE(cam_for_proj[p] >= 0 && cam_for_proj[p] < num_cams, "camera index out of range");
+  }
+  const int64_t npx = (int64_t)rows * cols;
+  XRH_REQUIRE(npx * num_projs < (1ll << 40), "DRR buffer too large");
+
+  std::vector<int32_t> rr((size_t)num_projs, rows), cc((size_t)num_projs, cols);
+  const TilePlan plan = build_tiles(rr, cc, 8);
+
+  auto align = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+  const uint64_t off_pose = 0;
+  const uint64_t off_cam = align(off_pose + sizeof(PoseIn) * num_projs);
+  const uint64_t off_vol = align(off_cam + sizeof(CamDev) * num_cams);
+  const uint64_t off_band = align(off_vol + sizeof(VolDesc));
+  const uint64_t off_c4p = align(off_band + sizeof(uint32_t) * plan.band_start.size());
+  const uint64_t off_cnt = align(off_c4p + sizeof(int64_t) * num_projs);
+  const uint64_t blob = align(off_cnt + 8);
+  const uint64_t off_items = blob;
+  const uint64_t total = align(off_items + sizeof(DrrItem) * num_projs);
+
+  XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // staging reuse
+  char* host = (char*)ctx_pinned(ctx, blob);
+  char* dev = (char*)ctx_scratch(ctx, total);
+  PoseIn* pin = (PoseIn*)(host + off_pose);
+  for (int64_t p = 0; p < num_projs; ++p) {
+    std::memset(&pin[p], 0, sizeof(PoseIn));
+    for (int k = 0; k < 16; ++k) pin[p].pose[k] = (double)xforms[16 * p + k];
+    pin[p].cam = (int32_t)cam_for_proj[p];
+    pin[p].vol = 0;
+    pin[p].out_off = p * npx;
+    pin[p].ncc_zf = nullptr;
+    pin[p].step = step;
+  }
+  CamDev* cd = (CamDev*)(host + off_cam);
+  for (int32_t c = 0; c < num_cams; ++c) cd[c] = make_cam_dev(cams[c]);
+  std::memcpy(host + off_vol, &vol->desc, sizeof(VolDesc));
+  std::memcpy(host + off_band, plan.band_start.data(), sizeof(uint32_t) * plan.band_start.size());
+  std::memcpy(host + off_c4p, cam_for_proj, sizeof(int64_t) * num_projs);
+  std::memset(host + off_cnt, 0, 8);
+  XRH_HIP_CHECK(hipMemcpyAsync(dev, host, blob, hipMemcpyHostToDevice, ctx->stream));
+
+  if (!samples_out) {
+    if (bg_dev && store == XRH_STORE_REPLACE) {
+      const int64_t n = npx * num_projs;
+      hipLaunchKernelGGL(copy_bg_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                         bg_dev, (const int64_t*)(dev + off_c4p), npx, num_projs, drr_dev);
+      XRH_HIP_CHECK(hipGetLastError());
+    }
+  }
+  DrrLaunch L;
+  std::memset(&L, 0, sizeof(L));
+  L.poses_dev = (const PoseIn*)(dev + off_pose);
+  L.vols_dev = (const VolDesc*)(dev + off_vol);
+  L.cams_dev = (const CamDev*)(dev + off_cam);
+  L.items_dev = (DrrItem*)(dev + off_items);
+  L.band_start_dev = (const uint32_t*)(dev + off_band);
+  L.num_items = num_projs;
+  L.bands = plan.bands;
+  L.max_band_tiles = plan.max_band_tiles;
+  L.geom = geom;
+  L.kernel = kernel;
+  L.store = (store == XRH_STORE_ACCUM || (bg_dev && store == XRH_STORE_REPLACE)) ? XRH_STORE_ACCUM
+                                                                                 : XRH_STORE_REPLACE;
+  L.fill = default_bg;
+  L.storage = vol->desc.storage;
+  L.layout = vol->desc.layout;
+  L.out = drr_dev;
+  L.sample_counter = samples_out ? (uint64_t*)(dev + off_cnt) : nullptr;
+  launch_drr_setup(L, ctx->stream);
+  launch_drr(L, ctx->stream);
+  if (samples_out) {
+    XRH_HIP_CHECK(hipMemcpyAsync(samples_out, dev + off_cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+}
+
+}  // namespace xrh
+
+using namespace xrh;
+
+extern "C" {
+
+const char* xrh_last_error(void) { return g_last_error.c_str(); }
+
+int32_t xrh_abi_version(void) { return XRH_ABI_VERSION; }
+
+xrh_status xrh_ctx_create(int32_t device, uint64_t hbm_budget_bytes, xrh_ctx** out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(out, "null output pointer");
+    int count = 0;
+    XRH_HIP_CHECK(hipGetDeviceCount(&count));
+    XRH_REQUIRE(device >= 0 && device < count, "HIP device index out of range");
+    XRH_HIP_CHECK(hipSetDevice(device));
+    auto* ctx = new xrh_ctx();
+    ctx->device = device;
+    ctx->budget = hbm_budget_bytes;
+    hipDeviceProp_t prop;
+    XRH_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    ctx->num_cus = prop.multiProcessorCount;
+    const hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete ctx;
+      fail(XRH_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    }
+    *out = ctx;
+  });
+}
+
+xrh_status xrh_ctx_destroy(xrh_ctx* ctx)
+{
+  return guarded([&] {
+    if (!ctx) return;
+    XRH_HIP_CHECK(hipSetDevice(ctx->device));
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->pinned) XRH_HIP_CHECK(hipHostFree(ctx->pinned));
+    for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
+    ctx->allocs.clear();
+    XRH_HIP_CHECK(hipStreamDestroy(ctx->stream));
+    delete ctx;
+  });
+}
+
+xrh_status xrh_ctx_synchronize(xrh_ctx* ctx)
+{
+  return guarded([&] {
+    XRH_REQUIRE(ctx, "null context");
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+void* xrh_ctx_stream(xrh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+uint64_t xrh_ctx_bytes_in_use(const xrh_ctx* ctx) { return ctx ? ctx->in_use : 0; }
+
+xrh_status xrh_device_alloc(xrh_ctx* ctx, uint64_t bytes, void** dptr)
+{
+  return guarded([&] {
+    XRH_REQUIRE(ctx && dptr, "null argument");
+    *dptr = ctx_alloc(ctx, bytes);
+  });
+}
+
+xrh_status xrh_device_free(xrh_ctx* ctx, void* dptr)
+{
+  return guarded([&] {
+    XRH_REQUIRE(ctx, "null context");
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx_free(ctx, dptr);
+  });
+}
+
+xrh_status xrh_copy_h2d(xrh_ctx* ctx, void* dst_dev, const void* src_host, uint64_t bytes)
+{
+  return guarded([&] {
+    XRH_REQUIRE(ctx && dst_dev && src_host, "null argument");
+    XRH_HIP_CHECK(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+xrh_status xrh_copy_d2h(xrh_ctx* ctx, void* dst_host, const void* src_dev, uint64_t bytes)
+{
+  return guarded([&] {
+    XRH_REQUIRE(ctx && dst_host && src_dev, "null argument");
+    XRH_HIP_CHECK(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+xrh_status xrh_volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
+                             const float idx_to_phys[16], int32_t storage, xrh_volume** out)
+{
+  return guarded([&] { volume_create(ctx, vox, size, idx_to_phys, storage, -1, out); });
+}
+
+xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_t size[3],
+                                    const float idx_to_phys[16], int32_t storage, int32_t layout,
+                                    xrh_volume** out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(layout >= 0 && layout <= 2, "bad layout");
+    volume_create(ctx, vox, size, idx_to_phys, storage, layout, out);
+  });
+}
+
+xrh_status xrh_volume_destroy(xrh_volume* vol)
+{
+  return guarded([&] {
+    if (!vol) return;
+    XRH_HIP_CHECK(hipStreamSynchronize(vol->ctx->stream));
+    ctx_free(vol->ctx, vol->data);
+    delete vol;
+  });
+}
+
+uint64_t xrh_volume_bytes(const xrh_volume* vol) { return vol ? vol->bytes : 0; }
+
+xrh_status xrh_line_int(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* cams,
+                        int32_t num_cams, const float* xforms_cam_to_phys,
+                        const int64_t* cam_for_proj, int64_t num_projs, float step,
+                        int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
+                        int32_t geom, float* drr_dev)
+{
+  return guarded([&] {
+    line_int_launch(ctx, vol, cams, num_cams, xforms_cam_to_phys, cam_for_proj, num_projs, step,
+                    kernel, store, bg_dev, default_bg, geom, drr_dev, nullptr);
+  });
+}
+
+xrh_status xrh_line_int_sample_count(xrh_ctx* ctx, const xrh_volume* vol,
+                                     const xrh_camera* cams, int32_t num_cams,
+                                     const float* xforms_cam_to_phys,
+                                     const int64_t* cam_for_proj, int64_t num_projs,
+                                     float step, int32_t geom, uint64_t* samples_out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(samples_out, "null output");
+    line_int_launch(ctx, vol, cams, num_cams, xforms_cam_to_phys, cam_for_proj, num_projs, step,
+                    XRH_LINE_INT_SUM, XRH_STORE_REPLACE, nullptr, 0.0f, geom, nullptr, samples_out);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// xrh_sim.h -- internal similarity-metric types of libxrh (not part of the ABI).
+#pragma once
+
+#include "xrh_internal.h"
+
+namespace xrh {
+
+// Device-side view of a processed fixed image (one per ImgSimMetric2D instance / view).
+struct FixedDev {
+  int32_t metric;
+  int32_t rows, cols;
+  int32_t smooth_ksize;
+  int32_t patch_radius, patch_stride;
+  int32_t ppr, ppc;          // patches per row / per column
+  int64_t num_patches;
+  int64_t count;             // pixels used by NCC (mask length when masked)
+  const float* zf;           // NCC: zero-mean fixed image (Grad-NCC: zero-mean fixed gx)
+  const float* zfy;          // Grad-NCC: zero-mean fixed gy
+  double sum_zf, sum_zfy;    // FP64 sums of the float zero-mean images (rounding residue)
+  double std_f, std_fy;      // fixed standard deviations (clamped at 1e-6)
+  const uint8_t* mask;       // optional NCC mask
+  const float* pf;           // patch metrics: fixed image (or fixed gx)
+  const float* pfy;          // patch-gradient: fixed gy
+  const float* pmean;        // per-patch fixed mean
+  const float* pden;         // per-patch fixed std * N
+  const float* pmeany;
+  const float* pdeny;
+};
+
+// One moving image to score.
+struct SimItem {
+  const float* mov;
+  const FixedDev* fixed;     // device pointer
+  float* gx;                 // scratch for patch-gradient metrics
+  float* gy;
+  int64_t out_index;
+};
+
+int sim_record_stride(int metric);
+int sim_tiles(const FixedDev& f);     // tiles of the partial-producing kernel
+int sim_img_tiles(const FixedDev& f); // 16x16 image tiles
+
+// Queues the similarity kernels for n_items moving images that all use `metric`.
+// partials: n_items * ptiles * record-stride doubles. If fused_ncc_done, the NCC sums were
+// already written by the DRR epilogue.
+void launch_sim(int metric, const SimItem* items_dev, int64_t n_items, int max_img_tiles,
+                int max_patch_tiles, double* partials, float* sim_out_dev, bool fused_ncc_done,
+                int patch_lds_bytes, hipStream_t s);
+// dynamic LDS bytes the patch kernel needs for a fixed image's patch geometry
+int sim_patch_lds_bytes(const FixedDev& f);
+
+}  // namespace xrh
+
+struct xrh_fixed {
+  xrh_ctx* ctx = nullptr;
+  xrh::FixedDev host{};          // host copy with device pointers
+  xrh::FixedDev* dev = nullptr;  // device copy
+  std::vector<void*> buffers;    // device allocations owned by this object
+};

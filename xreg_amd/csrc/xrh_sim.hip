@@ -1,0 +1,735 @@
+// xrh_sim.hip -- NCC, gradient-NCC, patch-NCC and patch-gradient-NCC on gfx950.
+//
+// Semantics (smaller is better, as in xReg):
+//   NCC        (1 - ncc)/2, ncc = sum(f~ m~)/(n sf sm), s = max(1e-6, sqrt(var/(n-1)))
+//              (lib/regi/sim_metrics_2d/xregImgSimMetric2DNCCCPU.cpp:52-236)
+//   Grad-NCC   0.5 (NCC(gx_f, gx_m) + NCC(gy_f, gy_m)) with OpenCV Sobel ksize 3 after an
+//              optional GaussianBlur(ksize, sigma 0), BORDER_REFLECT_101
+//              (xregImgSimMetric2DGradImgCPU.cpp:79-102, xregImgSimMetric2DGradNCCCPU.cpp:51-65)
+//   Patch-NCC  mean over (2r+1)^2 patches (stride s) of 1 - sum(((m-mu_m)/s_m) (f-mu_f)/(s_f N))
+//              (xregImgSimMetric2DPatchNCCCPU.cpp:190-287, 558-619)
+//   Patch-Grad-NCC  0.5 (PatchNCC(gx) + PatchNCC(gy)) (xregImgSimMetric2DPatchGradNCCCPU.cpp:104-224)
+//
+// CDNA4 mapping: one 256-lane workgroup per 16x16 output tile of one moving image; the
+// tile plus its halo is staged in LDS (REFLECT_101 applied while staging), per-lane FP64
+// partial sums are reduced across the wave with cross-lane shuffles, one FP64 record per
+// tile goes to a partial buffer, and a one-wave-per-image finish kernel sums the records in
+// a fixed order (deterministic, FP64) and applies the metric's closed form.
+#include "xrh_internal.h"
+#include "xrh_sim.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace xrh {
+namespace {
+
+constexpr int kT = 16;            // output tile edge
+constexpr int kThreads = kT * kT;
+constexpr int kMaxHalo = 4;       // blur radius 3 + Sobel 1
+constexpr int kGradLds = kT + 2 * kMaxHalo;  // 24
+
+__device__ inline int reflect101(int p, int n)
+{
+  if (n == 1) return 0;
+  // |p| < 2n always holds for the halos used here; loop is for safety
+  while (p < 0 || p >= n) {
+    if (p < 0) p = -p;
+    if (p >= n) p = 2 * (n - 1) - p;
+  }
+  return p;
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Reduces K per-lane doubles over the 256-lane workgroup; lane 0 writes them to dst.
+template <int K>
+__device__ inline void block_reduce_store(double (&v)[K], double* dst)
+{
+  __shared__ double red[4][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[w][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) dst[k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+  }
+}
+
+__device__ inline float gauss_k(int ksize, int j)
+{
+  // OpenCV getGaussianKernel fixed table for sigma <= 0, odd ksize <= 7
+  if (ksize == 3) {
+    const float k[3] = {0.25f, 0.5f, 0.25f};
+    return k[j];
+  } else if (ksize == 5) {
+    const float k[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    return k[j];
+  } else if (ksize == 7) {
+    const float k[7] = {0.03125f, 0.109375f, 0.21875f, 0.28125f, 0.21875f, 0.109375f, 0.03125f};
+    return k[j];
+  }
+  return 1.0f;
+}
+
+// Computes the Sobel gradients (after optional blur) of the 16x16 tile at (x0, y0) of an
+// image. Each lane returns gx, gy of its own pixel. LDS: raw (24x24), blurred-rows, blurred.
+__device__ inline void tile_gradients(const float* __restrict__ img, int rows, int cols, int x0,
+                                      int y0, int ksize, float& gx, float& gy, bool& inside)
+{
+#pragma clang fp contract(off)
+  __shared__ float raw[kGradLds][kGradLds];
+  __shared__ float tmp[kGradLds][kGradLds];
+  __shared__ float sm[kGradLds][kGradLds];
+  const int rb = (ksize > 1) ? ksize / 2 : 0;
+  const int halo = rb + 1;
+  const int W = kT + 2 * halo;
+  for (int i = threadIdx.x; i < W * W; i += kThreads) {
+    const int ly = i / W, lx = i % W;
+    const int y = reflect101(y0 - halo + ly, rows);
+    const int x = reflect101(x0 - halo + lx, cols);
+    raw[ly][lx] = img[(int64_t)y * cols + x];
+  }
+  __syncthreads();
+  // blurred image over the (16+2) x (16+2) region that the Sobel needs
+  const int B = kT + 2;
+  if (rb > 0) {
+    // row pass over W rows x B cols: acc = k[r] s0 + sum_j k[r+j] (s[-j] + s[+j])
+    for (int i = threadIdx.x; i < W * B; i += kThreads) {
+      const int ly = i / B, bx = i % B;
+      const int lx = bx + rb;  // centre in raw coordinates
+      float acc = gauss_k(ksize, rb) * raw[ly][lx];
+      for (int j = 1; j <= rb; ++j) acc = acc + gauss_k(ksize, rb + j) * (raw[ly][lx - j] + raw[ly][lx + j]);
+      tmp[ly][bx] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < B * B; i += kThreads) {
+      const int by = i / B, bx = i % B;
+      const int ly = by + rb;
+      float acc = gauss_k(ksize, rb) * tmp[ly][bx];
+      for (int j = 1; j <= rb; ++j) acc = acc + gauss_k(ksize, rb + j) * (tmp[ly - j][bx] + tmp[ly + j][bx]);
+      sm[by][bx] = acc;
+    }
+  } else {
+    for (int i = threadIdx.x; i < B * B; i += kThreads) {
+      const int by = i / B, bx = i % B;
+      sm[by][bx] = raw[by][bx];
+    }
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % kT, ty = threadIdx.x / kT;
+  const int x = x0 + tx, y = y0 + ty;
+  inside = (x < cols) && (y < rows);
+  // sm index (by, bx) = image (y0 - 1 + by, x0 - 1 + bx); the lane's pixel is (ty+1, tx+1).
+  // gx: row derivative then column [1 2 1]; gy: row [1 2 1] then column derivative
+  const int cy = ty + 1, cx = tx + 1;
+  const float dxm = sm[cy - 1][cx + 1] - sm[cy - 1][cx - 1];
+  const float dx0 = sm[cy][cx + 1] - sm[cy][cx - 1];
+  const float dxp = sm[cy + 1][cx + 1] - sm[cy + 1][cx - 1];
+  gx = (dxm + dxp) + dx0 * 2.0f;
+  const float sym = (sm[cy - 1][cx - 1] + sm[cy - 1][cx + 1]) + sm[cy - 1][cx] * 2.0f;
+  const float syp = (sm[cy + 1][cx - 1] + sm[cy + 1][cx + 1]) + sm[cy + 1][cx] * 2.0f;
+  gy = syp - sym;
+}
+
+// ---- kernels over SimItem tables: blockIdx.x = tile, blockIdx.y = item ----
+
+// NCC sums over raw moving images: per tile (sum m, sum m^2, sum m f~) [masked]
+__global__ __launch_bounds__(kThreads) void ncc_sums_kernel(const SimItem* __restrict__ items,
+                                                            double* __restrict__ partials, int ptiles)
+{
+  const SimItem it = items[blockIdx.y];
+  const FixedDev& fx = *it.fixed;
+  const int TX = (fx.cols + kT - 1) / kT, TY = (fx.rows + kT - 1) / kT;
+  if ((int)blockIdx.x >= TX * TY) return;
+  const int tx0 = (blockIdx.x % TX) * kT, ty0 = (blockIdx.x / TX) * kT;
+  const int x = tx0 + (threadIdx.x % kT), y = ty0 + (threadIdx.x / kT);
+  double v[3] = {0.0, 0.0, 0.0};
+  if (x < fx.cols && y < fx.rows) {
+    const int64_t o = (int64_t)y * fx.cols + x;
+    if (!fx.mask || fx.mask[o]) {
+      const double m = (double)it.mov[o];
+      v[0] = m;
+      v[1] = m * m;
+      v[2] = m * (double)fx.zf[o];
+    }
+  }
+  block_reduce_store<3>(v, partials + (it.out_index * ptiles + blockIdx.x) * 4);
+}
+
+// Grad-NCC sums: gradients in LDS, then sums for x and y directions
+__global__ __launch_bounds__(kThreads) void grad_ncc_sums_kernel(const SimItem* __restrict__ items,
+                                                                 double* __restrict__ partials, int ptiles)
+{
+  const SimItem it = items[blockIdx.y];
+  const FixedDev& fx = *it.fixed;
+  const int TX = (fx.cols + kT - 1) / kT, TY = (fx.rows + kT - 1) / kT;
+  if ((int)blockIdx.x >= TX * TY) return;
+  const int x0 = (blockIdx.x % TX) * kT, y0 = (blockIdx.x / TX) * kT;
+  float gx, gy;
+  bool inside;
+  tile_gradients(it.mov, fx.rows, fx.cols, x0, y0, fx.smooth_ksize, gx, gy, inside);
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (inside) {
+    const int64_t o = (int64_t)(y0 + threadIdx.x / kT) * fx.cols + x0 + (threadIdx.x % kT);
+    if (!fx.mask || fx.mask[o]) {
+      const double a = (double)gx, b = (double)gy;
+      v[0] = a;
+      v[1] = a * a;
+      v[2] = a * (double)fx.zf[o];
+      v[3] = b;
+      v[4] = b * b;
+      v[5] = b * (double)fx.zfy[o];
+    }
+  }
+  block_reduce_store<6>(v, partials + (it.out_index * ptiles + blockIdx.x) * 8);
+}
+
+// Gradient images of moving images (for the patch metric), written to it.gx / it.gy.
+__global__ __launch_bounds__(kThreads) void grad_images_kernel(const SimItem* __restrict__ items)
+{
+  const SimItem it = items[blockIdx.y];
+  const FixedDev& fx = *it.fixed;
+  const int TX = (fx.cols + kT - 1) / kT, TY = (fx.rows + kT - 1) / kT;
+  if ((int)blockIdx.x >= TX * TY) return;
+  const int x0 = (blockIdx.x % TX) * kT, y0 = (blockIdx.x / TX) * kT;
+  float gx, gy;
+  bool inside;
+  tile_gradients(it.mov, fx.rows, fx.cols, x0, y0, fx.smooth_ksize, gx, gy, inside);
+  if (inside) {
+    const int64_t o = (int64_t)(y0 + threadIdx.x / kT) * fx.cols + x0 + (threadIdx.x % kT);
+    it.gx[o] = gx;
+    it.gy[o] = gy;
+  }
+}
+
+// Patch NCC over one (or two, for gradients) image(s): each lane owns one patch of a 16x16
+// block of patch positions; the image region under the block is staged in LDS.
+// LDS edge for a 16x16 block of patches: (16-1)*stride + diameter (26 for r=5, s=1)
+__host__ __device__ inline int patch_lds_edge(int stride, int radius) { return (kT - 1) * stride + 2 * radius + 1; }
+
+__device__ inline float patch_score(const float* m, const float* f, int W, int ly, int lx, int d,
+                                    float mu_f, float den_f)
+{
+#pragma clang fp contract(off)
+  const int N = d * d;
+  float sum = 0.0f;
+  for (int yy = 0; yy < d; ++yy)
+    for (int xx = 0; xx < d; ++xx) sum += m[(ly + yy) * W + lx + xx];
+  const float mean = sum / (float)N;
+  float var = 0.0f, cross = 0.0f;
+  for (int yy = 0; yy < d; ++yy) {
+    for (int xx = 0; xx < d; ++xx) {
+      const float t = m[(ly + yy) * W + lx + xx] - mean;
+      var += t * t;
+      cross += t * (f[(ly + yy) * W + lx + xx] - mu_f);
+    }
+  }
+  var /= (float)N - 1.0f;
+  const float sd = fmaxf(1.0e-6f, sqrtf(var));
+  // sum((m - mu)/sd * (f - mu_f)/den_f) with den_f = s_f * N
+  return 1.0f - (cross / sd) / den_f;
+}
+
+template <bool kTwo>
+__global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restrict__ items,
+                                                         double* __restrict__ partials, int ptiles)
+{
+  extern __shared__ float patch_lds[];
+  const SimItem it = items[blockIdx.y];
+  const FixedDev& fx = *it.fixed;
+  const int s = fx.patch_stride, r = fx.patch_radius, d = 2 * r + 1;
+  const int ppr = fx.ppr, ppc = fx.ppc;
+  const int TX = (ppr + kT - 1) / kT, TY = (ppc + kT - 1) / kT;
+  if ((int)blockIdx.x >= TX * TY) return;
+  const int px0 = (blockIdx.x % TX) * kT, py0 = (blockIdx.x / TX) * kT;  // first patch index
+  const int X0 = px0 * s, Y0 = py0 * s;                                  // first image pixel
+  const int W = patch_lds_edge(s, r);
+  float* lm = patch_lds;
+  float* lf = patch_lds + W * W;
+  const int pxl = threadIdx.x % kT, pyl = threadIdx.x / kT;
+  const int pi = px0 + pxl, pj = py0 + pyl;
+  const bool valid = (pi < ppr) && (pj < ppc);
+  double acc[2] = {0.0, 0.0};
+  for (int dir = 0; dir < (kTwo ? 2 : 1); ++dir) {
+    const float* mimg = (dir == 0) ? (kTwo ? it.gx : it.mov) : it.gy;
+    const float* fimg = (dir == 0) ? fx.pf : fx.pfy;
+    const float* pmu = (dir == 0) ? fx.pmean : fx.pmeany;
+    const float* pden = (dir == 0) ? fx.pden : fx.pdeny;
+    __syncthreads();
+    for (int i = threadIdx.x; i < W * W; i += kThreads) {
+      const int ly = i / W, lx = i % W;
+      const int y = min(Y0 + ly, fx.rows - 1), x = min(X0 + lx, fx.cols - 1);
+      const int64_t o = (int64_t)y * fx.cols + x;
+      lm[ly * W + lx] = mimg[o];
+      lf[ly * W + lx] = fimg[o];
+    }
+    __syncthreads();
+    if (valid) {
+      const int64_t p = (int64_t)pj * ppr + pi;
+      acc[dir] = (double)patch_score(lm, lf, W, pyl * s, pxl * s, d, pmu[p], pden[p]);
+    }
+  }
+  block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
+}
+
+// fixed-image per-patch statistics (reference float order, ComputePatchMeanStdDev)
+__global__ void patch_stats_kernel(const float* __restrict__ img, int cols, int ppr, int ppc, int s,
+                                   int d, float* __restrict__ mean_out, float* __restrict__ den_out)
+{
+#pragma clang fp contract(off)
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t)ppr * ppc) return;
+  const int r0 = (int)(p / ppr) * s, c0 = (int)(p % ppr) * s;
+  const int N = d * d;
+  float mean = 0.0f;
+  for (int y = 0; y < d; ++y)
+    for (int x = 0; x < d; ++x) mean += img[(int64_t)(r0 + y) * cols + c0 + x];
+  mean /= (float)N;
+  float var = 0.0f;
+  for (int y = 0; y < d; ++y)
+    for (int x = 0; x < d; ++x) {
+      const float t = img[(int64_t)(r0 + y) * cols + c0 + x] - mean;
+      var += t * t;
+    }
+  var /= (float)N - 1.0f;
+  const float sd = fmaxf(1.0e-6f, sqrtf(var));
+  mean_out[p] = mean;
+  den_out[p] = sd * (float)N;
+}
+
+// fixed gradient images (one image)
+__global__ __launch_bounds__(kThreads) void fixed_grad_kernel(const float* __restrict__ img, int rows,
+                                                              int cols, int ksize, float* __restrict__ gx_out,
+                                                              float* __restrict__ gy_out)
+{
+  const int TX = (cols + kT - 1) / kT;
+  const int x0 = (blockIdx.x % TX) * kT, y0 = (blockIdx.x / TX) * kT;
+  float gx, gy;
+  bool inside;
+  tile_gradients(img, rows, cols, x0, y0, ksize, gx, gy, inside);
+  if (inside) {
+    const int64_t o = (int64_t)(y0 + threadIdx.x / kT) * cols + x0 + (threadIdx.x % kT);
+    gx_out[o] = gx;
+    gy_out[o] = gy;
+  }
+}
+
+// one image: sum and (after mean is known) sum of squared deviations, FP64, single block
+__global__ __launch_bounds__(1024) void image_stats_kernel(const float* __restrict__ img, const uint8_t* mask,
+                                                           int64_t n, double* __restrict__ out)
+{
+  __shared__ double red[16];
+  __shared__ double mean_sh;
+  double s = 0.0, c = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    if (!mask || mask[i]) {
+      s += (double)img[i];
+      c += 1.0;
+    }
+  }
+  s = wave_sum(s);
+  c = wave_sum(c);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[i];
+    out[0] = t;
+    out[1] = c;
+    mean_sh = t / c;
+  }
+  __syncthreads();
+  double v = 0.0;
+  const double mu = mean_sh;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    if (!mask || mask[i]) {
+      const double t = (double)img[i] - mu;
+      v += t * t;
+    }
+  }
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[i];
+    out[2] = t;
+  }
+}
+
+__global__ void sub_scalar_kernel(const float* __restrict__ src, float mean, int64_t n, float* __restrict__ dst)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i] - mean;
+}
+
+__global__ __launch_bounds__(1024) void sum_kernel(const float* __restrict__ img, const uint8_t* mask, int64_t n,
+                                                   double* __restrict__ out)
+{
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    if (!mask || mask[i]) s += (double)img[i];
+  }
+  s = wave_sum(s);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[i];
+    out[0] = t;
+  }
+}
+
+// NCC closed form from FP64 sums (one direction)
+__device__ inline double ncc_from_sums(double s1, double s2, double sf, double n, double sum_zf,
+                                       double std_f)
+{
+  const double mu = s1 / n;
+  double var = (s2 - s1 * mu) / (n - 1.0);
+  if (var < 0.0) var = 0.0;
+  const double sd = fmax(1.0e-6, sqrt(var));
+  // sum f~ (m - mu) = sum f~ m - mu sum f~
+  const double num = sf - mu * sum_zf;
+  const double ncc = num / (n * std_f * sd);
+  return (1.0 - ncc) * 0.5;
+}
+
+// finish: one wave per item sums the tile records in a fixed order
+__global__ void finish_kernel(const SimItem* __restrict__ items, const double* __restrict__ partials,
+                              int ptiles, int stride_rec, float* __restrict__ sim_out, int64_t n_items)
+{
+  const int64_t item = blockIdx.x;
+  if (item >= n_items) return;
+  const SimItem it = items[item];
+  const FixedDev& fx = *it.fixed;
+  int ntiles;
+  if (fx.metric == XRH_SIM_PATCH_NCC || fx.metric == XRH_SIM_PATCH_GRAD_NCC) {
+    ntiles = ((fx.ppr + kT - 1) / kT) * ((fx.ppc + kT - 1) / kT);
+  } else {
+    ntiles = ((fx.cols + kT - 1) / kT) * ((fx.rows + kT - 1) / kT);
+  }
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const double* base = partials + it.out_index * ptiles * stride_rec;
+  for (int t = threadIdx.x; t < ntiles; t += 64) {
+    for (int k = 0; k < 6 && k < stride_rec; ++k) acc[k] += base[(int64_t)t * stride_rec + k];
+  }
+  for (int k = 0; k < 6; ++k) acc[k] = wave_sum(acc[k]);
+  if (threadIdx.x != 0) return;
+  double sim = 0.0;
+  const double n = (double)fx.count;
+  switch (fx.metric) {
+    case XRH_SIM_NCC:
+      sim = ncc_from_sums(acc[0], acc[1], acc[2], n, fx.sum_zf, fx.std_f);
+      break;
+    case XRH_SIM_GRAD_NCC: {
+      const float sx = (float)ncc_from_sums(acc[0], acc[1], acc[2], n, fx.sum_zf, fx.std_f);
+      const float sy = (float)ncc_from_sums(acc[3], acc[4], acc[5], n, fx.sum_zfy, fx.std_fy);
+      sim = 0.5 * (double)(sx + sy);
+      break;
+    }
+    case XRH_SIM_PATCH_NCC:
+      sim = acc[0] / (double)fx.num_patches;
+      break;
+    default: {
+      const float sx = (float)(acc[0] / (double)fx.num_patches);
+      const float sy = (float)(acc[1] / (double)fx.num_patches);
+      sim = 0.5 * (double)(sx + sy);
+      break;
+    }
+  }
+  sim_out[it.out_index] = (float)sim;
+}
+
+}  // namespace
+
+int sim_record_stride(int metric)
+{
+  switch (metric) {
+    case XRH_SIM_NCC: return 4;
+    case XRH_SIM_GRAD_NCC: return 8;
+    default: return 2;
+  }
+}
+
+int sim_img_tiles(const FixedDev& f) { return ((f.cols + kT - 1) / kT) * ((f.rows + kT - 1) / kT); }
+
+int sim_patch_lds_bytes(const FixedDev& f)
+{
+  if (f.metric != XRH_SIM_PATCH_NCC && f.metric != XRH_SIM_PATCH_GRAD_NCC) return 0;
+  const int W = patch_lds_edge(f.patch_stride, f.patch_radius);
+  return 2 * W * W * (int)sizeof(float);
+}
+
+int sim_tiles(const FixedDev& f)
+{
+  if (f.metric == XRH_SIM_PATCH_NCC || f.metric == XRH_SIM_PATCH_GRAD_NCC) {
+    return ((f.ppr + kT - 1) / kT) * ((f.ppc + kT - 1) / kT);
+  }
+  return sim_img_tiles(f);
+}
+
+void launch_sim(int metric, const SimItem* items_dev, int64_t n_items, int max_img_tiles,
+                int max_patch_tiles, double* partials, float* sim_out_dev, bool fused_ncc_done, int patch_lds_bytes,
+                hipStream_t s)
+{
+  if (n_items <= 0) return;
+  XRH_REQUIRE(n_items < 65536, "too many moving images in one similarity launch");
+  const bool patch = (metric == XRH_SIM_PATCH_NCC || metric == XRH_SIM_PATCH_GRAD_NCC);
+  const int ptiles = patch ? max_patch_tiles : max_img_tiles;
+  const dim3 grid((unsigned)ptiles, (unsigned)n_items);
+  const int rec = sim_record_stride(metric);
+  switch (metric) {
+    case XRH_SIM_NCC:
+      if (!fused_ncc_done) hipLaunchKernelGGL(ncc_sums_kernel, grid, dim3(kThreads), 0, s, items_dev, partials, ptiles);
+      break;
+    case XRH_SIM_GRAD_NCC:
+      hipLaunchKernelGGL(grad_ncc_sums_kernel, grid, dim3(kThreads), 0, s, items_dev, partials, ptiles);
+      break;
+    case XRH_SIM_PATCH_NCC:
+      hipLaunchKernelGGL(patch_kernel<false>, grid, dim3(kThreads), patch_lds_bytes, s, items_dev, partials, ptiles);
+      break;
+    case XRH_SIM_PATCH_GRAD_NCC:
+      hipLaunchKernelGGL(grad_images_kernel, dim3((unsigned)max_img_tiles, (unsigned)n_items), dim3(kThreads), 0, s,
+                         items_dev);
+      hipLaunchKernelGGL(patch_kernel<true>, grid, dim3(kThreads), patch_lds_bytes, s, items_dev, partials, ptiles);
+      break;
+    default:
+      fail(XRH_ERR_INVALID, "unknown similarity metric");
+  }
+  XRH_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(finish_kernel, dim3((unsigned)n_items), dim3(64), 0, s, items_dev, partials, ptiles, rec,
+                     sim_out_dev, n_items);
+  XRH_HIP_CHECK(hipGetLastError());
+}
+
+// ---- fixed-image processing (ImgSimMetric2D*::allocate_resources / process_mask) ----
+
+static float* fixed_buf(xrh_fixed* f, uint64_t n)
+{
+  void* p = ctx_alloc(f->ctx, n * sizeof(float));
+  f->buffers.push_back(p);
+  return (float*)p;
+}
+
+// zero-mean copy + FP64 statistics of one fixed image (ComputeZeroMeanImageAndStats,
+// xregImgSimMetric2DNCCCPU.cpp:66-75, 211-236)
+static void ncc_prepare(xrh_fixed* f, const float* img, int64_t n, const uint8_t* mask_dev, float** zf_out,
+                        double* sum_out, double* std_out, int64_t* count_out)
+{
+  hipStream_t s = f->ctx->stream;
+  double* st = (double*)ctx_scratch(f->ctx, 64);
+  hipLaunchKernelGGL(image_stats_kernel, dim3(1), dim3(1024), 0, s, img, mask_dev, n, st);
+  XRH_HIP_CHECK(hipGetLastError());
+  double h[3];
+  XRH_HIP_CHECK(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+  XRH_HIP_CHECK(hipStreamSynchronize(s));
+  const double cnt = h[1];
+  XRH_REQUIRE(cnt >= 2.0, "fixed image (or mask) needs at least two pixels");
+  const float mean = (float)(h[0] / cnt);
+  double sd = std::sqrt(h[2] / (cnt - 1.0));
+  if (sd < 1.0e-6) sd = 1.0e-6;
+  float* zf = fixed_buf(f, (uint64_t)n);
+  hipLaunchKernelGGL(sub_scalar_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, img, mean, n, zf);
+  XRH_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, s, zf, mask_dev, n, st);
+  XRH_HIP_CHECK(hipGetLastError());
+  double z;
+  XRH_HIP_CHECK(hipMemcpyAsync(&z, st, sizeof(z), hipMemcpyDeviceToHost, s));
+  XRH_HIP_CHECK(hipStreamSynchronize(s));
+  *zf_out = zf;
+  *sum_out = z;
+  *std_out = (double)(float)sd;
+  *count_out = (int64_t)cnt;
+}
+
+static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, float** den_out)
+{
+  FixedDev& h = f->host;
+  float* mean = fixed_buf(f, (uint64_t)h.num_patches);
+  float* den = fixed_buf(f, (uint64_t)h.num_patches);
+  const int d = 2 * h.patch_radius + 1;
+  hipLaunchKernelGGL(patch_stats_kernel, dim3((unsigned)((h.num_patches + 255) / 256)), dim3(256), 0,
+                     f->ctx->stream, img, h.cols, h.ppr, h.ppc, h.patch_stride, d, mean, den);
+  XRH_HIP_CHECK(hipGetLastError());
+  *mean_out = mean;
+  *den_out = den;
+}
+
+static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, int32_t cols,
+                         const xrh_sim_params* p, const uint8_t* mask_host, xrh_fixed** out)
+{
+  XRH_REQUIRE(ctx && fixed_host && p && out, "null argument");
+  XRH_REQUIRE(rows >= 2 && cols >= 2, "fixed image must be at least 2x2");
+  XRH_REQUIRE(p->metric >= XRH_SIM_NCC && p->metric <= XRH_SIM_PATCH_GRAD_NCC, "unknown metric");
+  const bool grad = (p->metric == XRH_SIM_GRAD_NCC || p->metric == XRH_SIM_PATCH_GRAD_NCC);
+  const bool patch = (p->metric == XRH_SIM_PATCH_NCC || p->metric == XRH_SIM_PATCH_GRAD_NCC);
+  if (grad) {
+    XRH_REQUIRE(p->smooth_ksize == 0 || p->smooth_ksize == 1 || p->smooth_ksize == 3 || p->smooth_ksize == 5 ||
+                    p->smooth_ksize == 7,
+                "smoothing aperture must be 0 (off) or an odd size <= 7");
+  }
+  if (patch) {
+    XRH_REQUIRE(p->patch_radius >= 1 && p->patch_radius <= 10, "patch radius must be in [1, 10]");
+    XRH_REQUIRE(p->patch_stride >= 1 && p->patch_stride <= 4, "patch stride must be in [1, 4]");
+    XRH_REQUIRE(2 * p->patch_radius + 1 <= rows && 2 * p->patch_radius + 1 <= cols,
+                "patch diameter exceeds the image");  // xregImgSimMetric2DPatchCommon.cpp:271-272
+    if (mask_host) fail(XRH_ERR_UNSUPPORTED, "masks are supported for NCC and Grad-NCC only");
+  }
+  auto* f = new xrh_fixed();
+  f->ctx = ctx;
+  try {
+    FixedDev& h = f->host;
+    std::memset(&h, 0, sizeof(h));
+    h.metric = p->metric;
+    h.rows = rows;
+    h.cols = cols;
+    h.smooth_ksize = (grad && p->smooth_ksize > 1) ? p->smooth_ksize : 0;
+    h.patch_radius = p->patch_radius;
+    h.patch_stride = p->patch_stride > 0 ? p->patch_stride : 1;
+    const int64_t n = (int64_t)rows * cols;
+    hipStream_t s = ctx->stream;
+    float* img = fixed_buf(f, (uint64_t)n);
+    XRH_HIP_CHECK(hipMemcpyAsync(img, fixed_host, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    uint8_t* mask = nullptr;
+    if (mask_host) {
+      mask = (uint8_t*)ctx_alloc(ctx, (uint64_t)n);
+      f->buffers.push_back(mask);
+      XRH_HIP_CHECK(hipMemcpyAsync(mask, mask_host, (size_t)n, hipMemcpyHostToDevice, s));
+      h.mask = mask;
+    }
+    const float* gx = img;
+    const float* gy = nullptr;
+    if (grad) {
+      float* bx = fixed_buf(f, (uint64_t)n);
+      float* by = fixed_buf(f, (uint64_t)n);
+      const int TX = (cols + kT - 1) / kT, TY = (rows + kT - 1) / kT;
+      hipLaunchKernelGGL(fixed_grad_kernel, dim3((unsigned)(TX * TY)), dim3(kThreads), 0, s, img, rows, cols,
+                         h.smooth_ksize, bx, by);
+      XRH_HIP_CHECK(hipGetLastError());
+      gx = bx;
+      gy = by;
+    }
+    if (!patch) {
+      float* z = nullptr;
+      ncc_prepare(f, gx, n, mask, &z, &h.sum_zf, &h.std_f, &h.count);
+      h.zf = z;
+      if (grad) {
+        int64_t c2 = 0;
+        ncc_prepare(f, gy, n, mask, &z, &h.sum_zfy, &h.std_fy, &c2);
+        h.zfy = z;
+      }
+    } else {
+      const int d = 2 * h.patch_radius + 1;
+      h.ppr = (cols - d) / h.patch_stride + 1;
+      h.ppc = (rows - d) / h.patch_stride + 1;
+      h.num_patches = (int64_t)h.ppr * h.ppc;
+      h.count = n;
+      h.pf = gx;
+      float *mu = nullptr, *den = nullptr;
+      patch_prepare(f, gx, &mu, &den);
+      h.pmean = mu;
+      h.pden = den;
+      if (grad) {
+        h.pfy = gy;
+        patch_prepare(f, gy, &mu, &den);
+        h.pmeany = mu;
+        h.pdeny = den;
+      }
+    }
+    f->dev = (FixedDev*)ctx_alloc(ctx, sizeof(FixedDev));
+    f->buffers.push_back(f->dev);
+    XRH_HIP_CHECK(hipMemcpyAsync(f->dev, &h, sizeof(FixedDev), hipMemcpyHostToDevice, s));
+    XRH_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void* b : f->buffers) ctx_free(ctx, b);
+    delete f;
+    throw;
+  }
+  *out = f;
+}
+
+static void sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev, int64_t num_mov,
+                        float* sim_dev_out, float* sim_host_out)
+{
+  XRH_REQUIRE(ctx && fixed && mov_dev, "null argument");
+  XRH_REQUIRE(fixed->ctx == ctx, "fixed image belongs to another context");
+  XRH_REQUIRE(num_mov >= 1, "need at least one moving image");
+  XRH_REQUIRE(sim_dev_out || sim_host_out, "no output for similarity values");
+  const FixedDev& h = fixed->host;
+  const int64_t npix = (int64_t)h.rows * h.cols;
+  const int img_tiles = sim_img_tiles(h);
+  const int ptiles = sim_tiles(h);
+  const int rec = sim_record_stride(h.metric);
+  auto align = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+  const bool need_grad = (h.metric == XRH_SIM_PATCH_GRAD_NCC);
+  const uint64_t off_items = 0;
+  const uint64_t off_part = align(sizeof(SimItem) * num_mov);
+  const uint64_t off_sim = align(off_part + sizeof(double) * rec * ptiles * num_mov);
+  const uint64_t off_grad = align(off_sim + sizeof(float) * num_mov);
+  const uint64_t total = align(off_grad + (need_grad ? sizeof(float) * 2 * npix * num_mov : 0));
+  XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  char* dev = (char*)ctx_scratch(ctx, total);
+  SimItem* hi = (SimItem*)ctx_pinned(ctx, sizeof(SimItem) * num_mov);
+  for (int64_t i = 0; i < num_mov; ++i) {
+    hi[i].mov = mov_dev + i * npix;
+    hi[i].fixed = fixed->dev;
+    hi[i].gx = need_grad ? (float*)(dev + off_grad) + 2 * i * npix : nullptr;
+    hi[i].gy = need_grad ? (float*)(dev + off_grad) + (2 * i + 1) * npix : nullptr;
+    hi[i].out_index = i;
+  }
+  XRH_HIP_CHECK(hipMemcpyAsync(dev + off_items, hi, sizeof(SimItem) * num_mov, hipMemcpyHostToDevice, ctx->stream));
+  float* sims = sim_dev_out ? sim_dev_out : (float*)(dev + off_sim);
+  launch_sim(h.metric, (const SimItem*)(dev + off_items), num_mov, img_tiles, ptiles, (double*)(dev + off_part),
+             sims, false, sim_patch_lds_bytes(h), ctx->stream);
+  if (sim_host_out) {
+    XRH_HIP_CHECK(hipMemcpyAsync(sim_host_out, sims, sizeof(float) * num_mov, hipMemcpyDeviceToHost, ctx->stream));
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+}
+
+}  // namespace xrh
+
+using namespace xrh;
+
+extern "C" {
+
+xrh_status xrh_fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, int32_t cols,
+                            const xrh_sim_params* params, const uint8_t* mask_host, xrh_fixed** out)
+{
+  return guarded([&] { fixed_create(ctx, fixed_host, rows, cols, params, mask_host, out); });
+}
+
+xrh_status xrh_fixed_destroy(xrh_fixed* fixed)
+{
+  return guarded([&] {
+    if (!fixed) return;
+    XRH_HIP_CHECK(hipStreamSynchronize(fixed->ctx->stream));
+    for (void* b : fixed->buffers) ctx_free(fixed->ctx, b);
+    delete fixed;
+  });
+}
+
+xrh_status xrh_sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev, int64_t num_mov,
+                           float* sim_dev_out, float* sim_host_out)
+{
+  return guarded([&] { sim_compute(ctx, fixed, mov_dev, num_mov, sim_dev_out, sim_host_out); });
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+"""Host-side geometry in xReg's float conventions.
+
+CameraModel::setup (lib/transforms/xregPerspectiveXform.cpp:200-254) and the inverse
+extrinsics SE3Inv (lib/transforms/xregRigidUtils.cpp) restated in float32 so the values handed
+to the device are the ones xReg itself would hold. Poses are composed in float64.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._xrh import Camera
+
+F = np.float32
+
+FRAME_POS_Z, FRAME_NEG_Z, FRAME_ON_DETECTOR = 0, 1, 2
+
+
+def _mat3_inverse_f32(m: np.ndarray) -> np.ndarray:
+    """3x3 inverse in Eigen's cofactor order (every entry cofactor * (1/det)), float32."""
+    m = m.astype(F)
+
+    def cof(i, j):
+        i1, i2, j1, j2 = (i + 1) % 3, (i + 2) % 3, (j + 1) % 3, (j + 2) % 3
+        return F(m[i1, j1] * m[i2, j2]) - F(m[i1, j2] * m[i2, j1])
+
+    c00, c10, c20 = cof(0, 0), cof(1, 0), cof(2, 0)
+    det = F(F(F(c00 * m[0, 0]) + F(c10 * m[1, 0])) + F(c20 * m[2, 0]))
+    invdet = F(F(1.0) / det)
+    r = np.empty((3, 3), dtype=F)
+    r[0] = [F(c00 * invdet), F(c10 * invdet), F(c20 * invdet)]
+    r[1] = [F(cof(0, 1) * invdet), F(cof(1, 1) * invdet), F(cof(2, 1) * invdet)]
+    r[2] = [F(cof(0, 2) * invdet), F(cof(1, 2) * invdet), F(cof(2, 2) * invdet)]
+    return r
+
+
+def naive_intrins(focal: float, rows: int, cols: int, row_sp: float, col_sp: float) -> np.ndarray:
+    """MakeNaiveIntrins (xregPerspectiveXform.cpp:200-228) for the NEG_Z frame, float32."""
+    K = np.eye(3, dtype=F)
+    K[0, 0] = F(F(focal) / F(col_sp))
+    K[1, 1] = F(F(focal) / F(row_sp))
+    K[0, 0] *= F(-1)
+    K[1, 1] *= F(-1)
+    K[0, 2] = F((cols - 1) * 0.5)
+    K[1, 2] = F((rows - 1) * 0.5)
+    return K
+
+
+def se3_inv_f32(T: np.ndarray) -> np.ndarray:
+    """SE3Inv: [R^T | -R^T t], float32 with the reference's product order."""
+    T = T.astype(F)
+    R = T[:3, :3].T.copy()
+    t = T[:3, 3]
+    out = np.eye(4, dtype=F)
+    out[:3, :3] = R
+    for i in range(3):
+        s = F(F(F(R[i, 0] * t[0]) + F(R[i, 1] * t[1])) + F(R[i, 2] * t[2]))
+        out[i, 3] = F(F(-1) * s)
+    return out
+
+
+def camera(focal: float, rows: int, cols: int, row_sp: float, col_sp: float,
+           extrins: np.ndarray | None = None) -> Camera:
+    """An xrh_camera exactly as CameraModel::setup would fill it (NEG_Z frame)."""
+    c = Camera()
+    c.num_det_rows = int(rows)
+    c.num_det_cols = int(cols)
+    c.coord_frame = FRAME_NEG_Z
+    c.focal_len = float(F(focal))
+    kinv = _mat3_inverse_f32(naive_intrins(focal, rows, cols, row_sp, col_sp))
+    for i in range(9):
+        c.intrins_inv[i] = float(kinv.reshape(-1)[i])
+    if extrins is None:
+        einv = np.eye(4, dtype=F)
+    else:
+        einv = se3_inv_f32(np.asarray(extrins))
+    for i in range(16):
+        c.extrins_inv[i] = float(einv.reshape(-1)[i])
+    # NEG_Z / POS_Z: pinhole = extrins_inv translation (xregPerspectiveXform.cpp:285-288)
+    for i in range(3):
+        c.pinhole[i] = float(einv[i, 3])
+    c.det_row_spacing = float(F(row_sp))
+    c.det_col_spacing = float(F(col_sp))
+    return c
+
+
+def camera_arrays(c: Camera):
+    """(kinv 3x3, einv 4x4, pinhole 3) float32 views of a Camera."""
+    kinv = np.array(c.intrins_inv[:], dtype=F).reshape(3, 3)
+    einv = np.array(c.extrins_inv[:], dtype=F).reshape(4, 4)
+    ph = np.array(c.pinhole[:], dtype=F)
+    return kinv, einv, ph
+
+
+def downsample_camera(c: Camera, ds: float) -> Camera:
+    """DownsampleCameraModel (xregPerspectiveXform.cpp:654-688): scales the intrinsics,
+    rounds the detector size and divides the pixel spacing by ds."""
+    kinv, einv, _ = camera_arrays(c)
+    K = np.linalg.inv(kinv.astype(np.float64))
+    K[0, 0] *= ds
+    K[1, 1] *= ds
+    K[0, 2] *= ds
+    K[1, 2] *= ds
+    rows = int(round(c.num_det_rows * ds))
+    cols = int(round(c.num_det_cols * ds))
+    out = Camera()
+    out.num_det_rows = rows
+    out.num_det_cols = cols
+    out.coord_frame = c.coord_frame
+    out.focal_len = c.focal_len
+    kinv2 = _mat3_inverse_f32(K.astype(F))
+    for i in range(9):
+        out.intrins_inv[i] = float(kinv2.reshape(-1)[i])
+    for i in range(16):
+        out.extrins_inv[i] = c.extrins_inv[i]
+    for i in range(3):
+        out.pinhole[i] = c.pinhole[i]
+    out.det_row_spacing = float(F(c.det_row_spacing / ds))
+    out.det_col_spacing = float(F(c.det_col_spacing / ds))
+    return out
+
+
+# ------------------------------------------------------------------------------------------
+# poses (float64)
+
+def rot_x(a: float) -> np.ndarray:
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[1, 0, 0], [0, c, -s], [0, s, c]], dtype=np.float64)
+
+
+def rot_y(a: float) -> np.ndarray:
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]], dtype=np.float64)
+
+
+def rot_z(a: float) -> np.ndarray:
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]], dtype=np.float64)
+
+
+def rigid(R: np.ndarray | None = None, t=(0.0, 0.0, 0.0)) -> np.ndarray:
+    T = np.eye(4, dtype=np.float64)
+    if R is not None:
+        T[:3, :3] = R
+    T[:3, 3] = t
+    return T
+
+
+# Camera axes in volume axes for an antero-posterior view: detector columns along +x,
+# detector rows along +z (superior-inferior), rays (camera -z) along +y.
+R_AP = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, -1.0], [0.0, 1.0, 0.0]])
+
+
+def exp_se3(x: np.ndarray) -> np.ndarray:
+    """ExpSE3 of [w; v] (lib/transforms/xregRigidUtils.cpp:40-85), float64."""
+    w = np.asarray(x[:3], dtype=np.float64)
+    v = np.asarray(x[3:6], dtype=np.float64)
+    W = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]], dtype=np.float64)
+    th = float(np.linalg.norm(w))
+    T = np.eye(4)
+    if th > 1.0e-14:
+        A = np.eye(3) + (math.sin(th) / th) * W + ((1 - math.cos(th)) / th ** 2) * (W @ W)
+        V = np.eye(3) + ((1 - math.cos(th)) / th ** 2) * W + ((th - math.sin(th)) / th ** 3) * (W @ W)
+        T[:3, :3] = A
+        T[:3, 3] = V @ v
+    else:
+        T[:3, 3] = v
+    return T
+
+
+def gt_pose(vol_center_phys, src_to_iso: float, view_rot: np.ndarray = R_AP) -> np.ndarray:
+    """cam->phys pose putting the isocenter (camera (0,0,-src_to_iso)) at the volume centre."""
+    return rigid(None, vol_center_phys) @ rigid(view_rot) @ rigid(None, (0.0, 0.0, src_to_iso))
+
+
+def perturbed_pose(gt: np.ndarray, vol_center_phys, rot_rad, trans_mm) -> np.ndarray:
+    """Rotation (x, y, z Euler, radians) about the volume centre and a translation (mm),
+    applied on the volume side of the gt pose."""
+    c = np.asarray(vol_center_phys, dtype=np.float64)
+    R = rot_z(rot_rad[2]) @ rot_y(rot_rad[1]) @ rot_x(rot_rad[0])
+    D = rigid(None, c) @ rigid(R, trans_mm) @ rigid(None, -c)
+    return D @ gt
+
+
+def idx_to_phys(spacing, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """ITKImagePhysicalPointTransformsAsEigen for an identity direction (float32 4x4)."""
+    s = np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,))
+    M = np.eye(4, dtype=F)
+    for i in range(3):
+        M[i, i] = F(s[i])
+        M[i, 3] = F(origin[i])
+    return M
+
+
+def vol_center_phys(size_xyz, spacing, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Physical centre at continuous index size/2 (RayCaster::xform_img_center_to_itk_phys,
+    lib/ray_cast/xregRayCastInterface.cpp:200-224)."""
+    s = np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,))
+    return np.asarray(origin, dtype=np.float64) + s * (np.asarray(size_xyz, dtype=np.float64) / 2.0)
