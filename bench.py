@@ -136,17 +136,21 @@ def main():
 
     # ---- plain: one pose per launch, synchronous read-back ----
     n_plain = min(args.plain_poses, lam)
-    pl = gens[0][:n_plain]
-    for i in range(2):
-        X.line_int(ctx, vol, w.cams, pl[i].astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
-        fx.compute(buf, 1)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for i in range(n_plain):
-        X.line_int(ctx, vol, w.cams, pl[i].astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
-        fx.compute(buf, 1)
-    ctx.synchronize()
-    plain_value = n_plain / (time.perf_counter() - t0)
+    plain_value = None
+    if n_plain > 0:
+        pl = gens[0][:n_plain]
+        for i in range(min(2, n_plain)):
+            X.line_int(ctx, vol, w.cams, pl[i].astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf,
+                       geom=geom)
+            fx.compute(buf, 1)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for i in range(n_plain):
+            X.line_int(ctx, vol, w.cams, pl[i].astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf,
+                       geom=geom)
+            fx.compute(buf, 1)
+        ctx.synchronize()
+        plain_value = n_plain / (time.perf_counter() - t0)
 
     cpu = None
     if not args.no_cpu:
@@ -169,8 +173,8 @@ def main():
                    "detector": "384x384@0.776mm", "step_mm": w.step, "lambda": lam,
                    "similarity": f"patch-grad-ncc r5 smooth{args.smooth}", "geometry": args.geom,
                    "storage": args.storage, "parallelism": "single-gpu"},
-        "plain_poses_per_s": round(plain_value, 2),
-        "batched_over_plain": round(value / plain_value, 2),
+        "plain_poses_per_s": round(plain_value, 2) if plain_value else None,
+        "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4),

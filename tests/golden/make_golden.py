@@ -1,0 +1,63 @@
+"""Writes the committed golden fixtures under tests/golden/ (run from the repo root):
+
+    python tests/golden/make_golden.py
+
+The reference ships no vectors for this path, so these fixtures come from this repository's
+oracle (FP32 restatement and FP64 oracle), whose functions are pinned by closed forms and
+brute force in tests/test_oracle_*.py. They freeze small, fully specified cases (inputs
+included) so that any later change of the oracle or of the GPU path shows up as a diff.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from oracle import oracle as O  # noqa: E402
+from xreg_amd import geometry as G  # noqa: E402
+from xreg_amd import synth  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def cam_arrays(cam):
+    kinv, einv, ph = G.camera_arrays(cam)
+    meta = np.array([cam.num_det_rows, cam.num_det_cols, cam.coord_frame], np.int32)
+    return meta, np.float32(cam.focal_len), kinv, einv, ph
+
+
+def drr_case():
+    w = synth.config("c1", n_poses=5, small=True)
+    poses = np.concatenate([w.gt[None], w.poses]).astype(np.float32).reshape(-1, 16)
+    d32, s32 = O.drr_f32(w.vol, w.idx_to_phys, w.cams, poses, step=w.step, threads=1)
+    d64, s64 = O.drr_f64(w.vol, w.idx_to_phys, w.cams, poses.astype(np.float64), step=w.step, threads=1)
+    dmax, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, poses, step=w.step, kernel=1, threads=1)
+    meta, focal, kinv, einv, ph = cam_arrays(w.cams[0])
+    np.savez_compressed(OUT / "drr_c1_small.npz", vol=w.vol, idx_to_phys=w.idx_to_phys, step=np.float32(w.step),
+                        poses=poses, cam_meta=meta, cam_focal=focal, cam_kinv=kinv, cam_einv=einv, cam_pinhole=ph,
+                        drr_f32=d32, drr_f64=d64, drr_max_f32=dmax, samples=np.int64(s32),
+                        samples_f64=np.int64(s64))
+
+
+def sim_case():
+    w = synth.config("c2", n_poses=3, small=True)
+    poses = np.concatenate([w.gt[None], w.poses]).astype(np.float32).reshape(-1, 16)
+    drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, poses, step=w.step, threads=1)
+    fixed = synth.noisy_fixed(drr[0])
+    out = {"fixed": fixed, "mov": drr}
+    for metric in range(4):
+        for k in (0, 5):
+            out[f"sim{metric}_k{k}_f32"] = O.sim_f32(metric, fixed, drr, smooth_ksize=k, threads=1)
+            out[f"sim{metric}_k{k}_f64"] = O.sim_f64(metric, fixed.astype(np.float64), drr.astype(np.float64),
+                                                     smooth_ksize=k, threads=1)
+    np.savez_compressed(OUT / "sim_c2_small.npz", **out)
+
+
+if __name__ == "__main__":
+    drr_case()
+    sim_case()
+    print("wrote", sorted(p.name for p in OUT.glob("*.npz")))

@@ -1,0 +1,95 @@
+"""The oracle reproduces the committed golden fixtures (CPU), and the GPU path matches them
+(GPU-marked)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.helpers import DRR_REL_TOL, SIM_ABS_TOL, drr_rel_err
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def _cam(z):
+    from xreg_amd._xrh import Camera
+
+    c = Camera()
+    c.num_det_rows, c.num_det_cols, c.coord_frame = (int(v) for v in z["cam_meta"])
+    c.focal_len = float(z["cam_focal"])
+    for i, v in enumerate(z["cam_kinv"].reshape(-1)):
+        c.intrins_inv[i] = float(v)
+    for i, v in enumerate(z["cam_einv"].reshape(-1)):
+        c.extrins_inv[i] = float(v)
+    for i, v in enumerate(z["cam_pinhole"]):
+        c.pinhole[i] = float(v)
+    return c
+
+
+@pytest.fixture(scope="module")
+def gdrr():
+    return np.load(GOLD / "drr_c1_small.npz")
+
+
+@pytest.fixture(scope="module")
+def gsim():
+    return np.load(GOLD / "sim_c2_small.npz")
+
+
+def test_oracle_reproduces_golden_drr(oracle, gdrr):
+    z = gdrr
+    cam = _cam(z)
+    d32, s = oracle.drr_f32(z["vol"], z["idx_to_phys"], [cam], z["poses"], step=float(z["step"]))
+    assert np.array_equal(d32, z["drr_f32"]) and s == int(z["samples"])
+    d64, _ = oracle.drr_f64(z["vol"], z["idx_to_phys"], [cam], z["poses"].astype(np.float64),
+                            step=float(z["step"]))
+    assert np.allclose(d64, z["drr_f64"], rtol=1e-13, atol=1e-15)
+    dmax, _ = oracle.drr_f32(z["vol"], z["idx_to_phys"], [cam], z["poses"], step=float(z["step"]), kernel=1)
+    assert np.array_equal(dmax, z["drr_max_f32"])
+
+
+def test_oracle_reproduces_golden_sims(oracle, gsim):
+    z = gsim
+    for metric in range(4):
+        for k in (0, 5):
+            got = oracle.sim_f32(metric, z["fixed"], z["mov"], smooth_ksize=k)
+            assert np.array_equal(got, z[f"sim{metric}_k{k}_f32"]), (metric, k)
+            got64 = oracle.sim_f64(metric, z["fixed"].astype(np.float64), z["mov"].astype(np.float64),
+                                   smooth_ksize=k)
+            assert np.allclose(got64, z[f"sim{metric}_k{k}_f64"], rtol=0, atol=1e-13)
+
+
+@pytest.mark.gpu
+def test_gpu_matches_golden_drr(ctx, gdrr):
+    from xreg_amd import _xrh as X
+
+    z = gdrr
+    cam = _cam(z)
+    n = z["poses"].shape[0]
+    rows, cols = cam.num_det_rows, cam.num_det_cols
+    vol = X.Volume(ctx, z["vol"], z["idx_to_phys"])
+    buf = ctx.alloc(4 * n * rows * cols)
+    X.line_int(ctx, vol, [cam], z["poses"], np.zeros(n, np.int64), float(z["step"]), buf)
+    got = buf.download(np.float32).reshape(n, rows, cols)
+    assert drr_rel_err(got, z["drr_f32"]).max() <= DRR_REL_TOL
+    s = X.line_int_sample_count(ctx, vol, [cam], z["poses"], np.zeros(n, np.int64), float(z["step"]))
+    assert s == int(z["samples"])
+    buf.free()
+    vol.close()
+
+
+@pytest.mark.gpu
+def test_gpu_matches_golden_sims(ctx, gsim):
+    from xreg_amd import _xrh as X
+
+    z = gsim
+    mov = z["mov"]
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    for metric in range(4):
+        for k in (0, 5):
+            f = X.Fixed(ctx, z["fixed"], metric, smooth_ksize=k)
+            got = f.compute(buf, mov.shape[0])
+            f.close()
+            assert np.abs(got - z[f"sim{metric}_k{k}_f32"]).max() <= SIM_ABS_TOL, (metric, k)
+            assert np.abs(got - z[f"sim{metric}_k{k}_f64"]).max() <= SIM_ABS_TOL, (metric, k)
+    buf.free()

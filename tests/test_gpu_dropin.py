@@ -1,0 +1,80 @@
+"""The drop-in C++ classes (RayCasterLineIntHIP, ImgSimMetric2D*HIP, CombineMean) driven like
+Intensity2D3DRegi::setup / obj_fn give the CPU reference's DRRs and objective values."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.helpers import DRR_REL_TOL, SIM_ABS_TOL, drr_rel_err, poses_f32
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+DRIVER = ROOT / "xreg_amd" / "_build" / "dropin_driver"
+
+
+def _run(tmp_path, w, fixed, metric, smooth, extr, geom=0):
+    cam = w.cams[0]
+    rows, cols = cam.num_det_rows, cam.num_det_cols
+    X, Y, Z = w.size_xyz
+    nv, npose = len(fixed), w.poses.shape[0]
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        np.array([X, Y, Z, rows, cols, nv, npose, metric, smooth, 5, geom], np.int32).tofile(f)
+        np.array([w.spacing, w.step, 1020.0, cam.det_col_spacing], np.float32).tofile(f)
+        np.ascontiguousarray(w.vol, np.float32).tofile(f)
+        for fx in fixed:
+            np.ascontiguousarray(fx, np.float32).tofile(f)
+        for e in extr:
+            np.ascontiguousarray(e, np.float32).tofile(f)
+        poses_f32(w.poses).tofile(f)
+    out = tmp_path / "out.bin"
+    print(f"case: dropin metric={metric} views={nv} poses={npose}", flush=True)
+    r = subprocess.run([str(DRIVER), str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    sims = np.frombuffer(raw, np.float32, npose, 0)
+    drr = np.frombuffer(raw, np.float32, nv * npose * rows * cols, 4 * npose).reshape(nv * npose, rows, cols)
+    off = 4 * npose + 4 * nv * npose * rows * cols
+    samples = int(np.frombuffer(raw, np.uint64, 1, off)[0])
+    sims2 = np.frombuffer(raw, np.float32, npose, off + 8)
+    return sims, drr, samples, sims2
+
+
+@pytest.mark.parametrize("metric,smooth", [(0, 0), (1, 5), (3, 5)])
+def test_dropin_single_view(tmp_path, oracle, metric, smooth):
+    from xreg_amd import synth
+
+    w = synth.config("c2", n_poses=5, small=True)
+    g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(w.gt[None]), step=w.step)
+    fixed = synth.noisy_fixed(g[0])
+    sims, drr, samples, sims2 = _run(tmp_path, w, [fixed], metric, smooth, [np.eye(4)])
+    ref_drr, s_ref = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(w.poses), step=w.step)
+    assert drr_rel_err(drr, ref_drr).max() <= DRR_REL_TOL
+    assert samples == s_ref
+    ref = oracle.sim_f32(metric, fixed, ref_drr, smooth_ksize=smooth)
+    assert np.abs(sims - ref).max() <= SIM_ABS_TOL
+    # second obj_fn call with the poses reversed
+    assert np.array_equal(sims2, sims[::-1])
+
+
+def test_dropin_three_views_mean_combine(tmp_path, oracle):
+    from xreg_amd import geometry as G
+    from xreg_amd import synth
+
+    w = synth.config("c3", n_poses=4, small=True)
+    extr = synth.multiview_extrinsics(1020.0, 3, 30.0)
+    fixed = []
+    for cam in w.cams:
+        g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.gt[None]), step=w.step)
+        fixed.append(synth.noisy_fixed(g[0]))
+    sims, drr, _, _ = _run(tmp_path, w, fixed, 1, 5, extr)
+    n = w.poses.shape[0]
+    per_view = []
+    for v, cam in enumerate(w.cams):
+        d, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.poses), step=w.step)
+        assert drr_rel_err(drr[v * n:(v + 1) * n], d).max() <= DRR_REL_TOL
+        per_view.append(oracle.grad_ncc_f32(fixed[v], d, 5))
+    want = np.mean(np.stack(per_view), axis=0)
+    assert np.abs(sims - want).max() <= SIM_ABS_TOL

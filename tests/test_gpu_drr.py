@@ -86,10 +86,12 @@ def test_drr_half_storage_error_vs_fp64(ctx, oracle, c1s, storage, layout):
     ref64, _ = oracle.drr_f64(c1s.vol, c1s.idx_to_phys, c1s.cams, poses_f32(poses).astype(np.float64),
                               step=c1s.step)
     got = _gpu_drr(ctx, c1s, poses, storage=storage, layout=layout)
-    err = drr_rel_err(got, ref64)
-    # FP16 rounds voxels to 2^-11 relative, BF16 to 2^-8; line integrals average that down
-    bound = 1e-3 if storage == 1 else 8e-3
-    print(f"storage={storage}: max rel vs FP64 {err.max():.3e}")
+    # FP16 rounds each voxel by up to 2^-11 relative, BF16 by 2^-8: the storage error of a
+    # line integral is bounded by that fraction of the integral of |mu| along the ray, i.e.
+    # of the image maximum (grazing pixels inherit it undiluted, hence max-normalised)
+    err = np.abs(got - ref64) / ref64.max()
+    bound = 2.0 ** -11 if storage == 1 else 2.0 ** -8
+    print(f"storage={storage}: max |err| / max(DRR) vs FP64 {err.max():.3e} (bound {bound:.2e})")
     assert err.max() < bound
 
 

@@ -62,7 +62,34 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(sources)} sources")
+    build_cpp(verbose)
     return LIB
+
+
+CPP_LIB = HERE / "libxreg_hip.so"
+DRIVER = BUILD / "dropin_driver"
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def build_cpp(verbose: bool = False) -> Path:
+    """The drop-in C++ classes (plain g++, linked against libxrh.so only through the C-ABI) and
+    the driver that exercises them like Intensity2D3DRegi."""
+    src = HERE / "cpp" / "xreg_hip.cpp"
+    tmp = CPP_LIB.with_suffix(".so.tmp")
+    cmd = [CXX, *CXXFLAGS, "-shared", str(src), f"-L{HERE}", "-lxrh", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for xreg_hip.cpp:\n{res.stderr}")
+    os.replace(tmp, CPP_LIB)
+    drv = HERE.parent / "tests" / "cpp" / "dropin_driver.cpp"
+    cmd = [CXX, *CXXFLAGS, str(drv), f"-L{HERE}", "-lxreg_hip", "-lxrh", "-Wl,-rpath,$ORIGIN/..", "-o", str(DRIVER)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for dropin_driver.cpp:\n{res.stderr}")
+    if verbose:
+        print(f"built {CPP_LIB} and {DRIVER}")
+    return CPP_LIB
 
 
 def clean() -> None:

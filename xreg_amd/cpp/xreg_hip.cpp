@@ -1,0 +1,612 @@
+// xreg_hip.cpp -- drop-in HIP subclasses of RayCaster / ImgSimMetric2D over the C-ABI.
+// Plain C++17 (g++): everything device-side happens behind include/xrh.h.
+#include "xreg_hip.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+namespace xreg {
+
+// ------------------------------------------------------------------------------------------
+// small value types
+
+FrameTransform FrameTransform::Identity()
+{
+  FrameTransform t;
+  t.m.fill(0.0f);
+  t.m[0] = t.m[5] = t.m[10] = t.m[15] = 1.0f;
+  return t;
+}
+
+FrameTransform FrameTransform::operator*(const FrameTransform& b) const
+{
+  // Affine * Affine: linear = La*Lb, translation = La*tb + ta (Eigen Transform product)
+  FrameTransform r = Identity();
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      r.m[4 * i + j] = (m[4 * i] * b.m[j] + m[4 * i + 1] * b.m[4 + j]) + m[4 * i + 2] * b.m[8 + j];
+    }
+    r.m[4 * i + 3] = ((m[4 * i] * b.m[3] + m[4 * i + 1] * b.m[7]) + m[4 * i + 2] * b.m[11]) + m[4 * i + 3];
+  }
+  return r;
+}
+
+static std::array<float, 9> mat3_inverse(const std::array<float, 9>& m)
+{
+  // Eigen's cofactor form: entry = cofactor * (1/det)
+  auto M = [&](int r, int c) { return m[3 * r + c]; };
+  auto cof = [&](int i, int j) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    return M(i1, j1) * M(i2, j2) - M(i1, j2) * M(i2, j1);
+  };
+  const float c00 = cof(0, 0), c10 = cof(1, 0), c20 = cof(2, 0);
+  const float det = (c00 * M(0, 0) + c10 * M(1, 0)) + c20 * M(2, 0);
+  if (!(std::fabs(det) > 0.0f)) throw StringMessageException("singular intrinsic matrix");
+  const float inv = 1.0f / det;
+  return {c00 * inv, c10 * inv, c20 * inv, cof(0, 1) * inv, cof(1, 1) * inv, cof(2, 1) * inv,
+          cof(0, 2) * inv, cof(1, 2) * inv, cof(2, 2) * inv};
+}
+
+static FrameTransform se3_inv(const FrameTransform& T)
+{
+  // SE3Inv (lib/transforms/xregRigidUtils.cpp): [R^T | -R^T t]
+  FrameTransform r = FrameTransform::Identity();
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.m[4 * i + j] = T.m[4 * j + i];
+  for (int i = 0; i < 3; ++i) {
+    const float s = (r.m[4 * i] * T.m[3] + r.m[4 * i + 1] * T.m[7]) + r.m[4 * i + 2] * T.m[11];
+    r.m[4 * i + 3] = -1.0f * s;
+  }
+  return r;
+}
+
+void CameraModel::setup(float f, size_type nr, size_type nc, float rs, float cs)
+{
+  if (!(f > 1.0e-8f) || !nr || !nc || !(rs > 1.0e-8f) || !(cs > 1.0e-8f)) {
+    throw StringMessageException("invalid camera model parameters");
+  }
+  focal_len = f;
+  num_det_rows = nr;
+  num_det_cols = nc;
+  det_row_spacing = rs;
+  det_col_spacing = cs;
+  // MakeNaiveIntrins (xregPerspectiveXform.cpp:200-228)
+  intrins = {f / cs, 0, 0, 0, f / rs, 0, 0, 0, 1};
+  if (coord_frame_type == kORIGIN_AT_FOCAL_PT_DET_NEG_Z) {
+    intrins[0] *= -1.0f;
+    intrins[4] *= -1.0f;
+  }
+  intrins[2] = (float)((double)(nc - 1) * 0.5);
+  intrins[5] = (float)((double)(nr - 1) * 0.5);
+  intrins_inv = mat3_inverse(intrins);
+  extrins = FrameTransform::Identity();
+  extrins_inv = FrameTransform::Identity();
+  pinhole_pt = {0, 0, 0};
+}
+
+void CameraModel::setup(const std::array<float, 9>& K, const FrameTransform& E, size_type nr, size_type nc,
+                        float rs, float cs)
+{
+  if (!nr || !nc || !(rs > 1.0e-8f) || !(cs > 1.0e-8f)) throw StringMessageException("invalid camera model");
+  num_det_rows = nr;
+  num_det_cols = nc;
+  det_row_spacing = rs;
+  det_col_spacing = cs;
+  intrins = K;
+  intrins_inv = mat3_inverse(K);
+  // FocalLenFromIntrins (xregPerspectiveXform.cpp:186-190)
+  focal_len = (std::fabs(K[0] * cs) + std::fabs(K[4] * rs)) / 2.0f;
+  extrins = E;
+  extrins_inv = se3_inv(E);
+  if (coord_frame_type != kORIGIN_ON_DETECTOR) {
+    pinhole_pt = {extrins_inv.m[3], extrins_inv.m[7], extrins_inv.m[11]};
+  } else {
+    for (int i = 0; i < 3; ++i) pinhole_pt[i] = extrins_inv.m[4 * i + 2] * focal_len + extrins_inv.m[4 * i + 3];
+  }
+}
+
+xrh_camera CameraModel::to_xrh() const
+{
+  xrh_camera c;
+  std::memset(&c, 0, sizeof(c));
+  c.num_det_rows = (int32_t)num_det_rows;
+  c.num_det_cols = (int32_t)num_det_cols;
+  c.coord_frame = (int32_t)coord_frame_type;
+  c.focal_len = focal_len;
+  std::memcpy(c.intrins_inv, intrins_inv.data(), sizeof(c.intrins_inv));
+  std::memcpy(c.extrins_inv, extrins_inv.m.data(), sizeof(c.extrins_inv));
+  std::memcpy(c.pinhole, pinhole_pt.data(), sizeof(c.pinhole));
+  c.det_row_spacing = det_row_spacing;
+  c.det_col_spacing = det_col_spacing;
+  return c;
+}
+
+FrameTransform Volume::idx_to_phys() const
+{
+  FrameTransform t = FrameTransform::Identity();
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) t.m[4 * r + c] = (float)(direction[3 * r + c] * spacing[c]);
+    t.m[4 * r + 3] = (float)origin[r];
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------------------------------
+// context + errors
+
+void check_xrh(xrh_status s)
+{
+  if (s == XRH_OK) return;
+  if (s == XRH_ERR_UNSUPPORTED) throw RayCaster::UnsupportedOperationException();
+  throw StringMessageException(std::string("HIP backend error: ") + xrh_last_error());
+}
+
+HIPContext& HIPContext::get(int device)
+{
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<HIPContext>> ctxs;
+  std::lock_guard<std::mutex> lk(mu);
+  auto& p = ctxs[device];
+  if (!p) p.reset(new HIPContext(device));
+  return *p;
+}
+
+HIPContext::HIPContext(int device) { check_xrh(xrh_ctx_create(device, 0, &ctx_)); }
+
+HIPContext::~HIPContext()
+{
+  if (ctx_) xrh_ctx_destroy(ctx_);
+}
+
+// ------------------------------------------------------------------------------------------
+// RayCaster base (lib/ray_cast/xregRayCastInterface.cpp:30-351)
+
+void RayCaster::set_volume(VolPtr v)
+{
+  vols_ = {v};
+  vols_changed();
+}
+
+void RayCaster::set_volumes(const VolList& v)
+{
+  vols_ = v;
+  vols_changed();
+}
+
+void RayCaster::set_camera_model(const CameraModel& c)
+{
+  camera_models_ = {c};
+  camera_models_changed();
+}
+
+void RayCaster::set_camera_models(const CameraModelList& c)
+{
+  camera_models_ = c;
+  camera_models_changed();
+}
+
+void RayCaster::set_camera_model_proj_associations(const CamModelAssocList& a)
+{
+  if (a.size() != num_projs_) throw StringMessageException("association list size != num_projs");
+  cam_model_for_proj_ = a;
+}
+
+void RayCaster::distribute_xforms_among_cam_models(const FrameTransformList& xforms)
+{
+  const size_type n = xforms.size(), nc = num_camera_models();
+  if (n * nc != num_projs()) throw StringMessageException("xforms * cameras != num_projs");
+  size_type g = 0;
+  for (size_type c = 0; c < nc; ++c) {
+    for (size_type p = 0; p < n; ++p, ++g) {
+      xforms_cam_to_itk_phys_[g] = xforms[p];
+      cam_model_for_proj_[g] = c;
+    }
+  }
+}
+
+void RayCaster::distribute_xform_among_cam_models(const FrameTransform& x)
+{
+  distribute_xforms_among_cam_models(FrameTransformList(1, x));
+}
+
+void RayCaster::set_num_projs(size_type n)
+{
+  if (max_num_projs_ && num_projs_ > max_num_projs_) throw StringMessageException("too many projections");
+  num_projs_ = n;
+  cam_model_for_proj_.resize(num_projs_);
+}
+
+void RayCaster::set_xforms_cam_to_itk_phys(const FrameTransformList& x)
+{
+  set_num_projs(x.size());
+  xforms_cam_to_itk_phys_ = x;
+}
+
+FrameTransform RayCaster::xform_img_center_to_itk_phys(size_type vol_idx) const
+{
+  const Volume& v = *vols_[vol_idx];
+  FrameTransform x = FrameTransform::Identity();
+  for (int r = 0; r < 3; ++r) {
+    double p = v.origin[r];
+    for (int c = 0; c < 3; ++c) p += v.direction[3 * r + c] * v.spacing[c] * (v.size[c] / 2.0);
+    x.m[4 * r + 3] = (float)p;
+  }
+  return x;
+}
+
+FrameTransform RayCaster::xform_cam_wrt_carm_center_of_rot(size_type cam_idx) const
+{
+  FrameTransform x = FrameTransform::Identity();
+  x.m[11] -= camera_models_[cam_idx].focal_len / 2;
+  return x;
+}
+
+void RayCaster::post_multiply_all_xforms(const FrameTransform& x)
+{
+  for (size_type i = 0; i < num_projs_; ++i) xforms_cam_to_itk_phys_[i] = xforms_cam_to_itk_phys_[i] * x;
+}
+
+void RayCaster::pre_multiply_all_xforms(const FrameTransform& x)
+{
+  for (size_type i = 0; i < num_projs_; ++i) xforms_cam_to_itk_phys_[i] = x * xforms_cam_to_itk_phys_[i];
+}
+
+void RayCaster::allocate_resources()
+{
+  if (!num_projs_) throw StringMessageException("num_projs must be set before allocate_resources");
+  if (camera_models_.empty()) throw StringMessageException("camera models must be set before allocate_resources");
+  max_num_projs_ = num_projs_;
+  xforms_cam_to_itk_phys_.resize(num_projs_);
+  cam_model_for_proj_.resize(num_projs_, 0);
+  resources_allocated_ = true;
+}
+
+void RayCaster::set_bg_proj(ImagePtr& p, bool use)
+{
+  bg_projs_for_each_cam_.assign(num_camera_models(), p);
+  use_bg_projs_ = use;
+  bg_projs_updated_ = true;
+}
+
+void RayCaster::set_bg_projs(std::vector<ImagePtr>& ps, bool use)
+{
+  if (ps.size() != num_camera_models()) throw StringMessageException("one background per camera");
+  bg_projs_for_each_cam_ = ps;
+  use_bg_projs_ = use;
+  bg_projs_updated_ = true;
+}
+
+// ------------------------------------------------------------------------------------------
+// RayCasterLineIntHIP
+
+RayCasterLineIntHIP::RayCasterLineIntHIP(int device, int32_t geom, int32_t storage)
+    : ctx_(HIPContext::get(device).ctx()), geom_(geom), storage_(storage)
+{
+}
+
+RayCasterLineIntHIP::~RayCasterLineIntHIP()
+{
+  for (auto* v : dev_vols_) xrh_volume_destroy(v);
+  if (drr_dev_) xrh_device_free(ctx_, drr_dev_);
+  if (bg_dev_) xrh_device_free(ctx_, bg_dev_);
+}
+
+void RayCasterLineIntHIP::vols_changed()
+{
+  for (auto* v : dev_vols_) check_xrh(xrh_volume_destroy(v));
+  dev_vols_.clear();
+  for (const auto& vp : vols_) {
+    const FrameTransform i2p = vp->idx_to_phys();
+    const int64_t size[3] = {(int64_t)vp->size[0], (int64_t)vp->size[1], (int64_t)vp->size[2]};
+    xrh_volume* h = nullptr;
+    check_xrh(xrh_volume_create(ctx_, vp->buf.data(), size, i2p.m.data(), storage_, &h));
+    dev_vols_.push_back(h);
+  }
+}
+
+void RayCasterLineIntHIP::set_num_projs(size_type n)
+{
+  RayCaster::set_num_projs(n);
+  host_stale_ = true;
+}
+
+void RayCasterLineIntHIP::allocate_resources()
+{
+  RayCaster::allocate_resources();
+  const size_type rows = camera_models_[0].num_det_rows, cols = camera_models_[0].num_det_cols;
+  for (const auto& c : camera_models_) {
+    // xregRayCastBaseCPU.cpp:60-70
+    if (c.num_det_rows != rows) throw StringMessageException("Inconsistent number rows in camera model images!");
+    if (c.num_det_cols != cols) throw StringMessageException("Inconsistent number columns in camera model images!");
+  }
+  const size_type need = num_projs_ * rows * cols;
+  if (need > drr_capacity_) {
+    if (drr_dev_) check_xrh(xrh_device_free(ctx_, drr_dev_));
+    drr_dev_ = nullptr;
+    void* p = nullptr;
+    check_xrh(xrh_device_alloc(ctx_, need * sizeof(float), &p));
+    drr_dev_ = (float*)p;
+    drr_capacity_ = need;
+  }
+  if (!ext_host_buf_) host_buf_.resize(need);
+  host_stale_ = true;
+}
+
+std::vector<xrh_camera> RayCasterLineIntHIP::xrh_cams() const
+{
+  std::vector<xrh_camera> v;
+  for (const auto& c : camera_models_) v.push_back(c.to_xrh());
+  return v;
+}
+
+void RayCasterLineIntHIP::compute(size_type vol_idx)
+{
+  if (!resources_allocated_) throw StringMessageException("allocate_resources() must be called first");
+  if (vol_idx >= dev_vols_.size()) throw StringMessageException("volume index out of range");
+  if (interp_method_ != kRAY_CAST_INTERP_LINEAR) {
+    throw UnsupportedOperationException();  // linear only, as RayCasterOCL (xregRayCastBaseOCL.cpp:338-341)
+  }
+  const auto cams = xrh_cams();
+  std::vector<float> xforms(16 * num_projs_);
+  for (size_type p = 0; p < num_projs_; ++p) std::memcpy(&xforms[16 * p], xforms_cam_to_itk_phys_[p].m.data(), 64);
+  std::vector<int64_t> c4p(cam_model_for_proj_.begin(), cam_model_for_proj_.begin() + num_projs_);
+  const float* bg = nullptr;
+  if (use_bg_projs_) {
+    if (bg_projs_for_each_cam_.size() != num_camera_models()) throw StringMessageException("missing background images");
+    const size_type npx = camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
+    if (!bg_dev_ || bg_projs_updated_) {
+      if (bg_dev_) check_xrh(xrh_device_free(ctx_, bg_dev_));
+      void* p = nullptr;
+      check_xrh(xrh_device_alloc(ctx_, npx * num_camera_models() * sizeof(float), &p));
+      bg_dev_ = (float*)p;
+      std::vector<float> all(npx * num_camera_models());
+      for (size_type c = 0; c < num_camera_models(); ++c) {
+        std::memcpy(&all[c * npx], bg_projs_for_each_cam_[c]->buf.data(), npx * sizeof(float));
+      }
+      check_xrh(xrh_copy_h2d(ctx_, bg_dev_, all.data(), all.size() * sizeof(float)));
+      bg_projs_updated_ = false;
+    }
+    bg = bg_dev_;
+  }
+  const int32_t store = (proj_store_meth_ == kRAY_CAST_PIXEL_ACCUM) ? XRH_STORE_ACCUM : XRH_STORE_REPLACE;
+  if (store == XRH_STORE_ACCUM && host_stale_ == false && ext_host_buf_) {
+    // an external host buffer may have been written by the caller: it is the accumulation base
+    check_xrh(xrh_copy_h2d(ctx_, drr_dev_, ext_host_buf_, num_projs_ * camera_models_[0].num_det_rows *
+                                                              camera_models_[0].num_det_cols * sizeof(float)));
+  }
+  const int32_t k = (kernel_id_ == kRAY_CAST_LINE_INT_MAX_KERNEL) ? XRH_LINE_INT_MAX : XRH_LINE_INT_SUM;
+  check_xrh(xrh_line_int(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(), c4p.data(),
+                         (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, drr_dev_));
+  host_stale_ = true;
+}
+
+uint64_t RayCasterLineIntHIP::sample_count(size_type vol_idx)
+{
+  const auto cams = xrh_cams();
+  std::vector<float> xforms(16 * num_projs_);
+  for (size_type p = 0; p < num_projs_; ++p) std::memcpy(&xforms[16 * p], xforms_cam_to_itk_phys_[p].m.data(), 64);
+  std::vector<int64_t> c4p(cam_model_for_proj_.begin(), cam_model_for_proj_.begin() + num_projs_);
+  uint64_t s = 0;
+  check_xrh(xrh_line_int_sample_count(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(),
+                                      c4p.data(), (int64_t)num_projs_, ray_step_size_, geom_, &s));
+  return s;
+}
+
+void RayCasterLineIntHIP::sync_to_host()
+{
+  if (!host_stale_) return;
+  const size_type n = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
+  float* dst = ext_host_buf_ ? ext_host_buf_ : host_buf_.data();
+  check_xrh(xrh_copy_d2h(ctx_, dst, drr_dev_, n * sizeof(float)));
+  host_stale_ = false;
+}
+
+ImagePtr RayCasterLineIntHIP::proj(size_type proj_idx)
+{
+  // a deep copy: the reference's shallow alias is invalidated by the next compute() anyway
+  sync_to_host();
+  const auto& cam = camera_models_[cam_model_for_proj_[proj_idx]];
+  auto img = std::make_shared<Image2D>();
+  img->rows = cam.num_det_rows;
+  img->cols = cam.num_det_cols;
+  img->row_spacing = cam.det_row_spacing;
+  img->col_spacing = cam.det_col_spacing;
+  const float* src = (ext_host_buf_ ? ext_host_buf_ : host_buf_.data()) + proj_idx * img->rows * img->cols;
+  img->buf.assign(src, src + img->rows * img->cols);
+  return img;
+}
+
+RayCaster::PixelScalar2D* RayCasterLineIntHIP::raw_host_pixel_buf()
+{
+  sync_to_host();
+  return ext_host_buf_ ? ext_host_buf_ : host_buf_.data();
+}
+
+void RayCasterLineIntHIP::use_external_host_pixel_buf(void* buf)
+{
+  ext_host_buf_ = (float*)buf;
+  host_stale_ = true;
+}
+
+size_type RayCasterLineIntHIP::max_num_projs_possible() const
+{
+  // bounded by the context budget / device memory rather than a single-allocation cap
+  const size_type npx = camera_models_.empty() ? 1 : camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
+  const size_type per = npx * sizeof(float);
+  return (size_type)((64ull << 30) / (per ? per : 1));
+}
+
+void RayCasterLineIntHIP::use_other_proj_buf(RayCaster*) { throw UnsupportedOperationException(); }
+
+// ------------------------------------------------------------------------------------------
+// ImgSimMetric2D
+
+void ImgSimMetric2D::set_mov_imgs_buf_from_ray_caster(RayCaster*, size_type) { throw UnsupportedOperationException(); }
+
+void ImgSimMetric2D::set_mask(ImageMaskPtr m)
+{
+  mask_updated_ = mask_updated_ || (m.get() != mask_.get());
+  mask_ = m;
+}
+
+ImgSimMetric2DHIP::ImgSimMetric2DHIP(int32_t metric, int device) : ctx_(HIPContext::get(device).ctx())
+{
+  params_.metric = metric;
+  params_.smooth_ksize = 0;
+  params_.patch_radius = 5;
+  params_.patch_stride = 1;
+}
+
+ImgSimMetric2DHIP::~ImgSimMetric2DHIP()
+{
+  if (fixed_) xrh_fixed_destroy(fixed_);
+  if (mov_dev_) xrh_device_free(ctx_, mov_dev_);
+}
+
+void ImgSimMetric2DHIP::set_mov_imgs_buf_from_ray_caster(RayCaster* rc, size_type proj_offset)
+{
+  rc_ = rc;
+  proj_off_ = proj_offset;
+  host_mov_ = nullptr;
+}
+
+void ImgSimMetric2DHIP::set_mov_imgs_host_buf(Scalar* buf, size_type proj_offset)
+{
+  if (rc_) throw StringMessageException("moving images already bound to a ray caster");
+  proj_off_ = proj_offset;
+  host_mov_ = buf;
+}
+
+void ImgSimMetric2DHIP::allocate_resources()
+{
+  ImgSimMetric2D::allocate_resources();
+  if (!fixed_img_) throw StringMessageException("fixed image must be set before allocate_resources");
+  if (!rc_ && !host_mov_) throw StringMessageException("moving images must be bound before allocate_resources");
+  if (fixed_) check_xrh(xrh_fixed_destroy(fixed_));
+  fixed_ = nullptr;
+  const unsigned char* mk = mask_ ? mask_->data() : nullptr;
+  check_xrh(xrh_fixed_create(ctx_, fixed_img_->buf.data(), (int32_t)fixed_img_->rows, (int32_t)fixed_img_->cols,
+                             &params_, mk, &fixed_));
+  mask_updated_ = false;
+}
+
+void ImgSimMetric2DHIP::compute()
+{
+  if (!fixed_) throw StringMessageException("allocate_resources() must be called first");
+  if (mask_updated_) allocate_resources();
+  const size_type npx = fixed_img_->rows * fixed_img_->cols;
+  const float* mov = nullptr;
+  if (rc_) {
+    mov = rc_->device_pixel_buf() + proj_off_ * npx;
+  } else {
+    const size_type need = num_mov_imgs_ * npx;
+    if (need > mov_dev_capacity_) {
+      if (mov_dev_) check_xrh(xrh_device_free(ctx_, mov_dev_));
+      void* p = nullptr;
+      check_xrh(xrh_device_alloc(ctx_, need * sizeof(float), &p));
+      mov_dev_ = (float*)p;
+      mov_dev_capacity_ = need;
+    }
+    check_xrh(xrh_copy_h2d(ctx_, mov_dev_, host_mov_ + proj_off_ * npx, need * sizeof(float)));
+    mov = mov_dev_;
+  }
+  sim_vals_.resize(num_mov_imgs_);
+  check_xrh(xrh_sim_compute(ctx_, fixed_, mov, (int64_t)num_mov_imgs_, nullptr, sim_vals_.data()));
+}
+
+// ------------------------------------------------------------------------------------------
+
+void ImgSimMetric2DCombineMean::allocate_resources()
+{
+  sim_objs_.resize(num_sim_metrics_);
+  sim_vals_.resize(num_projs_per_sim_metric_);
+}
+
+void ImgSimMetric2DCombineMean::set_sim_metric(size_type i, ImgSimMetric2D* s)
+{
+  sim_objs_[i] = s;
+  if (s->num_moving_images() != num_projs_per_sim_metric_) throw StringMessageException("metric size mismatch");
+}
+
+void ImgSimMetric2DCombineMean::compute()
+{
+  sim_vals_.assign(num_projs_per_sim_metric_, 0.0f);
+  for (size_type s = 0; s < num_sim_metrics_; ++s)
+    for (size_type p = 0; p < num_projs_per_sim_metric_; ++p) sim_vals_[p] += sim_objs_[s]->sim_val(p);
+  for (size_type p = 0; p < num_projs_per_sim_metric_; ++p) sim_vals_[p] /= num_sim_metrics_;
+}
+
+// ------------------------------------------------------------------------------------------
+
+std::shared_ptr<RayCaster> LineIntRayCasterFromBackend(const std::string& b)
+{
+  if (b == "hip") return std::make_shared<RayCasterLineIntHIP>();
+  throw StringMessageException("Unsupported backend for Line Int. Ray Caster: " + b);
+}
+
+template <class T>
+static std::shared_ptr<ImgSimMetric2D> sim_from(const std::string& b)
+{
+  if (b == "hip") return std::make_shared<T>();
+  throw StringMessageException("Unsupported backend for Sim Metric: " + b);
+}
+
+std::shared_ptr<ImgSimMetric2D> NCCSimMetricFromBackend(const std::string& b) { return sim_from<ImgSimMetric2DNCCHIP>(b); }
+std::shared_ptr<ImgSimMetric2D> GradNCCSimMetricFromBackend(const std::string& b)
+{
+  return sim_from<ImgSimMetric2DGradNCCHIP>(b);
+}
+std::shared_ptr<ImgSimMetric2D> PatchNCCSimMetricFromBackend(const std::string& b)
+{
+  return sim_from<ImgSimMetric2DPatchNCCHIP>(b);
+}
+std::shared_ptr<ImgSimMetric2D> PatchGradNCCSimMetricFromBackend(const std::string& b)
+{
+  return sim_from<ImgSimMetric2DPatchGradNCCHIP>(b);
+}
+
+// ------------------------------------------------------------------------------------------
+
+Intensity2D3DObjFn::Intensity2D3DObjFn(std::shared_ptr<RayCaster> rc, std::vector<std::shared_ptr<ImgSimMetric2D>> sims,
+                                       size_type projs_per_view)
+    : rc_(std::move(rc)), sims_(std::move(sims)), projs_per_view_(projs_per_view)
+{
+}
+
+void Intensity2D3DObjFn::setup()
+{
+  // Intensity2D3DRegi::setup (xregIntensity2D3DRegi.cpp:43-133), single volume
+  const size_type nv = sims_.size();
+  if (rc_->num_camera_models() != nv) throw StringMessageException("one camera model per view");
+  rc_->set_num_projs(projs_per_view_ * nv);
+  rc_->allocate_resources();
+  for (size_type v = 0; v < nv; ++v) {
+    sims_[v]->set_num_moving_images(projs_per_view_);
+    sims_[v]->set_mov_imgs_buf_from_ray_caster(rc_.get(), projs_per_view_ * v);  // view-major
+  }
+  for (size_type v = 0; v < nv; ++v) sims_[v]->allocate_resources();
+  combine_.set_num_sim_metrics(nv);
+  combine_.set_num_projs_per_sim_metric(projs_per_view_);
+  combine_.allocate_resources();
+  for (size_type v = 0; v < nv; ++v) combine_.set_sim_metric(v, sims_[v].get());
+  num_evals_ = 0;
+  setup_done_ = true;
+}
+
+std::vector<float> Intensity2D3DObjFn::operator()(const FrameTransformList& poses)
+{
+  // Intensity2D3DRegi::obj_fn (xregIntensity2D3DRegi.cpp:571-696)
+  if (!setup_done_) throw StringMessageException("setup() first");
+  if (poses.size() != projs_per_view_) throw StringMessageException("one pose per projection of a view");
+  rc_->use_proj_store_replace_method();
+  rc_->distribute_xforms_among_cam_models(poses);
+  rc_->compute(0);
+  for (auto& s : sims_) s->compute();
+  combine_.compute();
+  ++num_evals_;
+  return combine_.sim_vals();
+}
+
+}  // namespace xreg

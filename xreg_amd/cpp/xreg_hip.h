@@ -1,0 +1,399 @@
+// xreg_hip.h -- drop-in HIP subclasses of xReg's RayCaster / ImgSimMetric2D interfaces.
+//
+// The classes keep the method names, argument meaning, call order and error behaviour of
+//   RayCaster                 lib/ray_cast/xregRayCastInterface.h:43-434
+//   RayCastLineIntParamInterface  xregRayCastInterface.h:581-591
+//   ImgSimMetric2D            lib/regi/sim_metrics_2d/xregImgSimMetric2D.h:42-156
+//   ImgSimMetric2DGradImgParamInterface  xregImgSimMetric2DGradImgParamInterface.h:31-39
+//   ImgSimMetric2DPatchCommon (patch radius / stride)  xregImgSimMetric2DPatchCommon.h:67-127
+//   ImgSimMetric2DCombineMean xregImgSimMetric2DCombine.h:108
+// so that Intensity2D3DRegi and its drivers can run unchanged. xReg's ITK / Eigen / OpenCV
+// value types are replaced by small POD equivalents here (this build has none of those
+// libraries); in the xReg tree the same classes would take itk::Image / Eigen arguments and
+// forward their buffers to the same C-ABI calls (include/xrh.h). Device memory, kernels and
+// reductions live behind that C-ABI; this layer only does bookkeeping.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/xrh.h"
+
+namespace xreg {
+
+using size_type = std::size_t;
+using CoordScalar = float;
+
+// xregThrow -> StringMessageException (lib/common/xregExceptionUtils.h:96)
+struct StringMessageException : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// 4x4 affine, row-major, float (Eigen::Transform<float,3,Affine> in xReg, xregCommon.h:65)
+struct FrameTransform {
+  std::array<float, 16> m{};
+  static FrameTransform Identity();
+  FrameTransform operator*(const FrameTransform& o) const;  // Eigen Affine*Affine order
+  float& operator()(int r, int c) { return m[4 * r + c]; }
+  float operator()(int r, int c) const { return m[4 * r + c]; }
+};
+using FrameTransformList = std::vector<FrameTransform>;
+
+// CameraModel (lib/transforms/xregPerspectiveXform.h:108-273), float members
+struct CameraModel {
+  enum CameraCoordFrame { kORIGIN_AT_FOCAL_PT_DET_POS_Z = 0, kORIGIN_AT_FOCAL_PT_DET_NEG_Z, kORIGIN_ON_DETECTOR };
+  CameraCoordFrame coord_frame_type = kORIGIN_AT_FOCAL_PT_DET_NEG_Z;
+  float focal_len = 0;
+  size_type num_det_rows = 0, num_det_cols = 0;
+  float det_row_spacing = 0, det_col_spacing = 0;
+  std::array<float, 9> intrins{}, intrins_inv{};
+  FrameTransform extrins = FrameTransform::Identity(), extrins_inv = FrameTransform::Identity();
+  std::array<float, 3> pinhole_pt{};
+
+  // CameraModel::setup(f, nr, nc, rs, cs) (xregPerspectiveXform.cpp:230-254)
+  void setup(float focal_len, size_type nr, size_type nc, float rs, float cs);
+  // CameraModel::setup(intrins, extrins, nr, nc, rs, cs) (xregPerspectiveXform.cpp:300-330)
+  void setup(const std::array<float, 9>& intrins, const FrameTransform& extrins, size_type nr, size_type nc,
+             float rs, float cs);
+  xrh_camera to_xrh() const;
+};
+
+// itk::Image<float,3> stand-in: x-fastest buffer, spacing, origin, direction (row-major)
+struct Volume {
+  std::vector<float> buf;
+  std::array<size_type, 3> size{};
+  std::array<double, 3> spacing{1, 1, 1};
+  std::array<double, 3> origin{0, 0, 0};
+  std::array<double, 9> direction{1, 0, 0, 0, 1, 0, 0, 0, 1};
+  // ITKImagePhysicalPointTransformsAsEigen (lib/itk/xregITKBasicImageUtils.h:131-168)
+  FrameTransform idx_to_phys() const;
+};
+using VolPtr = std::shared_ptr<Volume>;
+using VolList = std::vector<VolPtr>;
+
+// itk::Image<float,2> stand-in for projections / fixed images: row-major rows x cols
+struct Image2D {
+  std::vector<float> buf;
+  size_type rows = 0, cols = 0;
+  float row_spacing = 1, col_spacing = 1;
+};
+using ImagePtr = std::shared_ptr<Image2D>;
+using ImageMaskPtr = std::shared_ptr<std::vector<unsigned char>>;
+
+// One device context per process and device, shared by all HIP classes (the OpenCL
+// context/queue pair of the OCL classes).
+class HIPContext {
+ public:
+  static HIPContext& get(int device = 0);
+  xrh_ctx* ctx() const { return ctx_; }
+  ~HIPContext();
+
+ private:
+  explicit HIPContext(int device);
+  xrh_ctx* ctx_ = nullptr;
+};
+
+void check_xrh(xrh_status s);  // non-zero status -> exception (Unsupported / StringMessage)
+
+// ------------------------------------------------------------------------------------------
+
+class RayCaster {
+ public:
+  using PixelScalar2D = float;
+  using CameraModelList = std::vector<CameraModel>;
+  using CamModelAssocList = std::vector<size_type>;
+  class UnsupportedOperationException {};
+  enum InterpMethod { kRAY_CAST_INTERP_LINEAR = 0, kRAY_CAST_INTERP_NN, kRAY_CAST_INTERP_SINC, kRAY_CAST_INTERP_BSPLINE };
+  enum ProjPixelStoreMethod { kRAY_CAST_PIXEL_REPLACE = 0, kRAY_CAST_PIXEL_ACCUM };
+
+  RayCaster() = default;
+  virtual ~RayCaster() = default;
+  RayCaster(const RayCaster&) = delete;
+  RayCaster& operator=(const RayCaster&) = delete;
+
+  void set_volume(VolPtr v);
+  void set_volumes(const VolList& vols);
+  size_type num_vols() const { return vols_.size(); }
+  size_type num_camera_models() const { return camera_models_.size(); }
+  void set_camera_model(const CameraModel& c);
+  void set_camera_models(const CameraModelList& c);
+  const CameraModelList& camera_models() const { return camera_models_; }
+  const CameraModel& camera_model(size_type i = 0) const { return camera_models_[i]; }
+  void set_proj_cam_model(size_type proj_idx, size_type cam_idx) { cam_model_for_proj_[proj_idx] = cam_idx; }
+  const CamModelAssocList& camera_model_proj_associations() const { return cam_model_for_proj_; }
+  void set_camera_model_proj_associations(const CamModelAssocList& a);
+  void distribute_xforms_among_cam_models(const FrameTransformList& xforms);
+  void distribute_xform_among_cam_models(const FrameTransform& xform);
+  void set_ray_step_size(const CoordScalar& s) { ray_step_size_ = s; }
+  CoordScalar ray_step_size() const { return ray_step_size_; }
+  virtual void set_num_projs(size_type n);
+  size_type num_projs() const { return num_projs_; }
+  void set_interp_method(const InterpMethod& m) { interp_method_ = m; }
+  InterpMethod interp_method() const { return interp_method_; }
+  void use_linear_interp() { interp_method_ = kRAY_CAST_INTERP_LINEAR; }
+  void use_nn_interp() { interp_method_ = kRAY_CAST_INTERP_NN; }
+  void set_xforms_cam_to_itk_phys(const FrameTransformList& xforms);
+  const FrameTransformList& xforms_cam_to_itk_phys() const { return xforms_cam_to_itk_phys_; }
+  FrameTransform& xform_cam_to_itk_phys(size_type i) { return xforms_cam_to_itk_phys_[i]; }
+  const FrameTransform& xform_cam_to_itk_phys(size_type i) const { return xforms_cam_to_itk_phys_[i]; }
+  FrameTransform xform_img_center_to_itk_phys(size_type vol_idx = 0) const;
+  FrameTransform xform_cam_wrt_carm_center_of_rot(size_type cam_idx = 0) const;
+  void post_multiply_all_xforms(const FrameTransform& x);
+  void pre_multiply_all_xforms(const FrameTransform& x);
+  virtual void allocate_resources();
+  virtual void compute(size_type vol_idx = 0) = 0;
+  virtual ImagePtr proj(size_type proj_idx) = 0;
+  virtual PixelScalar2D* raw_host_pixel_buf() = 0;
+  virtual void use_external_host_pixel_buf(void* buf) = 0;
+  virtual size_type max_num_projs_possible() const = 0;
+  void set_proj_store_method(ProjPixelStoreMethod m) { proj_store_meth_ = m; }
+  ProjPixelStoreMethod proj_store_method() const { return proj_store_meth_; }
+  void use_proj_store_replace_method() { proj_store_meth_ = kRAY_CAST_PIXEL_REPLACE; }
+  void use_proj_store_accum_method() { proj_store_meth_ = kRAY_CAST_PIXEL_ACCUM; }
+  virtual void use_other_proj_buf(RayCaster* other) = 0;
+  void set_use_bg_projs(bool b) { use_bg_projs_ = b; }
+  bool use_bg_projs() const { return use_bg_projs_; }
+  void set_bg_proj(ImagePtr& p, bool use = true);
+  void set_bg_projs(std::vector<ImagePtr>& ps, bool use = true);
+  size_type max_num_projs() const { return max_num_projs_; }
+  PixelScalar2D default_bg_pixel_val() const { return default_bg_pixel_val_; }
+  void set_default_bg_pixel_val(PixelScalar2D v) { default_bg_pixel_val_ = v; }
+
+  // Device buffer hand-off for HIP similarity metrics (the RayCastSyncOCLBuf role,
+  // lib/ray_cast/xregRayCastSyncBuf.h:35-192): the DRRs never leave HBM.
+  virtual const float* device_pixel_buf() const { throw UnsupportedOperationException(); }
+
+ protected:
+  virtual void vols_changed() {}
+  virtual void camera_models_changed() {}
+
+  VolList vols_;
+  CameraModelList camera_models_;
+  size_type num_projs_ = 0;
+  size_type max_num_projs_ = 0;
+  FrameTransformList xforms_cam_to_itk_phys_;
+  CoordScalar ray_step_size_ = 1;
+  InterpMethod interp_method_ = kRAY_CAST_INTERP_LINEAR;
+  CamModelAssocList cam_model_for_proj_;
+  PixelScalar2D default_bg_pixel_val_ = 0;
+  ProjPixelStoreMethod proj_store_meth_ = kRAY_CAST_PIXEL_REPLACE;
+  bool use_bg_projs_ = false;
+  bool bg_projs_updated_ = false;
+  std::vector<ImagePtr> bg_projs_for_each_cam_;
+  bool resources_allocated_ = false;
+};
+
+enum RayCastLineIntKernel { kRAY_CAST_LINE_INT_SUM_KERNEL = 0, kRAY_CAST_LINE_INT_MAX_KERNEL };
+
+class RayCastLineIntParamInterface {
+ public:
+  RayCastLineIntKernel kernel_id() const { return kernel_id_; }
+  void set_kernel_id(RayCastLineIntKernel k) { kernel_id_ = k; }
+
+ protected:
+  RayCastLineIntKernel kernel_id_ = kRAY_CAST_LINE_INT_SUM_KERNEL;
+};
+
+// RayCasterLineIntCPU / RayCasterLineIntOCL counterpart. geom selects the ray set-up
+// arithmetic (XRH_GEOM_FP32 matches RayCasterLineIntCPU; XRH_GEOM_FP64 is more exact).
+class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterface {
+ public:
+  explicit RayCasterLineIntHIP(int device = 0, int32_t geom = XRH_GEOM_FP32, int32_t storage = XRH_F32);
+  ~RayCasterLineIntHIP() override;
+  void set_num_projs(size_type n) override;
+  void allocate_resources() override;
+  void compute(size_type vol_idx = 0) override;
+  ImagePtr proj(size_type proj_idx) override;
+  PixelScalar2D* raw_host_pixel_buf() override;
+  void use_external_host_pixel_buf(void* buf) override;
+  size_type max_num_projs_possible() const override;
+  void use_other_proj_buf(RayCaster* other) override;
+  const float* device_pixel_buf() const override { return drr_dev_; }
+  // exact trilinear sample count of the next compute() (roofline accounting)
+  uint64_t sample_count(size_type vol_idx = 0);
+
+ protected:
+  void vols_changed() override;
+
+ private:
+  void sync_to_host();
+  std::vector<xrh_camera> xrh_cams() const;
+  xrh_ctx* ctx_;
+  int32_t geom_, storage_;
+  std::vector<xrh_volume*> dev_vols_;
+  float* drr_dev_ = nullptr;
+  float* bg_dev_ = nullptr;
+  size_type drr_capacity_ = 0;
+  std::vector<float> host_buf_;
+  float* ext_host_buf_ = nullptr;
+  bool host_stale_ = true;
+};
+
+// ------------------------------------------------------------------------------------------
+
+class ImgSimMetric2D {
+ public:
+  using Scalar = float;
+  using ScalarList = std::vector<Scalar>;
+  class UnsupportedOperationException {};
+  ImgSimMetric2D() = default;
+  virtual ~ImgSimMetric2D() = default;
+  ImgSimMetric2D(const ImgSimMetric2D&) = delete;
+  ImgSimMetric2D& operator=(const ImgSimMetric2D&) = delete;
+
+  void set_fixed_image(ImagePtr f) { fixed_img_ = f; }
+  ImagePtr fixed_image() { return fixed_img_; }
+  virtual void allocate_resources() { sim_vals_.resize(num_mov_imgs_); }
+  virtual void compute() = 0;
+  void set_num_moving_images(size_type n) { num_mov_imgs_ = n; }
+  size_type num_moving_images() const { return num_mov_imgs_; }
+  Scalar& sim_val(size_type i) { return sim_vals_[i]; }
+  const Scalar& sim_val(size_type i) const { return sim_vals_[i]; }
+  ScalarList& sim_vals() { return sim_vals_; }
+  const ScalarList& sim_vals() const { return sim_vals_; }
+  virtual void set_mov_imgs_buf_from_ray_caster(RayCaster* rc, size_type proj_offset = 0);
+  virtual void set_mov_imgs_host_buf(Scalar* buf, size_type proj_offset = 0) = 0;
+  void set_mask(ImageMaskPtr m);
+  ImageMaskPtr mask() { return mask_; }
+  void set_save_aux_info(bool s) { save_aux_info_ = s; }
+
+ protected:
+  ImagePtr fixed_img_;
+  size_type num_mov_imgs_ = 0;
+  ScalarList sim_vals_;
+  ImageMaskPtr mask_;
+  bool mask_updated_ = true;
+  bool save_aux_info_ = false;
+};
+
+// HIP base: binds the moving images either to a HIP ray caster's device buffer (no copy)
+// or to a host buffer (uploaded per compute, like RayCastSyncOCLBuf's host->device path).
+class ImgSimMetric2DHIP : public ImgSimMetric2D {
+ public:
+  explicit ImgSimMetric2DHIP(int32_t metric, int device = 0);
+  ~ImgSimMetric2DHIP() override;
+  void allocate_resources() override;
+  void compute() override;
+  void set_mov_imgs_buf_from_ray_caster(RayCaster* rc, size_type proj_offset = 0) override;
+  void set_mov_imgs_host_buf(Scalar* buf, size_type proj_offset = 0) override;
+  const xrh_fixed* fixed_handle() const { return fixed_; }
+
+ protected:
+  xrh_sim_params params_{};
+
+ private:
+  xrh_ctx* ctx_;
+  xrh_fixed* fixed_ = nullptr;
+  RayCaster* rc_ = nullptr;
+  size_type proj_off_ = 0;
+  Scalar* host_mov_ = nullptr;
+  float* mov_dev_ = nullptr;
+  size_type mov_dev_capacity_ = 0;
+};
+
+class ImgSimMetric2DNCCHIP : public ImgSimMetric2DHIP {
+ public:
+  explicit ImgSimMetric2DNCCHIP(int device = 0) : ImgSimMetric2DHIP(XRH_SIM_NCC, device) {}
+};
+
+class ImgSimMetric2DGradImgParamInterface {
+ public:
+  virtual ~ImgSimMetric2DGradImgParamInterface() = default;
+  virtual size_type smooth_img_before_sobel_kernel_radius() const = 0;
+  virtual void set_smooth_img_before_sobel_kernel_radius(size_type r) = 0;
+};
+
+class ImgSimMetric2DGradNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2DGradImgParamInterface {
+ public:
+  explicit ImgSimMetric2DGradNCCHIP(int device = 0) : ImgSimMetric2DHIP(XRH_SIM_GRAD_NCC, device)
+  {
+    params_.smooth_ksize = 5;  // xregImgSimMetric2DGradImgCPU.h:81
+  }
+  size_type smooth_img_before_sobel_kernel_radius() const override { return (size_type)params_.smooth_ksize; }
+  void set_smooth_img_before_sobel_kernel_radius(size_type r) override { params_.smooth_ksize = (int32_t)r; }
+};
+
+class ImgSimMetric2DPatchCommonHIP {
+ public:
+  virtual ~ImgSimMetric2DPatchCommonHIP() = default;
+  size_type patch_radius() const { return (size_type)pp_->patch_radius; }
+  void set_patch_radius(size_type r) { pp_->patch_radius = (int32_t)r; }
+  size_type patch_stride() const { return (size_type)pp_->patch_stride; }
+  void set_patch_stride(size_type s) { pp_->patch_stride = (int32_t)s; }
+
+ protected:
+  explicit ImgSimMetric2DPatchCommonHIP(xrh_sim_params* p) : pp_(p) {}
+  xrh_sim_params* pp_;
+};
+
+class ImgSimMetric2DPatchNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2DPatchCommonHIP {
+ public:
+  explicit ImgSimMetric2DPatchNCCHIP(int device = 0)
+      : ImgSimMetric2DHIP(XRH_SIM_PATCH_NCC, device), ImgSimMetric2DPatchCommonHIP(&params_) {}
+};
+
+class ImgSimMetric2DPatchGradNCCHIP : public ImgSimMetric2DHIP,
+                                      public ImgSimMetric2DGradImgParamInterface,
+                                      public ImgSimMetric2DPatchCommonHIP {
+ public:
+  explicit ImgSimMetric2DPatchGradNCCHIP(int device = 0)
+      : ImgSimMetric2DHIP(XRH_SIM_PATCH_GRAD_NCC, device), ImgSimMetric2DPatchCommonHIP(&params_)
+  {
+    params_.smooth_ksize = 5;
+  }
+  size_type smooth_img_before_sobel_kernel_radius() const override { return (size_type)params_.smooth_ksize; }
+  void set_smooth_img_before_sobel_kernel_radius(size_type r) override { params_.smooth_ksize = (int32_t)r; }
+};
+
+// ImgSimMetric2DCombineMean (lib/regi/sim_metrics_2d/xregImgSimMetric2DCombine.cpp:79-98)
+class ImgSimMetric2DCombineMean {
+ public:
+  void set_num_sim_metrics(size_type n) { num_sim_metrics_ = n; }
+  void set_num_projs_per_sim_metric(size_type n) { num_projs_per_sim_metric_ = n; }
+  void allocate_resources();
+  void set_sim_metric(size_type i, ImgSimMetric2D* s);
+  void compute();
+  const std::vector<float>& sim_vals() const { return sim_vals_; }
+  float sim_val(size_type i) const { return sim_vals_[i]; }
+
+ private:
+  size_type num_sim_metrics_ = 0, num_projs_per_sim_metric_ = 0;
+  std::vector<ImgSimMetric2D*> sim_objs_;
+  std::vector<float> sim_vals_;
+};
+
+// Factories with the "hip" backend (the registration point of xregRayCastProgOpts.cpp:39-70 and
+// xregImgSimMetric2DProgOpts.cpp:45-94). Unknown backends throw like xregThrow does there.
+std::shared_ptr<RayCaster> LineIntRayCasterFromBackend(const std::string& backend);
+std::shared_ptr<ImgSimMetric2D> NCCSimMetricFromBackend(const std::string& backend);
+std::shared_ptr<ImgSimMetric2D> GradNCCSimMetricFromBackend(const std::string& backend);
+std::shared_ptr<ImgSimMetric2D> PatchNCCSimMetricFromBackend(const std::string& backend);
+std::shared_ptr<ImgSimMetric2D> PatchGradNCCSimMetricFromBackend(const std::string& backend);
+
+// Intensity2D3DRegi's evaluation path (xregIntensity2D3DRegi.cpp:43-133 setup, 571-696 obj_fn)
+// for a single volume: num_projs = projs_per_view * views, view-major projections, one
+// similarity metric per view bound at offset projs_per_view * view, mean over views.
+class Intensity2D3DObjFn {
+ public:
+  Intensity2D3DObjFn(std::shared_ptr<RayCaster> rc, std::vector<std::shared_ptr<ImgSimMetric2D>> sims,
+                     size_type projs_per_view);
+  void setup();
+  // poses: projs_per_view cam->phys transforms; returns the combined similarity per pose
+  std::vector<float> operator()(const FrameTransformList& poses);
+  size_type num_obj_fn_evals() const { return num_evals_; }
+
+ private:
+  std::shared_ptr<RayCaster> rc_;
+  std::vector<std::shared_ptr<ImgSimMetric2D>> sims_;
+  size_type projs_per_view_;
+  ImgSimMetric2DCombineMean combine_;
+  size_type num_evals_ = 0;
+  bool setup_done_ = false;
+};
+
+}  // namespace xreg

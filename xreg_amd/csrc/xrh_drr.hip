@@ -231,11 +231,15 @@ __device__ inline Ray setup_ray_f64(const DrrItem& it, int col, int row)
 
 struct VolView {
   __amdgpu_buffer_rsrc_t rsrc;
-  uint32_t py;   // row pitch (elements)
-  uint32_t pz;   // slice pitch (elements)
+  uint32_t py;     // row pitch (elements)
+  uint32_t pz;     // slice pitch (elements)
+  uint32_t apron;  // 1 + py + pz: element index of voxel (0, 0, 0)
 };
 
-__device__ inline float lerp(float a, float b, float w) { return fmaf(b - a, w, a); }
+// a (1 - w) + b w as two fused multiply-adds: as cheap as a + (b - a) w but without its
+// cancellation when the result is small next to a (tissue -> air edges), so the FP32 sample
+// stays within about an ulp of ITK's double-precision lerp.
+__device__ inline float lerp(float a, float b, float w) { return fmaf(w, b, fmaf(-w, a, a)); }
 
 __device__ inline float tri(float v000, float v100, float v010, float v110, float v001,
                             float v101, float v011, float v111, float wx, float wy, float wz)
@@ -253,14 +257,24 @@ __device__ inline float tri(float v000, float v100, float v010, float v110, floa
 __device__ inline float h2f(uint32_t bits16) { return __half2float(__ushort_as_half((unsigned short)bits16)); }
 __device__ inline float bf2f(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
 
+// floor(x) converted to int in one instruction (V_CVT_FLR_I32_F32). Inline asm counts as
+// convergent, so the sample loops below are unrolled by hand.
+__device__ inline int cvt_floor(float x)
+{
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 template <int kFmt>
 __device__ inline float sample(const VolView& v, float x, float y, float z)
 {
-  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
-  const float wx = x - fx, wy = y - fy, wz = z - fz;
-  // +1: the apron occupies padded index 0
-  const uint32_t e = (uint32_t)((int)fx + 1) + v.py * (uint32_t)((int)fy + 1) +
-                     v.pz * (uint32_t)((int)fz + 1);
+  // x - floor(x) (V_FRACT_F32) and floor(x) as int: two instructions per axis
+  const float wx = __builtin_amdgcn_fractf(x), wy = __builtin_amdgcn_fractf(y), wz = __builtin_amdgcn_fractf(z);
+  const int ix = cvt_floor(x), iy = cvt_floor(y), iz = cvt_floor(z);
+  // element index of the base corner; ix, iy, iz >= -1 and the pitches < 2^23, so the
+  // signed 24-bit multiply-adds are exact; v.apron = 1 + py + pz shifts past the apron
+  const uint32_t e = (uint32_t)(__mul24(iz, (int)v.pz) + __mul24(iy, (int)v.py) + ix) + v.apron;
   if constexpr (kFmt == 0) {
     const int o = (int)(e * 4u);
     const int oy = (int)(v.py * 4u);
@@ -311,6 +325,14 @@ __device__ inline float sample(const VolView& v, float x, float y, float z)
     }
     return tri(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], wx, wy, wz);
   }
+}
+
+// AccumLineIntKernel / MaxLineIntKernel (xregRayCastLineIntCPU.cpp:45-75)
+template <int kOp>
+__device__ inline float line_op(float acc, float s)
+{
+  if constexpr (kOp == XRH_LINE_INT_SUM) return acc + s;
+  else return (acc < s) ? s : acc;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -430,24 +452,36 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
                                                   (int)it.vol_bytes, 0x00020000);
       vv.py = it.pitch_y;
       vv.pz = it.pitch_z;
+      vv.apron = 1u + it.pitch_y + it.pitch_z;
+      // two samples per iteration (loads of both issue before either is interpolated);
+      // samples are still accumulated one after the other, in ray order
+      int i = 0;
       if constexpr (kGeom == XRH_GEOM_FP32) {
         // positions advance by repeated float addition, as the reference does (:276)
         float px = ray.sx, py = ray.sy, pz = ray.sz;
-#pragma unroll 2
-        for (int i = 0; i <= ray.n; ++i) {
-          const float s = sample<kFmt>(vv, px, py, pz);
-          if constexpr (kOp == XRH_LINE_INT_SUM) acc = acc + s; else acc = (acc < s) ? s : acc;
-          px = px + ray.dx;
-          py = py + ray.dy;
-          pz = pz + ray.dz;
+        for (; i < ray.n; i += 2) {
+          const float qx = px + ray.dx, qy = py + ray.dy, qz = pz + ray.dz;
+          const float s0 = sample<kFmt>(vv, px, py, pz);
+          const float s1 = sample<kFmt>(vv, qx, qy, qz);
+          acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
+          px = qx + ray.dx;
+          py = qy + ray.dy;
+          pz = qz + ray.dz;
         }
+        if (i == ray.n) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
       } else {
-#pragma unroll 2
-        for (int i = 0; i <= ray.n; ++i) {
-          const float fi = (float)i;
-          const float s = sample<kFmt>(vv, fmaf(fi, ray.dx, ray.sx), fmaf(fi, ray.dy, ray.sy),
-                                       fmaf(fi, ray.dz, ray.sz));
-          if constexpr (kOp == XRH_LINE_INT_SUM) acc = acc + s; else acc = (acc < s) ? s : acc;
+        for (; i < ray.n; i += 2) {
+          const float f0 = (float)i, f1 = (float)(i + 1);
+          const float s0 = sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+                                        fmaf(f0, ray.dz, ray.sz));
+          const float s1 = sample<kFmt>(vv, fmaf(f1, ray.dx, ray.sx), fmaf(f1, ray.dy, ray.sy),
+                                        fmaf(f1, ray.dz, ray.sz));
+          acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
+        }
+        if (i == ray.n) {
+          const float f0 = (float)i;
+          acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+                                                  fmaf(f0, ray.dz, ray.sz)));
         }
       }
       acc = acc * it.step;

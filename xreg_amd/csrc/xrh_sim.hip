@@ -331,6 +331,7 @@ __global__ __launch_bounds__(1024) void image_stats_kernel(const float* __restri
                                                            int64_t n, double* __restrict__ out)
 {
   __shared__ double red[16];
+  __shared__ double redc[16];
   __shared__ double mean_sh;
   double s = 0.0, c = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -342,14 +343,20 @@ __global__ __launch_bounds__(1024) void image_stats_kernel(const float* __restri
   s = wave_sum(s);
   c = wave_sum(c);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if ((threadIdx.x & 63) == 0) red[w] = s;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = s;
+    redc[w] = c;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < nw; ++i) t += red[i];
+    double t = 0.0, tc = 0.0;
+    for (int i = 0; i < nw; ++i) {
+      t += red[i];
+      tc += redc[i];
+    }
     out[0] = t;
-    out[1] = c;
-    mean_sh = t / c;
+    out[1] = tc;
+    mean_sh = t / tc;
   }
   __syncthreads();
   double v = 0.0;

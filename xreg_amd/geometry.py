@@ -85,6 +85,30 @@ def camera(focal: float, rows: int, cols: int, row_sp: float, col_sp: float,
     return c
 
 
+def camera_ke(K: np.ndarray, E: np.ndarray, rows: int, cols: int, row_sp: float, col_sp: float) -> Camera:
+    """CameraModel::setup(intrins, extrins, nr, nc, rs, cs) (xregPerspectiveXform.cpp:300-330):
+    focal length from FocalLenFromIntrins (:186-190), inverse intrinsics, SE3Inv extrinsics,
+    pinhole = extrins_inv translation (NEG_Z frame)."""
+    K = np.asarray(K, dtype=F)
+    c = Camera()
+    c.num_det_rows = int(rows)
+    c.num_det_cols = int(cols)
+    c.coord_frame = FRAME_NEG_Z
+    f = F(F(abs(F(K[0, 0] * F(col_sp))) + abs(F(K[1, 1] * F(row_sp)))) / F(2))
+    c.focal_len = float(f)
+    kinv = _mat3_inverse_f32(K)
+    for i in range(9):
+        c.intrins_inv[i] = float(kinv.reshape(-1)[i])
+    einv = se3_inv_f32(np.asarray(E))
+    for i in range(16):
+        c.extrins_inv[i] = float(einv.reshape(-1)[i])
+    for i in range(3):
+        c.pinhole[i] = float(einv[i, 3])
+    c.det_row_spacing = float(F(row_sp))
+    c.det_col_spacing = float(F(col_sp))
+    return c
+
+
 def camera_arrays(c: Camera):
     """(kinv 3x3, einv 4x4, pinhole 3) float32 views of a Camera."""
     kinv = np.array(c.intrins_inv[:], dtype=F).reshape(3, 3)

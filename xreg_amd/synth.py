@@ -79,16 +79,26 @@ def random_poses(gt: np.ndarray, center, n: int, rot_deg: float, trans_mm: float
     return out
 
 
-def multiview_cams(focal: float, rows: int, cols: int, px: float, n_views: int, sep_deg: float):
-    """Views separated by sep_deg about the camera y axis (the volume's superior-inferior axis
-    under the AP orientation) around the isocenter, as camera extrinsics."""
+def multiview_extrinsics(focal: float, n_views: int, sep_deg: float):
+    """float32 extrinsics (shared camera-world frame -> view camera frame) of views separated by
+    sep_deg about the camera y axis through the isocentre (the volume's superior-inferior axis
+    under the AP orientation). View 0 is the identity."""
     iso = focal / 2.0
-    cams = []
+    out = []
     for v in range(n_views):
         R = G.rot_y(np.deg2rad(sep_deg * v))
-        # extrinsics map the shared camera-world frame into view v's camera frame
-        world_to_cam = G.rigid(None, (0, 0, -iso)) @ G.rigid(R.T) @ G.rigid(None, (0, 0, iso))
-        cams.append(G.camera(focal, rows, cols, px, px, extrins=None if v == 0 else world_to_cam))
+        E = G.rigid(None, (0, 0, -iso)) @ G.rigid(R.T) @ G.rigid(None, (0, 0, iso))
+        out.append(E.astype(np.float32))
+    return out
+
+
+def multiview_cams(focal: float, rows: int, cols: int, px: float, n_views: int, sep_deg: float):
+    """View 0 via CameraModel::setup(f, ...); views v > 0 via setup(intrins, extrins, ...) with
+    the naive intrinsics and multiview_extrinsics, as an xReg multi-view set-up builds them."""
+    cams = []
+    K = G.naive_intrins(focal, rows, cols, px, px)
+    for v, E in enumerate(multiview_extrinsics(focal, n_views, sep_deg)):
+        cams.append(G.camera(focal, rows, cols, px, px) if v == 0 else G.camera_ke(K, E, rows, cols, px, px))
     return cams
 
 
