@@ -169,13 +169,16 @@ xrh_status xrh_line_int(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* c
                         int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
                         int32_t geom, float* drr_dev);
 
-/* Exact number of trilinear samples the same call would take (sum over rays of n+1,
- * the S_pose of the roofline byte model). Synchronous. */
+/* Exact number of trilinear samples the reference algorithm takes for the same call (sum
+ * over rays of n+1, the S_pose of the roofline byte model) and, optionally, how many of them
+ * the kernel evaluates after empty-space skipping (the others lie where every voxel of their
+ * trilinear support is exactly zero). Synchronous. */
 xrh_status xrh_line_int_sample_count(xrh_ctx* ctx, const xrh_volume* vol,
                                      const xrh_camera* cams, int32_t num_cams,
                                      const float* xforms_cam_to_phys,
                                      const int64_t* cam_for_proj, int64_t num_projs,
-                                     float step, int32_t geom, uint64_t* samples_out);
+                                     float step, int32_t geom, uint64_t* samples_out,
+                                     uint64_t* evaluated_out);
 
 /* ---------------------------------------------------------------------------------- */
 /* fixed images + similarity: ImgSimMetric2D::set_fixed_image + allocate_resources
@@ -237,9 +240,11 @@ xrh_status xrh_batch_wait(xrh_batch* batch, float* sim_host_out);
  * with HIP events around the DRR kernel and around the whole submission (milliseconds). */
 xrh_status xrh_batch_last_timing(const xrh_batch* batch, float* drr_kernel_ms, float* total_ms);
 
-/* Exact trilinear sample count of the last evaluated batch (computed on request). */
+/* Exact trilinear sample count of a batch (as xrh_line_int_sample_count; evaluated_out may
+ * be NULL). */
 xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view,
-                                  const double* item_pose, int64_t n, uint64_t* samples_out);
+                                  const double* item_pose, int64_t n, uint64_t* samples_out,
+                                  uint64_t* evaluated_out);
 
 #ifdef __cplusplus
 }

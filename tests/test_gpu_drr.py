@@ -54,6 +54,50 @@ def test_drr_fp32_matches_cpu_reference(ctx, oracle, c1s, layout):
     assert err.max() <= DRR_REL_TOL
 
 
+@pytest.mark.parametrize("name", ["c2", "c3", "c4"])
+def test_drr_fp32_matches_cpu_reference_other_configs(ctx, oracle, name):
+    """More rays (other geometries) through the FP32 set-up: no step-count flips anywhere."""
+    from xreg_amd import synth
+
+    w = synth.config(name, n_poses=8, small=True)
+    cams = w.cams[:1]
+    print(f"case: {name}-small 8 poses", flush=True)
+    ref, s_ref = oracle.drr_f32(w.vol, w.idx_to_phys, cams, poses_f32(w.poses), step=w.step)
+    got = _gpu_drr(ctx, w, w.poses, cams=cams)
+    err = drr_rel_err(got, ref)
+    k = np.unravel_index(np.argmax(err), err.shape)
+    print(f"max rel err {err.max():.3e} at {k}: gpu {got[k]!r} cpu {ref[k]!r}")
+    assert err.max() <= DRR_REL_TOL
+
+
+@pytest.mark.parametrize("name,geom", [("c1", 0), ("c2", 0), ("c2", 1), ("c4", 1)])
+def test_empty_space_skipping_is_exact(ctx, name, geom, monkeypatch):
+    """DRRs with the empty-space map equal DRRs without it bit for bit, and the map removes
+    samples (the phantom sits in air)."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    w = synth.config(name, n_poses=6, small=True)
+    print(f"case: skipping {name}-small geom={geom}", flush=True)
+    poses = np.concatenate([w.gt[None], w.poses])
+    with_skip = _gpu_drr(ctx, w, poses, geom=geom, cams=w.cams[:1])
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    total, evaluated = X.line_int_sample_count(ctx, vol, w.cams[:1], poses_f32(poses),
+                                               np.zeros(poses.shape[0], np.int64), w.step, geom=geom,
+                                               evaluated=True)
+    vol.close()
+    monkeypatch.setenv("XRH_SKIP_EMPTY", "0")
+    without = _gpu_drr(ctx, w, poses, geom=geom, cams=w.cams[:1])
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    t2, e2 = X.line_int_sample_count(ctx, vol, w.cams[:1], poses_f32(poses), np.zeros(poses.shape[0], np.int64),
+                                     w.step, geom=geom, evaluated=True)
+    vol.close()
+    print(f"samples {total}, evaluated with skipping {evaluated} ({evaluated / total:.3f})")
+    assert np.array_equal(with_skip, without)
+    assert t2 == total and e2 == total
+    assert evaluated < 0.9 * total
+
+
 def test_drr_sample_count_matches_oracle(ctx, oracle, c1s):
     from xreg_amd import _xrh as X
 

@@ -104,7 +104,7 @@ SIGNATURES = {
     "xrh_line_int": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
                             _I32, _P]),
     "xrh_line_int_sample_count": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32,
-                                         _PU64]),
+                                         _PU64, _PU64]),
     "xrh_fixed_create": (_I32, [_P, _PF, _I32, _I32, C.POINTER(SimParams), _PU8, C.POINTER(_P)]),
     "xrh_fixed_destroy": (_I32, [_P]),
     "xrh_sim_compute": (_I32, [_P, _P, _P, _I64, _P, _PF]),
@@ -115,7 +115,7 @@ SIGNATURES = {
     "xrh_batch_launch": (_I32, [_P, _PI32, _PD, _I64]),
     "xrh_batch_wait": (_I32, [_P, _PF]),
     "xrh_batch_last_timing": (_I32, [_P, _PF, _PF]),
-    "xrh_batch_sample_count": (_I32, [_P, _PI32, _PD, _I64, _PU64]),
+    "xrh_batch_sample_count": (_I32, [_P, _PI32, _PD, _I64, _PU64, _PU64]),
 }
 
 
@@ -252,14 +252,16 @@ def line_int(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_for_proj: 
 
 
 def line_int_sample_count(ctx: Context, vol: Volume, cams, xforms, cam_for_proj, step: float,
-                          geom: int = XRH_GEOM_FP32) -> int:
+                          geom: int = XRH_GEOM_FP32, evaluated: bool = False):
+    """Samples of the reference algorithm (and, with evaluated=True, also the number the
+    kernel evaluates after empty-space skipping)."""
     x = np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16)
     c4p = np.ascontiguousarray(cam_for_proj, dtype=np.int64)
     ca = _cams_array(cams)
-    out = C.c_uint64(0)
+    out, ev = C.c_uint64(0), C.c_uint64(0)
     check(lib().xrh_line_int_sample_count(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64),
-                                          x.shape[0], float(step), int(geom), C.byref(out)))
-    return int(out.value)
+                                          x.shape[0], float(step), int(geom), C.byref(out), C.byref(ev)))
+    return (int(out.value), int(ev.value)) if evaluated else int(out.value)
 
 
 class Fixed:
@@ -328,13 +330,13 @@ class Batch:
         check(lib().xrh_batch_last_timing(self.h, C.byref(a), C.byref(t)))
         return float(a.value), float(t.value)
 
-    def sample_count(self, item_view: np.ndarray, item_pose: np.ndarray) -> int:
+    def sample_count(self, item_view: np.ndarray, item_pose: np.ndarray, evaluated: bool = False):
         iv = np.ascontiguousarray(item_view, dtype=np.int32)
         ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
-        out = C.c_uint64(0)
+        out, ev = C.c_uint64(0), C.c_uint64(0)
         check(lib().xrh_batch_sample_count(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0],
-                                           C.byref(out)))
-        return int(out.value)
+                                           C.byref(out), C.byref(ev)))
+        return (int(out.value), int(ev.value)) if evaluated else int(out.value)
 
     def close(self) -> None:
         if self.h:

@@ -87,9 +87,17 @@ def build_cpp(verbose: bool = False) -> Path:
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"g++ failed for dropin_driver.cpp:\n{res.stderr}")
+    fp = HERE.parent / "tests" / "cpp" / "fp_rounding_check.hip"
+    cmd = [HIPCC, *COMMON, str(fp), "-o", str(FP_CHECK)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for fp_rounding_check.hip:\n{res.stderr}")
     if verbose:
-        print(f"built {CPP_LIB} and {DRIVER}")
+        print(f"built {CPP_LIB}, {DRIVER} and {FP_CHECK}")
     return CPP_LIB
+
+
+FP_CHECK = BUILD / "fp_rounding_check"
 
 
 def clean() -> None:

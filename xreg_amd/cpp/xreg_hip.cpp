@@ -390,7 +390,7 @@ uint64_t RayCasterLineIntHIP::sample_count(size_type vol_idx)
   std::vector<int64_t> c4p(cam_model_for_proj_.begin(), cam_model_for_proj_.begin() + num_projs_);
   uint64_t s = 0;
   check_xrh(xrh_line_int_sample_count(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(),
-                                      c4p.data(), (int64_t)num_projs_, ray_step_size_, geom_, &s));
+                                      c4p.data(), (int64_t)num_projs_, ray_step_size_, geom_, &s, nullptr));
   return s;
 }
 

@@ -385,7 +385,7 @@ xrh_status xrh_batch_last_timing(const xrh_batch* batch, float* drr_kernel_ms, f
 }
 
 xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view, const double* item_pose,
-                                  int64_t n, uint64_t* samples_out)
+                                  int64_t n, uint64_t* samples_out, uint64_t* evaluated_out)
 {
   return guarded([&] {
     XRH_REQUIRE(batch && samples_out, "null argument");
@@ -397,15 +397,16 @@ xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view, co
     DrrLaunch D = stage(batch, item_view, item_pose, n, nullptr, groups);
     const BlobLayout L = blob_layout(batch->max_items);
     XRH_HIP_CHECK(hipMemcpyAsync(batch->blob_dev, batch->blob_host, L.off_sim, hipMemcpyHostToDevice, s));
-    uint64_t* cnt = (uint64_t*)batch->sims_dev;  // reuse: >= 8 bytes when max_items >= 2
-    XRH_REQUIRE(batch->max_items >= 2, "sample count needs max_items >= 2");
-    const uint64_t zero = 0;
-    XRH_HIP_CHECK(hipMemcpyAsync(cnt, &zero, 8, hipMemcpyHostToDevice, s));
+    uint64_t* cnt = (uint64_t*)ctx_scratch(batch->ctx, 16);
+    uint64_t both[2] = {0, 0};
+    XRH_HIP_CHECK(hipMemcpyAsync(cnt, both, 16, hipMemcpyHostToDevice, s));
     D.sample_counter = cnt;
     launch_drr_setup(D, s);
     launch_drr(D, s);
-    XRH_HIP_CHECK(hipMemcpyAsync(samples_out, cnt, 8, hipMemcpyDeviceToHost, s));
+    XRH_HIP_CHECK(hipMemcpyAsync(both, cnt, 16, hipMemcpyDeviceToHost, s));
     XRH_HIP_CHECK(hipStreamSynchronize(s));
+    *samples_out = both[0];
+    if (evaluated_out) *evaluated_out = both[1];
   });
 }
 

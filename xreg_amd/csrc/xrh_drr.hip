@@ -85,6 +85,13 @@ __device__ inline void ind_to_det_d(const CamDev& c, double col, double row, dou
   affine_pt_d(c.einv_d, r, out);
 }
 
+// IEEE-correctly rounded float division and square root, obtained from the double result:
+// for these operations rounding a 53-bit result to 24 bits equals rounding the exact value
+// (53 >= 2*24 + 2), so the FP32 ray set-up matches the host's SSE arithmetic bit for bit.
+__device__ inline float div_rn(float a, float b) { return (float)((double)a / (double)b); }
+__device__ inline double div_rn(double a, double b) { return a / b; }
+__device__ inline float sqrt_rn(float x) { return (float)sqrt((double)x); }
+
 // RayRectIntersect with limit_to_segment (lib/spatial/xregSpatialPrimitives.cpp:175-222)
 template <class T>
 __device__ inline bool ray_rect(const T mx[3], const T p0[3], const T d[3], T& t0, T& t1)
@@ -95,7 +102,7 @@ __device__ inline bool ray_rect(const T mx[3], const T p0[3], const T d[3], T& t
     const T dk = d[k];
     // the reference compares the float component against the double literal 1.0e-8
     if ((double)fabs(dk) > 1.0e-8) {
-      const T inv = T(1) / dk;
+      const T inv = div_rn(T(1), dk);
       T a = (T(0) - p0[k]) * inv;
       T b = (mx[k] - p0[k]) * inv;
       if (b < a) {
@@ -149,17 +156,17 @@ __device__ inline Ray setup_ray_f32(const DrrItem& it, int col, int row)
     ray.sx = ph[0] + t0 * d[0];
     ray.sy = ph[1] + t0 * d[1];
     ray.sz = ph[2] + t0 * d[2];
-    const float dlen = __fsqrt_rn((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    const float dlen = sqrt_rn((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
     const float L = (t1 - t0) * dlen;
     float v0 = det[0] - it.cam.pinhole[0];
     float v1 = det[1] - it.cam.pinhole[1];
     float v2 = det[2] - it.cam.pinhole[2];
     const float vz = (v0 * v0 + v1 * v1) + v2 * v2;
     if (vz > 0.0f) {
-      const float vn = __fsqrt_rn(vz);
-      v0 = __fdiv_rn(v0, vn);
-      v1 = __fdiv_rn(v1, vn);
-      v2 = __fdiv_rn(v2, vn);
+      const float vn = sqrt_rn(vz);
+      v0 = div_rn(v0, vn);
+      v1 = div_rn(v1, vn);
+      v2 = div_rn(v2, vn);
     }
     v0 = v0 * it.step;
     v1 = v1 * it.step;
@@ -167,9 +174,9 @@ __device__ inline Ray setup_ray_f32(const DrrItem& it, int col, int row)
     const float s0 = dot3(&it.M[0], v0, v1, v2);
     const float s1 = dot3(&it.M[4], v0, v1, v2);
     const float s2 = dot3(&it.M[8], v0, v1, v2);
-    const float s = __fsqrt_rn((s0 * s0 + s1 * s1) + s2 * s2);
-    ray.n = (int)__fdiv_rn(L, s);
-    const float scale = __fdiv_rn(s, dlen);
+    const float s = sqrt_rn((s0 * s0 + s1 * s1) + s2 * s2);
+    ray.n = (int)div_rn(L, s);
+    const float scale = div_rn(s, dlen);
     ray.dx = d[0] * scale;
     ray.dy = d[1] * scale;
     ray.dz = d[2] * scale;
@@ -327,6 +334,54 @@ __device__ inline float sample(const VolView& v, float x, float y, float z)
   }
 }
 
+// Empty-space skipping. Walks from sample i towards sample `end` (dir = +1 or -1) over samples
+// that lie in bricks whose one-voxel-dilated trilinear support holds only zero voxels, so
+// their interpolated value is exactly 0 and leaving them out of a SUM changes nothing. At a
+// brick at Chebyshev distance d >= 1 from the nearest occupied brick, the cube of bricks
+// within d-1 is empty: the walk jumps to the last sample inside that cube (at least one
+// sample per iteration, so at most |end - i| + 1 iterations). Positions here are start +
+// i*step in float; they differ from the accumulated ones by far less than the one-voxel
+// dilation. Returns the first sample (in walk order) that may be non-zero.
+__device__ inline int skip_empty(const DrrItem& it, const Ray& r, int i, int end, int dir)
+{
+  const float B = (float)(1 << it.skip_shift);
+  const float vx = dir > 0 ? r.dx : -r.dx, vy = dir > 0 ? r.dy : -r.dy, vz = dir > 0 ? r.dz : -r.dz;
+  const float rx = __builtin_amdgcn_rcpf(vx), ry = __builtin_amdgcn_rcpf(vy), rz = __builtin_amdgcn_rcpf(vz);
+  while (dir > 0 ? (i <= end) : (i >= end)) {
+    const float fi = (float)i;
+    const float px = fmaf(fi, r.dx, r.sx), py = fmaf(fi, r.dy, r.sy), pz = fmaf(fi, r.dz, r.sz);
+    const int bx = min(max(cvt_floor(px) >> it.skip_shift, 0), it.skip_gx - 1);
+    const int by = min(max(cvt_floor(py) >> it.skip_shift, 0), it.skip_gy - 1);
+    const int bz = min(max(cvt_floor(pz) >> it.skip_shift, 0), it.skip_gz - 1);
+    const int d = it.skip[((int64_t)bz * it.skip_gy + by) * it.skip_gx + bx];
+    if (d == 0) break;
+    const float dm1 = (float)(d - 1);
+    float s = 3.0e38f;
+    if (vx > 0.0f) s = fminf(s, (((float)bx + dm1 + 1.0f) * B - px) * rx);
+    else if (vx < 0.0f) s = fminf(s, (((float)bx - dm1) * B - px) * rx);
+    if (vy > 0.0f) s = fminf(s, (((float)by + dm1 + 1.0f) * B - py) * ry);
+    else if (vy < 0.0f) s = fminf(s, (((float)by - dm1) * B - py) * ry);
+    if (vz > 0.0f) s = fminf(s, (((float)bz + dm1 + 1.0f) * B - pz) * rz);
+    else if (vz < 0.0f) s = fminf(s, (((float)bz - dm1) * B - pz) * rz);
+    const int jump = (s >= 1.0f) ? (int)fminf(s, 1.0e9f) : 1;
+    i += dir * jump;
+  }
+  return i;
+}
+
+// Sample range [first, last] that can hold non-zero values (SUM with an empty-space map), or
+// the whole ray [0, n].
+template <int kOp>
+__device__ inline void sample_range(const DrrItem& it, const Ray& r, int& first, int& last)
+{
+  first = 0;
+  last = r.n;
+  if (kOp == XRH_LINE_INT_SUM && it.skip) {
+    first = skip_empty(it, r, 0, r.n, 1);
+    last = (first <= r.n) ? skip_empty(it, r, r.n, first, -1) : first - 1;
+  }
+}
+
 // AccumLineIntKernel / MaxLineIntKernel (xregRayCastLineIntCPU.cpp:45-75)
 template <int kOp>
 __device__ inline float line_op(float acc, float s)
@@ -426,7 +481,11 @@ __global__ void drr_setup_kernel(const PoseIn* __restrict__ poses, const VolDesc
   it.ncc_zf = p.ncc_zf;
   it.tiles_x = (it.cam.cols + kTile - 1) / kTile;
   it.tiles_y = (it.cam.rows + kTile - 1) / kTile;
-  it.pad_[0] = it.pad_[1] = 0;
+  it.skip = v.skip;
+  it.skip_gx = v.skip_gx;
+  it.skip_gy = v.skip_gy;
+  it.skip_gz = v.skip_gz;
+  it.skip_shift = v.skip_shift;
   items[i] = it;
 }
 
@@ -455,11 +514,20 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
       vv.apron = 1u + it.pitch_y + it.pitch_z;
       // two samples per iteration (loads of both issue before either is interpolated);
       // samples are still accumulated one after the other, in ray order
-      int i = 0;
+      int i, last;
+      sample_range<kOp>(it, ray, i, last);
       if constexpr (kGeom == XRH_GEOM_FP32) {
-        // positions advance by repeated float addition, as the reference does (:276)
+        // positions advance by repeated float addition, as the reference does (:276),
+        // including over the leading samples that are skipped
         float px = ray.sx, py = ray.sy, pz = ray.sz;
-        for (; i < ray.n; i += 2) {
+        if (i <= last) {
+          for (int k = 0; k < i; ++k) {
+            px = px + ray.dx;
+            py = py + ray.dy;
+            pz = pz + ray.dz;
+          }
+        }
+        for (; i < last; i += 2) {
           const float qx = px + ray.dx, qy = py + ray.dy, qz = pz + ray.dz;
           const float s0 = sample<kFmt>(vv, px, py, pz);
           const float s1 = sample<kFmt>(vv, qx, qy, qz);
@@ -468,9 +536,9 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
           py = qy + ray.dy;
           pz = qz + ray.dz;
         }
-        if (i == ray.n) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
+        if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
       } else {
-        for (; i < ray.n; i += 2) {
+        for (; i < last; i += 2) {
           const float f0 = (float)i, f1 = (float)(i + 1);
           const float s0 = sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
                                         fmaf(f0, ray.dz, ray.sz));
@@ -478,7 +546,7 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
                                         fmaf(f1, ray.dz, ray.sz));
           acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
         }
-        if (i == ray.n) {
+        if (i == last) {
           const float f0 = (float)i;
           acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
                                                   fmaf(f0, ray.dz, ray.sz)));
@@ -536,13 +604,22 @@ __global__ __launch_bounds__(kThreads) void drr_count_kernel(
   const DrrItem& it = items[tr.item];
   int col, row;
   lane_pixel(tr.tx, tr.ty, col, row);
-  double cnt = 0.0;
+  double cnt = 0.0, evaluated = 0.0;
   if ((col < it.cam.cols) && (row < it.cam.rows)) {
     const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
     cnt = (double)(ray.n + 1);
+    if (ray.n >= 0) {
+      int first, last;
+      sample_range<XRH_LINE_INT_SUM>(it, ray, first, last);
+      evaluated = (double)(last - first + 1);
+    }
   }
   cnt = wave_sum_d(cnt);
-  if ((threadIdx.x & 63) == 0) atomicAdd(counter, (unsigned long long)cnt);
+  evaluated = wave_sum_d(evaluated);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&counter[0], (unsigned long long)cnt);
+    atomicAdd(&counter[1], (unsigned long long)evaluated);
+  }
 }
 
 template <int kGeom, int kOp>

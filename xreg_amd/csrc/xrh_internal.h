@@ -100,7 +100,8 @@ struct DrrItem {
   const float* ncc_zf;  // zero-mean fixed image for the fused NCC epilogue, or null
   int32_t tiles_x;
   int32_t tiles_y;
-  int32_t pad_[2];
+  const uint8_t* skip;  // empty-space map (see VolDesc), or null
+  int32_t skip_gx, skip_gy, skip_gz, skip_shift;
 };
 
 // Host-side pose record uploaded per projection before the set-up kernel runs.
@@ -125,6 +126,11 @@ struct VolDesc {
   int32_t layout;
   float min_spacing;  // smallest voxel edge (physical units), bounds the ray step from below
   int32_t pad_;
+  // empty-space map: Chebyshev distance (in bricks of 2^skip_shift voxels, capped at 255)
+  // from each brick to the nearest brick holding a non-zero voxel within one voxel of it;
+  // null when skipping is disabled
+  const uint8_t* skip;
+  int32_t skip_gx, skip_gy, skip_gz, skip_shift;
 };
 
 // Rejects steps so small that a ray would take millions of samples (keeps kernels short).
@@ -159,6 +165,7 @@ struct xrh_volume {
   xrh_ctx* ctx = nullptr;
   xrh::VolDesc desc{};
   void* data = nullptr;
+  void* skip = nullptr;
   uint64_t bytes = 0;
   int64_t size[3] = {0, 0, 0};
 };
@@ -205,7 +212,8 @@ struct DrrLaunch {
   // NCC epilogue (optional): items with a non-null ncc_zf write FP64 partial sums
   double* ncc_partials;     // [num_items][max tiles per item][4], or null
   int32_t ncc_tiles_per_item;
-  uint64_t* sample_counter; // optional: count samples instead of writing pixels
+  uint64_t* sample_counter; // optional: count samples instead of writing pixels; [0] = all
+                            // samples of the reference algorithm, [1] = samples evaluated
 };
 void launch_drr_setup(const DrrLaunch& L, hipStream_t s);
 void launch_drr(const DrrLaunch& L, hipStream_t s);

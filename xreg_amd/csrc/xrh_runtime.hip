@@ -215,6 +215,28 @@ __global__ void build_layout_kernel(const float* __restrict__ src, int X, int Y,
   }
 }
 
+// One thread per brick of B^3 voxels: 1 if any voxel within one voxel of the brick's
+// trilinear support [b*B - 1, b*B + B + 1] (clamped to the volume) is non-zero.
+__global__ void brick_occupancy_kernel(const float* __restrict__ src, int X, int Y, int Z, int B, int gx, int gy,
+                                       int gz, uint8_t* __restrict__ occ)
+{
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= (int64_t)gx * gy * gz) return;
+  const int bx = (int)(b % gx), by = (int)((b / gx) % gy), bz = (int)(b / ((int64_t)gx * gy));
+  const int x0 = max(bx * B - 1, 0), x1 = min(bx * B + B + 1, X - 1);
+  const int y0 = max(by * B - 1, 0), y1 = min(by * B + B + 1, Y - 1);
+  const int z0 = max(bz * B - 1, 0), z1 = min(bz * B + B + 1, Z - 1);
+  uint8_t any = 0;
+  for (int z = z0; z <= z1 && !any; ++z)
+    for (int y = y0; y <= y1 && !any; ++y)
+      for (int x = x0; x <= x1; ++x)
+        if (src[(int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * z)] != 0.0f) {
+          any = 1;
+          break;
+        }
+  occ[b] = any;
+}
+
 __global__ void copy_bg_kernel(const float* __restrict__ bg, const int64_t* __restrict__ cam_for_proj,
                                int64_t npx, int64_t num_projs, float* __restrict__ out)
 {
@@ -225,6 +247,43 @@ __global__ void copy_bg_kernel(const float* __restrict__ bg, const int64_t* __re
 }
 
 }  // namespace
+
+// Exact Chebyshev (chessboard) distance to the nearest occupied brick: two raster passes of
+// the 26-neighbour chamfer mask with unit weights. Values are capped at 255.
+static void chebyshev_distance(std::vector<uint8_t>& d, int gx, int gy, int gz)
+{
+  auto at = [&](int x, int y, int z) -> uint8_t& { return d[(size_t)x + (size_t)gx * ((size_t)y + (size_t)gy * z)]; };
+  for (auto& v : d) v = v ? 0 : 255;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int s = pass == 0 ? 1 : -1;
+    for (int zi = 0; zi < gz; ++zi) {
+      const int z = pass == 0 ? zi : gz - 1 - zi;
+      for (int yi = 0; yi < gy; ++yi) {
+        const int y = pass == 0 ? yi : gy - 1 - yi;
+        for (int xi = 0; xi < gx; ++xi) {
+          const int x = pass == 0 ? xi : gx - 1 - xi;
+          int best = at(x, y, z);
+          // neighbours already visited in this pass: earlier in raster order
+          for (int dz = -1; dz <= 0; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+              for (int dx = -1; dx <= 1; ++dx) {
+                if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
+                const int nx = x + s * dx, ny = y + s * dy, nz = z + s * dz;
+                if (nx < 0 || ny < 0 || nz < 0 || nx >= gx || ny >= gy || nz >= gz) continue;
+                best = std::min(best, (int)at(nx, ny, nz) + 1);
+              }
+          at(x, y, z) = (uint8_t)std::min(best, 255);
+        }
+      }
+    }
+  }
+}
+
+static bool skip_enabled()
+{
+  const char* env = std::getenv("XRH_SKIP_EMPTY");
+  return !(env && std::strcmp(env, "0") == 0);
+}
 
 static int default_layout(int storage)
 {
@@ -255,6 +314,8 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
   v->size[1] = Y;
   v->size[2] = Z;
   void* raw = nullptr;
+  int skip_dims[3] = {0, 0, 0};
+  constexpr int kSkipShift = 3;  // 8^3-voxel bricks
   try {
     v->data = ctx_alloc(ctx, bytes);
     v->bytes = bytes;
@@ -265,17 +326,41 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
     hipLaunchKernelGGL(build_layout_kernel, dim3((unsigned)blocks), dim3(threads), 0, ctx->stream,
                        (const float*)raw, (int)X, (int)Y, (int)Z, v->data, storage, layout);
     XRH_HIP_CHECK(hipGetLastError());
+    if (skip_enabled()) {
+      const int B = 1 << kSkipShift;
+      skip_dims[0] = (int)((X + B - 1) / B);
+      skip_dims[1] = (int)((Y + B - 1) / B);
+      skip_dims[2] = (int)((Z + B - 1) / B);
+      const int64_t nb = (int64_t)skip_dims[0] * skip_dims[1] * skip_dims[2];
+      v->skip = ctx_alloc(ctx, (uint64_t)nb);
+      hipLaunchKernelGGL(brick_occupancy_kernel, dim3((unsigned)((nb + 127) / 128)), dim3(128), 0, ctx->stream,
+                         (const float*)raw, (int)X, (int)Y, (int)Z, B, skip_dims[0], skip_dims[1], skip_dims[2],
+                         (uint8_t*)v->skip);
+      XRH_HIP_CHECK(hipGetLastError());
+      std::vector<uint8_t> occ((size_t)nb);
+      XRH_HIP_CHECK(hipMemcpyAsync(occ.data(), v->skip, (size_t)nb, hipMemcpyDeviceToHost, ctx->stream));
+      XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      chebyshev_distance(occ, skip_dims[0], skip_dims[1], skip_dims[2]);
+      XRH_HIP_CHECK(hipMemcpyAsync(v->skip, occ.data(), (size_t)nb, hipMemcpyHostToDevice, ctx->stream));
+    }
     XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx_free(ctx, raw);
     raw = nullptr;
   } catch (...) {
+    (void)hipStreamSynchronize(ctx->stream);
     if (raw) ctx_free(ctx, raw);
     if (v->data) ctx_free(ctx, v->data);
+    if (v->skip) ctx_free(ctx, v->skip);
     delete v;
     throw;
   }
   VolDesc& d = v->desc;
   std::memset(&d, 0, sizeof(d));
+  d.skip = (const uint8_t*)v->skip;
+  d.skip_gx = skip_dims[0];
+  d.skip_gy = skip_dims[1];
+  d.skip_gz = skip_dims[2];
+  d.skip_shift = kSkipShift;
   affine_inverse_f32(idx_to_phys, d.p2i);
   affine_inverse_f64(idx_to_phys, d.p2i_d);
   d.data = v->data;
@@ -337,7 +422,7 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   const uint64_t off_band = align(off_vol + sizeof(VolDesc));
   const uint64_t off_c4p = align(off_band + sizeof(uint32_t) * plan.band_start.size());
   const uint64_t off_cnt = align(off_c4p + sizeof(int64_t) * num_projs);
-  const uint64_t blob = align(off_cnt + 8);
+  const uint64_t blob = align(off_cnt + 16);
   const uint64_t off_items = blob;
   const uint64_t total = align(off_items + sizeof(DrrItem) * num_projs);
 
@@ -359,7 +444,7 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   std::memcpy(host + off_vol, &vol->desc, sizeof(VolDesc));
   std::memcpy(host + off_band, plan.band_start.data(), sizeof(uint32_t) * plan.band_start.size());
   std::memcpy(host + off_c4p, cam_for_proj, sizeof(int64_t) * num_projs);
-  std::memset(host + off_cnt, 0, 8);
+  std::memset(host + off_cnt, 0, 16);
   XRH_HIP_CHECK(hipMemcpyAsync(dev, host, blob, hipMemcpyHostToDevice, ctx->stream));
 
   if (!samples_out) {
@@ -392,7 +477,7 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   launch_drr_setup(L, ctx->stream);
   launch_drr(L, ctx->stream);
   if (samples_out) {
-    XRH_HIP_CHECK(hipMemcpyAsync(samples_out, dev + off_cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+    XRH_HIP_CHECK(hipMemcpyAsync(samples_out, dev + off_cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
     XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   }
 }
@@ -513,6 +598,7 @@ xrh_status xrh_volume_destroy(xrh_volume* vol)
     if (!vol) return;
     XRH_HIP_CHECK(hipStreamSynchronize(vol->ctx->stream));
     ctx_free(vol->ctx, vol->data);
+    if (vol->skip) ctx_free(vol->ctx, vol->skip);
     delete vol;
   });
 }
@@ -535,12 +621,16 @@ xrh_status xrh_line_int_sample_count(xrh_ctx* ctx, const xrh_volume* vol,
                                      const xrh_camera* cams, int32_t num_cams,
                                      const float* xforms_cam_to_phys,
                                      const int64_t* cam_for_proj, int64_t num_projs,
-                                     float step, int32_t geom, uint64_t* samples_out)
+                                     float step, int32_t geom, uint64_t* samples_out,
+                                     uint64_t* evaluated_out)
 {
   return guarded([&] {
     XRH_REQUIRE(samples_out, "null output");
+    uint64_t both[2] = {0, 0};
     line_int_launch(ctx, vol, cams, num_cams, xforms_cam_to_phys, cam_for_proj, num_projs, step,
-                    XRH_LINE_INT_SUM, XRH_STORE_REPLACE, nullptr, 0.0f, geom, nullptr, samples_out);
+                    XRH_LINE_INT_SUM, XRH_STORE_REPLACE, nullptr, 0.0f, geom, nullptr, both);
+    *samples_out = both[0];
+    if (evaluated_out) *evaluated_out = both[1];
   });
 }
 
