@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--geom", default="fp64", choices=["fp32", "fp64"])
     ap.add_argument("--smooth", type=int, default=5)
+    ap.add_argument("--no-skip", action="store_true", help="evaluate every sample (no empty-space map)")
     return ap.parse_args()
 
 
@@ -87,6 +88,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.no_skip:
+        os.environ["XRH_SKIP_EMPTY"] = "0"
     from xreg_amd import _xrh as X
     from xreg_amd import synth
 
@@ -128,8 +131,10 @@ def main():
         drr_ms.append(batch.last_timing()[0])
     batch.set_options(use_graph=True, time_kernels=False)
     drr_ms_avg = float(np.mean(drr_ms[1:])) if len(drr_ms) > 1 else float(drr_ms[0])
-    samples = [batch.sample_count(iv, gens[g]) for g in range(min(5, len(gens)))]
-    S = float(np.mean(samples[1:])) if len(samples) > 1 else float(samples[0])
+    # the same generations as the timed DRR launches above: algorithmic and evaluated samples
+    counts = [batch.sample_count(iv, gens[g], evaluated=True) for g in range(min(5, len(gens)))]
+    S = float(np.mean([c[0] for c in counts[1:]])) if len(counts) > 1 else float(counts[0][0])
+    S_eval = float(np.mean([c[1] for c in counts[1:]])) if len(counts) > 1 else float(counts[0][1])
     bv = {X.XRH_F32: 4, X.XRH_F16: 2, X.XRH_BF16: 2}[storage]
     bytes_launch = 8.0 * bv * S + 4.0 * P * lam  # gathers + DRR write
     achieved = bytes_launch / (drr_ms_avg * 1e-3) / 1e9
@@ -178,7 +183,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4),
-                     "samples_per_launch": int(S), "algorithmic_bytes_per_launch": int(bytes_launch)},
+                     "samples_per_launch": int(S), "samples_evaluated_per_launch": int(S_eval),
+                     "algorithmic_bytes_per_launch": int(bytes_launch),
+                     "evaluated_gbs": round((8.0 * bv * S_eval + 4.0 * P * lam) / (drr_ms_avg * 1e-3) / 1e9, 1)},
+        "empty_space_skipping": not args.no_skip,
         "cpu_baseline": cpu,
         "sim_checksum": float(np.sum(sims)),
     }

@@ -219,6 +219,15 @@ xrh_status xrh_batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_vie
                             int64_t max_items, int32_t geom, xrh_batch** out);
 xrh_status xrh_batch_destroy(xrh_batch* batch);
 
+/* Serving form: a batch sized for up to max_views views, max_items items and detectors up to
+ * max_rows x max_cols, whose view table is (re)set with xrh_batch_set_views between
+ * evaluations (e.g. the registrations that are ready at a scheduling step). Replacing the
+ * views drops cached graphs; device memory is allocated once, at creation. */
+xrh_status xrh_batch_create_empty(xrh_ctx* ctx, int32_t max_views, int64_t max_items,
+                                  int32_t max_rows, int32_t max_cols, int32_t geom,
+                                  xrh_batch** out);
+xrh_status xrh_batch_set_views(xrh_batch* batch, const xrh_view* views, int32_t num_views);
+
 /* use_graph: capture / replay hipGraphs per item->view signature (default 1).
  * time_kernels: submit eagerly with HIP events around the DRR kernel and the whole
  * submission, readable through xrh_batch_last_timing (default 0). */

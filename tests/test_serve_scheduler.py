@@ -1,0 +1,116 @@
+"""Scheduler logic of the registration server on the CPU, with a host-side fake device whose
+objective is a closed-form function of the pose (no GPU): admission under the HBM budget,
+packing whole generations under the item capacity, completion of every job, and identical
+results alone and in the mix."""
+import types
+
+import numpy as np
+import pytest
+
+from xreg_amd import geometry as G
+from xreg_amd import serve
+
+
+class FakeCtx:
+    def __init__(self):
+        self.used = 0
+
+    def bytes_in_use(self):
+        return self.used
+
+
+class FakeFixed:
+    def __init__(self, ctx, img, metric, smooth_ksize=5, patch_radius=5):
+        self.ctx, self.nbytes = ctx, img.size * 4 * 6
+        self.target = img  # the fake objective's target pose lives here
+        ctx.used += self.nbytes
+
+    def close(self):
+        self.ctx.used -= self.nbytes
+
+
+class FakeBatch:
+    steps = []
+
+    def __init__(self, ctx, views, max_items, geom=0, capacity=None):
+        self.max_items = max_items
+        self.views = None
+
+    def set_options(self, **kw):
+        pass
+
+    def set_views(self, views):
+        self.views = list(views)
+
+    def eval(self, item_view, item_pose):
+        assert len(item_view) <= self.max_items
+        FakeBatch.steps.append(len(item_view))
+        out = np.empty(len(item_view), np.float32)
+        for i, (v, p) in enumerate(zip(item_view, item_pose)):
+            t = self.views[v].fixed.target
+            out[i] = np.float32(np.sum((p[:3, 3] - t[:3, 3]) ** 2) + 100.0 * np.sum((p[:3, :3] - t[:3, :3]) ** 2))
+        return out
+
+
+FAKE = types.SimpleNamespace(Batch=FakeBatch, Fixed=FakeFixed,
+                             make_view=lambda vol, fx, cam, step: types.SimpleNamespace(fixed=fx, cam=cam))
+
+
+def _specs(n, lam=10, views=1, gens=15):
+    cam = G.camera(1020.0, 48, 48, 3.0, 3.0)
+    center = np.array([64.0, 64.0, 50.0])
+    gt = G.gt_pose(center, 510.0)
+    rng = np.random.default_rng(0)
+    out = []
+    for j in range(n):
+        init = G.perturbed_pose(gt, center, np.deg2rad(rng.uniform(-4, 4, 3)), rng.uniform(-5, 5, 3))
+        out.append(serve.JobSpec(job_id=j, arrival=0.0, volume="v", cams=[cam] * views,
+                                 fixed=[np.asarray(gt, np.float32)] * views, init_pose=init, center=center,
+                                 step=1.0, lam=lam + j % 3, max_gens=gens, seed=j))
+    return out
+
+
+def test_all_jobs_complete_and_converge():
+    ctx = FakeCtx()
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=64, max_views=8, max_det=64,
+                                   device=FAKE)
+    specs = _specs(6, gens=60)
+    res = srv.run(specs, realtime=False)
+    assert sorted(r.job_id for r in res) == list(range(6))
+    for r in res:
+        assert r.best_sim < r.history[0]
+    assert ctx.used == 0  # every job released its fixed images
+    assert srv.max_items_in_step <= 64
+
+
+def test_budget_limits_concurrency():
+    ctx = FakeCtx()
+    specs = _specs(5, lam=10, gens=3)
+    per_job = specs[0].fixed[0].size * 4 * 6
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=2 * per_job, max_items=1000, max_views=8,
+                                   max_det=64, device=FAKE)
+    assert srv._fixed_bytes(specs[0]) == per_job
+    FakeBatch.steps = []
+    res = srv.run(specs, realtime=False)
+    assert len(res) == 5
+    # never more than two jobs (<= 2 * 12 items) in one submission
+    assert max(FakeBatch.steps) <= 24
+
+
+def test_alone_equals_mixed_scheduler():
+    ctx = FakeCtx()
+    specs = _specs(4, views=2, gens=8)
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=40, max_views=8, max_det=64,
+                                   device=FAKE)
+    alone = {s.job_id: srv.run([s], realtime=False)[0] for s in specs}
+    mixed = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    for j in alone:
+        assert alone[j].history == mixed[j].history
+
+
+def test_oversized_job_is_rejected():
+    ctx = FakeCtx()
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=8, max_views=8, max_det=64,
+                                   device=FAKE)
+    with pytest.raises(ValueError):
+        srv.run(_specs(1, lam=20), realtime=False)

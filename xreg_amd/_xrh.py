@@ -110,6 +110,8 @@ SIGNATURES = {
     "xrh_sim_compute": (_I32, [_P, _P, _P, _I64, _P, _PF]),
     "xrh_batch_create": (_I32, [_P, C.POINTER(View), _I32, _I64, _I32, C.POINTER(_P)]),
     "xrh_batch_destroy": (_I32, [_P]),
+    "xrh_batch_create_empty": (_I32, [_P, _I32, _I64, _I32, _I32, _I32, C.POINTER(_P)]),
+    "xrh_batch_set_views": (_I32, [_P, C.POINTER(View), _I32]),
     "xrh_batch_set_options": (_I32, [_P, _I32, _I32]),
     "xrh_batch_eval": (_I32, [_P, _PI32, _PD, _I64, _PF, _P]),
     "xrh_batch_launch": (_I32, [_P, _PI32, _PD, _I64]),
@@ -294,14 +296,33 @@ class Fixed:
 
 
 class Batch:
-    def __init__(self, ctx: Context, views, max_items: int, geom: int = XRH_GEOM_FP32):
+    def __init__(self, ctx: Context, views, max_items: int, geom: int = XRH_GEOM_FP32,
+                 capacity: tuple[int, int, int] | None = None):
+        """views: initial view table. capacity=(max_views, max_rows, max_cols) creates an
+        empty, resizable batch instead (set_views before evaluating)."""
+        p = _P()
+        self.ctx = ctx
+        if capacity is not None:
+            mv, mr, mc = capacity
+            check(lib().xrh_batch_create_empty(ctx.h, int(mv), int(max_items), int(mr), int(mc), int(geom),
+                                               C.byref(p)))
+            self.h = p.value
+            self._views = None
+            if views:
+                self.set_views(views)
+            return
         arr = (View * len(views))()
         for i, v in enumerate(views):
             arr[i] = v
-        p = _P()
         check(lib().xrh_batch_create(ctx.h, arr, len(views), int(max_items), int(geom), C.byref(p)))
-        self.ctx = ctx
         self.h = p.value
+        self._views = arr
+
+    def set_views(self, views) -> None:
+        arr = (View * len(views))()
+        for i, v in enumerate(views):
+            arr[i] = v
+        check(lib().xrh_batch_set_views(self.h, arr, len(views)))
         self._views = arr
 
     def set_options(self, use_graph: bool = True, time_kernels: bool = False) -> None:

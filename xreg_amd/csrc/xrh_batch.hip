@@ -6,12 +6,15 @@
 // kernels grouped by metric, the FP64 finish, one download of the similarity values. The
 // whole sequence is captured in a hipGraph the first time an item->view signature is seen
 // and replayed afterwards (poses change between generations, shapes do not).
+//
+// A batch is sized once (views, items, detector rows x cols) and its view table can be
+// replaced between evaluations: a serving loop merges whichever registrations are ready into
+// one submission without reallocating device memory.
 #include "xrh_internal.h"
 #include "xrh_sim.h"
 
 #include <algorithm>
 #include <cstring>
-#include <functional>
 #include <map>
 
 using namespace xrh;
@@ -19,22 +22,25 @@ using namespace xrh;
 struct xrh_batch {
   xrh_ctx* ctx = nullptr;
   int32_t geom = XRH_GEOM_FP32;
-  std::vector<xrh_view> views;
-  std::vector<const xrh_volume*> vols;  // distinct volumes
-  std::vector<int32_t> view_vol;        // view -> index into vols
-  int32_t storage = XRH_F32, layout = kLayoutQuad;
+  // capacity
+  int32_t cap_views = 0;
   int64_t max_items = 0;
-  int64_t max_npix = 0;
+  int32_t cap_rows = 0, cap_cols = 0;
+  int64_t cap_npix = 0;
+  int cap_tiles = 0;
+  // current view table
+  std::vector<xrh_view> views;
+  std::vector<int32_t> view_vol;   // view -> index into the volume table
+  int32_t storage = XRH_F32, layout = kLayoutQuad;
   int max_img_tiles = 0, max_patch_tiles = 0, max_lds = 0;
-  bool any_patch_grad = false;
   bool fuse_ncc = true;            // NCC sums in the DRR epilogue (off if any NCC view is masked)
+  uint64_t version = 0;
   bool use_graph = true;
   bool time_kernels = false;
   // device
   VolDesc* vols_dev = nullptr;
   CamDev* cams_dev = nullptr;
   char* blob_dev = nullptr;        // poses | band plan | sim items
-  uint64_t blob_bytes = 0;
   DrrItem* items_dev = nullptr;
   float* drr_dev = nullptr;
   float* grad_dev = nullptr;
@@ -72,12 +78,19 @@ BlobLayout blob_layout(int64_t max_items)
   return L;
 }
 
+void clear_graphs(xrh_batch* b)
+{
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto g : b->graph_objs) (void)hipGraphDestroy(g);
+  b->graphs.clear();
+  b->graph_objs.clear();
+}
+
 void batch_free(xrh_batch* b)
 {
   if (!b) return;
   (void)hipStreamSynchronize(b->ctx->stream);
-  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto g : b->graph_objs) (void)hipGraphDestroy(g);
+  clear_graphs(b);
   for (void* p : {(void*)b->vols_dev, (void*)b->cams_dev, (void*)b->blob_dev, (void*)b->items_dev,
                   (void*)b->drr_dev, (void*)b->grad_dev, (void*)b->partials_dev, (void*)b->ncc_partials_dev,
                   (void*)b->sims_dev}) {
@@ -91,68 +104,33 @@ void batch_free(xrh_batch* b)
   delete b;
 }
 
-void batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views, int64_t max_items, int32_t geom,
-                  xrh_batch** out)
+void batch_alloc(xrh_ctx* ctx, int32_t max_views, int64_t max_items, int32_t max_rows, int32_t max_cols,
+                 int32_t geom, xrh_batch** out)
 {
-  XRH_REQUIRE(ctx && views && out, "null argument");
-  XRH_REQUIRE(num_views >= 1, "need at least one view");
+  XRH_REQUIRE(ctx && out, "null argument");
+  XRH_REQUIRE(max_views >= 1 && max_views <= 4096, "max_views must be in [1, 4096]");
   XRH_REQUIRE(max_items >= 1 && max_items < 65536, "max_items must be in [1, 65535]");
+  XRH_REQUIRE(max_rows >= 2 && max_cols >= 2 && max_rows <= 32768 && max_cols <= 32768, "bad detector capacity");
   XRH_REQUIRE(geom == XRH_GEOM_FP32 || geom == XRH_GEOM_FP64, "bad geometry mode");
   auto* b = new xrh_batch();
   b->ctx = ctx;
   b->geom = geom;
+  b->cap_views = max_views;
   b->max_items = max_items;
+  b->cap_rows = max_rows;
+  b->cap_cols = max_cols;
+  b->cap_npix = (int64_t)max_rows * max_cols;
+  b->cap_tiles = ((max_rows + 15) / 16) * ((max_cols + 15) / 16);
   try {
-    std::vector<CamDev> cams;
-    std::vector<VolDesc> vds;
-    for (int32_t v = 0; v < num_views; ++v) {
-      const xrh_view& w = views[v];
-      XRH_REQUIRE(w.vol && w.fixed, "view needs a volume and a fixed image");
-      XRH_REQUIRE(w.vol->ctx == ctx && w.fixed->ctx == ctx, "view objects belong to another context");
-      XRH_REQUIRE(w.kernel == XRH_LINE_INT_SUM || w.kernel == XRH_LINE_INT_MAX, "bad line-integral kernel");
-      XRH_REQUIRE(w.step > 0.0f, "ray step must be positive");
-      check_step(w.step, w.vol->desc);
-      XRH_REQUIRE(w.cam.num_det_rows == w.fixed->host.rows && w.cam.num_det_cols == w.fixed->host.cols,
-                  "fixed image size differs from the view's detector");
-      b->views.push_back(w);
-      cams.push_back(make_cam_dev(w.cam));
-      auto it = std::find(b->vols.begin(), b->vols.end(), w.vol);
-      if (it == b->vols.end()) {
-        b->view_vol.push_back((int32_t)b->vols.size());
-        b->vols.push_back(w.vol);
-        vds.push_back(w.vol->desc);
-      } else {
-        b->view_vol.push_back((int32_t)(it - b->vols.begin()));
-      }
-      const FixedDev& f = w.fixed->host;
-      b->max_npix = std::max<int64_t>(b->max_npix, (int64_t)f.rows * f.cols);
-      b->max_img_tiles = std::max(b->max_img_tiles, sim_img_tiles(f));
-      b->max_patch_tiles = std::max(b->max_patch_tiles, sim_tiles(f));
-      b->max_lds = std::max(b->max_lds, sim_patch_lds_bytes(f));
-      if (f.metric == XRH_SIM_PATCH_GRAD_NCC) b->any_patch_grad = true;
-      if (f.metric == XRH_SIM_NCC && f.mask) b->fuse_ncc = false;
-    }
-    b->storage = vds[0].storage;
-    b->layout = vds[0].layout;
-    for (const VolDesc& d : vds) {
-      XRH_REQUIRE(d.storage == b->storage && d.layout == b->layout,
-                  "all volumes of one batch must share storage type and layout");
-    }
-    b->vols_dev = (VolDesc*)ctx_alloc(ctx, sizeof(VolDesc) * vds.size());
-    b->cams_dev = (CamDev*)ctx_alloc(ctx, sizeof(CamDev) * cams.size());
-    XRH_HIP_CHECK(hipMemcpyAsync(b->vols_dev, vds.data(), sizeof(VolDesc) * vds.size(), hipMemcpyHostToDevice,
-                                 ctx->stream));
-    XRH_HIP_CHECK(hipMemcpyAsync(b->cams_dev, cams.data(), sizeof(CamDev) * cams.size(), hipMemcpyHostToDevice,
-                                 ctx->stream));
+    b->vols_dev = (VolDesc*)ctx_alloc(ctx, sizeof(VolDesc) * max_views);
+    b->cams_dev = (CamDev*)ctx_alloc(ctx, sizeof(CamDev) * max_views);
     const BlobLayout L = blob_layout(max_items);
-    b->blob_bytes = L.total;
     b->blob_dev = (char*)ctx_alloc(ctx, L.total);
     b->items_dev = (DrrItem*)ctx_alloc(ctx, sizeof(DrrItem) * max_items);
-    b->drr_dev = (float*)ctx_alloc(ctx, sizeof(float) * max_items * b->max_npix);
-    if (b->any_patch_grad) b->grad_dev = (float*)ctx_alloc(ctx, sizeof(float) * 2 * max_items * b->max_npix);
-    const int ptiles = std::max(b->max_img_tiles, b->max_patch_tiles);
-    b->partials_dev = (double*)ctx_alloc(ctx, sizeof(double) * 8 * ptiles * max_items);
-    b->ncc_partials_dev = (double*)ctx_alloc(ctx, sizeof(double) * 4 * b->max_img_tiles * max_items);
+    b->drr_dev = (float*)ctx_alloc(ctx, sizeof(float) * max_items * b->cap_npix);
+    b->grad_dev = (float*)ctx_alloc(ctx, sizeof(float) * 2 * max_items * b->cap_npix);
+    b->partials_dev = (double*)ctx_alloc(ctx, sizeof(double) * 8 * b->cap_tiles * max_items);
+    b->ncc_partials_dev = (double*)ctx_alloc(ctx, sizeof(double) * 4 * b->cap_tiles * max_items);
     b->sims_dev = (float*)ctx_alloc(ctx, sizeof(float) * max_items);
     XRH_HIP_CHECK(hipHostMalloc(&b->blob_host, L.total, hipHostMallocDefault));
     XRH_HIP_CHECK(hipHostMalloc(&b->sims_host, sizeof(float) * max_items, hipHostMallocDefault));
@@ -160,7 +138,85 @@ void batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views, int64_
     XRH_HIP_CHECK(hipEventCreate(&b->ev_drr1));
     XRH_HIP_CHECK(hipEventCreate(&b->ev_all0));
     XRH_HIP_CHECK(hipEventCreate(&b->ev_all1));
-    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    batch_free(b);
+    throw;
+  }
+  *out = b;
+}
+
+void set_views(xrh_batch* b, const xrh_view* views, int32_t num_views)
+{
+  XRH_REQUIRE(b && views, "null argument");
+  XRH_REQUIRE(!b->pending, "batch has a launch in flight");
+  XRH_REQUIRE(num_views >= 1 && num_views <= b->cap_views, "view count exceeds the batch capacity");
+  std::vector<CamDev> cams;
+  std::vector<VolDesc> vds;
+  std::vector<const xrh_volume*> vols;
+  std::vector<int32_t> view_vol;
+  int max_img = 0, max_patch = 0, max_lds = 0;
+  bool fuse = true;
+  for (int32_t v = 0; v < num_views; ++v) {
+    const xrh_view& w = views[v];
+    XRH_REQUIRE(w.vol && w.fixed, "view needs a volume and a fixed image");
+    XRH_REQUIRE(w.vol->ctx == b->ctx && w.fixed->ctx == b->ctx, "view objects belong to another context");
+    XRH_REQUIRE(w.kernel == XRH_LINE_INT_SUM || w.kernel == XRH_LINE_INT_MAX, "bad line-integral kernel");
+    XRH_REQUIRE(w.step > 0.0f, "ray step must be positive");
+    check_step(w.step, w.vol->desc);
+    XRH_REQUIRE(w.cam.num_det_rows == w.fixed->host.rows && w.cam.num_det_cols == w.fixed->host.cols,
+                "fixed image size differs from the view's detector");
+    XRH_REQUIRE(w.cam.num_det_rows <= b->cap_rows && w.cam.num_det_cols <= b->cap_cols,
+                "view detector exceeds the batch capacity");
+    XRH_REQUIRE(w.kernel == views[0].kernel, "all views of one batch must share the line-integral kernel");
+    cams.push_back(make_cam_dev(w.cam));
+    auto it = std::find(vols.begin(), vols.end(), w.vol);
+    if (it == vols.end()) {
+      view_vol.push_back((int32_t)vols.size());
+      vols.push_back(w.vol);
+      vds.push_back(w.vol->desc);
+    } else {
+      view_vol.push_back((int32_t)(it - vols.begin()));
+    }
+    const FixedDev& f = w.fixed->host;
+    max_img = std::max(max_img, sim_img_tiles(f));
+    max_patch = std::max(max_patch, sim_tiles(f));
+    max_lds = std::max(max_lds, sim_patch_lds_bytes(f));
+    if (f.metric == XRH_SIM_NCC && f.mask) fuse = false;
+  }
+  for (const VolDesc& d : vds) {
+    XRH_REQUIRE(d.storage == vds[0].storage && d.layout == vds[0].layout,
+                "all volumes of one batch must share storage type and layout");
+  }
+  hipStream_t s = b->ctx->stream;
+  XRH_HIP_CHECK(hipStreamSynchronize(s));
+  XRH_HIP_CHECK(hipMemcpyAsync(b->vols_dev, vds.data(), sizeof(VolDesc) * vds.size(), hipMemcpyHostToDevice, s));
+  XRH_HIP_CHECK(hipMemcpyAsync(b->cams_dev, cams.data(), sizeof(CamDev) * cams.size(), hipMemcpyHostToDevice, s));
+  XRH_HIP_CHECK(hipStreamSynchronize(s));
+  b->views.assign(views, views + num_views);
+  b->view_vol = view_vol;
+  b->storage = vds[0].storage;
+  b->layout = vds[0].layout;
+  b->max_img_tiles = max_img;
+  b->max_patch_tiles = max_patch;
+  b->max_lds = max_lds;
+  b->fuse_ncc = fuse;
+  ++b->version;
+  clear_graphs(b);
+}
+
+void batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views, int64_t max_items, int32_t geom,
+                  xrh_batch** out)
+{
+  XRH_REQUIRE(views && num_views >= 1, "need at least one view");
+  int32_t rows = 2, cols = 2;
+  for (int32_t v = 0; v < num_views; ++v) {
+    rows = std::max(rows, views[v].cam.num_det_rows);
+    cols = std::max(cols, views[v].cam.num_det_cols);
+  }
+  xrh_batch* b = nullptr;
+  batch_alloc(ctx, num_views, max_items, rows, cols, geom, &b);
+  try {
+    set_views(b, views, num_views);
   } catch (...) {
     batch_free(b);
     throw;
@@ -172,6 +228,7 @@ void batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views, int64_
 DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, float* drr_out,
                 std::vector<std::pair<int, std::vector<int64_t>>>& groups)
 {
+  XRH_REQUIRE(!b->views.empty(), "batch has no views");
   const BlobLayout L = blob_layout(b->max_items);
   PoseIn* pin = (PoseIn*)(b->blob_host + L.off_pose);
   SimItem* sin = (SimItem*)(b->blob_host + L.off_sim);
@@ -197,12 +254,13 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
     SimItem& s = sin[i];
     s.mov = drr + off;
     s.fixed = w.fixed->dev;
-    s.gx = b->grad_dev ? b->grad_dev + 2 * i * b->max_npix : nullptr;
-    s.gy = b->grad_dev ? b->grad_dev + (2 * i + 1) * b->max_npix : nullptr;
+    s.gx = b->grad_dev + 2 * i * b->cap_npix;
+    s.gy = b->grad_dev + (2 * i + 1) * b->cap_npix;
     s.out_index = i;
     off += (int64_t)rows[i] * cols[i];
   }
-  // items grouped by metric, in item order (sim items are re-ordered per group below)
+  XRH_REQUIRE(drr_out || off <= b->max_items * b->cap_npix, "DRR arena overflow");
+  // items grouped by metric, in item order
   for (int m = XRH_SIM_NCC; m <= XRH_SIM_PATCH_GRAD_NCC; ++m) {
     std::vector<int64_t> idx;
     for (int64_t i = 0; i < n; ++i) {
@@ -230,7 +288,7 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
   D.bands = plan.bands;
   D.max_band_tiles = plan.max_band_tiles;
   D.geom = b->geom;
-  D.kernel = b->views[item_view[0]].kernel;
+  D.kernel = b->views[0].kernel;
   D.store = XRH_STORE_REPLACE;
   D.fill = 0.0f;
   D.storage = b->storage;
@@ -265,10 +323,11 @@ void enqueue(xrh_batch* b, const DrrLaunch& D, const std::vector<std::pair<int, 
   XRH_HIP_CHECK(hipMemcpyAsync(b->sims_host, b->sims_dev, sizeof(float) * n, hipMemcpyDeviceToHost, s));
 }
 
-std::string signature(const int32_t* item_view, int64_t n, const float* drr_out)
+std::string signature(const xrh_batch* b, const int32_t* item_view, int64_t n, const float* drr_out)
 {
   std::string key((const char*)item_view, (size_t)n * sizeof(int32_t));
   key.append((const char*)&drr_out, sizeof(drr_out));
+  key.append((const char*)&b->version, sizeof(b->version));
   return key;
 }
 
@@ -285,7 +344,7 @@ void launch(xrh_batch* b, const int32_t* item_view, const double* item_pose, int
   b->timed_last = timed;
   if (timed) XRH_HIP_CHECK(hipEventRecord(b->ev_all0, s));
   if (b->use_graph && !b->time_kernels) {
-    const std::string key = signature(item_view, n, drr_out);
+    const std::string key = signature(b, item_view, n, drr_out);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       hipGraph_t g = nullptr;
@@ -330,6 +389,17 @@ xrh_status xrh_batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_vie
                             int32_t geom, xrh_batch** out)
 {
   return guarded([&] { batch_create(ctx, views, num_views, max_items, geom, out); });
+}
+
+xrh_status xrh_batch_create_empty(xrh_ctx* ctx, int32_t max_views, int64_t max_items, int32_t max_rows,
+                                  int32_t max_cols, int32_t geom, xrh_batch** out)
+{
+  return guarded([&] { batch_alloc(ctx, max_views, max_items, max_rows, max_cols, geom, out); });
+}
+
+xrh_status xrh_batch_set_views(xrh_batch* batch, const xrh_view* views, int32_t num_views)
+{
+  return guarded([&] { set_views(batch, views, num_views); });
 }
 
 xrh_status xrh_batch_destroy(xrh_batch* batch)

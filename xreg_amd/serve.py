@@ -1,0 +1,215 @@
+"""Continuous-batching 2D/3D registration server on one GPU.
+
+Jobs (single- or multi-view intensity-based registrations with CMA-ES, the loop of
+Intensity2D3DRegiCMAES::run, lib/regi/interfaces_2d_3d/xregIntensity2D3DRegiCMAES.cpp:75-261)
+arrive over time. Volumes stay resident; each admitted job owns its fixed images and CMA-ES
+state. At every scheduling step the server packs the pending generation of every ready job
+(lambda poses x its views, any mix of volumes, detector sizes and lambda) into one batched
+submission (xrh_batch_set_views + xrh_batch_eval), then hands each job its lambda objective
+values (mean over its views, ImgSimMetric2DCombineMean) and advances its optimiser. Jobs are
+admitted while the context's HBM use stays under the budget; a job's results do not depend on
+what it was batched with (every item is evaluated independently with fixed-order
+reductions).
+"""
+from __future__ import annotations
+
+import heapq
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _xrh as X
+from . import geometry as G
+from .cmaes import CMAES
+
+
+@dataclass
+class JobSpec:
+    job_id: int
+    arrival: float                 # seconds since the server clock started
+    volume: str                    # key of a resident volume
+    cams: list                     # xrh Camera per view
+    fixed: list                    # float32 fixed image per view
+    init_pose: np.ndarray          # 4x4 float64 cam->phys starting point
+    center: np.ndarray             # rotation centre (physical)
+    step: float
+    lam: int
+    max_gens: int
+    seed: int
+    sigma: tuple = (np.deg2rad(10.0),) * 3 + (15.0,) * 3
+    bounds: tuple | None = None
+    metric: int = X.XRH_SIM_PATCH_GRAD_NCC
+    smooth_ksize: int = 5
+    patch_radius: int = 5
+
+
+@dataclass
+class JobResult:
+    job_id: int
+    arrival: float
+    start: float
+    finish: float
+    generations: int
+    best_sim: float
+    best_params: np.ndarray
+    best_pose: np.ndarray
+    history: list = field(default_factory=list)
+
+    @property
+    def latency(self) -> float:
+        return self.finish - self.arrival
+
+
+def pose_from_params(init_pose: np.ndarray, center: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Lie-algebra delta about the rotation centre, applied on the volume side of the starting
+    pose (SE3OptVarsLieAlg, lib/regi/xregSE3OptVars.cpp:127-137)."""
+    c = np.asarray(center, dtype=np.float64)
+    return G.rigid(None, c) @ G.exp_se3(x) @ G.rigid(None, -c) @ init_pose
+
+
+class _Active:
+    def __init__(self, spec: JobSpec, fixeds, views, start: float):
+        self.spec = spec
+        self.fixeds = fixeds
+        self.views = views
+        self.start = start
+        self.es = CMAES(np.zeros(6), np.asarray(spec.sigma), spec.lam, spec.seed,
+                        bounds=None if spec.bounds is None else np.asarray(spec.bounds))
+        self.gen = 0
+        self.X = self.es.ask()
+        self.best = (np.inf, None)
+
+    def items(self):
+        poses = np.stack([pose_from_params(self.spec.init_pose, self.spec.center, x) for x in self.X])
+        return poses
+
+    def close(self):
+        for f in self.fixeds:
+            f.close()
+
+
+class RegistrationServer:
+    def __init__(self, ctx: X.Context, volumes: dict, hbm_budget: int, max_items: int = 4096,
+                 max_views: int = 256, max_det: int = 768, geom: int = X.XRH_GEOM_FP32,
+                 device=None):
+        """device: the object providing Batch / Fixed / make_view (the xrh binding by default;
+        tests substitute a host-side fake to exercise the scheduler without a GPU)."""
+        self.dev = device if device is not None else X
+        self.ctx = ctx
+        self.volumes = volumes      # name -> resident volume handle
+        self.budget = int(hbm_budget)
+        self.max_items = max_items
+        self.max_det = max_det
+        self.batch = self.dev.Batch(ctx, None, max_items, geom=geom, capacity=(max_views, max_det, max_det))
+        self.batch.set_options(use_graph=False, time_kernels=False)
+        self.max_views = max_views
+        self.steps = 0
+        self.poses_evaluated = 0
+        self.max_items_in_step = 0
+
+    def close(self):
+        self.batch.close()
+
+    def _fixed_bytes(self, spec: JobSpec) -> int:
+        per = 0
+        for f in spec.fixed:
+            per += f.size * 4 * (6 if spec.metric in (X.XRH_SIM_GRAD_NCC, X.XRH_SIM_PATCH_GRAD_NCC) else 3)
+        return per
+
+    def _admit(self, spec: JobSpec, now: float):
+        if spec.lam * len(spec.cams) > self.max_items or len(spec.cams) > self.max_views:
+            raise ValueError(f"job {spec.job_id}: one generation exceeds the server's batch capacity")
+        if self.ctx.bytes_in_use() + self._fixed_bytes(spec) > self.budget:
+            return None
+        fixeds, views = [], []
+        vol = self.volumes[spec.volume]
+        try:
+            for cam, img in zip(spec.cams, spec.fixed):
+                if cam.num_det_rows > self.max_det or cam.num_det_cols > self.max_det:
+                    raise ValueError(f"job {spec.job_id}: detector exceeds the server's capacity")
+                fx = self.dev.Fixed(self.ctx, img, spec.metric, smooth_ksize=spec.smooth_ksize,
+                                    patch_radius=spec.patch_radius)
+                fixeds.append(fx)
+                views.append(self.dev.make_view(vol, fx, cam, spec.step))
+        except X.XrhError as e:
+            for f in fixeds:
+                f.close()
+            if e.status == X.XRH_ERR_BUDGET:
+                return None
+            raise
+        return _Active(spec, fixeds, views, now)
+
+    def run(self, jobs: list[JobSpec], realtime: bool = True) -> list[JobResult]:
+        """Serves the jobs (sorted by arrival). With realtime=True arrivals follow the wall clock;
+        otherwise every job is available at once (throughput mode)."""
+        pending = [(j.arrival, j.job_id, j) for j in jobs]
+        heapq.heapify(pending)
+        active: list[_Active] = []
+        done: list[JobResult] = []
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+        while pending or active:
+            now = clock()
+            # admission: arrived jobs, in arrival order, while HBM and views allow
+            while pending and pending[0][0] <= now:
+                if sum(len(a.views) for a in active) + len(pending[0][2].cams) > self.max_views:
+                    break
+                a = self._admit(pending[0][2], time.perf_counter() - t0)
+                if a is None:
+                    break
+                heapq.heappop(pending)
+                active.append(a)
+            if not active:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(0.002, pending[0][0] - clock())))
+                continue
+            # pack ready generations into one submission (whole generations only)
+            packed, n_items = [], 0
+            for a in active:
+                k = a.spec.lam * len(a.views)
+                if n_items + k > self.max_items and packed:
+                    continue
+                packed.append(a)
+                n_items += k
+            views, item_view, item_pose, spans = [], [], [], []
+            for a in packed:
+                poses = a.items()
+                base_view = len(views)
+                views.extend(a.views)
+                start = len(item_view)
+                for v in range(len(a.views)):          # view-major, as xReg lays out projections
+                    item_view.extend([base_view + v] * len(poses))
+                    item_pose.extend(poses)
+                spans.append((start, len(a.views), len(poses)))
+            self.batch.set_views(views)
+            sims = self.batch.eval(np.array(item_view, np.int32), np.stack(item_pose))
+            self.steps += 1
+            self.poses_evaluated += len(item_view)
+            self.max_items_in_step = max(self.max_items_in_step, len(item_view))
+            finished = []
+            for a, (start, nv, lam) in zip(packed, spans):
+                s = sims[start:start + nv * lam].reshape(nv, lam)
+                f = s.astype(np.float64).mean(axis=0)   # mean over views
+                k = int(np.argmin(f))
+                if f[k] < a.best[0]:
+                    a.best = (float(f[k]), a.X[k].copy())
+                a.es.tell(f)
+                a.gen += 1
+                if a.gen >= a.spec.max_gens or a.es.stop():
+                    finished.append(a)
+                else:
+                    a.X = a.es.ask()
+            for a in finished:
+                t = time.perf_counter() - t0
+                bx = a.best[1]
+                done.append(JobResult(a.spec.job_id, a.spec.arrival if realtime else 0.0, a.start, t, a.gen,
+                                      a.best[0], bx, pose_from_params(a.spec.init_pose, a.spec.center, bx),
+                                      list(a.es.history)))
+                a.close()
+                active.remove(a)
+        return done
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else float("nan")
