@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--geom", default="fp64", choices=["fp32", "fp64"])
     ap.add_argument("--smooth", type=int, default=5)
     ap.add_argument("--no-skip", action="store_true", help="evaluate every sample (no empty-space map)")
+    ap.add_argument("--serve-jobs", type=int, default=24, help="served-latency leg: number of jobs (0 skips)")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--serve-gens", type=int, default=50, help="CMA-ES generations per job (upper bound)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run plain device copies of this many MiB (counter calibration under --pmc)")
     return ap.parse_args()
 
 
@@ -78,9 +83,58 @@ def cpu_baseline(w, fixed, poses, budget_s, smooth):
         if c - t0 >= budget_s and done >= 2:
             break
     el = time.perf_counter() - t0
+    # one pose on one thread, for the single-core figure
+    a = time.perf_counter()
+    drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, np.asarray(poses[0], np.float32)[None], step=w.step,
+                       threads=1)
+    b = time.perf_counter()
+    O.patch_grad_ncc_f32(fixed, drr, smooth_ksize=smooth, threads=1)
+    c = time.perf_counter()
     return {"value": done / el, "unit": "poses/s", "cores": threads, "kind": "port",
             "sample": f"{done} poses of the same 384^2 patch-grad-NCC workload, DRR {t_drr:.2f}s + "
-                      f"similarity {t_sim:.2f}s, oracle FP32 restatement, {threads} threads"}
+                      f"similarity {t_sim:.2f}s, oracle FP32 restatement, {threads} threads",
+            "single_thread": {"value": 1.0 / (c - a), "drr_s": round(b - a, 3), "similarity_s": round(c - b, 3)}}
+
+
+def serve_leg(ctx, vol, w, n_jobs, rate, max_gens, lam, geom, smooth, seed=21):
+    """Served registration latency: n_jobs single-view CMA-ES registrations of the same workload
+    (lambda poses per generation, up to max_gens generations) arriving as a Poisson process of
+    `rate` jobs/s. Each job has its own ground truth (the reference pose moved by U(+-3 deg,
+    +-5 mm)), its own noisy fixed image cast on the device, and starts from U(+-5 deg, +-10 mm)
+    about it. Latency = finish - arrival, on the wall clock."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import serve, synth
+    from xreg_amd import geometry as G
+
+    rng = np.random.default_rng(seed)
+    cam = w.cams[0]
+    P = cam.num_det_rows * cam.num_det_cols
+    buf = ctx.alloc(4 * P)
+    specs, t = [], 0.0
+    for j in range(n_jobs):
+        gt = G.perturbed_pose(w.gt, w.center, np.deg2rad(rng.uniform(-3, 3, 3)), rng.uniform(-5, 5, 3))
+        X.line_int(ctx, vol, [cam], gt.astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
+        fixed = synth.noisy_fixed(buf.download(np.float32).reshape(cam.num_det_rows, cam.num_det_cols),
+                                  seed=1000 + j)
+        init = G.perturbed_pose(gt, w.center, np.deg2rad(rng.uniform(-5, 5, 3)), rng.uniform(-10, 10, 3))
+        specs.append(serve.JobSpec(job_id=j, arrival=t, volume="c2", cams=[cam], fixed=[fixed], init_pose=init,
+                                   center=w.center, step=w.step, lam=lam, max_gens=max_gens, seed=5000 + j,
+                                   sigma=(np.deg2rad(5.0),) * 3 + (10.0,) * 3, smooth_ksize=smooth))
+        t += float(rng.exponential(1.0 / rate))
+    buf.free()
+    srv = serve.RegistrationServer(ctx, {"c2": vol}, hbm_budget=200 << 30, max_items=max(4096, lam),
+                                   max_views=64, max_det=max(cam.num_det_rows, cam.num_det_cols), geom=geom)
+    t0 = time.perf_counter()
+    res = srv.run(specs, realtime=True)
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = [r.latency for r in res]
+    return {"jobs": len(res), "arrival_rate_jobs_per_s": rate, "lambda": lam, "max_generations": max_gens,
+            "mean_generations": round(float(np.mean([r.generations for r in res])), 2),
+            "p50_s": round(serve.percentile(lat, 50), 4), "p99_s": round(serve.percentile(lat, 99), 4),
+            "max_s": round(max(lat), 4), "wall_s": round(wall, 3), "scheduler_steps": srv.steps,
+            "poses_per_s": round(srv.poses_evaluated / wall, 1),
+            "max_poses_per_submission": srv.max_items_in_step}
 
 
 def main():
@@ -157,6 +211,21 @@ def main():
         ctx.synchronize()
         plain_value = n_plain / (time.perf_counter() - t0)
 
+    if args.calib_copy_mib > 0:
+        # a plain copy kernel of known size (xrh_copy_kernel): reads and writes
+        # calib_copy_mib MiB per dispatch
+        src = ctx.alloc(args.calib_copy_mib << 20)
+        dst = ctx.alloc(args.calib_copy_mib << 20)
+        src.upload(np.ones((args.calib_copy_mib << 20) // 4, np.float32))
+        for _ in range(3):
+            dst.copy_from(src)
+        src.free()
+        dst.free()
+
+    served = None
+    if args.serve_jobs > 0:
+        served = serve_leg(ctx, vol, w, args.serve_jobs, args.serve_rate, args.serve_gens, lam, geom, args.smooth)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, fixed, gens[0], args.cpu_seconds, args.smooth)
@@ -187,6 +256,7 @@ def main():
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "evaluated_gbs": round((8.0 * bv * S_eval + 4.0 * P * lam) / (drr_ms_avg * 1e-3) / 1e9, 1)},
         "empty_space_skipping": not args.no_skip,
+        "served": served,
         "cpu_baseline": cpu,
         "sim_checksum": float(np.sum(sims)),
     }

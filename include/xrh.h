@@ -124,6 +124,12 @@ xrh_status xrh_device_free(xrh_ctx* ctx, void* dptr);
 /* Stream-ordered copies; _sync variants wait for completion. */
 xrh_status xrh_copy_h2d(xrh_ctx* ctx, void* dst_dev, const void* src_host, uint64_t bytes);
 xrh_status xrh_copy_d2h(xrh_ctx* ctx, void* dst_host, const void* src_dev, uint64_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and stores when both
+ * pointers and the size are 16-byte aligned), synchronous. Replaces the OCL buffer-to-buffer
+ * copies of the projection sync objects (lib/ray_cast/xregRayCastSyncBuf.cpp:27-194). Its
+ * dispatch "xrh_copy_kernel" reads and writes exactly `bytes`, which makes it the reference
+ * point for calibrating memory-traffic counters. */
+xrh_status xrh_copy_d2d(xrh_ctx* ctx, void* dst_dev, const void* src_dev, uint64_t bytes);
 
 /* ---------------------------------------------------------------------------------- */
 /* volumes: RayCaster::set_volume / vols_changed (xregRayCastInterface.h:87-90, 425;

@@ -24,13 +24,13 @@ def _jobs(oracle, ctx, n_jobs, seed0=0):
         cam = w.cams[0] if j % 3 else G.downsample_camera(w.cams[0], 0.5)
         g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.gt[None]), step=w.step)
         fixed = synth.noisy_fixed(g[0], seed=100 + j)
-        r = np.deg2rad(rng.uniform(-5, 5, 3))
-        t = rng.uniform(-8, 8, 3)
+        r = np.deg2rad(rng.uniform(-3, 3, 3))
+        t = rng.uniform(-5, 5, 3)
         init = G.perturbed_pose(w.gt, w.center, r, t)
         specs.append(serve.JobSpec(job_id=j, arrival=0.002 * j, volume=name, cams=[cam], fixed=[fixed],
                                    init_pose=init, center=w.center, step=w.step, lam=16 + 4 * (j % 3),
                                    max_gens=12, seed=1000 + j,
-                                   sigma=(np.deg2rad(4.0),) * 3 + (6.0,) * 3))
+                                   sigma=(np.deg2rad(3.0),) * 3 + (5.0,) * 3))
     return specs, {k: v[1] for k, v in vols.items()}, vols
 
 
@@ -52,6 +52,7 @@ def test_alone_equals_mixed(ctx, oracle):
 
 
 def test_registration_improves_pose(ctx, oracle):
+    from xreg_amd import geometry as G
     from xreg_amd import serve
 
     print("case: serve convergence", flush=True)
@@ -65,11 +66,16 @@ def test_registration_improves_pose(ctx, oracle):
     for s in specs:
         r = res[s.job_id]
         w = wl[s.volume][0]
-        err0 = np.linalg.norm(s.init_pose[:3, 3] - w.gt[:3, 3])
-        err1 = np.linalg.norm(r.best_pose[:3, 3] - w.gt[:3, 3])
-        print(f"job {s.job_id}: translation error {err0:.2f} -> {err1:.2f} mm, sim {r.history[0]:.4f} -> "
+        # mean projection distance of points about the volume centre (the in-plane error a
+        # single view constrains)
+        ext = np.array(w.vol.shape[::-1]) * w.spacing
+        pts = [w.center + 0.25 * ext * np.array(c) for c in [(-1, -1, -1), (1, 1, 1), (1, -1, 1), (-1, 1, -1),
+                                                              (0, 0, 0)]]
+        err0 = G.mean_projection_distance(s.init_pose, w.gt, pts, 1020.0)
+        err1 = G.mean_projection_distance(r.best_pose, w.gt, pts, 1020.0)
+        print(f"job {s.job_id}: mean projection distance {err0:.2f} -> {err1:.2f} mm, sim {r.history[0]:.4f} -> "
               f"{r.best_sim:.4f}, latency {r.latency:.3f}s")
         assert r.best_sim < r.history[0]
-        assert err1 < err0
+        assert err1 < 0.5 * err0
     for v in vols.values():
         v.close()

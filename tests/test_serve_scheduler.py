@@ -114,3 +114,23 @@ def test_oversized_job_is_rejected():
                                    device=FAKE)
     with pytest.raises(ValueError):
         srv.run(_specs(1, lam=20), realtime=False)
+
+
+def test_exp_se3_matches_matrix_exponential():
+    """exp_se3 and its batched form against scipy's matrix exponential of the twist."""
+    from scipy.linalg import expm
+
+    rng = np.random.default_rng(5)
+    X = np.concatenate([rng.normal(0, 0.3, (16, 3)), rng.normal(0, 10, (16, 3))], axis=1)
+    X[0, :3] = 0.0  # pure translation takes the small-angle branch
+    B = G.exp_se3_batch(X)
+    for x, b in zip(X, B):
+        tw = np.zeros((4, 4))
+        tw[:3, :3] = [[0, -x[2], x[1]], [x[2], 0, -x[0]], [-x[1], x[0], 0]]
+        tw[:3, 3] = x[3:]
+        np.testing.assert_allclose(G.exp_se3(x), expm(tw), atol=1e-10)
+        np.testing.assert_allclose(b, G.exp_se3(x), atol=1e-12)
+    init = G.gt_pose(np.array([10.0, 20.0, 30.0]), 510.0)
+    P = serve.poses_from_params(init, np.array([10.0, 20.0, 30.0]), X)
+    for x, p in zip(X, P):
+        np.testing.assert_allclose(p, serve.pose_from_params(init, np.array([10.0, 20.0, 30.0]), x), atol=1e-9)

@@ -97,6 +97,7 @@ SIGNATURES = {
     "xrh_device_free": (_I32, [_P, _P]),
     "xrh_copy_h2d": (_I32, [_P, _P, _P, _U64]),
     "xrh_copy_d2h": (_I32, [_P, _P, _P, _U64]),
+    "xrh_copy_d2d": (_I32, [_P, _P, _P, _U64]),
     "xrh_volume_create": (_I32, [_P, _PF, _PI64, _PF, _I32, C.POINTER(_P)]),
     "xrh_volume_create_layout": (_I32, [_P, _PF, _PI64, _PF, _I32, _I32, C.POINTER(_P)]),
     "xrh_volume_destroy": (_I32, [_P]),
@@ -172,6 +173,13 @@ class DeviceBuffer:
         out = np.empty(count, dtype=dtype)
         check(lib().xrh_copy_d2h(self.ctx.h, out.ctypes.data, self.ptr + offset_bytes, out.nbytes))
         return out
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None, dst_offset: int = 0,
+                  src_offset: int = 0) -> None:
+        """Device-to-device copy (xrh_copy_d2d)."""
+        n = min(self.nbytes - dst_offset, other.nbytes - src_offset) if nbytes is None else int(nbytes)
+        assert 0 <= n and dst_offset + n <= self.nbytes and src_offset + n <= other.nbytes
+        check(lib().xrh_copy_d2d(self.ctx.h, self.ptr + dst_offset, other.ptr + src_offset, n))
 
     def free(self) -> None:
         if self.ptr:

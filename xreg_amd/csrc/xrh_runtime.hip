@@ -576,6 +576,35 @@ xrh_status xrh_copy_d2h(xrh_ctx* ctx, void* dst_host, const void* src_dev, uint6
   });
 }
 
+// Streaming copy: 16-byte vectors when aligned (n16 of them), the tail (or an unaligned
+// copy) byte by byte. Grid-stride, so any grid size covers the buffer.
+__global__ void xrh_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16,
+                                unsigned char* __restrict__ dst_b, const unsigned char* __restrict__ src_b,
+                                uint64_t tail_begin, uint64_t tail_end)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i = tid; i < n16; i += stride) dst[i] = src[i];
+  for (uint64_t i = tail_begin + tid; i < tail_end; i += stride) dst_b[i] = src_b[i];
+}
+
+xrh_status xrh_copy_d2d(xrh_ctx* ctx, void* dst_dev, const void* src_dev, uint64_t bytes)
+{
+  return guarded([&] {
+    XRH_REQUIRE(ctx && dst_dev && src_dev, "null argument");
+    if (bytes == 0) return;
+    const bool aligned = ((uintptr_t)dst_dev % 16 == 0) && ((uintptr_t)src_dev % 16 == 0);
+    const uint64_t n16 = aligned ? bytes / 16 : 0;
+    const uint64_t work = n16 + (bytes - 16 * n16);
+    const uint64_t blocks = std::min<uint64_t>(std::max<uint64_t>((work + 255) / 256, 1), 8192);
+    hipLaunchKernelGGL(xrh_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (uint4*)dst_dev,
+                       (const uint4*)src_dev, n16, (unsigned char*)dst_dev, (const unsigned char*)src_dev,
+                       16 * n16, bytes);
+    XRH_HIP_CHECK(hipGetLastError());
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 xrh_status xrh_volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
                              const float idx_to_phys[16], int32_t storage, xrh_volume** out)
 {

@@ -193,6 +193,32 @@ def exp_se3(x: np.ndarray) -> np.ndarray:
     return T
 
 
+def exp_se3_batch(X: np.ndarray) -> np.ndarray:
+    """exp_se3 over rows of an (n, 6) array -> (n, 4, 4) float64; same formulas, vectorised."""
+    X = np.asarray(X, dtype=np.float64).reshape(-1, 6)
+    n = X.shape[0]
+    w, v = X[:, :3], X[:, 3:6]
+    W = np.zeros((n, 3, 3))
+    W[:, 0, 1], W[:, 0, 2] = -w[:, 2], w[:, 1]
+    W[:, 1, 0], W[:, 1, 2] = w[:, 2], -w[:, 0]
+    W[:, 2, 0], W[:, 2, 1] = -w[:, 1], w[:, 0]
+    WW = W @ W
+    th = np.linalg.norm(w, axis=1)
+    big = th > 1.0e-14
+    ts = np.where(big, th, 1.0)
+    a = np.where(big, np.sin(ts) / ts, 0.0)[:, None, None]
+    b = np.where(big, (1 - np.cos(ts)) / ts ** 2, 0.0)[:, None, None]
+    c = np.where(big, (ts - np.sin(ts)) / ts ** 3, 0.0)[:, None, None]
+    eye = np.eye(3)[None]
+    A = np.where(big[:, None, None], eye + a * W + b * WW, eye)
+    V = np.where(big[:, None, None], eye + b * W + c * WW, eye)
+    T = np.zeros((n, 4, 4))
+    T[:, :3, :3] = A
+    T[:, :3, 3] = np.einsum("nij,nj->ni", V, v)
+    T[:, 3, 3] = 1.0
+    return T
+
+
 def gt_pose(vol_center_phys, src_to_iso: float, view_rot: np.ndarray = R_AP) -> np.ndarray:
     """cam->phys pose putting the isocenter (camera (0,0,-src_to_iso)) at the volume centre."""
     return rigid(None, vol_center_phys) @ rigid(view_rot) @ rigid(None, (0.0, 0.0, src_to_iso))
@@ -222,3 +248,15 @@ def vol_center_phys(size_xyz, spacing, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
     lib/ray_cast/xregRayCastInterface.cpp:200-224)."""
     s = np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,))
     return np.asarray(origin, dtype=np.float64) + s * (np.asarray(size_xyz, dtype=np.float64) / 2.0)
+
+
+def mean_projection_distance(pose_est: np.ndarray, pose_gt: np.ndarray, pts_phys, focal: float) -> float:
+    """Mean distance (detector mm) between the projections of physical points under two
+    cam->phys poses (NEG_Z pinhole at the origin, detector at z = -focal)."""
+    d = []
+    for p in np.asarray(pts_phys, dtype=np.float64).reshape(-1, 3):
+        ph = np.append(p, 1.0)
+        a = np.linalg.solve(pose_est, ph)
+        b = np.linalg.solve(pose_gt, ph)
+        d.append(np.linalg.norm(focal * a[:2] / -a[2] - focal * b[:2] / -b[2]))
+    return float(np.mean(d))

@@ -68,6 +68,12 @@ def pose_from_params(init_pose: np.ndarray, center: np.ndarray, x: np.ndarray) -
     return G.rigid(None, c) @ G.exp_se3(x) @ G.rigid(None, -c) @ init_pose
 
 
+def poses_from_params(init_pose: np.ndarray, center: np.ndarray, X: np.ndarray) -> np.ndarray:
+    """pose_from_params over the rows of X, (n, 4, 4)."""
+    c = np.asarray(center, dtype=np.float64)
+    return G.rigid(None, c) @ G.exp_se3_batch(X) @ (G.rigid(None, -c) @ init_pose)
+
+
 class _Active:
     def __init__(self, spec: JobSpec, fixeds, views, start: float):
         self.spec = spec
@@ -81,8 +87,7 @@ class _Active:
         self.best = (np.inf, None)
 
     def items(self):
-        poses = np.stack([pose_from_params(self.spec.init_pose, self.spec.center, x) for x in self.X])
-        return poses
+        return poses_from_params(self.spec.init_pose, self.spec.center, np.asarray(self.X))
 
     def close(self):
         for f in self.fixeds:
