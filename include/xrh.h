@@ -149,6 +149,15 @@ xrh_status xrh_volume_destroy(xrh_volume* vol);
 xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_t size[3],
                                     const float idx_to_phys[16], int32_t storage,
                                     int32_t layout, xrh_volume** out);
+/* HUToLinAtt (lib/image/xregHUToLinAtt.cpp:45-80) fused into volume ingest: the voxels are
+ * Hounsfield units and are converted on the device, per voxel in double and rounded to
+ * float, to max((hu * s + mu_water) - mu_lower, 0) with s = (mu_water - mu_air) * 1e-3 and
+ * mu_lower = hu_lower * s + mu_water, before the layout is built. The reference's defaults
+ * are hu_lower -1000, mu_water 0.02683, mu_air 0.02485e-4 (xregHUToLinAtt.h:73-76).
+ * layout -1 picks the default for the storage type. */
+xrh_status xrh_volume_create_hu(xrh_ctx* ctx, const float* vox_hu, const int64_t size[3],
+                                const float idx_to_phys[16], int32_t storage, int32_t layout,
+                                double hu_lower, double mu_water, double mu_air, xrh_volume** out);
 /* Device bytes held by the volume. */
 uint64_t xrh_volume_bytes(const xrh_volume* vol);
 
@@ -260,6 +269,23 @@ xrh_status xrh_batch_last_timing(const xrh_batch* batch, float* drr_kernel_ms, f
 xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view,
                                   const double* item_pose, int64_t n, uint64_t* samples_out,
                                   uint64_t* evaluated_out);
+
+/* ---------------------------------------------------------------------------------- */
+/* multi-resolution: DownsampleImage (lib/itk/xregITKResampleUtils.h:51-120, 181-186) as
+ * DownsampleProjData applies it to fixed images (lib/image/xregProjData.cpp:40-50): ITK 5.1's
+ * DiscreteGaussianImageFilter (variance sigma^2 in pixels, sigma < 0 selects 0.5 / factor,
+ * MaximumError 0.01, MaximumKernelWidth 32; skipped when factor >= 1 or |sigma| <= 1e-6),
+ * then ResampleImageFilter with a cubic B-spline interpolator (mirror boundary) onto the
+ * grid of spacing / factor. Images are row-major [rows][cols] float on the host. */
+
+/* Output size of DownsampleImage: floor(n * factor + 0.5) per axis. */
+xrh_status xrh_downsample_size(int32_t rows, int32_t cols, double factor, int32_t* out_rows,
+                               int32_t* out_cols);
+/* Downsamples img (pixel spacing row_spacing x col_spacing) into out
+ * (out_rows x out_cols, which must be xrh_downsample_size's). Synchronous. */
+xrh_status xrh_downsample_image(xrh_ctx* ctx, const float* img, int32_t rows, int32_t cols,
+                                double row_spacing, double col_spacing, double factor, double sigma,
+                                float* out, int32_t out_rows, int32_t out_cols);
 
 #ifdef __cplusplus
 }

@@ -282,3 +282,33 @@ def cam_ind_to_det_f32(ocam: OrcCamF, col, row):
     out = np.zeros(3, dtype=np.float32)
     lib().orc_cam_ind_to_det_f32(C.byref(ocam), C.c_float(col), C.c_float(row), _f(out))
     return out
+
+
+def hu_to_lin_att(hu, hu_lower=-1000.0, mu_water=0.02683 * 1.0, mu_air=0.02485 * 0.0001):
+    """HUToLinAtt restated (orc_hu_to_lin_att); defaults of xregHUToLinAtt.h:73-76."""
+    a = np.ascontiguousarray(hu, dtype=np.float32)
+    out = np.empty_like(a)
+    lib().orc_hu_to_lin_att(_f(a), C.c_int64(a.size), C.c_double(hu_lower), C.c_double(mu_water),
+                            C.c_double(mu_air), _f(out))
+    return out
+
+
+def gaussian_kernel(variance, max_error=0.01, max_width=32):
+    """itk::GaussianOperator coefficients restated (orc_gaussian_kernel)."""
+    k = np.zeros(1025, dtype=np.float64)
+    n = lib().orc_gaussian_kernel(C.c_double(variance), C.c_double(max_error), C.c_int32(max_width),
+                                  k.ctypes.data_as(C.POINTER(C.c_double)), C.c_int32(k.size))
+    assert n > 0
+    return k[:n].copy()
+
+
+def downsample_image(img, factor, sigma=-1.0, spacing=(1.0, 1.0)):
+    """DownsampleImage restated (orc_downsample_image); spacing is (row, col)."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    orows = int((a.shape[0] * factor) + 0.5)
+    ocols = int((a.shape[1] * factor) + 0.5)
+    out = np.empty((orows, ocols), dtype=np.float32)
+    lib().orc_downsample_image(_f(a), C.c_int32(a.shape[0]), C.c_int32(a.shape[1]), C.c_double(spacing[0]),
+                               C.c_double(spacing[1]), C.c_double(factor), C.c_double(sigma), _f(out),
+                               C.c_int32(orows), C.c_int32(ocols))
+    return out

@@ -1307,3 +1307,188 @@ void orc_patch_grad_ncc_f64(const double* fixed, const double* mov, int32_t rows
   free(mgx);
   free(mgy);
 }
+
+/* ---------------------------------------------------------------------------------------- */
+/* HU -> linear attenuation (lib/image/xregHUToLinAtt.cpp:45-68) */
+
+void orc_hu_to_lin_att(const float* hu, int64_t n, double hu_lower, double mu_water, double mu_air, float* out)
+{
+  const double hu_scale = (mu_water - mu_air) * 1.0e-3;
+  const double mu_lower = (hu_lower * hu_scale) + mu_water;
+  for (int64_t i = 0; i < n; ++i) {
+    const double a = ((double)hu[i] * hu_scale) + mu_water - mu_lower;
+    /* std::max(a, 0.0) returns a unless a < 0 (a NaN passes through) */
+    out[i] = (float)((a < 0.0) ? 0.0 : a);
+  }
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* multi-resolution fixed images: ITK 5.1 DiscreteGaussianImageFilter + BSpline resampling as
+ * used by DownsampleImage (lib/itk/xregITKResampleUtils.h:51-120, 181-186). ITK is not part
+ * of the reference tree; its published algorithms are restated here. */
+
+/* Abramowitz & Stegun 9.8.1-9.8.4 (the approximations itk::GaussianOperator uses) */
+static double orc_bessel_i0(double x)
+{
+  double ax = fabs(x), t;
+  if (ax < 3.75) {
+    t = (x / 3.75) * (x / 3.75);
+    return 1.0 + t * (3.5156229 + t * (3.0899424 + t * (1.2067492 + t * (0.2659732 + t * (0.0360768 + t * 0.0045813)))));
+  }
+  t = 3.75 / ax;
+  return (exp(ax) / sqrt(ax)) *
+         (0.39894228 + t * (0.01328592 + t * (0.00225319 + t * (-0.00157565 + t * (0.00916281 + t * (-0.02057706 +
+          t * (0.02635537 + t * (-0.01647633 + t * 0.00392377))))))));
+}
+
+static double orc_bessel_i1(double x)
+{
+  double ax = fabs(x), t, r;
+  if (ax < 3.75) {
+    t = (x / 3.75) * (x / 3.75);
+    r = ax * (0.5 + t * (0.87890594 + t * (0.51498869 + t * (0.15084934 + t * (0.02658733 + t * (0.00301532 +
+         t * 0.00032411))))));
+  } else {
+    t = 3.75 / ax;
+    r = (exp(ax) / sqrt(ax)) *
+        (0.39894228 + t * (-0.03988024 + t * (-0.00362018 + t * (0.00163801 + t * (-0.01031555 + t * (0.02282967 +
+         t * (-0.02895312 + t * (0.01787654 - t * 0.00420059))))))));
+  }
+  return (x < 0.0) ? -r : r;
+}
+
+int32_t orc_gaussian_kernel(double variance, double max_error, int32_t max_width, double* k, int32_t cap)
+{
+  double half[1024];
+  int32_t nh = 0, i;
+  double sum = 0.0;
+  const double et = exp(-variance);
+  half[nh++] = et * orc_bessel_i0(variance);
+  sum += half[0];
+  half[nh++] = et * orc_bessel_i1(variance);
+  sum += 2.0 * half[1];
+  for (i = 2; sum < 1.0 - max_error && i < 1023; ++i) {
+    half[nh++] = half[i - 2] - 2 * (i - 1) / variance * half[i - 1];
+    sum += 2.0 * half[i];
+    if (half[i] <= 0.0) break;
+    if (nh > max_width) break;
+  }
+  for (i = 0; i < nh; ++i) half[i] /= sum;
+  const int32_t r = nh - 1;
+  if (2 * r + 1 > cap) return -1;
+  for (i = 0; i <= r; ++i) k[r + i] = k[r - i] = half[i];
+  return 2 * r + 1;
+}
+
+static void orc_bspline_line(double* c, int64_t n, int64_t stride)
+{
+  const double z = sqrt(3.0) - 2.0;
+  int64_t i;
+  if (n == 1) return;
+  for (i = 0; i < n; ++i) c[i * stride] *= (1.0 - z) * (1.0 - 1.0 / z);
+  {
+    const int64_t horizon = (int64_t)ceil(log(1.0e-10) / log(fabs(z)));
+    double zn = z;
+    if (horizon < n) {
+      double sum = c[0];
+      for (i = 1; i < horizon; ++i) {
+        sum += zn * c[i * stride];
+        zn *= z;
+      }
+      c[0] = sum;
+    } else {
+      const double iz = 1.0 / z;
+      double z2n = pow(z, (double)(n - 1));
+      double sum = c[0] + z2n * c[(n - 1) * stride];
+      z2n *= z2n * iz;
+      for (i = 1; i <= n - 2; ++i) {
+        sum += (zn + z2n) * c[i * stride];
+        zn *= z;
+        z2n *= iz;
+      }
+      c[0] = sum / (1.0 - zn * zn);
+    }
+  }
+  for (i = 1; i < n; ++i) c[i * stride] += z * c[(i - 1) * stride];
+  c[(n - 1) * stride] = (z / (z * z - 1.0)) * (z * c[(n - 2) * stride] + c[(n - 1) * stride]);
+  for (i = n - 2; i >= 0; --i) c[i * stride] = z * (c[(i + 1) * stride] - c[i * stride]);
+}
+
+static int64_t orc_mirror(int64_t i, int64_t n)
+{
+  int64_t n2;
+  if (n == 1) return 0;
+  n2 = 2 * n - 2;
+  if (i < 0) i = -i;
+  i %= n2;
+  return (i >= n) ? n2 - i : i;
+}
+
+void orc_downsample_image(const float* img, int32_t rows, int32_t cols, double row_sp, double col_sp,
+                          double factor, double sigma, float* out, int32_t out_rows, int32_t out_cols)
+{
+  const int64_t n = (int64_t)rows * cols;
+  float* sm = (float*)malloc((size_t)n * sizeof(float));
+  double* tmp = (double*)malloc((size_t)n * sizeof(double));
+  int64_t r, c, i, j;
+  if (factor < 1.0 && fabs(sigma) > 1.0e-6) {
+    double k[129];
+    const double s = (sigma < 0.0) ? 0.5 / factor : sigma;
+    const int32_t nk = orc_gaussian_kernel(s * s, 0.01, 32, k, 129);
+    const int32_t R = nk / 2;
+    for (r = 0; r < rows; ++r)
+      for (c = 0; c < cols; ++c) {
+        double acc = 0.0;
+        int32_t q;
+        for (q = -R; q <= R; ++q) {
+          int64_t cc = c + q;
+          cc = cc < 0 ? 0 : (cc >= cols ? cols - 1 : cc);
+          acc += k[q + R] * (double)img[r * cols + cc];
+        }
+        tmp[r * cols + c] = acc;
+      }
+    for (r = 0; r < rows; ++r)
+      for (c = 0; c < cols; ++c) {
+        double acc = 0.0;
+        int32_t q;
+        for (q = -R; q <= R; ++q) {
+          int64_t rr = r + q;
+          rr = rr < 0 ? 0 : (rr >= rows ? rows - 1 : rr);
+          acc += k[q + R] * tmp[rr * cols + c];
+        }
+        sm[r * cols + c] = (float)acc;
+      }
+  } else {
+    memcpy(sm, img, (size_t)n * sizeof(float));
+  }
+  for (i = 0; i < n; ++i) tmp[i] = (double)sm[i];
+  for (r = 0; r < rows; ++r) orc_bspline_line(tmp + r * cols, cols, 1);
+  for (c = 0; c < cols; ++c) orc_bspline_line(tmp + c, rows, cols);
+  for (i = 0; i < out_rows; ++i)
+    for (j = 0; j < out_cols; ++j) {
+      const double x = ((col_sp / factor) * (double)j) * (1.0 / col_sp);
+      const double y = ((row_sp / factor) * (double)i) * (1.0 / row_sp);
+      double v = 0.0;
+      if (x >= -0.5 && x < cols - 0.5 && y >= -0.5 && y < rows - 0.5) {
+        double wx[4], wy[4], t;
+        int64_t bx = (int64_t)floor(x) - 1, by = (int64_t)floor(y) - 1, p;
+        t = x - (double)(bx + 1);
+        wx[3] = (1.0 / 6.0) * t * t * t;
+        wx[0] = 1.0 / 6.0 + 0.5 * t * (t - 1.0) - wx[3];
+        wx[2] = t + wx[0] - 2.0 * wx[3];
+        wx[1] = 1.0 - wx[0] - wx[2] - wx[3];
+        t = y - (double)(by + 1);
+        wy[3] = (1.0 / 6.0) * t * t * t;
+        wy[0] = 1.0 / 6.0 + 0.5 * t * (t - 1.0) - wy[3];
+        wy[2] = t + wy[0] - 2.0 * wy[3];
+        wy[1] = 1.0 - wy[0] - wy[2] - wy[3];
+        for (p = 0; p < 16; ++p) {
+          const int64_t a = p % 4, b = p / 4;
+          v += (wx[a] * wy[b]) * tmp[orc_mirror(by + b, rows) * cols + orc_mirror(bx + a, cols)];
+        }
+      }
+      out[i * out_cols + j] = (float)v;
+    }
+  free(sm);
+  free(tmp);
+}

@@ -128,6 +128,22 @@ void orc_patch_grad_ncc_f64(const double* fixed, const double* mov, int32_t rows
 /* Eigen 3.3 SSE sum restatement, exported for its own test. */
 float orc_eigen_sum_f32(const float* x, int64_t n);
 
+/* HUToLinAttFilter::GenerateData (lib/image/xregHUToLinAtt.cpp:45-68): per voxel in double,
+ * std::max((hu * s + mu_water) - mu_lower, 0.0) rounded to float, s = (mu_water - mu_air) * 1e-3,
+ * mu_lower = hu_lower * s + mu_water. */
+/* itk::GaussianOperator::GenerateCoefficients (ITK 5.1): symmetric discrete-Gaussian kernel
+ * (e^-t I_n(t), Bessel recurrence, truncated at 1 - max_error of the mass, normalised) for
+ * variance t in pixels. Writes up to cap taps into k; returns the number of taps (2r+1). */
+int32_t orc_gaussian_kernel(double variance, double max_error, int32_t max_width, double* k, int32_t cap);
+/* DownsampleImage with the default B-spline interpolator (lib/itk/xregITKResampleUtils.h:
+ * 51-120): DiscreteGaussian (x then y, edge replication, double between passes, float result),
+ * cubic B-spline decomposition (x then y, mirror, tolerance 1e-10) and resampling at
+ * (spacing / factor) * index / spacing. out is out_rows x out_cols = floor(n * factor + 0.5). */
+void orc_downsample_image(const float* img, int32_t rows, int32_t cols, double row_sp, double col_sp,
+                          double factor, double sigma, float* out, int32_t out_rows, int32_t out_cols);
+
+void orc_hu_to_lin_att(const float* hu, int64_t n, double hu_lower, double mu_water, double mu_air, float* out);
+
 #ifdef __cplusplus
 }
 #endif

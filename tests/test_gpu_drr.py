@@ -212,3 +212,28 @@ def test_drr_multiview_extrinsics(ctx, oracle):
     assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
     # the three views differ
     assert not np.allclose(got[0], got[n])
+
+
+def test_hu_volume_matches_host_conversion(ctx, oracle):
+    """A volume created from Hounsfield units (converted on the device) casts the same DRR,
+    bit for bit, as one created from the oracle's host conversion."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    print("case: HU volume c1-small", flush=True)
+    w = synth.config("c1", n_poses=3, small=True)
+    hu = (w.vol / 0.02683 * 1000.0 - 1000.0).astype(np.float32)  # attenuation -> HU
+    att = oracle.hu_to_lin_att(hu)
+    poses = poses_f32(w.poses)
+    c4p = np.zeros(len(poses), np.int64)
+    rows, cols = w.cams[0].num_det_rows, w.cams[0].num_det_cols
+    out = []
+    for kw in ({"hu": {}}, {}):
+        vol = X.Volume(ctx, hu if kw else att, w.idx_to_phys, **kw)
+        buf = ctx.alloc(4 * len(poses) * rows * cols)
+        X.line_int(ctx, vol, w.cams, poses, c4p, w.step, buf)
+        out.append(buf.download(np.float32))
+        buf.free()
+        vol.close()
+    assert out[0].max() > 0
+    assert np.array_equal(out[0], out[1])

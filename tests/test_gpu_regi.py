@@ -1,0 +1,95 @@
+"""Registration drivers on the device objective: objective values against the oracle, the
+exhaustive search's batching and minimum, and a two-level (pyramid) registration."""
+import numpy as np
+import pytest
+
+from tests.helpers import SIM_ABS_TOL, poses_f32
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(oracle, ctx, name="c3", seed=5):
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    w = synth.config(name, n_poses=1, small=True)
+    fixed = []
+    for v, cam in enumerate(w.cams):
+        g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.gt[None]), step=w.step)
+        fixed.append(synth.noisy_fixed(g[0], seed=seed + v))
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    return w, vol, fixed
+
+
+def test_objective_matches_oracle_mean_over_views(ctx, oracle):
+    from xreg_amd import regi
+
+    print("case: objective c3-small 3 views", flush=True)
+    w, vol, fixed = _setup(oracle, ctx)
+    rng = np.random.default_rng(1)
+    init = w.gt
+    obj = regi.Objective(ctx, vol, list(zip(w.cams, fixed)), regi.PoseParam(init, w.center), w.step,
+                         metric=3, max_batch=16)
+    Xp = np.concatenate([np.zeros((1, 6)), rng.normal(0, [0.05] * 3 + [5.0] * 3, (20, 6))])
+    got = obj(Xp)
+    poses = obj.param(Xp)
+    per_view = []
+    for v, cam in enumerate(w.cams):
+        drr, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(poses), step=w.step)
+        per_view.append(oracle.patch_grad_ncc_f32(fixed[v], drr, smooth_ksize=5))
+    want = regi.combine_mean(np.stack(per_view))
+    print(f"max |gpu - oracle| {np.abs(got - want).max():.3e}")
+    assert np.abs(got - want).max() <= SIM_ABS_TOL
+    assert obj.calls == 2 and obj.evals == 21
+    obj.close()
+    vol.close()
+
+
+def test_exhaustive_batches_and_minimum(ctx, oracle):
+    from xreg_amd import regi
+
+    print("case: exhaustive c1-small", flush=True)
+    w, vol, fixed = _setup(oracle, ctx, "c1")
+    obj = regi.Objective(ctx, vol, [(w.cams[0], fixed[0])], regi.PoseParam(w.gt, w.center), w.step, metric=0,
+                         max_batch=32)
+    grid = regi.mesh_grid([[0.0], [0.0], np.deg2rad([-4, 0, 4]), np.linspace(-6, 6, 5), [0.0], np.linspace(-6, 6, 5)])
+    assert grid.shape == (75, 6)
+    r = regi.run_exhaustive(obj, grid, batch=20)
+    allv = r.extra["all_sim_vals"]
+    assert obj.calls == 4 and len(allv) == 75
+    # the same values one at a time
+    one = np.concatenate([obj(grid[i:i + 1]) for i in range(0, 75, 7)])
+    assert np.array_equal(one, allv[::7])
+    assert r.extra["best_index"] == int(np.argmin(allv))
+    assert np.allclose(r.x, 0.0)  # the ground truth is on the grid
+    obj.close()
+    vol.close()
+
+
+def test_two_level_registration_and_refiner(ctx, oracle):
+    from xreg_amd import _xrh as X
+    from xreg_amd import geometry as G
+    from xreg_amd import regi
+
+    print("case: two-level registration c2-small", flush=True)
+    w, vol, fixed = _setup(oracle, ctx, "c2")
+    init = G.perturbed_pose(w.gt, w.center, np.deg2rad([2.0, -2.5, 1.5]), [4.0, -3.0, 3.0])
+    cam = w.cams[0]
+
+    def ds(img, f):
+        return X.downsample_image(ctx, img, f, spacing=(cam.det_row_spacing, cam.det_col_spacing))
+
+    levels = [regi.Level(0.5, "cmaes", dict(lam=20, sigma=[np.deg2rad(3.0)] * 3 + [5.0] * 3, max_gens=30, seed=3)),
+              regi.Level(1.0, "refine", dict(rho_beg=[np.deg2rad(0.5)] * 3 + [1.0] * 3, max_evals=80))]
+    pose, res = regi.run_multi_level(ctx, vol, [cam], fixed, init, w.center, w.step, levels, ds)
+    ext = np.array(w.vol.shape[::-1]) * w.spacing
+    pts = [w.center + 0.25 * ext * np.array(c) for c in [(-1, -1, -1), (1, 1, 1), (1, -1, 1), (-1, 1, -1), (0, 0, 0)]]
+    e0 = G.mean_projection_distance(init, w.gt, pts, 1020.0)
+    e1 = G.mean_projection_distance(res[0].pose, w.gt, pts, 1020.0)
+    e2 = G.mean_projection_distance(pose, w.gt, pts, 1020.0)
+    print(f"mPD {e0:.3f} -> {e1:.3f} (level 0) -> {e2:.3f} (level 1), refine evals {res[1].evals}")
+    assert e1 < 0.5 * e0
+    assert res[1].fun <= res[1].history[0]
+    assert e2 < 0.5 * e0
+    assert res[1].evals <= 80 + 1
+    vol.close()

@@ -1,0 +1,106 @@
+"""Registration drivers on the CPU with a closed-form objective (no GPU): the exhaustive
+search's batching and first-minimum rule, the refiner's BOBYQA-shaped call pattern and
+convergence, CMA-ES convergence, and CombineMean's float arithmetic."""
+import numpy as np
+
+from xreg_amd import geometry as G
+from xreg_amd import regi
+
+
+class QuadObj:
+    """f(x) = sum(a * (x - x*)^2) as float32; records every call's size."""
+
+    def __init__(self, xstar, a, max_batch=16):
+        self.xstar, self.a = np.asarray(xstar, float), np.asarray(a, float)
+        self.max_batch = max_batch
+        self.evals, self.calls, self.sizes = 0, 0, []
+        self.param = regi.PoseParam(np.eye(4), np.zeros(3))
+
+    def __call__(self, X):
+        X = np.atleast_2d(X)
+        out = []
+        for s in range(0, len(X), self.max_batch):
+            c = X[s:s + self.max_batch]
+            self.sizes.append(len(c))
+            self.calls += 1
+            out.append(np.sum(self.a * (c - self.xstar) ** 2, axis=1).astype(np.float32))
+        self.evals += len(X)
+        return np.concatenate(out)
+
+
+def test_exhaustive_first_minimum_and_batches():
+    obj = QuadObj(np.zeros(6), np.ones(6), max_batch=16)
+    grid = regi.mesh_grid([[-1.0, 0.0, 1.0], [0.0], [0.0], [0.0], [0.0], [-1.0, 1.0]])
+    # (-1,..,-1) (−1,..,1) (0,..,-1) (0,..,1) (1,..,-1) (1,..,1): four equal minima at x0 = 0
+    r = regi.run_exhaustive(obj, grid, batch=4)
+    assert obj.sizes == [4, 2]
+    assert r.extra["best_index"] == 2  # the first of the equal minima
+    assert np.array_equal(r.x, grid[2])
+    assert len(r.extra["all_sim_vals"]) == 6
+
+
+def test_mesh_grid_order_first_parameter_most_significant():
+    g = regi.mesh_grid([[0, 1], [10, 20, 30], [5], [6], [7], [8]])
+    assert g.shape == (6, 6)
+    assert list(g[:, 0]) == [0, 0, 0, 1, 1, 1] and list(g[:, 1]) == [10, 20, 30, 10, 20, 30]
+
+
+def test_refiner_call_pattern_and_convergence():
+    xstar = np.array([0.02, -0.03, 0.01, 1.5, -2.0, 0.7])
+    obj = QuadObj(xstar, [400.0, 300.0, 500.0, 1.0, 2.0, 1.5])
+    r = regi.run_local_refine(obj, np.zeros(6), rho_beg=[0.05] * 3 + [2.0] * 3, rho_end=1e-4, max_evals=150)
+    assert obj.sizes[:13] == [1] * 13 and all(s == 1 for s in obj.sizes)  # single-pose evaluations
+    assert r.evals <= 150
+    assert np.allclose(r.x, xstar, atol=1e-2 * np.array([0.05] * 3 + [2.0] * 3))
+    assert r.fun < 1e-4
+
+
+def test_refiner_respects_bounds():
+    obj = QuadObj(np.full(6, 5.0), np.ones(6))
+    bounds = np.array([[-1.0, 1.0]] * 6)
+    r = regi.run_local_refine(obj, np.zeros(6), rho_beg=[0.5] * 6, max_evals=100, bounds=bounds)
+    assert np.all(r.x <= 1.0 + 1e-12) and np.all(r.x >= -1.0 - 1e-12)
+    assert np.allclose(r.x, 1.0, atol=1e-3)
+
+
+def test_cmaes_driver_converges():
+    xstar = np.array([0.05, -0.02, 0.03, 3.0, -4.0, 2.0])
+    obj = QuadObj(xstar, [100.0] * 3 + [1.0] * 3, max_batch=64)
+    r = regi.run_cmaes(obj, lam=20, sigma=[0.1] * 3 + [5.0] * 3, max_gens=200, seed=1)
+    assert all(s == 20 for s in obj.sizes)  # one submission per generation
+    assert np.allclose(r.x, xstar, atol=1e-3)
+
+
+def test_combine_mean_is_float_sum_over_views_divided():
+    v = np.array([[0.1, 0.3], [0.2, 0.7], [0.35, 0.05]], np.float32)
+    got = regi.combine_mean(v)
+    want = np.array([(np.float32(np.float32(np.float32(0.1) + np.float32(0.2)) + np.float32(0.35))) / np.float32(3),
+                     (np.float32(np.float32(np.float32(0.3) + np.float32(0.7)) + np.float32(0.05))) / np.float32(3)],
+                    np.float32)
+    assert got.dtype == np.float32 and np.array_equal(got, want)
+
+
+def test_pose_param_batch_matches_single():
+    init = G.gt_pose(np.array([10.0, 20.0, 30.0]), 510.0)
+    p = regi.PoseParam(init, np.array([10.0, 20.0, 30.0]))
+    X = np.random.default_rng(0).normal(0, [0.1] * 3 + [5.0] * 3, (8, 6))
+    B = p(X)
+    for x, b in zip(X, B):
+        np.testing.assert_allclose(b, p(x), atol=1e-9)
+    np.testing.assert_allclose(p(np.zeros(6)), init, atol=1e-12)
+
+
+def test_downsample_camera_follows_reference_arithmetic():
+    c = G.camera(1020.0, 375, 384, 0.776, 0.776)
+    d = G.downsample_camera(c, 0.5)
+    assert (d.num_det_rows, d.num_det_cols) == (188, 192)  # lround(187.5) = 188
+    K = G.naive_intrins(1020.0, 375, 384, 0.776, 0.776)
+    K2 = K.copy()
+    for r, k in ((0, 0), (1, 1), (0, 2), (1, 2)):
+        K2[r, k] = np.float32(K[r, k] * np.float32(0.5))
+    ref = G.camera_ke(K2, np.eye(4, dtype=np.float32), 188, 192, float(np.float32(0.776) / np.float32(0.5)),
+                      float(np.float32(0.776) / np.float32(0.5)))
+    assert list(d.intrins_inv) == list(ref.intrins_inv) and d.focal_len == ref.focal_len
+    e = G.downsample_camera(c, 0.5, force_even_dims=True)
+    assert (e.num_det_rows, e.num_det_cols) == (188, 192)
+    assert G.lround(2.5) == 3 and G.lround(-2.5) == -3 and G.lround(2.4999) == 2

@@ -98,8 +98,13 @@ SIGNATURES = {
     "xrh_copy_h2d": (_I32, [_P, _P, _P, _U64]),
     "xrh_copy_d2h": (_I32, [_P, _P, _P, _U64]),
     "xrh_copy_d2d": (_I32, [_P, _P, _P, _U64]),
+    "xrh_downsample_size": (_I32, [_I32, _I32, C.c_double, C.POINTER(_I32), C.POINTER(_I32)]),
+    "xrh_downsample_image": (_I32, [_P, _PF, _I32, _I32, C.c_double, C.c_double, C.c_double, C.c_double, _PF,
+                                    _I32, _I32]),
     "xrh_volume_create": (_I32, [_P, _PF, _PI64, _PF, _I32, C.POINTER(_P)]),
     "xrh_volume_create_layout": (_I32, [_P, _PF, _PI64, _PF, _I32, _I32, C.POINTER(_P)]),
+    "xrh_volume_create_hu": (_I32, [_P, _PF, _PI64, _PF, _I32, _I32, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(_P)]),
     "xrh_volume_destroy": (_I32, [_P]),
     "xrh_volume_bytes": (_U64, [_P]),
     "xrh_line_int": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
@@ -215,15 +220,30 @@ class Context:
             self.h = None
 
 
+# HUToLinAttFilter defaults (lib/image/xregHUToLinAtt.h:73-76)
+HU_MU_WATER = 0.02683 * 1.0
+HU_MU_AIR = 0.02485 * 0.0001
+HU_LOWER = -1000.0
+
+
 class Volume:
     def __init__(self, ctx: Context, vox_zyx: np.ndarray, idx_to_phys: np.ndarray, storage: int = XRH_F32,
-                 layout: int | None = None):
+                 layout: int | None = None, hu: dict | None = None):
+        """hu: None for attenuation voxels, or a dict (keys hu_lower, mu_water, mu_air, any
+        subset; the reference's defaults otherwise) for Hounsfield-unit voxels converted on the
+        device (xrh_volume_create_hu)."""
         vox = np.ascontiguousarray(vox_zyx, dtype=np.float32)
         assert vox.ndim == 3
         size = np.array([vox.shape[2], vox.shape[1], vox.shape[0]], dtype=np.int64)
         m = np.ascontiguousarray(idx_to_phys, dtype=np.float32).reshape(16)
         p = _P()
-        if layout is None:
+        if hu is not None:
+            check(lib().xrh_volume_create_hu(ctx.h, _fp(vox), size.ctypes.data_as(_PI64), _fp(m), int(storage),
+                                             -1 if layout is None else int(layout),
+                                             float(hu.get("hu_lower", HU_LOWER)),
+                                             float(hu.get("mu_water", HU_MU_WATER)),
+                                             float(hu.get("mu_air", HU_MU_AIR)), C.byref(p)))
+        elif layout is None:
             check(lib().xrh_volume_create(ctx.h, _fp(vox), size.ctypes.data_as(_PI64), _fp(m), int(storage),
                                           C.byref(p)))
         else:
@@ -383,3 +403,21 @@ def gpu_available() -> bool:
     if os.environ.get("HIP_VISIBLE_DEVICES", None) == "":
         return False
     return Path("/dev/kfd").exists()
+
+
+def downsample_size(rows: int, cols: int, factor: float) -> tuple[int, int]:
+    r, c = _I32(), _I32()
+    check(lib().xrh_downsample_size(int(rows), int(cols), float(factor), C.byref(r), C.byref(c)))
+    return r.value, c.value
+
+
+def downsample_image(ctx: Context, img: np.ndarray, factor: float, sigma: float = -1.0,
+                     spacing: tuple[float, float] = (1.0, 1.0)) -> np.ndarray:
+    """DownsampleImage of a fixed image on the device (xrh_downsample_image); spacing is
+    (row, col) pixel spacing."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    orows, ocols = downsample_size(a.shape[0], a.shape[1], factor)
+    out = np.empty((orows, ocols), dtype=np.float32)
+    check(lib().xrh_downsample_image(ctx.h, _fp(a), a.shape[0], a.shape[1], float(spacing[0]), float(spacing[1]),
+                                     float(factor), float(sigma), _fp(out), orows, ocols))
+    return out

@@ -292,9 +292,25 @@ static int default_layout(int storage)
   return (storage == XRH_F32) ? kLayoutQuad : kLayoutOct;
 }
 
+// HUToLinAttFilter::GenerateData (lib/image/xregHUToLinAtt.cpp:45-68): per voxel, in double,
+// max((hu * hu_scale + mu_water) - mu_lower, 0) rounded to float; std::max keeps a NaN.
+__global__ void hu_to_lin_att_kernel(float* __restrict__ v, int64_t n, double hu_scale, double mu_water,
+                                     double mu_lower)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = ((double)v[i] * hu_scale + mu_water) - mu_lower;
+  v[i] = (float)((a < 0.0) ? 0.0 : a);
+}
+
+struct HuParams {
+  double hu_lower, mu_water, mu_air;
+};
+
 static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
                           const float idx_to_phys[16], int32_t storage, int32_t layout,
-                          xrh_volume** out)
+                          xrh_volume** out, const HuParams* hu = nullptr)
 {
   XRH_REQUIRE(ctx && vox && size && idx_to_phys && out, "null argument");
   XRH_REQUIRE(storage == XRH_F32 || storage == XRH_F16 || storage == XRH_BF16, "bad storage type");
@@ -322,6 +338,14 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
     raw = ctx_alloc(ctx, (uint64_t)(X * Y * Z) * 4);
     XRH_HIP_CHECK(hipMemcpyAsync(raw, vox, (size_t)(X * Y * Z) * 4, hipMemcpyHostToDevice, ctx->stream));
     const int threads = 256;
+    if (hu) {
+      const double hu_scale = (hu->mu_water - hu->mu_air) * 1.0e-3;
+      const double mu_lower = (hu->hu_lower * hu_scale) + hu->mu_water;
+      const int64_t nv = X * Y * Z;
+      hipLaunchKernelGGL(hu_to_lin_att_kernel, dim3((unsigned)((nv + threads - 1) / threads)), dim3(threads), 0,
+                         ctx->stream, (float*)raw, nv, hu_scale, hu->mu_water, mu_lower);
+      XRH_HIP_CHECK(hipGetLastError());
+    }
     const int64_t blocks = (padded + threads - 1) / threads;
     hipLaunchKernelGGL(build_layout_kernel, dim3((unsigned)blocks), dim3(threads), 0, ctx->stream,
                        (const float*)raw, (int)X, (int)Y, (int)Z, v->data, storage, layout);
@@ -609,6 +633,17 @@ xrh_status xrh_volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[
                              const float idx_to_phys[16], int32_t storage, xrh_volume** out)
 {
   return guarded([&] { volume_create(ctx, vox, size, idx_to_phys, storage, -1, out); });
+}
+
+xrh_status xrh_volume_create_hu(xrh_ctx* ctx, const float* vox_hu, const int64_t size[3],
+                                const float idx_to_phys[16], int32_t storage, int32_t layout, double hu_lower,
+                                double mu_water, double mu_air, xrh_volume** out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(layout >= -1 && layout <= 2, "bad layout");
+    const HuParams hu{hu_lower, mu_water, mu_air};
+    volume_create(ctx, vox_hu, size, idx_to_phys, storage, layout, out, &hu);
+  });
 }
 
 xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_t size[3],

@@ -82,6 +82,9 @@ def camera(focal: float, rows: int, cols: int, row_sp: float, col_sp: float,
         c.pinhole[i] = float(einv[i, 3])
     c.det_row_spacing = float(F(row_sp))
     c.det_col_spacing = float(F(col_sp))
+    # the float intrinsics / extrinsics CameraModel keeps (used by downsample_camera)
+    c.K = naive_intrins(focal, rows, cols, row_sp, col_sp)
+    c.E = np.eye(4, dtype=F) if extrins is None else np.asarray(extrins, dtype=F)
     return c
 
 
@@ -106,6 +109,8 @@ def camera_ke(K: np.ndarray, E: np.ndarray, rows: int, cols: int, row_sp: float,
         c.pinhole[i] = float(einv[i, 3])
     c.det_row_spacing = float(F(row_sp))
     c.det_col_spacing = float(F(col_sp))
+    c.K = K.copy()
+    c.E = np.asarray(E, dtype=F).copy()
     return c
 
 
@@ -117,32 +122,33 @@ def camera_arrays(c: Camera):
     return kinv, einv, ph
 
 
-def downsample_camera(c: Camera, ds: float) -> Camera:
-    """DownsampleCameraModel (xregPerspectiveXform.cpp:654-688): scales the intrinsics,
-    rounds the detector size and divides the pixel spacing by ds."""
-    kinv, einv, _ = camera_arrays(c)
-    K = np.linalg.inv(kinv.astype(np.float64))
-    K[0, 0] *= ds
-    K[1, 1] *= ds
-    K[0, 2] *= ds
-    K[1, 2] *= ds
-    rows = int(round(c.num_det_rows * ds))
-    cols = int(round(c.num_det_cols * ds))
-    out = Camera()
-    out.num_det_rows = rows
-    out.num_det_cols = cols
-    out.coord_frame = c.coord_frame
-    out.focal_len = c.focal_len
-    kinv2 = _mat3_inverse_f32(K.astype(F))
-    for i in range(9):
-        out.intrins_inv[i] = float(kinv2.reshape(-1)[i])
-    for i in range(16):
-        out.extrins_inv[i] = c.extrins_inv[i]
-    for i in range(3):
-        out.pinhole[i] = c.pinhole[i]
-    out.det_row_spacing = float(F(c.det_row_spacing / ds))
-    out.det_col_spacing = float(F(c.det_col_spacing / ds))
-    return out
+def lround(x: float) -> int:
+    """std::lround: nearest integer, halves away from zero."""
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def downsample_camera(c: Camera, ds: float, force_even_dims: bool = False) -> Camera:
+    """DownsampleCameraModel (xregPerspectiveXform.cpp:654-688): the float intrinsics scaled by
+    ds (entries (0,0), (1,1), (0,2), (1,2)), detector size lround(n * ds) (made even on request),
+    pixel spacing / ds, then CameraModel::setup(intrins, extrins, ...). Cameras made by
+    camera() / camera_ke() carry their float K and E; for others K is recovered from the inverse
+    intrinsics and the extrinsics from the inverse extrinsics."""
+    ds32 = F(ds)
+    if hasattr(c, "K"):
+        K = np.asarray(c.K, dtype=F).copy()
+        E = np.asarray(c.E, dtype=F)
+    else:
+        kinv, einv, _ = camera_arrays(c)
+        K = np.linalg.inv(kinv.astype(np.float64)).astype(F)
+        E = np.linalg.inv(einv.astype(np.float64)).astype(F)
+    for (r, k) in ((0, 0), (1, 1), (0, 2), (1, 2)):
+        K[r, k] = F(K[r, k] * ds32)
+    rows = lround(float(F(F(c.num_det_rows) * ds32)))
+    cols = lround(float(F(F(c.num_det_cols) * ds32)))
+    if force_even_dims:
+        rows -= rows % 2
+        cols -= cols % 2
+    return camera_ke(K, E, rows, cols, float(F(F(c.det_row_spacing) / ds32)), float(F(F(c.det_col_spacing) / ds32)))
 
 
 # ------------------------------------------------------------------------------------------

@@ -1,0 +1,269 @@
+"""Registration drivers over the batched objective: CMA-ES, exhaustive (grid) search, a
+BOBYQA-pattern local refiner, and a multi-resolution driver.
+
+The objective follows Intensity2D3DRegi::obj_fn (lib/regi/interfaces_2d_3d/
+xregIntensity2D3DRegi.cpp:571-742): optimiser vectors -> poses (SE3OptVarsLieAlg,
+lib/regi/xregSE3OptVars.cpp:127-137, applied about a rotation centre on the volume side of a
+starting pose), all views x all candidates cast and scored in one batched submission
+(view-major, :605-621), then the per-view values combined by ImgSimMetric2DCombineMean
+(lib/regi/sim_metrics_2d/xregImgSimMetric2DCombine.cpp:79-98: float sum in view order, divided
+by the number of views).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _xrh as X
+from . import geometry as G
+from .cmaes import CMAES
+
+
+def combine_mean(per_view: np.ndarray) -> np.ndarray:
+    """ImgSimMetric2DCombineMean: float32 sum over views (axis 0, in order) / number of views."""
+    per_view = np.asarray(per_view, dtype=np.float32)
+    acc = np.zeros(per_view.shape[1], dtype=np.float32)
+    for v in range(per_view.shape[0]):
+        acc = (acc + per_view[v]).astype(np.float32)
+    return (acc / np.float32(per_view.shape[0])).astype(np.float32)
+
+
+class PoseParam:
+    """x (6: rotation [rad], translation [mm]) -> cam->phys pose: exp(x) about `center`,
+    composed on the volume side of `init_pose`."""
+
+    def __init__(self, init_pose: np.ndarray, center):
+        self.init = np.asarray(init_pose, dtype=np.float64)
+        self.center = np.asarray(center, dtype=np.float64)
+        self._post = G.rigid(None, -self.center) @ self.init
+        self._pre = G.rigid(None, self.center)
+
+    def __call__(self, Xp: np.ndarray) -> np.ndarray:
+        Xp = np.asarray(Xp, dtype=np.float64)
+        if Xp.ndim == 1:
+            return self._pre @ G.exp_se3(Xp) @ self._post
+        return self._pre @ G.exp_se3_batch(Xp) @ self._post
+
+
+class Objective:
+    """Multi-view similarity of candidate parameter vectors, one batched submission per call.
+
+    views: list of (camera, fixed image) pairs sharing one volume. Fixed images are processed
+    once (xrh_fixed_create); every call casts len(X) x V DRRs and scores them."""
+
+    def __init__(self, ctx: X.Context, vol: X.Volume, views, param: PoseParam, step: float,
+                 metric: int = X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize: int = 5, patch_radius: int = 5,
+                 max_batch: int = 256, geom: int = X.XRH_GEOM_FP32, use_graph: bool = True):
+        self.ctx, self.vol, self.param, self.step = ctx, vol, param, float(step)
+        self.fixed = [X.Fixed(ctx, img, metric, smooth_ksize=smooth_ksize, patch_radius=patch_radius)
+                      for _, img in views]
+        self.cams = [c for c, _ in views]
+        self.V = len(views)
+        self.max_batch = int(max_batch)
+        self.batch = X.Batch(ctx, [X.make_view(vol, f, c, step) for c, f in zip(self.cams, self.fixed)],
+                             max_items=self.V * self.max_batch, geom=geom)
+        self.batch.set_options(use_graph=use_graph, time_kernels=False)
+        self.evals = 0        # candidate poses evaluated
+        self.calls = 0        # batched submissions
+
+    def eval_poses(self, poses: np.ndarray) -> np.ndarray:
+        poses = np.asarray(poses, dtype=np.float64).reshape(-1, 4, 4)
+        out = np.empty(len(poses), dtype=np.float32)
+        for s in range(0, len(poses), self.max_batch):
+            p = poses[s:s + self.max_batch]
+            k = len(p)
+            iv = np.repeat(np.arange(self.V, dtype=np.int32), k)
+            sims = self.batch.eval(iv, np.tile(p, (self.V, 1, 1)))
+            out[s:s + k] = combine_mean(sims.reshape(self.V, k))
+            self.calls += 1
+        self.evals += len(poses)
+        return out
+
+    def __call__(self, Xp: np.ndarray) -> np.ndarray:
+        Xp = np.atleast_2d(np.asarray(Xp, dtype=np.float64))
+        return self.eval_poses(self.param(Xp))
+
+    def close(self):
+        self.batch.close()
+        for f in self.fixed:
+            f.close()
+
+
+# ------------------------------------------------------------------------------------------
+# drivers
+
+
+@dataclass
+class RegiResult:
+    x: np.ndarray
+    fun: float
+    pose: np.ndarray
+    evals: int
+    history: list = field(default_factory=list)
+    extra: dict = field(default_factory=dict)
+
+
+def run_cmaes(obj: Objective, lam: int, sigma, max_gens: int, seed: int, x0=None, bounds=None,
+              tol_fun: float = 1e-12, tol_x: float = 1e-11) -> RegiResult:
+    """Intensity2D3DRegiCMAES::run (xregIntensity2D3DRegiCMAES.cpp:75-261): lambda candidates per
+    generation evaluated in one submission, bounds by resampling."""
+    x0 = np.zeros(6) if x0 is None else np.asarray(x0, dtype=np.float64)
+    es = CMAES(x0, np.asarray(sigma, dtype=np.float64), lam, seed,
+               bounds=None if bounds is None else np.asarray(bounds, dtype=np.float64), tol_fun=tol_fun, tol_x=tol_x)
+    best_f, best_x = np.inf, x0.copy()
+    e0 = obj.evals
+    for _ in range(max_gens):
+        Xc = es.ask()
+        f = obj(np.asarray(Xc)).astype(np.float64)
+        k = int(np.argmin(f))
+        if f[k] < best_f:
+            best_f, best_x = float(f[k]), np.asarray(Xc[k]).copy()
+        es.tell(f)
+        if es.stop():
+            break
+    return RegiResult(best_x, best_f, obj.param(best_x), obj.evals - e0, list(es.history))
+
+
+def mesh_grid(axes) -> np.ndarray:
+    """All combinations of per-parameter sample vectors, first parameter most significant
+    (row-major, as the reference's mesh-grid iterator)."""
+    mg = np.meshgrid(*[np.asarray(a, dtype=np.float64) for a in axes], indexing="ij")
+    return np.stack([m.reshape(-1) for m in mg], axis=1)
+
+
+def run_exhaustive(obj: Objective, candidates: np.ndarray, batch: int | None = None,
+                   save_all: bool = True) -> RegiResult:
+    """Intensity2D3DRegiExhaustive::run (xregIntensity2D3DRegiExhaustive.cpp:77-300): the candidate
+    list is evaluated in consecutive batches of `batch` (the objective's batch size by
+    default); the minimum is updated only on a strictly smaller value, so the first of equal
+    minima wins; optionally every value is kept."""
+    cand = np.asarray(candidates, dtype=np.float64).reshape(-1, 6)
+    b = obj.max_batch if batch is None else int(batch)
+    all_vals = []
+    best_f, best_i = np.float32(np.finfo(np.float32).max), -1
+    e0 = obj.evals
+    for s in range(0, len(cand), b):
+        vals = obj(cand[s:s + b])
+        for j, v in enumerate(vals):
+            if best_f > v:
+                best_f, best_i = v, s + j
+        if save_all:
+            all_vals.append(vals)
+    x = cand[best_i] if best_i >= 0 else np.zeros(6)
+    return RegiResult(x, float(best_f), obj.param(x), obj.evals - e0,
+                      extra={"all_sim_vals": np.concatenate(all_vals) if all_vals else np.zeros(0, np.float32),
+                             "best_index": best_i})
+
+
+def run_local_refine(obj: Objective, x0, rho_beg, rho_end: float = 1e-3, max_evals: int = 200,
+                     bounds=None) -> RegiResult:
+    """A derivative-free trust-region refiner with BOBYQA's evaluation pattern
+    (Intensity2D3DRegiBOBYQA, xregIntensity2D3DRegiBOBYQA.cpp:29-39, through NLopt): 2n+1
+    initial single-pose evaluations (x0, x0 +- rho e_i), then one single-pose evaluation per
+    iteration. The model is a quadratic with diagonal curvature fitted (least squares) to the
+    best 2n+1..4n points seen; the step minimises it within a box trust region (scaled by
+    rho_beg per parameter) and the region follows the ratio of actual to predicted decrease.
+    NLopt's BOBYQA (Powell's full quadratic model, Powell 2009) is not reproduced; this
+    refiner keeps its calling pattern and bound handling."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    n = x0.size
+    scale = np.broadcast_to(np.asarray(rho_beg, dtype=np.float64), (n,)).copy()
+    lo = hi = None
+    if bounds is not None:
+        bnd = np.asarray(bounds, dtype=np.float64).reshape(n, 2)
+        lo, hi = bnd[:, 0], bnd[:, 1]
+
+    def clip(x):
+        return x if lo is None else np.minimum(np.maximum(x, lo), hi)
+
+    e0 = obj.evals
+    pts, vals = [], []
+
+    def f(x):
+        v = float(obj(x[None])[0])
+        pts.append(x.copy())
+        vals.append(v)
+        return v
+
+    # initial stencil, one pose per evaluation
+    fx = f(clip(x0))
+    for i in range(n):
+        for sgn in (1.0, -1.0):
+            e = np.zeros(n)
+            e[i] = sgn * scale[i]
+            f(clip(x0 + e))
+    delta = 1.0           # trust region radius in units of scale
+    best = int(np.argmin(vals))
+    xb, fb = pts[best].copy(), vals[best]
+    hist = [fx]
+    while obj.evals - e0 < max_evals and delta > rho_end:
+        # fit g, diag(H) about xb on the nearest points (scaled coordinates)
+        P = (np.asarray(pts) - xb) / scale
+        d2 = np.sum(P * P, axis=1)
+        m = min(len(pts), 4 * n)
+        idx = np.argsort(d2)[:m]
+        A = np.concatenate([np.ones((m, 1)), P[idx], 0.5 * P[idx] ** 2], axis=1)
+        coef, *_ = np.linalg.lstsq(A, np.asarray(vals)[idx] - fb, rcond=None)
+        g, h = coef[1:n + 1], coef[n + 1:]
+        s = np.where(h > 1e-12, -g / np.maximum(h, 1e-12), -np.sign(g) * delta)
+        s = np.clip(s, -delta, delta)
+        pred = -(g @ s + 0.5 * np.sum(h * s * s))
+        xn = clip(xb + s * scale)
+        if not np.any(np.abs(xn - xb) > 0) or pred <= 0:
+            delta *= 0.5
+            continue
+        fn = f(xn)
+        ratio = (fb - fn) / pred
+        if fn < fb:
+            xb, fb = xn, fn
+        delta = min(2.0 * delta, 2.0) if ratio > 0.7 else (delta if ratio > 0.1 else 0.5 * delta)
+        hist.append(fb)
+    return RegiResult(xb, fb, obj.param(xb), obj.evals - e0, hist)
+
+
+# ------------------------------------------------------------------------------------------
+# multi-resolution
+
+
+@dataclass
+class Level:
+    ds_factor: float                   # detector downsampling factor (1 = full resolution)
+    method: str = "cmaes"              # "cmaes" | "exhaustive" | "refine"
+    opts: dict = field(default_factory=dict)
+
+
+def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels, downsample_image,
+                    metric: int = X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize: int = 5, patch_radius: int = 5,
+                    geom: int = X.XRH_GEOM_FP32, max_batch: int = 256):
+    """MultiLevelMultiObjRegi::run (lib/regi/interfaces_2d_3d/xregMultiObjMultiLevel2D3DRegi.cpp:
+    164-531), single object: at each level the cameras are downsampled (DownsampleCameraModel)
+    and the fixed images with `downsample_image(img, ds)`; the level's registration starts
+    from the previous level's estimate, with its own objective (caster + metrics)."""
+    pose = np.asarray(init_pose, dtype=np.float64)
+    results = []
+    for lvl in levels:
+        ds = float(lvl.ds_factor)
+        lcams = [c if ds == 1.0 else G.downsample_camera(c, ds) for c in cams]
+        limgs = [img if ds == 1.0 else downsample_image(img, ds) for img in fixed_imgs]
+        for c, im in zip(lcams, limgs):
+            if im.shape != (c.num_det_rows, c.num_det_cols):
+                raise ValueError(f"level {ds}: image {im.shape} does not match camera "
+                                 f"{c.num_det_rows}x{c.num_det_cols}")
+        obj = Objective(ctx, vol, list(zip(lcams, limgs)), PoseParam(pose, center), step, metric=metric,
+                        smooth_ksize=smooth_ksize, patch_radius=patch_radius, geom=geom, max_batch=max_batch)
+        try:
+            o = dict(lvl.opts)
+            if lvl.method == "cmaes":
+                r = run_cmaes(obj, **o)
+            elif lvl.method == "exhaustive":
+                r = run_exhaustive(obj, **o)
+            elif lvl.method == "refine":
+                r = run_local_refine(obj, np.zeros(6), **o)
+            else:
+                raise ValueError(lvl.method)
+        finally:
+            obj.close()
+        results.append(r)
+        pose = r.pose
+    return pose, results
