@@ -66,7 +66,7 @@ def cpu_baseline(w, fixed, poses, budget_s, smooth):
     from oracle import oracle as O
 
     O.lib()
-    threads = os.cpu_count() or 1
+    threads = min(len(os.sched_getaffinity(0)), 16)
     t0 = time.perf_counter()
     done = 0
     t_drr = t_sim = 0.0

@@ -16,6 +16,7 @@
 #include <math.h>
 #include <float.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 #include <unistd.h>
@@ -51,8 +52,11 @@ static void orc_parallel_for(orc_range_fn fn, void* ctx, int64_t n, int64_t grai
 {
   if (n <= 0) return;
   if (threads <= 0) {
-    long hw = sysconf(_SC_NPROCESSORS_ONLN);
-    threads = (hw > 0) ? (int32_t)hw : 1;
+    /* the CPUs this process may run on, at most 16 */
+    cpu_set_t set;
+    long hw = (sched_getaffinity(0, sizeof(set), &set) == 0) ? (long)CPU_COUNT(&set)
+                                                             : sysconf(_SC_NPROCESSORS_ONLN);
+    threads = (hw > 0) ? (int32_t)(hw < 16 ? hw : 16) : 1;
   }
   if (grain < 1) grain = 1;
   orc_pool_job job = {fn, ctx, n, grain, 0};

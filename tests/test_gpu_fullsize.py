@@ -1,0 +1,132 @@
+"""Full-size workloads (384^2 / 768^2 detectors, 256x256x200 and 512x512x400 volumes): parity
+with the oracle on a few poses, and size-independent properties on whole generations
+(exact scaling, batch = plain, half-precision storage bounds)."""
+import numpy as np
+import pytest
+
+from tests.helpers import DRR_REL_TOL, SIM_ABS_TOL, drr_rel_err, poses_f32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2full():
+    from xreg_amd import synth
+
+    return synth.config("c2", n_poses=100)
+
+
+def _drr(ctx, w, vol, poses, geom=0):
+    from xreg_amd import _xrh as X
+
+    cam = w.cams[0]
+    n = len(poses)
+    buf = ctx.alloc(4 * n * cam.num_det_rows * cam.num_det_cols)
+    X.line_int(ctx, vol, [cam], poses_f32(poses), np.zeros(n, np.int64), w.step, buf, geom=geom)
+    out = buf.download(np.float32).reshape(n, cam.num_det_rows, cam.num_det_cols)
+    buf.free()
+    return out
+
+
+def test_c2_full_drr_matches_oracle(ctx, oracle, c2full):
+    from xreg_amd import _xrh as X
+
+    print("case: c2 full size DRR vs oracle, 3 poses", flush=True)
+    w = c2full
+    poses = np.concatenate([w.gt[None], w.poses[:2]])
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    got = _drr(ctx, w, vol, poses)
+    vol.close()
+    ref, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(poses), step=w.step)
+    err = drr_rel_err(got, ref)
+    print(f"max rel err {err.max():.3e}")
+    assert err.max() <= DRR_REL_TOL
+
+
+def test_c2_full_scaling_is_exact(ctx, c2full):
+    """Doubling every voxel doubles every sample and every sum exactly (powers of two commute
+    with rounding), so DRR(2V) == 2 DRR(V) bit for bit over a whole generation."""
+    from xreg_amd import _xrh as X
+
+    print("case: c2 full size DRR(2V) == 2 DRR(V), 100 poses", flush=True)
+    w = c2full
+    v1 = X.Volume(ctx, w.vol, w.idx_to_phys)
+    a = _drr(ctx, w, v1, w.poses)
+    v1.close()
+    v2 = X.Volume(ctx, 2.0 * w.vol, w.idx_to_phys)
+    b = _drr(ctx, w, v2, w.poses)
+    v2.close()
+    assert a.max() > 0
+    assert np.array_equal(b, 2.0 * a)
+
+
+def test_c2_full_batch_equals_plain(ctx, oracle, c2full):
+    """A lambda = 100 generation through the batched submission equals DRR + similarity one pose
+    at a time, and the similarity of the first poses matches the oracle."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    print("case: c2 full size batch vs plain, 100 poses", flush=True)
+    w = c2full
+    cam = w.cams[0]
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    g = _drr(ctx, w, vol, w.gt[None])
+    fixed = synth.noisy_fixed(g[0])
+    fx = X.Fixed(ctx, fixed, X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize=5, patch_radius=5)
+    batch = X.Batch(ctx, [X.make_view(vol, fx, cam, w.step)], max_items=100)
+    sims = batch.eval(np.zeros(100, np.int32), w.poses)
+    buf = ctx.alloc(4 * cam.num_det_rows * cam.num_det_cols)
+    plain = np.empty(100, np.float32)
+    for i in range(100):
+        X.line_int(ctx, vol, [cam], poses_f32(w.poses[i:i + 1]), np.zeros(1, np.int64), w.step, buf)
+        plain[i] = fx.compute(buf, 1)[0]
+    buf.free()
+    assert np.array_equal(sims, plain)
+    drr, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(w.poses[:2]), step=w.step)
+    want = oracle.patch_grad_ncc_f32(fixed, drr, smooth_ksize=5)
+    print(f"sim vs oracle {np.abs(sims[:2] - want).max():.3e}")
+    assert np.abs(sims[:2] - want).max() <= SIM_ABS_TOL
+    batch.close()
+    fx.close()
+    vol.close()
+
+
+@pytest.fixture(scope="module")
+def c4full():
+    from xreg_amd import synth
+
+    return synth.config("c4", n_poses=4)
+
+
+def test_c4_full_drr_matches_oracle(ctx, oracle, c4full):
+    from xreg_amd import _xrh as X
+
+    print("case: c4 full size DRR vs oracle, 1 pose", flush=True)
+    w = c4full
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    got = _drr(ctx, w, vol, w.poses[:1])
+    vol.close()
+    ref, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(w.poses[:1]), step=w.step)
+    err = drr_rel_err(got, ref)
+    print(f"max rel err {err.max():.3e}")
+    assert err.max() <= DRR_REL_TOL
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_c4_full_half_storage_bound(ctx, c4full, storage):
+    """FP16 / BF16 storage against FP32 storage on the same device geometry: the error of a line
+    integral is bounded by the voxel rounding (2^-11 / 2^-8) times the image maximum."""
+    from xreg_amd import _xrh as X
+
+    print(f"case: c4 full size storage={storage}, 4 poses", flush=True)
+    w = c4full
+    v32 = X.Volume(ctx, w.vol, w.idx_to_phys)
+    a = _drr(ctx, w, v32, w.poses, geom=1)
+    v32.close()
+    vh = X.Volume(ctx, w.vol, w.idx_to_phys, storage=storage)
+    b = _drr(ctx, w, vh, w.poses, geom=1)
+    vh.close()
+    err = np.abs(b.astype(np.float64) - a) / float(a.max())
+    bound = 2.0 ** -11 if storage == 1 else 2.0 ** -8
+    print(f"max |err| / max {err.max():.3e} (bound {bound:.2e})")
+    assert err.max() < bound
