@@ -211,16 +211,20 @@ def main():
         ctx.synchronize()
         plain_value = n_plain / (time.perf_counter() - t0)
 
-    if args.calib_copy_mib > 0:
-        # a plain copy kernel of known size (xrh_copy_kernel): reads and writes
-        # calib_copy_mib MiB per dispatch
-        src = ctx.alloc(args.calib_copy_mib << 20)
-        dst = ctx.alloc(args.calib_copy_mib << 20)
-        src.upload(np.ones((args.calib_copy_mib << 20) // 4, np.float32))
-        for _ in range(3):
-            dst.copy_from(src)
-        src.free()
-        dst.free()
+    # STREAM-style copy probe with the library's copy kernel (xrh_copy_kernel): measured HBM
+    # bandwidth next to the published peak; under --pmc it is also the counters' calibration
+    # (a dispatch that reads and writes exactly copy_mib MiB)
+    copy_mib = args.calib_copy_mib if args.calib_copy_mib > 0 else 1024
+    src = ctx.alloc(copy_mib << 20)
+    dst = ctx.alloc(copy_mib << 20)
+    src.upload(np.ones((copy_mib << 20) // 4, np.float32))
+    dst.copy_from(src)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_from(src)
+    copy_gbs = 5 * 2.0 * (copy_mib << 20) / (time.perf_counter() - t0) / 1e9
+    src.free()
+    dst.free()
 
     served = None
     if args.serve_jobs > 0:
@@ -254,7 +258,8 @@ def main():
                      "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4),
                      "samples_per_launch": int(S), "samples_evaluated_per_launch": int(S_eval),
                      "algorithmic_bytes_per_launch": int(bytes_launch),
-                     "evaluated_gbs": round((8.0 * bv * S_eval + 4.0 * P * lam) / (drr_ms_avg * 1e-3) / 1e9, 1)},
+                     "evaluated_gbs": round((8.0 * bv * S_eval + 4.0 * P * lam) / (drr_ms_avg * 1e-3) / 1e9, 1),
+                     "measured_copy_gbs": round(copy_gbs, 1)},
         "empty_space_skipping": not args.no_skip,
         "served": served,
         "cpu_baseline": cpu,

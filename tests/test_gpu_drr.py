@@ -41,7 +41,7 @@ def c1s():
     return synth.config("c1", n_poses=6, small=True)
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 3])
 def test_drr_fp32_matches_cpu_reference(ctx, oracle, c1s, layout):
     print(f"case: c1-small layout={layout}", flush=True)
     poses = np.concatenate([c1s.gt[None], c1s.poses])
@@ -237,3 +237,14 @@ def test_hu_volume_matches_host_conversion(ctx, oracle):
         vol.close()
     assert out[0].max() > 0
     assert np.array_equal(out[0], out[1])
+
+
+def test_fp32_layouts_agree(ctx, c1s):
+    """The coefficient layout evaluates the same trilinear interpolant as the corner layouts
+    (rounding aside): DRRs agree to 2e-6 of the image maximum."""
+    print("case: layouts 0/1/3 agree c1-small", flush=True)
+    poses = np.concatenate([c1s.gt[None], c1s.poses[:3]])
+    outs = [_gpu_drr(ctx, c1s, poses, layout=l) for l in (0, 1, 3)]
+    m = float(outs[0].max())
+    for o in outs[1:]:
+        assert np.abs(o.astype(np.float64) - outs[0]).max() <= 2e-6 * m

@@ -320,6 +320,16 @@ __device__ inline float sample(const VolView& v, float x, float y, float z)
 #pragma unroll
     for (int i = 0; i < 8; ++i) f[i] = (kFmt == 2) ? h2f(r[i]) : bf2f(r[i]);
     return tri(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], wx, wy, wz);
+  } else if constexpr (kFmt == 6) {
+    // trilinear polynomial coefficients of the cell: 2 adjacent 16-byte loads, 7 FMAs
+    const uint32_t ob = e * 32u;
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)ob, 0, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(ob + 16u), 0, 0);
+    const float p0 = fmaf(wy, fmaf(wx, __uint_as_float(a[3]), __uint_as_float(a[2])),
+                          fmaf(wx, __uint_as_float(a[1]), __uint_as_float(a[0])));
+    const float p1 = fmaf(wy, fmaf(wx, __uint_as_float(b[3]), __uint_as_float(b[2])),
+                          fmaf(wx, __uint_as_float(b[1]), __uint_as_float(b[0])));
+    return fmaf(wz, p1, p0);
   } else {
     const int o = (int)(e * 16u);
     const auto q = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, o, 0, 0);
@@ -538,13 +548,16 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
         }
         if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
       } else {
+        // sample indices as floats (exact: n < 2^24), advanced by float additions
+        float f0 = (float)i;
         for (; i < last; i += 2) {
-          const float f0 = (float)i, f1 = (float)(i + 1);
+          const float f1 = f0 + 1.0f;
           const float s0 = sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
                                         fmaf(f0, ray.dz, ray.sz));
           const float s1 = sample<kFmt>(vv, fmaf(f1, ray.dx, ray.sx), fmaf(f1, ray.dy, ray.sy),
                                         fmaf(f1, ray.dz, ray.sz));
           acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
+          f0 = f0 + 2.0f;
         }
         if (i == last) {
           const float f0 = (float)i;
@@ -629,7 +642,7 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
   hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev,   \
                      L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,        \
                      L.ncc_partials, L.ncc_tiles_per_item)
-  const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutQuad ? 1 : 0)
+  const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutCoef ? 6 : (L.layout == kLayoutQuad ? 1 : 0))
                 : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
                                          : (L.layout == kLayoutOct ? 5 : 4);
   switch (fmt) {
@@ -638,7 +651,8 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
     case 2: XRH_LAUNCH_FMT(2); break;
     case 3: XRH_LAUNCH_FMT(3); break;
     case 4: XRH_LAUNCH_FMT(4); break;
-    default: XRH_LAUNCH_FMT(5); break;
+    case 5: XRH_LAUNCH_FMT(5); break;
+    default: XRH_LAUNCH_FMT(6); break;
   }
 #undef XRH_LAUNCH_FMT
 }

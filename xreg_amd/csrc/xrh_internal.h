@@ -67,7 +67,10 @@ enum Layout : int32_t {
   kLayoutLinear = 0,  // padded x-fastest scalars: 8 loads of b_v bytes per sample
   kLayoutQuad = 1,    // f32 only: float4 (v[i,j,k], v[i+1,j,k], v[i,j+1,k], v[i+1,j+1,k]);
                       //   2 x 16-byte loads per sample (4x memory)
-  kLayoutOct = 2      // f16/bf16 only: all 8 corners in 16 bytes; 1 load per sample
+  kLayoutOct = 2,     // f16/bf16 only: all 8 corners in 16 bytes; 1 load per sample
+  kLayoutCoef = 3     // f32 only: the 8 coefficients of the cell's trilinear polynomial in
+                      //   its local coordinates (32 bytes, 2 adjacent 16-byte loads per
+                      //   sample, 8x memory); evaluation is 7 FMAs instead of 7 lerps
 };
 
 struct CamDev {
@@ -125,6 +128,7 @@ struct VolDesc {
   int32_t storage;
   int32_t layout;
   float min_spacing;  // smallest voxel edge (physical units), bounds the ray step from below
+  float max_chord;    // upper bound of any chord through the index box (physical units)
   int32_t pad_;
   // empty-space map: Chebyshev distance (in bricks of 2^skip_shift voxels, capped at 255)
   // from each brick to the nearest brick holding a non-zero voxel within one voxel of it;
@@ -138,6 +142,10 @@ inline void check_step(float step, const VolDesc& v)
 {
   if (!(step >= 1.0e-3f * v.min_spacing)) {
     throw Error{XRH_ERR_INVALID, "ray step must be at least 1e-3 of the smallest voxel spacing"};
+  }
+  // sample indices stay below 2^23 (the kernels count samples in float)
+  if (!((double)v.max_chord / (double)step < 8388608.0)) {
+    throw Error{XRH_ERR_INVALID, "ray step too small for the volume: over 2^23 samples per ray"};
   }
 }
 

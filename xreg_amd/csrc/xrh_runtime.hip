@@ -202,6 +202,18 @@ __global__ void build_layout_kernel(const float* __restrict__ src, int X, int Y,
     q.z = V(a, b + 1, c);
     q.w = V(a + 1, b + 1, c);
     ((float4*)dst)[i] = q;
+  } else if (layout == kLayoutCoef) {
+    // f(u, v, w) = c0 + c1 u + c2 v + c3 u v + w (c4 + c5 u + c6 v + c7 u v) over the cell's
+    // corners, differences taken in double and rounded once
+    const double v000 = V(a, b, c), v100 = V(a + 1, b, c), v010 = V(a, b + 1, c), v110 = V(a + 1, b + 1, c);
+    const double v001 = V(a, b, c + 1), v101 = V(a + 1, b, c + 1), v011 = V(a, b + 1, c + 1),
+                 v111 = V(a + 1, b + 1, c + 1);
+    const double c3 = (v110 - v010) - (v100 - v000);
+    const double c7 = ((v111 - v011) - (v101 - v001)) - c3;
+    float4* o = (float4*)dst + 2 * i;
+    o[0] = make_float4((float)v000, (float)(v100 - v000), (float)(v010 - v000), (float)c3);
+    o[1] = make_float4((float)(v001 - v000), (float)((v101 - v001) - (v100 - v000)),
+                       (float)((v011 - v001) - (v010 - v000)), (float)c7);
   } else {
     const float f[8] = {V(a, b, c),     V(a + 1, b, c),     V(a, b + 1, c),     V(a + 1, b + 1, c),
                         V(a, b, c + 1), V(a + 1, b, c + 1), V(a, b + 1, c + 1), V(a + 1, b + 1, c + 1)};
@@ -289,7 +301,9 @@ static int default_layout(int storage)
 {
   const char* env = std::getenv("XRH_VOLUME_LAYOUT");
   if (env && std::strcmp(env, "linear") == 0) return kLayoutLinear;
-  return (storage == XRH_F32) ? kLayoutQuad : kLayoutOct;
+  if (env && std::strcmp(env, "quad") == 0 && storage == XRH_F32) return kLayoutQuad;
+  // FP32: trilinear coefficients (falls back to quad when the layout would pass 4 GiB)
+  return (storage == XRH_F32) ? kLayoutCoef : kLayoutOct;
 }
 
 // HUToLinAttFilter::GenerateData (lib/image/xregHUToLinAtt.cpp:45-68): per voxel, in double,
@@ -315,12 +329,17 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
   XRH_REQUIRE(ctx && vox && size && idx_to_phys && out, "null argument");
   XRH_REQUIRE(storage == XRH_F32 || storage == XRH_F16 || storage == XRH_BF16, "bad storage type");
   for (int k = 0; k < 3; ++k) XRH_REQUIRE(size[k] >= 1 && size[k] < (1 << 20), "bad volume size");
-  if (layout < 0) layout = default_layout(storage);
-  if (layout == kLayoutQuad) XRH_REQUIRE(storage == XRH_F32, "quad layout needs FP32 storage");
+  const bool auto_layout = layout < 0;
+  if (auto_layout) layout = default_layout(storage);
+  if (layout == kLayoutQuad || layout == kLayoutCoef) XRH_REQUIRE(storage == XRH_F32, "layout needs FP32 storage");
   if (layout == kLayoutOct) XRH_REQUIRE(storage != XRH_F32, "oct layout needs 16-bit storage");
   const int64_t X = size[0], Y = size[1], Z = size[2];
   const int64_t padded = (X + 2) * (Y + 2) * (Z + 2);
-  const int64_t esize = (layout == kLayoutLinear) ? (storage == XRH_F32 ? 4 : 2) : 16;
+  if (layout == kLayoutCoef && (uint64_t)padded * 32ull >= 0xFFFFF000ull) {
+    XRH_REQUIRE(auto_layout, "coefficient layout exceeds 4 GiB; use the quad layout");
+    layout = kLayoutQuad;
+  }
+  const int64_t esize = (layout == kLayoutLinear) ? (storage == XRH_F32 ? 4 : 2) : (layout == kLayoutCoef ? 32 : 16);
   const uint64_t bytes = (uint64_t)(padded * esize);
   // buffer loads address with 32-bit byte offsets
   XRH_REQUIRE(bytes < 0xFFFFF000ull, "volume layout exceeds 4 GiB; use linear layout or split");
@@ -397,13 +416,17 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
   d.storage = storage;
   d.layout = layout;
   float ms = INFINITY;
+  double chord = 0.0;
+  const int64_t dims[3] = {X, Y, Z};
   for (int c = 0; c < 3; ++c) {
     const float n = std::sqrt(idx_to_phys[c] * idx_to_phys[c] + idx_to_phys[4 + c] * idx_to_phys[4 + c] +
                               idx_to_phys[8 + c] * idx_to_phys[8 + c]);
     ms = std::min(ms, n);
+    chord += (double)n * (double)(dims[c] + 1);  // +1: the nudge and edge samples
   }
   XRH_REQUIRE(ms > 0.0f && std::isfinite(ms), "idx_to_phys has a zero or non-finite column");
   d.min_spacing = ms;
+  d.max_chord = (float)chord;
   *out = v;
 }
 
@@ -640,7 +663,7 @@ xrh_status xrh_volume_create_hu(xrh_ctx* ctx, const float* vox_hu, const int64_t
                                 double mu_water, double mu_air, xrh_volume** out)
 {
   return guarded([&] {
-    XRH_REQUIRE(layout >= -1 && layout <= 2, "bad layout");
+    XRH_REQUIRE(layout >= -1 && layout <= 3, "bad layout");
     const HuParams hu{hu_lower, mu_water, mu_air};
     volume_create(ctx, vox_hu, size, idx_to_phys, storage, layout, out, &hu);
   });
@@ -651,7 +674,7 @@ xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_
                                     xrh_volume** out)
 {
   return guarded([&] {
-    XRH_REQUIRE(layout >= 0 && layout <= 2, "bad layout");
+    XRH_REQUIRE(layout >= 0 && layout <= 3, "bad layout");
     volume_create(ctx, vox, size, idx_to_phys, storage, layout, out);
   });
 }

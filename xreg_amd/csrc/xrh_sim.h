@@ -25,6 +25,8 @@ struct FixedDev {
   const float* pden;         // per-patch fixed std * N
   const float* pmeany;
   const float* pdeny;
+  const double* pg;          // per-patch sum(f) - N * mean_f (the float mean's residue)
+  const double* pgy;
 };
 
 // One moving image to score.

@@ -219,34 +219,19 @@ __global__ __launch_bounds__(kThreads) void grad_images_kernel(const SimItem* __
 // LDS edge for a 16x16 block of patches: (16-1)*stride + diameter (26 for r=5, s=1)
 __host__ __device__ inline int patch_lds_edge(int stride, int radius) { return (kT - 1) * stride + 2 * radius + 1; }
 
-__device__ inline float patch_score(const float* m, const float* f, int W, int ly, int lx, int d,
-                                    float mu_f, float den_f)
-{
-#pragma clang fp contract(off)
-  const int N = d * d;
-  float sum = 0.0f;
-  for (int yy = 0; yy < d; ++yy)
-    for (int xx = 0; xx < d; ++xx) sum += m[(ly + yy) * W + lx + xx];
-  const float mean = sum / (float)N;
-  float var = 0.0f, cross = 0.0f;
-  for (int yy = 0; yy < d; ++yy) {
-    for (int xx = 0; xx < d; ++xx) {
-      const float t = m[(ly + yy) * W + lx + xx] - mean;
-      var += t * t;
-      cross += t * (f[(ly + yy) * W + lx + xx] - mu_f);
-    }
-  }
-  var /= (float)N - 1.0f;
-  const float sd = fmaxf(1.0e-6f, sqrtf(var));
-  // sum((m - mu)/sd * (f - mu_f)/den_f) with den_f = s_f * N
-  return 1.0f - (cross / sd) / den_f;
-}
-
+// Patch NCC of one moving image (ImgSimMetric2DPatchNCCCPU, xregImgSimMetric2DPatchNCCCPU.cpp:
+// 212-253) from three box sums per patch, in double: S1 = sum m, S2 = sum m^2, Sf = sum m f.
+// mean = S1 / N, var = (S2 - mean S1) / (N - 1), sd = max(1e-6, sqrt(var)), and
+// sum (m - mean)(f - mean_f) = Sf - mean_f S1 - mean G with G = sum f - N mean_f (fixed,
+// per patch). The score is 1 - cross / (sd * sd_f * N). A 16 x 16 block of patches stages its
+// moving and fixed window in LDS (as doubles), forms the horizontal d-wide sums of every
+// window row for its 16 patch columns, then each lane sums d of those vertically: about
+// 5 d + 3 d operations per patch instead of 5 d^2.
 template <bool kTwo>
 __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restrict__ items,
                                                          double* __restrict__ partials, int ptiles)
 {
-  extern __shared__ float patch_lds[];
+  extern __shared__ double patch_lds[];
   const SimItem it = items[blockIdx.y];
   const FixedDev& fx = *it.fixed;
   const int s = fx.patch_stride, r = fx.patch_radius, d = 2 * r + 1;
@@ -256,29 +241,60 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
   const int px0 = (blockIdx.x % TX) * kT, py0 = (blockIdx.x / TX) * kT;  // first patch index
   const int X0 = px0 * s, Y0 = py0 * s;                                  // first image pixel
   const int W = patch_lds_edge(s, r);
-  float* lm = patch_lds;
-  float* lf = patch_lds + W * W;
+  double* lm = patch_lds;
+  double* lf = lm + W * W;
+  double* h1 = lf + W * W;
+  double* h2 = h1 + W * kT;
+  double* hf = h2 + W * kT;
   const int pxl = threadIdx.x % kT, pyl = threadIdx.x / kT;
   const int pi = px0 + pxl, pj = py0 + pyl;
   const bool valid = (pi < ppr) && (pj < ppc);
+  const double N = (double)(d * d);
   double acc[2] = {0.0, 0.0};
   for (int dir = 0; dir < (kTwo ? 2 : 1); ++dir) {
     const float* mimg = (dir == 0) ? (kTwo ? it.gx : it.mov) : it.gy;
     const float* fimg = (dir == 0) ? fx.pf : fx.pfy;
     const float* pmu = (dir == 0) ? fx.pmean : fx.pmeany;
     const float* pden = (dir == 0) ? fx.pden : fx.pdeny;
+    const double* pg = (dir == 0) ? fx.pg : fx.pgy;
     __syncthreads();
     for (int i = threadIdx.x; i < W * W; i += kThreads) {
       const int ly = i / W, lx = i % W;
       const int y = min(Y0 + ly, fx.rows - 1), x = min(X0 + lx, fx.cols - 1);
       const int64_t o = (int64_t)y * fx.cols + x;
-      lm[ly * W + lx] = mimg[o];
-      lf[ly * W + lx] = fimg[o];
+      lm[i] = (double)mimg[o];
+      lf[i] = (double)fimg[o];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < W * kT; e += kThreads) {
+      const int row = e / kT, x0 = (e % kT) * s;
+      const double* m = lm + row * W + x0;
+      const double* f = lf + row * W + x0;
+      double a1 = 0.0, a2 = 0.0, af = 0.0;
+      for (int xx = 0; xx < d; ++xx) {
+        a1 += m[xx];
+        a2 = fma(m[xx], m[xx], a2);
+        af = fma(m[xx], f[xx], af);
+      }
+      h1[e] = a1;
+      h2[e] = a2;
+      hf[e] = af;
     }
     __syncthreads();
     if (valid) {
+      double S1 = 0.0, S2 = 0.0, Sf = 0.0;
+      for (int yy = 0; yy < d; ++yy) {
+        const int e = (pyl * s + yy) * kT + pxl;
+        S1 += h1[e];
+        S2 += h2[e];
+        Sf += hf[e];
+      }
       const int64_t p = (int64_t)pj * ppr + pi;
-      acc[dir] = (double)patch_score(lm, lf, W, pyl * s, pxl * s, d, pmu[p], pden[p]);
+      const double mean = S1 / N;
+      const double var = fmax(S2 - mean * S1, 0.0) / (N - 1.0);
+      const double sd = fmax((double)1.0e-6f, sqrt(var));
+      const double cross = (Sf - (double)pmu[p] * S1) - mean * pg[p];
+      acc[dir] = 1.0 - (cross / sd) / (double)pden[p];
     }
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
@@ -286,7 +302,8 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
 
 // fixed-image per-patch statistics (reference float order, ComputePatchMeanStdDev)
 __global__ void patch_stats_kernel(const float* __restrict__ img, int cols, int ppr, int ppc, int s,
-                                   int d, float* __restrict__ mean_out, float* __restrict__ den_out)
+                                   int d, float* __restrict__ mean_out, float* __restrict__ den_out,
+                                   double* __restrict__ g_out)
 {
 #pragma clang fp contract(off)
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,6 +324,10 @@ __global__ void patch_stats_kernel(const float* __restrict__ img, int cols, int 
   const float sd = fmaxf(1.0e-6f, sqrtf(var));
   mean_out[p] = mean;
   den_out[p] = sd * (float)N;
+  double f1 = 0.0;
+  for (int y = 0; y < d; ++y)
+    for (int x = 0; x < d; ++x) f1 += (double)img[(int64_t)(r0 + y) * cols + c0 + x];
+  g_out[p] = f1 - (double)N * (double)mean;
 }
 
 // fixed gradient images (one image)
@@ -480,7 +501,7 @@ int sim_patch_lds_bytes(const FixedDev& f)
 {
   if (f.metric != XRH_SIM_PATCH_NCC && f.metric != XRH_SIM_PATCH_GRAD_NCC) return 0;
   const int W = patch_lds_edge(f.patch_stride, f.patch_radius);
-  return 2 * W * W * (int)sizeof(float);
+  return (2 * W * W + 3 * W * kT) * (int)sizeof(double);
 }
 
 int sim_tiles(const FixedDev& f)
@@ -565,17 +586,19 @@ static void ncc_prepare(xrh_fixed* f, const float* img, int64_t n, const uint8_t
   *count_out = (int64_t)cnt;
 }
 
-static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, float** den_out)
+static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, float** den_out, double** g_out)
 {
   FixedDev& h = f->host;
   float* mean = fixed_buf(f, (uint64_t)h.num_patches);
   float* den = fixed_buf(f, (uint64_t)h.num_patches);
+  double* g = (double*)fixed_buf(f, (uint64_t)h.num_patches * 2);
   const int d = 2 * h.patch_radius + 1;
   hipLaunchKernelGGL(patch_stats_kernel, dim3((unsigned)((h.num_patches + 255) / 256)), dim3(256), 0,
-                     f->ctx->stream, img, h.cols, h.ppr, h.ppc, h.patch_stride, d, mean, den);
+                     f->ctx->stream, img, h.cols, h.ppr, h.ppc, h.patch_stride, d, mean, den, g);
   XRH_HIP_CHECK(hipGetLastError());
   *mean_out = mean;
   *den_out = den;
+  *g_out = g;
 }
 
 static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, int32_t cols,
@@ -649,14 +672,25 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
       h.count = n;
       h.pf = gx;
       float *mu = nullptr, *den = nullptr;
-      patch_prepare(f, gx, &mu, &den);
+      double* g = nullptr;
+      patch_prepare(f, gx, &mu, &den, &g);
       h.pmean = mu;
       h.pden = den;
+      h.pg = g;
       if (grad) {
         h.pfy = gy;
-        patch_prepare(f, gy, &mu, &den);
+        patch_prepare(f, gy, &mu, &den, &g);
         h.pmeany = mu;
         h.pdeny = den;
+        h.pgy = g;
+      }
+    }
+    if (patch) {
+      int max_lds = 0;
+      XRH_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+      if (sim_patch_lds_bytes(h) > max_lds) {
+        fail(XRH_ERR_UNSUPPORTED, "patch radius / stride need " + std::to_string(sim_patch_lds_bytes(h)) +
+                                      " bytes of LDS per workgroup; the device has " + std::to_string(max_lds));
       }
     }
     f->dev = (FixedDev*)ctx_alloc(ctx, sizeof(FixedDev));
