@@ -96,6 +96,21 @@ def cpu_baseline(w, fixed, poses, budget_s, smooth):
             "single_thread": {"value": 1.0 / (c - a), "drr_s": round(b - a, 3), "similarity_s": round(c - b, 3)}}
 
 
+def profiled_traffic(key: str):
+    """HBM-side traffic per DRR launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a plain copy) for this exact configuration, from the committed profile summary; None
+    when the configuration was not profiled."""
+    p = ROOT / "profiles" / "drr_kernel_c2_lambda100.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic: ") and f" {key} " in line + " ":
+            for tok in line.split():
+                if tok.startswith("bytes_per_launch="):
+                    return float(tok.split("=", 1)[1])
+    return None
+
+
 def serve_leg(ctx, vol, w, n_jobs, rate, max_gens, lam, geom, smooth, seed=21):
     """Served registration latency: n_jobs single-view CMA-ES registrations of the same workload
     (lambda poses per generation, up to max_gens generations) arriving as a Poisson process of
@@ -254,7 +269,10 @@ def main():
         "plain_poses_per_s": round(plain_value, 2) if plain_value else None,
         "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(f"workload=cmaes-lambda{lam}-384sq-patch-grad-ncc storage={args.storage} "
+                                                 f"geometry={args.geom} skip={0 if args.no_skip else 1} "
+                                                 f"smooth={args.smooth}"),
                      "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4),
                      "samples_per_launch": int(S), "samples_evaluated_per_launch": int(S_eval),
                      "algorithmic_bytes_per_launch": int(bytes_launch),

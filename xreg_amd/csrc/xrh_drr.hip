@@ -24,6 +24,9 @@
 
 #include <hip/hip_fp16.h>
 
+#include <cstdlib>
+#include <cstring>
+
 namespace xrh {
 namespace {
 
@@ -241,6 +244,9 @@ struct VolView {
   uint32_t py;     // row pitch (elements)
   uint32_t pz;     // slice pitch (elements)
   uint32_t apron;  // 1 + py + pz: element index of voxel (0, 0, 0)
+  // last cell fetched (coefficient layout with cell reuse): its index and 32 bytes
+  uint32_t ce;
+  uint4 ca, cb;
 };
 
 // a (1 - w) + b w as two fused multiply-adds: as cheap as a + (b - a) w but without its
@@ -274,7 +280,7 @@ __device__ inline int cvt_floor(float x)
 }
 
 template <int kFmt>
-__device__ inline float sample(const VolView& v, float x, float y, float z)
+__device__ inline float sample(VolView& v, float x, float y, float z)
 {
   // x - floor(x) (V_FRACT_F32) and floor(x) as int: two instructions per axis
   const float wx = __builtin_amdgcn_fractf(x), wy = __builtin_amdgcn_fractf(y), wz = __builtin_amdgcn_fractf(z);
@@ -320,6 +326,22 @@ __device__ inline float sample(const VolView& v, float x, float y, float z)
 #pragma unroll
     for (int i = 0; i < 8; ++i) f[i] = (kFmt == 2) ? h2f(r[i]) : bf2f(r[i]);
     return tri(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], wx, wy, wz);
+  } else if constexpr (kFmt == 7) {
+    // coefficient layout, fetched only when the sample leaves the previous sample's cell
+    // (about half the samples at half-voxel steps); lanes that stay keep their registers
+    if (e != v.ce) {
+      const uint32_t ob = e * 32u;
+      const auto a = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)ob, 0, 0);
+      const auto b = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(ob + 16u), 0, 0);
+      v.ca = make_uint4(a[0], a[1], a[2], a[3]);
+      v.cb = make_uint4(b[0], b[1], b[2], b[3]);
+      v.ce = e;
+    }
+    const float p0 = fmaf(wy, fmaf(wx, __uint_as_float(v.ca.w), __uint_as_float(v.ca.z)),
+                          fmaf(wx, __uint_as_float(v.ca.y), __uint_as_float(v.ca.x)));
+    const float p1 = fmaf(wy, fmaf(wx, __uint_as_float(v.cb.w), __uint_as_float(v.cb.z)),
+                          fmaf(wx, __uint_as_float(v.cb.y), __uint_as_float(v.cb.x)));
+    return fmaf(wz, p1, p0);
   } else if constexpr (kFmt == 6) {
     // trilinear polynomial coefficients of the cell: 2 adjacent 16-byte loads, 7 FMAs
     const uint32_t ob = e * 32u;
@@ -522,6 +544,7 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
       vv.py = it.pitch_y;
       vv.pz = it.pitch_z;
       vv.apron = 1u + it.pitch_y + it.pitch_z;
+      vv.ce = 0xFFFFFFFFu;
       // two samples per iteration (loads of both issue before either is interpolated);
       // samples are still accumulated one after the other, in ray order
       int i, last;
@@ -537,16 +560,33 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
             pz = pz + ray.dz;
           }
         }
-        for (; i < last; i += 2) {
-          const float qx = px + ray.dx, qy = py + ray.dy, qz = pz + ray.dz;
-          const float s0 = sample<kFmt>(vv, px, py, pz);
-          const float s1 = sample<kFmt>(vv, qx, qy, qz);
-          acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
-          px = qx + ray.dx;
-          py = qy + ray.dy;
-          pz = qz + ray.dz;
+        if constexpr (kFmt == 7) {
+          // one sample per iteration: the cell cache lives in fixed registers
+          for (; i <= last; ++i) {
+            acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
+            px = px + ray.dx;
+            py = py + ray.dy;
+            pz = pz + ray.dz;
+          }
+        } else {
+          for (; i < last; i += 2) {
+            const float qx = px + ray.dx, qy = py + ray.dy, qz = pz + ray.dz;
+            const float s0 = sample<kFmt>(vv, px, py, pz);
+            const float s1 = sample<kFmt>(vv, qx, qy, qz);
+            acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
+            px = qx + ray.dx;
+            py = qy + ray.dy;
+            pz = qz + ray.dz;
+          }
+          if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
         }
-        if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
+      } else if constexpr (kFmt == 7) {
+        float f0 = (float)i;
+        for (; i <= last; ++i) {
+          acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+                                                  fmaf(f0, ray.dz, ray.sz)));
+          f0 = f0 + 1.0f;
+        }
       } else {
         // sample indices as floats (exact: n < 2^24), advanced by float additions
         float f0 = (float)i;
@@ -642,7 +682,11 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
   hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev,   \
                      L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,        \
                      L.ncc_partials, L.ncc_tiles_per_item)
-  const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutCoef ? 6 : (L.layout == kLayoutQuad ? 1 : 0))
+  static const bool reuse = [] {
+    const char* e = std::getenv("XRH_CELL_REUSE");
+    return e && std::strcmp(e, "1") == 0;
+  }();
+  const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6) : (L.layout == kLayoutQuad ? 1 : 0))
                 : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
                                          : (L.layout == kLayoutOct ? 5 : 4);
   switch (fmt) {
@@ -652,7 +696,8 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
     case 3: XRH_LAUNCH_FMT(3); break;
     case 4: XRH_LAUNCH_FMT(4); break;
     case 5: XRH_LAUNCH_FMT(5); break;
-    default: XRH_LAUNCH_FMT(6); break;
+    case 6: XRH_LAUNCH_FMT(6); break;
+    default: XRH_LAUNCH_FMT(7); break;
   }
 #undef XRH_LAUNCH_FMT
 }
