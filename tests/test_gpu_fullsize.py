@@ -130,3 +130,16 @@ def test_c4_full_half_storage_bound(ctx, c4full, storage):
     bound = 2.0 ** -11 if storage == 1 else 2.0 ** -8
     print(f"max |err| / max {err.max():.3e} (bound {bound:.2e})")
     assert err.max() < bound
+
+
+def test_c2_full_slab_kernel_is_bit_identical(ctx, c2full, monkeypatch):
+    from xreg_amd import _xrh as X
+
+    print("case: c2 full size slab vs global kernel, 100 poses", flush=True)
+    w = c2full
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    a = _drr(ctx, w, vol, w.poses, geom=1)
+    monkeypatch.setenv("XRH_DRR_SLAB", "0")
+    b = _drr(ctx, w, vol, w.poses, geom=1)
+    vol.close()
+    assert np.array_equal(a, b)
