@@ -251,18 +251,21 @@ def test_fp32_layouts_agree(ctx, c1s):
 
 
 @pytest.mark.parametrize("name", ["c1", "c2", "c4"])
+@pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("kernel,store", [(0, 0), (1, 0), (0, 1)])
-def test_slab_kernel_is_bit_identical(ctx, name, kernel, store, monkeypatch):
-    """The LDS slab-marching DRR (FP64 set-up, coefficient layout) equals the per-sample
-    global-load kernel bit for bit, for SUM and MAX, REPLACE and ACCUM, with skipping."""
+def test_cell_reuse_is_bit_identical(ctx, name, geom, kernel, store, monkeypatch):
+    """Sampling that keeps the previous sample's cell coefficients when the cell is unchanged
+    (the default) equals the kernel that loads every sample bit for bit, for SUM and MAX,
+    REPLACE and ACCUM, both ray set-ups, with skipping."""
     from xreg_amd import synth
 
-    print(f"case: slab vs global {name}-small kernel={kernel} store={store}", flush=True)
+    print(f"case: cell reuse vs per-sample loads {name}-small geom={geom} kernel={kernel} store={store}",
+          flush=True)
     w = synth.config(name, n_poses=8, small=True)
     poses = np.concatenate([w.gt[None], w.poses])
     init = np.random.default_rng(1).random((len(poses), w.cams[0].num_det_rows, w.cams[0].num_det_cols))
-    kw = dict(geom=1, kernel=kernel, store=store, cams=w.cams[:1], init=init if store else None)
+    kw = dict(geom=geom, kernel=kernel, store=store, cams=w.cams[:1], init=init if store else None)
     a = _gpu_drr(ctx, w, poses, **kw)
-    monkeypatch.setenv("XRH_DRR_SLAB", "0")
+    monkeypatch.setenv("XRH_CELL_REUSE", "0")
     b = _gpu_drr(ctx, w, poses, **kw)
     assert np.array_equal(a, b)

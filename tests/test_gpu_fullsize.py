@@ -132,14 +132,14 @@ def test_c4_full_half_storage_bound(ctx, c4full, storage):
     assert err.max() < bound
 
 
-def test_c2_full_slab_kernel_is_bit_identical(ctx, c2full, monkeypatch):
+def test_c2_full_cell_reuse_is_bit_identical(ctx, c2full, monkeypatch):
     from xreg_amd import _xrh as X
 
-    print("case: c2 full size slab vs global kernel, 100 poses", flush=True)
+    print("case: c2 full size cell reuse vs per-sample loads, 100 poses", flush=True)
     w = c2full
     vol = X.Volume(ctx, w.vol, w.idx_to_phys)
     a = _drr(ctx, w, vol, w.poses, geom=1)
-    monkeypatch.setenv("XRH_DRR_SLAB", "0")
+    monkeypatch.setenv("XRH_CELL_REUSE", "0")
     b = _drr(ctx, w, vol, w.poses, geom=1)
     vol.close()
     assert np.array_equal(a, b)

@@ -659,244 +659,6 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
   store_pixel<kOp>(it, tr, inside, col, row, acc, out, accumulate, fill, ncc_partials, ncc_tiles);
 }
 
-// ---------------------------------------------------------------------------------------
-// Slab-marching DRR (FP64 ray set-up, coefficient layout): the 256 rays of a tile are nearly
-// parallel, so over a few cells of their dominant axis they touch a small box of cells. The
-// block walks that axis in slabs of kSlabCells cells; per slab it computes the exact box of
-// cells its samples use (sample positions are fma(i, step, start) in float, monotone in i, so
-// a lane's first and last sample in the slab bound all of its samples), stages the box's
-// coefficients into LDS with coalesced 16-byte loads and then evaluates every sample from LDS.
-// One global load per staged cell replaces the two per sample: vector-memory issue, not
-// arithmetic, bounds the per-sample global version. A slab whose box exceeds the LDS budget,
-// and a tile whose rays do not all travel the same way along the axis, read global memory as
-// drr_kernel does. Positions, coefficients and arithmetic are those of drr_kernel<FP64, op,
-// coef>, so the results are bit-identical.
-
-constexpr int kSlabCells = 4;    // cells per slab along the dominant axis
-constexpr int kLdsCells = 608;   // staged cells per block (19 KB: 8 blocks fit a CU's 160 KB)
-
-__device__ inline int wave_min_i(int v)
-{
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-__device__ inline int wave_max_i(int v)
-{
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-__device__ inline float wave_sum_f(float v)
-{
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__device__ inline float coef_eval(const uint4 a, const uint4 b, float wx, float wy, float wz)
-{
-  const float p0 = fmaf(wy, fmaf(wx, __uint_as_float(a.w), __uint_as_float(a.z)),
-                        fmaf(wx, __uint_as_float(a.y), __uint_as_float(a.x)));
-  const float p1 = fmaf(wy, fmaf(wx, __uint_as_float(b.w), __uint_as_float(b.z)),
-                        fmaf(wx, __uint_as_float(b.y), __uint_as_float(b.x)));
-  return fmaf(wz, p1, p0);
-}
-
-template <int kOp>
-__global__ __launch_bounds__(kThreads) void drr_slab_kernel(
-    const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
-    int bands, float* __restrict__ out, int accumulate, float fill,
-    double* __restrict__ ncc_partials, int ncc_tiles)
-{
-  __shared__ uint4 cells[2 * kLdsCells];
-  __shared__ float red_f[4][3];
-  __shared__ int red_i[4][8];
-  const TileRef tr = decode_tile(band_start, num_items, bands, items);
-  if (!tr.valid) return;  // uniform over the block
-  const DrrItem& it = items[tr.item];
-  int col, row;
-  lane_pixel(tr.tx, tr.ty, col, row);
-  const bool inside = (col < it.cam.cols) && (row < it.cam.rows);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-
-  float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
-  Ray ray;
-  ray.sx = ray.sy = ray.sz = ray.dx = ray.dy = ray.dz = 0.0f;
-  ray.n = -1;
-  int first = 0, last = -1;
-  if (inside) {
-    ray = setup_ray_f64(it, col, row);
-    if (ray.n >= 0) sample_range<kOp>(it, ray, first, last);
-  }
-  const bool active = first <= last;
-  VolView vv;
-  vv.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(it.vol), (short)0, (int)it.vol_bytes, 0x00020000);
-  vv.py = it.pitch_y;
-  vv.pz = it.pitch_z;
-  vv.apron = 1u + it.pitch_y + it.pitch_z;
-  vv.ce = 0xFFFFFFFFu;
-
-  // dominant axis of the block's rays
-  {
-    const float ax = wave_sum_f(active ? fabsf(ray.dx) : 0.0f);
-    const float ay = wave_sum_f(active ? fabsf(ray.dy) : 0.0f);
-    const float az = wave_sum_f(active ? fabsf(ray.dz) : 0.0f);
-    if (l == 0) {
-      red_f[w][0] = ax;
-      red_f[w][1] = ay;
-      red_f[w][2] = az;
-    }
-  }
-  __syncthreads();
-  const float tx = (red_f[0][0] + red_f[1][0]) + (red_f[2][0] + red_f[3][0]);
-  const float ty = (red_f[0][1] + red_f[1][1]) + (red_f[2][1] + red_f[3][1]);
-  const float tz = (red_f[0][2] + red_f[1][2]) + (red_f[2][2] + red_f[3][2]);
-  const int axis = (tx >= ty && tx >= tz) ? 0 : (ty >= tz ? 1 : 2);
-  const float da = axis == 0 ? ray.dx : (axis == 1 ? ray.dy : ray.dz);
-  const float sa = axis == 0 ? ray.sx : (axis == 1 ? ray.sy : ray.sz);
-  const bool all_pos = __syncthreads_and(!active || da > 0.0f);
-  const bool all_neg = __syncthreads_and(!active || da < 0.0f);
-
-  if (!(all_pos || all_neg)) {
-    // mixed directions along the axis (not produced by a pinhole tile): per-sample global loads
-    for (int i = first; i <= last; ++i) {
-      const float f = (float)i;
-      acc = line_op<kOp>(acc, sample<6>(vv, fmaf(f, ray.dx, ray.sx), fmaf(f, ray.dy, ray.sy),
-                                           fmaf(f, ray.dz, ray.sz)));
-    }
-  } else {
-    const int sgn = all_pos ? 1 : -1;
-    // range of dominant-axis cells over the block's samples
-    int kmin = INT_MAX, kmax = INT_MIN;
-    if (active) {
-      const int k0 = cvt_floor(fmaf((float)first, da, sa)), k1 = cvt_floor(fmaf((float)last, da, sa));
-      kmin = min(k0, k1);
-      kmax = max(k0, k1);
-    }
-    kmin = wave_min_i(kmin);
-    kmax = wave_max_i(kmax);
-    __syncthreads();
-    if (l == 0) {
-      red_i[w][0] = kmin;
-      red_i[w][1] = kmax;
-    }
-    __syncthreads();
-    const int Kmin = min(min(red_i[0][0], red_i[1][0]), min(red_i[2][0], red_i[3][0]));
-    const int Kmax = max(max(red_i[0][1], red_i[1][1]), max(red_i[2][1], red_i[3][1]));
-    const int nslabs = (Kmin <= Kmax) ? (Kmax - Kmin) / kSlabCells + 1 : 0;
-    int cursor = first;
-    for (int sl = 0; sl < nslabs; ++sl) {
-      const int K0 = (sgn > 0) ? Kmin + sl * kSlabCells : Kmax - sl * kSlabCells - (kSlabCells - 1);
-      const int K1 = K0 + kSlabCells - 1;
-      // this lane's samples in the slab: cursor .. hi (positions along the axis are monotone)
-      int hi = cursor - 1;
-      if (active && cursor <= last) {
-        const float bound = (sgn > 0) ? (float)(K1 + 1) : (float)K0;
-        int h = cursor - 1;
-        const float t = (bound - sa) / da;
-        if (t > (float)cursor) h = (int)fminf(t, (float)last);
-        h = max(h, cursor - 1);
-        // a sample is in the slab iff floor(pos) <= K1 (sgn > 0) or >= K0 (sgn < 0)
-        while (h + 1 <= last) {
-          const int kc = cvt_floor(fmaf((float)(h + 1), da, sa));
-          if (sgn > 0 ? (kc > K1) : (kc < K0)) break;
-          ++h;
-        }
-        while (h >= cursor) {
-          const int kc = cvt_floor(fmaf((float)h, da, sa));
-          if (sgn > 0 ? (kc <= K1) : (kc >= K0)) break;
-          --h;
-        }
-        hi = h;
-      }
-      // box of cells of those samples, from the two end samples
-      int b[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
-      if (hi >= cursor) {
-        const float f0 = (float)cursor, f1 = (float)hi;
-        const int x0 = cvt_floor(fmaf(f0, ray.dx, ray.sx)), x1 = cvt_floor(fmaf(f1, ray.dx, ray.sx));
-        const int y0 = cvt_floor(fmaf(f0, ray.dy, ray.sy)), y1 = cvt_floor(fmaf(f1, ray.dy, ray.sy));
-        const int z0 = cvt_floor(fmaf(f0, ray.dz, ray.sz)), z1 = cvt_floor(fmaf(f1, ray.dz, ray.sz));
-        b[0] = min(x0, x1);
-        b[1] = min(y0, y1);
-        b[2] = min(z0, z1);
-        b[3] = max(x0, x1);
-        b[4] = max(y0, y1);
-        b[5] = max(z0, z1);
-      }
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        b[q] = wave_min_i(b[q]);
-        b[q + 3] = wave_max_i(b[q + 3]);
-      }
-      __syncthreads();  // the previous slab's readers of red_i and cells are done
-      if (l == 0) {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) red_i[w][q] = b[q];
-      }
-      __syncthreads();
-      const int X0 = min(min(red_i[0][0], red_i[1][0]), min(red_i[2][0], red_i[3][0]));
-      const int Y0 = min(min(red_i[0][1], red_i[1][1]), min(red_i[2][1], red_i[3][1]));
-      const int Z0 = min(min(red_i[0][2], red_i[1][2]), min(red_i[2][2], red_i[3][2]));
-      const int X1 = max(max(red_i[0][3], red_i[1][3]), max(red_i[2][3], red_i[3][3]));
-      const int Y1 = max(max(red_i[0][4], red_i[1][4]), max(red_i[2][4], red_i[3][4]));
-      const int Z1 = max(max(red_i[0][5], red_i[1][5]), max(red_i[2][5], red_i[3][5]));
-      if (X0 > X1) continue;  // no lane has samples in this slab (uniform)
-      const int Nx = X1 - X0 + 1, Ny = Y1 - Y0 + 1, Nz = Z1 - Z0 + 1;
-      const bool use_lds = (int64_t)Nx * Ny * Nz <= kLdsCells;
-      if (use_lds) {
-        const int nxy = Nx * Ny, ncell = nxy * Nz;
-        for (int c = threadIdx.x; c < ncell; c += kThreads) {
-          const int lz = c / nxy, rem = c - lz * nxy, ly = rem / Nx, lx = rem - ly * Nx;
-          const uint32_t e =
-              (uint32_t)(__mul24(Z0 + lz, (int)vv.pz) + __mul24(Y0 + ly, (int)vv.py) + (X0 + lx)) + vv.apron;
-          const uint32_t ob = e * 32u;
-          const auto a = __builtin_amdgcn_raw_buffer_load_b128(vv.rsrc, (int)ob, 0, 0);
-          const auto q = __builtin_amdgcn_raw_buffer_load_b128(vv.rsrc, (int)(ob + 16u), 0, 0);
-          cells[2 * c] = make_uint4(a[0], a[1], a[2], a[3]);
-          cells[2 * c + 1] = make_uint4(q[0], q[1], q[2], q[3]);
-        }
-        __syncthreads();
-        const int nx = Nx, nxy2 = nxy;
-        const int base = X0 + Nx * Y0 + nxy * Z0;
-        int i = cursor;
-        for (; i < hi; i += 2) {
-          const float f0 = (float)i, f1 = f0 + 1.0f;
-          const float px = fmaf(f0, ray.dx, ray.sx), py = fmaf(f0, ray.dy, ray.sy), pz = fmaf(f0, ray.dz, ray.sz);
-          const float qx = fmaf(f1, ray.dx, ray.sx), qy = fmaf(f1, ray.dy, ray.sy), qz = fmaf(f1, ray.dz, ray.sz);
-          const int c0 = __mul24(cvt_floor(pz), nxy2) + __mul24(cvt_floor(py), nx) + cvt_floor(px) - base;
-          const int c1 = __mul24(cvt_floor(qz), nxy2) + __mul24(cvt_floor(qy), nx) + cvt_floor(qx) - base;
-          const uint4 a0 = cells[2 * c0], b0 = cells[2 * c0 + 1];
-          const uint4 a1 = cells[2 * c1], b1 = cells[2 * c1 + 1];
-          const float s0 = coef_eval(a0, b0, __builtin_amdgcn_fractf(px), __builtin_amdgcn_fractf(py),
-                                     __builtin_amdgcn_fractf(pz));
-          const float s1 = coef_eval(a1, b1, __builtin_amdgcn_fractf(qx), __builtin_amdgcn_fractf(qy),
-                                     __builtin_amdgcn_fractf(qz));
-          acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
-        }
-        if (i == hi) {
-          const float f0 = (float)i;
-          const float px = fmaf(f0, ray.dx, ray.sx), py = fmaf(f0, ray.dy, ray.sy), pz = fmaf(f0, ray.dz, ray.sz);
-          const int c0 = __mul24(cvt_floor(pz), nxy2) + __mul24(cvt_floor(py), nx) + cvt_floor(px) - base;
-          acc = line_op<kOp>(acc, coef_eval(cells[2 * c0], cells[2 * c0 + 1], __builtin_amdgcn_fractf(px),
-                                            __builtin_amdgcn_fractf(py), __builtin_amdgcn_fractf(pz)));
-        }
-      } else {
-        for (int i = cursor; i <= hi; ++i) {
-          const float f = (float)i;
-          acc = line_op<kOp>(acc, sample<6>(vv, fmaf(f, ray.dx, ray.sx), fmaf(f, ray.dy, ray.sy),
-                                               fmaf(f, ray.dz, ray.sz)));
-        }
-      }
-      cursor = hi + 1;
-    }
-  }
-  if (active) acc = acc * it.step;
-  store_pixel<kOp>(it, tr, inside, col, row, acc, out, accumulate, fill, ncc_partials, ncc_tiles);
-}
-
 template <int kGeom>
 __global__ __launch_bounds__(kThreads) void drr_count_kernel(
     const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
@@ -932,19 +694,13 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
   hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev,   \
                      L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,        \
                      L.ncc_partials, L.ncc_tiles_per_item)
-  // read per launch (cheap), so a process can switch variants between calls
+  // cell reuse on by default; XRH_CELL_REUSE=0 selects the two-samples-per-iteration kernel
+  // that loads every sample (read per launch, so a process can switch between calls)
   const char* reuse_env = std::getenv("XRH_CELL_REUSE");
-  const bool reuse = reuse_env && std::strcmp(reuse_env, "1") == 0;
+  const bool reuse = !(reuse_env && std::strcmp(reuse_env, "0") == 0);
   const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6) : (L.layout == kLayoutQuad ? 1 : 0))
                 : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
                                          : (L.layout == kLayoutOct ? 5 : 4);
-  const char* slab_env = std::getenv("XRH_DRR_SLAB");
-  const bool slab = !(slab_env && std::strcmp(slab_env, "0") == 0);
-  if (kGeom == XRH_GEOM_FP64 && fmt == 6 && slab) {
-    hipLaunchKernelGGL((drr_slab_kernel<kOp>), grid, dim3(kThreads), 0, s, L.items_dev, L.band_start_dev,
-                       L.num_items, L.bands, L.out, accumulate, fill, L.ncc_partials, L.ncc_tiles_per_item);
-    return;
-  }
   switch (fmt) {
     case 0: XRH_LAUNCH_FMT(0); break;
     case 1: XRH_LAUNCH_FMT(1); break;
