@@ -255,8 +255,8 @@ def test_fp32_layouts_agree(ctx, c1s):
 @pytest.mark.parametrize("kernel,store", [(0, 0), (1, 0), (0, 1)])
 def test_cell_reuse_is_bit_identical(ctx, name, geom, kernel, store, monkeypatch):
     """Sampling that keeps the previous sample's cell coefficients when the cell is unchanged
-    (the default) equals the kernel that loads every sample bit for bit, for SUM and MAX,
-    REPLACE and ACCUM, both ray set-ups, with skipping."""
+    (chosen for launches that fill the GPU many times over) equals the kernel that loads every
+    sample bit for bit, for SUM and MAX, REPLACE and ACCUM, both ray set-ups, with skipping."""
     from xreg_amd import synth
 
     print(f"case: cell reuse vs per-sample loads {name}-small geom={geom} kernel={kernel} store={store}",
@@ -265,6 +265,7 @@ def test_cell_reuse_is_bit_identical(ctx, name, geom, kernel, store, monkeypatch
     poses = np.concatenate([w.gt[None], w.poses])
     init = np.random.default_rng(1).random((len(poses), w.cams[0].num_det_rows, w.cams[0].num_det_cols))
     kw = dict(geom=geom, kernel=kernel, store=store, cams=w.cams[:1], init=init if store else None)
+    monkeypatch.setenv("XRH_CELL_REUSE", "1")
     a = _gpu_drr(ctx, w, poses, **kw)
     monkeypatch.setenv("XRH_CELL_REUSE", "0")
     b = _gpu_drr(ctx, w, poses, **kw)

@@ -138,6 +138,7 @@ def test_c2_full_cell_reuse_is_bit_identical(ctx, c2full, monkeypatch):
     print("case: c2 full size cell reuse vs per-sample loads, 100 poses", flush=True)
     w = c2full
     vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    monkeypatch.setenv("XRH_CELL_REUSE", "1")
     a = _drr(ctx, w, vol, w.poses, geom=1)
     monkeypatch.setenv("XRH_CELL_REUSE", "0")
     b = _drr(ctx, w, vol, w.poses, geom=1)

@@ -687,6 +687,20 @@ __global__ __launch_bounds__(kThreads) void drr_count_kernel(
   }
 }
 
+// Compute units of the current device (cached per device).
+int device_cus()
+{
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cache[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cache[dev] = n;
+  }
+  return cache[dev];
+}
+
 template <int kGeom, int kOp>
 void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, float fill)
 {
@@ -694,10 +708,13 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
   hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev,   \
                      L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,        \
                      L.ncc_partials, L.ncc_tiles_per_item)
-  // cell reuse on by default; XRH_CELL_REUSE=0 selects the two-samples-per-iteration kernel
-  // that loads every sample (read per launch, so a process can switch between calls)
+  // Cell reuse (one sample per iteration, loads only on cell change) wins when the launch
+  // fills every CU many times over and the kernel is bound by load issue; small launches are
+  // latency-bound and prefer the two-samples-per-iteration kernel. XRH_CELL_REUSE=0 / =1
+  // force either (read per launch, so a process can switch between calls).
   const char* reuse_env = std::getenv("XRH_CELL_REUSE");
-  const bool reuse = !(reuse_env && std::strcmp(reuse_env, "0") == 0);
+  const bool reuse = reuse_env ? (std::strcmp(reuse_env, "0") != 0)
+                               : ((int64_t)grid.x * grid.y >= 16ll * device_cus());
   const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6) : (L.layout == kLayoutQuad ? 1 : 0))
                 : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
                                          : (L.layout == kLayoutOct ? 5 : 4);
