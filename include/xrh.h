@@ -201,8 +201,12 @@ xrh_status xrh_line_int_sample_count(xrh_ctx* ctx, const xrh_volume* vol,
  * (xregImgSimMetric2D.h:69-81) process the fixed image once (zero-mean and std for NCC;
  * Sobel gradients for Grad-NCC; per-patch statistics for the patch metrics); compute()
  * (xregImgSimMetric2D.h:88) evaluates num_mov moving images stored contiguously on the
- * device. mask: optional host uint8 image (non-zero = use), NCC only
- * (xregImgSimMetric2DNCCCPU.cpp:186-198); NULL otherwise. */
+ * device. mask: optional host uint8 image (non-zero = use), or NULL. NCC and Grad-NCC use
+ * the masked pixel count and sums (xregImgSimMetric2DNCCCPU.cpp:186-198); the patch metrics
+ * follow ImgSimMetric2DPatchCommon's default mask options (xregImgSimMetric2DPatchCommon.h:
+ * 150-157; .cpp:306-400): patch weight = unmasked fraction, normalised to sum to one,
+ * patches of weight <= 1e-6 skipped, patch statistics over the whole patch, correlation over
+ * unmasked pixels (xregImgSimMetric2DPatchNCCCPU.cpp:212-285). */
 xrh_status xrh_fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, int32_t cols,
                             const xrh_sim_params* params, const uint8_t* mask_host,
                             xrh_fixed** out);

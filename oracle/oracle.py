@@ -185,9 +185,16 @@ def sobel_f64(img, smooth_ksize=0):
     return gx, gy
 
 
-def grad_ncc_f32(fixed, mov, smooth_ksize=5):
+def grad_ncc_f32(fixed, mov, smooth_ksize=5, mask=None):
     f = np.ascontiguousarray(fixed, dtype=np.float32)
     m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
+    if mask is not None:
+        # ImgSimMetric2DGradNCCCPU with a mask: the masked NCC of the x and of the y gradients
+        fgx, fgy = sobel_f32(f, smooth_ksize)
+        g = [sobel_f32(mi, smooth_ksize) for mi in m]
+        sx = ncc_f32(fgx, np.stack([a for a, _ in g]), mask=mask)
+        sy = ncc_f32(fgy, np.stack([b for _, b in g]), mask=mask)
+        return (np.float32(0.5) * (sx + sy)).astype(np.float32)
     out = np.zeros(m.shape[0], dtype=np.float32)
     lib().orc_grad_ncc_f32(_f(f), _f(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), smooth_ksize, _f(out))
     return out
@@ -201,50 +208,61 @@ def grad_ncc_f64(fixed, mov, smooth_ksize=5):
     return out
 
 
-def patch_ncc_f32(fixed, mov, radius=5, stride=1, threads=0):
+def _mask_ptr(mask, shape):
+    if mask is None:
+        return None, None
+    mk = np.ascontiguousarray(mask, dtype=np.uint8).reshape(shape)
+    return mk, mk.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def patch_ncc_f32(fixed, mov, radius=5, stride=1, threads=0, mask=None):
     f = np.ascontiguousarray(fixed, dtype=np.float32)
     m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
     out = np.zeros(m.shape[0], dtype=np.float32)
-    lib().orc_patch_ncc_f32(_f(f), _f(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius, stride,
-                            _f(out), threads)
+    mk, mp = _mask_ptr(mask, f.shape)
+    lib().orc_patch_ncc_masked_f32(_f(f), _f(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius,
+                                   stride, _f(out), threads)
     return out
 
 
-def patch_ncc_f64(fixed, mov, radius=5, stride=1, threads=0):
+def patch_ncc_f64(fixed, mov, radius=5, stride=1, threads=0, mask=None):
     f = np.ascontiguousarray(fixed, dtype=np.float64)
     m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, *f.shape)
     out = np.zeros(m.shape[0], dtype=np.float64)
-    lib().orc_patch_ncc_f64(_d(f), _d(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius, stride,
-                            _d(out), threads)
+    mk, mp = _mask_ptr(mask, f.shape)
+    lib().orc_patch_ncc_masked_f64(_d(f), _d(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius,
+                                   stride, _d(out), threads)
     return out
 
 
-def patch_grad_ncc_f32(fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):
+def patch_grad_ncc_f32(fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0, mask=None):
     f = np.ascontiguousarray(fixed, dtype=np.float32)
     m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
     out = np.zeros(m.shape[0], dtype=np.float32)
-    lib().orc_patch_grad_ncc_f32(_f(f), _f(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), smooth_ksize,
-                                 radius, stride, _f(out), threads)
+    mk, mp = _mask_ptr(mask, f.shape)
+    lib().orc_patch_grad_ncc_masked_f32(_f(f), _f(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]),
+                                        smooth_ksize, radius, stride, _f(out), threads)
     return out
 
 
-def patch_grad_ncc_f64(fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):
+def patch_grad_ncc_f64(fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0, mask=None):
     f = np.ascontiguousarray(fixed, dtype=np.float64)
     m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, *f.shape)
     out = np.zeros(m.shape[0], dtype=np.float64)
-    lib().orc_patch_grad_ncc_f64(_d(f), _d(m), f.shape[0], f.shape[1], C.c_int64(m.shape[0]), smooth_ksize,
-                                 radius, stride, _d(out), threads)
+    mk, mp = _mask_ptr(mask, f.shape)
+    lib().orc_patch_grad_ncc_masked_f64(_d(f), _d(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]),
+                                        smooth_ksize, radius, stride, _d(out), threads)
     return out
 
 
-def sim_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):
+def sim_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0, mask=None):
     if metric == 0:
-        return ncc_f32(fixed, mov)
+        return ncc_f32(fixed, mov, mask=mask)
     if metric == 1:
-        return grad_ncc_f32(fixed, mov, smooth_ksize)
+        return grad_ncc_f32(fixed, mov, smooth_ksize, mask=mask)
     if metric == 2:
-        return patch_ncc_f32(fixed, mov, radius, stride, threads)
-    return patch_grad_ncc_f32(fixed, mov, smooth_ksize, radius, stride, threads)
+        return patch_ncc_f32(fixed, mov, radius, stride, threads, mask=mask)
+    return patch_grad_ncc_f32(fixed, mov, smooth_ksize, radius, stride, threads, mask=mask)
 
 
 def sim_f64(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0):

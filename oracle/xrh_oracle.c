@@ -1072,6 +1072,8 @@ typedef struct {
   int32_t stride;
   float* fhat; /* num_patches * diam * diam */
   float* vals;
+  const uint8_t* mask; /* optional fixed-image mask (non-zero = use the pixel) */
+  const float* wgt;    /* per-patch weights when masked */
 } patch_f32_ctx;
 
 static void patch_stats_f32(const float* img, int32_t cols, int32_t r0, int32_t c0, int32_t diam,
@@ -1124,23 +1126,55 @@ static void patch_mov_range(void* vctx, int64_t begin, int64_t end)
   for (int64_t p = begin; p < end; ++p) {
     const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
     const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
+    const float w = c->mask ? c->wgt[p] : 1.0f;
+    if (c->mask && !(fabsf(w) > 1.0e-6f)) {
+      c->vals[p] = 0.0f; /* low-weight patch skipped (:212) */
+      continue;
+    }
     float mean, sd;
+    /* patch statistics over the whole patch (use_mask_for_patch_stats_ is false) */
     patch_stats_f32(c->mov, c->cols, r0, c0, d, &mean, &sd);
     const float* fh = c->fhat + p * d * d;
     float accum = 0.0f;
     for (int32_t y = 0; y < d; ++y) {
       for (int32_t x = 0; x < d; ++x) {
-        accum += ((c->mov[(int64_t)(r0 + y) * c->cols + c0 + x] - mean) / sd) * fh[y * d + x];
+        const int64_t o = (int64_t)(r0 + y) * c->cols + c0 + x;
+        if (!c->mask || c->mask[o]) accum += ((c->mov[o] - mean) / sd) * fh[y * d + x];
       }
     }
-    /* weight 1 (weight_patch_sims_in_combine_ with unit weights) */
-    c->vals[p] = 1.0f * (1.0f - accum);
+    /* weighted by the patch weight (weight_patch_sims_in_combine_, :252-253) */
+    c->vals[p] = w * (1.0f - accum);
   }
+}
+
+/* ImgSimMetric2DPatchCommon::compute_weights with a mask and the default options
+ * (xregImgSimMetric2DPatchCommon.cpp:306-400): fraction of unmasked pixels in the patch, then
+ * normalised to sum to one (normalize_weights_as_prob_), all in float. */
+static void patch_mask_weights_f32(const uint8_t* mask, int32_t cols, int32_t ppr, int64_t np, int32_t stride,
+                                   int32_t d, float* w)
+{
+  float wsum = 0.0f;
+  for (int64_t p = 0; p < np; ++p) {
+    const int32_t r0 = (int32_t)(p / ppr) * stride, c0 = (int32_t)(p % ppr) * stride;
+    int64_t cnt = 0;
+    for (int32_t y = 0; y < d; ++y)
+      for (int32_t x = 0; x < d; ++x) cnt += mask[(int64_t)(r0 + y) * cols + c0 + x] ? 1 : 0;
+    w[p] = (float)cnt / (float)((int64_t)d * d);
+  }
+  for (int64_t p = 0; p < np; ++p) wsum += w[p];
+  for (int64_t p = 0; p < np; ++p) w[p] /= wsum;
 }
 
 void orc_patch_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32_t cols,
                        int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                        float* sim_out, int32_t threads)
+{
+  orc_patch_ncc_masked_f32(fixed, mov, NULL, rows, cols, num_mov, patch_radius, patch_stride, sim_out, threads);
+}
+
+void orc_patch_ncc_masked_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                              int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                              float* sim_out, int32_t threads)
 {
   const int32_t diam = 2 * patch_radius + 1;
   const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
@@ -1156,6 +1190,13 @@ void orc_patch_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32
   c.stride = patch_stride;
   c.fhat = (float*)malloc(sizeof(float) * (size_t)(np * diam * diam));
   c.vals = (float*)malloc(sizeof(float) * (size_t)np);
+  c.mask = mask;
+  float* wgt = NULL;
+  if (mask) {
+    wgt = (float*)malloc(sizeof(float) * (size_t)np);
+    patch_mask_weights_f32(mask, cols, ppr, np, patch_stride, diam, wgt);
+  }
+  c.wgt = wgt;
   orc_parallel_for(patch_fixed_range, &c, np, 256, threads);
   for (int64_t m = 0; m < num_mov; ++m) {
     c.mov = mov + m * npix;
@@ -1163,13 +1204,14 @@ void orc_patch_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32
     float sum = 0.0f;
     for (int64_t p = 0; p < np; ++p) sum += c.vals[p];
     /* compute_mean_of_patch_sims_ is false by default; weighted combine divides by the
-     * sum of the (unit) weights, accumulated in float (:272-285) */
+     * sum of the weights (unit without a mask), accumulated in float (:272-285) */
     float tot_wgt = 0.0f;
-    for (int64_t p = 0; p < np; ++p) tot_wgt += 1.0f;
+    for (int64_t p = 0; p < np; ++p) tot_wgt += wgt ? wgt[p] : 1.0f;
     sim_out[m] = sum / tot_wgt;
   }
   free(c.fhat);
   free(c.vals);
+  free(wgt);
 }
 
 typedef struct {
@@ -1182,6 +1224,8 @@ typedef struct {
   double* fmean;
   double* fsd;
   double* vals;
+  const uint8_t* mask;
+  const double* wgt;
 } patch_f64_ctx;
 
 static void patch_stats_f64(const double* img, int32_t cols, int32_t r0, int32_t c0, int32_t d,
@@ -1223,22 +1267,34 @@ static void patch_mov_range_f64(void* vctx, int64_t begin, int64_t end)
   for (int64_t p = begin; p < end; ++p) {
     const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
     const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
+    const double w = c->mask ? c->wgt[p] : 1.0;
+    if (c->mask && !(fabs(w) > 1.0e-6)) {
+      c->vals[p] = 0.0;
+      continue;
+    }
     double mean, sd;
     patch_stats_f64(c->mov, c->cols, r0, c0, d, &mean, &sd);
     double acc = 0.0;
     for (int32_t y = 0; y < d; ++y) {
       for (int32_t x = 0; x < d; ++x) {
         const int64_t o = (int64_t)(r0 + y) * c->cols + c0 + x;
-        acc += ((c->mov[o] - mean) / sd) * ((c->fixed[o] - c->fmean[p]) / (c->fsd[p] * n));
+        if (!c->mask || c->mask[o]) acc += ((c->mov[o] - mean) / sd) * ((c->fixed[o] - c->fmean[p]) / (c->fsd[p] * n));
       }
     }
-    c->vals[p] = 1.0 - acc;
+    c->vals[p] = w * (1.0 - acc);
   }
 }
 
 void orc_patch_ncc_f64(const double* fixed, const double* mov, int32_t rows, int32_t cols,
                        int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                        double* sim_out, int32_t threads)
+{
+  orc_patch_ncc_masked_f64(fixed, mov, NULL, rows, cols, num_mov, patch_radius, patch_stride, sim_out, threads);
+}
+
+void orc_patch_ncc_masked_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                              int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                              double* sim_out, int32_t threads)
 {
   const int32_t diam = 2 * patch_radius + 1;
   const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
@@ -1254,20 +1310,47 @@ void orc_patch_ncc_f64(const double* fixed, const double* mov, int32_t rows, int
   c.fmean = (double*)malloc(sizeof(double) * (size_t)np);
   c.fsd = (double*)malloc(sizeof(double) * (size_t)np);
   c.vals = (double*)malloc(sizeof(double) * (size_t)np);
+  c.mask = mask;
+  double* wgt = NULL;
+  double wtot = (double)np;
+  if (mask) {
+    wgt = (double*)malloc(sizeof(double) * (size_t)np);
+    double wsum = 0.0;
+    for (int64_t p = 0; p < np; ++p) {
+      const int32_t r0 = (int32_t)(p / ppr) * patch_stride, c0 = (int32_t)(p % ppr) * patch_stride;
+      int64_t cnt = 0;
+      for (int32_t y = 0; y < diam; ++y)
+        for (int32_t x = 0; x < diam; ++x) cnt += mask[(int64_t)(r0 + y) * cols + c0 + x] ? 1 : 0;
+      wgt[p] = (double)cnt / ((double)diam * diam);
+      wsum += wgt[p];
+    }
+    for (int64_t p = 0; p < np; ++p) wgt[p] /= wsum;
+    wtot = sum_f64(wgt, np);
+  }
+  c.wgt = wgt;
   orc_parallel_for(patch_fixed_range_f64, &c, np, 256, threads);
   for (int64_t m = 0; m < num_mov; ++m) {
     c.mov = mov + m * npix;
     orc_parallel_for(patch_mov_range_f64, &c, np, 256, threads);
-    sim_out[m] = sum_f64(c.vals, np) / (double)np;
+    sim_out[m] = sum_f64(c.vals, np) / wtot;
   }
   free(c.fmean);
   free(c.fsd);
   free(c.vals);
+  free(wgt);
 }
 
 void orc_patch_grad_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32_t cols,
                             int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                             int32_t patch_stride, float* sim_out, int32_t threads)
+{
+  orc_patch_grad_ncc_masked_f32(fixed, mov, NULL, rows, cols, num_mov, smooth_ksize, patch_radius, patch_stride,
+                                sim_out, threads);
+}
+
+void orc_patch_grad_ncc_masked_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                                   int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                   int32_t patch_stride, float* sim_out, int32_t threads)
 {
   /* ImgSimMetric2DPatchGradNCCCPU::compute (xregImgSimMetric2DPatchGradNCCCPU.cpp:104-224) */
   const int64_t npix = (int64_t)rows * cols;
@@ -1279,8 +1362,8 @@ void orc_patch_grad_ncc_f32(const float* fixed, const float* mov, int32_t rows, 
   for (int64_t m = 0; m < num_mov; ++m) {
     float sx, sy;
     orc_sobel_f32(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
-    orc_patch_ncc_f32(fgx, mgx, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
-    orc_patch_ncc_f32(fgy, mgy, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
+    orc_patch_ncc_masked_f32(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
+    orc_patch_ncc_masked_f32(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
     sim_out[m] = (float)(0.5 * (double)(sx + sy));
   }
   free(fgx);
@@ -1293,6 +1376,14 @@ void orc_patch_grad_ncc_f64(const double* fixed, const double* mov, int32_t rows
                             int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                             int32_t patch_stride, double* sim_out, int32_t threads)
 {
+  orc_patch_grad_ncc_masked_f64(fixed, mov, NULL, rows, cols, num_mov, smooth_ksize, patch_radius, patch_stride,
+                                sim_out, threads);
+}
+
+void orc_patch_grad_ncc_masked_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                                   int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                   int32_t patch_stride, double* sim_out, int32_t threads)
+{
   const int64_t npix = (int64_t)rows * cols;
   double* fgx = (double*)malloc(sizeof(double) * (size_t)npix);
   double* fgy = (double*)malloc(sizeof(double) * (size_t)npix);
@@ -1302,8 +1393,8 @@ void orc_patch_grad_ncc_f64(const double* fixed, const double* mov, int32_t rows
   for (int64_t m = 0; m < num_mov; ++m) {
     double sx, sy;
     orc_sobel_f64(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
-    orc_patch_ncc_f64(fgx, mgx, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
-    orc_patch_ncc_f64(fgy, mgy, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
+    orc_patch_ncc_masked_f64(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
+    orc_patch_ncc_masked_f64(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
     sim_out[m] = 0.5 * (sx + sy);
   }
   free(fgx);

@@ -118,6 +118,22 @@ void orc_patch_ncc_f64(const double* fixed, const double* mov, int32_t rows, int
                        int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                        double* sim_out, int32_t threads);
 
+/* The same with a fixed-image mask (non-zero = use) and the reference's default mask options
+ * (ImgSimMetric2DPatchCommon.h:150-157): patch weight = fraction of unmasked pixels, weights
+ * normalised to sum to one, patches of weight <= 1e-6 skipped, patch statistics over the
+ * whole patch, the correlation summed over unmasked pixels only, weighted combine. */
+void orc_patch_ncc_masked_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                              int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                              float* sim_out, int32_t threads);
+void orc_patch_ncc_masked_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                              int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                              double* sim_out, int32_t threads);
+void orc_patch_grad_ncc_masked_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                                   int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                   int32_t patch_stride, float* sim_out, int32_t threads);
+void orc_patch_grad_ncc_masked_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                                   int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                   int32_t patch_stride, double* sim_out, int32_t threads);
 void orc_patch_grad_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32_t cols,
                             int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                             int32_t patch_stride, float* sim_out, int32_t threads);

@@ -68,3 +68,40 @@ def test_sim_does_not_modify_moving_images(ctx, images):
     after = buf.download(np.float32).reshape(mov.shape)
     buf.free()
     assert np.array_equal(after, mov)
+
+
+@pytest.mark.parametrize("metric", [2, 3])
+def test_patch_metrics_with_mask(ctx, oracle, images, metric):
+    """Masked Patch-NCC / Patch-Grad-NCC (mask weights, skipped patches, correlation over
+    unmasked pixels) against the oracle, through the plain and the batched paths."""
+    from xreg_amd import _xrh as X
+
+    print(f"case: masked patch metric {metric}", flush=True)
+    fixed, mov = images
+    mask = np.ones(fixed.shape, np.uint8)
+    mask[:20, :30] = 0
+    mask[50:60, 10:80] = 0
+    mask[70:, 70:] = 0
+    ref = oracle.sim_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, mask=mask)
+    ref64 = (oracle.patch_ncc_f64 if metric == 2 else
+             (lambda f, m, **k: oracle.patch_grad_ncc_f64(f, m, smooth_ksize=5, **k)))(
+        fixed.astype(np.float64), mov.astype(np.float64), radius=5, stride=1, mask=mask)
+    unmasked = oracle.sim_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1)
+    assert np.abs(ref - unmasked).max() > 1e-4  # the mask matters for these images
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, metric, smooth_ksize=5, patch_radius=5, mask=mask)
+    got = f.compute(buf, mov.shape[0])
+    buf.free()
+    print(f"max |gpu-cpu| {np.abs(got - ref).max():.3e}; |gpu-fp64| {np.abs(got - ref64).max():.3e}")
+    assert np.abs(got - ref).max() <= SIM_ABS_TOL
+    assert np.abs(got - ref64).max() <= SIM_ABS_TOL
+    f.close()
+
+
+def test_patch_mask_without_unmasked_patch_is_rejected(ctx, images):
+    from xreg_amd import _xrh as X
+
+    fixed, _ = images
+    with pytest.raises(X.XrhError):
+        X.Fixed(ctx, fixed, 2, patch_radius=5, mask=np.zeros(fixed.shape, np.uint8))

@@ -132,3 +132,61 @@ def test_grad_ncc_brute_force(oracle):
     want = 0.5 * (ncc(gxf, gxm) + ncc(gyf, gym))
     assert abs(oracle.grad_ncc_f64(f, m[None], 0)[0] - want) < 1e-12
     assert abs(oracle.grad_ncc_f32(f, m[None], 0)[0] - want) < 2e-6
+
+
+def _patch_masked_brute(fixed, mov, mask, r=2, s=1):
+    """Masked Patch-NCC with the reference's default options, straight from the definition,
+    in float64: weights = unmasked fraction normalised to sum 1, patches of weight <= 1e-6
+    skipped, statistics over the whole patch, correlation over unmasked pixels only."""
+    d = 2 * r + 1
+    rows, cols = fixed.shape
+    N = d * d
+    centres = [(y, x) for y in range(r, rows - r, s) for x in range(r, cols - r, s)]
+    wr = np.array([mask[y - r:y + r + 1, x - r:x + r + 1].astype(bool).sum() / N for y, x in centres])
+    w = wr / wr.sum()
+    vals = []
+    for (y, x), wp in zip(centres, w):
+        if not abs(wp) > 1e-6:
+            vals.append(0.0)
+            continue
+        F = fixed[y - r:y + r + 1, x - r:x + r + 1].astype(np.float64)
+        M = mov[y - r:y + r + 1, x - r:x + r + 1].astype(np.float64)
+        K = mask[y - r:y + r + 1, x - r:x + r + 1].astype(bool)
+        sf = max(1e-6, np.sqrt(((F - F.mean()) ** 2).sum() / (N - 1)))
+        sm = max(1e-6, np.sqrt(((M - M.mean()) ** 2).sum() / (N - 1)))
+        acc = (((M - M.mean()) / sm) * ((F - F.mean()) / (sf * N)))[K].sum()
+        vals.append(wp * (1.0 - acc))
+    return float(np.sum(vals) / w.sum())
+
+
+def test_patch_ncc_mask_matches_definition(oracle):
+    rng = np.random.default_rng(21)
+    fixed = rng.random((14, 17))
+    mov = (0.7 * fixed + 0.3 * rng.random((14, 17)))
+    mask = np.ones((14, 17), np.uint8)
+    mask[:6, :8] = 0          # whole patches masked out (weight 0, skipped)
+    mask[9:, 11:13] = 0       # partial patches
+    want = _patch_masked_brute(fixed, mov, mask, r=2, s=1)
+    got64 = oracle.patch_ncc_f64(fixed, mov[None], radius=2, stride=1, mask=mask)[0]
+    got32 = oracle.patch_ncc_f32(fixed, mov[None], radius=2, stride=1, mask=mask)[0]
+    assert abs(got64 - want) < 1e-12
+    assert abs(got32 - want) < 2e-6
+    # a plausible mistake: statistics over unmasked pixels only changes the value
+    d, r = 5, 2
+    centres = [(y, x) for y in range(r, 14 - r) for x in range(r, 17 - r)]
+    assert len(centres) == (14 - 4) * (17 - 4)
+    no_mask = oracle.patch_ncc_f64(fixed, mov[None], radius=2, stride=1)[0]
+    assert abs(no_mask - want) > 1e-4
+
+
+def test_patch_ncc_all_ones_mask_is_unmasked(oracle):
+    rng = np.random.default_rng(22)
+    fixed = rng.random((20, 20)).astype(np.float32)
+    mov = rng.random((3, 20, 20)).astype(np.float32)
+    ones = np.ones((20, 20), np.uint8)
+    a = oracle.patch_ncc_f32(fixed, mov, radius=3, stride=2, mask=ones)
+    b = oracle.patch_ncc_f32(fixed, mov, radius=3, stride=2)
+    np.testing.assert_allclose(a, b, rtol=0, atol=2e-6)
+    a = oracle.patch_grad_ncc_f64(fixed, mov, smooth_ksize=3, radius=3, stride=1, mask=ones)
+    b = oracle.patch_grad_ncc_f64(fixed, mov, smooth_ksize=3, radius=3, stride=1)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)

@@ -25,8 +25,10 @@ struct FixedDev {
   const float* pden;         // per-patch fixed std * N
   const float* pmeany;
   const float* pdeny;
-  const double* pg;          // per-patch sum(f) - N * mean_f (the float mean's residue)
-  const double* pgy;
+  const double* pg;          // per-patch sum_U(f) - |U| mean_f over the used pixels U
+  const double* pgy;         //   (all pixels unmasked: the float mean's residue)
+  const float* pw;           // masked patch metrics: normalised per-patch weights, else null
+  double patch_norm;         // divisor of the patch-score sum (patch count, or weight sum)
 };
 
 // One moving image to score.

@@ -220,13 +220,16 @@ __global__ __launch_bounds__(kThreads) void grad_images_kernel(const SimItem* __
 __host__ __device__ inline int patch_lds_edge(int stride, int radius) { return (kT - 1) * stride + 2 * radius + 1; }
 
 // Patch NCC of one moving image (ImgSimMetric2DPatchNCCCPU, xregImgSimMetric2DPatchNCCCPU.cpp:
-// 212-253) from three box sums per patch, in double: S1 = sum m, S2 = sum m^2, Sf = sum m f.
-// mean = S1 / N, var = (S2 - mean S1) / (N - 1), sd = max(1e-6, sqrt(var)), and
-// sum (m - mean)(f - mean_f) = Sf - mean_f S1 - mean G with G = sum f - N mean_f (fixed,
-// per patch). The score is 1 - cross / (sd * sd_f * N). A 16 x 16 block of patches stages its
-// moving and fixed window in LDS (as doubles), forms the horizontal d-wide sums of every
-// window row for its 16 patch columns, then each lane sums d of those vertically: about
-// 5 d + 3 d operations per patch instead of 5 d^2.
+// 212-253) from box sums per patch, in double: S1 = sum m, S2 = sum m^2 over the patch and,
+// over the pixels U the correlation uses (all, or the unmasked ones), SfU = sum m f and
+// S1U = sum m. mean = S1 / N, var = (S2 - mean S1) / (N - 1), sd = max(1e-6, sqrt(var)), and
+// sum_U (m - mean)(f - mean_f) = SfU - mean_f S1U - mean G with G = sum_U f - |U| mean_f
+// (fixed, per patch). The score is w (1 - cross / (sd * sd_f * N)); with a mask, w is the
+// patch's normalised weight and patches with |w| <= 1e-6 contribute 0 (:212), without one
+// w = 1 and S1U = S1, SfU = Sf. A 16 x 16 block of patches stages its moving and fixed
+// (masked) window in LDS as doubles, forms the horizontal d-wide sums of every window row for
+// its 16 patch columns, then each lane sums d of those vertically: about 5 d + 3 d operations
+// per patch instead of 5 d^2.
 template <bool kTwo>
 __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restrict__ items,
                                                          double* __restrict__ partials, int ptiles)
@@ -241,11 +244,14 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
   const int px0 = (blockIdx.x % TX) * kT, py0 = (blockIdx.x / TX) * kT;  // first patch index
   const int X0 = px0 * s, Y0 = py0 * s;                                  // first image pixel
   const int W = patch_lds_edge(s, r);
+  const bool masked = fx.pw != nullptr;
   double* lm = patch_lds;
   double* lf = lm + W * W;
   double* h1 = lf + W * W;
   double* h2 = h1 + W * kT;
   double* hf = h2 + W * kT;
+  double* lk = hf + W * kT;  // masked only: mask window, then sum_U m
+  double* hu = lk + W * W;
   const int pxl = threadIdx.x % kT, pyl = threadIdx.x / kT;
   const int pi = px0 + pxl, pj = py0 + pyl;
   const bool valid = (pi < ppr) && (pj < ppc);
@@ -262,8 +268,10 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
       const int ly = i / W, lx = i % W;
       const int y = min(Y0 + ly, fx.rows - 1), x = min(X0 + lx, fx.cols - 1);
       const int64_t o = (int64_t)y * fx.cols + x;
+      const double k = masked ? (fx.mask[o] ? 1.0 : 0.0) : 1.0;
       lm[i] = (double)mimg[o];
-      lf[i] = (double)fimg[o];
+      lf[i] = (double)fimg[o] * k;
+      if (masked) lk[i] = k;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < W * kT; e += kThreads) {
@@ -279,31 +287,42 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
       h1[e] = a1;
       h2[e] = a2;
       hf[e] = af;
+      if (masked) {
+        const double* k = lk + row * W + x0;
+        double au = 0.0;
+        for (int xx = 0; xx < d; ++xx) au = fma(m[xx], k[xx], au);
+        hu[e] = au;
+      }
     }
     __syncthreads();
     if (valid) {
-      double S1 = 0.0, S2 = 0.0, Sf = 0.0;
+      double S1 = 0.0, S2 = 0.0, Sf = 0.0, SU = 0.0;
       for (int yy = 0; yy < d; ++yy) {
         const int e = (pyl * s + yy) * kT + pxl;
         S1 += h1[e];
         S2 += h2[e];
         Sf += hf[e];
+        if (masked) SU += hu[e];
       }
+      if (!masked) SU = S1;
       const int64_t p = (int64_t)pj * ppr + pi;
-      const double mean = S1 / N;
-      const double var = fmax(S2 - mean * S1, 0.0) / (N - 1.0);
-      const double sd = fmax((double)1.0e-6f, sqrt(var));
-      const double cross = (Sf - (double)pmu[p] * S1) - mean * pg[p];
-      acc[dir] = 1.0 - (cross / sd) / (double)pden[p];
+      const double w = masked ? (double)fx.pw[p] : 1.0;
+      if (!masked || fabs(w) > 1.0e-6) {
+        const double mean = S1 / N;
+        const double var = fmax(S2 - mean * S1, 0.0) / (N - 1.0);
+        const double sd = fmax((double)1.0e-6f, sqrt(var));
+        const double cross = (Sf - (double)pmu[p] * SU) - mean * pg[p];
+        acc[dir] = w * (1.0 - (cross / sd) / (double)pden[p]);
+      }
     }
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
 }
 
 // fixed-image per-patch statistics (reference float order, ComputePatchMeanStdDev)
-__global__ void patch_stats_kernel(const float* __restrict__ img, int cols, int ppr, int ppc, int s,
-                                   int d, float* __restrict__ mean_out, float* __restrict__ den_out,
-                                   double* __restrict__ g_out)
+__global__ void patch_stats_kernel(const float* __restrict__ img, const uint8_t* __restrict__ mask, int cols,
+                                   int ppr, int ppc, int s, int d, float* __restrict__ mean_out,
+                                   float* __restrict__ den_out, double* __restrict__ g_out)
 {
 #pragma clang fp contract(off)
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,10 +343,18 @@ __global__ void patch_stats_kernel(const float* __restrict__ img, int cols, int 
   const float sd = fmaxf(1.0e-6f, sqrtf(var));
   mean_out[p] = mean;
   den_out[p] = sd * (float)N;
-  double f1 = 0.0;
+  // over the pixels the correlation uses (statistics above use the whole patch, as
+  // use_mask_for_patch_stats_ is off by default)
+  double f1 = 0.0, nu = 0.0;
   for (int y = 0; y < d; ++y)
-    for (int x = 0; x < d; ++x) f1 += (double)img[(int64_t)(r0 + y) * cols + c0 + x];
-  g_out[p] = f1 - (double)N * (double)mean;
+    for (int x = 0; x < d; ++x) {
+      const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+      if (!mask || mask[o]) {
+        f1 += (double)img[o];
+        nu += 1.0;
+      }
+    }
+  g_out[p] = f1 - nu * (double)mean;
 }
 
 // fixed gradient images (one image)
@@ -472,11 +499,11 @@ __global__ void finish_kernel(const SimItem* __restrict__ items, const double* _
       break;
     }
     case XRH_SIM_PATCH_NCC:
-      sim = acc[0] / (double)fx.num_patches;
+      sim = acc[0] / fx.patch_norm;
       break;
     default: {
-      const float sx = (float)(acc[0] / (double)fx.num_patches);
-      const float sy = (float)(acc[1] / (double)fx.num_patches);
+      const float sx = (float)(acc[0] / fx.patch_norm);
+      const float sy = (float)(acc[1] / fx.patch_norm);
       sim = 0.5 * (double)(sx + sy);
       break;
     }
@@ -501,7 +528,7 @@ int sim_patch_lds_bytes(const FixedDev& f)
 {
   if (f.metric != XRH_SIM_PATCH_NCC && f.metric != XRH_SIM_PATCH_GRAD_NCC) return 0;
   const int W = patch_lds_edge(f.patch_stride, f.patch_radius);
-  return (2 * W * W + 3 * W * kT) * (int)sizeof(double);
+  return (f.pw ? (3 * W * W + 4 * W * kT) : (2 * W * W + 3 * W * kT)) * (int)sizeof(double);
 }
 
 int sim_tiles(const FixedDev& f)
@@ -594,7 +621,7 @@ static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, floa
   double* g = (double*)fixed_buf(f, (uint64_t)h.num_patches * 2);
   const int d = 2 * h.patch_radius + 1;
   hipLaunchKernelGGL(patch_stats_kernel, dim3((unsigned)((h.num_patches + 255) / 256)), dim3(256), 0,
-                     f->ctx->stream, img, h.cols, h.ppr, h.ppc, h.patch_stride, d, mean, den, g);
+                     f->ctx->stream, img, h.mask, h.cols, h.ppr, h.ppc, h.patch_stride, d, mean, den, g);
   XRH_HIP_CHECK(hipGetLastError());
   *mean_out = mean;
   *den_out = den;
@@ -619,7 +646,7 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
     XRH_REQUIRE(p->patch_stride >= 1 && p->patch_stride <= 4, "patch stride must be in [1, 4]");
     XRH_REQUIRE(2 * p->patch_radius + 1 <= rows && 2 * p->patch_radius + 1 <= cols,
                 "patch diameter exceeds the image");  // xregImgSimMetric2DPatchCommon.cpp:271-272
-    if (mask_host) fail(XRH_ERR_UNSUPPORTED, "masks are supported for NCC and Grad-NCC only");
+
   }
   auto* f = new xrh_fixed();
   f->ctx = ctx;
@@ -671,6 +698,34 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
       h.num_patches = (int64_t)h.ppr * h.ppc;
       h.count = n;
       h.pf = gx;
+      h.patch_norm = (double)h.num_patches;
+      if (mask_host) {
+        // ImgSimMetric2DPatchCommon::compute_weights with a mask and the default options
+        // (xregImgSimMetric2DPatchCommon.cpp:306-400): unmasked fraction of each patch,
+        // normalised to sum to one, in float; the score sum is divided by the weights' sum
+        // (xregImgSimMetric2DPatchNCCCPU.cpp:272-285)
+        std::vector<float> w((size_t)h.num_patches);
+        float wsum = 0.0f;
+        for (int64_t q = 0; q < h.num_patches; ++q) {
+          const int r0 = (int)(q / h.ppr) * h.patch_stride, c0 = (int)(q % h.ppr) * h.patch_stride;
+          int64_t cnt = 0;
+          for (int y = 0; y < d; ++y)
+            for (int x = 0; x < d; ++x) cnt += mask_host[(int64_t)(r0 + y) * cols + c0 + x] ? 1 : 0;
+          w[(size_t)q] = (float)cnt / (float)(d * d);
+        }
+        for (float v : w) wsum += v;
+        XRH_REQUIRE(wsum > 0.0f, "the mask leaves no unmasked pixel in any patch");
+        float tot = 0.0f;
+        for (float& v : w) {
+          v /= wsum;
+          tot += v;
+        }
+        float* wd = fixed_buf(f, (uint64_t)h.num_patches);
+        XRH_HIP_CHECK(hipMemcpyAsync(wd, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        XRH_HIP_CHECK(hipStreamSynchronize(s));
+        h.pw = wd;
+        h.patch_norm = (double)tot;
+      }
       float *mu = nullptr, *den = nullptr;
       double* g = nullptr;
       patch_prepare(f, gx, &mu, &den, &g);
