@@ -270,21 +270,3 @@ def test_cell_reuse_is_bit_identical(ctx, name, geom, kernel, store, monkeypatch
     monkeypatch.setenv("XRH_CELL_REUSE", "0")
     b = _gpu_drr(ctx, w, poses, **kw)
     assert np.array_equal(a, b)
-
-
-@pytest.mark.parametrize("storage", [1, 2])
-@pytest.mark.parametrize("geom", [0, 1])
-def test_cell_reuse_half_storage_is_bit_identical(ctx, storage, geom, monkeypatch):
-    """Cell reuse on the 16-bit oct layouts (corners widened once per cell) equals the
-    per-sample kernel bit for bit."""
-    from xreg_amd import synth
-
-    print(f"case: cell reuse half storage={storage} geom={geom}", flush=True)
-    w = synth.config("c2", n_poses=8, small=True)
-    poses = np.concatenate([w.gt[None], w.poses])
-    monkeypatch.setenv("XRH_CELL_REUSE", "1")
-    a = _gpu_drr(ctx, w, poses, storage=storage, geom=geom, cams=w.cams[:1])
-    monkeypatch.setenv("XRH_CELL_REUSE", "0")
-    b = _gpu_drr(ctx, w, poses, storage=storage, geom=geom, cams=w.cams[:1])
-    assert a.max() > 0
-    assert np.array_equal(a, b)

@@ -343,25 +343,6 @@ __device__ inline float sample(VolView& v, float x, float y, float z)
     const float p1 = fmaf(wy, fmaf(wx, __uint_as_float(v.cb.w), __uint_as_float(v.cb.z)),
                           fmaf(wx, __uint_as_float(v.cb.y), __uint_as_float(v.cb.x)));
     return fmaf(wz, p1, p0);
-  } else if constexpr (kFmt == 8 || kFmt == 9) {
-    // 16-bit oct layout with cell reuse: the 8 corners are fetched and widened to float only
-    // when the sample enters a new cell (8 = FP16, 9 = BF16); same interpolation as 3 / 5
-    if (e != v.ce) {
-      const auto q = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(e * 16u), 0, 0);
-      float f[8];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t lo = q[i] & 0xffffu, hi = q[i] >> 16;
-        f[2 * i] = (kFmt == 8) ? h2f(lo) : bf2f(lo);
-        f[2 * i + 1] = (kFmt == 8) ? h2f(hi) : bf2f(hi);
-      }
-      v.ca = make_uint4(__float_as_uint(f[0]), __float_as_uint(f[1]), __float_as_uint(f[2]), __float_as_uint(f[3]));
-      v.cb = make_uint4(__float_as_uint(f[4]), __float_as_uint(f[5]), __float_as_uint(f[6]), __float_as_uint(f[7]));
-      v.ce = e;
-    }
-    return tri(__uint_as_float(v.ca.x), __uint_as_float(v.ca.y), __uint_as_float(v.ca.z), __uint_as_float(v.ca.w),
-               __uint_as_float(v.cb.x), __uint_as_float(v.cb.y), __uint_as_float(v.cb.z), __uint_as_float(v.cb.w),
-               wx, wy, wz);
   } else if constexpr (kFmt == 6) {
     // trilinear polynomial coefficients of the cell: 2 adjacent 16-byte loads, 7 FMAs
     const uint32_t ob = e * 32u;
@@ -627,7 +608,7 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
             pz = pz + ray.dz;
           }
         }
-        if constexpr (kFmt >= 7) {
+        if constexpr (kFmt == 7) {
           // one sample per iteration: the cell cache lives in fixed registers
           for (; i <= last; ++i) {
             acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
@@ -647,7 +628,7 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
           }
           if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
         }
-      } else if constexpr (kFmt >= 7) {
+      } else if constexpr (kFmt == 7) {
         float f0 = (float)i;
         for (; i <= last; ++i) {
           acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
@@ -735,8 +716,8 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
   const bool reuse = reuse_env ? (std::strcmp(reuse_env, "0") != 0)
                                : ((int64_t)grid.x * grid.y >= 16ll * device_cus());
   const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6) : (L.layout == kLayoutQuad ? 1 : 0))
-                : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? (reuse ? 8 : 3) : 2)
-                                         : (L.layout == kLayoutOct ? (reuse ? 9 : 5) : 4);
+                : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
+                                         : (L.layout == kLayoutOct ? 5 : 4);
   switch (fmt) {
     case 0: XRH_LAUNCH_FMT(0); break;
     case 1: XRH_LAUNCH_FMT(1); break;
@@ -745,9 +726,7 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
     case 4: XRH_LAUNCH_FMT(4); break;
     case 5: XRH_LAUNCH_FMT(5); break;
     case 6: XRH_LAUNCH_FMT(6); break;
-    case 7: XRH_LAUNCH_FMT(7); break;
-    case 8: XRH_LAUNCH_FMT(8); break;
-    default: XRH_LAUNCH_FMT(9); break;
+    default: XRH_LAUNCH_FMT(7); break;
   }
 #undef XRH_LAUNCH_FMT
 }
