@@ -2,9 +2,10 @@
 BOBYQA-pattern local refiner, and a multi-resolution driver.
 
 The objective follows Intensity2D3DRegi::obj_fn (lib/regi/interfaces_2d_3d/
-xregIntensity2D3DRegi.cpp:571-742): optimiser vectors -> poses (SE3OptVarsLieAlg,
-lib/regi/xregSE3OptVars.cpp:127-137, applied about a rotation centre on the volume side of a
-starting pose), all views x all candidates cast and scored in one batched submission
+xregIntensity2D3DRegi.cpp:571-742): optimiser vectors -> poses (an SE3OptVars parameterisation
+inside the inter-frame sandwich, optvars.py; by default the se(3) Lie algebra about a rotation
+centre on the volume side of a starting pose), all views x all candidates cast and scored in
+one batched submission
 (view-major, :605-621), then the per-view values combined by ImgSimMetric2DCombineMean
 (lib/regi/sim_metrics_2d/xregImgSimMetric2DCombine.cpp:79-98: float sum in view order, divided
 by the number of views).
@@ -18,6 +19,7 @@ import numpy as np
 from . import _xrh as X
 from . import geometry as G
 from .cmaes import CMAES
+from .optvars import InterFramePose, SE3OptVars
 
 
 def combine_mean(per_view: np.ndarray) -> np.ndarray:
@@ -29,21 +31,16 @@ def combine_mean(per_view: np.ndarray) -> np.ndarray:
     return (acc / np.float32(per_view.shape[0])).astype(np.float32)
 
 
-class PoseParam:
-    """x (6: rotation [rad], translation [mm]) -> cam->phys pose: exp(x) about `center`,
-    composed on the volume side of `init_pose`."""
+class PoseParam(InterFramePose):
+    """The usual set-up of the hip-surgery drivers: optimiser vector (default 6: se(3) Lie
+    algebra, rotation [rad] then translation [mm]) about an intermediate frame at `center`
+    (volume axes), given w.r.t. the volume, around the guess `init_pose`:
+    pose = C * T(x) * C^-1 * init_pose."""
 
-    def __init__(self, init_pose: np.ndarray, center):
+    def __init__(self, init_pose: np.ndarray, center, opt_vars: SE3OptVars | None = None):
         self.init = np.asarray(init_pose, dtype=np.float64)
         self.center = np.asarray(center, dtype=np.float64)
-        self._post = G.rigid(None, -self.center) @ self.init
-        self._pre = G.rigid(None, self.center)
-
-    def __call__(self, Xp: np.ndarray) -> np.ndarray:
-        Xp = np.asarray(Xp, dtype=np.float64)
-        if Xp.ndim == 1:
-            return self._pre @ G.exp_se3(Xp) @ self._post
-        return self._pre @ G.exp_se3_batch(Xp) @ self._post
+        super().__init__(self.init, opt_vars, G.rigid(None, self.center), inter_frame_wrt_vol=True)
 
 
 class Objective:
@@ -108,7 +105,7 @@ def run_cmaes(obj: Objective, lam: int, sigma, max_gens: int, seed: int, x0=None
               tol_fun: float = 1e-12, tol_x: float = 1e-11) -> RegiResult:
     """Intensity2D3DRegiCMAES::run (xregIntensity2D3DRegiCMAES.cpp:75-261): lambda candidates per
     generation evaluated in one submission, bounds by resampling."""
-    x0 = np.zeros(6) if x0 is None else np.asarray(x0, dtype=np.float64)
+    x0 = np.zeros(obj.param.num_params) if x0 is None else np.asarray(x0, dtype=np.float64)
     es = CMAES(x0, np.asarray(sigma, dtype=np.float64), lam, seed,
                bounds=None if bounds is None else np.asarray(bounds, dtype=np.float64), tol_fun=tol_fun, tol_x=tol_x)
     best_f, best_x = np.inf, x0.copy()
@@ -138,7 +135,7 @@ def run_exhaustive(obj: Objective, candidates: np.ndarray, batch: int | None = N
     list is evaluated in consecutive batches of `batch` (the objective's batch size by
     default); the minimum is updated only on a strictly smaller value, so the first of equal
     minima wins; optionally every value is kept."""
-    cand = np.asarray(candidates, dtype=np.float64).reshape(-1, 6)
+    cand = np.asarray(candidates, dtype=np.float64).reshape(-1, obj.param.num_params)
     b = obj.max_batch if batch is None else int(batch)
     all_vals = []
     best_f, best_i = np.float32(np.finfo(np.float32).max), -1
@@ -150,7 +147,7 @@ def run_exhaustive(obj: Objective, candidates: np.ndarray, batch: int | None = N
                 best_f, best_i = v, s + j
         if save_all:
             all_vals.append(vals)
-    x = cand[best_i] if best_i >= 0 else np.zeros(6)
+    x = cand[best_i] if best_i >= 0 else np.zeros(obj.param.num_params)
     return RegiResult(x, float(best_f), obj.param(x), obj.evals - e0,
                       extra={"all_sim_vals": np.concatenate(all_vals) if all_vals else np.zeros(0, np.float32),
                              "best_index": best_i})
@@ -259,7 +256,7 @@ def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels,
             elif lvl.method == "exhaustive":
                 r = run_exhaustive(obj, **o)
             elif lvl.method == "refine":
-                r = run_local_refine(obj, np.zeros(6), **o)
+                r = run_local_refine(obj, np.zeros(obj.param.num_params), **o)
             else:
                 raise ValueError(lvl.method)
         finally:
