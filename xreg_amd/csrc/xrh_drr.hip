@@ -609,8 +609,10 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
           }
         }
         if constexpr (kFmt == 7) {
-          // one sample per iteration: the cell cache lives in fixed registers
-          for (; i <= last; ++i) {
+          // one sample per iteration: the cell cache lives in fixed registers; the count runs
+          // in float (exact below 2^24) so the loop test is one float compare
+          const float flast = (float)last;
+          for (float fi = (float)i; fi <= flast; fi = fi + 1.0f) {
             acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
             px = px + ray.dx;
             py = py + ray.dy;
@@ -629,11 +631,10 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
           if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
         }
       } else if constexpr (kFmt == 7) {
-        float f0 = (float)i;
-        for (; i <= last; ++i) {
+        const float flast = (float)last;
+        for (float f0 = (float)i; f0 <= flast; f0 = f0 + 1.0f) {
           acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
                                                   fmaf(f0, ray.dz, ray.sz)));
-          f0 = f0 + 1.0f;
         }
       } else {
         // sample indices as floats (exact: n < 2^24), advanced by float additions
