@@ -274,34 +274,24 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
       if (masked) lk[i] = k;
     }
     __syncthreads();
-    // horizontal d-wide sums for the 16 patch columns of every window row: wave q forms
-    // quantity q (m, m^2, m f, m k) for the rows, one row per lane; at stride 1 the window
-    // slides (add the entering term, subtract the leaving one)
-    {
-      const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
-      if (q < (masked ? 4 : 3)) {
-        double* hq = (q == 0) ? h1 : (q == 1) ? h2 : (q == 2) ? hf : hu;
-        const double* g = (q == 2) ? lf : lk;
-        for (int row = lane; row < W; row += 64) {
-          const double* m = lm + row * W;
-          const double* o = g + row * W;
-          auto term = [&](int x) -> double { return q == 0 ? m[x] : (q == 1 ? m[x] * m[x] : m[x] * o[x]); };
-          if (s == 1) {
-            double a = 0.0;
-            for (int xx = 0; xx < d; ++xx) a += term(xx);
-            hq[row * kT] = a;
-            for (int c = 1; c < kT; ++c) {
-              a += term(c + d - 1) - term(c - 1);
-              hq[row * kT + c] = a;
-            }
-          } else {
-            for (int c = 0; c < kT; ++c) {
-              double a = 0.0;
-              for (int xx = 0; xx < d; ++xx) a += term(c * s + xx);
-              hq[row * kT + c] = a;
-            }
-          }
-        }
+    for (int e = threadIdx.x; e < W * kT; e += kThreads) {
+      const int row = e / kT, x0 = (e % kT) * s;
+      const double* m = lm + row * W + x0;
+      const double* f = lf + row * W + x0;
+      double a1 = 0.0, a2 = 0.0, af = 0.0;
+      for (int xx = 0; xx < d; ++xx) {
+        a1 += m[xx];
+        a2 = fma(m[xx], m[xx], a2);
+        af = fma(m[xx], f[xx], af);
+      }
+      h1[e] = a1;
+      h2[e] = a2;
+      hf[e] = af;
+      if (masked) {
+        const double* k = lk + row * W + x0;
+        double au = 0.0;
+        for (int xx = 0; xx < d; ++xx) au = fma(m[xx], k[xx], au);
+        hu[e] = au;
       }
     }
     __syncthreads();
