@@ -41,7 +41,7 @@ def c1s():
     return synth.config("c1", n_poses=6, small=True)
 
 
-@pytest.mark.parametrize("layout", [0, 1, 3])
+@pytest.mark.parametrize("layout", [0, 1, 3, 4])
 def test_drr_fp32_matches_cpu_reference(ctx, oracle, c1s, layout):
     print(f"case: c1-small layout={layout}", flush=True)
     poses = np.concatenate([c1s.gt[None], c1s.poses])
@@ -244,27 +244,29 @@ def test_fp32_layouts_agree(ctx, c1s):
     (rounding aside): DRRs agree to 2e-6 of the image maximum."""
     print("case: layouts 0/1/3 agree c1-small", flush=True)
     poses = np.concatenate([c1s.gt[None], c1s.poses[:3]])
-    outs = [_gpu_drr(ctx, c1s, poses, layout=l) for l in (0, 1, 3)]
+    outs = [_gpu_drr(ctx, c1s, poses, layout=l) for l in (0, 1, 3, 4)]
     m = float(outs[0].max())
     for o in outs[1:]:
         assert np.abs(o.astype(np.float64) - outs[0]).max() <= 2e-6 * m
 
 
+@pytest.mark.parametrize("layout", [3, 4])
 @pytest.mark.parametrize("name", ["c1", "c2", "c4"])
 @pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("kernel,store", [(0, 0), (1, 0), (0, 1)])
-def test_cell_reuse_is_bit_identical(ctx, name, geom, kernel, store, monkeypatch):
+def test_cell_reuse_is_bit_identical(ctx, name, geom, kernel, store, layout, monkeypatch):
     """Sampling that keeps the previous sample's cell coefficients when the cell is unchanged
     (chosen for launches that fill the GPU many times over) equals the kernel that loads every
     sample bit for bit, for SUM and MAX, REPLACE and ACCUM, both ray set-ups, with skipping."""
     from xreg_amd import synth
 
-    print(f"case: cell reuse vs per-sample loads {name}-small geom={geom} kernel={kernel} store={store}",
+    print(f"case: cell reuse vs per-sample loads {name}-small geom={geom} kernel={kernel} store={store} "
+          f"layout={layout}",
           flush=True)
     w = synth.config(name, n_poses=8, small=True)
     poses = np.concatenate([w.gt[None], w.poses])
     init = np.random.default_rng(1).random((len(poses), w.cams[0].num_det_rows, w.cams[0].num_det_cols))
-    kw = dict(geom=geom, kernel=kernel, store=store, cams=w.cams[:1], init=init if store else None)
+    kw = dict(geom=geom, kernel=kernel, store=store, cams=w.cams[:1], init=init if store else None, layout=layout)
     monkeypatch.setenv("XRH_CELL_REUSE", "1")
     a = _gpu_drr(ctx, w, poses, **kw)
     monkeypatch.setenv("XRH_CELL_REUSE", "0")

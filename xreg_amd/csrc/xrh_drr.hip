@@ -327,6 +327,31 @@ __device__ inline float sample(VolView& v, float x, float y, float z)
 #pragma unroll
     for (int i = 0; i < 8; ++i) f[i] = (kFmt == 2) ? h2f(r[i]) : bf2f(r[i]);
     return tri(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], wx, wy, wz);
+  } else if constexpr (kFmt == 10 || kFmt == 11) {
+    // face layout: faces normal to x at x and x+1 (adjacent 16-byte elements), bilinear in y, z
+    // then linear along x. 11 keeps the faces while the sample stays in its cell.
+    uint4 f0, f1;
+    if constexpr (kFmt == 10) {
+      const auto a = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(e * 16u), 0, 0);
+      const auto b = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(e * 16u + 16u), 0, 0);
+      f0 = make_uint4(a[0], a[1], a[2], a[3]);
+      f1 = make_uint4(b[0], b[1], b[2], b[3]);
+    } else {
+      if (e != v.ce) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(e * 16u), 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(e * 16u + 16u), 0, 0);
+        v.ca = make_uint4(a[0], a[1], a[2], a[3]);
+        v.cb = make_uint4(b[0], b[1], b[2], b[3]);
+        v.ce = e;
+      }
+      f0 = v.ca;
+      f1 = v.cb;
+    }
+    const float p0 = fmaf(wz, fmaf(wy, __uint_as_float(f0.w), __uint_as_float(f0.z)),
+                          fmaf(wy, __uint_as_float(f0.y), __uint_as_float(f0.x)));
+    const float p1 = fmaf(wz, fmaf(wy, __uint_as_float(f1.w), __uint_as_float(f1.z)),
+                          fmaf(wy, __uint_as_float(f1.y), __uint_as_float(f1.x)));
+    return lerp(p0, p1, wx);
   } else if constexpr (kFmt == 7) {
     // coefficient layout, fetched only when the sample leaves the previous sample's cell
     // (about half the samples at half-voxel steps); lanes that stay keep their registers
@@ -522,6 +547,70 @@ __global__ void drr_setup_kernel(const PoseIn* __restrict__ poses, const VolDesc
   items[i] = it;
 }
 
+// Samples [i, last] of one ray into acc, in ray order.
+template <int kGeom, int kOp, int kFmt>
+__device__ inline float march(const Ray& ray, VolView& vv, int i, int last, float acc)
+{
+  if constexpr (kGeom == XRH_GEOM_FP32) {
+    // positions advance by repeated float addition, as the reference does (:276),
+    // including over the leading samples that are skipped
+    float px = ray.sx, py = ray.sy, pz = ray.sz;
+    if (i <= last) {
+      for (int k = 0; k < i; ++k) {
+        px = px + ray.dx;
+        py = py + ray.dy;
+        pz = pz + ray.dz;
+      }
+    }
+    if constexpr (kFmt == 7 || kFmt == 11) {
+      // one sample per iteration: the cell cache lives in fixed registers; the count runs
+      // in float (exact below 2^24) so the loop test is one float compare
+      const float flast = (float)last;
+      for (float fi = (float)i; fi <= flast; fi = fi + 1.0f) {
+        acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
+        px = px + ray.dx;
+        py = py + ray.dy;
+        pz = pz + ray.dz;
+      }
+    } else {
+      for (; i < last; i += 2) {
+        const float qx = px + ray.dx, qy = py + ray.dy, qz = pz + ray.dz;
+        const float s0 = sample<kFmt>(vv, px, py, pz);
+        const float s1 = sample<kFmt>(vv, qx, qy, qz);
+        acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
+        px = qx + ray.dx;
+        py = qy + ray.dy;
+        pz = qz + ray.dz;
+      }
+      if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
+    }
+  } else if constexpr (kFmt == 7 || kFmt == 11) {
+    const float flast = (float)last;
+    for (float f0 = (float)i; f0 <= flast; f0 = f0 + 1.0f) {
+      acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+                                              fmaf(f0, ray.dz, ray.sz)));
+    }
+  } else {
+    // sample indices as floats (exact: n < 2^24), advanced by float additions
+    float f0 = (float)i;
+    for (; i < last; i += 2) {
+      const float f1 = f0 + 1.0f;
+      const float s0 = sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+                                    fmaf(f0, ray.dz, ray.sz));
+      const float s1 = sample<kFmt>(vv, fmaf(f1, ray.dx, ray.sx), fmaf(f1, ray.dy, ray.sy),
+                                    fmaf(f1, ray.dz, ray.sz));
+      acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
+      f0 = f0 + 2.0f;
+    }
+    if (i == last) {
+      const float f0 = (float)i;
+      acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+                                              fmaf(f0, ray.dz, ray.sz)));
+    }
+  }
+  return acc;
+}
+
 // Pixel store (REPLACE / ACCUM with the line kernel, RayCasterCPU::pre_compute semantics) and
 // the fused NCC epilogue: FP64 sums of m, m^2 and m * zero-mean fixed, one record per tile.
 // Every thread of the block calls it (it synchronises).
@@ -597,63 +686,7 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
       // samples are still accumulated one after the other, in ray order
       int i, last;
       sample_range<kOp>(it, ray, i, last);
-      if constexpr (kGeom == XRH_GEOM_FP32) {
-        // positions advance by repeated float addition, as the reference does (:276),
-        // including over the leading samples that are skipped
-        float px = ray.sx, py = ray.sy, pz = ray.sz;
-        if (i <= last) {
-          for (int k = 0; k < i; ++k) {
-            px = px + ray.dx;
-            py = py + ray.dy;
-            pz = pz + ray.dz;
-          }
-        }
-        if constexpr (kFmt == 7) {
-          // one sample per iteration: the cell cache lives in fixed registers; the count runs
-          // in float (exact below 2^24) so the loop test is one float compare
-          const float flast = (float)last;
-          for (float fi = (float)i; fi <= flast; fi = fi + 1.0f) {
-            acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
-            px = px + ray.dx;
-            py = py + ray.dy;
-            pz = pz + ray.dz;
-          }
-        } else {
-          for (; i < last; i += 2) {
-            const float qx = px + ray.dx, qy = py + ray.dy, qz = pz + ray.dz;
-            const float s0 = sample<kFmt>(vv, px, py, pz);
-            const float s1 = sample<kFmt>(vv, qx, qy, qz);
-            acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
-            px = qx + ray.dx;
-            py = qy + ray.dy;
-            pz = qz + ray.dz;
-          }
-          if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
-        }
-      } else if constexpr (kFmt == 7) {
-        const float flast = (float)last;
-        for (float f0 = (float)i; f0 <= flast; f0 = f0 + 1.0f) {
-          acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
-                                                  fmaf(f0, ray.dz, ray.sz)));
-        }
-      } else {
-        // sample indices as floats (exact: n < 2^24), advanced by float additions
-        float f0 = (float)i;
-        for (; i < last; i += 2) {
-          const float f1 = f0 + 1.0f;
-          const float s0 = sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
-                                        fmaf(f0, ray.dz, ray.sz));
-          const float s1 = sample<kFmt>(vv, fmaf(f1, ray.dx, ray.sx), fmaf(f1, ray.dy, ray.sy),
-                                        fmaf(f1, ray.dz, ray.sz));
-          acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
-          f0 = f0 + 2.0f;
-        }
-        if (i == last) {
-          const float f0 = (float)i;
-          acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
-                                                  fmaf(f0, ray.dz, ray.sz)));
-        }
-      }
+      acc = march<kGeom, kOp, kFmt>(ray, vv, i, last, acc);
       acc = acc * it.step;
     }
   }
@@ -716,7 +749,10 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
   const char* reuse_env = std::getenv("XRH_CELL_REUSE");
   const bool reuse = reuse_env ? (std::strcmp(reuse_env, "0") != 0)
                                : ((int64_t)grid.x * grid.y >= 16ll * device_cus());
-  const int fmt = (L.storage == XRH_F32) ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6) : (L.layout == kLayoutQuad ? 1 : 0))
+  const int fmt = (L.storage == XRH_F32)
+                      ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6)
+                                                 : (L.layout == kLayoutFace ? (reuse ? 11 : 10)
+                                                                            : (L.layout == kLayoutQuad ? 1 : 0)))
                 : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
                                          : (L.layout == kLayoutOct ? 5 : 4);
   switch (fmt) {
@@ -727,7 +763,9 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
     case 4: XRH_LAUNCH_FMT(4); break;
     case 5: XRH_LAUNCH_FMT(5); break;
     case 6: XRH_LAUNCH_FMT(6); break;
-    default: XRH_LAUNCH_FMT(7); break;
+    case 7: XRH_LAUNCH_FMT(7); break;
+    case 10: XRH_LAUNCH_FMT(10); break;
+    default: XRH_LAUNCH_FMT(11); break;
   }
 #undef XRH_LAUNCH_FMT
 }

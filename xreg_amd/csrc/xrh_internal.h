@@ -68,9 +68,12 @@ enum Layout : int32_t {
   kLayoutQuad = 1,    // f32 only: float4 (v[i,j,k], v[i+1,j,k], v[i,j+1,k], v[i+1,j+1,k]);
                       //   2 x 16-byte loads per sample (4x memory)
   kLayoutOct = 2,     // f16/bf16 only: all 8 corners in 16 bytes; 1 load per sample
-  kLayoutCoef = 3     // f32 only: the 8 coefficients of the cell's trilinear polynomial in
+  kLayoutCoef = 3,    // f32 only: the 8 coefficients of the cell's trilinear polynomial in
                       //   its local coordinates (32 bytes, 2 adjacent 16-byte loads per
                       //   sample, 8x memory); evaluation is 7 FMAs instead of 7 lerps
+  kLayoutFace = 4     // f32 only: bilinear coefficients of each voxel's face normal to x
+                      //   (over y, z; 16 bytes, 4x memory); a cell is the faces at x and x+1,
+                      //   two adjacent 16-byte loads, bilinear twice then linear along x
 };
 
 struct CamDev {

@@ -202,6 +202,12 @@ __global__ void build_layout_kernel(const float* __restrict__ src, int X, int Y,
     q.z = V(a, b + 1, c);
     q.w = V(a + 1, b + 1, c);
     ((float4*)dst)[i] = q;
+  } else if (layout == kLayoutFace) {
+    // face normal to x at (a, b, c): corners (b, c), (b+1, c), (b, c+1), (b+1, c+1) over y, z;
+    // bilinear coefficients f(v, w) = c0 + c1 v + c2 w + c3 v w formed in double, rounded once
+    const double v00 = V(a, b, c), v10 = V(a, b + 1, c), v01 = V(a, b, c + 1), v11 = V(a, b + 1, c + 1);
+    ((float4*)dst)[i] = make_float4((float)v00, (float)(v10 - v00), (float)(v01 - v00),
+                                    (float)((v11 - v01) - (v10 - v00)));
   } else if (layout == kLayoutCoef) {
     // f(u, v, w) = c0 + c1 u + c2 v + c3 u v + w (c4 + c5 u + c6 v + c7 u v) over the cell's
     // corners, differences taken in double and rounded once
@@ -302,8 +308,9 @@ static int default_layout(int storage)
   const char* env = std::getenv("XRH_VOLUME_LAYOUT");
   if (env && std::strcmp(env, "linear") == 0) return kLayoutLinear;
   if (env && std::strcmp(env, "quad") == 0 && storage == XRH_F32) return kLayoutQuad;
-  // FP32: trilinear coefficients (falls back to quad when the layout would pass 4 GiB)
-  return (storage == XRH_F32) ? kLayoutCoef : kLayoutOct;
+  if (env && std::strcmp(env, "coef") == 0 && storage == XRH_F32) return kLayoutCoef;
+  // FP32: x-face bilinear coefficients; 16-bit: all corners in one 16-byte element
+  return (storage == XRH_F32) ? kLayoutFace : kLayoutOct;
 }
 
 // HUToLinAttFilter::GenerateData (lib/image/xregHUToLinAtt.cpp:45-68): per voxel, in double,
@@ -331,7 +338,9 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
   for (int k = 0; k < 3; ++k) XRH_REQUIRE(size[k] >= 1 && size[k] < (1 << 20), "bad volume size");
   const bool auto_layout = layout < 0;
   if (auto_layout) layout = default_layout(storage);
-  if (layout == kLayoutQuad || layout == kLayoutCoef) XRH_REQUIRE(storage == XRH_F32, "layout needs FP32 storage");
+  if (layout == kLayoutQuad || layout == kLayoutCoef || layout == kLayoutFace) {
+    XRH_REQUIRE(storage == XRH_F32, "layout needs FP32 storage");
+  }
   if (layout == kLayoutOct) XRH_REQUIRE(storage != XRH_F32, "oct layout needs 16-bit storage");
   const int64_t X = size[0], Y = size[1], Z = size[2];
   const int64_t padded = (X + 2) * (Y + 2) * (Z + 2);
@@ -663,7 +672,7 @@ xrh_status xrh_volume_create_hu(xrh_ctx* ctx, const float* vox_hu, const int64_t
                                 double mu_water, double mu_air, xrh_volume** out)
 {
   return guarded([&] {
-    XRH_REQUIRE(layout >= -1 && layout <= 3, "bad layout");
+    XRH_REQUIRE(layout >= -1 && layout <= 4, "bad layout");
     const HuParams hu{hu_lower, mu_water, mu_air};
     volume_create(ctx, vox_hu, size, idx_to_phys, storage, layout, out, &hu);
   });
@@ -674,7 +683,7 @@ xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_
                                     xrh_volume** out)
 {
   return guarded([&] {
-    XRH_REQUIRE(layout >= 0 && layout <= 3, "bad layout");
+    XRH_REQUIRE(layout >= 0 && layout <= 4, "bad layout");
     volume_create(ctx, vox, size, idx_to_phys, storage, layout, out);
   });
 }
