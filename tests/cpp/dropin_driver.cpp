@@ -9,6 +9,8 @@
 //          float32 vol[X*Y*Z] fixed[n_views][rows*cols] extrins[n_views][16] poses[n_poses][16]
 // out.bin: float32 sims[n_poses] drr[n_views*n_poses*rows*cols] uint64 samples
 //          then (second obj_fn call with the poses reversed) float32 sims2[n_poses]
+//          then uint64 shared_bad shared_nonzero (a second caster accumulating into the first
+//          one's buffer: pixels != 2 x the reversed-pose DRR, and non-zero pixels)
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -96,11 +98,45 @@ int main(int argc, char** argv)
     FrameTransformList rev(poses.rbegin(), poses.rend());
     const std::vector<float> vals2 = obj(rev);
 
+    // A second caster accumulating into the first one's device buffer (use_other_proj_buf with
+    // the ACCUM store, xregRayCastBaseOCL.cpp:258-269): the shared buffer then holds
+    // DRR + DRR == 2 DRR bit for bit, and the owner's host view sees it.
+    const size_t n_pix = (size_t)nv * np * rows * cols;
+    std::vector<float> drr_rev(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + n_pix);
+    uint64_t shared_bad = 0, shared_nonzero = 0;
+    {
+      RayCasterLineIntHIP rc2(0, geom ? XRH_GEOM_FP64 : XRH_GEOM_FP32);
+      rc2.set_volume(vol);
+      rc2.set_camera_models(cams);
+      rc2.set_ray_step_size(step);
+      rc2.set_num_projs(rc->num_projs());
+      rc2.allocate_resources();
+      rc2.set_camera_model_proj_associations(rc->camera_model_proj_associations());
+      rc2.set_xforms_cam_to_itk_phys(rc->xforms_cam_to_itk_phys());
+      rc2.use_other_proj_buf(rc.get());
+      rc2.use_proj_store_accum_method();
+      rc2.compute(0);
+      const float* acc = rc->raw_host_pixel_buf();
+      for (size_t i = 0; i < n_pix; ++i) {
+        shared_bad += (acc[i] != 2.0f * drr_rev[i]);
+        shared_nonzero += (acc[i] != 0.0f);
+      }
+      bool threw = false;
+      try {
+        rc2.use_other_proj_buf(&rc2);
+      } catch (const StringMessageException&) {
+        threw = true;
+      }
+      if (!threw) shared_bad += 1u << 30;
+    }
+
     std::ofstream out(argv[2], std::ios::binary);
     out.write(reinterpret_cast<const char*>(vals.data()), sizeof(float) * vals.size());
     out.write(reinterpret_cast<const char*>(drr), sizeof(float) * (size_t)nv * np * rows * cols);
     out.write(reinterpret_cast<const char*>(&samples), sizeof(samples));
     out.write(reinterpret_cast<const char*>(vals2.data()), sizeof(float) * vals2.size());
+    out.write(reinterpret_cast<const char*>(&shared_bad), sizeof(shared_bad));
+    out.write(reinterpret_cast<const char*>(&shared_nonzero), sizeof(shared_nonzero));
     std::cout << "dropin_driver: " << np << " poses x " << nv << " views, obj_fn evals " << obj.num_obj_fn_evals()
               << "\n";
   } catch (const StringMessageException& e) {

@@ -39,6 +39,10 @@ def _run(tmp_path, w, fixed, metric, smooth, extr, geom=0):
     off = 4 * npose + 4 * nv * npose * rows * cols
     samples = int(np.frombuffer(raw, np.uint64, 1, off)[0])
     sims2 = np.frombuffer(raw, np.float32, npose, off + 8)
+    shared_bad, shared_nonzero = (int(v) for v in np.frombuffer(raw, np.uint64, 2, off + 8 + 4 * npose))
+    # a second caster sharing the first one's device buffer with the ACCUM store doubles it
+    assert shared_bad == 0
+    assert shared_nonzero > 0
     return sims, drr, samples, sims2
 
 

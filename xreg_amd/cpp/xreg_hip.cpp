@@ -348,6 +348,11 @@ void RayCasterLineIntHIP::compute(size_type vol_idx)
   if (interp_method_ != kRAY_CAST_INTERP_LINEAR) {
     throw UnsupportedOperationException();  // linear only, as RayCasterOCL (xregRayCastBaseOCL.cpp:338-341)
   }
+  const size_type need = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
+  if (need > (other_ ? other_->drr_capacity_ : drr_capacity_) || xforms_cam_to_itk_phys_.size() < num_projs_) {
+    // set_num_projs grew the count after allocate_resources, or the shared buffer is smaller
+    throw StringMessageException("projection buffer is smaller than num_projs projections");
+  }
   const auto cams = xrh_cams();
   std::vector<float> xforms(16 * num_projs_);
   for (size_type p = 0; p < num_projs_; ++p) std::memcpy(&xforms[16 * p], xforms_cam_to_itk_phys_[p].m.data(), 64);
@@ -373,17 +378,22 @@ void RayCasterLineIntHIP::compute(size_type vol_idx)
   const int32_t store = (proj_store_meth_ == kRAY_CAST_PIXEL_ACCUM) ? XRH_STORE_ACCUM : XRH_STORE_REPLACE;
   if (store == XRH_STORE_ACCUM && host_stale_ == false && ext_host_buf_) {
     // an external host buffer may have been written by the caller: it is the accumulation base
-    check_xrh(xrh_copy_h2d(ctx_, drr_dev_, ext_host_buf_, num_projs_ * camera_models_[0].num_det_rows *
-                                                              camera_models_[0].num_det_cols * sizeof(float)));
+    check_xrh(xrh_copy_h2d(ctx_, out_buf(), ext_host_buf_, need * sizeof(float)));
   }
   const int32_t k = (kernel_id_ == kRAY_CAST_LINE_INT_MAX_KERNEL) ? XRH_LINE_INT_MAX : XRH_LINE_INT_SUM;
   check_xrh(xrh_line_int(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(), c4p.data(),
-                         (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, drr_dev_));
+                         (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, out_buf()));
   host_stale_ = true;
+  if (other_) other_->host_stale_ = true;
 }
 
 uint64_t RayCasterLineIntHIP::sample_count(size_type vol_idx)
 {
+  const size_type need = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
+  if (need > (other_ ? other_->drr_capacity_ : drr_capacity_) || xforms_cam_to_itk_phys_.size() < num_projs_) {
+    // set_num_projs grew the count after allocate_resources, or the shared buffer is smaller
+    throw StringMessageException("projection buffer is smaller than num_projs projections");
+  }
   const auto cams = xrh_cams();
   std::vector<float> xforms(16 * num_projs_);
   for (size_type p = 0; p < num_projs_; ++p) std::memcpy(&xforms[16 * p], xforms_cam_to_itk_phys_[p].m.data(), 64);
@@ -399,7 +409,8 @@ void RayCasterLineIntHIP::sync_to_host()
   if (!host_stale_) return;
   const size_type n = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
   float* dst = ext_host_buf_ ? ext_host_buf_ : host_buf_.data();
-  check_xrh(xrh_copy_d2h(ctx_, dst, drr_dev_, n * sizeof(float)));
+  if (!ext_host_buf_ && host_buf_.size() < n) host_buf_.resize(n);
+  check_xrh(xrh_copy_d2h(ctx_, dst, out_buf(), n * sizeof(float)));
   host_stale_ = false;
 }
 
@@ -438,7 +449,15 @@ size_type RayCasterLineIntHIP::max_num_projs_possible() const
   return (size_type)((64ull << 30) / (per ? per : 1));
 }
 
-void RayCasterLineIntHIP::use_other_proj_buf(RayCaster*) { throw UnsupportedOperationException(); }
+void RayCasterLineIntHIP::use_other_proj_buf(RayCaster* other_ray_caster)
+{
+  // share another HIP caster's device DRR buffer (RayCasterOCL::use_other_proj_buf,
+  // xregRayCastBaseOCL.cpp:258-269); the other caster must outlive this one
+  auto* other = dynamic_cast<RayCasterLineIntHIP*>(other_ray_caster);
+  if (!other || other == this) throw StringMessageException("Incompatible ray caster to share buffer from!");
+  other_ = other;
+  host_stale_ = true;
+}
 
 // ------------------------------------------------------------------------------------------
 // ImgSimMetric2D

@@ -213,7 +213,7 @@ class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterfac
   void use_external_host_pixel_buf(void* buf) override;
   size_type max_num_projs_possible() const override;
   void use_other_proj_buf(RayCaster* other) override;
-  const float* device_pixel_buf() const override { return drr_dev_; }
+  const float* device_pixel_buf() const override { return out_buf(); }
   // exact trilinear sample count of the next compute() (roofline accounting)
   uint64_t sample_count(size_type vol_idx = 0);
 
@@ -223,6 +223,10 @@ class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterfac
  private:
   void sync_to_host();
   std::vector<xrh_camera> xrh_cams() const;
+  // the DRR device buffer written by compute(): our own, or the one shared by
+  // use_other_proj_buf
+  float* out_buf() const { return other_ ? other_->drr_dev_ : drr_dev_; }
+  RayCasterLineIntHIP* other_ = nullptr;
   xrh_ctx* ctx_;
   int32_t geom_, storage_;
   std::vector<xrh_volume*> dev_vols_;
