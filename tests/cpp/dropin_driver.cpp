@@ -93,7 +93,8 @@ int main(int argc, char** argv)
     Intensity2D3DObjFn obj(rc, sims, np);
     obj.setup();
     const std::vector<float> vals = obj(poses);
-    const float* drr = rc->raw_host_pixel_buf();
+    const size_t n_pix = (size_t)nv * np * rows * cols;
+    const std::vector<float> drr(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + n_pix);
     const uint64_t samples = static_cast<RayCasterLineIntHIP*>(rc.get())->sample_count(0);
     FrameTransformList rev(poses.rbegin(), poses.rend());
     const std::vector<float> vals2 = obj(rev);
@@ -101,7 +102,6 @@ int main(int argc, char** argv)
     // A second caster accumulating into the first one's device buffer (use_other_proj_buf with
     // the ACCUM store, xregRayCastBaseOCL.cpp:258-269): the shared buffer then holds
     // DRR + DRR == 2 DRR bit for bit, and the owner's host view sees it.
-    const size_t n_pix = (size_t)nv * np * rows * cols;
     std::vector<float> drr_rev(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + n_pix);
     uint64_t shared_bad = 0, shared_nonzero = 0;
     {
@@ -132,7 +132,7 @@ int main(int argc, char** argv)
 
     std::ofstream out(argv[2], std::ios::binary);
     out.write(reinterpret_cast<const char*>(vals.data()), sizeof(float) * vals.size());
-    out.write(reinterpret_cast<const char*>(drr), sizeof(float) * (size_t)nv * np * rows * cols);
+    out.write(reinterpret_cast<const char*>(drr.data()), sizeof(float) * n_pix);
     out.write(reinterpret_cast<const char*>(&samples), sizeof(samples));
     out.write(reinterpret_cast<const char*>(vals2.data()), sizeof(float) * vals2.size());
     out.write(reinterpret_cast<const char*>(&shared_bad), sizeof(shared_bad));
