@@ -88,15 +88,19 @@ def camera(focal: float, rows: int, cols: int, row_sp: float, col_sp: float,
     return c
 
 
-def camera_ke(K: np.ndarray, E: np.ndarray, rows: int, cols: int, row_sp: float, col_sp: float) -> Camera:
+def camera_ke(K: np.ndarray, E: np.ndarray, rows: int, cols: int, row_sp: float, col_sp: float,
+              coord_frame: int = FRAME_NEG_Z) -> Camera:
     """CameraModel::setup(intrins, extrins, nr, nc, rs, cs) (xregPerspectiveXform.cpp:300-330):
-    focal length from FocalLenFromIntrins (:186-190), inverse intrinsics, SE3Inv extrinsics,
-    pinhole = extrins_inv translation (NEG_Z frame)."""
+    focal length from FocalLenFromIntrins (:186-190), inverse intrinsics, SE3Inv extrinsics;
+    pinhole = extrins_inv translation with the origin at the focal point (POS_Z / NEG_Z), else
+    extrins_inv * [0, 0, focal_len] (origin on the detector)."""
+    if coord_frame not in (FRAME_POS_Z, FRAME_NEG_Z, FRAME_ON_DETECTOR):
+        raise ValueError(f"unknown camera coordinate frame {coord_frame}")
     K = np.asarray(K, dtype=F)
     c = Camera()
     c.num_det_rows = int(rows)
     c.num_det_cols = int(cols)
-    c.coord_frame = FRAME_NEG_Z
+    c.coord_frame = int(coord_frame)
     f = F(F(abs(F(K[0, 0] * F(col_sp))) + abs(F(K[1, 1] * F(row_sp)))) / F(2))
     c.focal_len = float(f)
     kinv = _mat3_inverse_f32(K)
@@ -106,7 +110,11 @@ def camera_ke(K: np.ndarray, E: np.ndarray, rows: int, cols: int, row_sp: float,
     for i in range(16):
         c.extrins_inv[i] = float(einv.reshape(-1)[i])
     for i in range(3):
-        c.pinhole[i] = float(einv[i, 3])
+        if coord_frame == FRAME_ON_DETECTOR:
+            # affine * (0, 0, f): the linear part (only column 2 is non-zero) then the translation
+            c.pinhole[i] = float(F(F(einv[i, 2] * f) + einv[i, 3]))
+        else:
+            c.pinhole[i] = float(einv[i, 3])
     c.det_row_spacing = float(F(row_sp))
     c.det_col_spacing = float(F(col_sp))
     c.K = K.copy()

@@ -139,6 +139,7 @@ class SO3OptVarsOnly(SE3OptVars):
     num_params = 1
 
     def __init__(self, axis: int):
+        self.axis = int(axis)
         self.fn = (_rx, _ry, _rz)[axis]
 
     def __call__(self, x):
