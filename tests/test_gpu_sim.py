@@ -19,7 +19,8 @@ def images(oracle):
     return fixed, drr
 
 
-@pytest.mark.parametrize("metric,ksize", [(0, 0), (1, 0), (1, 5), (2, 0), (3, 0), (3, 5), (3, 3)])
+@pytest.mark.parametrize("metric,ksize", [(0, 0), (1, 0), (1, 3), (1, 5), (1, 7), (2, 0), (3, 0), (3, 5), (3, 3),
+                                          (3, 7)])
 def test_sim_matches_cpu_reference(ctx, oracle, images, metric, ksize):
     from xreg_amd import _xrh as X
 
@@ -37,6 +38,25 @@ def test_sim_matches_cpu_reference(ctx, oracle, images, metric, ksize):
           f"|cpu-fp64| {np.abs(ref - ref64).max():.3e}")
     assert np.abs(got - ref).max() <= SIM_ABS_TOL
     assert np.abs(got - ref64).max() <= SIM_ABS_TOL
+
+
+@pytest.mark.parametrize("radius,stride", [(2, 1), (3, 1), (4, 2), (5, 2), (6, 1), (7, 3), (8, 1)])
+@pytest.mark.parametrize("metric", [2, 3])
+def test_patch_radius_and_stride(ctx, oracle, images, metric, radius, stride):
+    """Patch diameters with a compiled kernel (7 ... 15) and without one (5, 17), strides 1-3."""
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    print(f"case: patch metric={metric} radius={radius} stride={stride}", flush=True)
+    ref = oracle.sim_f32(metric, fixed, mov, smooth_ksize=5, radius=radius, stride=stride)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, metric, smooth_ksize=5, patch_radius=radius, patch_stride=stride)
+    got = f.compute(buf, mov.shape[0])
+    f.close()
+    buf.free()
+    print(f"max |gpu-cpu| {np.abs(got - ref).max():.3e}")
+    assert np.abs(got - ref).max() <= SIM_ABS_TOL
 
 
 def test_ncc_mask(ctx, oracle, images):

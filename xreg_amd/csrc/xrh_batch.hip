@@ -260,13 +260,18 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
     off += (int64_t)rows[i] * cols[i];
   }
   XRH_REQUIRE(drr_out || off <= b->max_items * b->cap_npix, "DRR arena overflow");
-  // items grouped by metric, in item order
-  for (int m = XRH_SIM_NCC; m <= XRH_SIM_PATCH_GRAD_NCC; ++m) {
+  // items grouped by launch key (metric, then kernel variant), in item order within a group
+  std::vector<int> keys;
+  for (int64_t i = 0; i < n; ++i) keys.push_back(sim_group_key(b->views[item_view[i]].fixed->host));
+  std::vector<int> distinct(keys);
+  std::sort(distinct.begin(), distinct.end());
+  distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+  for (int key : distinct) {
     std::vector<int64_t> idx;
     for (int64_t i = 0; i < n; ++i) {
-      if (b->views[item_view[i]].fixed->host.metric == m) idx.push_back(i);
+      if (keys[(size_t)i] == key) idx.push_back(i);
     }
-    if (!idx.empty()) groups.emplace_back(m, std::move(idx));
+    groups.emplace_back(key, std::move(idx));
   }
   // SimItem table laid out group by group
   std::vector<SimItem> tmp(sin, sin + n);
@@ -315,7 +320,7 @@ void enqueue(xrh_batch* b, const DrrLaunch& D, const std::vector<std::pair<int, 
   for (auto& g : groups) {
     const int64_t cnt = (int64_t)g.second.size();
     // NCC sums were written by the DRR epilogue (records keyed by item index)
-    const bool fused = (g.first == XRH_SIM_NCC) && b->fuse_ncc;
+    const bool fused = ((g.first & ((1 << kSimKeyShift) - 1)) == XRH_SIM_NCC) && b->fuse_ncc;
     launch_sim(g.first, sim_items + k, cnt, b->max_img_tiles, b->max_patch_tiles,
                fused ? b->ncc_partials_dev : b->partials_dev, b->sims_dev, fused, b->max_lds, s);
     k += cnt;

@@ -44,10 +44,17 @@ int sim_record_stride(int metric);
 int sim_tiles(const FixedDev& f);     // tiles of the partial-producing kernel
 int sim_img_tiles(const FixedDev& f); // 16x16 image tiles
 
-// Queues the similarity kernels for n_items moving images that all use `metric`.
+// Launch-group key of a fixed image: the metric in the low bits; above kSimKeyShift, for
+// unmasked stride-1 patch metrics whose diameter has a compiled kernel, that diameter (else
+// 0); above kSimKeyBlurShift, the gradient metrics' blur radius (0 for none).
+constexpr int kSimKeyShift = 4;
+constexpr int kSimKeyBlurShift = 9;
+int sim_group_key(const FixedDev& f);
+
+// Queues the similarity kernels for n_items moving images that all share the group key `key`.
 // partials: n_items * ptiles * record-stride doubles. If fused_ncc_done, the NCC sums were
 // already written by the DRR epilogue.
-void launch_sim(int metric, const SimItem* items_dev, int64_t n_items, int max_img_tiles,
+void launch_sim(int key, const SimItem* items_dev, int64_t n_items, int max_img_tiles,
                 int max_patch_tiles, double* partials, float* sim_out_dev, bool fused_ncc_done,
                 int patch_lds_bytes, hipStream_t s);
 // dynamic LDS bytes the patch kernel needs for a fixed image's patch geometry

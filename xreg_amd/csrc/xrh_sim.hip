@@ -143,6 +143,88 @@ __device__ inline void tile_gradients(const float* __restrict__ img, int rows, i
   gy = syp - sym;
 }
 
+typedef const FixedDev __attribute__((address_space(4)))* ConstFixed;
+
+__device__ inline __amdgpu_buffer_rsrc_t ro_rsrc(const void* p, int64_t bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline float ld_f32(__amdgpu_buffer_rsrc_t r, int byte_off)
+{
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+__device__ inline double ld_f64(__amdgpu_buffer_rsrc_t r, int byte_off)
+{
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+  return __hiloint2double((int)v[1], (int)v[0]);
+}
+
+// tile_gradients with the blur radius kRb (0: no blur) fixed at compile time: the window
+// reads are issued together before any is used, and the LDS passes have constant trip counts.
+// Same arithmetic, in the same order, as tile_gradients.
+template <int kRb>
+__device__ inline void tile_gradients_k(const float* __restrict__ img, int rows, int cols, int x0, int y0,
+                                        float& gx, float& gy, bool& inside)
+{
+#pragma clang fp contract(off)
+  constexpr int kHalo = kRb + 1;
+  constexpr int kW = kT + 2 * kHalo;  // <= 24
+  constexpr int kB = kT + 2;
+  constexpr int kRows = (kW + 7) / 8;
+  constexpr int ksize = 2 * kRb + 1;
+  __shared__ float raw[kW][kW];
+  __shared__ float tmp[kRb > 0 ? kW : 1][kB];
+  __shared__ float smb[kRb > 0 ? kB : 1][kB];
+  const int lx = threadIdx.x & 31, ly0 = threadIdx.x >> 5;
+  const int xc = reflect101(x0 - kHalo + min(lx, kW - 1), cols);
+  const auto rs = ro_rsrc(img, (int64_t)rows * cols * 4);
+  float v[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int y = reflect101(y0 - kHalo + min(ly0 + 8 * r, kW - 1), rows);
+    v[r] = ld_f32(rs, (y * cols + xc) * 4);
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    if (lx < kW && ly0 + 8 * r < kW) raw[ly0 + 8 * r][lx] = v[r];
+  }
+  __syncthreads();
+  float (*sm)[kB] = nullptr;
+  if constexpr (kRb > 0) {
+    for (int i = threadIdx.x; i < kW * kB; i += kThreads) {
+      const int ly = i / kB, bx = i % kB;
+      const int lxx = bx + kRb;
+      float acc = gauss_k(ksize, kRb) * raw[ly][lxx];
+#pragma unroll
+      for (int j = 1; j <= kRb; ++j) acc = acc + gauss_k(ksize, kRb + j) * (raw[ly][lxx - j] + raw[ly][lxx + j]);
+      tmp[ly][bx] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kB * kB; i += kThreads) {
+      const int by = i / kB, bx = i % kB;
+      const int ly = by + kRb;
+      float acc = gauss_k(ksize, kRb) * tmp[ly][bx];
+#pragma unroll
+      for (int j = 1; j <= kRb; ++j) acc = acc + gauss_k(ksize, kRb + j) * (tmp[ly - j][bx] + tmp[ly + j][bx]);
+      smb[by][bx] = acc;
+    }
+    __syncthreads();
+    sm = smb;
+  } else {
+    sm = raw;  // kW == kB: the Sobel reads the raw window directly
+  }
+  const int tx = threadIdx.x % kT, ty = threadIdx.x / kT;
+  inside = (x0 + tx < cols) && (y0 + ty < rows);
+  const int cy = ty + 1, cx = tx + 1;
+  const float dxm = sm[cy - 1][cx + 1] - sm[cy - 1][cx - 1];
+  const float dx0 = sm[cy][cx + 1] - sm[cy][cx - 1];
+  const float dxp = sm[cy + 1][cx + 1] - sm[cy + 1][cx - 1];
+  gx = (dxm + dxp) + dx0 * 2.0f;
+  const float sym = (sm[cy - 1][cx - 1] + sm[cy - 1][cx + 1]) + sm[cy - 1][cx] * 2.0f;
+  const float syp = (sm[cy + 1][cx - 1] + sm[cy + 1][cx + 1]) + sm[cy + 1][cx] * 2.0f;
+  gy = syp - sym;
+}
+
 // ---- kernels over SimItem tables: blockIdx.x = tile, blockIdx.y = item ----
 
 // NCC sums over raw moving images: per tile (sum m, sum m^2, sum m f~) [masked]
@@ -169,6 +251,7 @@ __global__ __launch_bounds__(kThreads) void ncc_sums_kernel(const SimItem* __res
 }
 
 // Grad-NCC sums: gradients in LDS, then sums for x and y directions
+template <int kRb>
 __global__ __launch_bounds__(kThreads) void grad_ncc_sums_kernel(const SimItem* __restrict__ items,
                                                                  double* __restrict__ partials, int ptiles)
 {
@@ -179,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void grad_ncc_sums_kernel(const SimItem* 
   const int x0 = (blockIdx.x % TX) * kT, y0 = (blockIdx.x / TX) * kT;
   float gx, gy;
   bool inside;
-  tile_gradients(it.mov, fx.rows, fx.cols, x0, y0, fx.smooth_ksize, gx, gy, inside);
+  tile_gradients_k<kRb>(it.mov, fx.rows, fx.cols, x0, y0, gx, gy, inside);
   double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (inside) {
     const int64_t o = (int64_t)(y0 + threadIdx.x / kT) * fx.cols + x0 + (threadIdx.x % kT);
@@ -197,6 +280,7 @@ __global__ __launch_bounds__(kThreads) void grad_ncc_sums_kernel(const SimItem* 
 }
 
 // Gradient images of moving images (for the patch metric), written to it.gx / it.gy.
+template <int kRb>
 __global__ __launch_bounds__(kThreads) void grad_images_kernel(const SimItem* __restrict__ items)
 {
   const SimItem it = items[blockIdx.y];
@@ -206,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void grad_images_kernel(const SimItem* __
   const int x0 = (blockIdx.x % TX) * kT, y0 = (blockIdx.x / TX) * kT;
   float gx, gy;
   bool inside;
-  tile_gradients(it.mov, fx.rows, fx.cols, x0, y0, fx.smooth_ksize, gx, gy, inside);
+  tile_gradients_k<kRb>(it.mov, fx.rows, fx.cols, x0, y0, gx, gy, inside);
   if (inside) {
     const int64_t o = (int64_t)(y0 + threadIdx.x / kT) * fx.cols + x0 + (threadIdx.x % kT);
     it.gx[o] = gx;
@@ -314,6 +398,118 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
         const double cross = (Sf - (double)pmu[p] * SU) - mean * pg[p];
         acc[dir] = w * (1.0 - (cross / sd) / (double)pden[p]);
       }
+    }
+  }
+  block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
+}
+
+// patch_kernel for the common case -- no mask, stride 1, a compile-time patch diameter kD:
+// the same sums and closed form with every index known at compile time. All global reads of
+// the block (both gradient directions' moving and fixed windows and the per-patch fixed
+// statistics) are issued before the first one is waited for, so their latency overlaps;
+// staging walks the window as 8 rows x 32 columns of lanes, the horizontal and vertical
+// d-wide sums are unrolled with constant LDS offsets, and the closed form uses reciprocals
+// of N and N - 1 and one division.
+template <bool kTwo, int kD>
+__global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __restrict__ items,
+                                                           double* __restrict__ partials, int ptiles)
+{
+  constexpr int kW = (kT - 1) + kD;  // window edge at stride 1
+  constexpr int kRows = (kW + 7) / 8;
+  constexpr int kDirs = kTwo ? 2 : 1;
+  static_assert(kW <= 32, "one 32-lane column pass per window row");
+  extern __shared__ double patch_lds[];
+  const SimItem it = items[blockIdx.y];
+  const ConstFixed fx = (ConstFixed)it.fixed;  // uniform: scalar loads
+  const int rows = fx->rows, cols = fx->cols, ppr = fx->ppr, ppc = fx->ppc;
+  const int TX = (ppr + kT - 1) / kT, TY = (ppc + kT - 1) / kT;
+  if ((int)blockIdx.x >= TX * TY) return;
+  const int X0 = (blockIdx.x % TX) * kT, Y0 = (blockIdx.x / TX) * kT;  // first patch = first pixel
+  double* lm = patch_lds;
+  double* lf = lm + kW * kW;
+  double* h1 = lf + kW * kW;
+  double* h2 = h1 + kW * kT;
+  double* hf = h2 + kW * kT;
+  const int pxl = threadIdx.x % kT, pyl = threadIdx.x / kT;
+  const int pi = X0 + pxl, pj = Y0 + pyl;
+  const bool valid = (pi < ppr) && (pj < ppc);
+  const int p = valid ? pj * ppr + pi : 0;  // invalid lanes read patch 0 and discard it
+  constexpr double kN = (double)(kD * kD);
+  constexpr double kInvN = 1.0 / kN, kInvN1 = 1.0 / (kN - 1.0);
+  const int lx = threadIdx.x & 31, ly0 = threadIdx.x >> 5;
+  const bool col_ok = lx < kW;
+  // lanes outside the window read a clamped in-image pixel and do not store it
+  const int xo = min(X0 + min(lx, kW - 1), cols - 1);
+  const int64_t img_bytes = (int64_t)rows * cols * 4;
+  const int64_t np = (int64_t)ppr * ppc;
+
+  // every global read of the block, issued before any is consumed
+  float vm[kDirs][kRows], vf[kDirs][kRows];
+  float mu[kDirs], den[kDirs];
+  double g[kDirs];
+#pragma unroll
+  for (int dir = 0; dir < kDirs; ++dir) {
+    const auto rm = ro_rsrc((dir == 0) ? (kTwo ? it.gx : it.mov) : it.gy, img_bytes);
+    const auto rf = ro_rsrc((dir == 0) ? fx->pf : fx->pfy, img_bytes);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int ly = min(ly0 + 8 * r, kW - 1);
+      const int o = (min(Y0 + ly, rows - 1) * cols + xo) * 4;
+      vm[dir][r] = ld_f32(rm, o);
+      vf[dir][r] = ld_f32(rf, o);
+    }
+    mu[dir] = ld_f32(ro_rsrc((dir == 0) ? fx->pmean : fx->pmeany, np * 4), p * 4);
+    den[dir] = ld_f32(ro_rsrc((dir == 0) ? fx->pden : fx->pdeny, np * 4), p * 4);
+    g[dir] = ld_f64(ro_rsrc((dir == 0) ? fx->pg : fx->pgy, np * 8), p * 8);
+  }
+
+  double acc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int dir = 0; dir < kDirs; ++dir) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int ly = ly0 + 8 * r;
+      if (col_ok && ly < kW) {
+        lm[ly * kW + lx] = (double)vm[dir][r];
+        lf[ly * kW + lx] = (double)vf[dir][r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int row0 = 0; row0 < kW; row0 += kThreads / kT) {
+      const int row = row0 + (threadIdx.x >> 4);
+      if (row < kW) {
+        const int c = threadIdx.x & (kT - 1);
+        const double* m = lm + row * kW + c;
+        const double* f = lf + row * kW + c;
+        double a1 = 0.0, a2 = 0.0, af = 0.0;
+#pragma unroll
+        for (int xx = 0; xx < kD; ++xx) {
+          a1 += m[xx];
+          a2 = fma(m[xx], m[xx], a2);
+          af = fma(m[xx], f[xx], af);
+        }
+        h1[row * kT + c] = a1;
+        h2[row * kT + c] = a2;
+        hf[row * kT + c] = af;
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      const int e0 = pyl * kT + pxl;
+      double S1 = 0.0, S2 = 0.0, Sf = 0.0;
+#pragma unroll
+      for (int yy = 0; yy < kD; ++yy) {
+        S1 += h1[e0 + yy * kT];
+        S2 += h2[e0 + yy * kT];
+        Sf += hf[e0 + yy * kT];
+      }
+      const double mean = S1 * kInvN;
+      const double var = fmax(S2 - mean * S1, 0.0) * kInvN1;
+      const double sd = fmax((double)1.0e-6f, sqrt(var));
+      const double cross = (Sf - (double)mu[dir] * S1) - mean * g[dir];
+      acc[dir] = 1.0 - cross / (sd * (double)den[dir]);
     }
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
@@ -524,6 +720,29 @@ int sim_record_stride(int metric)
 
 int sim_img_tiles(const FixedDev& f) { return ((f.cols + kT - 1) / kT) * ((f.rows + kT - 1) / kT); }
 
+int sim_group_key(const FixedDev& f)
+{
+  int d = 0;
+  // (the specialised kernel addresses images and patch tables with 32-bit byte offsets)
+  const bool small = (int64_t)f.rows * f.cols * 4 < (int64_t)INT32_MAX && f.num_patches * 8 < (int64_t)INT32_MAX;
+  if ((f.metric == XRH_SIM_PATCH_NCC || f.metric == XRH_SIM_PATCH_GRAD_NCC) && !f.pw && f.patch_stride == 1 && small) {
+    const int dd = 2 * f.patch_radius + 1;
+    if (dd == 7 || dd == 9 || dd == 11 || dd == 13 || dd == 15) d = dd;
+  }
+  const int rb = (f.metric == XRH_SIM_GRAD_NCC || f.metric == XRH_SIM_PATCH_GRAD_NCC) && f.smooth_ksize > 1
+                     ? f.smooth_ksize / 2 : 0;
+  return f.metric | (d << kSimKeyShift) | (rb << kSimKeyBlurShift);
+}
+
+#define XRH_BLUR_DISPATCH(rb, KERNEL, ...)                                            \
+  switch (rb) {                                                                       \
+    case 0: hipLaunchKernelGGL(KERNEL<0>, __VA_ARGS__); break;                        \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                        \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                        \
+    case 3: hipLaunchKernelGGL(KERNEL<3>, __VA_ARGS__); break;                        \
+    default: fail(XRH_ERR_INVALID, "blur radius out of range");                       \
+  }
+
 int sim_patch_lds_bytes(const FixedDev& f)
 {
   if (f.metric != XRH_SIM_PATCH_NCC && f.metric != XRH_SIM_PATCH_GRAD_NCC) return 0;
@@ -539,11 +758,29 @@ int sim_tiles(const FixedDev& f)
   return sim_img_tiles(f);
 }
 
-void launch_sim(int metric, const SimItem* items_dev, int64_t n_items, int max_img_tiles,
+template <bool kTwo>
+static void launch_patch_d(int d, dim3 grid, int lds, hipStream_t s, const SimItem* items, double* partials,
+                           int ptiles)
+{
+  switch (d) {
+    case 7: hipLaunchKernelGGL((patch_kernel_d<kTwo, 7>), grid, dim3(kThreads), lds, s, items, partials, ptiles); break;
+    case 9: hipLaunchKernelGGL((patch_kernel_d<kTwo, 9>), grid, dim3(kThreads), lds, s, items, partials, ptiles); break;
+    case 11: hipLaunchKernelGGL((patch_kernel_d<kTwo, 11>), grid, dim3(kThreads), lds, s, items, partials, ptiles); break;
+    case 13: hipLaunchKernelGGL((patch_kernel_d<kTwo, 13>), grid, dim3(kThreads), lds, s, items, partials, ptiles); break;
+    case 15: hipLaunchKernelGGL((patch_kernel_d<kTwo, 15>), grid, dim3(kThreads), lds, s, items, partials, ptiles); break;
+    default: fail(XRH_ERR_INVALID, "no specialised patch kernel for this diameter");
+  }
+}
+
+void launch_sim(int key, const SimItem* items_dev, int64_t n_items, int max_img_tiles,
                 int max_patch_tiles, double* partials, float* sim_out_dev, bool fused_ncc_done, int patch_lds_bytes,
                 hipStream_t s)
 {
   if (n_items <= 0) return;
+  const int metric = key & ((1 << kSimKeyShift) - 1);
+  // compile-time patch diameter (0: the general patch kernel) and blur radius of the group
+  const int dfix = (key >> kSimKeyShift) & ((1 << (kSimKeyBlurShift - kSimKeyShift)) - 1);
+  const int rb = key >> kSimKeyBlurShift;
   XRH_REQUIRE(n_items < 65536, "too many moving images in one similarity launch");
   const bool patch = (metric == XRH_SIM_PATCH_NCC || metric == XRH_SIM_PATCH_GRAD_NCC);
   const int ptiles = patch ? max_patch_tiles : max_img_tiles;
@@ -554,15 +791,23 @@ void launch_sim(int metric, const SimItem* items_dev, int64_t n_items, int max_i
       if (!fused_ncc_done) hipLaunchKernelGGL(ncc_sums_kernel, grid, dim3(kThreads), 0, s, items_dev, partials, ptiles);
       break;
     case XRH_SIM_GRAD_NCC:
-      hipLaunchKernelGGL(grad_ncc_sums_kernel, grid, dim3(kThreads), 0, s, items_dev, partials, ptiles);
+      XRH_BLUR_DISPATCH(rb, grad_ncc_sums_kernel, grid, dim3(kThreads), 0, s, items_dev, partials, ptiles);
       break;
     case XRH_SIM_PATCH_NCC:
-      hipLaunchKernelGGL(patch_kernel<false>, grid, dim3(kThreads), patch_lds_bytes, s, items_dev, partials, ptiles);
+      if (dfix) {
+        launch_patch_d<false>(dfix, grid, patch_lds_bytes, s, items_dev, partials, ptiles);
+      } else {
+        hipLaunchKernelGGL(patch_kernel<false>, grid, dim3(kThreads), patch_lds_bytes, s, items_dev, partials, ptiles);
+      }
       break;
     case XRH_SIM_PATCH_GRAD_NCC:
-      hipLaunchKernelGGL(grad_images_kernel, dim3((unsigned)max_img_tiles, (unsigned)n_items), dim3(kThreads), 0, s,
-                         items_dev);
-      hipLaunchKernelGGL(patch_kernel<true>, grid, dim3(kThreads), patch_lds_bytes, s, items_dev, partials, ptiles);
+      XRH_BLUR_DISPATCH(rb, grad_images_kernel, dim3((unsigned)max_img_tiles, (unsigned)n_items), dim3(kThreads), 0,
+                        s, items_dev);
+      if (dfix) {
+        launch_patch_d<true>(dfix, grid, patch_lds_bytes, s, items_dev, partials, ptiles);
+      } else {
+        hipLaunchKernelGGL(patch_kernel<true>, grid, dim3(kThreads), patch_lds_bytes, s, items_dev, partials, ptiles);
+      }
       break;
     default:
       fail(XRH_ERR_INVALID, "unknown similarity metric");
@@ -633,6 +878,8 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
 {
   XRH_REQUIRE(ctx && fixed_host && p && out, "null argument");
   XRH_REQUIRE(rows >= 2 && cols >= 2, "fixed image must be at least 2x2");
+  // the gradient and patch kernels address an image with 32-bit byte offsets
+  XRH_REQUIRE((int64_t)rows * cols * 4 < (int64_t)INT32_MAX, "fixed image has 2^29 pixels or more");
   XRH_REQUIRE(p->metric >= XRH_SIM_NCC && p->metric <= XRH_SIM_PATCH_GRAD_NCC, "unknown metric");
   const bool grad = (p->metric == XRH_SIM_GRAD_NCC || p->metric == XRH_SIM_PATCH_GRAD_NCC);
   const bool patch = (p->metric == XRH_SIM_PATCH_NCC || p->metric == XRH_SIM_PATCH_GRAD_NCC);
@@ -792,7 +1039,7 @@ static void sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_d
   }
   XRH_HIP_CHECK(hipMemcpyAsync(dev + off_items, hi, sizeof(SimItem) * num_mov, hipMemcpyHostToDevice, ctx->stream));
   float* sims = sim_dev_out ? sim_dev_out : (float*)(dev + off_sim);
-  launch_sim(h.metric, (const SimItem*)(dev + off_items), num_mov, img_tiles, ptiles, (double*)(dev + off_part),
+  launch_sim(sim_group_key(h), (const SimItem*)(dev + off_items), num_mov, img_tiles, ptiles, (double*)(dev + off_part),
              sims, false, sim_patch_lds_bytes(h), ctx->stream);
   if (sim_host_out) {
     XRH_HIP_CHECK(hipMemcpyAsync(sim_host_out, sims, sizeof(float) * num_mov, hipMemcpyDeviceToHost, ctx->stream));
