@@ -214,6 +214,27 @@ def test_drr_multiview_extrinsics(ctx, oracle):
     assert not np.allclose(got[0], got[n])
 
 
+@pytest.mark.parametrize("frame", [0, 2])
+def test_drr_other_camera_frames(ctx, oracle, c1s, frame):
+    """Detector at z = +f (POS_Z) and origin on the detector: the pose is turned half a turn about
+    x so the volume sits between source and detector."""
+    from xreg_amd import geometry as G
+
+    c0 = c1s.cams[0]
+    rows, cols = c0.num_det_rows, c0.num_det_cols
+    K = G.naive_intrins(1020.0, rows, cols, c0.det_row_spacing, c0.det_col_spacing)
+    K[0, 0] *= -1
+    K[1, 1] *= -1
+    cam = G.camera_ke(K, np.eye(4), rows, cols, c0.det_row_spacing, c0.det_col_spacing, coord_frame=frame)
+    flip = G.rigid(G.rot_x(np.pi))
+    poses = np.stack([p @ flip for p in np.concatenate([c1s.gt[None], c1s.poses[:2]])])
+    print(f"case: camera frame {frame}, 3 poses", flush=True)
+    ref, _ = oracle.drr_f32(c1s.vol, c1s.idx_to_phys, [cam], poses_f32(poses), step=c1s.step)
+    got = _gpu_drr(ctx, c1s, poses, cams=[cam])
+    assert ref.max() > 0
+    assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
+
+
 def test_hu_volume_matches_host_conversion(ctx, oracle):
     """A volume created from Hounsfield units (converted on the device) casts the same DRR,
     bit for bit, as one created from the oracle's host conversion."""

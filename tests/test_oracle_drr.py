@@ -123,6 +123,27 @@ def test_axis_aligned_central_ray(oracle):
     assert abs(float(drr[0, r, c]) - want) < 1e-5 * want
 
 
+def test_pos_z_frame_is_the_neg_z_image_upside_down(oracle):
+    """A POS_Z camera (positive-focal intrinsics, detector at z = +f) with the pose turned half a
+    turn about x sees the volume from the same source along mirrored directions: its DRR is the
+    NEG_Z camera's with the rows reversed (the rays and their samples coincide)."""
+    from xreg_amd import geometry as G
+
+    X, Y, Z, sp, cam, gt, center = _geometry(odd=True)
+    vol = np.random.default_rng(5).uniform(0.0, 0.05, (Z, Y, X)).astype(np.float32)
+    rows, cols = cam.num_det_rows, cam.num_det_cols
+    K = G.naive_intrins(1020.0, rows, cols, cam.det_row_spacing, cam.det_col_spacing)
+    K[0, 0] *= -1
+    K[1, 1] *= -1
+    pz = G.camera_ke(K, np.eye(4), rows, cols, cam.det_row_spacing, cam.det_col_spacing, coord_frame=G.FRAME_POS_Z)
+    pose = G.perturbed_pose(gt, center, np.deg2rad([4.0, -3.0, 2.0]), [2.0, -1.0, 3.0])
+    i2p = np.diag([sp, sp, sp, 1.0])
+    neg, _ = oracle.drr_f32(vol, i2p, [cam], poses_f32(pose[None]), step=0.9)
+    pos, _ = oracle.drr_f32(vol, i2p, [pz], poses_f32((pose @ G.rigid(G.rot_x(np.pi)))[None]), step=0.9)
+    assert neg.max() > 0
+    assert np.abs(pos[0] - neg[0][::-1]).max() <= 1e-5 * neg.max()
+
+
 def test_store_methods_background_and_max(oracle):
     X, Y, Z, sp, cam, gt, center = _geometry()
     from xreg_amd import synth

@@ -32,6 +32,28 @@ def test_detector_points_closed_form(oracle, rows, cols, rs, cs):
     assert abs(p[0]) < 1e-4 and abs(p[1]) < 1e-4 and p[2] == np.float32(-f)
 
 
+@pytest.mark.parametrize("frame", [0, 2])
+def test_detector_points_other_frames(oracle, frame):
+    """Positive-focal intrinsics (MakeNaiveIntrins without the NEG_Z flip): with the origin at
+    the focal point (POS_Z) the detector is the plane z = +f and the pinhole the origin; with the
+    origin on the detector the detector is z = 0 and the pinhole (0, 0, f)."""
+    from xreg_amd import geometry as G
+
+    f, rows, cols, rs, cs = 1020.0, 41, 37, 0.6, 0.8
+    K = G.naive_intrins(f, rows, cols, rs, cs)
+    K[0, 0] *= -1
+    K[1, 1] *= -1
+    cam = G.camera_ke(K, np.eye(4), rows, cols, rs, cs, coord_frame=frame)
+    oc = oracle.cam_f(cam)
+    cx, cy = (cols - 1) / 2.0, (rows - 1) / 2.0
+    z = f if frame == 0 else 0.0
+    for (c, r) in [(0, 0), (cols - 1, rows - 1), (cx, cy), (5, 30)]:
+        p = oracle.cam_ind_to_det_f32(oc, c, r).astype(np.float64)
+        want = np.array([(c - cx) * cs, (r - cy) * rs, z])
+        assert np.allclose(p, want, rtol=0, atol=2e-4 * f), (c, r, p, want)
+    assert np.allclose(np.array(cam.pinhole[:]), [0.0, 0.0, 0.0 if frame == 0 else f], atol=1e-3)
+
+
 def test_detector_points_with_extrinsics(oracle):
     from xreg_amd import geometry as G
     from xreg_amd import synth
