@@ -47,6 +47,7 @@ def test_alone_equals_mixed(ctx, oracle):
         m = mixed[j]
         assert a.history == m.history, j
         assert np.array_equal(a.best_params, m.best_params), j
+        assert np.array_equal(a.params, m.params), j
     for v in vols.values():
         v.close()
 
@@ -72,7 +73,7 @@ def test_registration_improves_pose(ctx, oracle):
         pts = [w.center + 0.25 * ext * np.array(c) for c in [(-1, -1, -1), (1, 1, 1), (1, -1, 1), (-1, 1, -1),
                                                               (0, 0, 0)]]
         err0 = G.mean_projection_distance(s.init_pose, w.gt, pts, 1020.0)
-        err1 = G.mean_projection_distance(r.best_pose, w.gt, pts, 1020.0)
+        err1 = G.mean_projection_distance(r.pose, w.gt, pts, 1020.0)  # the CMA-ES mean
         print(f"job {s.job_id}: mean projection distance {err0:.2f} -> {err1:.2f} mm, sim {r.history[0]:.4f} -> "
               f"{r.best_sim:.4f}, latency {r.latency:.3f}s")
         assert r.best_sim < r.history[0]

@@ -2,6 +2,7 @@
 search's batching and first-minimum rule, the refiner's BOBYQA-shaped call pattern and
 convergence, CMA-ES convergence, and CombineMean's float arithmetic."""
 import numpy as np
+import pytest
 
 from xreg_amd import geometry as G
 from xreg_amd import regi
@@ -69,6 +70,28 @@ def test_cmaes_driver_converges():
     r = regi.run_cmaes(obj, lam=20, sigma=[0.1] * 3 + [5.0] * 3, max_gens=200, seed=1)
     assert all(s == 20 for s in obj.sizes)  # one submission per generation
     assert np.allclose(r.x, xstar, atol=1e-3)
+
+
+def test_cmaes_estimate_is_the_mean_and_debug_record(tmp_path):
+    """The estimate is the distribution mean after the last generation (as the reference); the
+    debug record holds one mean and one best-so-far value per generation, non-increasing."""
+    xstar = np.array([0.02, -0.01, 0.03, 1.0, -2.0, 0.5])
+    obj = QuadObj(xstar, [100.0] * 3 + [1.0] * 3, max_batch=64)
+    r = regi.run_cmaes(obj, lam=16, sigma=[0.05] * 3 + [2.0] * 3, max_gens=30, seed=4)
+    means, best = r.extra["iter_means"], r.extra["iter_best"]
+    assert means.shape == (len(best), 6) and np.array_equal(r.x, means[-1])
+    assert np.all(np.diff(best) <= 0) and best[-1] == r.fun
+    np.testing.assert_allclose(r.pose, obj.param(r.x), atol=0)
+    h5io = pytest.importorskip("xreg_amd.h5io")
+    if not h5io.available():
+        pytest.skip("HDF5 C library not available")
+    rec = regi.debug_results(r, obj.param, vol_index=0, regi_time_secs=1.25)
+    h5io.write_regi_debug(tmp_path / "dbg.h5", rec)
+    got = h5io.read_regi_debug(tmp_path / "dbg.h5")
+    assert np.array_equal(got.iter_vars[0], means.astype(np.float32))
+    assert np.array_equal(got.sims, best.astype(np.float32))
+    assert np.array_equal(got.final_poses[0], r.pose.astype(np.float32))
+    assert got.inter_frames_wrt_vol == [True] and type(got.se3_params).__name__ == "SE3OptVarsLieAlg"
 
 
 def test_combine_mean_is_float_sum_over_views_divided():

@@ -154,6 +154,7 @@ class InterFramePose:
         self.vars = opt_vars if opt_vars is not None else SE3OptVarsLieAlg()
         I = np.eye(4) if inter_frame is None else np.asarray(inter_frame, dtype=np.float64)
         g = np.asarray(guess, dtype=np.float64)
+        self.guess, self.inter_frame, self.inter_frame_wrt_vol = g.copy(), I.copy(), bool(inter_frame_wrt_vol)
         Iinv = np.linalg.inv(I)
         if inter_frame_wrt_vol:
             self.pre, self.post = I, Iinv @ g

@@ -104,11 +104,17 @@ class RegiResult:
 def run_cmaes(obj: Objective, lam: int, sigma, max_gens: int, seed: int, x0=None, bounds=None,
               tol_fun: float = 1e-12, tol_x: float = 1e-11) -> RegiResult:
     """Intensity2D3DRegiCMAES::run (xregIntensity2D3DRegiCMAES.cpp:75-261): lambda candidates per
-    generation evaluated in one submission, bounds by resampling."""
+    generation evaluated in one submission, bounds by resampling. As in the reference the
+    estimate is the distribution mean after the last generation (update_regi_xforms(
+    xforms_at_cur_mean()), :248); `fun` is the smallest value seen (its debug_sim_val_, :219).
+    Per generation, extra["iter_means"] holds the mean after the update and extra["iter_best"]
+    the smallest value so far -- the iter_vars / sims of the debug record (:340-384).
+    extra["best_x"] / ["best_pose"] keep the best candidate evaluated."""
     x0 = np.zeros(obj.param.num_params) if x0 is None else np.asarray(x0, dtype=np.float64)
     es = CMAES(x0, np.asarray(sigma, dtype=np.float64), lam, seed,
                bounds=None if bounds is None else np.asarray(bounds, dtype=np.float64), tol_fun=tol_fun, tol_x=tol_x)
     best_f, best_x = np.inf, x0.copy()
+    iter_means, iter_best = [], []
     e0 = obj.evals
     for _ in range(max_gens):
         Xc = es.ask()
@@ -117,9 +123,14 @@ def run_cmaes(obj: Objective, lam: int, sigma, max_gens: int, seed: int, x0=None
         if f[k] < best_f:
             best_f, best_x = float(f[k]), np.asarray(Xc[k]).copy()
         es.tell(f)
+        iter_means.append(es.m.copy())
+        iter_best.append(best_f)
         if es.stop():
             break
-    return RegiResult(best_x, best_f, obj.param(best_x), obj.evals - e0, list(es.history))
+    xm = es.m.copy()
+    return RegiResult(xm, best_f, obj.param(xm), obj.evals - e0, list(es.history),
+                      extra={"best_x": best_x, "best_pose": obj.param(best_x),
+                             "iter_means": np.array(iter_means), "iter_best": np.array(iter_best)})
 
 
 def mesh_grid(axes) -> np.ndarray:
@@ -264,3 +275,22 @@ def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels,
         results.append(r)
         pose = r.pose
     return pose, results
+
+
+def debug_results(res: RegiResult, param: InterFramePose, vol_index: int = 0, projs_used=(),
+                  regi_time_secs: float | None = None):
+    """The SingleRegiDebugResults record of one CMA-ES run (xregIntensity2D3DRegiDebug.h:45-141,
+    filled as Intensity2D3DRegi::run and the CMA-ES debug hooks fill it): parameterisation,
+    guess, final pose, intermediate frame, per-generation means and best-so-far values. Write it
+    with h5io.write_regi_debug."""
+    from . import h5io
+
+    means = res.extra.get("iter_means")
+    best = res.extra.get("iter_best")
+    if means is None or best is None:
+        raise ValueError("debug record needs a CMA-ES result (per-generation means and values)")
+    return h5io.RegiDebugResults(
+        se3_params=param.vars, vols_used=[int(vol_index)], init_poses=[np.asarray(param.guess, np.float32)],
+        final_poses=[np.asarray(res.pose, np.float32)], inter_frames=[np.asarray(param.inter_frame, np.float32)],
+        inter_frames_wrt_vol=[param.inter_frame_wrt_vol], iter_vars=[np.asarray(means, np.float32)],
+        sims=np.asarray(best, np.float32), projs_used=list(projs_used), regi_time_secs=regi_time_secs)

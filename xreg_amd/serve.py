@@ -22,6 +22,7 @@ import numpy as np
 from . import _xrh as X
 from . import geometry as G
 from .cmaes import CMAES
+from .regi import combine_mean
 
 
 @dataclass
@@ -55,6 +56,8 @@ class JobResult:
     best_params: np.ndarray
     best_pose: np.ndarray
     history: list = field(default_factory=list)
+    params: np.ndarray | None = None   # final estimate: the CMA-ES mean
+    pose: np.ndarray | None = None
 
     @property
     def latency(self) -> float:
@@ -195,7 +198,7 @@ class RegistrationServer:
             finished = []
             for a, (start, nv, lam) in zip(packed, spans):
                 s = sims[start:start + nv * lam].reshape(nv, lam)
-                f = s.astype(np.float64).mean(axis=0)   # mean over views
+                f = combine_mean(s).astype(np.float64)  # ImgSimMetric2DCombineMean over views
                 k = int(np.argmin(f))
                 if f[k] < a.best[0]:
                     a.best = (float(f[k]), a.X[k].copy())
@@ -208,9 +211,11 @@ class RegistrationServer:
             for a in finished:
                 t = time.perf_counter() - t0
                 bx = a.best[1]
+                xm = a.es.m.copy()  # the estimate is the distribution mean, as Intensity2D3DRegiCMAES
                 done.append(JobResult(a.spec.job_id, a.spec.arrival if realtime else 0.0, a.start, t, a.gen,
                                       a.best[0], bx, pose_from_params(a.spec.init_pose, a.spec.center, bx),
-                                      list(a.es.history)))
+                                      list(a.es.history), xm,
+                                      pose_from_params(a.spec.init_pose, a.spec.center, xm)))
                 a.close()
                 active.remove(a)
         return done
