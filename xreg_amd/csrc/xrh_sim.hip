@@ -403,13 +403,20 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
 }
 
-// patch_kernel for the common case -- no mask, stride 1, a compile-time patch diameter kD:
-// the same sums and closed form with every index known at compile time. All global reads of
-// the block (both gradient directions' moving and fixed windows and the per-patch fixed
-// statistics) are issued before the first one is waited for, so their latency overlaps;
-// staging walks the window as 8 rows x 32 columns of lanes, the horizontal and vertical
-// d-wide sums are unrolled with constant LDS offsets, and the closed form uses reciprocals
-// of N and N - 1 and one division.
+// patch_kernel for the common case -- no mask, stride 1, a compile-time patch diameter kD.
+// Same per-patch closed form; the box sums are formed with sliding windows so that LDS
+// traffic drops about threefold:
+//  * the first direction's window and every direction's per-patch fixed statistics are read
+//    before the first one is waited for; the second direction's window is read while the
+//    first one is being summed;
+//  * the window is staged in LDS as float (the values are floats; products are formed in
+//    double);
+//  * horizontal pass: 4 lanes per window row, each forming 4 consecutive d-wide sums -- the
+//    first directly, the next three by adding the entering and removing the leaving column;
+//  * vertical pass: 64 lanes, one per patch column and group of 4 patch rows, slide the same
+//    way down the column and evaluate the closed form for their 4 patches.
+// The sums are FP64 throughout; the sliding updates differ from direct sums by a few units in
+// the last place of a double.
 template <bool kTwo, int kD>
 __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __restrict__ items,
                                                            double* __restrict__ partials, int ptiles)
@@ -417,7 +424,10 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
   constexpr int kW = (kT - 1) + kD;  // window edge at stride 1
   constexpr int kRows = (kW + 7) / 8;
   constexpr int kDirs = kTwo ? 2 : 1;
+  constexpr int kSeg = 4;            // sums per lane in the sliding passes
+  constexpr int kSegs = kT / kSeg;   // lanes per window row / per patch column
   static_assert(kW <= 32, "one 32-lane column pass per window row");
+  static_assert(kW * kSegs <= kThreads && kT * kSegs <= kThreads, "sliding lanes fit the block");
   extern __shared__ double patch_lds[];
   const SimItem it = items[blockIdx.y];
   const ConstFixed fx = (ConstFixed)it.fixed;  // uniform: scalar loads
@@ -425,15 +435,11 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
   const int TX = (ppr + kT - 1) / kT, TY = (ppc + kT - 1) / kT;
   if ((int)blockIdx.x >= TX * TY) return;
   const int X0 = (blockIdx.x % TX) * kT, Y0 = (blockIdx.x / TX) * kT;  // first patch = first pixel
-  double* lm = patch_lds;
-  double* lf = lm + kW * kW;
-  double* h1 = lf + kW * kW;
+  float* lm = (float*)patch_lds;
+  float* lf = lm + kW * kW;
+  double* h1 = (double*)(lf + kW * kW);  // 2 kW^2 floats: a multiple of 8 bytes
   double* h2 = h1 + kW * kT;
   double* hf = h2 + kW * kT;
-  const int pxl = threadIdx.x % kT, pyl = threadIdx.x / kT;
-  const int pi = X0 + pxl, pj = Y0 + pyl;
-  const bool valid = (pi < ppr) && (pj < ppc);
-  const int p = valid ? pj * ppr + pi : 0;  // invalid lanes read patch 0 and discard it
   constexpr double kN = (double)(kD * kD);
   constexpr double kInvN = 1.0 / kN, kInvN1 = 1.0 / (kN - 1.0);
   const int lx = threadIdx.x & 31, ly0 = threadIdx.x >> 5;
@@ -442,25 +448,42 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
   const int xo = min(X0 + min(lx, kW - 1), cols - 1);
   const int64_t img_bytes = (int64_t)rows * cols * 4;
   const int64_t np = (int64_t)ppr * ppc;
+  // vertical-pass lanes: patch column vc, patch rows vs * kSeg ... + kSeg - 1
+  const bool vlane = threadIdx.x < kT * kSegs;
+  const int vc = threadIdx.x & (kT - 1), vs = (threadIdx.x >> 4) & (kSegs - 1);
 
-  // every global read of the block, issued before any is consumed
-  float vm[kDirs][kRows], vf[kDirs][kRows];
-  float mu[kDirs], den[kDirs];
-  double g[kDirs];
+  // the window rows this lane stages: byte offsets are shared by both directions
+  int wo[kRows];
 #pragma unroll
-  for (int dir = 0; dir < kDirs; ++dir) {
+  for (int r = 0; r < kRows; ++r) wo[r] = (min(Y0 + min(ly0 + 8 * r, kW - 1), rows - 1) * cols + xo) * 4;
+  float vm[kRows], vf[kRows];
+  auto load_window = [&](int dir) {
     const auto rm = ro_rsrc((dir == 0) ? (kTwo ? it.gx : it.mov) : it.gy, img_bytes);
     const auto rf = ro_rsrc((dir == 0) ? fx->pf : fx->pfy, img_bytes);
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      const int ly = min(ly0 + 8 * r, kW - 1);
-      const int o = (min(Y0 + ly, rows - 1) * cols + xo) * 4;
-      vm[dir][r] = ld_f32(rm, o);
-      vf[dir][r] = ld_f32(rf, o);
+      vm[r] = ld_f32(rm, wo[r]);
+      vf[r] = ld_f32(rf, wo[r]);
     }
-    mu[dir] = ld_f32(ro_rsrc((dir == 0) ? fx->pmean : fx->pmeany, np * 4), p * 4);
-    den[dir] = ld_f32(ro_rsrc((dir == 0) ? fx->pden : fx->pdeny, np * 4), p * 4);
-    g[dir] = ld_f64(ro_rsrc((dir == 0) ? fx->pg : fx->pgy, np * 8), p * 8);
+  };
+  load_window(0);
+  float mu[kDirs][kSeg], den[kDirs][kSeg];
+  double g[kDirs][kSeg];
+  bool pv[kSeg];
+#pragma unroll
+  for (int k = 0; k < kSeg; ++k) pv[k] = vlane && (X0 + vc < ppr) && (Y0 + vs * kSeg + k < ppc);
+#pragma unroll
+  for (int dir = 0; dir < kDirs; ++dir) {
+    const auto rmu = ro_rsrc((dir == 0) ? fx->pmean : fx->pmeany, np * 4);
+    const auto rden = ro_rsrc((dir == 0) ? fx->pden : fx->pdeny, np * 4);
+    const auto rg = ro_rsrc((dir == 0) ? fx->pg : fx->pgy, np * 8);
+#pragma unroll
+    for (int k = 0; k < kSeg; ++k) {
+      const int p = pv[k] ? (Y0 + vs * kSeg + k) * ppr + X0 + vc : 0;  // invalid: patch 0, discarded
+      mu[dir][k] = ld_f32(rmu, p * 4);
+      den[dir][k] = ld_f32(rden, p * 4);
+      g[dir][k] = ld_f64(rg, p * 8);
+    }
   }
 
   double acc[2] = {0.0, 0.0};
@@ -471,33 +494,44 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
     for (int r = 0; r < kRows; ++r) {
       const int ly = ly0 + 8 * r;
       if (col_ok && ly < kW) {
-        lm[ly * kW + lx] = (double)vm[dir][r];
-        lf[ly * kW + lx] = (double)vf[dir][r];
+        lm[ly * kW + lx] = vm[r];
+        lf[ly * kW + lx] = vf[r];
       }
     }
     __syncthreads();
+    if (dir + 1 < kDirs) load_window(dir + 1);
+    if (threadIdx.x < kW * kSegs) {
+      const int row = threadIdx.x / kSegs, c0 = (threadIdx.x % kSegs) * kSeg;
+      const float* m = lm + row * kW + c0;
+      const float* f = lf + row * kW + c0;
+      double* o1 = h1 + row * kT + c0;
+      double* o2 = h2 + row * kT + c0;
+      double* of = hf + row * kT + c0;
+      double a1 = 0.0, a2 = 0.0, af = 0.0;
 #pragma unroll
-    for (int row0 = 0; row0 < kW; row0 += kThreads / kT) {
-      const int row = row0 + (threadIdx.x >> 4);
-      if (row < kW) {
-        const int c = threadIdx.x & (kT - 1);
-        const double* m = lm + row * kW + c;
-        const double* f = lf + row * kW + c;
-        double a1 = 0.0, a2 = 0.0, af = 0.0;
+      for (int xx = 0; xx < kD; ++xx) {
+        const double dm = (double)m[xx];
+        a1 += dm;
+        a2 = fma(dm, dm, a2);
+        af = fma(dm, (double)f[xx], af);
+      }
+      o1[0] = a1;
+      o2[0] = a2;
+      of[0] = af;
 #pragma unroll
-        for (int xx = 0; xx < kD; ++xx) {
-          a1 += m[xx];
-          a2 = fma(m[xx], m[xx], a2);
-          af = fma(m[xx], f[xx], af);
-        }
-        h1[row * kT + c] = a1;
-        h2[row * kT + c] = a2;
-        hf[row * kT + c] = af;
+      for (int k = 1; k < kSeg; ++k) {
+        const double mo = (double)m[k - 1], mn = (double)m[k - 1 + kD];
+        a1 = (a1 - mo) + mn;
+        a2 = fma(mn, mn, fma(-mo, mo, a2));
+        af = fma(mn, (double)f[k - 1 + kD], fma(-mo, (double)f[k - 1], af));
+        o1[k] = a1;
+        o2[k] = a2;
+        of[k] = af;
       }
     }
     __syncthreads();
-    if (valid) {
-      const int e0 = pyl * kT + pxl;
+    if (vlane) {
+      const int e0 = vs * kSeg * kT + vc;
       double S1 = 0.0, S2 = 0.0, Sf = 0.0;
 #pragma unroll
       for (int yy = 0; yy < kD; ++yy) {
@@ -505,11 +539,22 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
         S2 += h2[e0 + yy * kT];
         Sf += hf[e0 + yy * kT];
       }
-      const double mean = S1 * kInvN;
-      const double var = fmax(S2 - mean * S1, 0.0) * kInvN1;
-      const double sd = fmax((double)1.0e-6f, sqrt(var));
-      const double cross = (Sf - (double)mu[dir] * S1) - mean * g[dir];
-      acc[dir] = 1.0 - cross / (sd * (double)den[dir]);
+#pragma unroll
+      for (int k = 0; k < kSeg; ++k) {
+        if (k > 0) {
+          const int eo = e0 + (k - 1) * kT, en = e0 + (k - 1 + kD) * kT;
+          S1 = (S1 - h1[eo]) + h1[en];
+          S2 = (S2 - h2[eo]) + h2[en];
+          Sf = (Sf - hf[eo]) + hf[en];
+        }
+        if (pv[k]) {
+          const double mean = S1 * kInvN;
+          const double var = fmax(S2 - mean * S1, 0.0) * kInvN1;
+          const double sd = fmax((double)1.0e-6f, sqrt(var));
+          const double cross = (Sf - (double)mu[dir][k] * S1) - mean * g[dir][k];
+          acc[dir] += 1.0 - cross / (sd * (double)den[dir][k]);
+        }
+      }
     }
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
@@ -747,6 +792,10 @@ int sim_patch_lds_bytes(const FixedDev& f)
 {
   if (f.metric != XRH_SIM_PATCH_NCC && f.metric != XRH_SIM_PATCH_GRAD_NCC) return 0;
   const int W = patch_lds_edge(f.patch_stride, f.patch_radius);
+  if ((sim_group_key(f) >> kSimKeyShift) & ((1 << (kSimKeyBlurShift - kSimKeyShift)) - 1)) {
+    // patch_kernel_d: float window pair + three double row-sum arrays
+    return 2 * W * W * (int)sizeof(float) + 3 * W * kT * (int)sizeof(double);
+  }
   return (f.pw ? (3 * W * W + 4 * W * kT) : (2 * W * W + 3 * W * kT)) * (int)sizeof(double);
 }
 
