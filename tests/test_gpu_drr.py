@@ -260,6 +260,31 @@ def test_hu_volume_matches_host_conversion(ctx, oracle):
     assert np.array_equal(out[0], out[1])
 
 
+def test_ct_file_to_drr(ctx, oracle, tmp_path):
+    """A Hounsfield-unit CT written as NIfTI, read back with its frame and converted on the
+    device, casts the oracle's DRR of the host-converted volume."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth, volio
+
+    print("case: NIfTI HU volume c1-small", flush=True)
+    w = synth.config("c1", n_poses=2, small=True)
+    hu = (w.vol / 0.02683 * 1000.0 - 1000.0).astype(np.float32)
+    i2p = w.idx_to_phys.astype(np.float64)
+    volio.write_nifti(tmp_path / "ct.nii.gz", volio.Volume3D(hu, np.diag(i2p)[:3], i2p[:3, 3], np.eye(3)))
+    vol, v3 = volio.load_ct(ctx, tmp_path / "ct.nii.gz", hu={})
+    assert np.array_equal(v3.idx_to_phys(), w.idx_to_phys.astype(np.float32))
+    poses = poses_f32(w.poses)
+    rows, cols = w.cams[0].num_det_rows, w.cams[0].num_det_cols
+    buf = ctx.alloc(4 * len(poses) * rows * cols)
+    X.line_int(ctx, vol, w.cams, poses, np.zeros(len(poses), np.int64), w.step, buf)
+    got = buf.download(np.float32).reshape(len(poses), rows, cols)
+    buf.free()
+    vol.close()
+    ref, _ = oracle.drr_f32(oracle.hu_to_lin_att(hu), w.idx_to_phys, w.cams, poses, step=w.step)
+    assert ref.max() > 0
+    assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
+
+
 def test_fp32_layouts_agree(ctx, c1s):
     """The coefficient layout evaluates the same trilinear interpolant as the corner layouts
     (rounding aside): DRRs agree to 2e-6 of the image maximum."""
