@@ -4,7 +4,8 @@
 Workload (CMA-ES population batching, single view): seeded 256x256x200 float32 ellipsoid
 phantom at 1.0 mm, 384x384 detector at 0.776 mm, focal 1020 mm, ray step 0.5 mm, lambda = 100
 poses per generation (sigma 10 deg / 15 mm about the ground truth), patch gradient-NCC
-(11x11 patches, Sobel after the default 5x5 smoothing, mean over patches).
+(11x11 patches, Sobel after the default 5x5 smoothing, mean over patches). `--config c4`
+runs the 512x512x400 @0.5 mm volume at 768^2 with 0.25 mm steps instead.
 
 One timed step = one generation: the lambda poses go to the device, one batched submission
 (graph-replayed) casts all DRRs and scores them, lambda similarity values come back. The plain
@@ -36,6 +37,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--lam", type=int, default=100)
+    ap.add_argument("--config", default="c2", choices=["c2", "c4"],
+                    help="c2: 256x256x200 @1 mm, 384^2, 0.5 mm steps (default); "
+                         "c4: 512x512x400 @0.5 mm, 768^2, 0.25 mm steps")
     ap.add_argument("--plain-poses", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -95,7 +99,8 @@ def cpu_baseline(w, fixed, poses, budget_s, smooth):
     O.patch_grad_ncc_f32(fixed, drr, smooth_ksize=smooth, threads=1)
     c = time.perf_counter()
     return {"value": done / el, "unit": "poses/s", "cores": threads, "kind": "port",
-            "sample": f"{done} poses of the same 384^2 patch-grad-NCC workload after one untimed pose, "
+            "sample": f"{done} poses of the same {w.cams[0].num_det_rows}^2 patch-grad-NCC workload after one "
+                      f"untimed pose, "
                       f"DRR {t_drr:.2f}s + similarity {t_sim:.2f}s, oracle FP32 restatement, {threads} threads",
             "single_thread": {"value": 1.0 / (c - a), "drr_s": round(b - a, 3), "similarity_s": round(c - b, 3)}}
 
@@ -169,9 +174,12 @@ def main():
     storage = {"f32": X.XRH_F32, "f16": X.XRH_F16, "bf16": X.XRH_BF16}[args.storage]
     geom = X.XRH_GEOM_FP64 if args.geom == "fp64" else X.XRH_GEOM_FP32
     lam = args.lam
-    w = synth.config("c2", n_poses=1)
+    w = synth.config(args.config, n_poses=1)
     cam = w.cams[0]
     P = cam.num_det_rows * cam.num_det_cols
+    det = f"{cam.num_det_rows}"
+    workload = f"cmaes-lambda{lam}-{det}sq-patch-grad-ncc"
+    X_, Y_, Z_ = w.size_xyz
 
     ctx = X.Context(0)
     vol = X.Volume(ctx, w.vol, w.idx_to_phys, storage=storage)
@@ -254,7 +262,7 @@ def main():
         cpu = cpu_baseline(w, fixed, gens[0], args.cpu_seconds, args.smooth)
 
     line = {
-        "metric": "poses/sec (DRR+grad-NCC, 384^2 det) batched vs plain; p50/p99 served regi latency",
+        "metric": f"poses/sec (DRR+grad-NCC, {det}^2 det) batched vs plain; p50/p99 served regi latency",
         "value": round(value, 2),
         "unit": "poses/s",
         "n_gpus": 1,
@@ -266,15 +274,16 @@ def main():
         "vs_baseline": None,
         "dtype": args.storage.replace("f32", "fp32"),
         "data": "synthetic seeded ellipsoid phantom",
-        "config": {"workload": "cmaes-lambda100-384sq-patch-grad-ncc", "volume": "256x256x200@1.0mm",
-                   "detector": "384x384@0.776mm", "step_mm": w.step, "lambda": lam,
+        "config": {"workload": workload, "volume": f"{X_}x{Y_}x{Z_}@{w.spacing:g}mm",
+                   "detector": f"{det}x{cam.num_det_cols}@{cam.det_col_spacing:.3f}mm", "step_mm": w.step,
+                   "lambda": lam,
                    "similarity": f"patch-grad-ncc r5 smooth{args.smooth}", "geometry": args.geom,
                    "storage": args.storage, "parallelism": "single-gpu"},
         "plain_poses_per_s": round(plain_value, 2) if plain_value else None,
         "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(f"workload=cmaes-lambda{lam}-384sq-patch-grad-ncc storage={args.storage} "
+                     "traffic": profiled_traffic(f"workload={workload} storage={args.storage} "
                                                  f"geometry={args.geom} skip={0 if args.no_skip else 1} "
                                                  f"smooth={args.smooth}"),
                      "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4),
