@@ -103,9 +103,10 @@ def multiview_cams(focal: float, rows: int, cols: int, px: float, n_views: int, 
 
 
 def config(name: str, n_poses: int | None = None, small: bool = False) -> Workload:
-    """BASELINE workloads by name: 'c1' (plain DRR+NCC), 'c2' (CMA-ES batch, 384^2),
-    'c3' (3 views, pyramid), 'c4' (512^2x400 precision). small=True shrinks the volume and
-    detector (same geometry) for fast parity tests."""
+    """Synthetic workloads by name: 'c1' (128x128x96 @2 mm, 192^2, single poses, NCC), 'c2'
+    (256x256x200 @1 mm, 384^2, CMA-ES batches, patch gradient-NCC), 'c3' (the c2 volume, three
+    views 30 degrees apart), 'c4' (512x512x400 @0.5 mm, 768^2, storage-precision study).
+    small=True shrinks the volume and detector (same geometry) for fast parity tests."""
     focal = 1020.0
     if name == "c1":
         size, sp, det, px, step, n, metric = (128, 128, 96), 2.0, 192, 1.552, 1.0, 100, 0
