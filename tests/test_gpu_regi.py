@@ -93,3 +93,43 @@ def test_two_level_registration_and_refiner(ctx, oracle):
     assert e2 < 0.5 * e0
     assert res[1].evals <= 80 + 1
     vol.close()
+
+
+def test_registration_from_files(ctx, oracle, tmp_path):
+    """The real-data route end to end: a Hounsfield-unit CT as NIfTI and the fixed projection
+    with its camera as projection data (HDF5), read back, registered with CMA-ES, and the debug
+    record written and re-read."""
+    from xreg_amd import geometry as G
+    from xreg_amd import h5io, regi, synth, volio
+
+    if not h5io.available():
+        pytest.skip("HDF5 C library not available")
+    print("case: files -> registration c2-small", flush=True)
+    w = synth.config("c2", n_poses=1, small=True)
+    hu = (w.vol / 0.02683 * 1000.0 - 1000.0).astype(np.float32)
+    i2p = w.idx_to_phys.astype(np.float64)
+    volio.write_nifti(tmp_path / "ct.nii.gz", volio.Volume3D(hu, np.diag(i2p)[:3], i2p[:3, 3], np.eye(3)))
+    g, _ = oracle.drr_f32(oracle.hu_to_lin_att(hu), w.idx_to_phys, w.cams, poses_f32(w.gt[None]), step=w.step)
+    fixed = synth.noisy_fixed(g[0], seed=3)
+    h5io.write_proj_data(tmp_path / "pd.h5", [h5io.ProjData(cam=w.cams[0], img=fixed)])
+
+    vol, _ = volio.load_ct(ctx, tmp_path / "ct.nii.gz", hu={})
+    pd = h5io.read_proj_data(tmp_path / "pd.h5", dtype=np.float32)
+    cam, img = pd[0].cam, pd[0].img.pixels
+    assert np.array_equal(img, fixed)
+    init = G.perturbed_pose(w.gt, w.center, np.deg2rad([2.0, -1.5, 1.0]), [3.0, -2.0, 2.5])
+    param = regi.PoseParam(init, w.center)
+    obj = regi.Objective(ctx, vol, [(cam, img)], param, w.step, metric=3, max_batch=32)
+    res = regi.run_cmaes(obj, lam=24, sigma=[np.deg2rad(2.0)] * 3 + [4.0] * 3, max_gens=40, seed=9)
+    obj.close()
+    vol.close()
+    ext = np.array(w.vol.shape[::-1]) * w.spacing
+    pts = [w.center + 0.25 * ext * np.array(c) for c in [(-1, -1, -1), (1, 1, 1), (1, -1, 1), (-1, 1, -1), (0, 0, 0)]]
+    e0 = G.mean_projection_distance(init, w.gt, pts, 1020.0)
+    e1 = G.mean_projection_distance(res.pose, w.gt, pts, 1020.0)
+    print(f"mPD {e0:.3f} -> {e1:.3f} mm over {len(res.extra['iter_best'])} generations")
+    assert e1 < 0.5 * e0
+    h5io.write_regi_debug(tmp_path / "dbg.h5", regi.debug_results(res, param, regi_time_secs=0.1))
+    dbg = h5io.read_regi_debug(tmp_path / "dbg.h5")
+    assert len(dbg.sims) == len(res.extra["iter_best"])
+    assert np.allclose(dbg.final_poses[0], res.pose, atol=1e-4)
