@@ -318,3 +318,44 @@ def test_cell_reuse_is_bit_identical(ctx, name, geom, kernel, store, layout, mon
     monkeypatch.setenv("XRH_CELL_REUSE", "0")
     b = _gpu_drr(ctx, w, poses, **kw)
     assert np.array_equal(a, b)
+
+
+def test_drr_edge_cases(ctx, oracle):
+    """Volume out of view (with and without a default background), volumes of one or two voxels
+    per axis, a step longer than every chord (one sample per ray), 1x1 and 3x17 detectors."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import geometry as G
+    from xreg_amd import synth
+
+    w = synth.config("c1", n_poses=2, small=True)
+
+    def both(vol, i2p, cams, poses, step, **kw):
+        ref, s_ref = oracle.drr_f32(vol, i2p, cams, poses_f32(poses), step=step, **kw)
+        rows, cols = cams[0].num_det_rows, cams[0].num_det_cols
+        v = X.Volume(ctx, vol, i2p)
+        buf = ctx.alloc(4 * len(poses) * rows * cols)
+        X.line_int(ctx, v, cams, poses_f32(poses), np.zeros(len(poses), np.int64), step, buf, **kw)
+        got = buf.download(np.float32).reshape(ref.shape)
+        buf.free()
+        v.close()
+        return got, ref
+
+    print("case: drr edge cases", flush=True)
+    far = G.rigid(None, (2000.0, 0.0, 0.0)) @ w.gt
+    got, ref = both(w.vol, w.idx_to_phys, w.cams, far[None], w.step)
+    assert np.array_equal(got, ref) and not got.any()
+    got, ref = both(w.vol, w.idx_to_phys, w.cams, far[None], w.step, default_bg=0.25)
+    assert np.array_equal(got, ref) and np.all(got == 0.25)
+    for shape in [(1, 1, 1), (1, 2, 3), (2, 2, 2), (3, 1, 4)]:
+        vol = np.random.default_rng(0).uniform(0.1, 1.0, shape).astype(np.float32)
+        i2p = G.idx_to_phys(8.0)
+        gt = G.gt_pose(G.vol_center_phys(shape[::-1], 8.0), 510.0)
+        got, ref = both(vol, i2p, w.cams, gt[None], 1.0)
+        assert drr_rel_err(got, ref).max() <= DRR_REL_TOL, shape
+    const = np.full(w.vol.shape, 0.5, np.float32)
+    got, ref = both(const, w.idx_to_phys, w.cams, w.gt[None], 1000.0)
+    assert ref.max() == np.float32(500.0) and np.array_equal(got, ref)
+    got, ref = both(w.vol, w.idx_to_phys, [G.camera(1020.0, 1, 1, 0.776, 0.776)], w.gt[None], w.step)
+    assert ref[0, 0, 0] > 0 and drr_rel_err(got, ref).max() <= DRR_REL_TOL
+    got, ref = both(w.vol, w.idx_to_phys, [G.camera(1020.0, 3, 17, 4.0, 2.0)], w.poses, w.step)
+    assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
