@@ -129,3 +129,37 @@ def test_patch_mask_without_unmasked_patch_is_rejected(ctx, images):
     fixed, _ = images
     with pytest.raises(X.XrhError):
         X.Fixed(ctx, fixed, 2, patch_radius=5, mask=np.zeros(fixed.shape, np.uint8))
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2, 3])
+def test_sim_special_images(ctx, oracle, images, metric):
+    """Moving image = fixed image (perfect correlation), a constant moving image (standard
+    deviation clamped at 1e-6), and a negated fixed image, against the oracle."""
+    from xreg_amd import _xrh as X
+
+    fixed, _ = images
+    print(f"case: special moving images, metric {metric}", flush=True)
+    mov = np.stack([fixed, np.full_like(fixed, 3.25), -fixed]).astype(np.float32)
+    ref = oracle.sim_f32(metric, fixed, mov, smooth_ksize=5)
+    ref64 = oracle.sim_f64(metric, fixed.astype(np.float64), mov.astype(np.float64), smooth_ksize=5)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, metric, smooth_ksize=5)
+    got = f.compute(buf, mov.shape[0])
+    f.close()
+    buf.free()
+    print(f"gpu {got}, oracle {ref}, fp64 {ref64}")
+    assert np.abs(got - ref64).max() <= SIM_ABS_TOL
+    # The reference adds the ~7e3 patch scores in a float running sum
+    # (xregImgSimMetric2DPatchNCCCPU.cpp:262-266); with scores near 2 (negated image) that sum
+    # alone moves its result ~2e-4 from the exact value. The device sums in FP64, so it is held
+    # to the FP32 restatement only up to the restatement's own distance from FP64.
+    assert np.all(np.abs(got - ref) <= SIM_ABS_TOL + np.abs(ref - ref64))
+    # closed forms: the correlation of identical images is (n - 1) / n (sample deviations over n
+    # products); NCC scores (1 -+ that) / 2, the patch scores 1 -+ that per 11x11 patch
+    if metric < 2:
+        n = fixed.size
+        want = [1.0 / (2 * n), 0.5, 1.0 - 1.0 / (2 * n)]
+    else:
+        want = [1.0 / 121, 1.0, 2.0 - 1.0 / 121]
+    assert np.abs(got - np.array(want)).max() <= SIM_ABS_TOL
