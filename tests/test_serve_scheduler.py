@@ -108,6 +108,23 @@ def test_alone_equals_mixed_scheduler():
         assert alone[j].history == mixed[j].history
 
 
+def test_deadline_finishes_jobs_with_their_current_estimate():
+    """A job past its in-service limit ends after the generation in flight, with the CMA-ES mean
+    as its estimate and status "deadline"; jobs without a limit run their generations."""
+    ctx = FakeCtx()
+    specs = _specs(3, gens=40)
+    specs[0].max_wall_s = 0.0
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=64, max_views=8, max_det=64,
+                                   device=FAKE)
+    res = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    assert res[0].status == "deadline" and res[0].generations == 1
+    assert res[0].pose is not None and np.all(np.isfinite(res[0].pose))
+    for j in (1, 2):
+        assert res[j].status in ("max_gens", "converged")
+        assert res[j].generations > 1
+    assert ctx.used == 0
+
+
 def test_oversized_job_is_rejected():
     ctx = FakeCtx()
     srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=8, max_views=8, max_det=64,

@@ -43,6 +43,7 @@ class JobSpec:
     metric: int = X.XRH_SIM_PATCH_GRAD_NCC
     smooth_ksize: int = 5
     patch_radius: int = 5
+    max_wall_s: float | None = None  # finish with the current estimate once this long in service
 
 
 @dataclass
@@ -58,6 +59,7 @@ class JobResult:
     history: list = field(default_factory=list)
     params: np.ndarray | None = None   # final estimate: the CMA-ES mean
     pose: np.ndarray | None = None
+    status: str = "max_gens"           # "max_gens", "converged" (CMA-ES stop test) or "deadline"
 
     @property
     def latency(self) -> float:
@@ -88,6 +90,7 @@ class _Active:
         self.gen = 0
         self.X = self.es.ask()
         self.best = (np.inf, None)
+        self.status = "max_gens"
 
     def items(self):
         return poses_from_params(self.spec.init_pose, self.spec.center, np.asarray(self.X))
@@ -204,10 +207,17 @@ class RegistrationServer:
                     a.best = (float(f[k]), a.X[k].copy())
                 a.es.tell(f)
                 a.gen += 1
-                if a.gen >= a.spec.max_gens or a.es.stop():
-                    finished.append(a)
+                in_service = (time.perf_counter() - t0) - a.start
+                if a.gen >= a.spec.max_gens:
+                    a.status = "max_gens"
+                elif a.es.stop():
+                    a.status = "converged"
+                elif a.spec.max_wall_s is not None and in_service >= a.spec.max_wall_s:
+                    a.status = "deadline"
                 else:
                     a.X = a.es.ask()
+                    continue
+                finished.append(a)
             for a in finished:
                 t = time.perf_counter() - t0
                 bx = a.best[1]
@@ -215,7 +225,7 @@ class RegistrationServer:
                 done.append(JobResult(a.spec.job_id, a.spec.arrival if realtime else 0.0, a.start, t, a.gen,
                                       a.best[0], bx, pose_from_params(a.spec.init_pose, a.spec.center, bx),
                                       list(a.es.history), xm,
-                                      pose_from_params(a.spec.init_pose, a.spec.center, xm)))
+                                      pose_from_params(a.spec.init_pose, a.spec.center, xm), a.status))
                 a.close()
                 active.remove(a)
         return done
