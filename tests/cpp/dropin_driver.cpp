@@ -15,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <vector>
 
 #include "../../xreg_amd/cpp/xreg_hip.h"
@@ -128,6 +129,28 @@ int main(int argc, char** argv)
         threw = true;
       }
       if (!threw) shared_bad += 1u << 30;
+    }
+    {
+      // the shared allocation outlives its owner: a caster whose owner is destroyed keeps
+      // casting into it, and its host view equals the owner's last DRR (REPLACE)
+      auto make = [&]() {
+        auto c = std::make_unique<RayCasterLineIntHIP>(0, geom ? XRH_GEOM_FP64 : XRH_GEOM_FP32);
+        c->set_volume(vol);
+        c->set_camera_models(cams);
+        c->set_ray_step_size(step);
+        c->set_num_projs(rc->num_projs());
+        c->allocate_resources();
+        c->set_camera_model_proj_associations(rc->camera_model_proj_associations());
+        c->set_xforms_cam_to_itk_phys(rc->xforms_cam_to_itk_phys());
+        return c;
+      };
+      auto owner = make();
+      auto sharer = make();
+      sharer->use_other_proj_buf(owner.get());
+      owner.reset();
+      sharer->compute(0);
+      const float* px = sharer->raw_host_pixel_buf();
+      for (size_t i = 0; i < n_pix; ++i) shared_bad += (px[i] != drr_rev[i]);
     }
 
     std::ofstream out(argv[2], std::ios::binary);

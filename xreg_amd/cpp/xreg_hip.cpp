@@ -2,6 +2,7 @@
 // Plain C++17 (g++): everything device-side happens behind include/xrh.h.
 #include "xreg_hip.h"
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -289,8 +290,29 @@ RayCasterLineIntHIP::RayCasterLineIntHIP(int device, int32_t geom, int32_t stora
 RayCasterLineIntHIP::~RayCasterLineIntHIP()
 {
   for (auto* v : dev_vols_) xrh_volume_destroy(v);
-  if (drr_dev_) xrh_device_free(ctx_, drr_dev_);
   if (bg_dev_) xrh_device_free(ctx_, bg_dev_);
+}
+
+RayCasterLineIntHIP::DeviceDrr::~DeviceDrr()
+{
+  if (dev) xrh_device_free(ctx, dev);
+}
+
+uint64_t RayCasterLineIntHIP::DeviceDrr::next_version()
+{
+  static std::atomic<uint64_t> ticket{1};
+  return ticket.fetch_add(1, std::memory_order_relaxed);
+}
+
+RayCasterLineIntHIP::DeviceDrr& RayCasterLineIntHIP::out_drr(size_type need) const
+{
+  DeviceDrr* b = out_->buf.get();
+  if (!b || need > b->capacity || xforms_cam_to_itk_phys_.size() < num_projs_) {
+    // not allocated, set_num_projs grew the count after allocate_resources, or the shared
+    // buffer is smaller
+    throw StringMessageException("projection buffer is smaller than num_projs projections");
+  }
+  return *b;
 }
 
 void RayCasterLineIntHIP::vols_changed()
@@ -322,13 +344,15 @@ void RayCasterLineIntHIP::allocate_resources()
     if (c.num_det_cols != cols) throw StringMessageException("Inconsistent number columns in camera model images!");
   }
   const size_type need = num_projs_ * rows * cols;
-  if (need > drr_capacity_) {
-    if (drr_dev_) check_xrh(xrh_device_free(ctx_, drr_dev_));
-    drr_dev_ = nullptr;
+  if (!own_->buf || need > own_->buf->capacity) {
+    own_->buf.reset();  // a caster still reading the old allocation keeps it alive
+    auto b = std::make_shared<DeviceDrr>();
+    b->ctx = ctx_;
     void* p = nullptr;
     check_xrh(xrh_device_alloc(ctx_, need * sizeof(float), &p));
-    drr_dev_ = (float*)p;
-    drr_capacity_ = need;
+    b->dev = (float*)p;
+    b->capacity = need;
+    own_->buf = std::move(b);
   }
   if (!ext_host_buf_) host_buf_.resize(need);
   host_stale_ = true;
@@ -349,10 +373,7 @@ void RayCasterLineIntHIP::compute(size_type vol_idx)
     throw UnsupportedOperationException();  // linear only, as RayCasterOCL (xregRayCastBaseOCL.cpp:338-341)
   }
   const size_type need = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
-  if (need > (other_ ? other_->drr_capacity_ : drr_capacity_) || xforms_cam_to_itk_phys_.size() < num_projs_) {
-    // set_num_projs grew the count after allocate_resources, or the shared buffer is smaller
-    throw StringMessageException("projection buffer is smaller than num_projs projections");
-  }
+  DeviceDrr& out = out_drr(need);
   const auto cams = xrh_cams();
   std::vector<float> xforms(16 * num_projs_);
   for (size_type p = 0; p < num_projs_; ++p) std::memcpy(&xforms[16 * p], xforms_cam_to_itk_phys_[p].m.data(), 64);
@@ -376,24 +397,21 @@ void RayCasterLineIntHIP::compute(size_type vol_idx)
     bg = bg_dev_;
   }
   const int32_t store = (proj_store_meth_ == kRAY_CAST_PIXEL_ACCUM) ? XRH_STORE_ACCUM : XRH_STORE_REPLACE;
-  if (store == XRH_STORE_ACCUM && host_stale_ == false && ext_host_buf_) {
+  const bool host_current = !host_stale_ && synced_buf_ == &out && synced_version_ == out.version;
+  if (store == XRH_STORE_ACCUM && host_current && ext_host_buf_) {
     // an external host buffer may have been written by the caller: it is the accumulation base
-    check_xrh(xrh_copy_h2d(ctx_, out_buf(), ext_host_buf_, need * sizeof(float)));
+    check_xrh(xrh_copy_h2d(ctx_, out.dev, ext_host_buf_, need * sizeof(float)));
   }
   const int32_t k = (kernel_id_ == kRAY_CAST_LINE_INT_MAX_KERNEL) ? XRH_LINE_INT_MAX : XRH_LINE_INT_SUM;
   check_xrh(xrh_line_int(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(), c4p.data(),
-                         (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, out_buf()));
-  host_stale_ = true;
-  if (other_) other_->host_stale_ = true;
+                         (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, out.dev));
+  out.version = DeviceDrr::next_version();  // every caster sharing the buffer now sees a stale host copy
 }
 
 uint64_t RayCasterLineIntHIP::sample_count(size_type vol_idx)
 {
   const size_type need = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
-  if (need > (other_ ? other_->drr_capacity_ : drr_capacity_) || xforms_cam_to_itk_phys_.size() < num_projs_) {
-    // set_num_projs grew the count after allocate_resources, or the shared buffer is smaller
-    throw StringMessageException("projection buffer is smaller than num_projs projections");
-  }
+  out_drr(need);  // the same preconditions as compute()
   const auto cams = xrh_cams();
   std::vector<float> xforms(16 * num_projs_);
   for (size_type p = 0; p < num_projs_; ++p) std::memcpy(&xforms[16 * p], xforms_cam_to_itk_phys_[p].m.data(), 64);
@@ -406,12 +424,15 @@ uint64_t RayCasterLineIntHIP::sample_count(size_type vol_idx)
 
 void RayCasterLineIntHIP::sync_to_host()
 {
-  if (!host_stale_) return;
   const size_type n = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
-  float* dst = ext_host_buf_ ? ext_host_buf_ : host_buf_.data();
+  DeviceDrr& out = out_drr(n);
+  if (!host_stale_ && synced_buf_ == &out && synced_version_ == out.version) return;
   if (!ext_host_buf_ && host_buf_.size() < n) host_buf_.resize(n);
-  check_xrh(xrh_copy_d2h(ctx_, dst, out_buf(), n * sizeof(float)));
+  float* dst = ext_host_buf_ ? ext_host_buf_ : host_buf_.data();
+  check_xrh(xrh_copy_d2h(ctx_, dst, out.dev, n * sizeof(float)));
   host_stale_ = false;
+  synced_buf_ = &out;
+  synced_version_ = out.version;
 }
 
 ImagePtr RayCasterLineIntHIP::proj(size_type proj_idx)
@@ -452,10 +473,10 @@ size_type RayCasterLineIntHIP::max_num_projs_possible() const
 void RayCasterLineIntHIP::use_other_proj_buf(RayCaster* other_ray_caster)
 {
   // share another HIP caster's device DRR buffer (RayCasterOCL::use_other_proj_buf,
-  // xregRayCastBaseOCL.cpp:258-269); the other caster must outlive this one
+  // xregRayCastBaseOCL.cpp:258-269): its slot, so the allocation outlives either caster
   auto* other = dynamic_cast<RayCasterLineIntHIP*>(other_ray_caster);
   if (!other || other == this) throw StringMessageException("Incompatible ray caster to share buffer from!");
-  other_ = other;
+  out_ = other->own_;
   host_stale_ = true;
 }
 

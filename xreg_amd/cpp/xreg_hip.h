@@ -213,7 +213,7 @@ class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterfac
   void use_external_host_pixel_buf(void* buf) override;
   size_type max_num_projs_possible() const override;
   void use_other_proj_buf(RayCaster* other) override;
-  const float* device_pixel_buf() const override { return out_buf(); }
+  const float* device_pixel_buf() const override { return out_->buf ? out_->buf->dev : nullptr; }
   // exact trilinear sample count of the next compute() (roofline accounting)
   uint64_t sample_count(size_type vol_idx = 0);
 
@@ -223,19 +223,36 @@ class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterfac
  private:
   void sync_to_host();
   std::vector<xrh_camera> xrh_cams() const;
-  // the DRR device buffer written by compute(): our own, or the one shared by
-  // use_other_proj_buf
-  float* out_buf() const { return other_ ? other_->drr_dev_ : drr_dev_; }
-  RayCasterLineIntHIP* other_ = nullptr;
+  // A DRR device allocation, freed with its last holder. `version` is a process-wide ticket
+  // taken at allocation and at every compute() that writes it (never reused), so every caster
+  // reading the buffer knows whether its host copy is current.
+  struct DeviceDrr {
+    xrh_ctx* ctx = nullptr;
+    float* dev = nullptr;
+    size_type capacity = 0;  // floats
+    uint64_t version = next_version();
+    ~DeviceDrr();
+    static uint64_t next_version();
+  };
+  // The slot a caster's allocate_resources() fills. use_other_proj_buf shares the other
+  // caster's slot (as RayCasterOCL shares a pointer to the other caster's buffer member, so a
+  // later re-allocation by the owner is seen), and the shared slot keeps the allocation alive
+  // even if the owner is destroyed first.
+  struct DrrSlot {
+    std::shared_ptr<DeviceDrr> buf;
+  };
+  DeviceDrr& out_drr(size_type need) const;  // the buffer compute() writes; throws if too small
   xrh_ctx* ctx_;
   int32_t geom_, storage_;
   std::vector<xrh_volume*> dev_vols_;
-  float* drr_dev_ = nullptr;
+  std::shared_ptr<DrrSlot> own_ = std::make_shared<DrrSlot>();
+  std::shared_ptr<DrrSlot> out_ = own_;
   float* bg_dev_ = nullptr;
-  size_type drr_capacity_ = 0;
   std::vector<float> host_buf_;
   float* ext_host_buf_ = nullptr;
   bool host_stale_ = true;
+  const DeviceDrr* synced_buf_ = nullptr;  // the allocation and version the host copy reflects
+  uint64_t synced_version_ = 0;
 };
 
 // ------------------------------------------------------------------------------------------
