@@ -539,7 +539,21 @@ void ImgSimMetric2DHIP::compute()
   const size_type npx = fixed_img_->rows * fixed_img_->cols;
   const float* mov = nullptr;
   if (rc_) {
-    mov = rc_->device_pixel_buf() + proj_off_ * npx;
+    // the moving images must lie inside the caster's device buffer and share the fixed image's
+    // size: a kernel must never read past an allocation
+    const auto* hip = dynamic_cast<const RayCasterLineIntHIP*>(rc_);
+    const float* base = rc_->device_pixel_buf();
+    if (!hip || !base || rc_->num_camera_models() == 0) {
+      throw StringMessageException("the bound ray caster has no device projection buffer");
+    }
+    const auto& cam = rc_->camera_model(0);
+    if (cam.num_det_rows != fixed_img_->rows || cam.num_det_cols != fixed_img_->cols) {
+      throw StringMessageException("fixed image and projection sizes differ");
+    }
+    if ((proj_off_ + num_mov_imgs_) * npx > hip->device_pixel_capacity()) {
+      throw StringMessageException("moving images run past the ray caster's projections");
+    }
+    mov = base + proj_off_ * npx;
   } else {
     const size_type need = num_mov_imgs_ * npx;
     if (need > mov_dev_capacity_) {

@@ -214,6 +214,8 @@ class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterfac
   size_type max_num_projs_possible() const override;
   void use_other_proj_buf(RayCaster* other) override;
   const float* device_pixel_buf() const override { return out_->buf ? out_->buf->dev : nullptr; }
+  // floats in the buffer device_pixel_buf() points to (0 before allocate_resources)
+  size_type device_pixel_capacity() const { return out_->buf ? out_->buf->capacity : 0; }
   // exact trilinear sample count of the next compute() (roofline accounting)
   uint64_t sample_count(size_type vol_idx = 0);
 
