@@ -9,6 +9,8 @@
  *     holds a message; no exception ever escapes;
  *   - host arrays are caller-owned and only read during the call; device buffers passed
  *     in ("_dev") are caller-owned device memory (xrh_device_alloc or any HIP allocation);
+ *     a range that starts inside an xrh_device_alloc allocation and runs past its end is
+ *     refused with XRH_ERR_INVALID before any work is queued (other pointers are trusted);
  *   - all device work is ordered on the context's stream; entry points that return host
  *     results synchronise that stream before returning (the reference's compute() calls
  *     are synchronous: xregRayCastLineIntOCL.cpp:192-197).

@@ -24,3 +24,35 @@ def test_copy_d2d(ctx, nbytes, src_off, dst_off):
     assert np.array_equal(got, want)
     src.free()
     dst.free()
+
+
+def test_buffer_ranges_are_checked_before_any_launch(ctx):
+    """A DRR, moving-image or copy range that starts inside one of the context's allocations
+    and runs past its end is refused with XRH_ERR_INVALID before anything reaches the GPU;
+    in-range offsets work."""
+    import types
+
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    print("case: range checks", flush=True)
+    w = synth.config("c1", n_poses=2, small=True)
+    cam = w.cams[0]
+    npx = cam.num_det_rows * cam.num_det_cols
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    buf = ctx.alloc(4 * npx * 2)  # room for two DRRs
+    poses = w.poses.astype(np.float32)
+    with pytest.raises(X.XrhError) as e:
+        X.line_int(ctx, vol, [cam], poses[:2], np.zeros(2, np.int64), w.step,
+                   types.SimpleNamespace(ptr=buf.ptr + 4 * npx))  # second half holds only one
+    assert e.value.status == X.XRH_ERR_INVALID
+    X.line_int(ctx, vol, [cam], poses[:1], np.zeros(1, np.int64), w.step, types.SimpleNamespace(ptr=buf.ptr + 4 * npx))
+    fx = X.Fixed(ctx, np.ones((cam.num_det_rows, cam.num_det_cols), np.float32), X.XRH_SIM_NCC)
+    with pytest.raises(X.XrhError):
+        fx.compute(buf, 2, offset_imgs=1)
+    assert fx.compute(buf, 1, offset_imgs=1).shape == (1,)
+    with pytest.raises(X.XrhError):
+        buf.download(np.float32, count=npx + 1, offset_bytes=4 * npx)
+    fx.close()
+    buf.free()
+    vol.close()

@@ -260,6 +260,7 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
     off += (int64_t)rows[i] * cols[i];
   }
   XRH_REQUIRE(drr_out || off <= b->max_items * b->cap_npix, "DRR arena overflow");
+  if (drr_out) ctx_check_range(b->ctx, drr_out, (uint64_t)off * 4, "DRR output");
   // items grouped by launch key (metric, then kernel variant), in item order within a group
   std::vector<int> keys;
   for (int64_t i = 0; i < n; ++i) keys.push_back(sim_group_key(b->views[item_view[i]].fixed->host));

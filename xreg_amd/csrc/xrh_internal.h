@@ -186,6 +186,9 @@ namespace xrh {
 // context-budgeted allocation helpers (defined in xrh_runtime.hip)
 void* ctx_alloc(xrh_ctx* ctx, uint64_t bytes);
 void ctx_free(xrh_ctx* ctx, void* p);
+// Fails with XRH_ERR_INVALID when [p, p + bytes) starts inside one of the context's
+// allocations and runs past its end; pointers the context did not allocate pass unchecked.
+void ctx_check_range(xrh_ctx* ctx, const void* p, uint64_t bytes, const char* what);
 // grows ctx->scratch to at least `bytes` (device) and returns it
 void* ctx_scratch(xrh_ctx* ctx, uint64_t bytes);
 void* ctx_pinned(xrh_ctx* ctx, uint64_t bytes);

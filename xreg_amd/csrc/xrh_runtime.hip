@@ -56,6 +56,25 @@ void ctx_free(xrh_ctx* ctx, void* p)
   XRH_HIP_CHECK(hipFree(p));
 }
 
+void ctx_check_range(xrh_ctx* ctx, const void* p, uint64_t bytes, const char* what)
+{
+  if (!p || bytes == 0) return;
+  const uintptr_t a = (uintptr_t)p;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  for (const auto& kv : ctx->allocs) {
+    const uintptr_t b = (uintptr_t)kv.first;
+    if (a >= b && a < b + kv.second) {
+      if (bytes > (b + kv.second) - a) {
+        fail(XRH_ERR_INVALID, std::string(what) + ": " + std::to_string(bytes) + " bytes from offset " +
+                                  std::to_string(a - b) + " run past the end of a " +
+                                  std::to_string(kv.second) + "-byte allocation");
+      }
+      return;
+    }
+  }
+  // not one of this context's allocations: the caller vouches for its size
+}
+
 void* ctx_scratch(xrh_ctx* ctx, uint64_t bytes)
 {
   if (ctx->scratch_bytes >= bytes) return ctx->scratch;
@@ -478,6 +497,8 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   }
   const int64_t npx = (int64_t)rows * cols;
   XRH_REQUIRE(npx * num_projs < (1ll << 40), "DRR buffer too large");
+  ctx_check_range(ctx, drr_dev, (uint64_t)(npx * num_projs) * 4, "DRR buffer");
+  if (bg_dev) ctx_check_range(ctx, bg_dev, (uint64_t)(npx * num_cams) * 4, "background images");
 
   std::vector<int32_t> rr((size_t)num_projs, rows), cc((size_t)num_projs, cols);
   const TilePlan plan = build_tiles(rr, cc, 8);
@@ -629,6 +650,7 @@ xrh_status xrh_copy_h2d(xrh_ctx* ctx, void* dst_dev, const void* src_host, uint6
 {
   return guarded([&] {
     XRH_REQUIRE(ctx && dst_dev && src_host, "null argument");
+    ctx_check_range(ctx, dst_dev, bytes, "copy destination");
     XRH_HIP_CHECK(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, ctx->stream));
     XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
@@ -638,6 +660,7 @@ xrh_status xrh_copy_d2h(xrh_ctx* ctx, void* dst_host, const void* src_dev, uint6
 {
   return guarded([&] {
     XRH_REQUIRE(ctx && dst_host && src_dev, "null argument");
+    ctx_check_range(ctx, src_dev, bytes, "copy source");
     XRH_HIP_CHECK(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
     XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
@@ -660,6 +683,8 @@ xrh_status xrh_copy_d2d(xrh_ctx* ctx, void* dst_dev, const void* src_dev, uint64
   return guarded([&] {
     XRH_REQUIRE(ctx && dst_dev && src_dev, "null argument");
     if (bytes == 0) return;
+    ctx_check_range(ctx, dst_dev, bytes, "copy destination");
+    ctx_check_range(ctx, src_dev, bytes, "copy source");
     const bool aligned = ((uintptr_t)dst_dev % 16 == 0) && ((uintptr_t)src_dev % 16 == 0);
     const uint64_t n16 = aligned ? bytes / 16 : 0;
     const uint64_t work = n16 + (bytes - 16 * n16);

@@ -1066,6 +1066,8 @@ static void sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_d
   XRH_REQUIRE(sim_dev_out || sim_host_out, "no output for similarity values");
   const FixedDev& h = fixed->host;
   const int64_t npix = (int64_t)h.rows * h.cols;
+  ctx_check_range(ctx, mov_dev, (uint64_t)(npix * num_mov) * 4, "moving images");
+  if (sim_dev_out) ctx_check_range(ctx, sim_dev_out, (uint64_t)num_mov * 4, "similarity output");
   const int img_tiles = sim_img_tiles(h);
   const int ptiles = sim_tiles(h);
   const int rec = sim_record_stride(h.metric);
