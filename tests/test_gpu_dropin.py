@@ -1,5 +1,6 @@
 """The drop-in C++ classes (RayCasterLineIntHIP, ImgSimMetric2D*HIP, CombineMean) driven like
 Intensity2D3DRegi::setup / obj_fn give the CPU reference's DRRs and objective values."""
+import os
 import subprocess
 from pathlib import Path
 
@@ -11,7 +12,8 @@ from tests.helpers import DRR_REL_TOL, SIM_ABS_TOL, drr_rel_err, poses_f32
 pytestmark = pytest.mark.gpu
 
 ROOT = Path(__file__).resolve().parents[1]
-DRIVER = ROOT / "xreg_amd" / "_build" / "dropin_driver"
+# XREG_AMD_DROPIN_DRIVER names another build of the same driver (e.g. a sanitizer build)
+DRIVER = Path(os.environ.get("XREG_AMD_DROPIN_DRIVER", ROOT / "xreg_amd" / "_build" / "dropin_driver"))
 
 
 def _run(tmp_path, w, fixed, metric, smooth, extr, geom=0):
