@@ -100,6 +100,52 @@ def build_cpp(verbose: bool = False) -> Path:
 FP_CHECK = BUILD / "fp_rounding_check"
 
 
+ASAN_DIR = BUILD / "asan"
+ASAN_DRIVER = ASAN_DIR / "dropin_driver"
+
+
+def build_asan(verbose: bool = False) -> Path:
+    """Host-side AddressSanitizer + UBSan build of libxrh (device code is built as usual) and of
+    the drop-in driver with the C++ classes compiled in, all with ROCm's clang so one sanitizer
+    runtime serves the process. Run the drop-in tests against it with
+    XREG_AMD_DROPIN_DRIVER=<ASAN_DRIVER> and
+    ASAN_OPTIONS=detect_leaks=0:verify_asan_link_order=0:protect_shadow_gap=0."""
+    ASAN_DIR.mkdir(parents=True, exist_ok=True)
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+    flags = [f for f in COMMON if f != "-O3"] + ["-O1"]
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        futs = []
+        for src in sorted(CSRC.glob("*.hip")):
+            obj = ASAN_DIR / (src.stem + ".o")
+            futs.append(ex.submit(subprocess.run, [HIPCC, *flags, *san, "-c", str(src), "-o", str(obj)],
+                                  capture_output=True, text=True))
+            objs.append(obj)
+        for f, src in zip(futs, sorted(CSRC.glob("*.hip"))):
+            res = f.result()
+            if res.returncode != 0:
+                raise RuntimeError(f"hipcc (asan) failed for {src.name}:\n{res.stderr}")
+    lib = ASAN_DIR / "libxrh.so"
+    res = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(lib), *map(str, objs)],
+                         capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link (asan) failed:\n{res.stderr}")
+    clang = Path(HIPCC).resolve().parent.parent / "llvm" / "bin" / "clang++"
+    root = HERE.parent
+    cmd = [str(clang), "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+           f"-I{root / 'include'}", str(root / "tests" / "cpp" / "dropin_driver.cpp"), str(HERE / "cpp" / "xreg_hip.cpp"),
+           f"-L{ASAN_DIR}", "-lxrh", "-Wl,-rpath,$ORIGIN", "-o", str(ASAN_DRIVER)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"clang++ (asan) failed for the driver:\n{res.stderr}")
+    for o in objs:
+        o.unlink()
+    if verbose:
+        print(f"built {ASAN_DRIVER}")
+    return ASAN_DRIVER
+
+
 def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
     if LIB.exists():
@@ -107,4 +153,7 @@ def clean() -> None:
 
 
 if __name__ == "__main__":
-    build(verbose=True, extra=sys.argv[1:])
+    if sys.argv[1:] == ["--asan"]:
+        build_asan(verbose=True)
+    else:
+        build(verbose=True, extra=sys.argv[1:])
