@@ -176,7 +176,9 @@ uint64_t xrh_volume_bytes(const xrh_volume* vol);
  *   kernel                  XRH_LINE_INT_SUM / _MAX
  *   store                   XRH_STORE_REPLACE (pre-fill) / XRH_STORE_ACCUM (add into drr_dev)
  *   bg_dev                  NULL, or num_cams background images on the device, copied into
- *                           each projection before casting (use_bg_projs_)
+ *                           each projection before casting under either store method, so
+ *                           ACCUM then adds onto the background, not onto drr_dev's previous
+ *                           contents (use_bg_projs_)
  *   default_bg              default_bg_pixel_val_ used by REPLACE without bg images
  *   geom                    XRH_GEOM_FP32 / XRH_GEOM_FP64
  *   drr_dev                 [num_projs][rows][cols] float on the device

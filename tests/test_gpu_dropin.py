@@ -12,8 +12,9 @@ from tests.helpers import DRR_REL_TOL, SIM_ABS_TOL, drr_rel_err, poses_f32
 pytestmark = pytest.mark.gpu
 
 ROOT = Path(__file__).resolve().parents[1]
-# XREG_AMD_DROPIN_DRIVER names another build of the same driver (e.g. a sanitizer build)
-DRIVER = Path(os.environ.get("XREG_AMD_DROPIN_DRIVER", ROOT / "xreg_amd" / "_build" / "dropin_driver"))
+# XREG_AMD_DROPIN_DIR names another build of the drivers (e.g. a sanitizer build)
+DRIVER_DIR = Path(os.environ.get("XREG_AMD_DROPIN_DIR", ROOT / "xreg_amd" / "_build"))
+DRIVER = DRIVER_DIR / "dropin_driver"
 
 
 def _run(tmp_path, w, fixed, metric, smooth, extr, geom=0):

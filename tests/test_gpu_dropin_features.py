@@ -1,0 +1,103 @@
+"""The rest of the drop-in classes' interface, driven as xReg's callers drive it
+(tests/cpp/dropin_features.cpp): MAX projections, two objects cast REPLACE then ACCUM, a static
+volume's DRR as background, an external host buffer, host-bound moving images, the exhaustive
+driver's short last batch and a masked metric, each against the oracle."""
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.helpers import DRR_REL_TOL, SIM_ABS_TOL, drr_rel_err, poses_f32
+from tests.test_gpu_dropin import DRIVER_DIR
+
+pytestmark = pytest.mark.gpu
+
+FEATURES = DRIVER_DIR / "dropin_features"
+
+
+def _records(raw: bytes) -> dict:
+    out, off = {}, 0
+    while off < len(raw):
+        n = int(np.frombuffer(raw, np.uint32, 1, off)[0])
+        name = raw[off + 4:off + 4 + n].decode()
+        nb = int(np.frombuffer(raw, np.uint64, 1, off + 4 + n)[0])
+        out[name] = raw[off + 12 + n:off + 12 + n + nb]
+        off += 12 + n + nb
+    return out
+
+
+@pytest.mark.parametrize("metric,smooth", [(0, 0), (3, 5)])
+def test_dropin_features(tmp_path, oracle, metric, smooth):
+    from xreg_amd import synth
+
+    w = synth.config("c2", n_poses=6, small=True)
+    cam = w.cams[0]
+    rows, cols = cam.num_det_rows, cam.num_det_cols
+    X, Y, Z = w.size_xyz
+    npose = w.poses.shape[0]
+    vol_a = np.ascontiguousarray(w.vol, np.float32)
+    vol_b = np.ascontiguousarray(w.vol[::-1, :, ::-1] * 0.7, np.float32)  # a second object, same grid
+    pa = w.poses
+    pb = np.roll(w.poses, 1, axis=0)
+    g, _ = oracle.drr_f32(vol_a, w.idx_to_phys, w.cams, poses_f32(w.gt[None]), step=w.step)
+    fixed = synth.noisy_fixed(g[0])
+    rng = np.random.default_rng(11)
+    bg = rng.uniform(0.0, 0.5 * float(g.max()), size=(rows, cols)).astype(np.float32)
+    rr, cc = np.mgrid[0:rows, 0:cols]
+    mask = (((rr - rows / 2) ** 2 + (cc - cols / 2) ** 2) <= (0.4 * min(rows, cols)) ** 2).astype(np.uint8)
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        np.array([X, Y, Z, rows, cols, npose, metric, smooth], np.int32).tofile(f)
+        np.array([w.spacing, w.step, 1020.0, cam.det_col_spacing], np.float32).tofile(f)
+        vol_a.tofile(f)
+        vol_b.tofile(f)
+        fixed.astype(np.float32).tofile(f)
+        bg.tofile(f)
+        mask.tofile(f)
+        poses_f32(pa).tofile(f)
+        poses_f32(pb).tofile(f)
+    out = tmp_path / "out.bin"
+    print(f"case: dropin features metric={metric} poses={npose}", flush=True)
+    r = subprocess.run([str(FEATURES), str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = _records(out.read_bytes())
+
+    def f32(name, shape=None):
+        a = np.frombuffer(rec[name], np.float32)
+        return a.reshape(shape) if shape else a
+
+    def u64(name):
+        return int(np.frombuffer(rec[name], np.uint64)[0])
+
+    shp = (npose, rows, cols)
+    ref_a, _ = oracle.drr_f32(vol_a, w.idx_to_phys, w.cams, poses_f32(pa), step=w.step)
+    # MAX kernel (REPLACE, default background 0)
+    ref_max, _ = oracle.drr_f32(vol_a, w.idx_to_phys, w.cams, poses_f32(pa), step=w.step, kernel=1)
+    assert drr_rel_err(f32("max_drr", shp), ref_max).max() <= DRR_REL_TOL
+    # external host buffer: raw_host_pixel_buf() is the caller's memory and proj(i) reads it
+    assert u64("ext_bad") == 0
+    assert drr_rel_err(f32("ext_drr", shp), ref_a).max() <= DRR_REL_TOL
+    # device-bound and host-bound moving images give the same values, and the oracle's
+    dev = f32("dev_sims")
+    assert np.array_equal(f32("host_sims"), dev)
+    ref_sims = oracle.sim_f32(metric, fixed, ref_a, smooth_ksize=smooth)
+    assert np.abs(dev - ref_sims).max() <= SIM_ABS_TOL
+    # a short batch: the first n - 2 poses only
+    assert np.array_equal(f32("short_sims"), dev[:npose - 2])
+    # masked metric
+    ref_mask = oracle.sim_f32(metric, fixed, ref_a, smooth_ksize=smooth, mask=mask)
+    assert np.abs(f32("mask_sims") - ref_mask).max() <= SIM_ABS_TOL
+    assert np.abs(ref_mask - ref_sims).max() > 1e-4  # the mask matters for these images
+    # two objects: REPLACE with A, then ACCUM with B (pre_compute keeps the buffer)
+    ref_two, _ = oracle.drr_f32(vol_b, w.idx_to_phys, w.cams, poses_f32(pb), step=w.step, store=1,
+                                out=ref_a.copy())
+    assert drr_rel_err(f32("two_obj_drr", shp), ref_two).max() <= DRR_REL_TOL
+    assert np.abs(f32("two_obj_sims") - oracle.sim_f32(metric, fixed, ref_two, smooth_ksize=smooth)).max() \
+        <= SIM_ABS_TOL
+    # object B over a static volume: background image + B
+    ref_st, _ = oracle.drr_f32(vol_b, w.idx_to_phys, w.cams, poses_f32(pb), step=w.step, bg=bg[None])
+    assert drr_rel_err(f32("static_drr", shp), ref_st).max() <= DRR_REL_TOL
+    assert np.abs(f32("static_sims") - oracle.sim_f32(metric, fixed, ref_st, smooth_ksize=smooth)).max() \
+        <= SIM_ABS_TOL
+    assert u64("bg_restored") == 1
+    assert u64("max_num_projs_possible") >= npose

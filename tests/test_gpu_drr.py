@@ -196,6 +196,21 @@ def test_drr_max_kernel_accum_and_bg(ctx, oracle, c1s):
     ref, _ = oracle.drr_f32(c1s.vol, c1s.idx_to_phys, c1s.cams, poses_f32(poses), step=c1s.step, bg=bg)
     got = _gpu_drr(ctx, c1s, poses, bg=bg)
     assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
+    # background images with ACCUM: the backgrounds replace the previous contents first
+    # (xregRayCastBaseCPU.cpp:134-143 runs before the store switch), so the result equals REPLACE
+    ref_acc, _ = oracle.drr_f32(c1s.vol, c1s.idx_to_phys, c1s.cams, poses_f32(poses), step=c1s.step, store=1,
+                                bg=bg, out=init.copy())
+    assert np.array_equal(ref_acc, ref)
+    got_acc = _gpu_drr(ctx, c1s, poses, store=1, init=init, bg=bg)
+    assert np.array_equal(got_acc, got)
+    # MAX with background images: max(bg, line maximum); a sample may differ from ITK's by an
+    # ulp (the trilinear layout's arithmetic), so the DRR bar applies, and the background wins
+    # exactly where it is larger
+    ref, _ = oracle.drr_f32(c1s.vol, c1s.idx_to_phys, c1s.cams, poses_f32(poses), step=c1s.step, kernel=1, bg=bg)
+    got = _gpu_drr(ctx, c1s, poses, kernel=1, bg=bg)
+    assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
+    assert np.all(got >= bg)
+    assert np.count_nonzero(got == bg) > 0 and np.count_nonzero(got > bg) > 0
 
 
 def test_drr_multiview_extrinsics(ctx, oracle):

@@ -68,6 +68,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
 
 CPP_LIB = HERE / "libxreg_hip.so"
 DRIVER = BUILD / "dropin_driver"
+FEATURES = BUILD / "dropin_features"
 CXX = os.environ.get("CXX", "g++")
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
@@ -82,18 +83,19 @@ def build_cpp(verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"g++ failed for xreg_hip.cpp:\n{res.stderr}")
     os.replace(tmp, CPP_LIB)
-    drv = HERE.parent / "tests" / "cpp" / "dropin_driver.cpp"
-    cmd = [CXX, *CXXFLAGS, str(drv), f"-L{HERE}", "-lxreg_hip", "-lxrh", "-Wl,-rpath,$ORIGIN/..", "-o", str(DRIVER)]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"g++ failed for dropin_driver.cpp:\n{res.stderr}")
+    for name, exe in (("dropin_driver.cpp", DRIVER), ("dropin_features.cpp", FEATURES)):
+        drv = HERE.parent / "tests" / "cpp" / name
+        cmd = [CXX, *CXXFLAGS, str(drv), f"-L{HERE}", "-lxreg_hip", "-lxrh", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"g++ failed for {name}:\n{res.stderr}")
     fp = HERE.parent / "tests" / "cpp" / "fp_rounding_check.hip"
     cmd = [HIPCC, *COMMON, str(fp), "-o", str(FP_CHECK)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for fp_rounding_check.hip:\n{res.stderr}")
     if verbose:
-        print(f"built {CPP_LIB}, {DRIVER} and {FP_CHECK}")
+        print(f"built {CPP_LIB}, {DRIVER}, {FEATURES} and {FP_CHECK}")
     return CPP_LIB
 
 
@@ -101,14 +103,13 @@ FP_CHECK = BUILD / "fp_rounding_check"
 
 
 ASAN_DIR = BUILD / "asan"
-ASAN_DRIVER = ASAN_DIR / "dropin_driver"
 
 
 def build_asan(verbose: bool = False) -> Path:
     """Host-side AddressSanitizer + UBSan build of libxrh (device code is built as usual) and of
-    the drop-in driver with the C++ classes compiled in, all with ROCm's clang so one sanitizer
-    runtime serves the process. Run the drop-in tests against it with
-    XREG_AMD_DROPIN_DRIVER=<ASAN_DRIVER> and
+    the drop-in drivers with the C++ classes compiled in, all with ROCm's clang so one sanitizer
+    runtime serves the process. Run the drop-in tests against them with
+    XREG_AMD_DROPIN_DIR=<ASAN_DIR> and
     ASAN_OPTIONS=detect_leaks=0:verify_asan_link_order=0:protect_shadow_gap=0."""
     ASAN_DIR.mkdir(parents=True, exist_ok=True)
     san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
@@ -133,17 +134,18 @@ def build_asan(verbose: bool = False) -> Path:
         raise RuntimeError(f"link (asan) failed:\n{res.stderr}")
     clang = Path(HIPCC).resolve().parent.parent / "llvm" / "bin" / "clang++"
     root = HERE.parent
-    cmd = [str(clang), "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
-           f"-I{root / 'include'}", str(root / "tests" / "cpp" / "dropin_driver.cpp"), str(HERE / "cpp" / "xreg_hip.cpp"),
-           f"-L{ASAN_DIR}", "-lxrh", "-Wl,-rpath,$ORIGIN", "-o", str(ASAN_DRIVER)]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"clang++ (asan) failed for the driver:\n{res.stderr}")
+    for name in ("dropin_driver", "dropin_features"):
+        cmd = [str(clang), "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+               f"-I{root / 'include'}", str(root / "tests" / "cpp" / f"{name}.cpp"), str(HERE / "cpp" / "xreg_hip.cpp"),
+               f"-L{ASAN_DIR}", "-lxrh", "-Wl,-rpath,$ORIGIN", "-o", str(ASAN_DIR / name)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"clang++ (asan) failed for {name}:\n{res.stderr}")
     for o in objs:
         o.unlink()
     if verbose:
-        print(f"built {ASAN_DRIVER}")
-    return ASAN_DRIVER
+        print(f"built the drop-in drivers in {ASAN_DIR}")
+    return ASAN_DIR
 
 
 def clean() -> None:

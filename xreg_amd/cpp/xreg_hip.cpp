@@ -624,16 +624,21 @@ std::shared_ptr<ImgSimMetric2D> PatchGradNCCSimMetricFromBackend(const std::stri
 // ------------------------------------------------------------------------------------------
 
 Intensity2D3DObjFn::Intensity2D3DObjFn(std::shared_ptr<RayCaster> rc, std::vector<std::shared_ptr<ImgSimMetric2D>> sims,
-                                       size_type projs_per_view)
-    : rc_(std::move(rc)), sims_(std::move(sims)), projs_per_view_(projs_per_view)
+                                       size_type projs_per_view, std::vector<size_type> vol_inds_in_ray_caster)
+    : rc_(std::move(rc)), sims_(std::move(sims)), projs_per_view_(projs_per_view),
+      vol_inds_(std::move(vol_inds_in_ray_caster))
 {
+  if (vol_inds_.empty()) throw StringMessageException("at least one object to register");
 }
 
 void Intensity2D3DObjFn::setup()
 {
-  // Intensity2D3DRegi::setup (xregIntensity2D3DRegi.cpp:43-133), single volume
+  // Intensity2D3DRegi::setup (xregIntensity2D3DRegi.cpp:43-133)
   const size_type nv = sims_.size();
   if (rc_->num_camera_models() != nv) throw StringMessageException("one camera model per view");
+  for (size_type vi : vol_inds_) {
+    if (vi >= rc_->num_vols()) throw StringMessageException("object volume index out of range");
+  }
   rc_->set_num_projs(projs_per_view_ * nv);
   rc_->allocate_resources();
   for (size_type v = 0; v < nv; ++v) {
@@ -651,14 +656,29 @@ void Intensity2D3DObjFn::setup()
 
 std::vector<float> Intensity2D3DObjFn::operator()(const FrameTransformList& poses)
 {
+  return (*this)(std::vector<FrameTransformList>{poses});
+}
+
+std::vector<float> Intensity2D3DObjFn::operator()(const std::vector<FrameTransformList>& poses_per_obj)
+{
   // Intensity2D3DRegi::obj_fn (xregIntensity2D3DRegi.cpp:571-696)
   if (!setup_done_) throw StringMessageException("setup() first");
-  if (poses.size() != projs_per_view_) throw StringMessageException("one pose per projection of a view");
+  if (poses_per_obj.size() != vol_inds_.size()) throw StringMessageException("one pose list per object");
+  for (const auto& poses : poses_per_obj) {
+    if (poses.size() != projs_per_view_) throw StringMessageException("one pose per projection of a view");
+  }
+  const bool orig_use_bg = rc_->use_bg_projs();
+  if (has_a_static_vol_) rc_->set_use_bg_projs(true);
   rc_->use_proj_store_replace_method();
-  rc_->distribute_xforms_among_cam_models(poses);
-  rc_->compute(0);
+  for (size_type o = 0; o < vol_inds_.size(); ++o) {
+    rc_->distribute_xforms_among_cam_models(poses_per_obj[o]);
+    rc_->compute(vol_inds_[o]);
+    if (has_a_static_vol_) rc_->set_use_bg_projs(false);
+    rc_->use_proj_store_accum_method();
+  }
   for (auto& s : sims_) s->compute();
   combine_.compute();
+  if (has_a_static_vol_) rc_->set_use_bg_projs(orig_use_bg);
   ++num_evals_;
   return combine_.sim_vals();
 }

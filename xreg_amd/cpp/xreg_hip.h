@@ -398,22 +398,33 @@ std::shared_ptr<ImgSimMetric2D> GradNCCSimMetricFromBackend(const std::string& b
 std::shared_ptr<ImgSimMetric2D> PatchNCCSimMetricFromBackend(const std::string& backend);
 std::shared_ptr<ImgSimMetric2D> PatchGradNCCSimMetricFromBackend(const std::string& backend);
 
-// Intensity2D3DRegi's evaluation path (xregIntensity2D3DRegi.cpp:43-133 setup, 571-696 obj_fn)
-// for a single volume: num_projs = projs_per_view * views, view-major projections, one
-// similarity metric per view bound at offset projs_per_view * view, mean over views.
+// Intensity2D3DRegi's evaluation path (xregIntensity2D3DRegi.cpp:43-133 setup, 571-696 obj_fn):
+// num_projs = projs_per_view * views, view-major projections, one similarity metric per view
+// bound at offset projs_per_view * view, mean over views. Several objects (volumes) are cast
+// one after another into the same projections, the first with the REPLACE store and the rest
+// with ACCUM; with a static volume the caster's background images (the static volumes' DRRs)
+// seed the first cast.
 class Intensity2D3DObjFn {
  public:
   Intensity2D3DObjFn(std::shared_ptr<RayCaster> rc, std::vector<std::shared_ptr<ImgSimMetric2D>> sims,
-                     size_type projs_per_view);
+                     size_type projs_per_view, std::vector<size_type> vol_inds_in_ray_caster = {0});
   void setup();
-  // poses: projs_per_view cam->phys transforms; returns the combined similarity per pose
+  // has_a_static_vol_ (xregIntensity2D3DRegi.cpp:593-625): the caster's background images are
+  // used for the first object's cast only
+  void set_has_a_static_vol(bool s) { has_a_static_vol_ = s; }
+  // poses: projs_per_view cam->phys transforms of the single object; returns the combined
+  // similarity per pose
   std::vector<float> operator()(const FrameTransformList& poses);
+  // poses_per_obj[o]: projs_per_view transforms of object o (volume vol_inds_in_ray_caster[o])
+  std::vector<float> operator()(const std::vector<FrameTransformList>& poses_per_obj);
   size_type num_obj_fn_evals() const { return num_evals_; }
 
  private:
   std::shared_ptr<RayCaster> rc_;
   std::vector<std::shared_ptr<ImgSimMetric2D>> sims_;
   size_type projs_per_view_;
+  std::vector<size_type> vol_inds_;
+  bool has_a_static_vol_ = false;
   ImgSimMetric2DCombineMean combine_;
   size_type num_evals_ = 0;
   bool setup_done_ = false;

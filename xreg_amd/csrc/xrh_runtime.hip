@@ -536,7 +536,9 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   XRH_HIP_CHECK(hipMemcpyAsync(dev, host, blob, hipMemcpyHostToDevice, ctx->stream));
 
   if (!samples_out) {
-    if (bg_dev && store == XRH_STORE_REPLACE) {
+    // RayCasterCPU::pre_compute (xregRayCastBaseCPU.cpp:128-158): background images replace the
+    // buffer's contents under either store method; the cast then accumulates onto them
+    if (bg_dev) {
       const int64_t n = npx * num_projs;
       hipLaunchKernelGGL(copy_bg_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
                          bg_dev, (const int64_t*)(dev + off_c4p), npx, num_projs, drr_dev);
@@ -555,8 +557,7 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   L.max_band_tiles = plan.max_band_tiles;
   L.geom = geom;
   L.kernel = kernel;
-  L.store = (store == XRH_STORE_ACCUM || (bg_dev && store == XRH_STORE_REPLACE)) ? XRH_STORE_ACCUM
-                                                                                 : XRH_STORE_REPLACE;
+  L.store = (store == XRH_STORE_ACCUM || bg_dev) ? XRH_STORE_ACCUM : XRH_STORE_REPLACE;
   L.fill = default_bg;
   L.storage = vol->desc.storage;
   L.layout = vol->desc.layout;
