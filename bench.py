@@ -11,6 +11,11 @@ One timed step = one generation: the lambda poses go to the device, one batched 
 (graph-replayed) casts all DRRs and scores them, lambda similarity values come back. The plain
 leg evaluates poses one at a time (DRR launch, similarity launch, synchronous read-back).
 
+The served legs run registrations arriving as a Poisson process: `--serve-jobs` of the C2
+workload (default), and with `--serve-mix-jobs` a mixed stream of three resident phantoms,
+detectors of 192^2 to 768^2, lambda 20-200, one or three views, batched and then served
+serially for comparison.
+
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits: this build runs on one
 GPU.
 """
@@ -50,6 +55,15 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=24, help="served-latency leg: number of jobs (0 skips)")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-gens", type=int, default=50, help="CMA-ES generations per job (upper bound)")
+    ap.add_argument("--serve-policy", default="srpt", choices=["srpt", "fifo"],
+                    help="served legs: order in which jobs fill a submission (least remaining work first, "
+                         "or arrival order)")
+    ap.add_argument("--serve-step-samples", type=float, default=2.0e10,
+                    help="served legs: estimated trilinear samples per submission (0: no limit)")
+    ap.add_argument("--serve-mix-jobs", type=int, default=0,
+                    help="mixed served stream (three resident phantoms, 192/384/768 detectors, lambda 20-200, "
+                         "30-80 generations, one or three views): number of jobs, run batched and serially")
+    ap.add_argument("--serve-mix-rate", type=float, default=0.25, help="mixed stream: Poisson rate, jobs/s")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run plain device copies of this many MiB (counter calibration under --pmc)")
     return ap.parse_args()
@@ -120,7 +134,7 @@ def profiled_traffic(key: str):
     return None
 
 
-def serve_leg(ctx, vol, w, n_jobs, rate, max_gens, lam, geom, smooth, seed=21):
+def serve_leg(ctx, vol, w, n_jobs, rate, max_gens, lam, geom, smooth, policy, step_samples, seed=21):
     """Served registration latency: n_jobs single-view CMA-ES registrations of the same workload
     (lambda poses per generation, up to max_gens generations) arriving as a Poisson process of
     `rate` jobs/s. Each job has its own ground truth (the reference pose moved by U(+-3 deg,
@@ -147,7 +161,8 @@ def serve_leg(ctx, vol, w, n_jobs, rate, max_gens, lam, geom, smooth, seed=21):
         t += float(rng.exponential(1.0 / rate))
     buf.free()
     srv = serve.RegistrationServer(ctx, {"c2": vol}, hbm_budget=200 << 30, max_items=max(4096, lam),
-                                   max_views=64, max_det=max(cam.num_det_rows, cam.num_det_cols), geom=geom)
+                                   max_views=64, max_det=max(cam.num_det_rows, cam.num_det_cols), geom=geom,
+                                   policy=policy, step_samples=step_samples)
     t0 = time.perf_counter()
     res = srv.run(specs, realtime=True)
     wall = time.perf_counter() - t0
@@ -157,12 +172,96 @@ def serve_leg(ctx, vol, w, n_jobs, rate, max_gens, lam, geom, smooth, seed=21):
             "mean_generations": round(float(np.mean([r.generations for r in res])), 2),
             "p50_s": round(serve.percentile(lat, 50), 4), "p99_s": round(serve.percentile(lat, 99), 4),
             "max_s": round(max(lat), 4), "wall_s": round(wall, 3), "scheduler_steps": srv.steps,
+            "policy": policy, "step_samples": step_samples,
             "poses_per_s": round(srv.poses_evaluated / wall, 1),
             "max_poses_per_submission": srv.max_items_in_step}
 
 
+# Mixed served stream: three resident phantoms (name, voxels x y z, spacing mm) and a 298 mm
+# detector read out at 192^2, 384^2 or 768^2 pixels.
+MIX_PHANTOMS = (("ph192", (192, 192, 192), 1.5), ("ph256", (256, 256, 200), 1.0), ("ph512", (512, 512, 400), 0.5))
+MIX_DETS = (192, 384, 768)
+MIX_DET_MM = 192 * 1.552
+
+
+def serve_mix_leg(ctx, n_jobs, rate, geom, smooth, policy, step_samples, seed=31):
+    """Mixed served stream: n_jobs registrations arriving as a Poisson process of `rate` jobs/s,
+    each drawn uniformly: phantom (three resident volumes), detector size, one or three views
+    (30 degrees apart), CMA-ES lambda in [20, 200] and a generation cap in [30, 80], patch
+    gradient-NCC, ray step half a voxel. Each job has its own ground truth (U(+-3 deg, +-5 mm)
+    about the phantom's reference pose), noisy fixed images cast on the device, and a start
+    U(+-5 deg, +-10 mm) about it. The same jobs run twice on the wall clock: continuously
+    batched, then served one at a time in arrival order (the serial baseline)."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import serve, synth
+    from xreg_amd import geometry as G
+
+    rng = np.random.default_rng(seed)
+    vols, meta = {}, {}
+    for name, size, sp in MIX_PHANTOMS:
+        vols[name] = X.Volume(ctx, synth.phantom(size), G.idx_to_phys(sp))
+        meta[name] = (sp, G.vol_center_phys(size, sp))
+    buf = ctx.alloc(4 * max(MIX_DETS) ** 2)
+    specs, t = [], 0.0
+    for j in range(n_jobs):
+        name = MIX_PHANTOMS[int(rng.integers(len(MIX_PHANTOMS)))][0]
+        sp, center = meta[name]
+        det = int(MIX_DETS[int(rng.integers(len(MIX_DETS)))])
+        nv = 1 if rng.random() < 0.5 else 3
+        lam, gens = int(rng.integers(20, 201)), int(rng.integers(30, 81))
+        cams = synth.multiview_cams(1020.0, det, det, MIX_DET_MM / det, 3, 30.0)[:nv]
+        gt = G.perturbed_pose(G.gt_pose(center, 510.0), center, np.deg2rad(rng.uniform(-3, 3, 3)),
+                              rng.uniform(-5, 5, 3))
+        fixed = []
+        for v, cam in enumerate(cams):
+            X.line_int(ctx, vols[name], [cam], gt.astype(np.float32)[None], np.zeros(1, np.int64), sp / 2, buf,
+                       geom=geom)
+            fixed.append(synth.noisy_fixed(buf.download(np.float32, count=det * det).reshape(det, det),
+                                           seed=2000 + 3 * j + v))
+        init = G.perturbed_pose(gt, center, np.deg2rad(rng.uniform(-5, 5, 3)), rng.uniform(-10, 10, 3))
+        specs.append(serve.JobSpec(job_id=j, arrival=t, volume=name, cams=cams, fixed=fixed, init_pose=init,
+                                   center=center, step=sp / 2, lam=lam, max_gens=gens, seed=9000 + j,
+                                   sigma=(np.deg2rad(5.0),) * 3 + (10.0,) * 3, smooth_ksize=smooth))
+        t += float(rng.exponential(1.0 / rate))
+    buf.free()
+
+    def run(max_active):
+        srv = serve.RegistrationServer(ctx, vols, hbm_budget=200 << 30, max_items=2048, max_views=64,
+                                       max_det=max(MIX_DETS), geom=geom, max_active=max_active, policy=policy,
+                                       step_samples=step_samples)
+        t0 = time.perf_counter()
+        mode = "serial" if max_active == 1 else "batched"
+        res = srv.run(specs, realtime=True,
+                      on_finish=lambda r: print(f"served mix ({mode}): job {r.job_id} done, {r.generations} "
+                                                f"generations, latency {r.latency:.2f} s", file=sys.stderr,
+                                                flush=True))
+        wall = time.perf_counter() - t0
+        srv.close()
+        lat = [r.latency for r in res]
+        out = {"wall_s": round(wall, 3), "jobs_per_s": round(len(res) / wall, 4),
+               "poses_per_s": round(srv.poses_evaluated / wall, 1),
+               "p50_s": round(serve.percentile(lat, 50), 4), "p99_s": round(serve.percentile(lat, 99), 4),
+               "max_s": round(max(lat), 4), "scheduler_steps": srv.steps,
+               "max_poses_per_submission": srv.max_items_in_step}
+        print(f"served mix ({mode}): {out}", file=sys.stderr, flush=True)
+        return out, srv.poses_evaluated
+
+    batched, n_b = run(None)
+    serial, n_s = run(1)
+    for v in vols.values():
+        v.close()
+    return {"jobs": n_jobs, "arrival_rate_jobs_per_s": rate,
+            "phantoms": [f"{n}:{s[0]}x{s[1]}x{s[2]}@{sp}mm" for n, s, sp in MIX_PHANTOMS],
+            "detectors": list(MIX_DETS), "lambda_range": [20, 200], "generation_cap_range": [30, 80],
+            "views": [1, 3], "poses_evaluated": n_b, "policy": policy, "step_samples": step_samples,
+            "batched": batched, "serial": serial,
+            "same_work": n_b == n_s}
+
+
 def main():
     args = parse()
+    if args.serve_step_samples <= 0:
+        args.serve_step_samples = None
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -255,7 +354,13 @@ def main():
 
     served = None
     if args.serve_jobs > 0:
-        served = serve_leg(ctx, vol, w, args.serve_jobs, args.serve_rate, args.serve_gens, lam, geom, args.smooth)
+        served = serve_leg(ctx, vol, w, args.serve_jobs, args.serve_rate, args.serve_gens, lam, geom, args.smooth,
+                           args.serve_policy, args.serve_step_samples)
+
+    served_mix = None
+    if args.serve_mix_jobs > 0:
+        served_mix = serve_mix_leg(ctx, args.serve_mix_jobs, args.serve_mix_rate, geom, args.smooth,
+                                   args.serve_policy, args.serve_step_samples)
 
     cpu = None
     if not args.no_cpu:
@@ -293,6 +398,7 @@ def main():
                      "measured_copy_gbs": round(copy_gbs, 1)},
         "empty_space_skipping": not args.no_skip,
         "served": served,
+        "served_mix": served_mix,
         "cpu_baseline": cpu,
         "sim_checksum": float(np.sum(sims)),
     }

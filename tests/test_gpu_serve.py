@@ -21,13 +21,19 @@ def _jobs(oracle, ctx, n_jobs, seed0=0):
     for j in range(n_jobs):
         name = ("c1", "c2")[j % 2]
         w, _ = vols[name]
-        cam = w.cams[0] if j % 3 else G.downsample_camera(w.cams[0], 0.5)
-        g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.gt[None]), step=w.step)
-        fixed = synth.noisy_fixed(g[0], seed=100 + j)
+        if j % 4 == 3:  # three views 30 degrees apart
+            c0 = w.cams[0]
+            cams = synth.multiview_cams(1020.0, c0.num_det_rows, c0.num_det_cols, c0.det_col_spacing, 3, 30.0)
+        else:
+            cams = [w.cams[0] if j % 3 else G.downsample_camera(w.cams[0], 0.5)]
+        fixed = []
+        for v, cam in enumerate(cams):
+            g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.gt[None]), step=w.step)
+            fixed.append(synth.noisy_fixed(g[0], seed=100 + 3 * j + v))
         r = np.deg2rad(rng.uniform(-3, 3, 3))
         t = rng.uniform(-5, 5, 3)
         init = G.perturbed_pose(w.gt, w.center, r, t)
-        specs.append(serve.JobSpec(job_id=j, arrival=0.002 * j, volume=name, cams=[cam], fixed=[fixed],
+        specs.append(serve.JobSpec(job_id=j, arrival=0.002 * j, volume=name, cams=cams, fixed=fixed,
                                    init_pose=init, center=w.center, step=w.step, lam=16 + 4 * (j % 3),
                                    max_gens=12, seed=1000 + j,
                                    sigma=(np.deg2rad(3.0),) * 3 + (5.0,) * 3))

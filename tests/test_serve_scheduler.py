@@ -108,6 +108,81 @@ def test_alone_equals_mixed_scheduler():
         assert alone[j].history == mixed[j].history
 
 
+def test_serial_mode_serves_one_job_at_a_time_with_the_same_results():
+    """max_active=1 (the serial baseline): every submission holds one job's generation, jobs
+    finish in arrival order, and each job's run equals its continuously batched run."""
+    ctx = FakeCtx()
+    specs = _specs(4, views=2, gens=6)
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=200, max_views=8, max_det=64,
+                                   device=FAKE, max_active=1)
+    FakeBatch.steps = []
+    serial = srv.run(specs, realtime=False)
+    assert [r.job_id for r in serial] == [0, 1, 2, 3]
+    assert max(FakeBatch.steps) <= 2 * max(s.lam for s in specs)
+    assert len(FakeBatch.steps) == sum(r.generations for r in serial)
+    for a, b in zip(serial[:-1], serial[1:]):
+        assert a.finish <= b.start
+    srv2 = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=200, max_views=8, max_det=64,
+                                    device=FAKE)
+    mixed = {r.job_id: r for r in srv2.run(specs, realtime=False)}
+    for r in serial:
+        assert r.history == mixed[r.job_id].history
+
+
+def test_srpt_packing_runs_short_jobs_first_within_the_step_budget():
+    """policy="srpt": the job with the least remaining work goes first; a submission holds
+    further jobs only while the estimated samples stay within step_samples; a long job then runs
+    alone; every job's run equals its run alone."""
+    ctx = FakeCtx()
+    specs = _specs(4, lam=10, gens=6)
+    specs[0].max_gens, specs[0].lam = 30, 40        # the long job arrives first
+    gen = [s.lam * 48 * 48 for s in specs]            # samples per generation (extent/step = 1)
+    # room for job 0 alone, or for the three short jobs together
+    budget = float(max(gen[0], sum(gen[1:])))
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=1000, max_views=8, max_det=64,
+                                   device=FAKE, policy="srpt", step_samples=budget)
+    FakeBatch.steps = []
+    res = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    short_items = sum(s.lam for s in specs[1:])
+    # the short jobs share submissions until they finish; the long job comes after them
+    assert FakeBatch.steps[0] == short_items
+    assert FakeBatch.steps[-1] == specs[0].lam
+    assert 40 not in FakeBatch.steps[:max(res[j].generations for j in (1, 2, 3))]
+    assert max(res[j].finish for j in (1, 2, 3)) <= res[0].finish
+    for s in specs:
+        alone = srv.run([s], realtime=False)[0]
+        assert alone.history == res[s.job_id].history
+    with pytest.raises(ValueError):
+        serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1, device=FAKE, policy="lifo")
+
+
+def test_fifo_policy_without_limit_packs_every_admitted_job():
+    ctx = FakeCtx()
+    specs = _specs(3, lam=10, gens=4)
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=1000, max_views=8, max_det=64,
+                                   device=FAKE, policy="fifo", step_samples=None)
+    FakeBatch.steps = []
+    srv.run(specs, realtime=False)
+    assert FakeBatch.steps[0] == sum(s.lam for s in specs)
+
+
+def test_fifo_policy_with_limit_keeps_arrival_order():
+    """policy="fifo" with a step limit: the long job that arrived first runs alone until it
+    finishes; the short jobs then share submissions."""
+    ctx = FakeCtx()
+    specs = _specs(4, lam=10, gens=6)
+    specs[0].max_gens, specs[0].lam = 30, 40
+    gen = [s.lam * 48 * 48 for s in specs]
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=1000, max_views=8, max_det=64,
+                                   device=FAKE, policy="fifo", step_samples=float(max(gen[0], sum(gen[1:]))))
+    FakeBatch.steps = []
+    res = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    n0 = res[0].generations
+    assert FakeBatch.steps[:n0] == [specs[0].lam] * n0
+    assert FakeBatch.steps[n0] == sum(s.lam for s in specs[1:])
+    assert res[0].finish <= min(res[j].finish for j in (1, 2, 3))
+
+
 def test_deadline_finishes_jobs_with_their_current_estimate():
     """A job past its in-service limit ends after the generation in flight, with the CMA-ES mean
     as its estimate and status "deadline"; jobs without a limit run their generations."""

@@ -252,6 +252,9 @@ class Volume:
         self.ctx = ctx
         self.h = p.value
         self.size = tuple(int(s) for s in size)
+        # longest physical edge of the volume box (mm): the schedulers' per-ray work estimate
+        axes = np.asarray(idx_to_phys, dtype=np.float64).reshape(4, 4)[:3, :3]
+        self.extent_mm = float(max(self.size[i] * np.linalg.norm(axes[:, i]) for i in range(3)))
 
     @property
     def nbytes(self) -> int:

@@ -80,11 +80,15 @@ def poses_from_params(init_pose: np.ndarray, center: np.ndarray, X: np.ndarray) 
 
 
 class _Active:
-    def __init__(self, spec: JobSpec, fixeds, views, start: float):
+    def __init__(self, spec: JobSpec, fixeds, views, start: float, extent_mm: float = 1.0):
         self.spec = spec
         self.fixeds = fixeds
         self.views = views
         self.start = start
+        # trilinear samples of one generation, estimated: lambda x detector pixels of every view
+        # x (volume extent / ray step)
+        rays = spec.lam * sum(int(c.num_det_rows) * int(c.num_det_cols) for c in spec.cams)
+        self.gen_work = float(rays) * max(1.0, extent_mm / float(spec.step))
         self.es = CMAES(np.zeros(6), np.asarray(spec.sigma), spec.lam, spec.seed,
                         bounds=None if spec.bounds is None else np.asarray(spec.bounds))
         self.gen = 0
@@ -95,6 +99,9 @@ class _Active:
     def items(self):
         return poses_from_params(self.spec.init_pose, self.spec.center, np.asarray(self.X))
 
+    def remaining_work(self) -> float:
+        return self.gen_work * (self.spec.max_gens - self.gen)
+
     def close(self):
         for f in self.fixeds:
             f.close()
@@ -103,9 +110,21 @@ class _Active:
 class RegistrationServer:
     def __init__(self, ctx: X.Context, volumes: dict, hbm_budget: int, max_items: int = 4096,
                  max_views: int = 256, max_det: int = 768, geom: int = X.XRH_GEOM_FP32,
-                 device=None):
+                 device=None, max_active: int | None = None, policy: str = "srpt",
+                 step_samples: float | None = 2.0e10):
         """device: the object providing Batch / Fixed / make_view (the xrh binding by default;
-        tests substitute a host-side fake to exercise the scheduler without a GPU)."""
+        tests substitute a host-side fake to exercise the scheduler without a GPU).
+        max_active: at most this many jobs in service at once (None: as many as fit; 1 serves
+        the jobs one after another in arrival order, the serial baseline).
+        policy: the order in which admitted jobs' generations fill a submission. "fifo":
+        admission order. "srpt": least estimated remaining work first (generations left x
+        samples per generation; ties by arrival). Jobs are packed in that order while the
+        submission's estimated samples stay within step_samples (None: no limit; the first job
+        always goes) and its items within max_items. With a limit, a long job runs alone at full
+        speed; "srpt" then serves short jobs ahead of long ones (lower median and mean latency),
+        "fifo" keeps arrival order (the smallest maximum latency)."""
+        if policy not in ("fifo", "srpt"):
+            raise ValueError(f"unknown packing policy {policy!r}")
         self.dev = device if device is not None else X
         self.ctx = ctx
         self.volumes = volumes      # name -> resident volume handle
@@ -115,6 +134,9 @@ class RegistrationServer:
         self.batch = self.dev.Batch(ctx, None, max_items, geom=geom, capacity=(max_views, max_det, max_det))
         self.batch.set_options(use_graph=False, time_kernels=False)
         self.max_views = max_views
+        self.max_active = max_active
+        self.policy = policy
+        self.step_samples = step_samples
         self.steps = 0
         self.poses_evaluated = 0
         self.max_items_in_step = 0
@@ -149,11 +171,12 @@ class RegistrationServer:
             if e.status == X.XRH_ERR_BUDGET:
                 return None
             raise
-        return _Active(spec, fixeds, views, now)
+        return _Active(spec, fixeds, views, now, float(getattr(vol, "extent_mm", 1.0)))
 
-    def run(self, jobs: list[JobSpec], realtime: bool = True) -> list[JobResult]:
+    def run(self, jobs: list[JobSpec], realtime: bool = True, on_finish=None) -> list[JobResult]:
         """Serves the jobs (sorted by arrival). With realtime=True arrivals follow the wall clock;
-        otherwise every job is available at once (throughput mode)."""
+        otherwise every job is available at once (throughput mode). on_finish(result) is called
+        as each job completes."""
         pending = [(j.arrival, j.job_id, j) for j in jobs]
         heapq.heapify(pending)
         active: list[_Active] = []
@@ -164,6 +187,8 @@ class RegistrationServer:
             now = clock()
             # admission: arrived jobs, in arrival order, while HBM and views allow
             while pending and pending[0][0] <= now:
+                if self.max_active is not None and len(active) >= self.max_active:
+                    break
                 if sum(len(a.views) for a in active) + len(pending[0][2].cams) > self.max_views:
                     break
                 a = self._admit(pending[0][2], time.perf_counter() - t0)
@@ -176,13 +201,19 @@ class RegistrationServer:
                     time.sleep(max(0.0, min(0.002, pending[0][0] - clock())))
                 continue
             # pack ready generations into one submission (whole generations only)
-            packed, n_items = [], 0
-            for a in active:
+            if self.policy == "srpt":
+                order = sorted(active, key=lambda a: (a.remaining_work(), a.spec.arrival, a.spec.job_id))
+            else:
+                order = active
+            budget = self.step_samples
+            packed, n_items, work = [], 0, 0.0
+            for a in order:
                 k = a.spec.lam * len(a.views)
-                if n_items + k > self.max_items and packed:
+                if packed and (n_items + k > self.max_items or (budget is not None and work + a.gen_work > budget)):
                     continue
                 packed.append(a)
                 n_items += k
+                work += a.gen_work
             views, item_view, item_pose, spans = [], [], [], []
             for a in packed:
                 poses = a.items()
@@ -228,6 +259,8 @@ class RegistrationServer:
                                       pose_from_params(a.spec.init_pose, a.spec.center, xm), a.status))
                 a.close()
                 active.remove(a)
+                if on_finish is not None:
+                    on_finish(done[-1])
         return done
 
 
