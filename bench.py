@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=24, help="served-latency leg: number of jobs (0 skips)")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-gens", type=int, default=50, help="CMA-ES generations per job (upper bound)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default DRR launch variant instead of tuning it on the first generation")
     ap.add_argument("--serve-policy", default="srpt", choices=["srpt", "fifo"],
                     help="served legs: order in which jobs fill a submission (least remaining work first, "
                          "or arrival order)")
@@ -291,6 +293,9 @@ def main():
     gens = generation_poses(w, lam, args.warmup + args.steps)
     iv = np.zeros(lam, np.int32)
 
+    # ---- DRR launch variant: tuned on the first generation (untimed) ----
+    tune = None if args.no_autotune else batch.autotune(iv, gens[0], reps=2)
+
     # ---- batched: warmup, then exactly K generations between synchronisations ----
     for g in range(args.warmup):
         batch.eval(iv, gens[g])
@@ -397,6 +402,7 @@ def main():
                      "evaluated_gbs": round((8.0 * bv * S_eval + 4.0 * P * lam) / (drr_ms_avg * 1e-3) / 1e9, 1),
                      "measured_copy_gbs": round(copy_gbs, 1)},
         "empty_space_skipping": not args.no_skip,
+        "autotune": tune,
         "served": served,
         "served_mix": served_mix,
         "cpu_baseline": cpu,

@@ -273,6 +273,45 @@ xrh_status xrh_batch_wait(xrh_batch* batch, float* sim_host_out);
  * with HIP events around the DRR kernel and around the whole submission (milliseconds). */
 xrh_status xrh_batch_last_timing(const xrh_batch* batch, float* drr_kernel_ms, float* total_ms);
 
+/* DRR launch variants. Every variant casts the same rays with the same arithmetic, so DRRs
+ * are identical under all of them (fused NCC sums may differ in the last FP64 bits: the
+ * per-tile summation order follows the wave footprint).
+ *   cell_reuse  1: one sample per loop iteration, the current cell's 32 bytes kept in
+ *               registers and loaded only when a sample enters a new cell; 0: two samples per
+ *               iteration, each loading its cell; -1: by launch size (1 from 16 workgroups
+ *               per CU up)
+ *   wave_shape  pixels per wave64 inside the 16x16 workgroup tile: 0 = 8x8, 1 = 16x4, 2 = 4x16
+ *   bands       detector-row bands of the XCD-aware workgroup order, 1..16 (default 8)
+ * The environment variable XRH_CELL_REUSE, when set, still overrides cell_reuse. */
+typedef struct xrh_drr_variant {
+  int32_t cell_reuse;
+  int32_t wave_shape;
+  int32_t bands;
+} xrh_drr_variant;
+
+typedef struct xrh_tune_result {
+  xrh_drr_variant chosen;  /* cell_reuse resolved to 0 / 1 */
+  float chosen_ms;         /* DRR kernel time of the chosen variant (minimum over reps) */
+  float default_ms;        /* the same for the default variant {-1, 0, 8} */
+  int32_t variants_timed;
+} xrh_tune_result;
+
+/* Autotuner of the ray-tile layout and sample loop (no counterpart in the reference, whose
+ * OpenCL caster uses one fixed 1-D NDRange, xregRayCastLineIntOCL.cpp:160-197): evaluates
+ * the given submission once per variant (the default, then cell_reuse {0,1} x wave_shape
+ * {0,1,2} x bands {4,8,16}, then the default again), each reps + 1 times with HIP events
+ * around the DRR kernel (the minimum of the last reps), and keeps the fastest if it beats the
+ * default's better timing by 1%, for this batch and, in the context, for later batch and
+ * xrh_line_int launches with the same storage, layout, geometry mode, line kernel, largest
+ * detector and workgroups per CU (power-of-two bucket). Synchronous; runs 20 * (reps + 1)
+ * submissions. */
+xrh_status xrh_batch_autotune(xrh_batch* batch, const int32_t* item_view, const double* item_pose,
+                              int64_t n_items, int32_t reps, xrh_tune_result* out);
+
+/* Forces a DRR variant for this batch's later submissions; NULL returns to the context's
+ * tuned table / the default. */
+xrh_status xrh_batch_set_drr_variant(xrh_batch* batch, const xrh_drr_variant* variant);
+
 /* Exact trilinear sample count of a batch (as xrh_line_int_sample_count; evaluated_out may
  * be NULL). */
 xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view,

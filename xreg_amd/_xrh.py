@@ -85,6 +85,15 @@ _PU64 = C.POINTER(C.c_uint64)
 _PU8 = C.POINTER(C.c_uint8)
 
 # name -> (restype, argtypes); the full exported surface of include/xrh.h
+class DrrVariant(C.Structure):
+    _fields_ = [("cell_reuse", C.c_int32), ("wave_shape", C.c_int32), ("bands", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("chosen", DrrVariant), ("chosen_ms", C.c_float), ("default_ms", C.c_float),
+                ("variants_timed", C.c_int32)]
+
+
 SIGNATURES = {
     "xrh_last_error": (C.c_char_p, []),
     "xrh_abi_version": (_I32, []),
@@ -119,6 +128,8 @@ SIGNATURES = {
     "xrh_batch_create_empty": (_I32, [_P, _I32, _I64, _I32, _I32, _I32, C.POINTER(_P)]),
     "xrh_batch_set_views": (_I32, [_P, C.POINTER(View), _I32]),
     "xrh_batch_set_options": (_I32, [_P, _I32, _I32]),
+    "xrh_batch_autotune": (_I32, [_P, _PI32, C.POINTER(C.c_double), C.c_int64, _I32, C.POINTER(TuneResult)]),
+    "xrh_batch_set_drr_variant": (_I32, [_P, C.POINTER(DrrVariant)]),
     "xrh_batch_eval": (_I32, [_P, _PI32, _PD, _I64, _PF, _P]),
     "xrh_batch_launch": (_I32, [_P, _PI32, _PD, _I64]),
     "xrh_batch_wait": (_I32, [_P, _PF]),
@@ -376,6 +387,26 @@ class Batch:
         out = np.empty(self._iv.shape[0], dtype=np.float32)
         check(lib().xrh_batch_wait(self.h, _fp(out)))
         return out
+
+    def autotune(self, item_view: np.ndarray, item_pose: np.ndarray, reps: int = 2) -> dict:
+        """Times every DRR launch variant on this submission and keeps the fastest
+        (xrh_batch_autotune)."""
+        iv = np.ascontiguousarray(item_view, dtype=np.int32)
+        ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
+        r = TuneResult()
+        check(lib().xrh_batch_autotune(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0], int(reps),
+                                       C.byref(r)))
+        return {"cell_reuse": r.chosen.cell_reuse, "wave_shape": r.chosen.wave_shape, "bands": r.chosen.bands,
+                "chosen_ms": float(r.chosen_ms), "default_ms": float(r.default_ms),
+                "variants_timed": int(r.variants_timed)}
+
+    def set_drr_variant(self, cell_reuse: int | None = None, wave_shape: int = 0, bands: int = 8) -> None:
+        """Forces a DRR launch variant; cell_reuse=None returns to the tuned / default choice."""
+        if cell_reuse is None:
+            check(lib().xrh_batch_set_drr_variant(self.h, None))
+        else:
+            v = DrrVariant(int(cell_reuse), int(wave_shape), int(bands))
+            check(lib().xrh_batch_set_drr_variant(self.h, C.byref(v)))
 
     def last_timing(self) -> tuple[float, float]:
         a, t = C.c_float(0), C.c_float(0)

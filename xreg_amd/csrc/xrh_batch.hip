@@ -56,6 +56,11 @@ struct xrh_batch {
   std::vector<hipGraph_t> graph_objs;
   int64_t last_n = 0;
   bool pending = false;
+  // DRR launch variant: set explicitly / by xrh_batch_autotune, else looked up in the
+  // context's tuned table, else the defaults
+  bool variant_set = false;
+  DrrVariant variant;
+  DrrVariant last_variant;         // the variant of the last staged launch
 };
 
 namespace {
@@ -70,7 +75,7 @@ struct BlobLayout {
 BlobLayout blob_layout(int64_t max_items)
 {
   BlobLayout L;
-  L.band_words = (uint64_t)8 * (max_items + 1);
+  L.band_words = (uint64_t)kMaxDrrBands * (max_items + 1);
   L.off_pose = 0;
   L.off_band = align256(sizeof(PoseIn) * max_items);
   L.off_sim = align256(L.off_band + sizeof(uint32_t) * L.band_words);
@@ -224,6 +229,19 @@ void batch_create(xrh_ctx* ctx, const xrh_view* views, int32_t num_views, int64_
   *out = b;
 }
 
+// Tuned-variant key of a launch over items of these detector sizes.
+std::string tune_key(const xrh_batch* b, const std::vector<int32_t>& rows, const std::vector<int32_t>& cols)
+{
+  int32_t mr = 0, mc = 0;
+  int64_t wgs = 0;
+  for (size_t i = 0; i < rows.size(); ++i) {
+    mr = std::max(mr, rows[i]);
+    mc = std::max(mc, cols[i]);
+    wgs += (int64_t)((rows[i] + 15) / 16) * ((cols[i] + 15) / 16);
+  }
+  return drr_tune_key(b->storage, b->layout, b->geom, b->views[0].kernel, mr, mc, wgs);
+}
+
 // Fills the pinned staging for n items; returns the DRR launch description.
 DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, float* drr_out,
                 std::vector<std::pair<int, std::vector<int64_t>>>& groups)
@@ -280,7 +298,17 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
   for (auto& g : groups) {
     for (int64_t i : g.second) sin[k++] = tmp[(size_t)i];
   }
-  const TilePlan plan = build_tiles(rows, cols, 8);
+  DrrVariant var = b->variant;
+  if (!b->variant_set) {
+    var = DrrVariant();
+    const std::string key = tune_key(b, rows, cols);
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    auto it = b->ctx->tuned.find(key);
+    if (it != b->ctx->tuned.end()) var = it->second;
+  }
+  b->last_variant = var;
+  const TilePlan plan = build_tiles(rows, cols, var.bands);
+  XRH_REQUIRE(plan.band_start.size() <= L.band_words, "band plan exceeds its staging");
   std::memcpy(b->blob_host + L.off_band, plan.band_start.data(), sizeof(uint32_t) * plan.band_start.size());
 
   DrrLaunch D;
@@ -302,6 +330,8 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
   D.out = drr;
   D.ncc_partials = b->ncc_partials_dev;
   D.ncc_tiles_per_item = b->max_img_tiles;
+  D.reuse = var.reuse;
+  D.wave_shape = var.shape;
   return D;
 }
 
@@ -334,6 +364,7 @@ std::string signature(const xrh_batch* b, const int32_t* item_view, int64_t n, c
   std::string key((const char*)item_view, (size_t)n * sizeof(int32_t));
   key.append((const char*)&drr_out, sizeof(drr_out));
   key.append((const char*)&b->version, sizeof(b->version));
+  key.append((const char*)&b->last_variant, sizeof(b->last_variant));
   return key;
 }
 
@@ -387,6 +418,99 @@ void wait(xrh_batch* b, float* sim_host_out)
   if (sim_host_out) std::memcpy(sim_host_out, b->sims_host, sizeof(float) * b->last_n);
 }
 
+// Times the DRR launch of this submission under every variant (HIP events around the DRR
+// kernel, eager submissions, the minimum over `reps` runs after one untimed run) and keeps
+// the fastest for this batch and, in the context, for launches of the same key. The default
+// is timed first and last; a variant replaces it only if it is 1% faster than the better of
+// the two.
+void autotune(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, int32_t reps,
+              xrh_tune_result* out)
+{
+  XRH_REQUIRE(b && item_view && item_pose && out, "null argument");
+  XRH_REQUIRE(reps >= 1 && reps <= 50, "reps must be in [1, 50]");
+  XRH_REQUIRE(!b->pending, "batch has a launch in flight");
+  XRH_REQUIRE(n >= 1 && n <= b->max_items, "item count must be in [1, max_items]");
+  const bool saved_graph = b->use_graph, saved_time = b->time_kernels, saved_set = b->variant_set;
+  const DrrVariant saved_var = b->variant;
+  auto restore = [&] {
+    b->use_graph = saved_graph;
+    b->time_kernels = saved_time;
+    b->variant_set = saved_set;
+    b->variant = saved_var;
+  };
+  b->use_graph = false;
+  b->time_kernels = true;
+  auto time_variant = [&](const DrrVariant& v) {
+    b->variant = v;
+    b->variant_set = true;
+    float best = 0.0f;
+    for (int r = 0; r <= reps; ++r) {
+      launch(b, item_view, item_pose, n, nullptr);
+      wait(b, nullptr);
+      float ms = 0.0f;
+      XRH_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_drr0, b->ev_drr1));
+      if (r == 1 || (r > 1 && ms < best)) best = ms;
+    }
+    return best;
+  };
+  DrrVariant best_v;
+  float best_ms = 0.0f, def_ms = 0.0f;
+  int tried = 0;
+  std::vector<int32_t> rows, cols;
+  try {
+    def_ms = time_variant(DrrVariant());
+    best_ms = def_ms;
+    tried = 1;
+    for (int reuse = 0; reuse <= 1; ++reuse) {
+      for (int shape = 0; shape <= 2; ++shape) {
+        for (int bands : {4, 8, 16}) {
+          DrrVariant v;
+          v.reuse = reuse;
+          v.shape = shape;
+          v.bands = bands;
+          const float ms = time_variant(v);
+          ++tried;
+          if (ms < best_ms) {
+            best_ms = ms;
+            best_v = v;
+          }
+        }
+      }
+    }
+    // the default once more (the first timing runs on colder caches and clocks); a variant
+    // must beat the better of the two by 1%
+    def_ms = std::min(def_ms, time_variant(DrrVariant()));
+    ++tried;
+    if (!(best_ms < 0.99f * def_ms)) {
+      best_v = DrrVariant();
+      best_ms = def_ms;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      rows.push_back(b->views[item_view[i]].cam.num_det_rows);
+      cols.push_back(b->views[item_view[i]].cam.num_det_cols);
+    }
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
+  b->variant = best_v;
+  b->variant_set = true;
+  int64_t wgs = 0;
+  for (size_t i = 0; i < rows.size(); ++i) wgs += (int64_t)((rows[i] + 15) / 16) * ((cols[i] + 15) / 16);
+  {
+    const std::string key = tune_key(b, rows, cols);
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    b->ctx->tuned[key] = best_v;
+  }
+  out->chosen.cell_reuse = best_v.reuse >= 0 ? best_v.reuse : (drr_default_reuse(wgs) ? 1 : 0);
+  out->chosen.wave_shape = best_v.shape;
+  out->chosen.bands = best_v.bands;
+  out->chosen_ms = best_ms;
+  out->default_ms = def_ms;
+  out->variants_timed = tried;
+}
+
 }  // namespace
 
 extern "C" {
@@ -411,6 +535,32 @@ xrh_status xrh_batch_set_views(xrh_batch* batch, const xrh_view* views, int32_t 
 xrh_status xrh_batch_destroy(xrh_batch* batch)
 {
   return guarded([&] { batch_free(batch); });
+}
+
+xrh_status xrh_batch_autotune(xrh_batch* batch, const int32_t* item_view, const double* item_pose, int64_t n_items,
+                              int32_t reps, xrh_tune_result* out)
+{
+  return guarded([&] { autotune(batch, item_view, item_pose, n_items, reps, out); });
+}
+
+xrh_status xrh_batch_set_drr_variant(xrh_batch* batch, const xrh_drr_variant* v)
+{
+  return guarded([&] {
+    XRH_REQUIRE(batch, "null batch");
+    XRH_REQUIRE(!batch->pending, "batch has a launch in flight");
+    if (!v) {
+      batch->variant_set = false;
+      batch->variant = DrrVariant();
+      return;
+    }
+    XRH_REQUIRE(v->cell_reuse >= -1 && v->cell_reuse <= 1, "cell_reuse must be -1, 0 or 1");
+    XRH_REQUIRE(v->wave_shape >= 0 && v->wave_shape <= 2, "wave_shape must be 0, 1 or 2");
+    XRH_REQUIRE(v->bands >= 1 && v->bands <= kMaxDrrBands, "bands must be in [1, 16]");
+    batch->variant.reuse = v->cell_reuse;
+    batch->variant.shape = v->wave_shape;
+    batch->variant.bands = v->bands;
+    batch->variant_set = true;
+  });
 }
 
 xrh_status xrh_batch_set_options(xrh_batch* batch, int32_t use_graph, int32_t time_kernels)

@@ -482,12 +482,23 @@ __device__ inline TileRef decode_tile(const uint32_t* band_start, int64_t num_it
   return t;
 }
 
-__device__ inline void lane_pixel(int tx, int ty, int& col, int& row)
+// The pixel of this lane in tile (tx, ty). shape (uniform) picks each wave's footprint in
+// the 16x16 tile: 0 = 8x8 quadrants, 1 = 16x4 row strips, 2 = 4x16 column strips; every
+// shape covers the tile's 256 pixels once.
+__device__ inline void lane_pixel(int tx, int ty, int shape, int& col, int& row)
 {
   const int t = threadIdx.x;
   const int w = t >> 6, l = t & 63;
-  col = tx * kTile + (w & 1) * 8 + (l & 7);
-  row = ty * kTile + (w >> 1) * 8 + (l >> 3);
+  if (shape == 1) {
+    col = tx * kTile + (l & 15);
+    row = ty * kTile + w * 4 + (l >> 4);
+  } else if (shape == 2) {
+    col = tx * kTile + w * 4 + (l & 3);
+    row = ty * kTile + (l >> 2);
+  } else {
+    col = tx * kTile + (w & 1) * 8 + (l & 7);
+    row = ty * kTile + (w >> 1) * 8 + (l >> 3);
+  }
 }
 
 __device__ inline double wave_sum_d(double v)
@@ -662,13 +673,13 @@ template <int kGeom, int kOp, int kFmt>
 __global__ __launch_bounds__(kThreads) void drr_kernel(
     const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
     int bands, float* __restrict__ out, int accumulate, float fill,
-    double* __restrict__ ncc_partials, int ncc_tiles)
+    double* __restrict__ ncc_partials, int ncc_tiles, int shape)
 {
   const TileRef tr = decode_tile(band_start, num_items, bands, items);
   if (!tr.valid) return;
   const DrrItem& it = items[tr.item];
   int col, row;
-  lane_pixel(tr.tx, tr.ty, col, row);
+  lane_pixel(tr.tx, tr.ty, shape, col, row);
   const bool inside = (col < it.cam.cols) && (row < it.cam.rows);
 
   float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
@@ -702,7 +713,7 @@ __global__ __launch_bounds__(kThreads) void drr_count_kernel(
   if (!tr.valid) return;
   const DrrItem& it = items[tr.item];
   int col, row;
-  lane_pixel(tr.tx, tr.ty, col, row);
+  lane_pixel(tr.tx, tr.ty, 0, col, row);
   double cnt = 0.0, evaluated = 0.0;
   if ((col < it.cam.cols) && (row < it.cam.rows)) {
     const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
@@ -741,14 +752,17 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
 #define XRH_LAUNCH_FMT(F)                                                                    \
   hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev,   \
                      L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,        \
-                     L.ncc_partials, L.ncc_tiles_per_item)
+                     L.ncc_partials, L.ncc_tiles_per_item, shape)
   // Cell reuse (one sample per iteration, loads only on cell change) wins when the launch
   // fills every CU many times over and the kernel is bound by load issue; small launches are
   // latency-bound and prefer the two-samples-per-iteration kernel. XRH_CELL_REUSE=0 / =1
-  // force either (read per launch, so a process can switch between calls).
+  // force either (read per launch, so a process can switch between calls); otherwise a tuned
+  // variant decides, else the launch size.
   const char* reuse_env = std::getenv("XRH_CELL_REUSE");
   const bool reuse = reuse_env ? (std::strcmp(reuse_env, "0") != 0)
-                               : ((int64_t)grid.x * grid.y >= 16ll * device_cus());
+                     : (L.reuse >= 0) ? (L.reuse != 0)
+                                      : ((int64_t)grid.x * grid.y >= 16ll * device_cus());
+  const int shape = (L.wave_shape >= 0 && L.wave_shape <= 2) ? L.wave_shape : 0;
   const int fmt = (L.storage == XRH_F32)
                       ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6)
                                                  : (L.layout == kLayoutFace ? (reuse ? 11 : 10)
@@ -771,6 +785,20 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
 }
 
 }  // namespace
+
+bool drr_default_reuse(int64_t workgroups) { return workgroups >= 16ll * device_cus(); }
+
+std::string drr_tune_key(int32_t storage, int32_t layout, int32_t geom, int32_t kernel, int32_t max_rows,
+                         int32_t max_cols, int64_t workgroups)
+{
+  const int64_t per_cu = workgroups / (int64_t)device_cus();
+  int lg = 0;
+  while (lg < 40 && (2ll << lg) <= per_cu) ++lg;
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "s%d.l%d.g%d.k%d.%dx%d.wg%d", storage, layout, geom, kernel, max_rows, max_cols,
+                per_cu > 0 ? lg : -1);
+  return buf;
+}
 
 TilePlan build_tiles(const std::vector<int32_t>& rows, const std::vector<int32_t>& cols, int32_t bands)
 {

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -157,6 +158,19 @@ inline void check_step(float step, const VolDesc& v)
 // ------------------------------------------------------------------------------------
 // opaque objects
 
+namespace xrh {
+// How a DRR launch is laid out (results are identical under every variant):
+//   reuse  1: one sample per loop iteration, loads only when a sample enters a new cell;
+//          0: two samples per iteration, every sample loads; -1: chosen by launch size
+//   shape  pixels per wave64 inside the 16x16 workgroup tile: 0 = 8x8, 1 = 16x4, 2 = 4x16
+//   bands  detector-row bands of the XCD-aware workgroup order
+struct DrrVariant {
+  int32_t reuse = -1;
+  int32_t shape = 0;
+  int32_t bands = 8;
+};
+}  // namespace xrh
+
 struct xrh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -170,6 +184,8 @@ struct xrh_ctx {
   uint64_t scratch_bytes = 0;
   void* pinned = nullptr;
   uint64_t pinned_bytes = 0;
+  // DRR variants chosen by xrh_batch_autotune, keyed by drr_tune_key
+  std::map<std::string, xrh::DrrVariant> tuned;
 };
 
 struct xrh_volume {
@@ -228,7 +244,16 @@ struct DrrLaunch {
   int32_t ncc_tiles_per_item;
   uint64_t* sample_counter; // optional: count samples instead of writing pixels; [0] = all
                             // samples of the reference algorithm, [1] = samples evaluated
+  int32_t reuse;            // DrrVariant::reuse
+  int32_t wave_shape;       // DrrVariant::shape
 };
+// Key of the tuned-variant table: storage, layout, geometry, line kernel, the largest
+// detector of the launch and its workgroups per CU (rounded down to a power of two).
+std::string drr_tune_key(int32_t storage, int32_t layout, int32_t geom, int32_t kernel, int32_t max_rows,
+                         int32_t max_cols, int64_t workgroups);
+// The cell-reuse choice of an untuned launch of `workgroups` workgroups (reuse = -1).
+bool drr_default_reuse(int64_t workgroups);
+constexpr int32_t kMaxDrrBands = 16;
 void launch_drr_setup(const DrrLaunch& L, hipStream_t s);
 void launch_drr(const DrrLaunch& L, hipStream_t s);
 

@@ -500,8 +500,17 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   ctx_check_range(ctx, drr_dev, (uint64_t)(npx * num_projs) * 4, "DRR buffer");
   if (bg_dev) ctx_check_range(ctx, bg_dev, (uint64_t)(npx * num_cams) * 4, "background images");
 
+  // a variant tuned for this launch shape (xrh_batch_autotune), else the defaults
+  DrrVariant var;
+  {
+    const int64_t wgs = (int64_t)((rows + 15) / 16) * ((cols + 15) / 16) * num_projs;
+    const std::string key = drr_tune_key(vol->desc.storage, vol->desc.layout, geom, kernel, rows, cols, wgs);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) var = it->second;
+  }
   std::vector<int32_t> rr((size_t)num_projs, rows), cc((size_t)num_projs, cols);
-  const TilePlan plan = build_tiles(rr, cc, 8);
+  const TilePlan plan = build_tiles(rr, cc, var.bands);
 
   auto align = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
   const uint64_t off_pose = 0;
@@ -563,6 +572,8 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   L.layout = vol->desc.layout;
   L.out = drr_dev;
   L.sample_counter = samples_out ? (uint64_t*)(dev + off_cnt) : nullptr;
+  L.reuse = var.reuse;
+  L.wave_shape = var.shape;
   launch_drr_setup(L, ctx->stream);
   launch_drr(L, ctx->stream);
   if (samples_out) {
