@@ -11,8 +11,9 @@ from tests.helpers import SIM_ABS_TOL, poses_f32
 pytestmark = pytest.mark.gpu
 
 
-def _mixed_batch(ctx, oracle):
-    """Two views of different detector sizes (one NCC, one patch gradient-NCC), 5 poses each."""
+def _mixed_batch(ctx, oracle, metrics=(0, 3)):
+    """Two views of different detector sizes (metrics: NCC and patch gradient-NCC by default),
+    5 poses each."""
     from xreg_amd import _xrh as X
     from xreg_amd import geometry as G
     from xreg_amd import synth
@@ -21,7 +22,7 @@ def _mixed_batch(ctx, oracle):
     vol = X.Volume(ctx, w.vol, w.idx_to_phys)
     cams = [w.cams[0], G.downsample_camera(w.cams[0], 0.5)]
     fixeds, views, fixed_imgs = [], [], []
-    for cam, metric in zip(cams, (X.XRH_SIM_NCC, X.XRH_SIM_PATCH_GRAD_NCC)):
+    for cam, metric in zip(cams, metrics):
         g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.gt[None]), step=w.step)
         img = synth.noisy_fixed(g[0])
         fx = X.Fixed(ctx, img, metric)
@@ -35,11 +36,13 @@ def _mixed_batch(ctx, oracle):
     return w, vol, cams, fixeds, fixed_imgs, views, iv, ip, npx
 
 
-def test_every_variant_casts_the_same_drrs(ctx, oracle):
+@pytest.mark.parametrize("metrics", [(0, 3), (1, 3)])
+def test_every_variant_casts_the_same_drrs(ctx, oracle, metrics):
+    """With and without an NCC view (whose sums the DRR epilogue fuses)."""
     from xreg_amd import _xrh as X
 
-    print("case: drr variants", flush=True)
-    w, vol, cams, fixeds, imgs, views, iv, ip, npx = _mixed_batch(ctx, oracle)
+    print(f"case: drr variants metrics={metrics}", flush=True)
+    w, vol, cams, fixeds, imgs, views, iv, ip, npx = _mixed_batch(ctx, oracle, metrics)
     b = X.Batch(ctx, views, max_items=16)
     out = ctx.alloc(4 * npx)
     ref_sims = b.eval(iv, ip, drr=out)
@@ -48,15 +51,18 @@ def test_every_variant_casts_the_same_drrs(ctx, oracle):
     # the default variant matches the CPU reference
     for v, cam in enumerate(cams):
         d, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses_f32(w.poses), step=w.step)
-        ref = oracle.sim_f32((0, 3)[v], imgs[v], d)
+        ref = oracle.sim_f32(metrics[v], imgs[v], d)
         assert np.abs(ref_sims[v * n:(v + 1) * n] - ref).max() <= SIM_ABS_TOL
     for reuse, shape, bands in itertools.product((0, 1), (0, 1, 2), (1, 4, 8, 16)):
         b.set_drr_variant(reuse, shape, bands)
         sims = b.eval(iv, ip, drr=out)
         assert np.array_equal(out.download(np.float32), ref_drr), (reuse, shape, bands)
-        # the patch metric reads the DRRs back: identical; the fused NCC sums change order
+        # metrics that read the DRRs back: identical; fused NCC sums change order
         assert np.array_equal(sims[n:], ref_sims[n:]), (reuse, shape, bands)
-        assert np.abs(sims[:n] - ref_sims[:n]).max() <= 1e-6, (reuse, shape, bands)
+        if metrics[0] == 0:
+            assert np.abs(sims[:n] - ref_sims[:n]).max() <= 1e-6, (reuse, shape, bands)
+        else:
+            assert np.array_equal(sims[:n], ref_sims[:n]), (reuse, shape, bands)
     b.set_drr_variant(None)
     assert np.array_equal(b.eval(iv, ip), ref_sims)
     for bad in ((2, 0, 8), (0, 3, 8), (0, 0, 0), (0, 0, 17)):
