@@ -66,6 +66,9 @@ def parse():
                     help="mixed served stream (three resident phantoms, 192/384/768 detectors, lambda 20-200, "
                          "30-80 generations, one or three views): number of jobs, run batched and serially")
     ap.add_argument("--serve-mix-rate", type=float, default=0.25, help="mixed stream: Poisson rate, jobs/s")
+    ap.add_argument("--c3-regi", action="store_true",
+                    help="also time a three-view, three-level registration (96^2 CMA-ES, then the refiner "
+                         "at 192^2 and 384^2)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run plain device copies of this many MiB (counter calibration under --pmc)")
     return ap.parse_args()
@@ -260,6 +263,56 @@ def serve_mix_leg(ctx, n_jobs, rate, geom, smooth, policy, step_samples, seed=31
             "same_work": n_b == n_s}
 
 
+def c3_leg(ctx, geom, smooth, seed=41):
+    """Multi-view, multi-resolution registration: three views 30 degrees apart of the
+    256x256x200 phantom with 384^2 detectors, gradient-NCC averaged over the views. CMA-ES
+    (lambda 50, up to 50 generations) on the 96^2 level, then the BOBYQA-pattern refiner (up to
+    20 x 13 single-pose evaluations) at 192^2 and at 384^2. The fixed images are the
+    ground-truth DRRs plus 1% noise; the start is U(+-5 deg, +-10 mm) away. Reports each
+    level's wall-clock time (set-up included), evaluations and the mean projection distance."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import geometry as G
+    from xreg_amd import regi, synth
+
+    rng = np.random.default_rng(seed)
+    w = synth.config("c3", n_poses=1)
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    fixed = []
+    for v, cam in enumerate(w.cams):
+        buf = ctx.alloc(4 * cam.num_det_rows * cam.num_det_cols)
+        X.line_int(ctx, vol, [cam], w.gt.astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
+        fixed.append(synth.noisy_fixed(buf.download(np.float32).reshape(cam.num_det_rows, cam.num_det_cols),
+                                       seed=40 + v))
+        buf.free()
+    init = G.perturbed_pose(w.gt, w.center, np.deg2rad(rng.uniform(-5, 5, 3)), rng.uniform(-10, 10, 3))
+    cam0 = w.cams[0]
+
+    def ds(img, f):
+        return X.downsample_image(ctx, img, f, spacing=(cam0.det_row_spacing, cam0.det_col_spacing))
+
+    levels = [regi.Level(0.25, "cmaes", dict(lam=50, sigma=[np.deg2rad(5.0)] * 3 + [10.0] * 3, max_gens=50,
+                                             seed=seed)),
+              regi.Level(0.5, "refine", dict(rho_beg=[np.deg2rad(0.5)] * 3 + [1.0] * 3, max_evals=260)),
+              regi.Level(1.0, "refine", dict(rho_beg=[np.deg2rad(0.25)] * 3 + [0.5] * 3, max_evals=260))]
+    t0 = time.perf_counter()
+    pose, res = regi.run_multi_level(ctx, vol, w.cams, fixed, init, w.center, w.step, levels, ds,
+                                     metric=X.XRH_SIM_GRAD_NCC, smooth_ksize=smooth, geom=geom)
+    wall = time.perf_counter() - t0
+    vol.close()
+    ext = np.array(w.vol.shape[::-1]) * w.spacing
+    pts = [w.center + 0.25 * ext * np.array(c) for c in [(-1, -1, -1), (1, 1, 1), (1, -1, 1), (-1, 1, -1),
+                                                          (0, 0, 0)]]
+    mpd0 = G.mean_projection_distance(init, w.gt, pts, 1020.0)
+    out = {"views": 3, "total_wall_s": round(wall, 3), "mpd_start_mm": round(mpd0, 3), "levels": []}
+    for lvl, r in zip(levels, res):
+        det = int(round(cam0.num_det_rows * lvl.ds_factor))
+        out["levels"].append({"detector": det, "method": lvl.method, "evals": int(r.evals),
+                              "wall_s": round(float(r.extra["wall_s"]), 3),
+                              "mpd_mm": round(G.mean_projection_distance(r.pose, w.gt, pts, 1020.0), 3)})
+    print(f"c3 registration: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse()
     if args.serve_step_samples <= 0:
@@ -367,6 +420,8 @@ def main():
         served_mix = serve_mix_leg(ctx, args.serve_mix_jobs, args.serve_mix_rate, geom, args.smooth,
                                    args.serve_policy, args.serve_step_samples)
 
+    c3 = c3_leg(ctx, geom, args.smooth) if args.c3_regi else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, fixed, gens[0], args.cpu_seconds, args.smooth)
@@ -405,6 +460,7 @@ def main():
         "autotune": tune,
         "served": served,
         "served_mix": served_mix,
+        "c3_registration": c3,
         "cpu_baseline": cpu,
         "sim_checksum": float(np.sum(sims)),
     }

@@ -12,6 +12,7 @@ by the number of views).
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -247,10 +248,12 @@ def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels,
     """MultiLevelMultiObjRegi::run (lib/regi/interfaces_2d_3d/xregMultiObjMultiLevel2D3DRegi.cpp:
     164-531), single object: at each level the cameras are downsampled (DownsampleCameraModel)
     and the fixed images with `downsample_image(img, ds)`; the level's registration starts
-    from the previous level's estimate, with its own objective (caster + metrics)."""
+    from the previous level's estimate, with its own objective (caster + metrics). Each
+    result's extra["wall_s"] is the level's wall-clock time, set-up included."""
     pose = np.asarray(init_pose, dtype=np.float64)
     results = []
     for lvl in levels:
+        t0 = time.perf_counter()
         ds = float(lvl.ds_factor)
         lcams = [c if ds == 1.0 else G.downsample_camera(c, ds) for c in cams]
         limgs = [img if ds == 1.0 else downsample_image(img, ds) for img in fixed_imgs]
@@ -272,6 +275,7 @@ def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels,
                 raise ValueError(lvl.method)
         finally:
             obj.close()
+        r.extra["wall_s"] = time.perf_counter() - t0
         results.append(r)
         pose = r.pose
     return pose, results
