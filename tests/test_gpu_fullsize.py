@@ -183,3 +183,34 @@ def test_c3_full_multiview_pyramid_registration(ctx, oracle):
     print(f"mPD {e0:.3f} -> " + " -> ".join(f"{e:.3f}" for e in es) + f"; evals {[r.evals for r in res]}")
     assert es[-1] < 0.3 * e0
     assert res[-1].fun <= res[-1].history[0]
+
+
+def test_volume_past_4gib_packed_layout_falls_back_to_linear(ctx, oracle):
+    """A 700x700x560 FP32 volume: its 16-byte-per-voxel packed layout would pass the 4 GiB
+    reach of 32-bit buffer offsets, so the automatic choice keeps the padded linear layout
+    (1.1 GB), and the DRRs still match the oracle."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import geometry as G
+
+    print("case: 700x700x560 volume, linear fallback", flush=True)
+    X_, Y_, Z_ = 700, 700, 560
+    sp = 0.5
+    vol = np.zeros((Z_, Y_, X_), np.float32)
+    vol[200:360, 250:450, 250:450] = 0.02
+    vol[250:300, 300:350, 320:420] = 0.05
+    vol[280:282, 100:600, 349:351] = 0.08
+    i2p = G.idx_to_phys(sp)
+    center = G.vol_center_phys((X_, Y_, Z_), sp)
+    gt = G.gt_pose(center, 510.0)
+    cam = G.camera(1020.0, 64, 64, 4.656, 4.656)
+    poses = np.stack([gt, G.perturbed_pose(gt, center, np.deg2rad([4.0, -3.0, 5.0]), [5.0, -4.0, 3.0])])
+    v = X.Volume(ctx, vol, i2p)
+    assert v.nbytes < 2 << 30  # 4 bytes per padded voxel, not 16
+    buf = ctx.alloc(4 * 2 * 64 * 64)
+    X.line_int(ctx, v, [cam], poses_f32(poses), np.zeros(2, np.int64), 0.25, buf)
+    got = buf.download(np.float32).reshape(2, 64, 64)
+    buf.free()
+    v.close()
+    ref, _ = oracle.drr_f32(vol, i2p, [cam], poses_f32(poses), step=0.25)
+    assert ref.max() > 0.5
+    assert drr_rel_err(got, ref).max() <= DRR_REL_TOL

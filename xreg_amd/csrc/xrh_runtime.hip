@@ -378,10 +378,15 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
     XRH_REQUIRE(auto_layout, "coefficient layout exceeds 4 GiB; use the quad layout");
     layout = kLayoutQuad;
   }
+  // buffer loads address with 32-bit byte offsets: with the automatic choice, a volume whose
+  // packed layout would pass 4 GiB is kept in the padded linear layout (one scalar per voxel)
+  const uint64_t packed_esize = (layout == kLayoutCoef) ? 32 : 16;
+  if (auto_layout && layout != kLayoutLinear && (uint64_t)padded * packed_esize >= 0xFFFFF000ull) {
+    layout = kLayoutLinear;
+  }
   const int64_t esize = (layout == kLayoutLinear) ? (storage == XRH_F32 ? 4 : 2) : (layout == kLayoutCoef ? 32 : 16);
   const uint64_t bytes = (uint64_t)(padded * esize);
-  // buffer loads address with 32-bit byte offsets
-  XRH_REQUIRE(bytes < 0xFFFFF000ull, "volume layout exceeds 4 GiB; use linear layout or split");
+  XRH_REQUIRE(bytes < 0xFFFFF000ull, "volume layout exceeds 4 GiB; use the linear layout or split the volume");
   auto* v = new xrh_volume();
   v->ctx = ctx;
   v->size[0] = X;
