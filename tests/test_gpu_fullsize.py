@@ -214,3 +214,42 @@ def test_volume_past_4gib_packed_layout_falls_back_to_linear(ctx, oracle):
     ref, _ = oracle.drr_f32(vol, i2p, [cam], poses_f32(poses), step=0.25)
     assert ref.max() > 0.5
     assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
+
+
+def test_c4_precision_against_fp64_oracle(ctx, oracle, c4full):
+    """The 512x512x400 @0.5 mm volume stored as FP32, FP16 and BF16 (same seeded content; FP32
+    accumulation, FP64 ray set-up), against the FP64 oracle (FP64 volume and accumulation):
+    per-pixel DRR error relative to the image maximum and NCC error. The detector is the C4
+    one read out at 192^2 (same field of view) so the FP64 oracle finishes in seconds."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import geometry as G
+    from xreg_amd import synth
+
+    print("case: c4 precision vs FP64 oracle, 3 storages x 2 poses", flush=True)
+    w = c4full
+    cam = G.downsample_camera(w.cams[0], 0.25)
+    poses = w.poses[:2]
+    ref, _ = oracle.drr_f64(w.vol, w.idx_to_phys, [cam], poses, step=w.step)
+    gt, _ = oracle.drr_f64(w.vol, w.idx_to_phys, [cam], w.gt[None], step=w.step)
+    fixed = synth.noisy_fixed(gt[0].astype(np.float32))
+    ncc_ref = oracle.ncc_f64(fixed, ref)
+    scale = float(ref.max())
+    # FP32: float accumulation over ~1,600 samples per ray; FP16 / BF16: voxel rounding
+    # (2^-11 / 2^-8 relative) on top
+    bounds = {0: (3e-5, 1e-5), 1: (2.0 ** -11, 1e-4), 2: (2.0 ** -8, 1e-3)}
+    fx = X.Fixed(ctx, fixed, X.XRH_SIM_NCC)
+    for storage in (0, 1, 2):
+        v = X.Volume(ctx, w.vol, w.idx_to_phys, storage=storage)
+        buf = ctx.alloc(4 * 2 * cam.num_det_rows * cam.num_det_cols)
+        X.line_int(ctx, v, [cam], poses_f32(poses), np.zeros(2, np.int64), w.step, buf, geom=1)
+        got = buf.download(np.float32).reshape(ref.shape)
+        ncc = fx.compute(buf, 2)
+        buf.free()
+        v.close()
+        derr = np.abs(got.astype(np.float64) - ref) / scale
+        nerr = np.abs(ncc.astype(np.float64) - ncc_ref)
+        print(f"storage {('fp32', 'fp16', 'bf16')[storage]}: DRR |err|/max max {derr.max():.3e} "
+              f"median {np.median(derr):.3e}; NCC |err| max {nerr.max():.3e}", flush=True)
+        assert derr.max() < bounds[storage][0]
+        assert nerr.max() < bounds[storage][1]
+    fx.close()
