@@ -253,3 +253,35 @@ def test_c4_precision_against_fp64_oracle(ctx, oracle, c4full):
         assert derr.max() < bounds[storage][0]
         assert nerr.max() < bounds[storage][1]
     fx.close()
+
+
+def test_c1_full_drr_and_ncc_match_oracle(ctx, oracle):
+    """The plain single-view case at full size: 128x128x96 @2 mm, 192^2 detector, 1 mm steps,
+    100 poses (U(+-15 deg, +-20 mm)), fixed image = ground-truth DRR + 1% noise: the 100 DRRs
+    and 100 NCC values against the FP32 restatement of the CPU reference, through the plain
+    path (one launch for the DRRs, one for the similarities)."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    print("case: c1 full size, 100 poses DRR + NCC", flush=True)
+    w = synth.config("c1", n_poses=100)
+    cam = w.cams[0]
+    n = w.poses.shape[0]
+    g, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(w.gt[None]), step=w.step)
+    fixed = synth.noisy_fixed(g[0])
+    ref, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(w.poses), step=w.step)
+    ref_ncc = oracle.ncc_f32(fixed, ref)
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    buf = ctx.alloc(4 * n * cam.num_det_rows * cam.num_det_cols)
+    X.line_int(ctx, vol, [cam], poses_f32(w.poses), np.zeros(n, np.int64), w.step, buf)
+    got = buf.download(np.float32).reshape(ref.shape)
+    fx = X.Fixed(ctx, fixed, X.XRH_SIM_NCC)
+    ncc = fx.compute(buf, n)
+    fx.close()
+    buf.free()
+    vol.close()
+    err = drr_rel_err(got, ref)
+    print(f"max DRR rel err {err.max():.3e}; max NCC err {np.abs(ncc - ref_ncc).max():.3e}")
+    assert err.max() <= DRR_REL_TOL
+    assert np.abs(ncc - ref_ncc).max() <= SIM_ABS_TOL
+    assert np.abs(ncc - oracle.ncc_f64(fixed, ref)).max() <= SIM_ABS_TOL
