@@ -92,6 +92,14 @@ def test_two_level_registration_and_refiner(ctx, oracle):
     assert res[1].fun <= res[1].history[0]
     assert e2 < 0.5 * e0
     assert res[1].evals <= 80 + 1
+    # the same registration with the DRR launch variant tuned per level and submission size:
+    # DRRs are identical under every variant, so is every value and the whole trajectory
+    pose_t, res_t = regi.run_multi_level(ctx, vol, [cam], fixed, init, w.center, w.step, levels, ds, autotune=True)
+    assert np.array_equal(pose_t, pose)
+    for a, b in zip(res, res_t):
+        assert a.history == b.history and a.evals == b.evals
+    assert set(res_t[0].extra["autotune"]) == {20}  # lambda poses per CMA-ES submission
+    assert set(res_t[1].extra["autotune"]) == {1}   # one pose per refiner evaluation
     vol.close()
 
 

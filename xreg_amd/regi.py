@@ -52,8 +52,14 @@ class Objective:
 
     def __init__(self, ctx: X.Context, vol: X.Volume, views, param: PoseParam, step: float,
                  metric: int = X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize: int = 5, patch_radius: int = 5,
-                 max_batch: int = 256, geom: int = X.XRH_GEOM_FP32, use_graph: bool = True):
+                 max_batch: int = 256, geom: int = X.XRH_GEOM_FP32, use_graph: bool = True,
+                 autotune: bool = False):
+        """autotune: tune the DRR launch variant (xrh_batch_autotune) on each new submission
+        size the first time it is evaluated (values are unchanged; the first call of each size
+        then costs 26 x 3 extra submissions)."""
         self.ctx, self.vol, self.param, self.step = ctx, vol, param, float(step)
+        self.autotune = bool(autotune)
+        self.tuned_sizes = {}     # poses per submission -> xrh_batch_autotune result
         self.fixed = [X.Fixed(ctx, img, metric, smooth_ksize=smooth_ksize, patch_radius=patch_radius)
                       for _, img in views]
         self.cams = [c for c, _ in views]
@@ -72,7 +78,10 @@ class Objective:
             p = poses[s:s + self.max_batch]
             k = len(p)
             iv = np.repeat(np.arange(self.V, dtype=np.int32), k)
-            sims = self.batch.eval(iv, np.tile(p, (self.V, 1, 1)))
+            ip = np.tile(p, (self.V, 1, 1))
+            if self.autotune and k not in self.tuned_sizes:
+                self.tuned_sizes[k] = self.batch.autotune(iv, ip, reps=2)
+            sims = self.batch.eval(iv, ip)
             out[s:s + k] = combine_mean(sims.reshape(self.V, k))
             self.calls += 1
         self.evals += len(poses)
@@ -244,12 +253,13 @@ class Level:
 
 def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels, downsample_image,
                     metric: int = X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize: int = 5, patch_radius: int = 5,
-                    geom: int = X.XRH_GEOM_FP32, max_batch: int = 256):
+                    geom: int = X.XRH_GEOM_FP32, max_batch: int = 256, autotune: bool = False):
     """MultiLevelMultiObjRegi::run (lib/regi/interfaces_2d_3d/xregMultiObjMultiLevel2D3DRegi.cpp:
     164-531), single object: at each level the cameras are downsampled (DownsampleCameraModel)
     and the fixed images with `downsample_image(img, ds)`; the level's registration starts
     from the previous level's estimate, with its own objective (caster + metrics). Each
-    result's extra["wall_s"] is the level's wall-clock time, set-up included."""
+    result's extra["wall_s"] is the level's wall-clock time, set-up included. autotune: each
+    level's objective tunes the DRR launch variant for its detector size (Objective)."""
     pose = np.asarray(init_pose, dtype=np.float64)
     results = []
     for lvl in levels:
@@ -262,7 +272,8 @@ def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels,
                 raise ValueError(f"level {ds}: image {im.shape} does not match camera "
                                  f"{c.num_det_rows}x{c.num_det_cols}")
         obj = Objective(ctx, vol, list(zip(lcams, limgs)), PoseParam(pose, center), step, metric=metric,
-                        smooth_ksize=smooth_ksize, patch_radius=patch_radius, geom=geom, max_batch=max_batch)
+                        smooth_ksize=smooth_ksize, patch_radius=patch_radius, geom=geom, max_batch=max_batch,
+                        autotune=autotune)
         try:
             o = dict(lvl.opts)
             if lvl.method == "cmaes":
@@ -276,6 +287,8 @@ def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels,
         finally:
             obj.close()
         r.extra["wall_s"] = time.perf_counter() - t0
+        if autotune:
+            r.extra["autotune"] = dict(obj.tuned_sizes)
         results.append(r)
         pose = r.pose
     return pose, results
