@@ -110,3 +110,32 @@ def test_batch_uneven_sizes_and_mixed_metrics(ctx, oracle):
     b.close()
     for o in objs:
         o.close()
+
+
+def test_async_launch_and_wait(ctx, oracle):
+    """xrh_batch_launch queues a generation and returns; xrh_batch_wait delivers the same values
+    as xrh_batch_eval. A second launch before the wait, and a wait with nothing launched, are
+    refused without touching the queued work."""
+    from xreg_amd import _xrh as X
+
+    print("case: batch launch / wait", flush=True)
+    w, vol, fx, fixed = _setup(ctx, oracle, "c2", 3)
+    b = X.Batch(ctx, [X.make_view(vol, fx, w.cams[0], w.step)], max_items=16)
+    iv = np.zeros(w.poses.shape[0], np.int32)
+    want = b.eval(iv, w.poses)
+    assert X.lib().xrh_ctx_stream(ctx.h) is not None
+    with pytest.raises(X.XrhError):
+        b.wait()  # nothing launched
+    for use_graph in (True, False):
+        b.set_options(use_graph=use_graph, time_kernels=False)
+        b.launch(iv, w.poses)
+        _ = np.linalg.svd(np.random.default_rng(1).normal(size=(96, 96)))  # host work meanwhile
+        with pytest.raises(X.XrhError):
+            b.launch(iv, w.poses)
+        got = b.wait()
+        assert np.array_equal(got, want)
+        with pytest.raises(X.XrhError):
+            b.wait()
+    b.close()
+    fx.close()
+    vol.close()

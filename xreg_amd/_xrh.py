@@ -379,12 +379,13 @@ class Batch:
         return out
 
     def launch(self, item_view: np.ndarray, item_pose: np.ndarray) -> None:
-        self._iv = np.ascontiguousarray(item_view, dtype=np.int32)
-        self._ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
-        check(lib().xrh_batch_launch(self.h, self._iv.ctypes.data_as(_PI32), _dp(self._ip), self._iv.shape[0]))
+        iv = np.ascontiguousarray(item_view, dtype=np.int32)
+        ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
+        check(lib().xrh_batch_launch(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0]))
+        self._launched = iv.shape[0]  # only a launch the library accepted sizes the wait
 
     def wait(self) -> np.ndarray:
-        out = np.empty(self._iv.shape[0], dtype=np.float32)
+        out = np.empty(getattr(self, "_launched", 0), dtype=np.float32)
         check(lib().xrh_batch_wait(self.h, _fp(out)))
         return out
 
