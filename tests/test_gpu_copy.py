@@ -56,3 +56,36 @@ def test_buffer_ranges_are_checked_before_any_launch(ctx):
     fx.close()
     buf.free()
     vol.close()
+
+
+def test_context_hbm_budget(ctx):
+    """Everything a context allocates is charged to its budget: an allocation or a volume that
+    would pass it fails with XRH_ERR_BUDGET and leaves the charge unchanged; frees return it."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    print("case: hbm budget", flush=True)
+    c = X.Context(0, hbm_budget_bytes=64 << 20)
+    try:
+        base = c.bytes_in_use()
+        a = c.alloc(32 << 20)
+        assert c.bytes_in_use() == base + (32 << 20)
+        with pytest.raises(X.XrhError) as e:
+            c.alloc(48 << 20)
+        assert e.value.status == X.XRH_ERR_BUDGET
+        assert c.bytes_in_use() == base + (32 << 20)
+        w = synth.config("c2", n_poses=1)  # 256x256x200: a 215 MB layout
+        with pytest.raises(X.XrhError) as e:
+            X.Volume(c, w.vol, w.idx_to_phys)
+        assert e.value.status == X.XRH_ERR_BUDGET
+        assert c.bytes_in_use() == base + (32 << 20)
+        a.free()
+        assert c.bytes_in_use() == base
+        small = synth.config("c1", n_poses=1, small=True)
+        v = X.Volume(c, small.vol, small.idx_to_phys)
+        assert 0 < v.nbytes <= c.bytes_in_use() - base
+        v.close()
+        c.synchronize()
+        assert c.bytes_in_use() == base
+    finally:
+        c.close()
