@@ -18,8 +18,9 @@
 //     (hardware range check: a stray offset returns 0, it can never fault); the quad (f32)
 //     and oct (f16/bf16) layouts pre-pack interpolation corners so a sample costs 2 or 1
 //     16-byte loads instead of 8 scalar gathers;
-//   - trilinear weights and lerps are FP32 in software (hardware texture filtering has
-//     8-bit weights, far outside the 1e-4 per-pixel tolerance).
+//   - trilinear weights and lerps are FP32 in software: HIP's image/texture API is not
+//     available for gfx950 (hipcc rejects tex3D for this target), so there is no hardware
+//     filtering to fall back on.
 #include "xrh_internal.h"
 
 #include <hip/hip_fp16.h>
