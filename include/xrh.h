@@ -163,6 +163,10 @@ xrh_status xrh_volume_create_hu(xrh_ctx* ctx, const float* vox_hu, const int64_t
                                 double hu_lower, double mu_water, double mu_air, xrh_volume** out);
 /* Device bytes held by the volume. */
 uint64_t xrh_volume_bytes(const xrh_volume* vol);
+/* Storage type and HBM layout a volume was built with (the automatic choice included: a
+ * volume whose packed layout would pass 4 GiB is kept linear). Volumes batched together must
+ * share both. */
+xrh_status xrh_volume_info(const xrh_volume* vol, int32_t* storage, int32_t* layout);
 
 /* ---------------------------------------------------------------------------------- */
 /* line-integral ray casting: RayCasterLineIntCPU::compute(vol_idx)

@@ -86,3 +86,27 @@ def test_registration_improves_pose(ctx, oracle):
         assert err1 < 0.5 * err0
     for v in vols.values():
         v.close()
+
+
+def test_volumes_of_different_storage_in_one_server(ctx, oracle):
+    """Jobs on an FP32 volume and on an FP16 volume served together: each submission holds one
+    storage type, and every job's result equals its run alone."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import serve
+
+    print("case: serve FP32 + FP16 volumes", flush=True)
+    specs, vols, wl = _jobs(oracle, ctx, 4)
+    w2 = wl["c2"][0]
+    vols["c2"].close()
+    vols["c2"] = X.Volume(ctx, w2.vol, w2.idx_to_phys, storage=X.XRH_F16)
+    assert (vols["c1"].storage, vols["c2"].storage) == (X.XRH_F32, X.XRH_F16)
+    assert vols["c1"].layout != vols["c2"].layout
+    srv = serve.RegistrationServer(ctx, vols, hbm_budget=8 << 30, max_items=256, max_views=16, max_det=96)
+    mixed = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    assert len(mixed) == 4
+    for s in specs:
+        alone = srv.run([s], realtime=False)[0]
+        assert alone.history == mixed[s.job_id].history, s.job_id
+    srv.close()
+    for v in vols.values():
+        v.close()

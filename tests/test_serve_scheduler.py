@@ -183,6 +183,34 @@ def test_fifo_policy_with_limit_keeps_arrival_order():
     assert res[0].finish <= min(res[j].finish for j in (1, 2, 3))
 
 
+def test_jobs_on_volumes_of_different_layouts_never_share_a_submission():
+    """A batch casts volumes of one storage type and layout: jobs on an FP16 volume and on an
+    FP32 one alternate between submissions instead of failing, with the same results."""
+    ctx = FakeCtx()
+    vols = {"a": types.SimpleNamespace(storage=0, layout=4), "b": types.SimpleNamespace(storage=1, layout=2)}
+    specs = _specs(4, lam=10, gens=5)
+    for j, sp in enumerate(specs):
+        sp.volume = "ab"[j % 2]
+    seen = []
+
+    class LayoutBatch(FakeBatch):
+        def eval(self, item_view, item_pose):
+            keys = {(self.views[v].vol.storage, self.views[v].vol.layout) for v in set(item_view)}
+            seen.append(keys)
+            return super().eval(item_view, item_pose)
+
+    dev = types.SimpleNamespace(Batch=LayoutBatch, Fixed=FakeFixed,
+                                make_view=lambda vol, fx, cam, step: types.SimpleNamespace(fixed=fx, cam=cam, vol=vol))
+    srv = serve.RegistrationServer(ctx, vols, hbm_budget=1 << 40, max_items=1000, max_views=8, max_det=64,
+                                   device=dev, policy="fifo", step_samples=None)
+    res = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    assert len(res) == 4 and all(len(k) == 1 for k in seen)
+    assert {next(iter(k)) for k in seen} == {(0, 4), (1, 2)}
+    for sp in specs:
+        alone = srv.run([sp], realtime=False)[0]
+        assert alone.history == res[sp.job_id].history
+
+
 def test_deadline_finishes_jobs_with_their_current_estimate():
     """A job past its in-service limit ends after the generation in flight, with the CMA-ES mean
     as its estimate and status "deadline"; jobs without a limit run their generations."""

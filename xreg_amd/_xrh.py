@@ -116,6 +116,7 @@ SIGNATURES = {
                                     C.POINTER(_P)]),
     "xrh_volume_destroy": (_I32, [_P]),
     "xrh_volume_bytes": (_U64, [_P]),
+    "xrh_volume_info": (_I32, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "xrh_line_int": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
                             _I32, _P]),
     "xrh_line_int_sample_count": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32,
@@ -266,6 +267,9 @@ class Volume:
         # longest physical edge of the volume box (mm): the schedulers' per-ray work estimate
         axes = np.asarray(idx_to_phys, dtype=np.float64).reshape(4, 4)[:3, :3]
         self.extent_mm = float(max(self.size[i] * np.linalg.norm(axes[:, i]) for i in range(3)))
+        st, ly = C.c_int32(0), C.c_int32(0)
+        check(lib().xrh_volume_info(self.h, C.byref(st), C.byref(ly)))
+        self.storage, self.layout = int(st.value), int(ly.value)
 
     @property
     def nbytes(self) -> int:

@@ -754,6 +754,15 @@ xrh_status xrh_volume_destroy(xrh_volume* vol)
 
 uint64_t xrh_volume_bytes(const xrh_volume* vol) { return vol ? vol->bytes : 0; }
 
+xrh_status xrh_volume_info(const xrh_volume* vol, int32_t* storage, int32_t* layout)
+{
+  return guarded([&] {
+    XRH_REQUIRE(vol && storage && layout, "null argument");
+    *storage = vol->desc.storage;
+    *layout = vol->desc.layout;
+  });
+}
+
 xrh_status xrh_line_int(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* cams,
                         int32_t num_cams, const float* xforms_cam_to_phys,
                         const int64_t* cam_for_proj, int64_t num_projs, float step,

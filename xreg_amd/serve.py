@@ -80,8 +80,9 @@ def poses_from_params(init_pose: np.ndarray, center: np.ndarray, X: np.ndarray) 
 
 
 class _Active:
-    def __init__(self, spec: JobSpec, fixeds, views, start: float, extent_mm: float = 1.0):
+    def __init__(self, spec: JobSpec, fixeds, views, start: float, extent_mm: float = 1.0, layout_key=(0, 0)):
         self.spec = spec
+        self.layout_key = layout_key   # (storage, layout) of the job's volume
         self.fixeds = fixeds
         self.views = views
         self.start = start
@@ -120,7 +121,8 @@ class RegistrationServer:
         admission order. "srpt": least estimated remaining work first (generations left x
         samples per generation; ties by arrival). Jobs are packed in that order while the
         submission's estimated samples stay within step_samples (None: no limit; the first job
-        always goes) and its items within max_items. With a limit, a long job runs alone at full
+        always goes) and its items within max_items; one submission holds only jobs whose
+        volumes share the first job's storage type and layout. With a limit, a long job runs alone at full
         speed; "srpt" then serves short jobs ahead of long ones (lower median and mean latency),
         "fifo" keeps arrival order (the smallest maximum latency)."""
         if policy not in ("fifo", "srpt"):
@@ -171,7 +173,8 @@ class RegistrationServer:
             if e.status == X.XRH_ERR_BUDGET:
                 return None
             raise
-        return _Active(spec, fixeds, views, now, float(getattr(vol, "extent_mm", 1.0)))
+        return _Active(spec, fixeds, views, now, float(getattr(vol, "extent_mm", 1.0)),
+                       (getattr(vol, "storage", 0), getattr(vol, "layout", 0)))
 
     def run(self, jobs: list[JobSpec], realtime: bool = True, on_finish=None) -> list[JobResult]:
         """Serves the jobs (sorted by arrival). With realtime=True arrivals follow the wall clock;
@@ -211,6 +214,8 @@ class RegistrationServer:
                 k = a.spec.lam * len(a.views)
                 if packed and (n_items + k > self.max_items or (budget is not None and work + a.gen_work > budget)):
                     continue
+                if packed and a.layout_key != packed[0].layout_key:
+                    continue  # one submission casts volumes of one storage type and layout
                 packed.append(a)
                 n_items += k
                 work += a.gen_work
