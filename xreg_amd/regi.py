@@ -97,6 +97,90 @@ class Objective:
             f.close()
 
 
+class MultiParam:
+    """Several objects' parameterisations stacked into one optimiser vector (object o owns the
+    next params[o].num_params entries), as Intensity2D3DRegi concatenates the per-object
+    SE3OptVars. Called on one vector it returns the list of poses; on k vectors a list of
+    (k, 4, 4) arrays, one per object."""
+
+    def __init__(self, params):
+        self.params = list(params)
+        self.sizes = [p.num_params for p in self.params]
+        self.num_params = int(sum(self.sizes))
+
+    def __call__(self, Xp: np.ndarray):
+        Xp = np.asarray(Xp, dtype=np.float64)
+        out, o = [], 0
+        for p, n in zip(self.params, self.sizes):
+            out.append(p(Xp[..., o:o + n]))
+            o += n
+        return out
+
+
+class MultiObjective:
+    """Intensity2D3DRegi::obj_fn with several objects (xregIntensity2D3DRegi.cpp:571-696): per
+    candidate, every object's DRR is cast into the same projections (view-major, one per view
+    and candidate), the first object with the REPLACE store -- on top of the static volumes'
+    background images when `bg` is given (has_a_static_vol_, :593-625) -- and the others with
+    ACCUM; then each view is scored and the views are averaged (ImgSimMetric2DCombineMean).
+    All cameras share one detector size, as RayCasterCPU requires (xregRayCastBaseCPU.cpp:
+    54-70). Each object is one xrh_line_int launch over all views and candidates."""
+
+    def __init__(self, ctx: X.Context, vols, views, params, step: float,
+                 metric: int = X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize: int = 5, patch_radius: int = 5,
+                 max_batch: int = 64, geom: int = X.XRH_GEOM_FP32, bg=None):
+        if len(vols) != len(params) or not vols:
+            raise ValueError("one parameterisation per object")
+        self.ctx, self.vols, self.step, self.geom = ctx, list(vols), float(step), int(geom)
+        self.param = MultiParam(params)
+        self.cams = [c for c, _ in views]
+        self.V = len(views)
+        rows, cols = self.cams[0].num_det_rows, self.cams[0].num_det_cols
+        if any((c.num_det_rows, c.num_det_cols) != (rows, cols) for c in self.cams):
+            raise ValueError("all views must share one detector size")
+        self.npx = rows * cols
+        self.fixed = [X.Fixed(ctx, img, metric, smooth_ksize=smooth_ksize, patch_radius=patch_radius)
+                      for _, img in views]
+        self.max_batch = int(max_batch)
+        self.drr = ctx.alloc(4 * self.V * self.max_batch * self.npx)
+        self.bg = None
+        if bg is not None:
+            b = np.ascontiguousarray(bg, dtype=np.float32).reshape(self.V, rows, cols)
+            self.bg = ctx.alloc(b.nbytes)
+            self.bg.upload(b)
+        self.evals = 0
+        self.calls = 0
+
+    def eval_poses(self, poses_per_obj) -> np.ndarray:
+        poses_per_obj = [np.asarray(p, dtype=np.float64).reshape(-1, 4, 4) for p in poses_per_obj]
+        n = len(poses_per_obj[0])
+        out = np.empty(n, dtype=np.float32)
+        for s in range(0, n, self.max_batch):
+            k = min(self.max_batch, n - s)
+            c4p = np.repeat(np.arange(self.V, dtype=np.int64), k)
+            for o, vol in enumerate(self.vols):
+                xf = np.tile(poses_per_obj[o][s:s + k], (self.V, 1, 1)).astype(np.float32)
+                X.line_int(self.ctx, vol, self.cams, xf, c4p, self.step, self.drr,
+                           store=X.XRH_STORE_REPLACE if o == 0 else X.XRH_STORE_ACCUM,
+                           bg=self.bg if o == 0 else None, geom=self.geom)
+            sims = np.stack([self.fixed[v].compute(self.drr, k, offset_imgs=v * k) for v in range(self.V)])
+            out[s:s + k] = combine_mean(sims)
+            self.calls += 1
+        self.evals += n
+        return out
+
+    def __call__(self, Xp: np.ndarray) -> np.ndarray:
+        Xp = np.atleast_2d(np.asarray(Xp, dtype=np.float64))
+        return self.eval_poses(self.param(Xp))
+
+    def close(self):
+        self.drr.free()
+        if self.bg is not None:
+            self.bg.free()
+        for f in self.fixed:
+            f.close()
+
+
 # ------------------------------------------------------------------------------------------
 # drivers
 
