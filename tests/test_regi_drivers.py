@@ -127,3 +127,28 @@ def test_downsample_camera_follows_reference_arithmetic():
     e = G.downsample_camera(c, 0.5, force_even_dims=True)
     assert (e.num_det_rows, e.num_det_cols) == (188, 192)
     assert G.lround(2.5) == 3 and G.lround(-2.5) == -3 and G.lround(2.4999) == 2
+
+
+def test_multi_param_stacks_object_parameters():
+    """MultiParam: object o reads its own slice of the optimiser vector; zeros give each
+    object's starting pose; one vector gives a list of poses, k vectors a list of (k, 4, 4)."""
+    from xreg_amd import geometry as G
+    from xreg_amd import regi
+
+    c = np.array([10.0, 20.0, 30.0])
+    a0 = G.perturbed_pose(np.eye(4), c, np.deg2rad([5.0, 0, 0]), [1.0, 2.0, 3.0])
+    b0 = G.perturbed_pose(np.eye(4), c, np.deg2rad([0, -7.0, 0]), [-4.0, 0.0, 2.0])
+    mp = regi.MultiParam([regi.PoseParam(a0, c), regi.PoseParam(b0, c)])
+    assert mp.num_params == 12
+    pa, pb = mp(np.zeros(12))
+    np.testing.assert_allclose(pa, a0, atol=1e-12)
+    np.testing.assert_allclose(pb, b0, atol=1e-12)
+    x = np.zeros((3, 12))
+    x[1, 3] = 5.0     # object A, x translation
+    x[2, 9] = -2.0    # object B, x translation
+    A, B = mp(x)
+    assert A.shape == (3, 4, 4) and B.shape == (3, 4, 4)
+    np.testing.assert_allclose(A[2], a0, atol=1e-12)   # B's parameters leave A alone
+    np.testing.assert_allclose(B[1], b0, atol=1e-12)
+    np.testing.assert_allclose(A[1], regi.PoseParam(a0, c)(x[1, :6]), atol=1e-12)
+    np.testing.assert_allclose(B[2], regi.PoseParam(b0, c)(x[2, 6:]), atol=1e-12)
