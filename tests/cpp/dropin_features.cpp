@@ -120,6 +120,18 @@ int main(int argc, char** argv)
     rc->set_num_projs(np);
     rc->allocate_resources();
     rec.put_u64("max_num_projs_possible", rc->max_num_projs_possible());
+    // interface methods kept as the reference's base class has them
+    uint64_t iface_bad = 0;
+    try {
+      rc->to_host_buf();
+      ++iface_bad;
+    } catch (const RayCaster::UnsupportedOperationException&) {
+    }
+    try {
+      rc->to_ocl_buf();
+      ++iface_bad;
+    } catch (const RayCaster::UnsupportedOperationException&) {
+    }
 
     // maximum-intensity projections of object A
     rc->set_kernel_id(kRAY_CAST_LINE_INT_MAX_KERNEL);
@@ -150,6 +162,9 @@ int main(int argc, char** argv)
     dev_sim->allocate_resources();
     dev_sim->compute();
     rec.put("dev_sims", dev_sim->sim_vals());
+    dev_sim->set_save_aux_info(true);
+    iface_bad += (dev_sim->aux_info() != nullptr);
+    rec.put_u64("iface_bad", iface_bad);
     auto host_sim = make_metric(metric, smooth);
     host_sim->set_fixed_image(fixed);
     host_sim->set_num_moving_images(np);

@@ -100,4 +100,5 @@ def test_dropin_features(tmp_path, oracle, metric, smooth):
     assert np.abs(f32("static_sims") - oracle.sim_f32(metric, fixed, ref_st, smooth_ksize=smooth)).max() \
         <= SIM_ABS_TOL
     assert u64("bg_restored") == 1
+    assert u64("iface_bad") == 0  # to_host_buf / to_ocl_buf throw, aux_info is empty
     assert u64("max_num_projs_possible") >= npose

@@ -102,6 +102,12 @@ void check_xrh(xrh_status s);  // non-zero status -> exception (Unsupported / St
 
 // ------------------------------------------------------------------------------------------
 
+// Sync-buffer and HDF5 types of xReg, used here only as return types of interface methods the
+// HIP classes do not implement (the device buffer hand-off replaces the sync objects).
+class RayCastSyncOCLBuf;
+class RayCastSyncHostBuf;
+class H5ReadWriteInterface;
+
 class RayCaster {
  public:
   using PixelScalar2D = float;
@@ -167,6 +173,10 @@ class RayCaster {
   // Device buffer hand-off for HIP similarity metrics (the RayCastSyncOCLBuf role,
   // lib/ray_cast/xregRayCastSyncBuf.h:35-192): the DRRs never leave HBM.
   virtual const float* device_pixel_buf() const { throw UnsupportedOperationException(); }
+  // xregRayCastInterface.h:324-328 (the base throws, xregRayCastInterface.cpp:299-307); the HIP
+  // caster keeps that behaviour: metrics bind to device_pixel_buf() instead
+  virtual RayCastSyncOCLBuf* to_ocl_buf() { throw UnsupportedOperationException(); }
+  virtual RayCastSyncHostBuf* to_host_buf() { throw UnsupportedOperationException(); }
 
  protected:
   virtual void vols_changed() {}
@@ -284,6 +294,9 @@ class ImgSimMetric2D {
   void set_mask(ImageMaskPtr m);
   ImageMaskPtr mask() { return mask_; }
   void set_save_aux_info(bool s) { save_aux_info_ = s; }
+  // ImgSimMetric2D::aux_info (xregImgSimMetric2D.cpp:99-102): none by default; the HIP metrics
+  // keep no per-patch debug record (the CPU patch metrics' override)
+  virtual std::shared_ptr<H5ReadWriteInterface> aux_info() { return nullptr; }
 
  protected:
   ImagePtr fixed_img_;
