@@ -128,14 +128,12 @@ def profiled_traffic(key: str):
     """HBM-side traffic per DRR launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated
     against a plain copy) for this exact configuration, from the committed profile summary; None
     when the configuration was not profiled."""
-    p = ROOT / "profiles" / "drr_kernel_c2_lambda100.md"
-    if not p.exists():
-        return None
-    for line in p.read_text().splitlines():
-        if line.startswith("traffic: ") and f" {key} " in line + " ":
-            for tok in line.split():
-                if tok.startswith("bytes_per_launch="):
-                    return float(tok.split("=", 1)[1])
+    for p in sorted((ROOT / "profiles").glob("drr_kernel_*.md")):
+        for line in p.read_text().splitlines():
+            if line.startswith("traffic: ") and f" {key} " in line + " ":
+                for tok in line.split():
+                    if tok.startswith("bytes_per_launch="):
+                        return float(tok.split("=", 1)[1])
     return None
 
 
