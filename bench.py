@@ -393,6 +393,23 @@ def main():
         ctx.synchronize()
         plain_value = n_plain / (time.perf_counter() - t0)
 
+    # ---- drop-in call pattern: what Intensity2D3DRegi gets through RayCasterLineIntHIP and
+    # ImgSimMetric2D*HIP -- one caster compute() over all lambda projections (xrh_line_int),
+    # then one metric compute() (xrh_sim_compute) per generation, no graph replay
+    dbuf = ctx.alloc(4 * P * lam)
+    zeros = np.zeros(lam, np.int64)
+    for g in range(args.warmup):
+        X.line_int(ctx, vol, w.cams, gens[g].astype(np.float32), zeros, w.step, dbuf, geom=geom)
+        fx.compute(dbuf, lam)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for g in range(args.warmup, args.warmup + args.steps):
+        X.line_int(ctx, vol, w.cams, gens[g].astype(np.float32), zeros, w.step, dbuf, geom=geom)
+        fx.compute(dbuf, lam)
+    ctx.synchronize()
+    dropin_value = args.steps * lam / (time.perf_counter() - t0)
+    dbuf.free()
+
     # STREAM-style copy probe with the library's copy kernel (xrh_copy_kernel): measured HBM
     # bandwidth next to the published peak; under --pmc it is also the counters' calibration
     # (a dispatch that reads and writes exactly copy_mib MiB)
@@ -445,6 +462,7 @@ def main():
                    "storage": args.storage, "parallelism": "single-gpu"},
         "plain_poses_per_s": round(plain_value, 2) if plain_value else None,
         "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
+        "dropin_poses_per_s": round(dropin_value, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(f"workload={workload} storage={args.storage} "

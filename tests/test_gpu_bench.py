@@ -28,3 +28,4 @@ def test_bench_json_line():
     assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3
     assert d["served"]["jobs"] == 1 and d["c3_registration"]["levels"][0]["evals"] > 0
     assert d["autotune"]["variants_timed"] == 20
+    assert d["dropin_poses_per_s"] > 0
