@@ -322,6 +322,15 @@ xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view,
                                   const double* item_pose, int64_t n, uint64_t* samples_out,
                                   uint64_t* evaluated_out);
 
+/* Per-ray step counts of a batch: for every item (in item order, rows * cols int32 per item,
+ * laid out like drr_dev_out of xrh_batch_eval) the last sample index n that
+ * RayCasterLineIntCPU's floor(chord / step) gives (xregRayCastLineIntCPU.cpp:253-254) under
+ * the batch's geometry mode, -1 where the ray misses the volume. Lets a parity check tell a
+ * pixel that differs from the CPU reference by a whole sample (a step-count flip) from one
+ * whose samples differ. Synchronous. */
+xrh_status xrh_batch_ray_steps(xrh_batch* batch, const int32_t* item_view, const double* item_pose,
+                               int64_t n, int32_t* steps_dev_out);
+
 /* ---------------------------------------------------------------------------------- */
 /* multi-resolution: DownsampleImage (lib/itk/xregITKResampleUtils.h:51-120, 181-186) as
  * DownsampleProjData applies it to fixed images (lib/image/xregProjData.cpp:40-50): ITK 5.1's

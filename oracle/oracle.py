@@ -145,6 +145,30 @@ def drr_f64(vol_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, ker
     return out, int(s.value)
 
 
+def ray_steps(vol_shape_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, fp64=False, threads=0):
+    """Per-ray last sample index n (-1: miss) of drr_f32 (or drr_f64 with fp64=True): the ray
+    set-up alone, [n_proj, rows, cols] int32."""
+    dims = np.array([vol_shape_zyx[2], vol_shape_zyx[1], vol_shape_zyx[0]], dtype=np.int64)
+    dt = np.float64 if fp64 else np.float32
+    x = np.ascontiguousarray(np.asarray(xforms, dtype=dt).reshape(-1, 16))
+    n = x.shape[0]
+    c4p = np.zeros(n, dtype=np.int64) if cam_for_proj is None else np.ascontiguousarray(cam_for_proj, np.int64)
+    rows, cols = cams[0].num_det_rows, cams[0].num_det_cols
+    out = np.empty((n, rows, cols), dtype=np.int32)
+    op = out.ctypes.data_as(C.POINTER(C.c_int32))
+    if fp64:
+        ca = (OrcCamD * len(cams))(*[cam_d(c) for c in cams])
+        p2i = p2i_f64(np.asarray(idx_to_phys, dtype=np.float64))
+        lib().orc_ray_steps_f64(_i64(dims), _d(p2i), ca, _d(x), _i64(c4p), C.c_int64(n), C.c_double(step), op,
+                                threads)
+    else:
+        ca = (OrcCamF * len(cams))(*[cam_f(c) for c in cams])
+        p2i = p2i_f32(idx_to_phys)
+        lib().orc_ray_steps_f32(_i64(dims), _f(p2i), ca, _f(x), _i64(c4p), C.c_int64(n), C.c_float(step), op,
+                                threads)
+    return out
+
+
 def ncc_f32(fixed, mov, mask=None):
     f = np.ascontiguousarray(fixed, dtype=np.float32)
     m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, f.size)

@@ -440,6 +440,8 @@ typedef struct {
   int32_t cols0;
   float* out;
   int64_t samples;
+  int32_t* steps;     /* optional: per ray, the last sample index n (-1: the ray misses) */
+  int32_t steps_only; /* set-up only: write steps, take no samples, leave out untouched */
 } drr_f32_ctx;
 
 static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
@@ -470,6 +472,7 @@ static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
     float acc = (c->kernel == 0) ? 0.0f : -FLT_MAX;                           /* :221 */
     float t0 = 0.0f, t1 = 0.0f;
     const int32_t hit = orc_ray_rect_f32(aabb_min, c->aabb_max, ph, d, 1, &t0, &t1); /* :227 */
+    if (c->steps) c->steps[r] = -1;
     if (hit && ((t1 - t0) > (float)(2.0f * kTol))) {                            /* :233 */
       t0 += kTol;
       t1 -= kTol;
@@ -493,6 +496,8 @@ static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
       linear_vec_f32(M, v, sv);
       const float s = sqrtf((sv[0] * sv[0] + sv[1] * sv[1]) + sv[2] * sv[2]);
       const int64_t n = (int64_t)(L / s);                                      /* :253-254 */
+      if (c->steps) c->steps[r] = (int32_t)n;
+      if (c->steps_only) continue;
       const float scale = s / dlen;                                            /* :263-264 */
       const float dx = d[0] * scale, dy = d[1] * scale, dz = d[2] * scale;
       float p[3] = {start[0], start[1], start[2]};
@@ -510,6 +515,7 @@ static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
       samples += n + 1;
       acc = acc * c->step;                                                     /* :279 */
     }
+    if (c->steps_only) continue;
     /* aa_sum = acc * 1 (:282), store with the same kernel (:285) */
     const float aa = 0.0f + acc * 1.0f;
     if (c->kernel == 0) {
@@ -582,6 +588,8 @@ typedef struct {
   int32_t cols0;
   double* out;
   int64_t samples;
+  int32_t* steps;     /* as in drr_f32_ctx */
+  int32_t steps_only;
 } drr_f64_ctx;
 
 static void drr_f64_range(void* vctx, int64_t begin, int64_t end)
@@ -609,6 +617,7 @@ static void drr_f64_range(void* vctx, int64_t begin, int64_t end)
     double acc = (c->kernel == 0) ? 0.0 : -DBL_MAX;
     double t0 = 0.0, t1 = 0.0;
     const int32_t hit = orc_ray_rect_f64(aabb_min, c->aabb_max, ph, d, 1, &t0, &t1);
+    if (c->steps) c->steps[r] = -1;
     if (hit && ((t1 - t0) > 2.0 * kTol)) {
       t0 += kTol;
       t1 -= kTol;
@@ -629,6 +638,8 @@ static void drr_f64_range(void* vctx, int64_t begin, int64_t end)
       for (int i = 0; i < 3; ++i) sv[i] = M[4 * i] * v[0] + M[4 * i + 1] * v[1] + M[4 * i + 2] * v[2];
       const double s = sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]);
       const int64_t n = (int64_t)(L / s);
+      if (c->steps) c->steps[r] = (int32_t)n;
+      if (c->steps_only) continue;
       const double scale = s / dlen;
       const double dx = d[0] * scale, dy = d[1] * scale, dz = d[2] * scale;
       for (int64_t i = 0; i <= n; ++i) {
@@ -645,6 +656,7 @@ static void drr_f64_range(void* vctx, int64_t begin, int64_t end)
       samples += n + 1;
       acc *= c->step;
     }
+    if (c->steps_only) continue;
     if (c->kernel == 0) {
       c->out[r] += acc;
     } else if (c->out[r] < acc) {
@@ -687,6 +699,53 @@ void orc_drr_f64(const double* vol, const int64_t dims[3], const double p2i[16],
   }
   orc_parallel_for(drr_f64_range, &c, num_projs * c.npx, 2048, threads);
   if (samples_out) *samples_out = c.samples;
+}
+
+/* Per-ray step counts: the ray set-up of orc_drr_f32 / orc_drr_f64 alone (no samples), writing
+ * every ray's last sample index n = floor(chord / step_len) (xregRayCastLineIntCPU.cpp:253-254),
+ * -1 where the ray misses, to steps_out[proj][row][col]. */
+void orc_ray_steps_f32(const int64_t dims[3], const float p2i[16], const orc_cam_f* cams,
+                       const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                       float step, int32_t* steps_out, int32_t threads)
+{
+  drr_f32_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.dims = dims;
+  c.aabb_max[0] = (float)(dims[0] - 1);
+  c.aabb_max[1] = (float)(dims[1] - 1);
+  c.aabb_max[2] = (float)(dims[2] - 1);
+  c.p2i = p2i;
+  c.cams = cams;
+  c.xforms = xforms;
+  c.cam_for_proj = cam_for_proj;
+  c.step = step;
+  c.cols0 = cams[0].cols;
+  c.npx = (int64_t)cams[0].rows * cams[0].cols;
+  c.steps = steps_out;
+  c.steps_only = 1;
+  orc_parallel_for(drr_f32_range, &c, num_projs * c.npx, 2048, threads);
+}
+
+void orc_ray_steps_f64(const int64_t dims[3], const double p2i[16], const orc_cam_d* cams,
+                       const double* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                       double step, int32_t* steps_out, int32_t threads)
+{
+  drr_f64_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.dims = dims;
+  c.aabb_max[0] = (double)(dims[0] - 1);
+  c.aabb_max[1] = (double)(dims[1] - 1);
+  c.aabb_max[2] = (double)(dims[2] - 1);
+  c.p2i = p2i;
+  c.cams = cams;
+  c.xforms = xforms;
+  c.cam_for_proj = cam_for_proj;
+  c.step = step;
+  c.cols0 = cams[0].cols;
+  c.npx = (int64_t)cams[0].rows * cams[0].cols;
+  c.steps = steps_out;
+  c.steps_only = 1;
+  orc_parallel_for(drr_f64_range, &c, num_projs * c.npx, 2048, threads);
 }
 
 /* ------------------------------------------------------------------------------------ */

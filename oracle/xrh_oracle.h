@@ -88,6 +88,14 @@ void orc_drr_f64(const double* vol, const int64_t dims[3], const double p2i[16],
                  double step, int32_t kernel, int32_t store, const double* bg, double default_bg,
                  double* out, int32_t threads, int64_t* samples_out);
 
+/* Per-ray last sample index n (-1: miss) of orc_drr_f32 / orc_drr_f64, set-up only. */
+void orc_ray_steps_f32(const int64_t dims[3], const float p2i[16], const orc_cam_f* cams,
+                       const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                       float step, int32_t* steps_out, int32_t threads);
+void orc_ray_steps_f64(const int64_t dims[3], const double p2i[16], const orc_cam_d* cams,
+                       const double* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                       double step, int32_t* steps_out, int32_t threads);
+
 /* ---- similarity metrics (values are xReg's: smaller is better) ---- */
 /* NCC: returns (1 - ncc)/2 per moving image. mov is NOT modified (the reference zero-means
  * it in place; callers that need that side effect use orc_zero_mean_inplace_f32). */

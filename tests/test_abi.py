@@ -45,3 +45,27 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(X.Camera) == 4 * (4 + 9 + 16 + 3 + 2)
     assert ctypes.sizeof(X.SimParams) == 16
     assert X.View.cam.offset == 16
+
+
+def test_volume_slice_past_24bit_index_reach_is_refused_without_gpu():
+    """The DRR kernels multiply indices by the padded pitches in signed 24-bit arithmetic, so a
+    volume whose padded slice (X+2)*(Y+2) reaches 2^23 voxels is refused (XRH_ERR_UNSUPPORTED)
+    before any device work; the argument check runs ahead of the context check, so no GPU is
+    needed to see it."""
+    import numpy as np
+
+    from xreg_amd import _xrh as X
+
+    lib = X.lib()
+    vox = np.zeros(4, np.float32)
+    i2p = np.eye(4, dtype=np.float32).reshape(16)
+    out = ctypes.c_void_p()
+    for size, want in (((2900, 2900, 20), X.XRH_ERR_UNSUPPORTED), ((2894, 2894, 20), X.XRH_ERR_INVALID),
+                       ((512, 512, 400), X.XRH_ERR_INVALID)):
+        sz = np.array(size, np.int64)
+        st = lib.xrh_volume_create(None, X._fp(vox), sz.ctypes.data_as(X._PI64), X._fp(i2p), X.XRH_F32,
+                                   ctypes.byref(out))
+        msg = lib.xrh_last_error().decode()
+        assert st == want, (size, st, msg)
+        if want == X.XRH_ERR_INVALID:
+            assert "null context" in msg

@@ -215,3 +215,28 @@ def test_planted_mistake_is_caught(oracle):
     n = np.floor((t1 - t0) * np.linalg.norm(d, axis=-1) / (1.0 / sp))
     wrong = np.where(hit, 0.02 * (n + 2), 0.0)  # one sample too many
     assert not np.allclose(drr[0][hit], wrong[hit], rtol=1e-5)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_ray_steps_match_the_drr_of_a_unit_volume(oracle, fp64):
+    """oracle.ray_steps (set-up only) against what the DRR itself took: in a volume of ones every
+    sample is exactly 1, so a pixel is step * (n + 1) and recovers the ray's n; a miss is 0 and
+    -1. The per-ray counts also add up to the DRR's own sample total."""
+    from xreg_amd import synth
+
+    X, Y, Z, sp, cam, gt, center = _geometry()
+    vol = np.ones((Z, Y, X), np.float32)
+    poses = np.concatenate([gt[None], synth.random_poses(gt, center, 4, 25.0, 30.0, seed=17)])
+    step = 0.75
+    i2p = np.diag([sp, sp, sp, 1.0])
+    if fp64:
+        pf = poses_f32(poses).astype(np.float64)
+        drr, total = oracle.drr_f64(vol, i2p, [cam], pf, step=step)
+    else:
+        pf = poses_f32(poses)
+        drr, total = oracle.drr_f32(vol, i2p, [cam], pf, step=step)
+    steps = oracle.ray_steps(vol.shape, i2p, [cam], pf, step=step, fp64=fp64)
+    n_from_drr = np.rint(np.asarray(drr, np.float64) / step).astype(np.int64) - 1
+    assert (steps == -1).any() and (steps > 10).any()
+    assert np.array_equal(steps, n_from_drr)
+    assert int((steps + 1).sum()) == total

@@ -144,11 +144,12 @@ def test_srpt_packing_runs_short_jobs_first_within_the_step_budget():
     FakeBatch.steps = []
     res = {r.job_id: r for r in srv.run(specs, realtime=False)}
     short_items = sum(s.lam for s in specs[1:])
-    # the short jobs share submissions until they finish; the long job comes after them
-    assert FakeBatch.steps[0] == short_items
+    # the short jobs go first; the long job fills what is left of each step's budget (7 of its
+    # 40 items), so its generations spread over several steps until the short jobs are done
+    assert FakeBatch.steps[0] == short_items + (specs[0].lam - short_items)
     assert FakeBatch.steps[-1] == specs[0].lam
-    assert 40 not in FakeBatch.steps[:max(res[j].generations for j in (1, 2, 3))]
     assert max(res[j].finish for j in (1, 2, 3)) <= res[0].finish
+    assert srv.max_items_in_step == specs[0].lam
     for s in specs:
         alone = srv.run([s], realtime=False)[0]
         assert alone.history == res[s.job_id].history
@@ -228,12 +229,74 @@ def test_deadline_finishes_jobs_with_their_current_estimate():
     assert ctx.used == 0
 
 
-def test_oversized_job_is_rejected():
+def test_generation_larger_than_a_submission_is_split_with_the_same_results():
+    """lambda = 20 with 2 views through submissions of at most 8 items: each generation spans
+    five steps, and the run equals the one with room for whole generations."""
     ctx = FakeCtx()
-    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=8, max_views=8, max_det=64,
+    spec = _specs(1, lam=20, views=2, gens=5)[0]
+    small = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=8, max_views=8, max_det=64,
+                                     device=FAKE)
+    FakeBatch.steps = []
+    a = small.run([spec], realtime=False)[0]
+    assert max(FakeBatch.steps) == 8 and len(FakeBatch.steps) == 5 * a.generations
+    big = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=64, max_views=8, max_det=64,
                                    device=FAKE)
-    with pytest.raises(ValueError):
-        srv.run(_specs(1, lam=20), realtime=False)
+    b = big.run([spec], realtime=False)[0]
+    assert a.history == b.history and a.generations == b.generations
+
+
+def test_step_budget_splits_a_large_generation_and_short_jobs_wait_one_step():
+    """A large job is in service when a short one arrives: with a step budget smaller than the
+    large generation, the short job's generations go into the next submission instead of
+    waiting for the large generation to end."""
+    ctx = FakeCtx()
+    specs = _specs(2, lam=10, gens=4)
+    specs[0].lam, specs[0].max_gens = 60, 4          # 60 items of 48^2 rays per generation
+    per_item = 48 * 48
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=1000, max_views=8, max_det=64,
+                                   device=FAKE, policy="srpt", step_samples=float(15 * per_item))
+    FakeBatch.steps = []
+    res = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    assert max(FakeBatch.steps) == 15
+    # the short job (lambda 11) finishes within its first generations' steps, well before the
+    # large job's 4 x 60 items are through
+    assert res[1].finish < res[0].finish
+    for sp in specs:
+        assert srv.run([sp], realtime=False)[0].history == res[sp.job_id].history
+
+
+def test_job_with_too_many_views_is_rejected_without_blocking():
+    ctx = FakeCtx()
+    specs = _specs(3, views=1, gens=3)
+    specs[1].cams = specs[1].cams * 9
+    specs[1].fixed = specs[1].fixed * 9
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=64, max_views=8, max_det=64,
+                                   device=FAKE)
+    res = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    assert res[1].status == "rejected" and "views" in res[1].reason
+    assert res[0].generations == 3 and res[2].generations == 3
+
+
+def test_job_whose_fixed_images_exceed_the_budget_is_rejected_without_blocking():
+    ctx = FakeCtx()
+    specs = _specs(3, views=1, gens=3)
+    specs[1].fixed = [np.zeros((2048, 2048), np.float32)]   # 96 MiB of fixed-image buffers
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=16 << 20, max_items=64, max_views=8, max_det=64,
+                                   device=FAKE)
+    res = {r.job_id: r for r in srv.run(specs, realtime=True)}
+    assert res[1].status == "rejected" and "budget" in res[1].reason
+    assert res[0].generations == 3 and res[2].generations == 3
+    assert ctx.used == 0
+
+
+def test_job_with_too_large_a_detector_is_rejected():
+    ctx = FakeCtx()
+    specs = _specs(2, gens=2)
+    specs[0].cams = [G.camera(1020.0, 96, 96, 1.5, 1.5)]
+    srv = serve.RegistrationServer(ctx, {"v": None}, hbm_budget=1 << 40, max_items=64, max_views=8, max_det=64,
+                                   device=FAKE)
+    res = {r.job_id: r for r in srv.run(specs, realtime=False)}
+    assert res[0].status == "rejected" and res[1].generations == 2
 
 
 def test_exp_se3_matches_matrix_exponential():

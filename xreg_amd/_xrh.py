@@ -136,6 +136,7 @@ SIGNATURES = {
     "xrh_batch_wait": (_I32, [_P, _PF]),
     "xrh_batch_last_timing": (_I32, [_P, _PF, _PF]),
     "xrh_batch_sample_count": (_I32, [_P, _PI32, _PD, _I64, _PU64, _PU64]),
+    "xrh_batch_ray_steps": (_I32, [_P, _PI32, _PD, _I64, _P]),
 }
 
 
@@ -425,6 +426,13 @@ class Batch:
         check(lib().xrh_batch_sample_count(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0],
                                            C.byref(out), C.byref(ev)))
         return (int(out.value), int(ev.value)) if evaluated else int(out.value)
+
+    def ray_steps(self, item_view: np.ndarray, item_pose: np.ndarray, out: DeviceBuffer) -> None:
+        """Every ray's last sample index n (-1: miss) into `out` (int32, item order, rows * cols per
+        item), under the batch's geometry mode (xrh_batch_ray_steps)."""
+        iv = np.ascontiguousarray(item_view, dtype=np.int32)
+        ip = np.ascontiguousarray(item_pose, dtype=np.float64).reshape(-1, 16)
+        check(lib().xrh_batch_ray_steps(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0], out.ptr))
 
     def close(self) -> None:
         if self.h:

@@ -198,6 +198,10 @@ void RayCaster::distribute_xforms_among_cam_models(const FrameTransformList& xfo
 {
   const size_type n = xforms.size(), nc = num_camera_models();
   if (n * nc != num_projs()) throw StringMessageException("xforms * cameras != num_projs");
+  // set_num_projs may have grown the count after allocate_resources (it checks the previous
+  // count, as the reference's assert does): keep the pose table in bounds here and let
+  // compute() report that the projection buffer is too small
+  if (xforms_cam_to_itk_phys_.size() < num_projs()) xforms_cam_to_itk_phys_.resize(num_projs());
   size_type g = 0;
   for (size_type c = 0; c < nc; ++c) {
     for (size_type p = 0; p < n; ++p, ++g) {

@@ -511,6 +511,35 @@ void autotune(xrh_batch* b, const int32_t* item_view, const double* item_pose, i
   out->variants_timed = tried;
 }
 
+// The sample-counting pass of the DRR kernel over one submission: totals of the reference
+// algorithm's samples and of those evaluated after skipping, and optionally every ray's last
+// sample index written to steps_dev (item order, rows * cols per item, like the DRRs).
+void count_samples(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n,
+                   uint64_t* samples_out, uint64_t* evaluated_out, int32_t* steps_dev)
+{
+  XRH_REQUIRE(b && item_view && item_pose, "null argument");
+  XRH_REQUIRE(!b->pending, "batch has a launch in flight");
+  XRH_REQUIRE(n >= 1 && n <= b->max_items, "item count must be in [1, max_items]");
+  hipStream_t s = b->ctx->stream;
+  XRH_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<std::pair<int, std::vector<int64_t>>> groups;
+  // staged as if writing DRRs to steps_dev, so every item's out_off indexes that buffer
+  DrrLaunch D = stage(b, item_view, item_pose, n, (float*)steps_dev, groups);
+  const BlobLayout L = blob_layout(b->max_items);
+  XRH_HIP_CHECK(hipMemcpyAsync(b->blob_dev, b->blob_host, L.off_sim, hipMemcpyHostToDevice, s));
+  uint64_t* cnt = (uint64_t*)ctx_scratch(b->ctx, 16);
+  uint64_t both[2] = {0, 0};
+  XRH_HIP_CHECK(hipMemcpyAsync(cnt, both, 16, hipMemcpyHostToDevice, s));
+  D.sample_counter = cnt;
+  D.ray_steps = steps_dev;
+  launch_drr_setup(D, s);
+  launch_drr(D, s);
+  XRH_HIP_CHECK(hipMemcpyAsync(both, cnt, 16, hipMemcpyDeviceToHost, s));
+  XRH_HIP_CHECK(hipStreamSynchronize(s));
+  if (samples_out) *samples_out = both[0];
+  if (evaluated_out) *evaluated_out = both[1];
+}
+
 }  // namespace
 
 extern "C" {
@@ -614,25 +643,17 @@ xrh_status xrh_batch_sample_count(xrh_batch* batch, const int32_t* item_view, co
                                   int64_t n, uint64_t* samples_out, uint64_t* evaluated_out)
 {
   return guarded([&] {
-    XRH_REQUIRE(batch && samples_out, "null argument");
-    XRH_REQUIRE(!batch->pending, "batch has a launch in flight");
-    XRH_REQUIRE(n >= 1 && n <= batch->max_items, "item count must be in [1, max_items]");
-    hipStream_t s = batch->ctx->stream;
-    XRH_HIP_CHECK(hipStreamSynchronize(s));
-    std::vector<std::pair<int, std::vector<int64_t>>> groups;
-    DrrLaunch D = stage(batch, item_view, item_pose, n, nullptr, groups);
-    const BlobLayout L = blob_layout(batch->max_items);
-    XRH_HIP_CHECK(hipMemcpyAsync(batch->blob_dev, batch->blob_host, L.off_sim, hipMemcpyHostToDevice, s));
-    uint64_t* cnt = (uint64_t*)ctx_scratch(batch->ctx, 16);
-    uint64_t both[2] = {0, 0};
-    XRH_HIP_CHECK(hipMemcpyAsync(cnt, both, 16, hipMemcpyHostToDevice, s));
-    D.sample_counter = cnt;
-    launch_drr_setup(D, s);
-    launch_drr(D, s);
-    XRH_HIP_CHECK(hipMemcpyAsync(both, cnt, 16, hipMemcpyDeviceToHost, s));
-    XRH_HIP_CHECK(hipStreamSynchronize(s));
-    *samples_out = both[0];
-    if (evaluated_out) *evaluated_out = both[1];
+    XRH_REQUIRE(samples_out, "null argument");
+    count_samples(batch, item_view, item_pose, n, samples_out, evaluated_out, nullptr);
+  });
+}
+
+xrh_status xrh_batch_ray_steps(xrh_batch* batch, const int32_t* item_view, const double* item_pose, int64_t n,
+                               int32_t* steps_dev_out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(steps_dev_out, "null argument");
+    count_samples(batch, item_view, item_pose, n, nullptr, nullptr, steps_dev_out);
   });
 }
 

@@ -708,7 +708,7 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
 template <int kGeom>
 __global__ __launch_bounds__(kThreads) void drr_count_kernel(
     const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
-    int bands, unsigned long long* __restrict__ counter)
+    int bands, unsigned long long* __restrict__ counter, int32_t* __restrict__ steps)
 {
   const TileRef tr = decode_tile(band_start, num_items, bands, items);
   if (!tr.valid) return;
@@ -718,6 +718,7 @@ __global__ __launch_bounds__(kThreads) void drr_count_kernel(
   double cnt = 0.0, evaluated = 0.0;
   if ((col < it.cam.cols) && (row < it.cam.rows)) {
     const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
+    if (steps) steps[it.out_off + (int64_t)row * it.cam.cols + col] = ray.n;
     cnt = (double)(ray.n + 1);
     if (ray.n >= 0) {
       int first, last;
@@ -844,11 +845,11 @@ void launch_drr(const DrrLaunch& L, hipStream_t s)
     if (L.geom == XRH_GEOM_FP32) {
       hipLaunchKernelGGL((drr_count_kernel<XRH_GEOM_FP32>), grid, dim3(kThreads), 0, s, L.items_dev,
                          L.band_start_dev, L.num_items, L.bands,
-                         (unsigned long long*)L.sample_counter);
+                         (unsigned long long*)L.sample_counter, L.ray_steps);
     } else {
       hipLaunchKernelGGL((drr_count_kernel<XRH_GEOM_FP64>), grid, dim3(kThreads), 0, s, L.items_dev,
                          L.band_start_dev, L.num_items, L.bands,
-                         (unsigned long long*)L.sample_counter);
+                         (unsigned long long*)L.sample_counter, L.ray_steps);
     }
     XRH_HIP_CHECK(hipGetLastError());
     return;

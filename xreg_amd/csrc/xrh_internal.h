@@ -244,6 +244,8 @@ struct DrrLaunch {
   int32_t ncc_tiles_per_item;
   uint64_t* sample_counter; // optional: count samples instead of writing pixels; [0] = all
                             // samples of the reference algorithm, [1] = samples evaluated
+  int32_t* ray_steps;       // optional with sample_counter: per pixel (at out_off), the last
+                            // sample index n of the reference algorithm (-1: the ray misses)
   int32_t reuse;            // DrrVariant::reuse
   int32_t wave_shape;       // DrrVariant::shape
 };

@@ -363,9 +363,15 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
                           const float idx_to_phys[16], int32_t storage, int32_t layout,
                           xrh_volume** out, const HuParams* hu = nullptr)
 {
-  XRH_REQUIRE(ctx && vox && size && idx_to_phys && out, "null argument");
+  XRH_REQUIRE(vox && size && idx_to_phys && out, "null argument");
   XRH_REQUIRE(storage == XRH_F32 || storage == XRH_F16 || storage == XRH_BF16, "bad storage type");
   for (int k = 0; k < 3; ++k) XRH_REQUIRE(size[k] >= 1 && size[k] < (1 << 20), "bad volume size");
+  // the DRR kernels form element indices with signed 24-bit multiply-adds (iz * pitch_z and
+  // iy * pitch_y), exact only while both padded pitches stay below 2^23 elements
+  if (!((size[0] + 2) * (size[1] + 2) < (int64_t(1) << 23))) {
+    fail(XRH_ERR_UNSUPPORTED, "volume slice too large: (size_x + 2) * (size_y + 2) must stay below 2^23 voxels");
+  }
+  XRH_REQUIRE(ctx, "null context");
   const bool auto_layout = layout < 0;
   if (auto_layout) layout = default_layout(storage);
   if (layout == kLayoutQuad || layout == kLayoutCoef || layout == kLayoutFace) {

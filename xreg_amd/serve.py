@@ -3,12 +3,13 @@
 Jobs (single- or multi-view intensity-based registrations with CMA-ES, the loop of
 Intensity2D3DRegiCMAES::run, lib/regi/interfaces_2d_3d/xregIntensity2D3DRegiCMAES.cpp:75-261)
 arrive over time. Volumes stay resident; each admitted job owns its fixed images and CMA-ES
-state. At every scheduling step the server packs the pending generation of every ready job
-(lambda poses x its views, any mix of volumes, detector sizes and lambda) into one batched
-submission (xrh_batch_set_views + xrh_batch_eval), then hands each job its lambda objective
-values (mean over its views, ImgSimMetric2DCombineMean) and advances its optimiser. Jobs are
-admitted while the context's HBM use stays under the budget; a job's results do not depend on
-what it was batched with (every item is evaluated independently with fixed-order
+state. At every scheduling step the server packs the pending items of admitted jobs (a job's
+generation is lambda poses x its views, any mix of volumes, detector sizes and lambda; a large
+generation may be spread over several steps) into one batched submission
+(xrh_batch_set_views + xrh_batch_eval); once a job's generation is complete it gets its lambda
+objective values (mean over its views, ImgSimMetric2DCombineMean) and its optimiser advances.
+Jobs are admitted while the context's HBM use stays under the budget; a job's results do not
+depend on what it was batched with (every item is evaluated independently with fixed-order
 reductions).
 """
 from __future__ import annotations
@@ -59,7 +60,9 @@ class JobResult:
     history: list = field(default_factory=list)
     params: np.ndarray | None = None   # final estimate: the CMA-ES mean
     pose: np.ndarray | None = None
-    status: str = "max_gens"           # "max_gens", "converged" (CMA-ES stop test) or "deadline"
+    status: str = "max_gens"           # "max_gens", "converged" (CMA-ES stop test), "deadline", or
+                                       # "rejected" (can never be admitted: see `reason`)
+    reason: str = ""
 
     @property
     def latency(self) -> float:
@@ -80,28 +83,55 @@ def poses_from_params(init_pose: np.ndarray, center: np.ndarray, X: np.ndarray) 
 
 
 class _Active:
+    """An admitted job: its fixed images and views on the device, its CMA-ES state, and how far
+    the current generation's items (view-major, lambda per view) have been evaluated."""
+
     def __init__(self, spec: JobSpec, fixeds, views, start: float, extent_mm: float = 1.0, layout_key=(0, 0)):
         self.spec = spec
         self.layout_key = layout_key   # (storage, layout) of the job's volume
         self.fixeds = fixeds
         self.views = views
         self.start = start
-        # trilinear samples of one generation, estimated: lambda x detector pixels of every view
-        # x (volume extent / ray step)
-        rays = spec.lam * sum(int(c.num_det_rows) * int(c.num_det_cols) for c in spec.cams)
-        self.gen_work = float(rays) * max(1.0, extent_mm / float(spec.step))
+        # trilinear samples per item, estimated: the view's detector pixels x (volume extent /
+        # ray step); a generation is lambda items per view
+        per_ray = max(1.0, extent_mm / float(spec.step))
+        self.view_work = [float(int(c.num_det_rows) * int(c.num_det_cols)) * per_ray for c in spec.cams]
+        self.gen_work = float(spec.lam) * sum(self.view_work)
+        self.n_items = spec.lam * len(spec.cams)
         self.es = CMAES(np.zeros(6), np.asarray(spec.sigma), spec.lam, spec.seed,
                         bounds=None if spec.bounds is None else np.asarray(spec.bounds))
         self.gen = 0
-        self.X = self.es.ask()
         self.best = (np.inf, None)
         self.status = "max_gens"
+        self._new_generation(self.es.ask())
+
+    def _new_generation(self, X):
+        self.X = X
+        self.poses = poses_from_params(self.spec.init_pose, self.spec.center, np.asarray(self.X))
+        self.cursor = 0                # next item of this generation to evaluate
+        self.sims = np.empty(self.n_items, np.float32)
 
     def items(self):
-        return poses_from_params(self.spec.init_pose, self.spec.center, np.asarray(self.X))
+        return self.poses
+
+    def item_work(self, i: int) -> float:
+        return self.view_work[i // self.spec.lam]
+
+    def chunk(self, budget: float | None, max_items: int, first: bool) -> int:
+        """How many of the generation's remaining items fit in `budget` estimated samples and
+        `max_items` (at least one when this job opens the submission)."""
+        k, work = 0, 0.0
+        while self.cursor + k < self.n_items and k < max_items:
+            w = self.item_work(self.cursor + k)
+            if budget is not None and work + w > budget and not (first and k == 0):
+                break
+            work += w
+            k += 1
+        return k
 
     def remaining_work(self) -> float:
-        return self.gen_work * (self.spec.max_gens - self.gen)
+        done = sum(self.item_work(i) for i in range(self.cursor)) if self.cursor else 0.0
+        return self.gen_work * (self.spec.max_gens - self.gen) - done
 
     def close(self):
         for f in self.fixeds:
@@ -117,14 +147,14 @@ class RegistrationServer:
         tests substitute a host-side fake to exercise the scheduler without a GPU).
         max_active: at most this many jobs in service at once (None: as many as fit; 1 serves
         the jobs one after another in arrival order, the serial baseline).
-        policy: the order in which admitted jobs' generations fill a submission. "fifo":
-        admission order. "srpt": least estimated remaining work first (generations left x
-        samples per generation; ties by arrival). Jobs are packed in that order while the
+        policy: the order in which admitted jobs' items fill a submission. "fifo": admission
+        order. "srpt": least estimated remaining work first (samples left in the job's
+        generations; ties by arrival). Jobs contribute items in that order while the
         submission's estimated samples stay within step_samples (None: no limit; the first job
-        always goes) and its items within max_items; one submission holds only jobs whose
-        volumes share the first job's storage type and layout. With a limit, a long job runs alone at full
-        speed; "srpt" then serves short jobs ahead of long ones (lower median and mean latency),
-        "fifo" keeps arrival order (the smallest maximum latency)."""
+        always gets at least one item) and its items within max_items; one submission holds
+        only jobs whose volumes share the first job's storage type and layout. "srpt" serves
+        short jobs ahead of long ones (lower median latency, and a large generation no longer
+        holds up jobs that arrive during it), "fifo" keeps arrival order."""
         if policy not in ("fifo", "srpt"):
             raise ValueError(f"unknown packing policy {policy!r}")
         self.dev = device if device is not None else X
@@ -153,8 +183,6 @@ class RegistrationServer:
         return per
 
     def _admit(self, spec: JobSpec, now: float):
-        if spec.lam * len(spec.cams) > self.max_items or len(spec.cams) > self.max_views:
-            raise ValueError(f"job {spec.job_id}: one generation exceeds the server's batch capacity")
         if self.ctx.bytes_in_use() + self._fixed_bytes(spec) > self.budget:
             return None
         fixeds, views = [], []
@@ -176,26 +204,64 @@ class RegistrationServer:
         return _Active(spec, fixeds, views, now, float(getattr(vol, "extent_mm", 1.0)),
                        (getattr(vol, "storage", 0), getattr(vol, "layout", 0)))
 
+    def _can_ever_fit(self, spec: JobSpec) -> str:
+        """Why a job can never be admitted by this server ("" if it can)."""
+        if len(spec.cams) > self.max_views:
+            return f"{len(spec.cams)} views exceed the server's {self.max_views}"
+        if any(c.num_det_rows > self.max_det or c.num_det_cols > self.max_det for c in spec.cams):
+            return f"a detector exceeds the server's {self.max_det}^2 capacity"
+        return ""
+
     def run(self, jobs: list[JobSpec], realtime: bool = True, on_finish=None) -> list[JobResult]:
         """Serves the jobs (sorted by arrival). With realtime=True arrivals follow the wall clock;
         otherwise every job is available at once (throughput mode). on_finish(result) is called
-        as each job completes."""
+        as each job completes. A job that can never be admitted (more views or a larger detector
+        than the server holds, or fixed images larger than the whole HBM budget with nothing
+        else in service) is returned at once with status "rejected" instead of blocking the
+        queue.
+
+        Each scheduling step fills one submission in policy order: every job contributes the
+        next items of its current generation (view-major) while the step's estimated samples
+        stay within step_samples and its items within max_items; the job that opens the step
+        always gets at least one item. A generation larger than the step budget is therefore
+        spread over several steps, so that short jobs arriving meanwhile wait for one step, not
+        for a whole large generation. A job's optimiser advances once all of its generation's
+        items are in."""
         pending = [(j.arrival, j.job_id, j) for j in jobs]
         heapq.heapify(pending)
         active: list[_Active] = []
         done: list[JobResult] = []
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+
+        def reject(spec: JobSpec, why: str):
+            t = time.perf_counter() - t0
+            done.append(JobResult(spec.job_id, spec.arrival if realtime else 0.0, t, t, 0, float("inf"),
+                                  None, None, [], None, None, "rejected", why))
+            if on_finish is not None:
+                on_finish(done[-1])
+
         while pending or active:
             now = clock()
             # admission: arrived jobs, in arrival order, while HBM and views allow
             while pending and pending[0][0] <= now:
+                spec = pending[0][2]
+                why = self._can_ever_fit(spec)
+                if why:
+                    heapq.heappop(pending)
+                    reject(spec, why)
+                    continue
                 if self.max_active is not None and len(active) >= self.max_active:
                     break
-                if sum(len(a.views) for a in active) + len(pending[0][2].cams) > self.max_views:
+                if sum(len(a.views) for a in active) + len(spec.cams) > self.max_views:
                     break
-                a = self._admit(pending[0][2], time.perf_counter() - t0)
+                a = self._admit(spec, time.perf_counter() - t0)
                 if a is None:
+                    if not active:
+                        # nothing in service holds memory that could be released for it
+                        heapq.heappop(pending)
+                        reject(spec, "fixed images exceed the HBM budget")
+                        continue
                     break
                 heapq.heappop(pending)
                 active.append(a)
@@ -203,44 +269,50 @@ class RegistrationServer:
                 if pending and realtime:
                     time.sleep(max(0.0, min(0.002, pending[0][0] - clock())))
                 continue
-            # pack ready generations into one submission (whole generations only)
             if self.policy == "srpt":
                 order = sorted(active, key=lambda a: (a.remaining_work(), a.spec.arrival, a.spec.job_id))
             else:
                 order = active
             budget = self.step_samples
-            packed, n_items, work = [], 0, 0.0
+            packed, n_items = [], 0
             for a in order:
-                k = a.spec.lam * len(a.views)
-                if packed and (n_items + k > self.max_items or (budget is not None and work + a.gen_work > budget)):
-                    continue
-                if packed and a.layout_key != packed[0].layout_key:
+                if packed and a.layout_key != packed[0][0].layout_key:
                     continue  # one submission casts volumes of one storage type and layout
-                packed.append(a)
+                k = a.chunk(budget, self.max_items - n_items, first=not packed)
+                if k == 0:
+                    continue
+                packed.append((a, a.cursor, k))
                 n_items += k
-                work += a.gen_work
-            views, item_view, item_pose, spans = [], [], [], []
-            for a in packed:
-                poses = a.items()
+                if budget is not None:
+                    budget = max(0.0, budget - sum(a.item_work(a.cursor + i) for i in range(k)))
+                if n_items >= self.max_items:
+                    break
+            views, item_view, item_pose = [], [], []
+            for a, c0, k in packed:
                 base_view = len(views)
                 views.extend(a.views)
-                start = len(item_view)
-                for v in range(len(a.views)):          # view-major, as xReg lays out projections
-                    item_view.extend([base_view + v] * len(poses))
-                    item_pose.extend(poses)
-                spans.append((start, len(a.views), len(poses)))
+                lam = a.spec.lam
+                for i in range(c0, c0 + k):              # view-major, as xReg lays out projections
+                    item_view.append(base_view + i // lam)
+                    item_pose.append(a.poses[i % lam])
             self.batch.set_views(views)
             sims = self.batch.eval(np.array(item_view, np.int32), np.stack(item_pose))
             self.steps += 1
             self.poses_evaluated += len(item_view)
             self.max_items_in_step = max(self.max_items_in_step, len(item_view))
             finished = []
-            for a, (start, nv, lam) in zip(packed, spans):
-                s = sims[start:start + nv * lam].reshape(nv, lam)
+            pos = 0
+            for a, c0, k in packed:
+                a.sims[c0:c0 + k] = sims[pos:pos + k]
+                pos += k
+                a.cursor = c0 + k
+                if a.cursor < a.n_items:
+                    continue  # the rest of this generation goes in later steps
+                s = a.sims.reshape(len(a.views), a.spec.lam)
                 f = combine_mean(s).astype(np.float64)  # ImgSimMetric2DCombineMean over views
-                k = int(np.argmin(f))
-                if f[k] < a.best[0]:
-                    a.best = (float(f[k]), a.X[k].copy())
+                j = int(np.argmin(f))
+                if f[j] < a.best[0]:
+                    a.best = (float(f[j]), a.X[j].copy())
                 a.es.tell(f)
                 a.gen += 1
                 in_service = (time.perf_counter() - t0) - a.start
@@ -251,7 +323,7 @@ class RegistrationServer:
                 elif a.spec.max_wall_s is not None and in_service >= a.spec.max_wall_s:
                     a.status = "deadline"
                 else:
-                    a.X = a.es.ask()
+                    a._new_generation(a.es.ask())
                     continue
                 finished.append(a)
             for a in finished:
