@@ -11,10 +11,14 @@ One timed step = one generation: the lambda poses go to the device, one batched 
 (graph-replayed) casts all DRRs and scores them, lambda similarity values come back. The plain
 leg evaluates poses one at a time (DRR launch, similarity launch, synchronous read-back).
 
-The served legs run registrations arriving as a Poisson process: `--serve-jobs` of the C2
-workload (default), and with `--serve-mix-jobs` a mixed stream of three resident phantoms,
-detectors of 192^2 to 768^2, lambda 20-200, one or three views, batched and then served
-serially for comparison.
+Beside the headline the line carries: the same generations with the other ray-geometry mode
+(FP32 set-up, whose step counts equal RayCasterLineIntCPU's, when the headline uses FP64); the
+plain and drop-in call patterns; a three-view, three-level registration (`c3_registration`:
+gradient-NCC, CMA-ES at 96^2, the refiner at 192^2 and 384^2); the 512x512x400 @0.5 mm volume at
+768^2 stored as FP32, FP16 and BF16 (`c4`); and two served legs, registrations arriving as a
+Poisson process: `served` (the C2 workload) and `served_mix` (three resident phantoms, detectors
+of 192^2 to 768^2, lambda 20-200, one or three views; batched, then served one job at a time for
+comparison; latency percentiles per job class).
 
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits: this build runs on one
 GPU.
@@ -22,6 +26,7 @@ GPU.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -34,6 +39,26 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+PEAK_CLOCK_GHZ = 2.4      # MI355X peak engine clock
+MI355X_CUS = 256
+
+
+def device_cus() -> int:
+    """Compute units of the first GPU node in the KFD topology (simd_count / 4 SIMDs per CU),
+    else the MI355X's 256."""
+    base = Path("/sys/class/kfd/kfd/topology/nodes")
+    try:
+        for node in sorted(base.iterdir()):
+            props = {}
+            for line in (node / "properties").read_text().splitlines():
+                k, _, v = line.partition(" ")
+                props[k] = v
+            simd = int(props.get("simd_count", "0"))
+            if simd > 0:
+                return simd // 4
+    except (OSError, ValueError):
+        pass
+    return MI355X_CUS
 
 
 def parse():
@@ -62,13 +87,16 @@ def parse():
                          "or arrival order)")
     ap.add_argument("--serve-step-samples", type=float, default=2.0e10,
                     help="served legs: estimated trilinear samples per submission (0: no limit)")
-    ap.add_argument("--serve-mix-jobs", type=int, default=0,
+    ap.add_argument("--serve-mix-jobs", type=int, default=24,
                     help="mixed served stream (three resident phantoms, 192/384/768 detectors, lambda 20-200, "
                          "30-80 generations, one or three views): number of jobs, run batched and serially")
-    ap.add_argument("--serve-mix-rate", type=float, default=0.25, help="mixed stream: Poisson rate, jobs/s")
-    ap.add_argument("--c3-regi", action="store_true",
-                    help="also time a three-view, three-level registration (96^2 CMA-ES, then the refiner "
-                         "at 192^2 and 384^2)")
+    ap.add_argument("--serve-mix-rate", type=float, default=0.5, help="mixed stream: Poisson rate, jobs/s")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the three-view, three-level registration (96^2 CMA-ES, then the refiner at "
+                         "192^2 and 384^2)")
+    ap.add_argument("--c4-steps", type=int, default=2,
+                    help="reduced-precision storage leg (512x512x400 @0.5 mm, 768^2): timed generations per "
+                         "storage type (0 skips)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run plain device copies of this many MiB (counter calibration under --pmc)")
     return ap.parse_args()
@@ -83,58 +111,87 @@ def generation_poses(w, lam, n_gen, seed=11):
     return out
 
 
-def cpu_baseline(w, fixed, poses, budget_s, smooth):
-    """The CPU restatement of RayCasterLineIntCPU + patch gradient-NCC (oracle), all host
-    threads, on as many poses of the same workload as fit in about budget_s seconds."""
+def cpu_info() -> dict:
+    """The host the CPU baseline ran on: logical CPUs, the affinity mask, the thread budget the
+    environment grants (OMP_NUM_THREADS) and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": int(omp) if omp.isdigit() else None, "model": model}
+
+
+def cpu_baseline(w, fixed, poses, smooth, reps=5, per_rep=16):
+    """The CPU restatement of RayCasterLineIntCPU + patch gradient-NCC (oracle) on the GPU host.
+    Every CPU of the affinity mask is used unless OMP_NUM_THREADS grants fewer (the host's share
+    for this GPU). The DRR and similarity stages are timed separately: one untimed repetition,
+    then the median of `reps` repetitions, each over `per_rep` poses of the timed generations
+    (one caster call over all of them, as RayCasterLineIntCPU::compute gets a generation, then
+    one metric call); the single-thread figure is the median of `reps` one-pose repetitions."""
     from oracle import oracle as O
 
     O.lib()
-    threads = min(len(os.sched_getaffinity(0)), 16)
-    # one untimed pose first (page faults, thread start-up)
-    drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, np.asarray(poses[0], np.float32)[None], step=w.step,
-                       threads=threads)
-    O.patch_grad_ncc_f32(fixed, drr, smooth_ksize=smooth, threads=threads)
-    t0 = time.perf_counter()
-    done = 0
-    t_drr = t_sim = 0.0
-    for p in poses:
+    info = cpu_info()
+    threads = info["affinity_cpus"]
+    if info["omp_num_threads"]:
+        threads = min(threads, info["omp_num_threads"])
+
+    def stages(ps, th):
         a = time.perf_counter()
-        drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, np.asarray(p, np.float32)[None], step=w.step,
-                           threads=threads)
+        drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, np.asarray(ps, np.float32), step=w.step, threads=th)
         b = time.perf_counter()
-        O.patch_grad_ncc_f32(fixed, drr, smooth_ksize=smooth, threads=threads)
-        c = time.perf_counter()
-        t_drr += b - a
-        t_sim += c - b
-        done += 1
-        if c - t0 >= budget_s and done >= 2:
-            break
-    el = time.perf_counter() - t0
-    # one pose on one thread, for the single-core figure
-    a = time.perf_counter()
-    drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, np.asarray(poses[0], np.float32)[None], step=w.step,
-                       threads=1)
-    b = time.perf_counter()
-    O.patch_grad_ncc_f32(fixed, drr, smooth_ksize=smooth, threads=1)
-    c = time.perf_counter()
-    return {"value": done / el, "unit": "poses/s", "cores": threads, "kind": "port",
-            "sample": f"{done} poses of the timed generations ({w.cams[0].num_det_rows}^2 patch-grad-NCC, same "
-                      f"volume and fixed image) after one untimed pose, "
-                      f"DRR {t_drr:.2f}s + similarity {t_sim:.2f}s, oracle FP32 restatement, {threads} threads",
-            "single_thread": {"value": 1.0 / (c - a), "drr_s": round(b - a, 3), "similarity_s": round(c - b, 3)}}
+        O.patch_grad_ncc_f32(fixed, drr, smooth_ksize=smooth, threads=th)
+        return b - a, time.perf_counter() - b
+
+    def med(ps_of_rep, th):
+        stages(ps_of_rep(0), th)  # untimed: page faults, thread start-up
+        runs = [stages(ps_of_rep(r), th) for r in range(reps)]
+        return float(np.median([r[0] for r in runs])), float(np.median([r[1] for r in runs]))
+
+    n = len(poses)
+    t_drr, t_sim = med(lambda r: [poses[(r * per_rep + k) % n] for k in range(per_rep)], threads)
+    t1_drr, t1_sim = med(lambda r: [poses[r % n]], 1)
+    return {"value": per_rep / (t_drr + t_sim), "unit": "poses/s", "cores": threads, "kind": "port",
+            "sample": f"median of {reps} repetitions (after one untimed) of {per_rep} poses of the timed "
+                      f"generations ({w.cams[0].num_det_rows}^2 patch-grad-NCC, same volume and fixed image): "
+                      f"DRR {t_drr:.3f}s + similarity {t_sim:.3f}s per repetition, oracle FP32 restatement, "
+                      f"{threads} threads",
+            "host": info,
+            "drr_s_per_pose": round(t_drr / per_rep, 4), "similarity_s_per_pose": round(t_sim / per_rep, 4),
+            "single_thread": {"value": 1.0 / (t1_drr + t1_sim), "drr_s": round(t1_drr, 3),
+                              "similarity_s": round(t1_sim, 3)}}
 
 
-def profiled_traffic(key: str):
-    """HBM-side traffic per DRR launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated
-    against a plain copy) for this exact configuration, from the committed profile summary; None
-    when the configuration was not profiled."""
-    for p in sorted((ROOT / "profiles").glob("drr_kernel_*.md")):
+def kernel_source_key() -> str:
+    """Hash of the DRR kernel's sources: profile figures are looked up under it, so a figure
+    measured on other kernel code is never reported."""
+    h = hashlib.sha256()
+    for f in ("xrh_drr.hip", "xrh_internal.h"):
+        h.update((ROOT / "xreg_amd" / "csrc" / f).read_bytes())
+    return h.hexdigest()[:12]
+
+
+def profiled(key: str, folder: Path | None = None) -> dict:
+    """Per-launch counter figures of the DRR kernel measured by rocprofv3 for this exact
+    configuration and kernel source, from the committed profile summaries (lines `counters: <key>
+    name=value ...`): valu_insts (SQ_INSTS_VALU), vmem_insts (SQ_INSTS_VMEM_RD), traffic_bytes
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy). Empty when not profiled."""
+    out = {}
+    for p in sorted((folder or ROOT / "profiles").glob("*.md")):
         for line in p.read_text().splitlines():
-            if line.startswith("traffic: ") and f" {key} " in line + " ":
+            if line.startswith("counters: ") and f" {key} " in line + " ":
                 for tok in line.split():
-                    if tok.startswith("bytes_per_launch="):
-                        return float(tok.split("=", 1)[1])
-    return None
+                    if "=" in tok and tok.split("=", 1)[0] in ("valu_insts", "vmem_insts", "traffic_bytes"):
+                        k, v = tok.split("=", 1)
+                        out[k] = float(v)
+    return out
 
 
 def serve_leg(ctx, vol, w, n_jobs, rate, max_gens, lam, geom, smooth, policy, step_samples, seed=21):
@@ -246,8 +303,25 @@ def serve_mix_leg(ctx, n_jobs, rate, geom, smooth, policy, step_samples, seed=31
                "p50_s": round(serve.percentile(lat, 50), 4), "p99_s": round(serve.percentile(lat, 99), 4),
                "max_s": round(max(lat), 4), "scheduler_steps": srv.steps,
                "max_poses_per_submission": srv.max_items_in_step}
+        out["classes"] = by_class(res)
         print(f"served mix ({mode}): {out}", file=sys.stderr, flush=True)
         return out, srv.poses_evaluated
+
+    def by_class(res):
+        """p50 / p99 / max latency per job class (phantom, detector, views), and whether its p99
+        stays under the 2 s service-level objective."""
+        cls = {}
+        for r in res:
+            sp = specs[r.job_id]
+            det = int(sp.cams[0].num_det_rows)
+            cls.setdefault(f"{sp.volume}/{det}sq/{len(sp.cams)}v", []).append(r.latency)
+        out = {}
+        for k in sorted(cls):
+            lat = cls[k]
+            p99 = serve.percentile(lat, 99)
+            out[k] = {"jobs": len(lat), "p50_s": round(serve.percentile(lat, 50), 3), "p99_s": round(p99, 3),
+                      "max_s": round(max(lat), 3), "p99_under_2s": bool(p99 < 2.0)}
+        return out
 
     batched, n_b = run(None)
     serial, n_s = run(1)
@@ -258,6 +332,7 @@ def serve_mix_leg(ctx, n_jobs, rate, geom, smooth, policy, step_samples, seed=31
             "detectors": list(MIX_DETS), "lambda_range": [20, 200], "generation_cap_range": [30, 80],
             "views": [1, 3], "poses_evaluated": n_b, "policy": policy, "step_samples": step_samples,
             "batched": batched, "serial": serial,
+            "classes_meeting_2s_p99": sorted(k for k, v in batched["classes"].items() if v["p99_under_2s"]),
             "same_work": n_b == n_s}
 
 
@@ -306,8 +381,76 @@ def c3_leg(ctx, geom, smooth, seed=41):
         det = int(round(cam0.num_det_rows * lvl.ds_factor))
         out["levels"].append({"detector": det, "method": lvl.method, "evals": int(r.evals),
                               "wall_s": round(float(r.extra["wall_s"]), 3),
-                              "mpd_mm": round(G.mean_projection_distance(r.pose, w.gt, pts, 1020.0), 3)})
+                              "mpd_mm": round(G.mean_projection_distance(r.pose, w.gt, pts, 1020.0), 3),
+                              "fun_start": float(r.history[0]), "fun_end": float(r.fun)})
     print(f"c3 registration: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def batched_leg(ctx, batch, lam, gens, warmup, steps, autotune):
+    """Times exactly `steps` generations (after `warmup`) of one batched, graph-replayed
+    submission per generation: the lambda poses go to the device, all DRRs are cast and scored,
+    lambda similarity values come back. Returns poses/s, the tuning record, the last
+    generation's similarity values, the DRR kernel's average duration (HIP events on the
+    library stream, eager submissions of the same generations) and the sample counts of those
+    launches (reference algorithm, and evaluated after empty-space skipping)."""
+    iv = np.zeros(lam, np.int32)
+    tune = batch.autotune(iv, gens[0], reps=2) if autotune else None
+    for g in range(warmup):
+        batch.eval(iv, gens[g])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sims = None
+    for g in range(warmup, warmup + steps):
+        sims = batch.eval(iv, gens[g])
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    batch.set_options(use_graph=False, time_kernels=True)
+    n_t = min(5, len(gens))
+    drr_ms = []
+    for g in range(n_t):
+        batch.eval(iv, gens[g])
+        drr_ms.append(batch.last_timing()[0])
+    batch.set_options(use_graph=True, time_kernels=False)
+    counts = [batch.sample_count(iv, gens[g], evaluated=True) for g in range(n_t)]
+    k0 = 1 if n_t > 1 else 0   # the first eager launch is not averaged
+    return {"poses_per_s": steps * lam / t, "seconds": t, "tune": tune, "sims": sims,
+            "drr_ms": float(np.mean(drr_ms[k0:])), "S": float(np.mean([c[0] for c in counts[k0:]])),
+            "S_eval": float(np.mean([c[1] for c in counts[k0:]]))}
+
+
+def c4_leg(ctx, lam, steps, geom, smooth):
+    """Reduced-precision volume storage: the seeded 512x512x400 phantom at 0.5 mm stored as FP32,
+    FP16 and BF16 (identical contents), 768^2 detector at 0.388 mm, 0.25 mm steps, patch
+    gradient-NCC against the FP32 ground-truth DRR plus 1% noise; lambda poses per generation,
+    one untimed generation, then `steps` timed generations per storage type."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    w = synth.config("c4", n_poses=1)
+    cam = w.cams[0]
+    P = cam.num_det_rows * cam.num_det_cols
+    gens = generation_poses(w, lam, 1 + steps, seed=71)
+    out = {"volume": "512x512x400@0.5mm", "detector": "768x768@0.388mm", "step_mm": w.step, "lambda": lam,
+           "poses_timed": lam * steps, "geometry": "fp64" if geom == X.XRH_GEOM_FP64 else "fp32"}
+    fixed = None
+    for name, st in (("f32", X.XRH_F32), ("f16", X.XRH_F16), ("bf16", X.XRH_BF16)):
+        vol = X.Volume(ctx, w.vol, w.idx_to_phys, storage=st)
+        if fixed is None:
+            buf = ctx.alloc(4 * P)
+            X.line_int(ctx, vol, w.cams, w.gt.astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf,
+                       geom=geom)
+            fixed = synth.noisy_fixed(buf.download(np.float32).reshape(cam.num_det_rows, cam.num_det_cols))
+            buf.free()
+        fx = X.Fixed(ctx, fixed, X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize=smooth, patch_radius=5)
+        batch = X.Batch(ctx, [X.make_view(vol, fx, cam, w.step)], max_items=lam, geom=geom)
+        r = batched_leg(ctx, batch, lam, gens, 1, steps, autotune=False)
+        out[name] = {"poses_per_s": round(r["poses_per_s"], 1), "drr_ms_per_launch": round(r["drr_ms"], 3),
+                     "samples_per_launch": int(r["S"]), "samples_evaluated_per_launch": int(r["S_eval"]),
+                     "volume_bytes": vol.nbytes, "sim_checksum": float(np.sum(r["sims"]))}
+        print(f"c4 {name}: {out[name]}", file=sys.stderr, flush=True)
+        for o in (batch, fx, vol):
+            o.close()
     return out
 
 
@@ -322,6 +465,9 @@ def main():
         os.environ["XRH_SKIP_EMPTY"] = "0"
     from xreg_amd import _xrh as X
     from xreg_amd import synth
+
+    def note(msg):
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
 
     storage = {"f32": X.XRH_F32, "f16": X.XRH_F16, "bf16": X.XRH_BF16}[args.storage]
     geom = X.XRH_GEOM_FP64 if args.geom == "fp64" else X.XRH_GEOM_FP32
@@ -340,40 +486,28 @@ def main():
     X.line_int(ctx, vol, w.cams, w.gt.astype(np.float32)[None], np.zeros(1, np.int64), w.step, buf, geom=geom)
     fixed = synth.noisy_fixed(buf.download(np.float32).reshape(cam.num_det_rows, cam.num_det_cols))
     fx = X.Fixed(ctx, fixed, X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize=args.smooth, patch_radius=5)
-    batch = X.Batch(ctx, [X.make_view(vol, fx, cam, w.step)], max_items=lam, geom=geom)
     gens = generation_poses(w, lam, args.warmup + args.steps)
-    iv = np.zeros(lam, np.int32)
 
-    # ---- DRR launch variant: tuned on the first generation (untimed) ----
-    tune = None if args.no_autotune else batch.autotune(iv, gens[0], reps=2)
+    # ---- batched (the headline): DRR launch variant tuned on the first generation (untimed),
+    # warmup, then exactly K generations between synchronisations ----
+    batch = X.Batch(ctx, [X.make_view(vol, fx, cam, w.step)], max_items=lam, geom=geom)
+    main_leg = batched_leg(ctx, batch, lam, gens, args.warmup, args.steps, not args.no_autotune)
+    value, tune, sims = main_leg["poses_per_s"], main_leg["tune"], main_leg["sims"]
+    t_batched = main_leg["seconds"]
+    drr_ms_avg, S, S_eval = main_leg["drr_ms"], main_leg["S"], main_leg["S_eval"]
+    note(f"batched {value:.1f} poses/s, DRR {drr_ms_avg:.3f} ms per launch")
 
-    # ---- batched: warmup, then exactly K generations between synchronisations ----
-    for g in range(args.warmup):
-        batch.eval(iv, gens[g])
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    sims = None
-    for g in range(args.warmup, args.warmup + args.steps):
-        sims = batch.eval(iv, gens[g])
-    ctx.synchronize()
-    t_batched = time.perf_counter() - t0
-    value = args.steps * lam / t_batched
+    # ---- the same generations with FP32 ray geometry (the reference's float set-up, step
+    # counts identical to RayCasterLineIntCPU's) ----
+    other_geom = X.XRH_GEOM_FP32 if geom == X.XRH_GEOM_FP64 else X.XRH_GEOM_FP64
+    b2 = X.Batch(ctx, [X.make_view(vol, fx, cam, w.step)], max_items=lam, geom=other_geom)
+    alt = batched_leg(ctx, b2, lam, gens, args.warmup, args.steps, not args.no_autotune)
+    b2.close()
+    note(f"batched, {'fp32' if other_geom == X.XRH_GEOM_FP32 else 'fp64'} geometry: {alt['poses_per_s']:.1f} poses/s")
 
-    # ---- DRR kernel duration (HIP events on the library stream, eager submissions) ----
-    batch.set_options(use_graph=False, time_kernels=True)
-    drr_ms = []
-    for g in range(min(5, len(gens))):
-        batch.eval(iv, gens[g])
-        drr_ms.append(batch.last_timing()[0])
-    batch.set_options(use_graph=True, time_kernels=False)
-    drr_ms_avg = float(np.mean(drr_ms[1:])) if len(drr_ms) > 1 else float(drr_ms[0])
-    # the same generations as the timed DRR launches above: algorithmic and evaluated samples
-    counts = [batch.sample_count(iv, gens[g], evaluated=True) for g in range(min(5, len(gens)))]
-    S = float(np.mean([c[0] for c in counts[1:]])) if len(counts) > 1 else float(counts[0][0])
-    S_eval = float(np.mean([c[1] for c in counts[1:]])) if len(counts) > 1 else float(counts[0][1])
     bv = {X.XRH_F32: 4, X.XRH_F16: 2, X.XRH_BF16: 2}[storage]
     bytes_launch = 8.0 * bv * S + 4.0 * P * lam  # gathers + DRR write
-    achieved = bytes_launch / (drr_ms_avg * 1e-3) / 1e9
+    algorithmic_gbs = bytes_launch / (drr_ms_avg * 1e-3) / 1e9
 
     # ---- plain: one pose per launch, synchronous read-back ----
     n_plain = min(args.plain_poses, lam)
@@ -418,29 +552,68 @@ def main():
     dst = ctx.alloc(copy_mib << 20)
     src.upload(np.ones((copy_mib << 20) // 4, np.float32))
     dst.copy_from(src)
+    ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(5):
         dst.copy_from(src)
+    ctx.synchronize()
     copy_gbs = 5 * 2.0 * (copy_mib << 20) / (time.perf_counter() - t0) / 1e9
     src.free()
     dst.free()
+    note(f"plain {plain_value}, drop-in {dropin_value:.1f} poses/s, copy {copy_gbs:.0f} GB/s")
 
     served = None
     if args.serve_jobs > 0:
         served = serve_leg(ctx, vol, w, args.serve_jobs, args.serve_rate, args.serve_gens, lam, geom, args.smooth,
                            args.serve_policy, args.serve_step_samples)
+        note(f"served C2 class: p50 {served['p50_s']} s, p99 {served['p99_s']} s")
+
+    c3 = None if args.no_c3 else c3_leg(ctx, geom, args.smooth)
+
+    c4 = None
+    if args.c4_steps > 0 and args.config == "c2":
+        c4 = c4_leg(ctx, lam, args.c4_steps, geom, args.smooth)
 
     served_mix = None
     if args.serve_mix_jobs > 0:
         served_mix = serve_mix_leg(ctx, args.serve_mix_jobs, args.serve_mix_rate, geom, args.smooth,
                                    args.serve_policy, args.serve_step_samples)
 
-    c3 = c3_leg(ctx, geom, args.smooth) if args.c3_regi else None
-
     cpu = None
     if not args.no_cpu:
-        # poses of the timed generations, in order, until the time budget is spent
-        cpu = cpu_baseline(w, fixed, np.concatenate(gens[args.warmup:]), args.cpu_seconds, args.smooth)
+        note("cpu baseline")
+        cpu = cpu_baseline(w, fixed, np.concatenate(gens[args.warmup:]), args.smooth)
+
+    # ---- roofline of the DRR kernel (the dominant kernel) ----
+    # The DRR's gathers are served mostly by L1/L2/Infinity Cache (the C2 volume is cache
+    # resident), so the algorithmic gather bytes of the reference algorithm exceed what HBM
+    # moves; the bound that the counters show is vector-ALU issue. `achieved` is the kernel's
+    # VALU wave-instructions per launch (SQ_INSTS_VALU, rocprofv3, same configuration and kernel
+    # source) over its live-measured duration; `peak` is one VALU wave-instruction per CU per
+    # clock at the 2.4 GHz peak engine clock. `hbm_frac` is the measured HBM-side traffic
+    # (FETCH_SIZE + WRITE_SIZE, calibrated) over the same duration against 8 TB/s.
+    key = (f"workload={workload} storage={args.storage} geometry={args.geom} skip={0 if args.no_skip else 1} "
+           f"smooth={args.smooth} src={kernel_source_key()}")
+    prof = profiled(key)
+    cus = device_cus()
+    valu_peak = cus * PEAK_CLOCK_GHZ  # G wave-instructions / s
+    valu_rate = prof["valu_insts"] / (drr_ms_avg * 1e-3) / 1e9 if "valu_insts" in prof else None
+    traffic = prof.get("traffic_bytes")
+    roofline = {"bound": "valu", "achieved": round(valu_rate, 1) if valu_rate else None, "peak": valu_peak,
+                "unit": "G VALU wave-instructions/s", "frac": round(valu_rate / valu_peak, 4) if valu_rate else None,
+                "traffic": traffic,
+                "hbm_gbs": round(traffic / (drr_ms_avg * 1e-3) / 1e9, 1) if traffic else None,
+                "hbm_frac": round(traffic / (drr_ms_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                "hbm_peak_gbs": HBM_PEAK_GBS, "profile_key": key,
+                "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4), "cus": cus,
+                "valu_insts_per_launch": prof.get("valu_insts"), "vmem_insts_per_launch": prof.get("vmem_insts"),
+                "samples_per_launch": int(S), "samples_evaluated_per_launch": int(S_eval),
+                "valu_lane_ops_per_evaluated_sample": (round(64.0 * prof["valu_insts"] / S_eval, 2)
+                                                       if "valu_insts" in prof else None),
+                "algorithmic_bytes_per_launch": int(bytes_launch),
+                "algorithmic_gbs": round(algorithmic_gbs, 1),
+                "evaluated_gbs": round((8.0 * bv * S_eval + 4.0 * P * lam) / (drr_ms_avg * 1e-3) / 1e9, 1),
+                "measured_copy_gbs": round(copy_gbs, 1)}
 
     line = {
         "metric": f"poses/sec (DRR+grad-NCC, {det}^2 det) batched vs plain; p50/p99 served regi latency",
@@ -463,21 +636,15 @@ def main():
         "plain_poses_per_s": round(plain_value, 2) if plain_value else None,
         "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
         "dropin_poses_per_s": round(dropin_value, 2),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(f"workload={workload} storage={args.storage} "
-                                                 f"geometry={args.geom} skip={0 if args.no_skip else 1} "
-                                                 f"smooth={args.smooth}"),
-                     "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4),
-                     "samples_per_launch": int(S), "samples_evaluated_per_launch": int(S_eval),
-                     "algorithmic_bytes_per_launch": int(bytes_launch),
-                     "evaluated_gbs": round((8.0 * bv * S_eval + 4.0 * P * lam) / (drr_ms_avg * 1e-3) / 1e9, 1),
-                     "measured_copy_gbs": round(copy_gbs, 1)},
+        f"{'fp32' if other_geom == X.XRH_GEOM_FP32 else 'fp64'}_geometry_poses_per_s": round(alt["poses_per_s"], 2),
+        f"{'fp32' if other_geom == X.XRH_GEOM_FP32 else 'fp64'}_geometry_drr_ms": round(alt["drr_ms"], 4),
+        "roofline": roofline,
         "empty_space_skipping": not args.no_skip,
         "autotune": tune,
         "served": served,
         "served_mix": served_mix,
         "c3_registration": c3,
+        "c4": c4,
         "cpu_baseline": cpu,
         "sim_checksum": float(np.sum(sims)),
     }

@@ -147,8 +147,9 @@ xrh_status xrh_volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[
 xrh_status xrh_volume_destroy(xrh_volume* vol);
 /* Same, with an explicit HBM layout (0 = padded linear, 1 = quad-packed FP32 corners,
  * 2 = oct-packed 16-bit corners, 3 = FP32 trilinear-polynomial coefficients per cell,
- * 4 = FP32 bilinear coefficients of the face normal to x per voxel); xrh_volume_create picks
- * 4 for FP32 and 2 otherwise (XRH_VOLUME_LAYOUT=linear, =quad or =coef overrides). */
+ * 4 = FP32 bilinear coefficients of the face normal to x per voxel, 5 = the same faces stored
+ * in 4x4x4-cell bricks); xrh_volume_create picks 4 for FP32 and 2 otherwise
+ * (XRH_VOLUME_LAYOUT=linear, =quad, =coef or =brick overrides). */
 xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_t size[3],
                                     const float idx_to_phys[16], int32_t storage,
                                     int32_t layout, xrh_volume** out);

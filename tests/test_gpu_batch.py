@@ -139,3 +139,24 @@ def test_async_launch_and_wait(ctx, oracle):
     b.close()
     fx.close()
     vol.close()
+
+
+def test_graph_cache_evicts_and_rebuilds(ctx, oracle):
+    """More distinct submission shapes than the batch keeps graphs for (32): every submission,
+    including one whose graph was evicted and is captured again, gives the eager values."""
+    from xreg_amd import _xrh as X
+
+    print("case: graph cache eviction (40 submission shapes)", flush=True)
+    w, vol, fx, fixed = _setup(ctx, oracle, "c2", 3, n=8)
+    b = X.Batch(ctx, [X.make_view(vol, fx, w.cams[0], w.step)], max_items=40)
+    poses = np.concatenate([w.poses] * 5)[:40]
+    iv = np.zeros(40, np.int32)
+    b.set_options(use_graph=False, time_kernels=False)
+    eager = b.eval(iv, poses)
+    b.set_options(use_graph=True, time_kernels=False)
+    for n in list(range(1, 41)) + [1, 2, 40, 3]:
+        got = b.eval(iv[:n], poses[:n])
+        assert np.array_equal(got, eager[:n]), n
+    b.close()
+    fx.close()
+    vol.close()

@@ -6,7 +6,7 @@ xregRayCastInterface.h: float FrameTransform) so both sides see the same rays.""
 import numpy as np
 import pytest
 
-from tests.helpers import SIM_ABS_TOL, attribute_drr_errors, drr_rel_err, poses_f32
+from tests.helpers import SIM_ABS_TOL, attribute_drr_errors, describe_pixels, drr_rel_err, poses_f32
 
 pytestmark = pytest.mark.gpu
 
@@ -64,6 +64,10 @@ def test_bench_path_matches_cpu_reference(ctx, oracle, geom):
           f"step counts differ on {int((steps_dev != steps_cpu).sum())} rays", flush=True)
     e64 = np.abs(got.astype(np.float64) - ref64) / float(ref64.max())
     print(f"DRR vs FP64 oracle: |err|/max max {e64.max():.3e} median {np.median(e64):.3e}", flush=True)
+    print(f"drift pixels where the device is within 1e-4 (or 1e-6 of the image maximum) of FP64 but not nearer "
+          f"than the restatement: {a['ref_drift_within_tol_only']}")
+    if a["unexplained"]:
+        print("unexplained: " + describe_pixels(a["unexplained_mask"], got, ref32, ref64))
     assert a["unexplained"] == 0
     if geom == 0:
         # the FP32 set-up repeats the reference's float arithmetic: identical step counts
@@ -76,6 +80,13 @@ def test_bench_path_matches_cpu_reference(ctx, oracle, geom):
     want64 = oracle.patch_grad_ncc_f64(fixed, ref64, smooth_ksize=5)
     d32 = np.abs(sims[:N_CHECK] - want32)
     d64 = np.abs(sims[:N_CHECK].astype(np.float64) - want64)
-    print(f"patch-grad-NCC: |gpu - fp32 restatement| max {d32.max():.3e}; |gpu - fp64 oracle| max {d64.max():.3e}")
+    # the FP32 restatement's own distance from FP64: with FP32 ray geometry the sample positions
+    # drift as the reference's do, so that path is held to the reference's distance plus 1e-5
+    ref_gap = np.abs(want32.astype(np.float64) - want64)
+    print(f"patch-grad-NCC: |gpu - fp32 restatement| max {d32.max():.3e}; |gpu - fp64 oracle| max {d64.max():.3e}; "
+          f"|fp32 restatement - fp64 oracle| max {ref_gap.max():.3e}")
     assert d32.max() <= SIM_ABS_TOL
-    assert d64.max() <= SIM_ABS_TOL
+    if geom == 1:
+        assert d64.max() <= SIM_ABS_TOL
+    else:
+        assert np.all(d64 <= SIM_ABS_TOL + ref_gap)

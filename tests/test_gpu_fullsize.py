@@ -147,42 +147,25 @@ def test_c2_full_cell_reuse_is_bit_identical(ctx, c2full, monkeypatch):
 
 
 def test_c3_full_multiview_pyramid_registration(ctx, oracle):
-    """Three views (30 degrees apart) of the 256x256x200 volume at 384^2, registered over a
-    three-level pyramid (detector 96^2, 192^2, 384^2) with CMA-ES at the coarse levels and the
-    local refiner at full resolution: the estimate moves towards the ground truth."""
-    from xreg_amd import _xrh as X
-    from xreg_amd import geometry as G
-    from xreg_amd import regi, synth
+    """C3 as specified, through the same code bench.py reports (`c3_leg`): three views (30
+    degrees apart) of the 256x256x200 volume at 384^2, gradient-NCC averaged over the views,
+    registered over a three-level pyramid (detector 96^2, 192^2, 384^2): CMA-ES (lambda 50, up
+    to 50 generations) at the coarse level, then the BOBYQA-pattern refiner (13 single-pose
+    evaluations, then one per iteration, at most 20 x 13) at 192^2 and 384^2, FP64 ray
+    geometry. The estimate ends near the ground truth and every level improves on its start."""
+    import bench
 
-    print("case: c3 full size 3 views x 3 levels registration", flush=True)
-    w = synth.config("c3", n_poses=1)
-    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
-    fixed = []
-    for v, cam in enumerate(w.cams):
-        buf = ctx.alloc(4 * cam.num_det_rows * cam.num_det_cols)
-        X.line_int(ctx, vol, [cam], poses_f32(w.gt[None]), np.zeros(1, np.int64), w.step, buf, geom=1)
-        fixed.append(synth.noisy_fixed(buf.download(np.float32).reshape(cam.num_det_rows, cam.num_det_cols),
-                                       seed=40 + v))
-        buf.free()
-    init = G.perturbed_pose(w.gt, w.center, np.deg2rad([3.0, -2.0, 2.5]), [6.0, -5.0, 4.0])
-    cam0 = w.cams[0]
-
-    def ds(img, f):
-        return X.downsample_image(ctx, img, f, spacing=(cam0.det_row_spacing, cam0.det_col_spacing))
-
-    sig = [np.deg2rad(3.0)] * 3 + [6.0] * 3
-    levels = [regi.Level(0.25, "cmaes", dict(lam=50, sigma=sig, max_gens=25, seed=5)),
-              regi.Level(0.5, "cmaes", dict(lam=50, sigma=[x / 3 for x in sig], max_gens=15, seed=6)),
-              regi.Level(1.0, "refine", dict(rho_beg=[np.deg2rad(0.3)] * 3 + [0.6] * 3, max_evals=60))]
-    pose, res = regi.run_multi_level(ctx, vol, w.cams, fixed, init, w.center, w.step, levels, ds, geom=1)
-    vol.close()
-    ext = np.array(w.vol.shape[::-1]) * w.spacing
-    pts = [w.center + 0.25 * ext * np.array(c) for c in [(-1, -1, -1), (1, 1, 1), (1, -1, 1), (-1, 1, -1), (0, 0, 0)]]
-    e0 = G.mean_projection_distance(init, w.gt, pts, 1020.0)
-    es = [G.mean_projection_distance(r.pose, w.gt, pts, 1020.0) for r in res]
-    print(f"mPD {e0:.3f} -> " + " -> ".join(f"{e:.3f}" for e in es) + f"; evals {[r.evals for r in res]}")
-    assert es[-1] < 0.3 * e0
-    assert res[-1].fun <= res[-1].history[0]
+    print("case: c3 full size 3 views x 3 levels registration (bench c3 leg)", flush=True)
+    out = bench.c3_leg(ctx, 1, 5)
+    lv = out["levels"]
+    print(f"mPD {out['mpd_start_mm']} -> " + " -> ".join(str(x["mpd_mm"]) for x in lv) +
+          f"; evals {[x['evals'] for x in lv]}; wall {out['total_wall_s']} s")
+    assert [x["method"] for x in lv] == ["cmaes", "refine", "refine"]
+    assert [x["detector"] for x in lv] == [96, 192, 384]
+    assert lv[0]["evals"] % 50 == 0 and 13 <= lv[1]["evals"] <= 260 and 13 <= lv[2]["evals"] <= 260
+    assert lv[-1]["mpd_mm"] < 0.1 * out["mpd_start_mm"]
+    for x in lv:
+        assert x["fun_end"] <= x["fun_start"]
 
 
 def test_volume_past_4gib_packed_layout_falls_back_to_linear(ctx, oracle):

@@ -26,7 +26,7 @@ XRH_LINE_INT_SUM, XRH_LINE_INT_MAX = 0, 1
 XRH_STORE_REPLACE, XRH_STORE_ACCUM = 0, 1
 XRH_GEOM_FP32, XRH_GEOM_FP64 = 0, 1
 XRH_SIM_NCC, XRH_SIM_GRAD_NCC, XRH_SIM_PATCH_NCC, XRH_SIM_PATCH_GRAD_NCC = 0, 1, 2, 3
-LAYOUT_LINEAR, LAYOUT_QUAD, LAYOUT_OCT, LAYOUT_COEF, LAYOUT_FACE = 0, 1, 2, 3, 4
+LAYOUT_LINEAR, LAYOUT_QUAD, LAYOUT_OCT, LAYOUT_COEF, LAYOUT_FACE, LAYOUT_BRICK = 0, 1, 2, 3, 4, 5
 
 
 class XrhError(RuntimeError):

@@ -52,8 +52,16 @@ struct xrh_batch {
   float* sims_host = nullptr;      // pinned
   hipEvent_t ev_drr0 = nullptr, ev_drr1 = nullptr, ev_all0 = nullptr, ev_all1 = nullptr;
   bool timed_last = false;
-  std::map<std::string, hipGraphExec_t> graphs;
-  std::vector<hipGraph_t> graph_objs;
+  // instantiated graphs per submission signature, least recently used evicted beyond
+  // kMaxCachedGraphs (a server whose item-to-view assignment changes every step would
+  // otherwise keep one graph per distinct assignment)
+  struct CachedGraph {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t last_use = 0;
+  };
+  std::map<std::string, CachedGraph> graphs;
+  uint64_t graph_clock = 0;
   int64_t last_n = 0;
   bool pending = false;
   // DRR launch variant: set explicitly / by xrh_batch_autotune, else looked up in the
@@ -83,12 +91,30 @@ BlobLayout blob_layout(int64_t max_items)
   return L;
 }
 
+constexpr size_t kMaxCachedGraphs = 32;
+
 void clear_graphs(xrh_batch* b)
 {
-  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto g : b->graph_objs) (void)hipGraphDestroy(g);
+  for (auto& kv : b->graphs) {
+    (void)hipGraphExecDestroy(kv.second.exec);
+    (void)hipGraphDestroy(kv.second.graph);
+  }
   b->graphs.clear();
-  b->graph_objs.clear();
+}
+
+// Destroys least recently used graphs until at most `keep` remain (the stream has been
+// synchronised, so none of them is executing).
+void evict_graphs(xrh_batch* b, size_t keep)
+{
+  while (b->graphs.size() > keep) {
+    auto lru = b->graphs.begin();
+    for (auto it = b->graphs.begin(); it != b->graphs.end(); ++it) {
+      if (it->second.last_use < lru->second.last_use) lru = it;
+    }
+    (void)hipGraphExecDestroy(lru->second.exec);
+    (void)hipGraphDestroy(lru->second.graph);
+    b->graphs.erase(lru);
+  }
 }
 
 void batch_free(xrh_batch* b)
@@ -384,6 +410,7 @@ void launch(xrh_batch* b, const int32_t* item_view, const double* item_pose, int
     const std::string key = signature(b, item_view, n, drr_out);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
+      evict_graphs(b, kMaxCachedGraphs - 1);
       hipGraph_t g = nullptr;
       XRH_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
@@ -396,11 +423,18 @@ void launch(xrh_batch* b, const int32_t* item_view, const double* item_pose, int
       }
       XRH_HIP_CHECK(hipStreamEndCapture(s, &g));
       hipGraphExec_t ge = nullptr;
-      XRH_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      b->graph_objs.push_back(g);
-      it = b->graphs.emplace(key, ge).first;
+      const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        XRH_HIP_CHECK(ie);
+      }
+      xrh_batch::CachedGraph cg;
+      cg.exec = ge;
+      cg.graph = g;
+      it = b->graphs.emplace(key, cg).first;
     }
-    XRH_HIP_CHECK(hipGraphLaunch(it->second, s));
+    it->second.last_use = ++b->graph_clock;
+    XRH_HIP_CHECK(hipGraphLaunch(it->second.exec, s));
   } else {
     enqueue(b, D, groups, n, timed);
   }

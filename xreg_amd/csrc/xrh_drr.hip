@@ -289,7 +289,17 @@ __device__ inline float sample(VolView& v, float x, float y, float z)
   const int ix = cvt_floor(x), iy = cvt_floor(y), iz = cvt_floor(z);
   // element index of the base corner; ix, iy, iz >= -1 and the pitches < 2^23, so the
   // signed 24-bit multiply-adds are exact; v.apron = 1 + py + pz shifts past the apron
-  const uint32_t e = (uint32_t)(__mul24(iz, (int)v.pz) + __mul24(iy, (int)v.py) + ix) + v.apron;
+  uint32_t e;
+  if constexpr (kFmt == 12 || kFmt == 13) {
+    // bricked x-faces: padded cell (ix+1, iy+1, iz+1), brick (./4) in brick order with the
+    // brick pitches in py / pz, 5x4x4 faces per brick (x fastest, the last x-face repeated)
+    const uint32_t ax = (uint32_t)(ix + 1), ay = (uint32_t)(iy + 1), az = (uint32_t)(iz + 1);
+    const uint32_t brick = __umul24(az >> 2, v.pz) + __umul24(ay >> 2, v.py) + (ax >> 2);
+    const uint32_t local = __umul24(az & 3u, 20u) + __umul24(ay & 3u, 5u) + (ax & 3u);
+    e = __umul24(brick, (uint32_t)kBrickFaces) + local;
+  } else {
+    e = (uint32_t)(__mul24(iz, (int)v.pz) + __mul24(iy, (int)v.py) + ix) + v.apron;
+  }
   if constexpr (kFmt == 0) {
     const int o = (int)(e * 4u);
     const int oy = (int)(v.py * 4u);
@@ -328,11 +338,12 @@ __device__ inline float sample(VolView& v, float x, float y, float z)
 #pragma unroll
     for (int i = 0; i < 8; ++i) f[i] = (kFmt == 2) ? h2f(r[i]) : bf2f(r[i]);
     return tri(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], wx, wy, wz);
-  } else if constexpr (kFmt == 10 || kFmt == 11) {
+  } else if constexpr (kFmt == 10 || kFmt == 11 || kFmt == 12 || kFmt == 13) {
     // face layout: faces normal to x at x and x+1 (adjacent 16-byte elements), bilinear in y, z
-    // then linear along x. 11 keeps the faces while the sample stays in its cell.
+    // then linear along x. 11 and 13 keep the faces while the sample stays in its cell; 12 and
+    // 13 read the bricked arrangement.
     uint4 f0, f1;
-    if constexpr (kFmt == 10) {
+    if constexpr (kFmt == 10 || kFmt == 12) {
       const auto a = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(e * 16u), 0, 0);
       const auto b = __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, (int)(e * 16u + 16u), 0, 0);
       f0 = make_uint4(a[0], a[1], a[2], a[3]);
@@ -574,7 +585,7 @@ __device__ inline float march(const Ray& ray, VolView& vv, int i, int last, floa
         pz = pz + ray.dz;
       }
     }
-    if constexpr (kFmt == 7 || kFmt == 11) {
+    if constexpr (kFmt == 7 || kFmt == 11 || kFmt == 13) {
       // one sample per iteration: the cell cache lives in fixed registers; the count runs
       // in float (exact below 2^24) so the loop test is one float compare
       const float flast = (float)last;
@@ -596,7 +607,7 @@ __device__ inline float march(const Ray& ray, VolView& vv, int i, int last, floa
       }
       if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
     }
-  } else if constexpr (kFmt == 7 || kFmt == 11) {
+  } else if constexpr (kFmt == 7 || kFmt == 11 || kFmt == 13) {
     const float flast = (float)last;
     for (float f0 = (float)i; f0 <= flast; f0 = f0 + 1.0f) {
       acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
@@ -768,7 +779,8 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
   const int fmt = (L.storage == XRH_F32)
                       ? (L.layout == kLayoutCoef ? (reuse ? 7 : 6)
                                                  : (L.layout == kLayoutFace ? (reuse ? 11 : 10)
-                                                                            : (L.layout == kLayoutQuad ? 1 : 0)))
+                                                    : (L.layout == kLayoutBrick ? (reuse ? 13 : 12)
+                                                                                : (L.layout == kLayoutQuad ? 1 : 0))))
                 : (L.storage == XRH_F16) ? (L.layout == kLayoutOct ? 3 : 2)
                                          : (L.layout == kLayoutOct ? 5 : 4);
   switch (fmt) {
@@ -781,6 +793,8 @@ void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, fl
     case 6: XRH_LAUNCH_FMT(6); break;
     case 7: XRH_LAUNCH_FMT(7); break;
     case 10: XRH_LAUNCH_FMT(10); break;
+    case 12: XRH_LAUNCH_FMT(12); break;
+    case 13: XRH_LAUNCH_FMT(13); break;
     default: XRH_LAUNCH_FMT(11); break;
   }
 #undef XRH_LAUNCH_FMT

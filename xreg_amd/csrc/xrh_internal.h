@@ -72,10 +72,17 @@ enum Layout : int32_t {
   kLayoutCoef = 3,    // f32 only: the 8 coefficients of the cell's trilinear polynomial in
                       //   its local coordinates (32 bytes, 2 adjacent 16-byte loads per
                       //   sample, 8x memory); evaluation is 7 FMAs instead of 7 lerps
-  kLayoutFace = 4     // f32 only: bilinear coefficients of each voxel's face normal to x
+  kLayoutFace = 4,    // f32 only: bilinear coefficients of each voxel's face normal to x
                       //   (over y, z; 16 bytes, 4x memory); a cell is the faces at x and x+1,
                       //   two adjacent 16-byte loads, bilinear twice then linear along x
+  kLayoutBrick = 5    // f32 only: the x-face elements stored brick by brick, bricks of 4x4x4
+                      //   cells in x-fastest brick order, each holding its 5x4x4 faces (the
+                      //   next brick's first x-face repeated, so a cell's two faces stay
+                      //   adjacent); 5x memory. Same samples as kLayoutFace.
 };
+// brick geometry of kLayoutBrick: cells per brick edge and faces per brick
+constexpr int32_t kBrickEdge = 4;
+constexpr int32_t kBrickFaces = (kBrickEdge + 1) * kBrickEdge * kBrickEdge;
 
 struct CamDev {
   float kinv[9];

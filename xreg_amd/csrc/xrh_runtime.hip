@@ -227,6 +227,25 @@ __global__ void build_layout_kernel(const float* __restrict__ src, int X, int Y,
     const double v00 = V(a, b, c), v10 = V(a, b + 1, c), v01 = V(a, b, c + 1), v11 = V(a, b + 1, c + 1);
     ((float4*)dst)[i] = make_float4((float)v00, (float)(v10 - v00), (float)(v01 - v00),
                                     (float)((v11 - v01) - (v10 - v00)));
+  } else if (layout == kLayoutBrick) {
+    // the face of (a, b, c) as in kLayoutFace, written to its brick and, when it is the first
+    // x-face of a brick, also as the last x-face of the previous brick in x
+    const double v00 = V(a, b, c), v10 = V(a, b + 1, c), v01 = V(a, b, c + 1), v11 = V(a, b + 1, c + 1);
+    const float4 f = make_float4((float)v00, (float)(v10 - v00), (float)(v01 - v00),
+                                 (float)((v11 - v01) - (v10 - v00)));
+    const int nbx = (int)((Xp - 1 + kBrickEdge - 1) / kBrickEdge);   // bricks of cells 0..X
+    const int nby = (int)((Yp + kBrickEdge - 1) / kBrickEdge);
+    const int by = b / kBrickEdge, bz = c / kBrickEdge;
+    const int local_yz = (c % kBrickEdge) * (kBrickEdge + 1) * kBrickEdge + (b % kBrickEdge) * (kBrickEdge + 1);
+    const int bx = a / kBrickEdge;
+    if (bx < nbx) {
+      const int64_t brick = ((int64_t)bz * nby + by) * nbx + bx;
+      ((float4*)dst)[brick * kBrickFaces + local_yz + (a % kBrickEdge)] = f;
+    }
+    if (a % kBrickEdge == 0 && a > 0) {
+      const int64_t brick = ((int64_t)bz * nby + by) * nbx + (bx - 1);
+      ((float4*)dst)[brick * kBrickFaces + local_yz + kBrickEdge] = f;
+    }
   } else if (layout == kLayoutCoef) {
     // f(u, v, w) = c0 + c1 u + c2 v + c3 u v + w (c4 + c5 u + c6 v + c7 u v) over the cell's
     // corners, differences taken in double and rounded once
@@ -339,6 +358,7 @@ static int default_layout(int storage)
   if (env && std::strcmp(env, "linear") == 0) return kLayoutLinear;
   if (env && std::strcmp(env, "quad") == 0 && storage == XRH_F32) return kLayoutQuad;
   if (env && std::strcmp(env, "coef") == 0 && storage == XRH_F32) return kLayoutCoef;
+  if (env && std::strcmp(env, "brick") == 0 && storage == XRH_F32) return kLayoutBrick;
   // FP32: x-face bilinear coefficients; 16-bit: all corners in one 16-byte element
   return (storage == XRH_F32) ? kLayoutFace : kLayoutOct;
 }
@@ -374,7 +394,7 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
   XRH_REQUIRE(ctx, "null context");
   const bool auto_layout = layout < 0;
   if (auto_layout) layout = default_layout(storage);
-  if (layout == kLayoutQuad || layout == kLayoutCoef || layout == kLayoutFace) {
+  if (layout == kLayoutQuad || layout == kLayoutCoef || layout == kLayoutFace || layout == kLayoutBrick) {
     XRH_REQUIRE(storage == XRH_F32, "layout needs FP32 storage");
   }
   if (layout == kLayoutOct) XRH_REQUIRE(storage != XRH_F32, "oct layout needs 16-bit storage");
@@ -391,7 +411,11 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
     layout = kLayoutLinear;
   }
   const int64_t esize = (layout == kLayoutLinear) ? (storage == XRH_F32 ? 4 : 2) : (layout == kLayoutCoef ? 32 : 16);
-  const uint64_t bytes = (uint64_t)(padded * esize);
+  // bricks: cells 0..X in x, padded faces 0..Y+1 and 0..Z+1 in y and z, 5x4x4 faces each
+  const int64_t nbx = (X + 1 + kBrickEdge - 1) / kBrickEdge, nby = (Y + 2 + kBrickEdge - 1) / kBrickEdge,
+                nbz = (Z + 2 + kBrickEdge - 1) / kBrickEdge;
+  const uint64_t bytes = (layout == kLayoutBrick) ? (uint64_t)(nbx * nby * nbz * kBrickFaces * 16)
+                                                  : (uint64_t)(padded * esize);
   XRH_REQUIRE(bytes < 0xFFFFF000ull, "volume layout exceeds 4 GiB; use the linear layout or split the volume");
   auto* v = new xrh_volume();
   v->ctx = ctx;
@@ -416,6 +440,8 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
       XRH_HIP_CHECK(hipGetLastError());
     }
     const int64_t blocks = (padded + threads - 1) / threads;
+    // brick entries past the volume's faces are never sampled; zero them all the same
+    if (layout == kLayoutBrick) XRH_HIP_CHECK(hipMemsetAsync(v->data, 0, bytes, ctx->stream));
     hipLaunchKernelGGL(build_layout_kernel, dim3((unsigned)blocks), dim3(threads), 0, ctx->stream,
                        (const float*)raw, (int)X, (int)Y, (int)Z, v->data, storage, layout);
     XRH_HIP_CHECK(hipGetLastError());
@@ -461,8 +487,14 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
   d.X = (int32_t)X;
   d.Y = (int32_t)Y;
   d.Z = (int32_t)Z;
-  d.pitch_y = (uint32_t)(X + 2);
-  d.pitch_z = (uint32_t)((X + 2) * (Y + 2));
+  if (layout == kLayoutBrick) {
+    // brick pitches (bricks per row, per slice)
+    d.pitch_y = (uint32_t)nbx;
+    d.pitch_z = (uint32_t)(nbx * nby);
+  } else {
+    d.pitch_y = (uint32_t)(X + 2);
+    d.pitch_z = (uint32_t)((X + 2) * (Y + 2));
+  }
   d.storage = storage;
   d.layout = layout;
   float ms = INFINITY;
@@ -731,7 +763,7 @@ xrh_status xrh_volume_create_hu(xrh_ctx* ctx, const float* vox_hu, const int64_t
                                 double mu_water, double mu_air, xrh_volume** out)
 {
   return guarded([&] {
-    XRH_REQUIRE(layout >= -1 && layout <= 4, "bad layout");
+    XRH_REQUIRE(layout >= -1 && layout <= 5, "bad layout");
     const HuParams hu{hu_lower, mu_water, mu_air};
     volume_create(ctx, vox_hu, size, idx_to_phys, storage, layout, out, &hu);
   });
@@ -742,7 +774,7 @@ xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_
                                     xrh_volume** out)
 {
   return guarded([&] {
-    XRH_REQUIRE(layout >= 0 && layout <= 4, "bad layout");
+    XRH_REQUIRE(layout >= 0 && layout <= 5, "bad layout");
     volume_create(ctx, vox, size, idx_to_phys, storage, layout, out);
   });
 }

@@ -63,6 +63,8 @@ class JobResult:
     status: str = "max_gens"           # "max_gens", "converged" (CMA-ES stop test), "deadline", or
                                        # "rejected" (can never be admitted: see `reason`)
     reason: str = ""
+    first_poses: np.ndarray | None = None  # the first generation's lambda poses (lambda, 4, 4)
+    first_sims: np.ndarray | None = None   # and their similarity values per view (views, lambda)
 
     @property
     def latency(self) -> float:
@@ -101,6 +103,7 @@ class _Active:
         self.es = CMAES(np.zeros(6), np.asarray(spec.sigma), spec.lam, spec.seed,
                         bounds=None if spec.bounds is None else np.asarray(spec.bounds))
         self.gen = 0
+        self.first = (None, None)
         self.best = (np.inf, None)
         self.status = "max_gens"
         self._new_generation(self.es.ask())
@@ -309,6 +312,8 @@ class RegistrationServer:
                 if a.cursor < a.n_items:
                     continue  # the rest of this generation goes in later steps
                 s = a.sims.reshape(len(a.views), a.spec.lam)
+                if a.gen == 0:
+                    a.first = (a.poses.copy(), s.copy())
                 f = combine_mean(s).astype(np.float64)  # ImgSimMetric2DCombineMean over views
                 j = int(np.argmin(f))
                 if f[j] < a.best[0]:
@@ -333,7 +338,8 @@ class RegistrationServer:
                 done.append(JobResult(a.spec.job_id, a.spec.arrival if realtime else 0.0, a.start, t, a.gen,
                                       a.best[0], bx, pose_from_params(a.spec.init_pose, a.spec.center, bx),
                                       list(a.es.history), xm,
-                                      pose_from_params(a.spec.init_pose, a.spec.center, xm), a.status))
+                                      pose_from_params(a.spec.init_pose, a.spec.center, xm), a.status,
+                                      first_poses=a.first[0], first_sims=a.first[1]))
                 a.close()
                 active.remove(a)
                 if on_finish is not None:

@@ -19,23 +19,26 @@ PHANTOM_SEED, POSE_SEED, NOISE_SEED = 1, 2, 3
 def phantom(size_xyz, seed: int = PHANTOM_SEED) -> np.ndarray:
     X, Y, Z = (int(s) for s in size_xyz)
     rng = np.random.default_rng(seed)
-    z = np.arange(Z, dtype=np.float32)[:, None, None]
-    y = np.arange(Y, dtype=np.float32)[None, :, None]
-    x = np.arange(X, dtype=np.float32)[None, None, :]
+    z = np.arange(Z, dtype=np.float32)
+    y = np.arange(Y, dtype=np.float32)[:, None]
+    x = np.arange(X, dtype=np.float32)[None, :]
     cx, cy, cz = (X - 1) / 2.0, (Y - 1) / 2.0, (Z - 1) / 2.0
 
     def ell(fx, fy, fz, shrink=0.0):
+        # float32 terms of ((x-cx)/ax)^2 + ((y-cy)/ay)^2 + ((z-cz)/az)^2, summed in that order
         ax, ay, az = fx * X - shrink, fy * Y - shrink, fz * Z - shrink
-        return ((x - cx) / ax) ** 2 + ((y - cy) / ay) ** 2 + ((z - cz) / az) ** 2 <= 1.0
+        return ((x - cx) / ax) ** 2 + ((y - cy) / ay) ** 2, ((z - cz) / az) ** 2
 
+    outer, shell_outer, inner = ell(0.45, 0.45, 0.45), ell(0.30, 0.30, 0.30), ell(0.30, 0.30, 0.30, shrink=2.0)
     vol = np.zeros((Z, Y, X), dtype=np.float32)
-    vol[ell(0.45, 0.45, 0.45)] = 0.02
-    shell_outer = ell(0.30, 0.30, 0.30)
-    inner = ell(0.30, 0.30, 0.30, shrink=2.0)
-    vol[shell_outer] = 0.05
-    noise = rng.uniform(-0.1, 0.1, size=(Z, Y, X)).astype(np.float32)
-    trab = 0.03 * (1.0 + noise)
-    vol[inner] = trab[inner]
+    # slice by slice (the noise stream is drawn in the same order as one (Z, Y, X) draw)
+    for k in range(Z):
+        sl = vol[k]
+        sl[outer[0] + outer[1][k] <= 1.0] = 0.02
+        sl[shell_outer[0] + shell_outer[1][k] <= 1.0] = 0.05
+        noise = rng.uniform(-0.1, 0.1, size=(Y, X)).astype(np.float32)
+        m = inner[0] + inner[1][k] <= 1.0
+        sl[m] = (0.03 * (1.0 + noise))[m]
     return vol
 
 
