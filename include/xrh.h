@@ -194,6 +194,25 @@ xrh_status xrh_line_int(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* c
                         int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
                         int32_t geom, float* drr_dev);
 
+/* xrh_line_int without any allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, semantics and DRRs as xrh_line_int; the
+ * pose, camera and tile tables go through a fixed staging area compiled into the library
+ * (about 0.8 MiB of device memory per GPU, held until the process ends) instead of the
+ * context's growable scratch, and the default DRR launch variant is used (tuned variants are
+ * not consulted). Limits: num_projs <= XRH_NOALLOC_MAX_PROJS, num_cams <=
+ * XRH_NOALLOC_MAX_CAMS; larger calls fail with XRH_ERR_INVALID before anything is queued.
+ * Calls are serialised within the process and each returns after its work has finished
+ * (synchronous), so overlapping calls never share the staging; its contents are those of the
+ * last call. */
+#define XRH_NOALLOC_MAX_PROJS 1024
+#define XRH_NOALLOC_MAX_CAMS 64
+xrh_status xrh_line_int_noalloc(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* cams,
+                                int32_t num_cams, const float* xforms_cam_to_phys,
+                                const int64_t* cam_for_proj, int64_t num_projs, float step,
+                                int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
+                                int32_t geom, float* drr_dev);
+
 /* Exact number of trilinear samples the reference algorithm takes for the same call (sum
  * over rays of n+1, the S_pose of the roofline byte model) and, optionally, how many of them
  * the kernel evaluates after empty-space skipping (the others lie where every voxel of their

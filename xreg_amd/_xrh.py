@@ -119,6 +119,8 @@ SIGNATURES = {
     "xrh_volume_info": (_I32, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "xrh_line_int": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
                             _I32, _P]),
+    "xrh_line_int_noalloc": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
+                                    _I32, _P]),
     "xrh_line_int_sample_count": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32,
                                          _PU64, _PU64]),
     "xrh_fixed_create": (_I32, [_P, _PF, _I32, _I32, C.POINTER(SimParams), _PU8, C.POINTER(_P)]),
@@ -298,6 +300,23 @@ def line_int(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_for_proj: 
     check(lib().xrh_line_int(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64), x.shape[0],
                              float(step), int(kernel), int(store), bg.ptr if bg else None, float(default_bg),
                              int(geom), drr.ptr))
+
+
+XRH_NOALLOC_MAX_PROJS = 1024
+XRH_NOALLOC_MAX_CAMS = 64
+
+
+def line_int_noalloc(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_for_proj: np.ndarray, step: float,
+                     drr: DeviceBuffer, kernel: int = XRH_LINE_INT_SUM, store: int = XRH_STORE_REPLACE,
+                     bg: DeviceBuffer | None = None, default_bg: float = 0.0, geom: int = XRH_GEOM_FP32) -> None:
+    """line_int through the fixed staging area (xrh_line_int_noalloc): nothing is allocated;
+    at most XRH_NOALLOC_MAX_PROJS projections and XRH_NOALLOC_MAX_CAMS cameras; synchronous."""
+    x = np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16)
+    c4p = np.ascontiguousarray(cam_for_proj, dtype=np.int64)
+    ca = _cams_array(cams)
+    check(lib().xrh_line_int_noalloc(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64), x.shape[0],
+                                     float(step), int(kernel), int(store), bg.ptr if bg else None,
+                                     float(default_bg), int(geom), drr.ptr))
 
 
 def line_int_sample_count(ctx: Context, vol: Volume, cams, xforms, cam_for_proj, step: float,

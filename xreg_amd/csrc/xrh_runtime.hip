@@ -512,13 +512,43 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
   *out = v;
 }
 
-// Packs PoseIn/CamDev/VolDesc/band tables into one pinned blob, uploads it and launches
-// the set-up + DRR kernels. Shared by xrh_line_int and its sample counter.
+// Band-major tile table of num_projs equal detectors into band_start ([bands][num_projs + 1],
+// the layout build_tiles produces); returns the largest band's tile count. Writes only into
+// the given array.
+static int64_t build_tiles_uniform(int32_t rows, int32_t cols, int64_t num_projs, int32_t bands,
+                                   uint32_t* band_start)
+{
+  const int TX = (cols + 15) / 16, TY = (rows + 15) / 16;
+  int64_t max_tiles = 0;
+  for (int b = 0; b < bands; ++b) {
+    const int r0 = (int)(((int64_t)b * TY) / bands);
+    const int r1 = (int)(((int64_t)(b + 1) * TY) / bands);
+    const uint32_t per = (uint32_t)((r1 - r0) * TX);
+    for (int64_t i = 0; i <= num_projs; ++i) band_start[(size_t)b * (num_projs + 1) + i] = (uint32_t)(i * per);
+    max_tiles = std::max<int64_t>(max_tiles, (int64_t)per * num_projs);
+  }
+  return max_tiles;
+}
+
+// A caller-provided staging area (xrh_line_int_noalloc): `host` for the packed tables, `dev`
+// for their device copy and the set-up kernel's items; both `bytes` long.
+struct FixedStaging {
+  unsigned char* host;
+  uint64_t host_bytes;
+  unsigned char* dev;
+  uint64_t dev_bytes;
+};
+
+// Packs PoseIn/CamDev/VolDesc/band tables into one blob, uploads it and launches the set-up +
+// DRR kernels. Shared by xrh_line_int, its sample counter and xrh_line_int_noalloc. Without
+// `fixed` the blob goes through the context's pinned staging and device scratch (grown on
+// demand) and a tuned launch variant applies; with `fixed` nothing is allocated: the blob
+// must fit the given area and the default variant is used.
 static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* cams,
                             int32_t num_cams, const float* xforms, const int64_t* cam_for_proj,
                             int64_t num_projs, float step, int32_t kernel, int32_t store,
                             const float* bg_dev, float default_bg, int32_t geom, float* drr_dev,
-                            uint64_t* samples_out)
+                            uint64_t* samples_out, const FixedStaging* fixed = nullptr)
 {
   XRH_REQUIRE(ctx && vol && cams && xforms && cam_for_proj, "null argument");
   XRH_REQUIRE(num_cams >= 1 && num_projs >= 1, "need at least one camera and one projection");
@@ -545,30 +575,39 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
 
   // a variant tuned for this launch shape (xrh_batch_autotune), else the defaults
   DrrVariant var;
-  {
+  if (!fixed) {
     const int64_t wgs = (int64_t)((rows + 15) / 16) * ((cols + 15) / 16) * num_projs;
     const std::string key = drr_tune_key(vol->desc.storage, vol->desc.layout, geom, kernel, rows, cols, wgs);
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) var = it->second;
   }
-  std::vector<int32_t> rr((size_t)num_projs, rows), cc((size_t)num_projs, cols);
-  const TilePlan plan = build_tiles(rr, cc, var.bands);
+  const int32_t bands = std::min(std::max(var.bands, 1), kMaxDrrBands);
 
   auto align = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
   const uint64_t off_pose = 0;
   const uint64_t off_cam = align(off_pose + sizeof(PoseIn) * num_projs);
   const uint64_t off_vol = align(off_cam + sizeof(CamDev) * num_cams);
   const uint64_t off_band = align(off_vol + sizeof(VolDesc));
-  const uint64_t off_c4p = align(off_band + sizeof(uint32_t) * plan.band_start.size());
+  const uint64_t off_c4p = align(off_band + sizeof(uint32_t) * (uint64_t)bands * (num_projs + 1));
   const uint64_t off_cnt = align(off_c4p + sizeof(int64_t) * num_projs);
   const uint64_t blob = align(off_cnt + 16);
   const uint64_t off_items = blob;
   const uint64_t total = align(off_items + sizeof(DrrItem) * num_projs);
 
   XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // staging reuse
-  char* host = (char*)ctx_pinned(ctx, blob);
-  char* dev = (char*)ctx_scratch(ctx, total);
+  char* host;
+  char* dev;
+  if (fixed) {
+    XRH_REQUIRE(blob <= fixed->host_bytes && total <= fixed->dev_bytes,
+                "projections and cameras exceed the fixed staging area");
+    host = (char*)fixed->host;
+    dev = (char*)fixed->dev;
+  } else {
+    host = (char*)ctx_pinned(ctx, blob);
+    dev = (char*)ctx_scratch(ctx, total);
+  }
+  const int64_t max_band_tiles = build_tiles_uniform(rows, cols, num_projs, bands, (uint32_t*)(host + off_band));
   PoseIn* pin = (PoseIn*)(host + off_pose);
   for (int64_t p = 0; p < num_projs; ++p) {
     std::memset(&pin[p], 0, sizeof(PoseIn));
@@ -582,7 +621,6 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   CamDev* cd = (CamDev*)(host + off_cam);
   for (int32_t c = 0; c < num_cams; ++c) cd[c] = make_cam_dev(cams[c]);
   std::memcpy(host + off_vol, &vol->desc, sizeof(VolDesc));
-  std::memcpy(host + off_band, plan.band_start.data(), sizeof(uint32_t) * plan.band_start.size());
   std::memcpy(host + off_c4p, cam_for_proj, sizeof(int64_t) * num_projs);
   std::memset(host + off_cnt, 0, 16);
   XRH_HIP_CHECK(hipMemcpyAsync(dev, host, blob, hipMemcpyHostToDevice, ctx->stream));
@@ -605,8 +643,8 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   L.items_dev = (DrrItem*)(dev + off_items);
   L.band_start_dev = (const uint32_t*)(dev + off_band);
   L.num_items = num_projs;
-  L.bands = plan.bands;
-  L.max_band_tiles = plan.max_band_tiles;
+  L.bands = bands;
+  L.max_band_tiles = max_band_tiles;
   L.geom = geom;
   L.kernel = kernel;
   L.store = (store == XRH_STORE_ACCUM || bg_dev) ? XRH_STORE_ACCUM : XRH_STORE_REPLACE;
@@ -623,6 +661,48 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
     XRH_HIP_CHECK(hipMemcpyAsync(samples_out, dev + off_cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
     XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fixed staging of xrh_line_int_noalloc. The device half is the file-scope array `scratch`
+// (every GPU the library runs on holds its own copy until the process ends); host code reaches
+// it only through the address hipGetSymbolAddress returns on that device, kept per device in
+// g_scratch_dev at context creation. The host half is a static array. One mutex serialises
+// the entry point within the process and each call waits for its work before releasing it, so
+// two calls never use a device's scratch at once.
+
+constexpr uint64_t na_align(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+// the packed tables for the largest call (the same offsets line_int_launch computes)
+constexpr uint64_t kNoAllocHostBytes =
+    na_align(na_align(na_align(na_align(na_align(sizeof(PoseIn) * XRH_NOALLOC_MAX_PROJS) +
+                                        sizeof(CamDev) * XRH_NOALLOC_MAX_CAMS) +
+                               sizeof(VolDesc)) +
+                      sizeof(uint32_t) * kMaxDrrBands * (XRH_NOALLOC_MAX_PROJS + 1)) +
+             sizeof(int64_t) * XRH_NOALLOC_MAX_PROJS) +
+    na_align(16);
+// ... followed on the device by the set-up kernel's per-projection items
+constexpr uint64_t kNoAllocDevBytes = na_align(kNoAllocHostBytes + sizeof(DrrItem) * XRH_NOALLOC_MAX_PROJS);
+static_assert(kNoAllocDevBytes <= (2u << 20), "fixed staging exceeds its stated 2 MiB bound");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocDevBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static unsigned char* g_scratch_dev[kNoAllocMaxDevices];
+static unsigned char g_noalloc_host[kNoAllocHostBytes];
+static std::mutex g_noalloc_mu;
+
+// Called with `device` current (context creation): records the device address of `scratch`.
+static void noalloc_init_device(int device)
+{
+  XRH_REQUIRE(device >= 0 && device < kNoAllocMaxDevices, "device index beyond the fixed staging table");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fail(XRH_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  }
+  g_scratch_dev[device] = (unsigned char*)p;
 }
 
 }  // namespace xrh
@@ -649,6 +729,12 @@ xrh_status xrh_ctx_create(int32_t device, uint64_t hbm_budget_bytes, xrh_ctx** o
     hipDeviceProp_t prop;
     XRH_HIP_CHECK(hipGetDeviceProperties(&prop, device));
     ctx->num_cus = prop.multiProcessorCount;
+    try {
+      noalloc_init_device(device);
+    } catch (...) {
+      delete ctx;
+      throw;
+    }
     const hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       delete ctx;
@@ -810,6 +896,32 @@ xrh_status xrh_line_int(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* c
   return guarded([&] {
     line_int_launch(ctx, vol, cams, num_cams, xforms_cam_to_phys, cam_for_proj, num_projs, step,
                     kernel, store, bg_dev, default_bg, geom, drr_dev, nullptr);
+  });
+}
+
+xrh_status xrh_line_int_noalloc(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camera* cams,
+                                int32_t num_cams, const float* xforms_cam_to_phys,
+                                const int64_t* cam_for_proj, int64_t num_projs, float step,
+                                int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
+                                int32_t geom, float* drr_dev)
+{
+  return guarded([&] {
+    XRH_REQUIRE(ctx, "null context");
+    XRH_REQUIRE(num_projs <= XRH_NOALLOC_MAX_PROJS,
+                "xrh_line_int_noalloc: " + std::to_string(num_projs) + " projections exceed the fixed staging "
+                "limit of " + std::to_string(XRH_NOALLOC_MAX_PROJS));
+    XRH_REQUIRE(num_cams <= XRH_NOALLOC_MAX_CAMS,
+                "xrh_line_int_noalloc: " + std::to_string(num_cams) + " cameras exceed the fixed staging "
+                "limit of " + std::to_string(XRH_NOALLOC_MAX_CAMS));
+    XRH_REQUIRE(ctx->device >= 0 && ctx->device < kNoAllocMaxDevices, "device index beyond the staging table");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    unsigned char* dev = g_scratch_dev[ctx->device];
+    XRH_REQUIRE(dev, "fixed staging of this device was not set up");
+    const FixedStaging st{g_noalloc_host, kNoAllocHostBytes, dev, kNoAllocDevBytes};
+    line_int_launch(ctx, vol, cams, num_cams, xforms_cam_to_phys, cam_for_proj, num_projs, step, kernel, store,
+                    bg_dev, default_bg, geom, drr_dev, nullptr, &st);
+    // the staging is free for the next call (of any context on any device) once this work ends
+    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
 }
 
