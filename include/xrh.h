@@ -240,6 +240,15 @@ xrh_status xrh_fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows,
                             xrh_fixed** out);
 xrh_status xrh_fixed_destroy(xrh_fixed* fixed);
 
+/* The patch list of a patch metric, as ImgSimMetric2DPatchCommon keeps it and its SimAux
+ * debug record stores it per compute call (xregImgSimMetric2DPatchCommon.h:49-61,
+ * xregImgSimMetric2DPatchNCCCPU.cpp:290-298, 443-485): inclusive start / stop rows and columns
+ * of each patch, rows of centres outer, and its weight (1, or the normalised unmasked fraction
+ * with a mask). *num_patches receives the count; with capacity 0 nothing else is written,
+ * otherwise capacity must hold it and any of the arrays may be NULL. Synchronous. */
+xrh_status xrh_fixed_patches(const xrh_fixed* fixed, int64_t capacity, int32_t* start_row, int32_t* start_col,
+                             int32_t* stop_row, int32_t* stop_col, float* weight, int64_t* num_patches);
+
 /* sim_host_out: num_mov floats (synchronises); sim_dev_out: optional device destination
  * (stream-ordered). At least one must be non-NULL. The moving images are not modified
  * (unlike the reference NCC, which zero-means them in place: xregImgSimMetric2DNCCCPU.h). */
@@ -286,6 +295,23 @@ xrh_status xrh_batch_set_options(xrh_batch* batch, int32_t use_graph, int32_t ti
  * order, each rows*cols of its view); NULL skips the copy-out when the metric allows. */
 xrh_status xrh_batch_eval(xrh_batch* batch, const int32_t* item_view, const double* item_pose,
                           int64_t n, float* sim_host_out, float* drr_dev_out);
+
+/* Several objects per item, as Intensity2D3DRegi::obj_fn casts a multi-object registration
+ * (xregIntensity2D3DRegi.cpp:571-629: the first volume with the REPLACE store -- onto the
+ * static volumes' background images when the registration has them, :593-625 -- the others
+ * with ACCUM, into the same projection, then the similarity). Entry i * num_objs + k of
+ * entry_view / entry_pose (FP64 row-major cam->phys) is object k of item i; the entries of
+ * one item share the detector size, and the first entry's view supplies the fixed image the
+ * item is scored against. Per pixel the DRR is the first object's line integral added onto
+ * the background (or 0) and the others' added in object order -- the float operations of a
+ * REPLACE cast followed by ACCUM casts, in one launch. bg_dev: NULL, or background images
+ * back to back in view order (rows x cols of each view, up to the last view an item's first
+ * object uses); the first object of an item is cast onto its view's image.
+ * n_items * num_objs <= max_items. Returns
+ * n_items similarity values (synchronises); drr_dev_out as in xrh_batch_eval. */
+xrh_status xrh_batch_eval_objects(xrh_batch* batch, const int32_t* entry_view, const double* entry_pose,
+                                  int64_t n_items, int32_t num_objs, const float* bg_dev,
+                                  float* sim_host_out, float* drr_dev_out);
 
 /* Asynchronous form for the serving loop: queues the same work and returns; the values
  * land in sim_host_out (pinned by the batch) once xrh_batch_wait returns. */

@@ -183,6 +183,43 @@ int main(int argc, char** argv)
     rec.put("short_sims", dev_sim->sim_vals());
     rc->set_num_projs(np);
 
+    // the patch metrics' debug record (aux_info, xregImgSimMetric2D.h:134-136): save_aux_info was
+    // set before the last compute, so one call's patch list is recorded (x and y for the
+    // gradient form)
+    {
+      uint64_t calls = 0, npatch = 0, bad = 0;
+      auto check_rec = [&](const std::shared_ptr<PatchNCCSimAux>& a) {
+        if (!a) {
+          ++bad;
+          return;
+        }
+        calls = a->patch_infos_per_compute_call.size();
+        bad += (a->patch_indices_per_compute_call.size() != calls);
+        for (size_t c = 0; c < calls; ++c) {
+          const auto& pl = a->patch_infos_per_compute_call[c];
+          const auto& ix = a->patch_indices_per_compute_call[c];
+          npatch = pl.size();
+          bad += (ix.size() != pl.size());
+          for (size_t q = 0; q < pl.size(); ++q) {
+            bad += (ix[q] != q) + (pl[q].stop_row - pl[q].start_row != 10) + (pl[q].stop_col - pl[q].start_col != 10);
+            bad += (pl[q].weight != 1.0f) + (pl[q].stop_row >= (size_type)rows) + (pl[q].stop_col >= (size_type)cols);
+          }
+        }
+      };
+      const auto aux = dev_sim->aux_info();
+      if (auto g = std::dynamic_pointer_cast<PatchGradNCCSimAux>(aux)) {
+        check_rec(g->sim_aux_x);
+        check_rec(g->sim_aux_y);
+      } else if (auto p1 = std::dynamic_pointer_cast<PatchNCCSimAux>(aux)) {
+        check_rec(p1);
+      } else if (aux) {
+        ++bad;
+      }
+      rec.put_u64("aux_calls", calls);
+      rec.put_u64("aux_patches", npatch);
+      rec.put_u64("aux_bad", bad);
+    }
+
     // a masked metric on the same projections (mask changes re-process the fixed image)
     auto mask_sim = make_metric(metric, smooth);
     mask_sim->set_fixed_image(fixed);

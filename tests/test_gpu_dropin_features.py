@@ -100,5 +100,12 @@ def test_dropin_features(tmp_path, oracle, metric, smooth):
     assert np.abs(f32("static_sims") - oracle.sim_f32(metric, fixed, ref_st, smooth_ksize=smooth)).max() \
         <= SIM_ABS_TOL
     assert u64("bg_restored") == 1
-    assert u64("iface_bad") == 0  # to_host_buf / to_ocl_buf throw, aux_info is empty
+    assert u64("iface_bad") == 0  # to_host_buf / to_ocl_buf throw, aux_info is empty before saving
+    # with save_aux_info the patch metrics record each compute call's patch list (r = 5,
+    # stride 1: (rows - 10) x (cols - 10) patches, rows of centres outer, weights 1)
+    if metric in (2, 3):
+        assert u64("aux_calls") == 1 and u64("aux_patches") == (rows - 10) * (cols - 10)
+    else:
+        assert u64("aux_calls") == 0
+    assert u64("aux_bad") == 0
     assert u64("max_num_projs_possible") >= npose

@@ -129,6 +129,7 @@ def test_registration_from_files(ctx, oracle, tmp_path):
     param = regi.PoseParam(init, w.center)
     obj = regi.Objective(ctx, vol, [(cam, img)], param, w.step, metric=3, max_batch=32)
     res = regi.run_cmaes(obj, lam=24, sigma=[np.deg2rad(2.0)] * 3 + [4.0] * 3, max_gens=40, seed=9)
+    sims_aux = obj.sims_aux()
     obj.close()
     vol.close()
     ext = np.array(w.vol.shape[::-1]) * w.spacing
@@ -137,7 +138,22 @@ def test_registration_from_files(ctx, oracle, tmp_path):
     e1 = G.mean_projection_distance(res.pose, w.gt, pts, 1020.0)
     print(f"mPD {e0:.3f} -> {e1:.3f} mm over {len(res.extra['iter_best'])} generations")
     assert e1 < 0.5 * e0
-    h5io.write_regi_debug(tmp_path / "dbg.h5", regi.debug_results(res, param, regi_time_secs=0.1))
+    h5io.write_regi_debug(tmp_path / "dbg.h5", regi.debug_results(res, param, regi_time_secs=0.1,
+                                                                  sims_aux=sims_aux))
+    # the optimiser's and the metric's records: every generation's population, values and
+    # covariance; one patch list per objective call (86 x 86 patches of the 96^2 detector)
+    ngen = len(res.extra["iter_best"])
+    aux = sims_aux[0]
+    assert aux["type"] == "patch-grad-ncc-aux" and aux["x"]["calls"] == ngen
+    nr, nc = cam.num_det_rows - 10, cam.num_det_cols - 10
+    assert len(aux["x"]["patches"]["weights"]) == nr * nc
+    assert list(aux["x"]["patches"]["start_rows"][[0, nc, nr * nc - 1]]) == [0, 1, nr - 1]
+    with h5io.open_file(tmp_path / "dbg.h5") as f:
+        with f.open_group("opt-aux") as ag:
+            assert ag.read("name", str) == "CMA-ES" and int(ag.scalar("num-iterations", np.uint64)) == ngen
+            assert ag.read("all-sim-vals", np.float32).shape == (ngen, 24)
+        with f.open_group("sims-aux") as sa, sa.open_group("000") as g0, g0.open_group("x") as gx:
+            assert f"{ngen - 1:04d}" in gx and f"{ngen:04d}" not in gx
     dbg = h5io.read_regi_debug(tmp_path / "dbg.h5")
     assert len(dbg.sims) == len(res.extra["iter_best"])
     assert np.allclose(dbg.final_poses[0], res.pose, atol=1e-4)
@@ -173,6 +189,13 @@ def test_multi_object_objective_and_registration(ctx, oracle):
                                   [regi.PoseParam(w.gt, w.center), regi.PoseParam(gt_b, w.center)], w.step,
                                   metric=X.XRH_SIM_GRAD_NCC, bg=bg if use_bg else None)
         got = obj.eval_poses([pa, pb])
+        # the reference's call pattern (one REPLACE cast, one ACCUM cast, one metric call per
+        # view) gives the same values as the one batched submission
+        seq = regi.MultiObjective(ctx, [A, B], list(zip(w.cams, fixed)),
+                                  [regi.PoseParam(w.gt, w.center), regi.PoseParam(gt_b, w.center)], w.step,
+                                  metric=X.XRH_SIM_GRAD_NCC, bg=bg if use_bg else None, batched=False)
+        assert np.array_equal(seq.eval_poses([pa, pb]), got), use_bg
+        seq.close()
         per_view = []
         for v, cam in enumerate(w.cams):
             d, _ = oracle.drr_f32(va, w.idx_to_phys, [cam], poses_f32(pa), step=w.step,

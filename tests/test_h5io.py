@@ -252,3 +252,66 @@ def test_regi_debug_round_trip(tmp_path):
         assert f.storage("sim-vals") == {"chunk": (37,), "deflate": 9}
         with f.open_group("init-poses") as g:
             assert g.storage("obj-001") == {"chunk": None, "deflate": None}
+
+
+def test_regi_debug_optimizer_and_metric_aux_groups(tmp_path):
+    """The optional "opt-aux" (CMA-ES and exhaustive records) and "sims-aux" (patch metrics)
+    groups of the debug record: names, on-disk types and shapes of the reference's writers
+    (xregIntensity2D3DRegiCMAES.cpp:437-509, xregIntensity2D3DRegiExhaustive.cpp:397-440,
+    xregImgSimMetric2DPatchNCCCPU.cpp:443-485, xregImgSimMetric2DPatchGradNCCCPU.cpp:411-420)."""
+    rng = np.random.default_rng(5)
+    nits, lam = 3, 4
+    cma = {"name": "CMA-ES", "pop_size": lam, "obj_dims": [6, 6],
+           "pop_params": [rng.standard_normal((lam, 12)) for _ in range(nits)],
+           "sim_vals": [rng.random(lam) for _ in range(nits)],
+           "cov_mats": [np.eye(12) * (i + 1) for i in range(nits)], "num_rejects": [0, 2, 1]}
+    patches = {"start_rows": np.array([0, 0, 1], np.int32), "start_cols": np.array([0, 1, 0], np.int32),
+               "stop_rows": np.array([2, 2, 3], np.int32), "stop_cols": np.array([2, 3, 2], np.int32),
+               "weights": np.array([1, 1, 1], np.float32)}
+    pn = {"type": "patch-ncc-aux", "calls": 2, "patches": patches, "patch_inds": np.arange(3, dtype=np.uint64)}
+    pg = {"type": "patch-grad-ncc-aux", "x": pn, "y": pn}
+    r = h5io.RegiDebugResults(
+        se3_params=OV.SE3OptVarsEuler(), vols_used=[0, 1], init_poses=[np.eye(4, dtype=np.float32)] * 2,
+        final_poses=[np.eye(4, dtype=np.float32)] * 2, inter_frames=[np.eye(4, dtype=np.float32)] * 2,
+        inter_frames_wrt_vol=[True, True], iter_vars=[rng.standard_normal((nits, 6)).astype(np.float32)] * 2,
+        sims=rng.random(nits).astype(np.float32), opt_aux=cma, sims_aux=[pg, None, pn])
+    path = tmp_path / "dbg_aux.h5"
+    h5io.write_regi_debug(path, r)
+    with h5io.open_file(path) as f:
+        with f.open_group("opt-aux") as ag:
+            assert ag.read("name", str) == "CMA-ES"
+            assert int(ag.scalar("num-iterations", np.uint64)) == nits and int(ag.scalar("pop-size", np.uint64)) == lam
+            sv = ag.read("all-sim-vals", np.float32)
+            assert sv.shape == (nits, lam) and np.allclose(sv, np.stack(cma["sim_vals"]), atol=1e-7)
+            with ag.open_group("all-pop-params") as pp, pp.open_group("iter-001") as it:
+                assert np.allclose(it.read("obj-001-pop", np.float32), cma["pop_params"][1][:, 6:], atol=1e-6)
+            with ag.open_group("covariances") as cg:
+                assert np.array_equal(cg.read("iter-002", np.float32), np.eye(12, dtype=np.float32) * 3)
+            assert list(ag.read("all-num-rejects", np.uint64)) == [0, 2, 1]
+        with f.open_group("sims-aux") as sa:
+            assert "001" not in sa
+            with sa.open_group("000") as g0:
+                assert g0.read("sim-aux-type", str) == "patch-grad-ncc-aux"
+                with g0.open_group("y") as gy, gy.open_group("0001") as c1:
+                    assert list(c1.read("stop-cols", np.uint64)) == [2, 3, 2]
+                    assert list(c1.read("patch-inds", np.uint64)) == [0, 1, 2]
+            with sa.open_group("002") as g2:
+                assert g2.read("sim-aux-type", str) == "patch-ncc-aux"
+                assert "0001" in g2 and "0002" not in g2
+    # the exhaustive record
+    ex = {"name": "Exhaustive", "cam_wrt_vols": [np.stack([np.eye(4)] * 3)], "has_min": True, "min_sim_val": 0.25,
+          "has_all_sim_vals": True, "min_sim_val_idx": 2, "all_sim_vals": np.array([0.5, 0.75, 0.25], np.float32)}
+    r.opt_aux, r.sims_aux = ex, []
+    h5io.write_regi_debug(tmp_path / "dbg_ex.h5", r)
+    with h5io.open_file(tmp_path / "dbg_ex.h5") as f, f.open_group("opt-aux") as ag:
+        assert ag.read("name", str) == "Exhaustive"
+        assert float(ag.scalar("min-sim-val", np.float32)) == 0.25 and int(ag.scalar("min-sim-val-idx", np.uint64)) == 2
+        with ag.open_group("cam-wrt-vols") as cw, cw.open_group("vol-00") as vg:
+            assert np.array_equal(vg.read("000002", np.float32), np.eye(4, dtype=np.float32))
+        assert "sims-aux" not in f
+    tool = _tool("h5dump")
+    if tool is not None:
+        txt = " ".join(subprocess.run([tool, "-H", str(path)], capture_output=True, text=True, check=True).stdout.split())
+        assert 'DATASET "all-sim-vals" { DATATYPE H5T_IEEE_F32LE DATASPACE SIMPLE { ( 3, 4 ) / ( 3, 4 ) }' in txt
+        assert 'DATASET "all-num-rejects" { DATATYPE H5T_STD_U64LE' in txt
+        assert 'DATASET "weights" { DATATYPE H5T_IEEE_F32LE' in txt and 'DATASET "start-rows" { DATATYPE H5T_STD_U64LE' in txt

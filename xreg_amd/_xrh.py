@@ -125,6 +125,7 @@ SIGNATURES = {
                                          _PU64, _PU64]),
     "xrh_fixed_create": (_I32, [_P, _PF, _I32, _I32, C.POINTER(SimParams), _PU8, C.POINTER(_P)]),
     "xrh_fixed_destroy": (_I32, [_P]),
+    "xrh_fixed_patches": (_I32, [_P, _I64, _PI32, _PI32, _PI32, _PI32, _PF, _PI64]),
     "xrh_sim_compute": (_I32, [_P, _P, _P, _I64, _P, _PF]),
     "xrh_batch_create": (_I32, [_P, C.POINTER(View), _I32, _I64, _I32, C.POINTER(_P)]),
     "xrh_batch_destroy": (_I32, [_P]),
@@ -134,6 +135,7 @@ SIGNATURES = {
     "xrh_batch_autotune": (_I32, [_P, _PI32, C.POINTER(C.c_double), C.c_int64, _I32, C.POINTER(TuneResult)]),
     "xrh_batch_set_drr_variant": (_I32, [_P, C.POINTER(DrrVariant)]),
     "xrh_batch_eval": (_I32, [_P, _PI32, _PD, _I64, _PF, _P]),
+    "xrh_batch_eval_objects": (_I32, [_P, _PI32, _PD, _I64, _I32, _P, _PF, _P]),
     "xrh_batch_launch": (_I32, [_P, _PI32, _PD, _I64]),
     "xrh_batch_wait": (_I32, [_P, _PF]),
     "xrh_batch_last_timing": (_I32, [_P, _PF, _PF]),
@@ -355,6 +357,19 @@ class Fixed:
                                     _fp(out)))
         return out
 
+    def patches(self) -> dict:
+        """The patch list of a patch metric (xrh_fixed_patches): start / stop rows and columns
+        (inclusive) and weights, rows of centres outer."""
+        n = C.c_int64(0)
+        check(lib().xrh_fixed_patches(self.h, 0, None, None, None, None, None, C.byref(n)))
+        a = {k: np.empty(n.value, np.int32) for k in ("start_rows", "start_cols", "stop_rows", "stop_cols")}
+        wt = np.empty(n.value, np.float32)
+        check(lib().xrh_fixed_patches(self.h, n.value, a["start_rows"].ctypes.data_as(_PI32),
+                                      a["start_cols"].ctypes.data_as(_PI32), a["stop_rows"].ctypes.data_as(_PI32),
+                                      a["stop_cols"].ctypes.data_as(_PI32), _fp(wt), C.byref(n)))
+        a["weights"] = wt
+        return a
+
     def close(self) -> None:
         if self.h:
             check(lib().xrh_fixed_destroy(self.h))
@@ -400,6 +415,22 @@ class Batch:
         out = np.empty(iv.shape[0], dtype=np.float32)
         check(lib().xrh_batch_eval(self.h, iv.ctypes.data_as(_PI32), _dp(ip), iv.shape[0], _fp(out),
                                    drr.ptr if drr else None))
+        return out
+
+    def eval_objects(self, entry_view: np.ndarray, entry_pose: np.ndarray, bg: DeviceBuffer | None = None,
+                     drr: DeviceBuffer | None = None) -> np.ndarray:
+        """Several objects per item (xrh_batch_eval_objects): entry_view is (n_items, num_objs)
+        view indices, entry_pose (n_items, num_objs, 4, 4) poses; object 0 of an item is cast onto
+        its view's background image in `bg` (one image per view, back to back), the others added
+        in order. Returns n_items similarity values."""
+        ev = np.ascontiguousarray(entry_view, dtype=np.int32)
+        if ev.ndim != 2:
+            raise ValueError("entry_view must be (n_items, num_objs)")
+        n, k = ev.shape
+        ep = np.ascontiguousarray(entry_pose, dtype=np.float64).reshape(n * k, 16)
+        out = np.empty(n, dtype=np.float32)
+        check(lib().xrh_batch_eval_objects(self.h, ev.ctypes.data_as(_PI32), _dp(ep), n, k,
+                                           bg.ptr if bg else None, _fp(out), drr.ptr if drr else None))
         return out
 
     def launch(self, item_view: np.ndarray, item_pose: np.ndarray) -> None:

@@ -572,6 +572,40 @@ void ImgSimMetric2DHIP::compute()
   }
   sim_vals_.resize(num_mov_imgs_);
   check_xrh(xrh_sim_compute(ctx_, fixed_, mov, (int64_t)num_mov_imgs_, nullptr, sim_vals_.data()));
+  if (save_aux_info_ && (params_.metric == XRH_SIM_PATCH_NCC || params_.metric == XRH_SIM_PATCH_GRAD_NCC)) {
+    // ImgSimMetric2DPatchNCCCPU::compute with save_aux_info (xregImgSimMetric2DPatchNCCCPU.cpp:
+    // 290-298): every call appends the patch list and the indices used (all: random-patch mode
+    // is off); the gradient form keeps the same for x and y (xregImgSimMetric2DPatchGradNCCCPU.cpp:
+    // 246-252)
+    int64_t n = 0;
+    check_xrh(xrh_fixed_patches(fixed_, 0, nullptr, nullptr, nullptr, nullptr, nullptr, &n));
+    std::vector<int32_t> r0((size_t)n), c0((size_t)n), r1((size_t)n), c1((size_t)n);
+    std::vector<float> w((size_t)n);
+    check_xrh(xrh_fixed_patches(fixed_, n, r0.data(), c0.data(), r1.data(), c1.data(), w.data(), &n));
+    PatchInfoList pl((size_t)n);
+    PatchIndexList inds((size_t)n);
+    for (int64_t q = 0; q < n; ++q) {
+      pl[(size_t)q] = {(size_type)r0[(size_t)q], (size_type)c0[(size_t)q], (size_type)r1[(size_t)q],
+                       (size_type)c1[(size_t)q], w[(size_t)q]};
+      inds[(size_t)q] = (size_type)q;
+    }
+    auto append = [&](std::shared_ptr<PatchNCCSimAux>& a) {
+      if (!a) a = std::make_shared<PatchNCCSimAux>();
+      a->patch_infos_per_compute_call.push_back(pl);
+      a->patch_indices_per_compute_call.push_back(inds);
+    };
+    if (params_.metric == XRH_SIM_PATCH_NCC) {
+      auto a = std::static_pointer_cast<PatchNCCSimAux>(sim_aux_);
+      append(a);
+      sim_aux_ = a;
+    } else {
+      auto g = std::static_pointer_cast<PatchGradNCCSimAux>(sim_aux_);
+      if (!g) g = std::make_shared<PatchGradNCCSimAux>();
+      append(g->sim_aux_x);
+      append(g->sim_aux_y);
+      sim_aux_ = g;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------

@@ -106,7 +106,34 @@ void check_xrh(xrh_status s);  // non-zero status -> exception (Unsupported / St
 // HIP classes do not implement (the device buffer hand-off replaces the sync objects).
 class RayCastSyncOCLBuf;
 class RayCastSyncHostBuf;
-class H5ReadWriteInterface;
+// xReg's H5ReadWriteInterface (lib/hdf5/xregHDF5ReadWriteInterface.h): the records below
+// derive from it; in the xReg tree their write() / read() are the reference's writers over the
+// same fields (this repo has no HDF5 C++ API; xreg_amd.h5io writes the same layout).
+class H5ReadWriteInterface {
+ public:
+  virtual ~H5ReadWriteInterface() = default;
+};
+
+// ImgSimMetric2DPatchCommon::PatchInfo (xregImgSimMetric2DPatchCommon.h:49-61): inclusive patch
+// bounds and weight.
+struct PatchInfo {
+  size_type start_row, start_col, stop_row, stop_col;
+  float weight;
+};
+using PatchInfoList = std::vector<PatchInfo>;
+using PatchIndexList = std::vector<size_type>;
+
+// ImgSimMetric2DPatchNCCCPU::SimAux (xregImgSimMetric2DPatchNCCCPU.h:57-65): per compute call,
+// the patch list and the indices of the patches used.
+struct PatchNCCSimAux : public H5ReadWriteInterface {
+  std::vector<PatchInfoList> patch_infos_per_compute_call;
+  std::vector<PatchIndexList> patch_indices_per_compute_call;
+};
+
+// ImgSimMetric2DPatchGradNCCCPU::SimAux: the x- and y-gradient records.
+struct PatchGradNCCSimAux : public H5ReadWriteInterface {
+  std::shared_ptr<PatchNCCSimAux> sim_aux_x, sim_aux_y;
+};
 
 class RayCaster {
  public:
@@ -294,8 +321,8 @@ class ImgSimMetric2D {
   void set_mask(ImageMaskPtr m);
   ImageMaskPtr mask() { return mask_; }
   void set_save_aux_info(bool s) { save_aux_info_ = s; }
-  // ImgSimMetric2D::aux_info (xregImgSimMetric2D.cpp:99-102): none by default; the HIP metrics
-  // keep no per-patch debug record (the CPU patch metrics' override)
+  // ImgSimMetric2D::aux_info (xregImgSimMetric2D.cpp:99-102): none by default; the patch
+  // metrics return their per-compute-call patch record when save_aux_info is set
   virtual std::shared_ptr<H5ReadWriteInterface> aux_info() { return nullptr; }
 
  protected:
@@ -318,9 +345,12 @@ class ImgSimMetric2DHIP : public ImgSimMetric2D {
   void set_mov_imgs_buf_from_ray_caster(RayCaster* rc, size_type proj_offset = 0) override;
   void set_mov_imgs_host_buf(Scalar* buf, size_type proj_offset = 0) override;
   const xrh_fixed* fixed_handle() const { return fixed_; }
+  // the patch metrics' record (PatchNCCSimAux / PatchGradNCCSimAux), else null
+  std::shared_ptr<H5ReadWriteInterface> aux_info() override { return sim_aux_; }
 
  protected:
   xrh_sim_params params_{};
+  std::shared_ptr<H5ReadWriteInterface> sim_aux_;
 
  private:
   xrh_ctx* ctx_;

@@ -268,31 +268,65 @@ std::string tune_key(const xrh_batch* b, const std::vector<int32_t>& rows, const
   return drr_tune_key(b->storage, b->layout, b->geom, b->views[0].kernel, mr, mc, wgs);
 }
 
-// Fills the pinned staging for n items; returns the DRR launch description.
+// Fills the pinned staging for n items of `objs` entries each (entry i * objs + k is object k of
+// item i; one entry per item in the single-object case); returns the DRR launch description.
+// bg: NULL, or background images of the views back to back in view order, the first object of
+// an item being cast onto its view's image.
 DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, float* drr_out,
-                std::vector<std::pair<int, std::vector<int64_t>>>& groups)
+                std::vector<std::pair<int, std::vector<int64_t>>>& groups, int32_t objs = 1,
+                const float* bg = nullptr)
 {
   XRH_REQUIRE(!b->views.empty(), "batch has no views");
+  XRH_REQUIRE(objs >= 1 && n * objs <= b->max_items, "items x objects exceed the batch capacity (max_items)");
   const BlobLayout L = blob_layout(b->max_items);
   PoseIn* pin = (PoseIn*)(b->blob_host + L.off_pose);
   SimItem* sin = (SimItem*)(b->blob_host + L.off_sim);
   float* drr = drr_out ? drr_out : b->drr_dev;
   std::vector<int32_t> rows((size_t)n), cols((size_t)n);
+  std::vector<int64_t> bg_off;
+  if (bg) {
+    // view v's image starts after those of views 0..v-1; the buffer must reach the end of the
+    // last image an item's first object uses
+    int64_t o = 0, need = 0;
+    for (const xrh_view& w : b->views) {
+      bg_off.push_back(o);
+      o += (int64_t)w.cam.num_det_rows * w.cam.num_det_cols;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t v = item_view[i * objs];
+      XRH_REQUIRE(v >= 0 && v < (int32_t)b->views.size(), "item view index out of range");
+      need = std::max<int64_t>(need, bg_off[(size_t)v] + (int64_t)b->views[v].cam.num_det_rows *
+                                                             b->views[v].cam.num_det_cols);
+    }
+    ctx_check_range(b->ctx, bg, (uint64_t)need * 4, "background images");
+  }
+  for (int64_t e = 0; e < n * objs; ++e) {
+    const int32_t v = item_view[e];
+    XRH_REQUIRE(v >= 0 && v < (int32_t)b->views.size(), "item view index out of range");
+    const xrh_view& w = b->views[v];
+    const xrh_view& w0 = b->views[item_view[e - e % objs]];
+    XRH_REQUIRE(w.cam.num_det_rows == w0.cam.num_det_rows && w.cam.num_det_cols == w0.cam.num_det_cols,
+                "the objects of one item must share the detector size");
+  }
   int64_t off = 0;
   groups.clear();
   for (int64_t i = 0; i < n; ++i) {
-    const int32_t v = item_view[i];
-    XRH_REQUIRE(v >= 0 && v < (int32_t)b->views.size(), "item view index out of range");
+    const int32_t v = item_view[i * objs];
     const xrh_view& w = b->views[v];
     const FixedDev& f = w.fixed->host;
-    PoseIn& p = pin[i];
-    std::memcpy(p.pose, item_pose + 16 * i, sizeof(double) * 16);
-    p.cam = v;
-    p.vol = b->view_vol[v];
-    p.out_off = off;
-    p.ncc_zf = (f.metric == XRH_SIM_NCC && b->fuse_ncc) ? f.zf : nullptr;
-    p.step = w.step;
-    p.pad_ = 0;
+    for (int32_t k = 0; k < objs; ++k) {
+      const int64_t e = i * objs + k;
+      const int32_t vk = item_view[e];
+      PoseIn& p = pin[e];
+      std::memcpy(p.pose, item_pose + 16 * e, sizeof(double) * 16);
+      p.cam = vk;
+      p.vol = b->view_vol[vk];
+      p.out_off = off;
+      p.ncc_zf = (k == 0 && f.metric == XRH_SIM_NCC && b->fuse_ncc) ? f.zf : nullptr;
+      p.bg = (k == 0 && bg) ? bg + bg_off[(size_t)v] : nullptr;
+      p.step = b->views[vk].step;
+      p.pad_ = 0;
+    }
     rows[i] = w.cam.num_det_rows;
     cols[i] = w.cam.num_det_cols;
     SimItem& s = sin[i];
@@ -307,7 +341,7 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
   if (drr_out) ctx_check_range(b->ctx, drr_out, (uint64_t)off * 4, "DRR output");
   // items grouped by launch key (metric, then kernel variant), in item order within a group
   std::vector<int> keys;
-  for (int64_t i = 0; i < n; ++i) keys.push_back(sim_group_key(b->views[item_view[i]].fixed->host));
+  for (int64_t i = 0; i < n; ++i) keys.push_back(sim_group_key(b->views[item_view[i * objs]].fixed->host));
   std::vector<int> distinct(keys);
   std::sort(distinct.begin(), distinct.end());
   distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
@@ -358,6 +392,7 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
   D.ncc_tiles_per_item = b->max_img_tiles;
   D.reuse = var.reuse;
   D.wave_shape = var.shape;
+  D.objs = objs;
   return D;
 }
 
@@ -385,16 +420,20 @@ void enqueue(xrh_batch* b, const DrrLaunch& D, const std::vector<std::pair<int, 
   XRH_HIP_CHECK(hipMemcpyAsync(b->sims_host, b->sims_dev, sizeof(float) * n, hipMemcpyDeviceToHost, s));
 }
 
-std::string signature(const xrh_batch* b, const int32_t* item_view, int64_t n, const float* drr_out)
+std::string signature(const xrh_batch* b, const int32_t* item_view, int64_t n, const float* drr_out,
+                      int32_t objs, const float* bg)
 {
-  std::string key((const char*)item_view, (size_t)n * sizeof(int32_t));
+  std::string key((const char*)item_view, (size_t)(n * objs) * sizeof(int32_t));
+  key.append((const char*)&objs, sizeof(objs));
+  key.append((const char*)&bg, sizeof(bg));
   key.append((const char*)&drr_out, sizeof(drr_out));
   key.append((const char*)&b->version, sizeof(b->version));
   key.append((const char*)&b->last_variant, sizeof(b->last_variant));
   return key;
 }
 
-void launch(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, float* drr_out)
+void launch(xrh_batch* b, const int32_t* item_view, const double* item_pose, int64_t n, float* drr_out,
+            int32_t objs = 1, const float* bg = nullptr)
 {
   XRH_REQUIRE(b && item_view && item_pose, "null argument");
   XRH_REQUIRE(n >= 1 && n <= b->max_items, "item count must be in [1, max_items]");
@@ -402,12 +441,12 @@ void launch(xrh_batch* b, const int32_t* item_view, const double* item_pose, int
   hipStream_t s = b->ctx->stream;
   XRH_HIP_CHECK(hipStreamSynchronize(s));  // pinned staging is reused
   std::vector<std::pair<int, std::vector<int64_t>>> groups;
-  const DrrLaunch D = stage(b, item_view, item_pose, n, drr_out, groups);
+  const DrrLaunch D = stage(b, item_view, item_pose, n, drr_out, groups, objs, bg);
   const bool timed = b->time_kernels || !b->use_graph;
   b->timed_last = timed;
   if (timed) XRH_HIP_CHECK(hipEventRecord(b->ev_all0, s));
   if (b->use_graph && !b->time_kernels) {
-    const std::string key = signature(b, item_view, n, drr_out);
+    const std::string key = signature(b, item_view, n, drr_out, objs, bg);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       evict_graphs(b, kMaxCachedGraphs - 1);
@@ -641,6 +680,18 @@ xrh_status xrh_batch_eval(xrh_batch* batch, const int32_t* item_view, const doub
 {
   return guarded([&] {
     launch(batch, item_view, item_pose, n, drr_dev_out);
+    wait(batch, sim_host_out);
+  });
+}
+
+xrh_status xrh_batch_eval_objects(xrh_batch* batch, const int32_t* entry_view, const double* entry_pose,
+                                  int64_t n_items, int32_t num_objs, const float* bg_dev, float* sim_host_out,
+                                  float* drr_dev_out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(num_objs >= 1 && num_objs <= 64, "objects per item must be in [1, 64]");
+    XRH_REQUIRE(n_items >= 1, "need at least one item");
+    launch(batch, entry_view, entry_pose, n_items, drr_dev_out, num_objs, bg_dev);
     wait(batch, sim_host_out);
   });
 }

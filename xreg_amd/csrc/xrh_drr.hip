@@ -134,7 +134,7 @@ __device__ inline bool ray_rect(const T mx[3], const T p0[3], const T d[3], T& t
 }
 
 struct Ray {
-  float sx, sy, sz;  // first sample (continuous index)
+  float sx, sy, sz;  // first sample (continuous index; FP64 set-up: padded-grid index, +1 per axis)
   float dx, dy, dz;  // step vector (index units)
   int n;             // last sample index (n+1 samples), -1 = ray misses
 };
@@ -227,9 +227,10 @@ __device__ inline Ray setup_ray_f64(const DrrItem& it, int col, int row)
     const double s = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
     ray.n = (int)(L / s);
     const double scale = s / dlen;
-    ray.sx = (float)(ph[0] + t0 * d[0]);
-    ray.sy = (float)(ph[1] + t0 * d[1]);
-    ray.sz = (float)(ph[2] + t0 * d[2]);
+    // start in the padded grid's index frame (+1 per axis, the apron), rounded once
+    ray.sx = (float)((ph[0] + t0 * d[0]) + 1.0);
+    ray.sy = (float)((ph[1] + t0 * d[1]) + 1.0);
+    ray.sz = (float)((ph[2] + t0 * d[2]) + 1.0);
     ray.dx = (float)(d[0] * scale);
     ray.dy = (float)(d[1] * scale);
     ray.dz = (float)(d[2] * scale);
@@ -281,7 +282,26 @@ __device__ inline int cvt_floor(float x)
   return r;
 }
 
-template <int kFmt>
+// a * b + c for 24-bit unsigned a, b as one V_MAD_U32_U24 (the compiler otherwise forms two
+// multiplies and a three-way add)
+__device__ inline uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+{
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// the same for signed 24-bit a and c (V_MAD_I32_I24)
+__device__ inline int32_t mad_i24(int32_t a, int32_t b, int32_t c)
+{
+  int32_t r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// kPadded: the position is already in the padded grid's index frame (FP64 ray set-up), so the
+// element index needs no apron offset: two 24-bit multiply-adds.
+template <int kFmt, bool kPadded>
 __device__ inline float sample(VolView& v, float x, float y, float z)
 {
   // x - floor(x) (V_FRACT_F32) and floor(x) as int: two instructions per axis
@@ -293,12 +313,16 @@ __device__ inline float sample(VolView& v, float x, float y, float z)
   if constexpr (kFmt == 12 || kFmt == 13) {
     // bricked x-faces: padded cell (ix+1, iy+1, iz+1), brick (./4) in brick order with the
     // brick pitches in py / pz, 5x4x4 faces per brick (x fastest, the last x-face repeated)
-    const uint32_t ax = (uint32_t)(ix + 1), ay = (uint32_t)(iy + 1), az = (uint32_t)(iz + 1);
+    const uint32_t ax = (uint32_t)(kPadded ? ix : ix + 1), ay = (uint32_t)(kPadded ? iy : iy + 1),
+                   az = (uint32_t)(kPadded ? iz : iz + 1);
     const uint32_t brick = __umul24(az >> 2, v.pz) + __umul24(ay >> 2, v.py) + (ax >> 2);
     const uint32_t local = __umul24(az & 3u, 20u) + __umul24(ay & 3u, 5u) + (ax & 3u);
     e = __umul24(brick, (uint32_t)kBrickFaces) + local;
+  } else if constexpr (kPadded) {
+    // padded indices are >= 0 for every position within a voxel of the index box
+    e = mad_u24((uint32_t)iz, v.pz, mad_u24((uint32_t)iy, v.py, (uint32_t)ix));
   } else {
-    e = (uint32_t)(__mul24(iz, (int)v.pz) + __mul24(iy, (int)v.py) + ix) + v.apron;
+    e = (uint32_t)mad_i24(iz, (int32_t)v.pz, mad_i24(iy, (int32_t)v.py, ix)) + v.apron;
   }
   if constexpr (kFmt == 0) {
     const int o = (int)(e * 4u);
@@ -412,14 +436,14 @@ __device__ inline float sample(VolView& v, float x, float y, float z)
 // sample per iteration, so at most |end - i| + 1 iterations). Positions here are start +
 // i*step in float; they differ from the accumulated ones by far less than the one-voxel
 // dilation. Returns the first sample (in walk order) that may be non-zero.
-__device__ inline int skip_empty(const DrrItem& it, const Ray& r, int i, int end, int dir)
+__device__ inline int skip_empty(const DrrItem& it, const Ray& r, int i, int end, int dir, float org)
 {
   const float B = (float)(1 << it.skip_shift);
   const float vx = dir > 0 ? r.dx : -r.dx, vy = dir > 0 ? r.dy : -r.dy, vz = dir > 0 ? r.dz : -r.dz;
   const float rx = __builtin_amdgcn_rcpf(vx), ry = __builtin_amdgcn_rcpf(vy), rz = __builtin_amdgcn_rcpf(vz);
   while (dir > 0 ? (i <= end) : (i >= end)) {
     const float fi = (float)i;
-    const float px = fmaf(fi, r.dx, r.sx), py = fmaf(fi, r.dy, r.sy), pz = fmaf(fi, r.dz, r.sz);
+    const float px = fmaf(fi, r.dx, r.sx) - org, py = fmaf(fi, r.dy, r.sy) - org, pz = fmaf(fi, r.dz, r.sz) - org;
     const int bx = min(max(cvt_floor(px) >> it.skip_shift, 0), it.skip_gx - 1);
     const int by = min(max(cvt_floor(py) >> it.skip_shift, 0), it.skip_gy - 1);
     const int bz = min(max(cvt_floor(pz) >> it.skip_shift, 0), it.skip_gz - 1);
@@ -440,15 +464,16 @@ __device__ inline int skip_empty(const DrrItem& it, const Ray& r, int i, int end
 }
 
 // Sample range [first, last] that can hold non-zero values (SUM with an empty-space map), or
-// the whole ray [0, n].
-template <int kOp>
+// the whole ray [0, n]. FP64 set-up rays start in the padded frame (org 1).
+template <int kOp, int kGeom>
 __device__ inline void sample_range(const DrrItem& it, const Ray& r, int& first, int& last)
 {
   first = 0;
   last = r.n;
   if (kOp == XRH_LINE_INT_SUM && it.skip) {
-    first = skip_empty(it, r, 0, r.n, 1);
-    last = (first <= r.n) ? skip_empty(it, r, r.n, first, -1) : first - 1;
+    const float org = (kGeom == XRH_GEOM_FP64) ? 1.0f : 0.0f;
+    first = skip_empty(it, r, 0, r.n, 1, org);
+    last = (first <= r.n) ? skip_empty(it, r, r.n, first, -1, org) : first - 1;
   }
 }
 
@@ -470,7 +495,7 @@ struct TileRef {
 };
 
 __device__ inline TileRef decode_tile(const uint32_t* band_start, int64_t num_items, int bands,
-                                      const DrrItem* items)
+                                      const DrrItem* items, int stride = 1)
 {
   TileRef t;
   const uint32_t b = blockIdx.x;
@@ -487,7 +512,7 @@ __device__ inline TileRef decode_tile(const uint32_t* band_start, int64_t num_it
   }
   t.item = lo;
   const uint32_t local = j - bs[lo];
-  const int TX = items[lo].tiles_x, TY = items[lo].tiles_y;
+  const int TX = items[lo * stride].tiles_x, TY = items[lo * stride].tiles_y;
   const int row0 = (int)(((int64_t)band * TY) / bands);
   t.ty = row0 + (int)(local / (uint32_t)TX);
   t.tx = (int)(local % (uint32_t)TX);
@@ -560,6 +585,7 @@ __global__ void drr_setup_kernel(const PoseIn* __restrict__ poses, const VolDesc
   it.step = p.step;
   it.out_off = p.out_off;
   it.ncc_zf = p.ncc_zf;
+  it.bg = p.bg;
   it.tiles_x = (it.cam.cols + kTile - 1) / kTile;
   it.tiles_y = (it.cam.rows + kTile - 1) / kTile;
   it.skip = v.skip;
@@ -590,7 +616,7 @@ __device__ inline float march(const Ray& ray, VolView& vv, int i, int last, floa
       // in float (exact below 2^24) so the loop test is one float compare
       const float flast = (float)last;
       for (float fi = (float)i; fi <= flast; fi = fi + 1.0f) {
-        acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
+        acc = line_op<kOp>(acc, sample<kFmt, false>(vv, px, py, pz));
         px = px + ray.dx;
         py = py + ray.dy;
         pz = pz + ray.dz;
@@ -598,19 +624,19 @@ __device__ inline float march(const Ray& ray, VolView& vv, int i, int last, floa
     } else {
       for (; i < last; i += 2) {
         const float qx = px + ray.dx, qy = py + ray.dy, qz = pz + ray.dz;
-        const float s0 = sample<kFmt>(vv, px, py, pz);
-        const float s1 = sample<kFmt>(vv, qx, qy, qz);
+        const float s0 = sample<kFmt, false>(vv, px, py, pz);
+        const float s1 = sample<kFmt, false>(vv, qx, qy, qz);
         acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
         px = qx + ray.dx;
         py = qy + ray.dy;
         pz = qz + ray.dz;
       }
-      if (i == last) acc = line_op<kOp>(acc, sample<kFmt>(vv, px, py, pz));
+      if (i == last) acc = line_op<kOp>(acc, sample<kFmt, false>(vv, px, py, pz));
     }
   } else if constexpr (kFmt == 7 || kFmt == 11 || kFmt == 13) {
     const float flast = (float)last;
     for (float f0 = (float)i; f0 <= flast; f0 = f0 + 1.0f) {
-      acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+      acc = line_op<kOp>(acc, sample<kFmt, true>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
                                               fmaf(f0, ray.dz, ray.sz)));
     }
   } else {
@@ -618,38 +644,27 @@ __device__ inline float march(const Ray& ray, VolView& vv, int i, int last, floa
     float f0 = (float)i;
     for (; i < last; i += 2) {
       const float f1 = f0 + 1.0f;
-      const float s0 = sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+      const float s0 = sample<kFmt, true>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
                                     fmaf(f0, ray.dz, ray.sz));
-      const float s1 = sample<kFmt>(vv, fmaf(f1, ray.dx, ray.sx), fmaf(f1, ray.dy, ray.sy),
+      const float s1 = sample<kFmt, true>(vv, fmaf(f1, ray.dx, ray.sx), fmaf(f1, ray.dy, ray.sy),
                                     fmaf(f1, ray.dz, ray.sz));
       acc = line_op<kOp>(line_op<kOp>(acc, s0), s1);
       f0 = f0 + 2.0f;
     }
     if (i == last) {
       const float f0 = (float)i;
-      acc = line_op<kOp>(acc, sample<kFmt>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
+      acc = line_op<kOp>(acc, sample<kFmt, true>(vv, fmaf(f0, ray.dx, ray.sx), fmaf(f0, ray.dy, ray.sy),
                                               fmaf(f0, ray.dz, ray.sz)));
     }
   }
   return acc;
 }
 
-// Pixel store (REPLACE / ACCUM with the line kernel, RayCasterCPU::pre_compute semantics) and
-// the fused NCC epilogue: FP64 sums of m, m^2 and m * zero-mean fixed, one record per tile.
-// Every thread of the block calls it (it synchronises).
-template <int kOp>
-__device__ inline void store_pixel(const DrrItem& it, const TileRef& tr, bool inside, int col, int row, float acc,
-                                   float* __restrict__ out, int accumulate, float fill,
-                                   double* __restrict__ ncc_partials, int ncc_tiles)
+// The fused NCC epilogue: FP64 sums of m, m^2 and m * zero-mean fixed over the tile, one
+// record per tile. Every thread of the block calls it (it synchronises).
+__device__ inline void ncc_epilogue(const DrrItem& it, const TileRef& tr, bool inside, int col, int row, float pix,
+                                    double* __restrict__ ncc_partials, int ncc_tiles)
 {
-  float pix = 0.0f;
-  if (inside) {
-    const int64_t o = it.out_off + (int64_t)row * it.cam.cols + col;
-    const float prev = accumulate ? out[o] : fill;
-    const float aa = 0.0f + acc * 1.0f;
-    if constexpr (kOp == XRH_LINE_INT_SUM) pix = prev + aa; else pix = (prev < aa) ? aa : prev;
-    out[o] = pix;
-  }
   if (ncc_partials && it.ncc_zf) {
     // NCC epilogue: sums of m, m^2 and m * zero-mean fixed, FP64, one record per tile
     double s1 = 0.0, s2 = 0.0, sf = 0.0;
@@ -681,6 +696,47 @@ __device__ inline void store_pixel(const DrrItem& it, const TileRef& tr, bool in
   }
 }
 
+// Pixel store (REPLACE / ACCUM with the line kernel, RayCasterCPU::pre_compute semantics) and
+// the fused NCC epilogue. Every thread of the block calls it (it synchronises).
+template <int kOp>
+__device__ inline void store_pixel(const DrrItem& it, const TileRef& tr, bool inside, int col, int row, float acc,
+                                   float* __restrict__ out, int accumulate, float fill,
+                                   double* __restrict__ ncc_partials, int ncc_tiles)
+{
+  float pix = 0.0f;
+  if (inside) {
+    const int64_t o = it.out_off + (int64_t)row * it.cam.cols + col;
+    const float prev = accumulate ? out[o] : fill;
+    const float aa = 0.0f + acc * 1.0f;
+    if constexpr (kOp == XRH_LINE_INT_SUM) pix = prev + aa; else pix = (prev < aa) ? aa : prev;
+    out[o] = pix;
+  }
+  ncc_epilogue(it, tr, inside, col, row, pix, ncc_partials, ncc_tiles);
+}
+
+// One ray of one item: the line integral (SUM) or maximum (MAX) of its samples times the step,
+// or the kernel's initial value when the ray misses the volume.
+template <int kGeom, int kOp, int kFmt>
+__device__ inline float cast_ray(const DrrItem& it, int col, int row)
+{
+  float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
+  const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
+  if (ray.n >= 0) {
+    VolView vv;
+    vv.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(it.vol), (short)0,
+                                                (int)it.vol_bytes, 0x00020000);
+    vv.py = it.pitch_y;
+    vv.pz = it.pitch_z;
+    vv.apron = 1u + it.pitch_y + it.pitch_z;
+    vv.ce = 0xFFFFFFFFu;
+    int i, last;
+    sample_range<kOp, kGeom>(it, ray, i, last);
+    acc = march<kGeom, kOp, kFmt>(ray, vv, i, last, acc);
+    acc = acc * it.step;
+  }
+  return acc;
+}
+
 template <int kGeom, int kOp, int kFmt>
 __global__ __launch_bounds__(kThreads) void drr_kernel(
     const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
@@ -693,27 +749,40 @@ __global__ __launch_bounds__(kThreads) void drr_kernel(
   int col, row;
   lane_pixel(tr.tx, tr.ty, shape, col, row);
   const bool inside = (col < it.cam.cols) && (row < it.cam.rows);
-
-  float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
-  if (inside) {
-    const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
-    if (ray.n >= 0) {
-      VolView vv;
-      vv.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(it.vol), (short)0,
-                                                  (int)it.vol_bytes, 0x00020000);
-      vv.py = it.pitch_y;
-      vv.pz = it.pitch_z;
-      vv.apron = 1u + it.pitch_y + it.pitch_z;
-      vv.ce = 0xFFFFFFFFu;
-      // two samples per iteration (loads of both issue before either is interpolated);
-      // samples are still accumulated one after the other, in ray order
-      int i, last;
-      sample_range<kOp>(it, ray, i, last);
-      acc = march<kGeom, kOp, kFmt>(ray, vv, i, last, acc);
-      acc = acc * it.step;
-    }
-  }
+  const float acc = inside ? cast_ray<kGeom, kOp, kFmt>(it, col, row)
+                           : ((kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f);
   store_pixel<kOp>(it, tr, inside, col, row, acc, out, accumulate, fill, ncc_partials, ncc_tiles);
+}
+
+// Several objects per item (Intensity2D3DRegi::obj_fn with more than one volume,
+// xregIntensity2D3DRegi.cpp:598-629): entries item * objs + k, k = 0..objs-1, are cast in
+// object order into the item's pixels, the first onto its background image (or `fill`), the
+// others added with the line kernel -- per pixel the same float operations as one REPLACE
+// launch followed by ACCUM launches. The item's first entry carries the output offset, the
+// detector and the fused NCC reference.
+template <int kGeom, int kOp, int kFmt>
+__global__ __launch_bounds__(kThreads) void drr_objects_kernel(
+    const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
+    int bands, float* __restrict__ out, float fill, double* __restrict__ ncc_partials, int ncc_tiles,
+    int shape, int objs)
+{
+  const TileRef tr = decode_tile(band_start, num_items, bands, items, objs);
+  if (!tr.valid) return;
+  const DrrItem& it0 = items[tr.item * objs];
+  int col, row;
+  lane_pixel(tr.tx, tr.ty, shape, col, row);
+  const bool inside = (col < it0.cam.cols) && (row < it0.cam.rows);
+  float pix = 0.0f;
+  if (inside) {
+    const int64_t px = (int64_t)row * it0.cam.cols + col;
+    pix = it0.bg ? it0.bg[px] : fill;
+    for (int k = 0; k < objs; ++k) {
+      const float aa = 0.0f + cast_ray<kGeom, kOp, kFmt>(items[tr.item * objs + k], col, row) * 1.0f;
+      if constexpr (kOp == XRH_LINE_INT_SUM) pix = pix + aa; else pix = (pix < aa) ? aa : pix;
+    }
+    out[it0.out_off + px] = pix;
+  }
+  ncc_epilogue(it0, tr, inside, col, row, pix, ncc_partials, ncc_tiles);
 }
 
 template <int kGeom>
@@ -733,7 +802,7 @@ __global__ __launch_bounds__(kThreads) void drr_count_kernel(
     cnt = (double)(ray.n + 1);
     if (ray.n >= 0) {
       int first, last;
-      sample_range<XRH_LINE_INT_SUM>(it, ray, first, last);
+      sample_range<XRH_LINE_INT_SUM, kGeom>(it, ray, first, last);
       evaluated = (double)(last - first + 1);
     }
   }
@@ -762,10 +831,18 @@ int device_cus()
 template <int kGeom, int kOp>
 void launch_fmt(const DrrLaunch& L, dim3 grid, hipStream_t s, int accumulate, float fill)
 {
-#define XRH_LAUNCH_FMT(F)                                                                    \
-  hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev,   \
-                     L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,        \
-                     L.ncc_partials, L.ncc_tiles_per_item, shape)
+#define XRH_LAUNCH_FMT(F)                                                                      \
+  do {                                                                                         \
+    if (L.objs > 1) {                                                                          \
+      hipLaunchKernelGGL((drr_objects_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s,      \
+                         L.items_dev, L.band_start_dev, L.num_items, L.bands, L.out, fill,     \
+                         L.ncc_partials, L.ncc_tiles_per_item, shape, L.objs);                 \
+    } else {                                                                                   \
+      hipLaunchKernelGGL((drr_kernel<kGeom, kOp, F>), grid, dim3(kThreads), 0, s, L.items_dev, \
+                         L.band_start_dev, L.num_items, L.bands, L.out, accumulate, fill,      \
+                         L.ncc_partials, L.ncc_tiles_per_item, shape);                         \
+    }                                                                                          \
+  } while (0)
   // Cell reuse (one sample per iteration, loads only on cell change) wins when the launch
   // fills every CU many times over and the kernel is bound by load issue; small launches are
   // latency-bound and prefer the two-samples-per-iteration kernel. XRH_CELL_REUSE=0 / =1
@@ -842,10 +919,11 @@ TilePlan build_tiles(const std::vector<int32_t>& rows, const std::vector<int32_t
 void launch_drr_setup(const DrrLaunch& L, hipStream_t s)
 {
   if (L.num_items <= 0) return;
+  const int64_t entries = L.num_items * (L.objs > 1 ? L.objs : 1);
   const int threads = 64;
-  const int blocks = (int)((L.num_items + threads - 1) / threads);
+  const int blocks = (int)((entries + threads - 1) / threads);
   hipLaunchKernelGGL(drr_setup_kernel, dim3(blocks), dim3(threads), 0, s, L.poses_dev, L.vols_dev,
-                     L.cams_dev, L.items_dev, L.num_items);
+                     L.cams_dev, L.items_dev, entries);
   XRH_HIP_CHECK(hipGetLastError());
 }
 
@@ -856,6 +934,7 @@ void launch_drr(const DrrLaunch& L, hipStream_t s)
   XRH_REQUIRE(nblocks < (int64_t)INT32_MAX, "too many DRR tiles in one launch");
   const dim3 grid((unsigned)nblocks);
   if (L.sample_counter) {
+    XRH_REQUIRE(L.objs <= 1, "sample counts are taken per single-object item");
     if (L.geom == XRH_GEOM_FP32) {
       hipLaunchKernelGGL((drr_count_kernel<XRH_GEOM_FP32>), grid, dim3(kThreads), 0, s, L.items_dev,
                          L.band_start_dev, L.num_items, L.bands,

@@ -112,6 +112,7 @@ struct DrrItem {
   float step;
   int64_t out_off;   // element offset of this projection's DRR
   const float* ncc_zf;  // zero-mean fixed image for the fused NCC epilogue, or null
+  const float* bg;      // background image the first object of an item is added onto, or null
   int32_t tiles_x;
   int32_t tiles_y;
   const uint8_t* skip;  // empty-space map (see VolDesc), or null
@@ -125,6 +126,7 @@ struct PoseIn {
   int32_t vol;         // index into the volume table
   int64_t out_off;
   const float* ncc_zf;
+  const float* bg;     // background image (first object of a multi-object item), or null
   float step;
   int32_t pad_;
 };
@@ -255,6 +257,10 @@ struct DrrLaunch {
                             // sample index n of the reference algorithm (-1: the ray misses)
   int32_t reuse;            // DrrVariant::reuse
   int32_t wave_shape;       // DrrVariant::shape
+  int32_t objs;             // objects per item (0 or 1: one); items_dev / poses_dev then hold
+                            // num_items * objs entries, object-minor, cast in object order into
+                            // the item's pixels (first onto its background or fill, the rest
+                            // added), REPLACE only
 };
 // Key of the tuned-variant table: storage, layout, geometry, line kernel, the largest
 // detector of the launch and its workgroups per CU (rounded down to a power of two).

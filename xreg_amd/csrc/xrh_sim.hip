@@ -1120,6 +1120,38 @@ xrh_status xrh_fixed_destroy(xrh_fixed* fixed)
   });
 }
 
+xrh_status xrh_fixed_patches(const xrh_fixed* fixed, int64_t capacity, int32_t* start_row, int32_t* start_col,
+                             int32_t* stop_row, int32_t* stop_col, float* weight, int64_t* num_patches)
+{
+  return guarded([&] {
+    XRH_REQUIRE(fixed && num_patches, "null argument");
+    const FixedDev& h = fixed->host;
+    XRH_REQUIRE(h.metric == XRH_SIM_PATCH_NCC || h.metric == XRH_SIM_PATCH_GRAD_NCC, "not a patch metric");
+    *num_patches = h.num_patches;
+    if (capacity <= 0) return;
+    XRH_REQUIRE(capacity >= h.num_patches, "patch arrays shorter than the patch count");
+    // ImgSimMetric2DPatchCommon::setup_patches (xregImgSimMetric2DPatchCommon.cpp:274-292):
+    // centres r, r + stride, ... in each axis, rows outer
+    const int r = h.patch_radius;
+    for (int64_t q = 0; q < h.num_patches; ++q) {
+      const int cr = r + (int)(q / h.ppr) * h.patch_stride, cc = r + (int)(q % h.ppr) * h.patch_stride;
+      if (start_row) start_row[q] = cr - r;
+      if (start_col) start_col[q] = cc - r;
+      if (stop_row) stop_row[q] = cr + r;
+      if (stop_col) stop_col[q] = cc + r;
+    }
+    if (weight) {
+      if (h.pw) {
+        XRH_HIP_CHECK(hipMemcpyAsync(weight, h.pw, sizeof(float) * h.num_patches, hipMemcpyDeviceToHost,
+                                     fixed->ctx->stream));
+        XRH_HIP_CHECK(hipStreamSynchronize(fixed->ctx->stream));
+      } else {
+        for (int64_t q = 0; q < h.num_patches; ++q) weight[q] = 1.0f;
+      }
+    }
+  });
+}
+
 xrh_status xrh_sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev, int64_t num_mov,
                            float* sim_dev_out, float* sim_host_out)
 {

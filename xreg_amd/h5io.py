@@ -811,8 +811,10 @@ def read_se3_opt_vars(g: Group):
 
 @dataclass
 class RegiDebugResults:
-    """SingleRegiDebugResults (xregIntensity2D3DRegiDebug.h:45-141) without the optional
-    optimiser / metric auxiliary records. iter_vars[obj] is (iterations, num_params)."""
+    """SingleRegiDebugResults (xregIntensity2D3DRegiDebug.h:45-141). iter_vars[obj] is
+    (iterations, num_params). opt_aux: the optimiser's record (regi.run_cmaes /
+    regi.run_exhaustive put it in RegiResult.extra["opt_aux"]) or None; sims_aux: one metric
+    record per view (regi.Objective.sims_aux) or None per view."""
     se3_params: object
     vols_used: list
     init_poses: list
@@ -826,6 +828,81 @@ class RegiDebugResults:
     static_vol_poses: list = field(default_factory=list)
     do_not_use_static_vol_heuristic: bool = False
     regi_time_secs: float | None = None
+    opt_aux: dict | None = None
+    sims_aux: list = field(default_factory=list)
+
+
+def write_opt_aux(g: Group, aux: dict) -> None:
+    """The optimiser's "opt-aux" group: Intensity2D3DRegiCMAES::OptAux::write
+    (xregIntensity2D3DRegiCMAES.cpp:437-509) or Intensity2D3DRegiExhaustive::OptAux::write
+    (xregIntensity2D3DRegiExhaustive.cpp:397-440), same names and on-disk types."""
+    with g.create_group("opt-aux") as ag:
+        ag.write_string("name", aux["name"])
+        if aux["name"] == "CMA-ES":
+            nits = len(aux["cov_mats"])
+            write_ulong(ag, "num-iterations", nits)
+            write_ulong(ag, "pop-size", aux["pop_size"])
+            dims = list(aux.get("obj_dims") or [np.asarray(aux["pop_params"][0]).shape[1] if nits else 0])
+            with ag.create_group("all-pop-params") as pg:
+                for i, P in enumerate(aux["pop_params"]):
+                    with pg.create_group(f"iter-{i:03d}") as ig:
+                        o = 0
+                        for oi, d in enumerate(dims):
+                            write_matrix(ig, f"obj-{oi:03d}-pop", np.asarray(P)[:, o:o + d])
+                            o += d
+            if nits:
+                write_matrix(ag, "all-sim-vals", np.stack([np.asarray(v) for v in aux["sim_vals"]]))
+            with ag.create_group("covariances") as cg:
+                for i, Cm in enumerate(aux["cov_mats"]):
+                    write_matrix(cg, f"iter-{i:03d}", Cm)
+            if len(aux.get("num_rejects", [])):
+                write_vector(ag, "all-num-rejects", aux["num_rejects"], np.uint64)
+        elif aux["name"] == "Exhaustive":
+            vols = aux.get("cam_wrt_vols") or []
+            if vols and len(vols[0]):
+                with ag.create_group("cam-wrt-vols") as cwg:
+                    for vi, poses in enumerate(vols):
+                        with cwg.create_group(f"vol-{vi:02d}") as vg:
+                            for i, T in enumerate(np.asarray(poses).reshape(-1, 4, 4)):
+                                write_matrix(vg, f"{i:06d}", T)
+            if aux.get("has_min"):
+                ag.write_array("min-sim-val", np.float32(aux["min_sim_val"]))
+                if aux.get("has_all_sim_vals"):
+                    write_ulong(ag, "min-sim-val-idx", aux["min_sim_val_idx"])
+            if aux.get("has_all_sim_vals"):
+                write_vector(ag, "all-sim-vals", aux["all_sim_vals"], np.float32)
+        else:
+            raise H5Error(f"unknown optimiser record {aux['name']!r}")
+
+
+def _write_patch_ncc_aux(g: Group, rec: dict) -> None:
+    """ImgSimMetric2DPatchNCCCPU::SimAux::write (xregImgSimMetric2DPatchNCCCPU.cpp:443-485): one
+    group per compute call with the patch bounds, weights and the indices of the patches used."""
+    g.write_string("sim-aux-type", "patch-ncc-aux")
+    p = rec["patches"]
+    for i in range(int(rec["calls"])):
+        with g.create_group(f"{i:04d}") as cg:
+            write_vector(cg, "start-rows", p["start_rows"], np.uint64)
+            write_vector(cg, "start-cols", p["start_cols"], np.uint64)
+            write_vector(cg, "stop-rows", p["stop_rows"], np.uint64)
+            write_vector(cg, "stop-cols", p["stop_cols"], np.uint64)
+            write_vector(cg, "weights", p["weights"], np.float32)
+            write_vector(cg, "patch-inds", rec["patch_inds"], np.uint64)
+
+
+def write_sim_aux(g: Group, rec: dict) -> None:
+    """One metric's record: patch NCC, or patch gradient-NCC with its x and y records
+    (ImgSimMetric2DPatchGradNCCCPU::SimAux::write, xregImgSimMetric2DPatchGradNCCCPU.cpp:411-420)."""
+    if rec["type"] == "patch-grad-ncc-aux":
+        g.write_string("sim-aux-type", "patch-grad-ncc-aux")
+        with g.create_group("x") as xg:
+            _write_patch_ncc_aux(xg, rec["x"])
+        with g.create_group("y") as yg:
+            _write_patch_ncc_aux(yg, rec["y"])
+    elif rec["type"] == "patch-ncc-aux":
+        _write_patch_ncc_aux(g, rec)
+    else:
+        raise H5Error(f"unknown metric record {rec['type']!r}")
 
 
 def write_regi_debug_group(g: Group, r: RegiDebugResults) -> None:
@@ -870,6 +947,14 @@ def write_regi_debug_group(g: Group, r: RegiDebugResults) -> None:
         g.write_array("regi-elapsed-seconds", np.float64(r.regi_time_secs))
     if nits:
         write_vector(g, "sim-vals", r.sims, np.float32)
+    if r.opt_aux is not None:
+        write_opt_aux(g, r.opt_aux)
+    if any(a is not None for a in r.sims_aux):
+        with g.create_group("sims-aux") as sag:
+            for si, rec in enumerate(r.sims_aux):
+                if rec is not None:
+                    with sag.create_group(f"{si:03d}") as sg:
+                        write_sim_aux(sg, rec)
 
 
 def read_regi_debug_group(g: Group) -> RegiDebugResults:

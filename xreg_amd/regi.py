@@ -70,6 +70,14 @@ class Objective:
         self.batch.set_options(use_graph=use_graph, time_kernels=False)
         self.evals = 0        # candidate poses evaluated
         self.calls = 0        # batched submissions
+        self.metric = int(metric)
+
+    def sims_aux(self):
+        """The metrics' debug records (ImgSimMetric2D::aux_info, xregImgSimMetric2D.h:134-136):
+        per view, for patch metrics, the patch list each compute call used -- one call per view
+        per submission here, as one ImgSimMetric2D::compute per obj_fn call in the reference --
+        else None (the reference's NCC and gradient-NCC metrics keep none)."""
+        return metric_aux_records(self.fixed, self.metric, self.calls)
 
     def eval_poses(self, poses: np.ndarray) -> np.ndarray:
         poses = np.asarray(poses, dtype=np.float64).reshape(-1, 4, 4)
@@ -95,6 +103,26 @@ class Objective:
         self.batch.close()
         for f in self.fixed:
             f.close()
+
+
+def metric_aux_records(fixed, metric: int, calls: int):
+    """Per fixed image: for the patch metrics the record ImgSimMetric2DPatchNCCCPU keeps with
+    save_aux_info (xregImgSimMetric2DPatchNCCCPU.cpp:290-298: every compute call appends the
+    patch list and the indices of the patches used -- all of them, random-patch mode being off),
+    and for patch gradient-NCC the same for the x and y gradients
+    (xregImgSimMetric2DPatchGradNCCCPU.cpp:246-252); None for the other metrics."""
+    out = []
+    for f in fixed:
+        if metric not in (X.XRH_SIM_PATCH_NCC, X.XRH_SIM_PATCH_GRAD_NCC):
+            out.append(None)
+            continue
+        pl = f.patches()
+        rec = {"type": "patch-ncc-aux", "calls": int(calls), "patches": pl,
+               "patch_inds": np.arange(len(pl["weights"]), dtype=np.uint64)}
+        if metric == X.XRH_SIM_PATCH_GRAD_NCC:
+            rec = {"type": "patch-grad-ncc-aux", "x": rec, "y": dict(rec)}
+        out.append(rec)
+    return out
 
 
 class MultiParam:
@@ -124,17 +152,24 @@ class MultiObjective:
     background images when `bg` is given (has_a_static_vol_, :593-625) -- and the others with
     ACCUM; then each view is scored and the views are averaged (ImgSimMetric2DCombineMean).
     All cameras share one detector size, as RayCasterCPU requires (xregRayCastBaseCPU.cpp:
-    54-70). Each object is one xrh_line_int launch over all views and candidates."""
+    54-70).
+
+    batched=True (default): one xrh_batch_eval_objects submission per call covers every view,
+    candidate and object (the objects of a candidate are cast in order inside one launch,
+    replayed from a graph when the shape repeats). batched=False: one xrh_line_int launch per
+    object over all views and candidates, then one metric call per view -- the reference's
+    call pattern. Both give the same DRRs and values."""
 
     def __init__(self, ctx: X.Context, vols, views, params, step: float,
                  metric: int = X.XRH_SIM_PATCH_GRAD_NCC, smooth_ksize: int = 5, patch_radius: int = 5,
-                 max_batch: int = 64, geom: int = X.XRH_GEOM_FP32, bg=None):
+                 max_batch: int = 64, geom: int = X.XRH_GEOM_FP32, bg=None, batched: bool = True):
         if len(vols) != len(params) or not vols:
             raise ValueError("one parameterisation per object")
         self.ctx, self.vols, self.step, self.geom = ctx, list(vols), float(step), int(geom)
         self.param = MultiParam(params)
         self.cams = [c for c, _ in views]
         self.V = len(views)
+        self.O = len(self.vols)
         rows, cols = self.cams[0].num_det_rows, self.cams[0].num_det_cols
         if any((c.num_det_rows, c.num_det_cols) != (rows, cols) for c in self.cams):
             raise ValueError("all views must share one detector size")
@@ -142,28 +177,52 @@ class MultiObjective:
         self.fixed = [X.Fixed(ctx, img, metric, smooth_ksize=smooth_ksize, patch_radius=patch_radius)
                       for _, img in views]
         self.max_batch = int(max_batch)
-        self.drr = ctx.alloc(4 * self.V * self.max_batch * self.npx)
         self.bg = None
         if bg is not None:
             b = np.ascontiguousarray(bg, dtype=np.float32).reshape(self.V, rows, cols)
             self.bg = ctx.alloc(b.nbytes)
             self.bg.upload(b)
+        self.batched = bool(batched)
+        self.drr = None
+        self.batch = None
+        if self.batched:
+            # view table object-major: view o * V + v binds object o's volume to camera v (the
+            # fixed image of view v; only object 0's views are scored and carry backgrounds)
+            tbl = [X.make_view(vol, self.fixed[v], self.cams[v], self.step) for vol in self.vols for v in range(self.V)]
+            self.batch = X.Batch(ctx, tbl, max_items=self.O * self.V * self.max_batch, geom=self.geom)
+        else:
+            self.drr = ctx.alloc(4 * self.V * self.max_batch * self.npx)
         self.evals = 0
         self.calls = 0
+        self.metric = int(metric)
 
-    def eval_poses(self, poses_per_obj) -> np.ndarray:
+    def sims_aux(self):
+        """Per view, the metric's debug record (see Objective.sims_aux)."""
+        return metric_aux_records(self.fixed, self.metric, self.calls)
+
+    def eval_poses(self, poses_per_obj, drr: X.DeviceBuffer | None = None) -> np.ndarray:
+        """Objective values of n candidates given each object's n poses. drr (batched form only):
+        optional device buffer receiving the view-major DRRs of the last chunk."""
         poses_per_obj = [np.asarray(p, dtype=np.float64).reshape(-1, 4, 4) for p in poses_per_obj]
         n = len(poses_per_obj[0])
         out = np.empty(n, dtype=np.float32)
         for s in range(0, n, self.max_batch):
             k = min(self.max_batch, n - s)
-            c4p = np.repeat(np.arange(self.V, dtype=np.int64), k)
-            for o, vol in enumerate(self.vols):
-                xf = np.tile(poses_per_obj[o][s:s + k], (self.V, 1, 1)).astype(np.float32)
-                X.line_int(self.ctx, vol, self.cams, xf, c4p, self.step, self.drr,
-                           store=X.XRH_STORE_REPLACE if o == 0 else X.XRH_STORE_ACCUM,
-                           bg=self.bg if o == 0 else None, geom=self.geom)
-            sims = np.stack([self.fixed[v].compute(self.drr, k, offset_imgs=v * k) for v in range(self.V)])
+            if self.batched:
+                # items view-major (view v, candidate c) -> entries (object o): view o * V + v
+                ev = (np.arange(self.O)[None, None, :] * self.V + np.arange(self.V)[:, None, None]
+                      + np.zeros((1, k, 1), np.int64)).reshape(self.V * k, self.O)
+                ep = np.stack([np.broadcast_to(poses_per_obj[o][s:s + k][None], (self.V, k, 4, 4))
+                               for o in range(self.O)], axis=2).reshape(self.V * k, self.O, 4, 4)
+                sims = self.batch.eval_objects(ev, ep, bg=self.bg, drr=drr).reshape(self.V, k)
+            else:
+                c4p = np.repeat(np.arange(self.V, dtype=np.int64), k)
+                for o, vol in enumerate(self.vols):
+                    xf = np.tile(poses_per_obj[o][s:s + k], (self.V, 1, 1)).astype(np.float32)
+                    X.line_int(self.ctx, vol, self.cams, xf, c4p, self.step, self.drr,
+                               store=X.XRH_STORE_REPLACE if o == 0 else X.XRH_STORE_ACCUM,
+                               bg=self.bg if o == 0 else None, geom=self.geom)
+                sims = np.stack([self.fixed[v].compute(self.drr, k, offset_imgs=v * k) for v in range(self.V)])
             out[s:s + k] = combine_mean(sims)
             self.calls += 1
         self.evals += n
@@ -174,7 +233,10 @@ class MultiObjective:
         return self.eval_poses(self.param(Xp))
 
     def close(self):
-        self.drr.free()
+        if self.batch is not None:
+            self.batch.close()
+        if self.drr is not None:
+            self.drr.free()
         if self.bg is not None:
             self.bg.free()
         for f in self.fixed:
@@ -209,14 +271,23 @@ def run_cmaes(obj: Objective, lam: int, sigma, max_gens: int, seed: int, x0=None
                bounds=None if bounds is None else np.asarray(bounds, dtype=np.float64), tol_fun=tol_fun, tol_x=tol_x)
     best_f, best_x = np.inf, x0.copy()
     iter_means, iter_best = [], []
+    aux = {"name": "CMA-ES", "pop_size": int(lam), "pop_params": [], "sim_vals": [], "cov_mats": [],
+           "num_rejects": [], "obj_dims": list(getattr(obj.param, "sizes", [obj.param.num_params]))}
     e0 = obj.evals
     for _ in range(max_gens):
+        r0 = es.num_rejects
         Xc = es.ask()
+        aux["num_rejects"].append(es.num_rejects - r0)
         f = obj(np.asarray(Xc)).astype(np.float64)
         k = int(np.argmin(f))
         if f[k] < best_f:
             best_f, best_x = float(f[k]), np.asarray(Xc[k]).copy()
         es.tell(f)
+        # the CMA-ES opt-aux record (xregIntensity2D3DRegiCMAES.cpp:225-243): population,
+        # values and the covariance after the update
+        aux["pop_params"].append(np.asarray(Xc, np.float64).copy())
+        aux["sim_vals"].append(f.copy())
+        aux["cov_mats"].append(es.C.copy())
         iter_means.append(es.m.copy())
         iter_best.append(best_f)
         if es.stop():
@@ -224,7 +295,8 @@ def run_cmaes(obj: Objective, lam: int, sigma, max_gens: int, seed: int, x0=None
     xm = es.m.copy()
     return RegiResult(xm, best_f, obj.param(xm), obj.evals - e0, list(es.history),
                       extra={"best_x": best_x, "best_pose": obj.param(best_x),
-                             "iter_means": np.array(iter_means), "iter_best": np.array(iter_best)})
+                             "iter_means": np.array(iter_means), "iter_best": np.array(iter_best),
+                             "opt_aux": aux})
 
 
 def mesh_grid(axes) -> np.ndarray:
@@ -253,9 +325,15 @@ def run_exhaustive(obj: Objective, candidates: np.ndarray, batch: int | None = N
         if save_all:
             all_vals.append(vals)
     x = cand[best_i] if best_i >= 0 else np.zeros(obj.param.num_params)
+    all_v = np.concatenate(all_vals) if all_vals else np.zeros(0, np.float32)
+    # the Exhaustive opt-aux record (xregIntensity2D3DRegiExhaustive.cpp:397-440): the candidate
+    # poses per object, the minimum and, with every value kept, all values and the index
+    poses = obj.param(cand) if len(cand) else []
+    aux = {"name": "Exhaustive", "cam_wrt_vols": poses if isinstance(poses, list) else [poses],
+           "has_min": best_i >= 0, "min_sim_val": float(best_f), "has_all_sim_vals": bool(save_all),
+           "min_sim_val_idx": int(best_i), "all_sim_vals": all_v}
     return RegiResult(x, float(best_f), obj.param(x), obj.evals - e0,
-                      extra={"all_sim_vals": np.concatenate(all_vals) if all_vals else np.zeros(0, np.float32),
-                             "best_index": best_i})
+                      extra={"all_sim_vals": all_v, "best_index": best_i, "opt_aux": aux})
 
 
 def run_local_refine(obj: Objective, x0, rho_beg, rho_end: float = 1e-3, max_evals: int = 200,
@@ -379,11 +457,12 @@ def run_multi_level(ctx, vol, cams, fixed_imgs, init_pose, center, step, levels,
 
 
 def debug_results(res: RegiResult, param: InterFramePose, vol_index: int = 0, projs_used=(),
-                  regi_time_secs: float | None = None):
+                  regi_time_secs: float | None = None, sims_aux=None):
     """The SingleRegiDebugResults record of one CMA-ES run (xregIntensity2D3DRegiDebug.h:45-141,
     filled as Intensity2D3DRegi::run and the CMA-ES debug hooks fill it): parameterisation,
-    guess, final pose, intermediate frame, per-generation means and best-so-far values. Write it
-    with h5io.write_regi_debug."""
+    guess, final pose, intermediate frame, per-generation means and best-so-far values, the
+    optimiser's record (population, values, covariance and rejections per generation) and, when
+    given, the metrics' records (Objective.sims_aux()). Write it with h5io.write_regi_debug."""
     from . import h5io
 
     means = res.extra.get("iter_means")
@@ -394,4 +473,5 @@ def debug_results(res: RegiResult, param: InterFramePose, vol_index: int = 0, pr
         se3_params=param.vars, vols_used=[int(vol_index)], init_poses=[np.asarray(param.guess, np.float32)],
         final_poses=[np.asarray(res.pose, np.float32)], inter_frames=[np.asarray(param.inter_frame, np.float32)],
         inter_frames_wrt_vol=[param.inter_frame_wrt_vol], iter_vars=[np.asarray(means, np.float32)],
-        sims=np.asarray(best, np.float32), projs_used=list(projs_used), regi_time_secs=regi_time_secs)
+        sims=np.asarray(best, np.float32), projs_used=list(projs_used), regi_time_secs=regi_time_secs,
+        opt_aux=res.extra.get("opt_aux"), sims_aux=list(sims_aux) if sims_aux is not None else [])
