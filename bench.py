@@ -173,7 +173,7 @@ def kernel_source_key() -> str:
     """Hash of the DRR kernel's sources: profile figures are looked up under it, so a figure
     measured on other kernel code is never reported."""
     h = hashlib.sha256()
-    for f in ("xrh_drr.hip", "xrh_internal.h"):
+    for f in ("xrh_drr.hip", "xrh_ray.h", "xrh_internal.h"):
         h.update((ROOT / "xreg_amd" / "csrc" / f).read_bytes())
     return h.hexdigest()[:12]
 

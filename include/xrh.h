@@ -57,6 +57,16 @@ enum { XRH_STORE_REPLACE = 0, XRH_STORE_ACCUM = 1 };
  *   count) in double; samples are placed at start + i*step in float. */
 enum { XRH_GEOM_FP32 = 0, XRH_GEOM_FP64 = 1 };
 
+/* Volume interpolation: RayCaster::InterpMethod (lib/ray_cast/xregRayCastInterface.h:61-67),
+ * the interpolators RayCasterLineIntCPU creates per range (xregRayCastLineIntCPU.cpp:115-147). */
+enum {
+  XRH_INTERP_LINEAR = 0,  /* itk::LinearInterpolateImageFunction                           */
+  XRH_INTERP_NN = 1,      /* itk::NearestNeighborInterpolateImageFunction                  */
+  XRH_INTERP_SINC = 2,    /* itk::WindowedSincInterpolateImageFunction, radius 4, Lanczos
+                             window, constant-zero boundary                               */
+  XRH_INTERP_BSPLINE = 3  /* itk::BSplineInterpolateImageFunction, order 3                 */
+};
+
 /* ImgSimMetric2D subclasses on the hot path (lib/regi/sim_metrics_2d/). */
 enum {
   XRH_SIM_NCC = 0,            /* ImgSimMetric2DNCCCPU            */
@@ -162,7 +172,7 @@ xrh_status xrh_volume_create_layout(xrh_ctx* ctx, const float* vox, const int64_
 xrh_status xrh_volume_create_hu(xrh_ctx* ctx, const float* vox_hu, const int64_t size[3],
                                 const float idx_to_phys[16], int32_t storage, int32_t layout,
                                 double hu_lower, double mu_water, double mu_air, xrh_volume** out);
-/* Device bytes held by the volume. */
+/* Device bytes held by the volume (its layout, empty-space map and any interpolation grids). */
 uint64_t xrh_volume_bytes(const xrh_volume* vol);
 /* Storage type and HBM layout a volume was built with (the automatic choice included: a
  * volume whose packed layout would pass 4 GiB is kept linear). Volumes batched together must
@@ -212,6 +222,30 @@ xrh_status xrh_line_int_noalloc(xrh_ctx* ctx, const xrh_volume* vol, const xrh_c
                                 const int64_t* cam_for_proj, int64_t num_projs, float step,
                                 int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
                                 int32_t geom, float* drr_dev);
+
+/* xrh_line_int with the volume interpolator chosen by RayCaster::set_interp_method
+ * (xregRayCastInterface.h:165-191), as RayCasterLineIntCPU evaluates it at every sample
+ * (xregRayCastLineIntCPU.cpp:115-147, 270-277). Same arguments as xrh_line_int except:
+ *   interp   XRH_INTERP_*; XRH_INTERP_LINEAR is xrh_line_int with XRH_GEOM_FP32
+ *   (no geom) rays are always set up with XRH_GEOM_FP32, the reference's own arithmetic, so
+ *            each sample lands where the reference's does (nearest-neighbour lookups depend
+ *            on it)
+ * Nearest neighbour rounds each continuous index half up as ITK 5.1 does on x86-64;
+ * windowed sinc takes the 8x8x8 voxels around floor(index) with Lanczos-4 weights and zeros
+ * outside the volume; the B-spline samples 4x4x4 cubic B-spline coefficients (computed on the
+ * device in double with ITK's mirror-boundary recursive filter) with mirrored indices. Sinc and
+ * B-spline weights and sums are FP64, as ITK evaluates them, each sample rounded to float once;
+ * the line sums are FP32 as in the reference. The first call per volume for nearest neighbour
+ * or sinc (one grid, 4 B per voxel) and for B-spline (another, 8 B per voxel) builds that grid:
+ * synchronous, allocated from the volume's context, counted in xrh_volume_bytes and freed with
+ * the volume;
+ * the volume is therefore not const here. A volume's grids are built from its stored voxels,
+ * so a 16-bit volume interpolates its 16-bit values. Empty-space skipping is not used. */
+xrh_status xrh_line_int_interp(xrh_ctx* ctx, xrh_volume* vol, const xrh_camera* cams,
+                               int32_t num_cams, const float* xforms_cam_to_phys,
+                               const int64_t* cam_for_proj, int64_t num_projs, float step,
+                               int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
+                               int32_t interp, float* drr_dev);
 
 /* Exact number of trilinear samples the reference algorithm takes for the same call (sum
  * over rays of n+1, the S_pose of the roofline byte model) and, optionally, how many of them

@@ -100,8 +100,9 @@ def p2i_f64(idx_to_phys: np.ndarray) -> np.ndarray:
 
 
 def drr_f32(vol_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, kernel=0, store=0, bg=None,
-            default_bg=0.0, out=None, threads=0):
-    """RayCasterLineIntCPU restatement. Returns (drr [n, rows, cols] float32, samples)."""
+            default_bg=0.0, out=None, threads=0, interp=0):
+    """RayCasterLineIntCPU restatement. Returns (drr [n, rows, cols] float32, samples).
+    interp: RayCaster::InterpMethod (0 linear, 1 nearest neighbour, 2 windowed sinc, 3 B-spline)."""
     vol = np.ascontiguousarray(vol_zyx, dtype=np.float32)
     dims = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], dtype=np.int64)
     x = np.ascontiguousarray(np.asarray(xforms, dtype=np.float32).reshape(-1, 16))
@@ -117,8 +118,9 @@ def drr_f32(vol_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, ker
         bgp = _f(bga)
     s = C.c_int64(0)
     p2i = p2i_f32(idx_to_phys)
-    lib().orc_drr_f32(_f(vol), _i64(dims), _f(p2i), ca, len(cams), _f(x), _i64(c4p), C.c_int64(n),
-                      C.c_float(step), kernel, store, bgp, C.c_float(default_bg), _f(out), threads, C.byref(s))
+    lib().orc_drr_interp_f32(_f(vol), _i64(dims), _f(p2i), ca, len(cams), _f(x), _i64(c4p), C.c_int64(n),
+                             C.c_float(step), kernel, store, bgp, C.c_float(default_bg), int(interp), _f(out),
+                             threads, C.byref(s))
     return out, int(s.value)
 
 
@@ -305,6 +307,38 @@ def itk_interp(vol_zyx, idx):
     p = np.ascontiguousarray(idx, dtype=np.float32)
     lib().orc_itk_linear_interp.restype = C.c_double
     return float(lib().orc_itk_linear_interp(_f(vol), _i64(dims), _f(p)))
+
+
+def itk_interp_points(vol_zyx, idx, interp):
+    """The reference's interpolator `interp` (0 linear, 1 nearest neighbour, 2 windowed sinc,
+    3 B-spline) at each float continuous index (x, y, z) of idx [n, 3]; returns float64 [n]."""
+    vol = np.ascontiguousarray(vol_zyx, dtype=np.float32)
+    dims = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], dtype=np.int64)
+    pts = np.ascontiguousarray(np.asarray(idx, dtype=np.float32).reshape(-1, 3))
+    out = np.empty(pts.shape[0], dtype=np.float64)
+    L = lib()
+    if interp == 3:
+        coef = np.empty(vol.size, dtype=np.float64)
+        L.orc_bspline_coefficients(_f(vol), _i64(dims), _d(coef))
+        fn = L.orc_itk_bspline_interp
+        fn.restype = C.c_double
+        for i, p in enumerate(pts):
+            out[i] = fn(_d(coef), _i64(dims), _f(np.ascontiguousarray(p)))
+        return out
+    fn = {0: L.orc_itk_linear_interp, 1: L.orc_itk_nn_interp, 2: L.orc_itk_sinc_interp}[interp]
+    fn.restype = C.c_double
+    for i, p in enumerate(pts):
+        out[i] = fn(_f(vol), _i64(dims), _f(np.ascontiguousarray(p)))
+    return out
+
+
+def bspline_coefficients(vol_zyx):
+    """orc_bspline_coefficients: the cubic B-spline coefficients of the volume (float64, zyx)."""
+    vol = np.ascontiguousarray(vol_zyx, dtype=np.float32)
+    dims = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], dtype=np.int64)
+    coef = np.empty(vol.shape, dtype=np.float64)
+    lib().orc_bspline_coefficients(_f(vol), _i64(dims), _d(coef))
+    return coef
 
 
 def ray_rect_f32(mn, mx, p0, d, limit=True):

@@ -424,6 +424,134 @@ double orc_linear_interp_f64(const double* vol, const int64_t dims[3], const dou
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* The CPU caster's other interpolators (xregRayCastLineIntCPU.cpp:115-147), each evaluated
+ * at a float continuous index and returning double, as ITK 5.1's EvaluateAtContinuousIndex
+ * does for Image<float,3> with CoordRep float.
+ * parity unpinned against ITK itself (absent here); pinned by the closed forms and library
+ * routines in tests/test_oracle_interp.py. */
+
+/* itk::Math::RoundHalfIntegerUp<int64>(float) and itk::Math::Floor<int64>(float) as ITK 5.1
+ * compiles them on x86-64 with SSE2 (itkMathDetail.h): cvtss2si, i.e. round-to-nearest-even
+ * under the default MXCSR, of 2x + 0.5f (resp. 2x - 0.5f) computed in float, shifted right. */
+static int64_t itk_round_half_up(float x) { return ((int64_t)nearbyintf(x + x + 0.5f)) >> 1; }
+static int64_t itk_floor(float x) { return ((int64_t)nearbyintf(x + x - 0.5f)) >> 1; }
+
+static double vox_or_zero(const float* vol, const int64_t dims[3], int64_t x, int64_t y, int64_t z)
+{
+  if (x < 0 || y < 0 || z < 0 || x >= dims[0] || y >= dims[1] || z >= dims[2]) return 0.0;
+  return (double)vol[x + dims[0] * (y + dims[1] * z)];
+}
+
+/* NearestNeighborInterpolateImageFunction: the voxel at the rounded index (ITK does not test
+ * the bounds; every sample of a ray lies inside the index box, and out-of-range reads 0 here). */
+double orc_itk_nn_interp(const float* vol, const int64_t dims[3], const float idx[3])
+{
+  return vox_or_zero(vol, dims, itk_round_half_up(idx[0]), itk_round_half_up(idx[1]), itk_round_half_up(idx[2]));
+}
+
+/* WindowedSincInterpolateImageFunction<Image<float,3>, 4, LanczosWindowFunction<4>,
+ * ConstantBoundaryCondition, float>: base = Floor(index), distance in double; per dimension
+ * x runs distance + 3 down to distance - 4 and the weight is Lanczos(x) * Sinc(x) with
+ * Sinc(x) = sin(pi x) / (pi x) and Lanczos(x) = sin(pi x / 4) / (pi x / 4) (1 at x = 0); a zero
+ * distance gives the unit impulse on offset 0. The neighbourhood (offsets -3..4, x fastest)
+ * is summed in iteration order, each term the pixel times the x, y and z weights in that order;
+ * pixels outside the image are 0. */
+double orc_itk_sinc_interp(const float* vol, const int64_t dims[3], const float idx[3])
+{
+  const double pi = 3.14159265358979323846;
+  int64_t base[3];
+  double w[3][8];
+  int k, i, j, l;
+  for (k = 0; k < 3; ++k) {
+    base[k] = itk_floor(idx[k]);
+    const double dist = (double)idx[k] - (double)base[k];
+    if (dist == 0.0) {
+      for (i = 0; i < 8; ++i) w[k][i] = (i == 3) ? 1.0 : 0.0;
+    } else {
+      double x = dist + 4.0;
+      for (i = 0; i < 8; ++i) {
+        x -= 1.0;
+        const double zl = (pi / 4.0) * x;
+        const double lanczos = (x == 0.0) ? 1.0 : sin(zl) / zl;
+        const double px = pi * x;
+        const double sinc = (x == 0.0) ? 1.0 : sin(px) / px;
+        w[k][i] = lanczos * sinc;
+      }
+    }
+  }
+  double sum = 0.0;
+  for (l = 0; l < 8; ++l)
+    for (j = 0; j < 8; ++j)
+      for (i = 0; i < 8; ++i) {
+        double v = vox_or_zero(vol, dims, base[0] - 3 + i, base[1] - 3 + j, base[2] - 3 + l);
+        v *= w[0][i];
+        v *= w[1][j];
+        v *= w[2][l];
+        sum += v;
+      }
+  return sum;
+}
+
+static void orc_bspline_line(double* c, int64_t n, int64_t stride);
+
+/* BSplineDecompositionImageFilter (order 3) of a whole volume: the voxels in double, filtered
+ * along x, then y, then z with the mirror-boundary recursion of orc_bspline_line. */
+void orc_bspline_coefficients(const float* vol, const int64_t dims[3], double* coef)
+{
+  const int64_t X = dims[0], Y = dims[1], Z = dims[2];
+  int64_t i, a, b;
+  for (i = 0; i < X * Y * Z; ++i) coef[i] = (double)vol[i];
+  for (b = 0; b < Z; ++b)
+    for (a = 0; a < Y; ++a) orc_bspline_line(coef + X * (a + Y * b), X, 1);
+  for (b = 0; b < Z; ++b)
+    for (a = 0; a < X; ++a) orc_bspline_line(coef + a + X * Y * b, Y, X);
+  for (b = 0; b < Y; ++b)
+    for (a = 0; a < X; ++a) orc_bspline_line(coef + a + X * b, Z, X * Y);
+}
+
+/* ITK's single-reflection mirror (BSplineInterpolateImageFunction::
+ * ApplyMirrorBoundaryConditions): i < 0 -> -i, i >= n - 1 -> 2 (n - 1) - i; length 1 -> 0. */
+static int64_t itk_mirror(int64_t i, int64_t n)
+{
+  if (n == 1) return 0;
+  if (i < 0) i = -i;
+  if (i >= n - 1) i = 2 * (n - 1) - i;
+  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+}
+
+/* BSplineInterpolateImageFunction (order 3, double coefficients)::EvaluateAtContinuousIndex:
+ * support floor(x) - 1 .. floor(x) + 2 per dimension, weights of w = x - (floor(x)) as ITK's
+ * SetInterpolationWeights writes them, mirrored indices, and the 64 points in ITK's order
+ * (p = i + 4 j + 16 l, x fastest), each weight the product over x, y, z. */
+double orc_itk_bspline_interp(const double* coef, const int64_t dims[3], const float idx[3])
+{
+  int64_t ind[3][4];
+  double w[3][4];
+  int k, p;
+  for (k = 0; k < 3; ++k) {
+    const int64_t b = (int64_t)floor((double)idx[k]) - 1;
+    int q;
+    for (q = 0; q < 4; ++q) ind[k][q] = b + q;
+    const double t = (double)idx[k] - (double)ind[k][1];
+    w[k][3] = (1.0 / 6.0) * t * t * t;
+    w[k][0] = (1.0 / 6.0) + 0.5 * t * (t - 1.0) - w[k][3];
+    w[k][2] = t + w[k][0] - 2.0 * w[k][3];
+    w[k][1] = 1.0 - w[k][0] - w[k][2] - w[k][3];
+    for (q = 0; q < 4; ++q) ind[k][q] = itk_mirror(ind[k][q], dims[k]);
+  }
+  double v = 0.0;
+  for (p = 0; p < 64; ++p) {
+    const int i = p % 4, j = (p / 4) % 4, l = p / 16;
+    double wt = 1.0;
+    wt *= w[0][i];
+    wt *= w[1][j];
+    wt *= w[2][l];
+    v += wt * coef[ind[0][i] + dims[0] * (ind[1][j] + dims[1] * ind[2][l])];
+  }
+  return v;
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* DRR, FP32 restatement of ComputeLineInts<Kernel> (xregRayCastLineIntCPU.cpp:105-290). */
 
 typedef struct {
@@ -442,6 +570,8 @@ typedef struct {
   int64_t samples;
   int32_t* steps;     /* optional: per ray, the last sample index n (-1: the ray misses) */
   int32_t steps_only; /* set-up only: write steps, take no samples, leave out untouched */
+  int32_t interp;     /* RayCaster::InterpMethod: 0 linear, 1 NN, 2 sinc, 3 B-spline */
+  const double* coef; /* B-spline coefficients (interp 3) */
 } drr_f32_ctx;
 
 static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
@@ -502,7 +632,14 @@ static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
       const float dx = d[0] * scale, dy = d[1] * scale, dz = d[2] * scale;
       float p[3] = {start[0], start[1], start[2]};
       for (int64_t i = 0; i <= n; ++i) {                                       /* :270-277 */
-        const float val = (float)orc_itk_linear_interp(c->vol, c->dims, p);
+        double dv;
+        switch (c->interp) {
+          case 1: dv = orc_itk_nn_interp(c->vol, c->dims, p); break;
+          case 2: dv = orc_itk_sinc_interp(c->vol, c->dims, p); break;
+          case 3: dv = orc_itk_bspline_interp(c->coef, c->dims, p); break;
+          default: dv = orc_itk_linear_interp(c->vol, c->dims, p); break;
+        }
+        const float val = (float)dv;
         if (c->kernel == 0) {
           acc = acc + val;
         } else {
@@ -549,9 +686,30 @@ void orc_drr_f32(const float* vol, const int64_t dims[3], const float p2i[16],
                  float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
                  float* out, int32_t threads, int64_t* samples_out)
 {
+  orc_drr_interp_f32(vol, dims, p2i, cams, num_cams, xforms, cam_for_proj, num_projs, step, kernel, store, bg,
+                     default_bg, 0, out, threads, samples_out);
+}
+
+/* orc_drr_f32 with the interpolator RayCasterLineIntCPU picks from interp_method
+ * (xregRayCastLineIntCPU.cpp:126-147): 0 linear, 1 nearest neighbour, 2 windowed sinc,
+ * 3 cubic B-spline (coefficients computed once here; the reference recomputes them in
+ * every range's interpolator from the same volume, with the same result). */
+void orc_drr_interp_f32(const float* vol, const int64_t dims[3], const float p2i[16],
+                        const orc_cam_f* cams, int32_t num_cams,
+                        const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                        float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
+                        int32_t interp, float* out, int32_t threads, int64_t* samples_out)
+{
   (void)num_cams;
   drr_f32_ctx c;
+  double* coef = NULL;
   memset(&c, 0, sizeof(c));
+  if (interp == 3) {
+    coef = (double*)malloc(sizeof(double) * (size_t)(dims[0] * dims[1] * dims[2]));
+    orc_bspline_coefficients(vol, dims, coef);
+  }
+  c.interp = interp;
+  c.coef = coef;
   c.vol = vol;
   c.dims = dims;
   /* ITKImageIndexBoundsAsEigen: max index = size - 1 (lib/itk/xregITKBasicImageUtils.h:56-75) */
@@ -570,6 +728,7 @@ void orc_drr_f32(const float* vol, const int64_t dims[3], const float p2i[16],
   pre_compute_f32(out, cams, cam_for_proj, num_projs, c.npx, store, bg, default_bg);
   orc_parallel_for(drr_f32_range, &c, num_projs * c.npx, 2048, threads);
   if (samples_out) *samples_out = c.samples;
+  free(coef);
 }
 
 /* DRR, FP64 oracle: identical algorithm, every quantity in double. */

@@ -67,6 +67,12 @@ int32_t orc_ray_rect_f32(const float mn[3], const float mx[3], const float p0[3]
 int32_t orc_ray_rect_f64(const double mn[3], const double mx[3], const double p0[3],
                          const double d[3], int32_t limit_to_segment, double* t0, double* t1);
 double orc_itk_linear_interp(const float* vol, const int64_t dims[3], const float idx[3]);
+/* nearest neighbour, Lanczos-4 windowed sinc and cubic B-spline as the reference's ITK
+ * interpolators evaluate them (B-spline from orc_bspline_coefficients' double coefficients) */
+double orc_itk_nn_interp(const float* vol, const int64_t dims[3], const float idx[3]);
+double orc_itk_sinc_interp(const float* vol, const int64_t dims[3], const float idx[3]);
+void orc_bspline_coefficients(const float* vol, const int64_t dims[3], double* coef);
+double orc_itk_bspline_interp(const double* coef, const int64_t dims[3], const float idx[3]);
 double orc_linear_interp_f64(const double* vol, const int64_t dims[3], const double idx[3]);
 
 /* ---- DRR (RayCasterLineIntCPU::compute for one volume) ----
@@ -81,6 +87,12 @@ void orc_drr_f32(const float* vol, const int64_t dims[3], const float p2i[16],
                  const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
                  float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
                  float* out, int32_t threads, int64_t* samples_out);
+/* the same with RayCaster::InterpMethod interp: 0 linear, 1 NN, 2 sinc, 3 B-spline */
+void orc_drr_interp_f32(const float* vol, const int64_t dims[3], const float p2i[16],
+                        const orc_cam_f* cams, int32_t num_cams,
+                        const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
+                        float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
+                        int32_t interp, float* out, int32_t threads, int64_t* samples_out);
 
 void orc_drr_f64(const double* vol, const int64_t dims[3], const double p2i[16],
                  const orc_cam_d* cams, int32_t num_cams,

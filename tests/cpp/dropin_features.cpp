@@ -1,6 +1,8 @@
 // dropin_features.cpp -- drives the rest of the drop-in classes' interface (xreg_amd/cpp/xreg_hip.h)
 // the way xReg's callers use it:
 //   * set_kernel_id(MAX) on the caster (RayCastLineIntParamInterface, xregRayCastInterface.h:581-591);
+//   * nearest-neighbour, B-spline and windowed-sinc interpolation (set_interp_method,
+//     xregRayCastInterface.h:165-191);
 //   * two objects cast into the same projections, REPLACE then ACCUM, through compute(vol_idx)
 //     (Intensity2D3DRegi::obj_fn, xregIntensity2D3DRegi.cpp:593-625);
 //   * a static volume's DRR as the background of the first cast (has_a_static_vol_, same lines;
@@ -140,6 +142,18 @@ int main(int argc, char** argv)
     rc->compute(0);
     rec.put("max_drr", std::vector<float>(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + np * npx));
     rc->set_kernel_id(kRAY_CAST_LINE_INT_SUM_KERNEL);
+
+    // the other interpolators of the CPU caster (set_interp_method, xregRayCastInterface.h:165-191)
+    rc->use_nn_interp();
+    rc->compute(0);
+    rec.put("nn_drr", std::vector<float>(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + np * npx));
+    rc->use_bspline_interp();
+    rc->compute(0);
+    rec.put("bspline_drr", std::vector<float>(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + np * npx));
+    rc->set_interp_method(RayCaster::kRAY_CAST_INTERP_SINC);
+    rc->compute(0);
+    rec.put("sinc_drr", std::vector<float>(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + np * npx));
+    rc->use_linear_interp();
 
     // line integrals of A into an external host buffer; proj(i) reads the same memory
     std::vector<float> ext(np * npx, -1.0f);

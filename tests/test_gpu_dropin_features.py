@@ -74,6 +74,14 @@ def test_dropin_features(tmp_path, oracle, metric, smooth):
     # MAX kernel (REPLACE, default background 0)
     ref_max, _ = oracle.drr_f32(vol_a, w.idx_to_phys, w.cams, poses_f32(pa), step=w.step, kernel=1)
     assert drr_rel_err(f32("max_drr", shp), ref_max).max() <= DRR_REL_TOL
+    # nearest-neighbour (bit for bit), B-spline and windowed-sinc interpolation through
+    # set_interp_method / use_*_interp
+    for name, interp in (("nn_drr", 1), ("bspline_drr", 3), ("sinc_drr", 2)):
+        ref_i, _ = oracle.drr_f32(vol_a, w.idx_to_phys, w.cams, poses_f32(pa), step=w.step, interp=interp)
+        if interp == 1:
+            assert np.array_equal(f32(name, shp), ref_i)
+        else:
+            assert drr_rel_err(f32(name, shp), ref_i).max() <= DRR_REL_TOL, name
     # external host buffer: raw_host_pixel_buf() is the caller's memory and proj(i) reads it
     assert u64("ext_bad") == 0
     assert drr_rel_err(f32("ext_drr", shp), ref_a).max() <= DRR_REL_TOL

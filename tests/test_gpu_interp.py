@@ -1,0 +1,153 @@
+"""The non-trilinear interpolators of the CPU caster (RayCaster::set_interp_method,
+lib/ray_cast/xregRayCastInterface.h:165-191; xregRayCastLineIntCPU.cpp:115-147) on the device,
+through xrh_line_int_interp, against the oracle on the same volume, rays and poses:
+  * nearest neighbour: bit for bit (the device casts the reference's float rays, so every
+    sample reads the same voxel and the float sums run in the same order);
+  * cubic B-spline and Lanczos-4 windowed sinc: per pixel within 1e-4 relative (both sides
+    evaluate a sample in double and round it to float; the float line sums match).
+Also: MAX projections, ACCUM onto background images, 16-bit storage (the grid holds the stored
+16-bit values), every FP32 layout giving the same DRRs, and the grids' bytes."""
+import numpy as np
+import pytest
+
+from tests.helpers import DRR_REL_TOL, drr_rel_err, poses_f32
+
+pytestmark = pytest.mark.gpu
+
+NAMES = {1: "nn", 2: "sinc", 3: "bspline"}
+
+
+@pytest.fixture(scope="module")
+def c1i():
+    from xreg_amd import geometry as G
+    from xreg_amd import synth
+
+    w = synth.config("c1", n_poses=3)
+    return w, G.downsample_camera(w.cams[0], 0.5)
+
+
+def _cast(ctx, vol, cam, poses, step, interp, kernel=0, store=0, bg=None, default_bg=0.0, prefill=None):
+    from xreg_amd import _xrh as X
+
+    n = poses.shape[0]
+    npx = cam.num_det_rows * cam.num_det_cols
+    buf = ctx.alloc(4 * n * npx)
+    if prefill is not None:
+        buf.upload(np.ascontiguousarray(prefill, np.float32))
+    X.line_int_interp(ctx, vol, [cam], poses, np.zeros(n, np.int64), step, buf, interp, kernel=kernel,
+                      store=store, bg=bg, default_bg=default_bg)
+    out = buf.download(np.float32).reshape(n, cam.num_det_rows, cam.num_det_cols)
+    buf.free()
+    return out
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("interp", [1, 3, 2])
+def test_interp_drr_matches_oracle(ctx, oracle, c1i, interp, kernel):
+    from xreg_amd import _xrh as X
+
+    w, cam = c1i
+    print(f"case: interp {NAMES[interp]} kernel {kernel}, c1 volume, {cam.num_det_rows}^2, 3 poses", flush=True)
+    poses = poses_f32(w.poses)
+    fill = 0.25 if kernel == 0 else 0.0      # REPLACE pre-fill; MAX projections stay below 0.25
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    got = _cast(ctx, vol, cam, poses, w.step, interp, kernel=kernel, default_bg=fill)
+    vol.close()
+    ref, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses, step=w.step, kernel=kernel, default_bg=fill,
+                            interp=interp)
+    err = drr_rel_err(got, ref)
+    print(f"max rel err {err.max():.3e}; pixels differing {int((got != ref).sum())} of {got.size}; "
+          f"image max {ref.max():.4g}", flush=True)
+    assert ref.max() > (1.0 if kernel == 0 else 0.01)
+    if interp == 1:
+        assert np.array_equal(got, ref)
+    else:
+        assert err.max() <= DRR_REL_TOL
+
+
+@pytest.mark.parametrize("interp", [1, 3])
+def test_interp_accumulates_onto_background(ctx, oracle, c1i, interp):
+    from xreg_amd import _xrh as X
+
+    w, cam = c1i
+    print(f"case: interp {NAMES[interp]} ACCUM onto background images", flush=True)
+    poses = poses_f32(w.poses[:2])
+    rng = np.random.default_rng(2)
+    bg = rng.uniform(0.0, 3.0, size=(1, cam.num_det_rows, cam.num_det_cols)).astype(np.float32)
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    bgd = ctx.alloc(bg.nbytes)
+    bgd.upload(bg)
+    got = _cast(ctx, vol, cam, poses, w.step, interp, store=1, bg=bgd)
+    # ACCUM without background images adds onto what the buffer holds
+    prev = rng.uniform(0.0, 1.0, size=(2, cam.num_det_rows, cam.num_det_cols)).astype(np.float32)
+    got2 = _cast(ctx, vol, cam, poses, w.step, interp, store=1, prefill=prev)
+    bgd.free()
+    vol.close()
+    ref, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses, step=w.step, store=1, bg=bg, interp=interp)
+    ref2 = prev.copy()
+    oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses, step=w.step, store=1, out=ref2, interp=interp)
+    if interp == 1:
+        assert np.array_equal(got, ref) and np.array_equal(got2, ref2)
+    else:
+        assert drr_rel_err(got, ref).max() <= DRR_REL_TOL
+        assert drr_rel_err(got2, ref2).max() <= DRR_REL_TOL
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_interp_reads_the_stored_16bit_voxels(ctx, oracle, c1i, storage):
+    from xreg_amd import _xrh as X
+
+    w, cam = c1i
+    print(f"case: interp nn on {('f16', 'bf16')[storage - 1]} storage", flush=True)
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(w.vol, np.float32))
+    rounded = t.to(torch.float16 if storage == 1 else torch.bfloat16).to(torch.float32).numpy()
+    poses = poses_f32(w.poses[:2])
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys, storage=storage)
+    got = _cast(ctx, vol, cam, poses, w.step, 1)
+    vol.close()
+    ref, _ = oracle.drr_f32(rounded, w.idx_to_phys, [cam], poses, step=w.step, interp=1)
+    assert np.array_equal(got, ref)
+
+
+def test_interp_grids_are_the_same_for_every_fp32_layout(ctx, c1i):
+    from xreg_amd import _xrh as X
+
+    w, cam = c1i
+    print("case: interp nn + bspline over the linear, quad, coef, face and brick layouts", flush=True)
+    poses = poses_f32(w.poses[:2])
+    outs = {}
+    for layout in (X.LAYOUT_LINEAR, X.LAYOUT_QUAD, X.LAYOUT_COEF, X.LAYOUT_FACE, X.LAYOUT_BRICK):
+        vol = X.Volume(ctx, w.vol, w.idx_to_phys, layout=layout)
+        outs[layout] = (_cast(ctx, vol, cam, poses, w.step, 1), _cast(ctx, vol, cam, poses, w.step, 3))
+        vol.close()
+    first = outs[X.LAYOUT_LINEAR]
+    for layout, (nn, bs) in outs.items():
+        assert np.array_equal(nn, first[0]), layout
+        assert np.array_equal(bs, first[1]), layout
+
+
+def test_interp_grid_bytes_and_linear_passthrough(ctx, c1i):
+    from xreg_amd import _xrh as X
+
+    w, cam = c1i
+    print("case: interp grid bytes, linear = xrh_line_int with FP32 geometry", flush=True)
+    poses = poses_f32(w.poses[:2])
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    b0 = vol.nbytes
+    lin = _cast(ctx, vol, cam, poses, w.step, 0)
+    assert vol.nbytes == b0
+    buf = ctx.alloc(lin.nbytes)
+    X.line_int(ctx, vol, [cam], poses, np.zeros(2, np.int64), w.step, buf)
+    assert np.array_equal(lin, buf.download(np.float32).reshape(lin.shape))
+    buf.free()
+    Z, Y, X_ = w.vol.shape
+    _cast(ctx, vol, cam, poses, w.step, 1)
+    b1 = vol.nbytes
+    assert b1 - b0 == 4 * (X_ + 8) * (Y + 8) * (Z + 8)
+    _cast(ctx, vol, cam, poses, w.step, 2)      # sinc shares the nearest-neighbour grid
+    assert vol.nbytes == b1
+    _cast(ctx, vol, cam, poses, w.step, 3)
+    assert vol.nbytes - b1 == 8 * (X_ + 4) * (Y + 4) * (Z + 4)
+    vol.close()

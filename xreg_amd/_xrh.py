@@ -25,6 +25,7 @@ XRH_F32, XRH_F16, XRH_BF16 = 0, 1, 2
 XRH_LINE_INT_SUM, XRH_LINE_INT_MAX = 0, 1
 XRH_STORE_REPLACE, XRH_STORE_ACCUM = 0, 1
 XRH_GEOM_FP32, XRH_GEOM_FP64 = 0, 1
+XRH_INTERP_LINEAR, XRH_INTERP_NN, XRH_INTERP_SINC, XRH_INTERP_BSPLINE = 0, 1, 2, 3
 XRH_SIM_NCC, XRH_SIM_GRAD_NCC, XRH_SIM_PATCH_NCC, XRH_SIM_PATCH_GRAD_NCC = 0, 1, 2, 3
 LAYOUT_LINEAR, LAYOUT_QUAD, LAYOUT_OCT, LAYOUT_COEF, LAYOUT_FACE, LAYOUT_BRICK = 0, 1, 2, 3, 4, 5
 
@@ -121,6 +122,8 @@ SIGNATURES = {
                             _I32, _P]),
     "xrh_line_int_noalloc": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
                                     _I32, _P]),
+    "xrh_line_int_interp": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
+                                   _I32, _P]),
     "xrh_line_int_sample_count": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32,
                                          _PU64, _PU64]),
     "xrh_fixed_create": (_I32, [_P, _PF, _I32, _I32, C.POINTER(SimParams), _PU8, C.POINTER(_P)]),
@@ -319,6 +322,20 @@ def line_int_noalloc(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_fo
     check(lib().xrh_line_int_noalloc(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64), x.shape[0],
                                      float(step), int(kernel), int(store), bg.ptr if bg else None,
                                      float(default_bg), int(geom), drr.ptr))
+
+
+def line_int_interp(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_for_proj: np.ndarray, step: float,
+                    drr: DeviceBuffer, interp: int, kernel: int = XRH_LINE_INT_SUM, store: int = XRH_STORE_REPLACE,
+                    bg: DeviceBuffer | None = None, default_bg: float = 0.0) -> None:
+    """line_int with RayCaster::InterpMethod `interp` (XRH_INTERP_*), FP32 ray set-up
+    (xrh_line_int_interp). The first nearest-neighbour / sinc and the first B-spline call on a
+    volume build its interpolation grid (synchronous, counted in the volume's bytes)."""
+    x = np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16)
+    c4p = np.ascontiguousarray(cam_for_proj, dtype=np.int64)
+    ca = _cams_array(cams)
+    check(lib().xrh_line_int_interp(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64), x.shape[0],
+                                    float(step), int(kernel), int(store), bg.ptr if bg else None,
+                                    float(default_bg), int(interp), drr.ptr))
 
 
 def line_int_sample_count(ctx: Context, vol: Volume, cams, xforms, cam_for_proj, step: float,

@@ -373,9 +373,6 @@ void RayCasterLineIntHIP::compute(size_type vol_idx)
 {
   if (!resources_allocated_) throw StringMessageException("allocate_resources() must be called first");
   if (vol_idx >= dev_vols_.size()) throw StringMessageException("volume index out of range");
-  if (interp_method_ != kRAY_CAST_INTERP_LINEAR) {
-    throw UnsupportedOperationException();  // linear only, as RayCasterOCL (xregRayCastBaseOCL.cpp:338-341)
-  }
   const size_type need = num_projs_ * camera_models_[0].num_det_rows * camera_models_[0].num_det_cols;
   DeviceDrr& out = out_drr(need);
   const auto cams = xrh_cams();
@@ -407,8 +404,19 @@ void RayCasterLineIntHIP::compute(size_type vol_idx)
     check_xrh(xrh_copy_h2d(ctx_, out.dev, ext_host_buf_, need * sizeof(float)));
   }
   const int32_t k = (kernel_id_ == kRAY_CAST_LINE_INT_MAX_KERNEL) ? XRH_LINE_INT_MAX : XRH_LINE_INT_SUM;
-  check_xrh(xrh_line_int(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(), c4p.data(),
-                         (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, out.dev));
+  if (interp_method_ == kRAY_CAST_INTERP_LINEAR) {
+    check_xrh(xrh_line_int(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(), c4p.data(),
+                           (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, out.dev));
+  } else {
+    // the CPU caster's other ITK interpolators (xregRayCastLineIntCPU.cpp:126-147), cast with
+    // the reference's float ray set-up whatever geom_ says
+    const int32_t interp = (interp_method_ == kRAY_CAST_INTERP_NN)     ? XRH_INTERP_NN
+                           : (interp_method_ == kRAY_CAST_INTERP_SINC) ? XRH_INTERP_SINC
+                                                                       : XRH_INTERP_BSPLINE;
+    check_xrh(xrh_line_int_interp(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(),
+                                  c4p.data(), (int64_t)num_projs_, ray_step_size_, k, store, bg,
+                                  default_bg_pixel_val_, interp, out.dev));
+  }
   out.version = DeviceDrr::next_version();  // every caster sharing the buffer now sees a stale host copy
 }
 

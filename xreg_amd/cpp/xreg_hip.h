@@ -170,6 +170,8 @@ class RayCaster {
   InterpMethod interp_method() const { return interp_method_; }
   void use_linear_interp() { interp_method_ = kRAY_CAST_INTERP_LINEAR; }
   void use_nn_interp() { interp_method_ = kRAY_CAST_INTERP_NN; }
+  void use_sinc_interp() { interp_method_ = kRAY_CAST_INTERP_SINC; }
+  void use_bspline_interp() { interp_method_ = kRAY_CAST_INTERP_BSPLINE; }
   void set_xforms_cam_to_itk_phys(const FrameTransformList& xforms);
   const FrameTransformList& xforms_cam_to_itk_phys() const { return xforms_cam_to_itk_phys_; }
   FrameTransform& xform_cam_to_itk_phys(size_type i) { return xforms_cam_to_itk_phys_[i]; }

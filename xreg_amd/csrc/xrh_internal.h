@@ -197,6 +197,22 @@ struct xrh_ctx {
   std::map<std::string, xrh::DrrVariant> tuned;
 };
 
+namespace xrh {
+// A voxel grid for the non-trilinear interpolators (float voxels, or double B-spline
+// coefficients), x fastest, with an apron of `apron` elements on every side: element
+// (x + apron) + py (y + apron) + pz (z + apron) belongs to voxel (x, y, z).
+struct InterpGrid {
+  const void* data = nullptr;
+  uint32_t bytes = 0;
+  uint32_t py = 0, pz = 0;
+  int32_t apron = 0;
+};
+// grids of a volume: the voxels with a zero apron (nearest neighbour, windowed sinc) and the
+// cubic B-spline coefficients with a mirrored apron
+constexpr int kGridVoxels = 0;
+constexpr int kGridBSpline = 1;
+}  // namespace xrh
+
 struct xrh_volume {
   xrh_ctx* ctx = nullptr;
   xrh::VolDesc desc{};
@@ -204,6 +220,10 @@ struct xrh_volume {
   void* skip = nullptr;
   uint64_t bytes = 0;
   int64_t size[3] = {0, 0, 0};
+  // built on first use by an interpolating cast (xrh_line_int_interp), freed with the volume
+  xrh::InterpGrid grid[2];
+  void* grid_mem[2] = {nullptr, nullptr};
+  uint64_t grid_bytes[2] = {0, 0};
 };
 
 namespace xrh {
@@ -271,5 +291,10 @@ bool drr_default_reuse(int64_t workgroups);
 constexpr int32_t kMaxDrrBands = 16;
 void launch_drr_setup(const DrrLaunch& L, hipStream_t s);
 void launch_drr(const DrrLaunch& L, hipStream_t s);
+// Interpolating casts (xrh_interp.hip). interp_prepare builds the grid `interp` reads, once per
+// volume (synchronous; allocates from the volume's context); launch_drr_interp runs the cast of
+// a set-up launch with FP32 ray geometry (L.geom is ignored) on that grid.
+const InterpGrid& interp_prepare(xrh_volume* vol, int32_t interp);
+void launch_drr_interp(const DrrLaunch& L, const InterpGrid& g, int32_t interp, hipStream_t s);
 
 }  // namespace xrh

@@ -548,7 +548,8 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
                             int32_t num_cams, const float* xforms, const int64_t* cam_for_proj,
                             int64_t num_projs, float step, int32_t kernel, int32_t store,
                             const float* bg_dev, float default_bg, int32_t geom, float* drr_dev,
-                            uint64_t* samples_out, const FixedStaging* fixed = nullptr)
+                            uint64_t* samples_out, const FixedStaging* fixed = nullptr,
+                            int32_t interp = XRH_INTERP_LINEAR, const InterpGrid* igrid = nullptr)
 {
   XRH_REQUIRE(ctx && vol && cams && xforms && cam_for_proj, "null argument");
   XRH_REQUIRE(num_cams >= 1 && num_projs >= 1, "need at least one camera and one projection");
@@ -656,7 +657,12 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   L.reuse = var.reuse;
   L.wave_shape = var.shape;
   launch_drr_setup(L, ctx->stream);
-  launch_drr(L, ctx->stream);
+  if (interp != XRH_INTERP_LINEAR && !samples_out) {
+    XRH_REQUIRE(igrid, "interpolation grid not prepared");
+    launch_drr_interp(L, *igrid, interp, ctx->stream);
+  } else {
+    launch_drr(L, ctx->stream);
+  }
   if (samples_out) {
     XRH_HIP_CHECK(hipMemcpyAsync(samples_out, dev + off_cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
     XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -872,11 +878,17 @@ xrh_status xrh_volume_destroy(xrh_volume* vol)
     XRH_HIP_CHECK(hipStreamSynchronize(vol->ctx->stream));
     ctx_free(vol->ctx, vol->data);
     if (vol->skip) ctx_free(vol->ctx, vol->skip);
+    for (void* g : vol->grid_mem) {
+      if (g) ctx_free(vol->ctx, g);
+    }
     delete vol;
   });
 }
 
-uint64_t xrh_volume_bytes(const xrh_volume* vol) { return vol ? vol->bytes : 0; }
+uint64_t xrh_volume_bytes(const xrh_volume* vol)
+{
+  return vol ? vol->bytes + vol->grid_bytes[0] + vol->grid_bytes[1] : 0;
+}
 
 xrh_status xrh_volume_info(const xrh_volume* vol, int32_t* storage, int32_t* layout)
 {
@@ -922,6 +934,24 @@ xrh_status xrh_line_int_noalloc(xrh_ctx* ctx, const xrh_volume* vol, const xrh_c
                     bg_dev, default_bg, geom, drr_dev, nullptr, &st);
     // the staging is free for the next call (of any context on any device) once this work ends
     XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+xrh_status xrh_line_int_interp(xrh_ctx* ctx, xrh_volume* vol, const xrh_camera* cams,
+                               int32_t num_cams, const float* xforms_cam_to_phys,
+                               const int64_t* cam_for_proj, int64_t num_projs, float step,
+                               int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
+                               int32_t interp, float* drr_dev)
+{
+  return guarded([&] {
+    XRH_REQUIRE(interp == XRH_INTERP_LINEAR || interp == XRH_INTERP_NN || interp == XRH_INTERP_SINC ||
+                    interp == XRH_INTERP_BSPLINE,
+                "bad interpolation method");
+    XRH_REQUIRE(ctx && vol, "null argument");
+    XRH_REQUIRE(vol->ctx == ctx, "volume belongs to another context");
+    const InterpGrid* g = (interp == XRH_INTERP_LINEAR) ? nullptr : &interp_prepare(vol, interp);
+    line_int_launch(ctx, vol, cams, num_cams, xforms_cam_to_phys, cam_for_proj, num_projs, step, kernel, store,
+                    bg_dev, default_bg, XRH_GEOM_FP32, drr_dev, nullptr, nullptr, interp, g);
   });
 }
 
