@@ -223,29 +223,40 @@ xrh_status xrh_line_int_noalloc(xrh_ctx* ctx, const xrh_volume* vol, const xrh_c
                                 int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
                                 int32_t geom, float* drr_dev);
 
-/* xrh_line_int with the volume interpolator chosen by RayCaster::set_interp_method
- * (xregRayCastInterface.h:165-191), as RayCasterLineIntCPU evaluates it at every sample
- * (xregRayCastLineIntCPU.cpp:115-147, 270-277). Same arguments as xrh_line_int except:
- *   interp   XRH_INTERP_*; XRH_INTERP_LINEAR is xrh_line_int with XRH_GEOM_FP32
+/* xrh_line_int with RayCasterLineIntCPU's other sampling options: the volume interpolator
+ * chosen by RayCaster::set_interp_method (xregRayCastInterface.h:165-191; the interpolators of
+ * xregRayCastLineIntCPU.cpp:115-147, evaluated at every sample, :270-277) and the anti-aliasing
+ * factor of RayCasterCPU::set_anti_alias_factor (xregRayCastBaseCPU.h:113-130; jittered rays
+ * averaged per pixel, xregRayCastLineIntCPU.cpp:150-183). Same arguments as xrh_line_int except:
+ *   interp   XRH_INTERP_*
+ *   aa_rays  0: one ray through each pixel's index (no anti-aliasing); k in 1..XRH_MAX_AA_RAYS:
+ *            k rays through the index jittered by uniform [-0.5, 0.5) offsets in column and row,
+ *            their values averaged (aa_sum += value * (1/k), float) and the average stored with
+ *            the line kernel; k = 1 still jitters, as in the reference
+ *   aa_seed  the jitter is a pure function of (aa_seed, projection * rows * cols + pixel, ray,
+ *            component) -- counter-based SplitMix64 -- so a seed reproduces a cast exactly; the
+ *            reference seeds from std::random_device and never repeats. Ignored when aa_rays = 0.
  *   (no geom) rays are always set up with XRH_GEOM_FP32, the reference's own arithmetic, so
  *            each sample lands where the reference's does (nearest-neighbour lookups depend
  *            on it)
- * Nearest neighbour rounds each continuous index half up as ITK 5.1 does on x86-64;
- * windowed sinc takes the 8x8x8 voxels around floor(index) with Lanczos-4 weights and zeros
- * outside the volume; the B-spline samples 4x4x4 cubic B-spline coefficients (computed on the
- * device in double with ITK's mirror-boundary recursive filter) with mirrored indices. Sinc and
- * B-spline weights and sums are FP64, as ITK evaluates them, each sample rounded to float once;
- * the line sums are FP32 as in the reference. The first call per volume for nearest neighbour
- * or sinc (one grid, 4 B per voxel) and for B-spline (another, 8 B per voxel) builds that grid:
- * synchronous, allocated from the volume's context, counted in xrh_volume_bytes and freed with
- * the volume;
- * the volume is therefore not const here. A volume's grids are built from its stored voxels,
- * so a 16-bit volume interpolates its 16-bit values. Empty-space skipping is not used. */
+ * XRH_INTERP_LINEAR with aa_rays = 0 is xrh_line_int with XRH_GEOM_FP32. Otherwise: nearest
+ * neighbour rounds each continuous index half up as ITK 5.1 does on x86-64; windowed sinc takes
+ * the 8x8x8 voxels around floor(index) with Lanczos-4 weights and zeros outside the volume; the
+ * B-spline samples 4x4x4 cubic B-spline coefficients (computed on the device in double with
+ * ITK's mirror-boundary recursive filter) with mirrored indices; linear (with anti-aliasing)
+ * follows ITK's LinearInterpolateImageFunction. Their weights and sums are FP64, as ITK
+ * evaluates them, each sample rounded to float once; the line sums are FP32 as in the reference.
+ * The first such call per volume for linear, nearest neighbour or sinc (one grid, 4 B per voxel)
+ * and for B-spline (another, 8 B per voxel) builds that grid: synchronous, allocated from the
+ * volume's context, counted in xrh_volume_bytes and freed with the volume; the volume is
+ * therefore not const here. A volume's grids are built from its stored voxels, so a 16-bit
+ * volume interpolates its 16-bit values. Empty-space skipping is not used. */
+#define XRH_MAX_AA_RAYS 4096
 xrh_status xrh_line_int_interp(xrh_ctx* ctx, xrh_volume* vol, const xrh_camera* cams,
                                int32_t num_cams, const float* xforms_cam_to_phys,
                                const int64_t* cam_for_proj, int64_t num_projs, float step,
                                int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
-                               int32_t interp, float* drr_dev);
+                               int32_t interp, int32_t aa_rays, uint64_t aa_seed, float* drr_dev);
 
 /* Exact number of trilinear samples the reference algorithm takes for the same call (sum
  * over rays of n+1, the S_pose of the roofline byte model) and, optionally, how many of them

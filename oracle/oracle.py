@@ -100,9 +100,10 @@ def p2i_f64(idx_to_phys: np.ndarray) -> np.ndarray:
 
 
 def drr_f32(vol_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, kernel=0, store=0, bg=None,
-            default_bg=0.0, out=None, threads=0, interp=0):
+            default_bg=0.0, out=None, threads=0, interp=0, aa_rays=0, aa_seed=0):
     """RayCasterLineIntCPU restatement. Returns (drr [n, rows, cols] float32, samples).
-    interp: RayCaster::InterpMethod (0 linear, 1 nearest neighbour, 2 windowed sinc, 3 B-spline)."""
+    interp: RayCaster::InterpMethod (0 linear, 1 nearest neighbour, 2 windowed sinc, 3 B-spline);
+    aa_rays: RayCasterCPU's anti-aliasing factor (0 off), jitter drawn by aa_jitter(aa_seed, ...)."""
     vol = np.ascontiguousarray(vol_zyx, dtype=np.float32)
     dims = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], dtype=np.int64)
     x = np.ascontiguousarray(np.asarray(xforms, dtype=np.float32).reshape(-1, 16))
@@ -119,8 +120,8 @@ def drr_f32(vol_zyx, idx_to_phys, cams, xforms, cam_for_proj=None, step=1.0, ker
     s = C.c_int64(0)
     p2i = p2i_f32(idx_to_phys)
     lib().orc_drr_interp_f32(_f(vol), _i64(dims), _f(p2i), ca, len(cams), _f(x), _i64(c4p), C.c_int64(n),
-                             C.c_float(step), kernel, store, bgp, C.c_float(default_bg), int(interp), _f(out),
-                             threads, C.byref(s))
+                             C.c_float(step), kernel, store, bgp, C.c_float(default_bg), int(interp),
+                             int(aa_rays), C.c_uint64(int(aa_seed) & (2**64 - 1)), _f(out), threads, C.byref(s))
     return out, int(s.value)
 
 
@@ -330,6 +331,14 @@ def itk_interp_points(vol_zyx, idx, interp):
     for i, p in enumerate(pts):
         out[i] = fn(_f(vol), _i64(dims), _f(np.ascontiguousarray(p)))
     return out
+
+
+def aa_jitter(seed, pixel, k, comp):
+    """orc_aa_jitter: the jitter of ray k of range index `pixel`, component 0 (column) or 1 (row)."""
+    fn = lib().orc_aa_jitter
+    fn.restype = C.c_float
+    fn.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.c_int32]
+    return float(fn(int(seed) & (2**64 - 1), int(pixel), int(k), int(comp)))
 
 
 def bspline_coefficients(vol_zyx):

@@ -572,14 +572,110 @@ typedef struct {
   int32_t steps_only; /* set-up only: write steps, take no samples, leave out untouched */
   int32_t interp;     /* RayCaster::InterpMethod: 0 linear, 1 NN, 2 sinc, 3 B-spline */
   const double* coef; /* B-spline coefficients (interp 3) */
+  int32_t aa_rays;    /* RayCasterCPU::aa_fact_: 0 off, k >= 1 jittered rays per pixel */
+  uint64_t aa_seed;   /* seed of orc_aa_jitter */
 } drr_f32_ctx;
+
+/* SplitMix64's output function (Steele, Lea and Flood, OOPSLA 2014). */
+static uint64_t splitmix64(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* Anti-aliasing jitter of ray k of range index `pixel`, component 0 (column) or 1 (row): a
+ * uniform draw on [-0.5, 0.5) in steps of 2^-24, standing in for the reference's
+ * uniform_real_distribution<float>(-0.5, 0.5) over a random_device-seeded mt19937
+ * (xregRayCastLineIntCPU.cpp:162-174), which no two runs share; this one is a pure function of
+ * its arguments (counter-based SplitMix64), the same as the device's. */
+float orc_aa_jitter(uint64_t seed, uint64_t pixel, int32_t k, int32_t comp)
+{
+  const uint64_t counter = (pixel << 17) | ((uint64_t)k << 1) | (uint64_t)comp;
+  const uint64_t h = splitmix64(seed + 0x9E3779B97F4A7C15ull * (counter + 1));
+  return (float)(uint32_t)(h >> 40) * 0x1.0p-24f - 0.5f;
+}
+
+/* One ray through detector index (colf, rowf) of `cam` under pose-composed M (:198-279):
+ * returns the line value (the kernel's initial value when the ray misses the volume) and, in
+ * *n_out, the last sample index (-1: miss). With take_samples = 0 only the set-up runs. */
+static float ray_f32(const drr_f32_ctx* c, const orc_cam_f* cam, const float M[16], float colf, float rowf,
+                     int take_samples, int64_t* n_out, int64_t* samples)
+{
+  const float aabb_min[3] = {0.0f, 0.0f, 0.0f};
+  const float kTol = 1.0e-3f; /* RayCasterCPU::kVOL_BB_STEP_INC_TOL (xregRayCastBaseCPU.h:37) */
+  float det[3];
+  orc_cam_ind_to_det_f32(cam, colf, rowf, det);                               /* :204 */
+  float ph[3], dp[3], d[3];
+  orc_affine_pt_f32(M, cam->pinhole, ph);                                    /* :211 */
+  orc_affine_pt_f32(M, det, dp);                                             /* :215-216 */
+  d[0] = dp[0] - ph[0];
+  d[1] = dp[1] - ph[1];
+  d[2] = dp[2] - ph[2];
+
+  float acc = (c->kernel == 0) ? 0.0f : -FLT_MAX;                           /* :221 */
+  float t0 = 0.0f, t1 = 0.0f;
+  const int32_t hit = orc_ray_rect_f32(aabb_min, c->aabb_max, ph, d, 1, &t0, &t1); /* :227 */
+  *n_out = -1;
+  if (hit && ((t1 - t0) > (float)(2.0f * kTol))) {                            /* :233 */
+    t0 += kTol;
+    t1 -= kTol;
+    const float start[3] = {ph[0] + t0 * d[0], ph[1] + t0 * d[1], ph[2] + t0 * d[2]};
+    /* Eigen norm(): sqrt of the linear 3-element redux of squares */
+    const float dlen = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);   /* :243-244 */
+    const float L = (t1 - t0) * dlen;                                        /* :245-246 */
+    /* step length in index space: ||M3x3 * (normalized(det - pinhole) * step)|| (:249-251) */
+    float v[3] = {det[0] - cam->pinhole[0], det[1] - cam->pinhole[1], det[2] - cam->pinhole[2]};
+    const float vz = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    if (vz > 0.0f) {
+      const float vn = sqrtf(vz);
+      v[0] = v[0] / vn;
+      v[1] = v[1] / vn;
+      v[2] = v[2] / vn;
+    }
+    v[0] = v[0] * c->step;
+    v[1] = v[1] * c->step;
+    v[2] = v[2] * c->step;
+    float sv[3];
+    linear_vec_f32(M, v, sv);
+    const float s = sqrtf((sv[0] * sv[0] + sv[1] * sv[1]) + sv[2] * sv[2]);
+    const int64_t n = (int64_t)(L / s);                                      /* :253-254 */
+    *n_out = n;
+    if (!take_samples) return acc;
+    const float scale = s / dlen;                                            /* :263-264 */
+    const float dx = d[0] * scale, dy = d[1] * scale, dz = d[2] * scale;
+    float p[3] = {start[0], start[1], start[2]};
+    for (int64_t i = 0; i <= n; ++i) {                                       /* :270-277 */
+      double dv;
+      switch (c->interp) {
+        case 1: dv = orc_itk_nn_interp(c->vol, c->dims, p); break;
+        case 2: dv = orc_itk_sinc_interp(c->vol, c->dims, p); break;
+        case 3: dv = orc_itk_bspline_interp(c->coef, c->dims, p); break;
+        default: dv = orc_itk_linear_interp(c->vol, c->dims, p); break;
+      }
+      const float val = (float)dv;
+      if (c->kernel == 0) {
+        acc = acc + val;
+      } else {
+        acc = (acc < val) ? val : acc; /* std::max(a, b) = (a < b) ? b : a */
+      }
+      p[0] = p[0] + dx;
+      p[1] = p[1] + dy;
+      p[2] = p[2] + dz;
+    }
+    *samples += n + 1;
+    acc = acc * c->step;                                                     /* :279 */
+  }
+  return acc;
+}
 
 static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
 {
   drr_f32_ctx* c = (drr_f32_ctx*)vctx;
-  const float aabb_min[3] = {0.0f, 0.0f, 0.0f};
-  const float kTol = 1.0e-3f; /* RayCasterCPU::kVOL_BB_STEP_INC_TOL (xregRayCastBaseCPU.h:37) */
   int64_t samples = 0;
+  /* rays per pixel and their weight (:151-157) */
+  const int32_t rays = (c->aa_rays > 0) ? c->aa_rays : 1;
+  const float inv = 1.0f / (float)rays;
   for (int64_t r = begin; r < end; ++r) {
     /* index decomposition uses camera 0's size (:151-152, 179-183) */
     const int64_t proj = r / c->npx;
@@ -587,78 +683,27 @@ static void drr_f32_range(void* vctx, int64_t begin, int64_t end)
     const int64_t row = off / c->cols0;
     const int64_t col = off - (int64_t)c->cols0 * row;
     const orc_cam_f* cam = &c->cams[c->cam_for_proj[proj]];
-
-    float det[3];
-    orc_cam_ind_to_det_f32(cam, (float)col, (float)row, det);                 /* :204 */
     float M[16];
     orc_affine_mul_f32(c->p2i, &c->xforms[16 * proj], M);                      /* :207-208 */
-    float ph[3], dp[3], d[3];
-    orc_affine_pt_f32(M, cam->pinhole, ph);                                    /* :211 */
-    orc_affine_pt_f32(M, det, dp);                                             /* :215-216 */
-    d[0] = dp[0] - ph[0];
-    d[1] = dp[1] - ph[1];
-    d[2] = dp[2] - ph[2];
-
-    float acc = (c->kernel == 0) ? 0.0f : -FLT_MAX;                           /* :221 */
-    float t0 = 0.0f, t1 = 0.0f;
-    const int32_t hit = orc_ray_rect_f32(aabb_min, c->aabb_max, ph, d, 1, &t0, &t1); /* :227 */
-    if (c->steps) c->steps[r] = -1;
-    if (hit && ((t1 - t0) > (float)(2.0f * kTol))) {                            /* :233 */
-      t0 += kTol;
-      t1 -= kTol;
-      const float start[3] = {ph[0] + t0 * d[0], ph[1] + t0 * d[1], ph[2] + t0 * d[2]};
-      /* Eigen norm(): sqrt of the linear 3-element redux of squares */
-      const float dlen = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);   /* :243-244 */
-      const float L = (t1 - t0) * dlen;                                        /* :245-246 */
-      /* step length in index space: ||M3x3 * (normalized(det - pinhole) * step)|| (:249-251) */
-      float v[3] = {det[0] - cam->pinhole[0], det[1] - cam->pinhole[1], det[2] - cam->pinhole[2]};
-      const float vz = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
-      if (vz > 0.0f) {
-        const float vn = sqrtf(vz);
-        v[0] = v[0] / vn;
-        v[1] = v[1] / vn;
-        v[2] = v[2] / vn;
+    float aa_sum = 0.0f;
+    int32_t k;
+    for (k = 0; k < rays; ++k) {                                               /* :186-283 */
+      float colf = (float)col, rowf = (float)row;
+      if (c->aa_rays > 0) {
+        colf += orc_aa_jitter(c->aa_seed, (uint64_t)r, k, 0);
+        rowf += orc_aa_jitter(c->aa_seed, (uint64_t)r, k, 1);
       }
-      v[0] = v[0] * c->step;
-      v[1] = v[1] * c->step;
-      v[2] = v[2] * c->step;
-      float sv[3];
-      linear_vec_f32(M, v, sv);
-      const float s = sqrtf((sv[0] * sv[0] + sv[1] * sv[1]) + sv[2] * sv[2]);
-      const int64_t n = (int64_t)(L / s);                                      /* :253-254 */
-      if (c->steps) c->steps[r] = (int32_t)n;
-      if (c->steps_only) continue;
-      const float scale = s / dlen;                                            /* :263-264 */
-      const float dx = d[0] * scale, dy = d[1] * scale, dz = d[2] * scale;
-      float p[3] = {start[0], start[1], start[2]};
-      for (int64_t i = 0; i <= n; ++i) {                                       /* :270-277 */
-        double dv;
-        switch (c->interp) {
-          case 1: dv = orc_itk_nn_interp(c->vol, c->dims, p); break;
-          case 2: dv = orc_itk_sinc_interp(c->vol, c->dims, p); break;
-          case 3: dv = orc_itk_bspline_interp(c->coef, c->dims, p); break;
-          default: dv = orc_itk_linear_interp(c->vol, c->dims, p); break;
-        }
-        const float val = (float)dv;
-        if (c->kernel == 0) {
-          acc = acc + val;
-        } else {
-          acc = (acc < val) ? val : acc; /* std::max(a, b) = (a < b) ? b : a */
-        }
-        p[0] = p[0] + dx;
-        p[1] = p[1] + dy;
-        p[2] = p[2] + dz;
-      }
-      samples += n + 1;
-      acc = acc * c->step;                                                     /* :279 */
+      int64_t n;
+      const float acc = ray_f32(c, cam, M, colf, rowf, !c->steps_only, &n, &samples);
+      if (c->steps && k == 0) c->steps[r] = (int32_t)n;
+      aa_sum += acc * inv;                                                     /* :282 */
     }
     if (c->steps_only) continue;
-    /* aa_sum = acc * 1 (:282), store with the same kernel (:285) */
-    const float aa = 0.0f + acc * 1.0f;
+    /* store with the same kernel (:285) */
     if (c->kernel == 0) {
-      c->out[r] = c->out[r] + aa;
+      c->out[r] = c->out[r] + aa_sum;
     } else {
-      c->out[r] = (c->out[r] < aa) ? aa : c->out[r];
+      c->out[r] = (c->out[r] < aa_sum) ? aa_sum : c->out[r];
     }
   }
   __atomic_fetch_add(&c->samples, samples, __ATOMIC_RELAXED);
@@ -687,18 +732,20 @@ void orc_drr_f32(const float* vol, const int64_t dims[3], const float p2i[16],
                  float* out, int32_t threads, int64_t* samples_out)
 {
   orc_drr_interp_f32(vol, dims, p2i, cams, num_cams, xforms, cam_for_proj, num_projs, step, kernel, store, bg,
-                     default_bg, 0, out, threads, samples_out);
+                     default_bg, 0, 0, 0, out, threads, samples_out);
 }
 
 /* orc_drr_f32 with the interpolator RayCasterLineIntCPU picks from interp_method
  * (xregRayCastLineIntCPU.cpp:126-147): 0 linear, 1 nearest neighbour, 2 windowed sinc,
  * 3 cubic B-spline (coefficients computed once here; the reference recomputes them in
- * every range's interpolator from the same volume, with the same result). */
+ * every range's interpolator from the same volume, with the same result), and aa_rays
+ * jittered rays per pixel (RayCasterCPU::aa_fact_, :150-183; orc_aa_jitter from aa_seed). */
 void orc_drr_interp_f32(const float* vol, const int64_t dims[3], const float p2i[16],
                         const orc_cam_f* cams, int32_t num_cams,
                         const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
                         float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
-                        int32_t interp, float* out, int32_t threads, int64_t* samples_out)
+                        int32_t interp, int32_t aa_rays, uint64_t aa_seed, float* out, int32_t threads,
+                        int64_t* samples_out)
 {
   (void)num_cams;
   drr_f32_ctx c;
@@ -710,6 +757,8 @@ void orc_drr_interp_f32(const float* vol, const int64_t dims[3], const float p2i
   }
   c.interp = interp;
   c.coef = coef;
+  c.aa_rays = aa_rays;
+  c.aa_seed = aa_seed;
   c.vol = vol;
   c.dims = dims;
   /* ITKImageIndexBoundsAsEigen: max index = size - 1 (lib/itk/xregITKBasicImageUtils.h:56-75) */

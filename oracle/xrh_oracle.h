@@ -87,12 +87,16 @@ void orc_drr_f32(const float* vol, const int64_t dims[3], const float p2i[16],
                  const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
                  float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
                  float* out, int32_t threads, int64_t* samples_out);
-/* the same with RayCaster::InterpMethod interp: 0 linear, 1 NN, 2 sinc, 3 B-spline */
+/* the same with RayCaster::InterpMethod interp (0 linear, 1 NN, 2 sinc, 3 B-spline) and
+ * RayCasterCPU's anti-aliasing: aa_rays jittered rays per pixel (0: off), jitter from aa_seed */
 void orc_drr_interp_f32(const float* vol, const int64_t dims[3], const float p2i[16],
                         const orc_cam_f* cams, int32_t num_cams,
                         const float* xforms, const int64_t* cam_for_proj, int64_t num_projs,
                         float step, int32_t kernel, int32_t store, const float* bg, float default_bg,
-                        int32_t interp, float* out, int32_t threads, int64_t* samples_out);
+                        int32_t interp, int32_t aa_rays, uint64_t aa_seed, float* out, int32_t threads,
+                        int64_t* samples_out);
+/* the jitter of ray k of range index `pixel`, component 0 (column) / 1 (row): [-0.5, 0.5) */
+float orc_aa_jitter(uint64_t seed, uint64_t pixel, int32_t k, int32_t comp);
 
 void orc_drr_f64(const double* vol, const int64_t dims[3], const double p2i[16],
                  const orc_cam_d* cams, int32_t num_cams,

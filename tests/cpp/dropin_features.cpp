@@ -2,7 +2,8 @@
 // the way xReg's callers use it:
 //   * set_kernel_id(MAX) on the caster (RayCastLineIntParamInterface, xregRayCastInterface.h:581-591);
 //   * nearest-neighbour, B-spline and windowed-sinc interpolation (set_interp_method,
-//     xregRayCastInterface.h:165-191);
+//     xregRayCastInterface.h:165-191) and anti-aliasing (set_anti_alias_factor,
+//     xregRayCastBaseCPU.h:113-130);
 //   * two objects cast into the same projections, REPLACE then ACCUM, through compute(vol_idx)
 //     (Intensity2D3DRegi::obj_fn, xregIntensity2D3DRegi.cpp:593-625);
 //   * a static volume's DRR as the background of the first cast (has_a_static_vol_, same lines;
@@ -154,6 +155,13 @@ int main(int argc, char** argv)
     rc->compute(0);
     rec.put("sinc_drr", std::vector<float>(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + np * npx));
     rc->use_linear_interp();
+    // anti-aliasing (RayCasterCPU::set_anti_alias_factor, xregRayCastBaseCPU.h:113-130) with a
+    // fixed jitter seed, then back to one ray per pixel
+    rc->set_anti_alias_factor(3);
+    rc->set_anti_alias_seed(20240611);
+    rc->compute(0);
+    rec.put("aa_drr", std::vector<float>(rc->raw_host_pixel_buf(), rc->raw_host_pixel_buf() + np * npx));
+    rc->set_anti_alias_factor(0);
 
     // line integrals of A into an external host buffer; proj(i) reads the same memory
     std::vector<float> ext(np * npx, -1.0f);

@@ -82,6 +82,9 @@ def test_dropin_features(tmp_path, oracle, metric, smooth):
             assert np.array_equal(f32(name, shp), ref_i)
         else:
             assert drr_rel_err(f32(name, shp), ref_i).max() <= DRR_REL_TOL, name
+    # anti-aliasing: 3 jittered rays per pixel with the seed the driver fixed
+    ref_aa, _ = oracle.drr_f32(vol_a, w.idx_to_phys, w.cams, poses_f32(pa), step=w.step, aa_rays=3, aa_seed=20240611)
+    assert np.array_equal(f32("aa_drr", shp), ref_aa)
     # external host buffer: raw_host_pixel_buf() is the caller's memory and proj(i) reads it
     assert u64("ext_bad") == 0
     assert drr_rel_err(f32("ext_drr", shp), ref_a).max() <= DRR_REL_TOL

@@ -6,7 +6,8 @@ through xrh_line_int_interp, against the oracle on the same volume, rays and pos
   * cubic B-spline and Lanczos-4 windowed sinc: per pixel within 1e-4 relative (both sides
     evaluate a sample in double and round it to float; the float line sums match).
 Also: MAX projections, ACCUM onto background images, 16-bit storage (the grid holds the stored
-16-bit values), every FP32 layout giving the same DRRs, and the grids' bytes."""
+16-bit values), every FP32 layout giving the same DRRs, the grids' bytes, and anti-aliasing
+(jittered rays averaged per pixel) against the oracle with the same jitter seed."""
 import numpy as np
 import pytest
 
@@ -26,7 +27,8 @@ def c1i():
     return w, G.downsample_camera(w.cams[0], 0.5)
 
 
-def _cast(ctx, vol, cam, poses, step, interp, kernel=0, store=0, bg=None, default_bg=0.0, prefill=None):
+def _cast(ctx, vol, cam, poses, step, interp, kernel=0, store=0, bg=None, default_bg=0.0, prefill=None, aa_rays=0,
+          aa_seed=0):
     from xreg_amd import _xrh as X
 
     n = poses.shape[0]
@@ -35,7 +37,7 @@ def _cast(ctx, vol, cam, poses, step, interp, kernel=0, store=0, bg=None, defaul
     if prefill is not None:
         buf.upload(np.ascontiguousarray(prefill, np.float32))
     X.line_int_interp(ctx, vol, [cam], poses, np.zeros(n, np.int64), step, buf, interp, kernel=kernel,
-                      store=store, bg=bg, default_bg=default_bg)
+                      store=store, bg=bg, default_bg=default_bg, aa_rays=aa_rays, aa_seed=aa_seed)
     out = buf.download(np.float32).reshape(n, cam.num_det_rows, cam.num_det_cols)
     buf.free()
     return out
@@ -151,3 +153,32 @@ def test_interp_grid_bytes_and_linear_passthrough(ctx, c1i):
     _cast(ctx, vol, cam, poses, w.step, 3)
     assert vol.nbytes - b1 == 8 * (X_ + 4) * (Y + 4) * (Z + 4)
     vol.close()
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("interp", [0, 1, 3])
+def test_antialiased_drr_matches_oracle(ctx, oracle, c1i, interp, kernel):
+    """RayCasterCPU::set_anti_alias_factor (xregRayCastBaseCPU.h:113-130): 4 jittered rays per
+    pixel, averaged; the same seed gives the oracle the same jitter. Linear (ITK's double lerp on
+    both sides) and nearest neighbour are bit for bit; B-spline within 1e-4."""
+    from xreg_amd import _xrh as X
+
+    w, cam = c1i
+    print(f"case: anti-aliasing 4 rays, interp {interp}, kernel {kernel}, 2 poses", flush=True)
+    poses = poses_f32(w.poses[:2])
+    seed = 0xC0FFEE
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    got = _cast(ctx, vol, cam, poses, w.step, interp, kernel=kernel, aa_rays=4, aa_seed=seed)
+    again = _cast(ctx, vol, cam, poses, w.step, interp, kernel=kernel, aa_rays=4, aa_seed=seed)
+    other = _cast(ctx, vol, cam, poses, w.step, interp, kernel=kernel, aa_rays=4, aa_seed=seed + 1)
+    vol.close()
+    ref, _ = oracle.drr_f32(w.vol, w.idx_to_phys, [cam], poses, step=w.step, kernel=kernel, interp=interp,
+                            aa_rays=4, aa_seed=seed)
+    err = drr_rel_err(got, ref)
+    print(f"max rel err {err.max():.3e}; pixels differing {int((got != ref).sum())} of {got.size}", flush=True)
+    assert np.array_equal(got, again)
+    assert not np.array_equal(got, other)
+    if interp in (0, 1):
+        assert np.array_equal(got, ref)
+    else:
+        assert err.max() <= DRR_REL_TOL

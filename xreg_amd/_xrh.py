@@ -123,7 +123,7 @@ SIGNATURES = {
     "xrh_line_int_noalloc": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
                                     _I32, _P]),
     "xrh_line_int_interp": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32, _I32, _P, _F,
-                                   _I32, _P]),
+                                   _I32, _I32, _U64, _P]),
     "xrh_line_int_sample_count": (_I32, [_P, _P, C.POINTER(Camera), _I32, _PF, _PI64, _I64, _F, _I32,
                                          _PU64, _PU64]),
     "xrh_fixed_create": (_I32, [_P, _PF, _I32, _I32, C.POINTER(SimParams), _PU8, C.POINTER(_P)]),
@@ -324,18 +324,24 @@ def line_int_noalloc(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_fo
                                      float(default_bg), int(geom), drr.ptr))
 
 
+XRH_MAX_AA_RAYS = 4096
+
+
 def line_int_interp(ctx: Context, vol: Volume, cams, xforms: np.ndarray, cam_for_proj: np.ndarray, step: float,
                     drr: DeviceBuffer, interp: int, kernel: int = XRH_LINE_INT_SUM, store: int = XRH_STORE_REPLACE,
-                    bg: DeviceBuffer | None = None, default_bg: float = 0.0) -> None:
-    """line_int with RayCaster::InterpMethod `interp` (XRH_INTERP_*), FP32 ray set-up
-    (xrh_line_int_interp). The first nearest-neighbour / sinc and the first B-spline call on a
-    volume build its interpolation grid (synchronous, counted in the volume's bytes)."""
+                    bg: DeviceBuffer | None = None, default_bg: float = 0.0, aa_rays: int = 0,
+                    aa_seed: int = 0) -> None:
+    """line_int with RayCaster::InterpMethod `interp` (XRH_INTERP_*) and RayCasterCPU's
+    anti-aliasing factor `aa_rays` (jitter reproducible from `aa_seed`), FP32 ray set-up
+    (xrh_line_int_interp). The first call on a volume that needs an interpolation grid builds it
+    (synchronous, counted in the volume's bytes)."""
     x = np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16)
     c4p = np.ascontiguousarray(cam_for_proj, dtype=np.int64)
     ca = _cams_array(cams)
     check(lib().xrh_line_int_interp(ctx.h, vol.h, ca, len(cams), _fp(x), c4p.ctypes.data_as(_PI64), x.shape[0],
                                     float(step), int(kernel), int(store), bg.ptr if bg else None,
-                                    float(default_bg), int(interp), drr.ptr))
+                                    float(default_bg), int(interp), int(aa_rays), int(aa_seed) & (2**64 - 1),
+                                    drr.ptr))
 
 
 def line_int_sample_count(ctx: Context, vol: Volume, cams, xforms, cam_for_proj, step: float,

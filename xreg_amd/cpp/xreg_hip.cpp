@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <random>
 
 namespace xreg {
 
@@ -404,18 +405,25 @@ void RayCasterLineIntHIP::compute(size_type vol_idx)
     check_xrh(xrh_copy_h2d(ctx_, out.dev, ext_host_buf_, need * sizeof(float)));
   }
   const int32_t k = (kernel_id_ == kRAY_CAST_LINE_INT_MAX_KERNEL) ? XRH_LINE_INT_MAX : XRH_LINE_INT_SUM;
-  if (interp_method_ == kRAY_CAST_INTERP_LINEAR) {
+  if (interp_method_ == kRAY_CAST_INTERP_LINEAR && aa_fact_ == 0) {
     check_xrh(xrh_line_int(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(), c4p.data(),
                            (int64_t)num_projs_, ray_step_size_, k, store, bg, default_bg_pixel_val_, geom_, out.dev));
   } else {
-    // the CPU caster's other ITK interpolators (xregRayCastLineIntCPU.cpp:126-147), cast with
-    // the reference's float ray set-up whatever geom_ says
-    const int32_t interp = (interp_method_ == kRAY_CAST_INTERP_NN)     ? XRH_INTERP_NN
-                           : (interp_method_ == kRAY_CAST_INTERP_SINC) ? XRH_INTERP_SINC
-                                                                       : XRH_INTERP_BSPLINE;
+    // the CPU caster's other ITK interpolators (xregRayCastLineIntCPU.cpp:126-147) and its
+    // anti-aliasing (:150-183), cast with the reference's float ray set-up whatever geom_ says
+    const int32_t interp = (interp_method_ == kRAY_CAST_INTERP_NN)        ? XRH_INTERP_NN
+                           : (interp_method_ == kRAY_CAST_INTERP_SINC)    ? XRH_INTERP_SINC
+                           : (interp_method_ == kRAY_CAST_INTERP_BSPLINE) ? XRH_INTERP_BSPLINE
+                                                                          : XRH_INTERP_LINEAR;
+    if (aa_fact_ > XRH_MAX_AA_RAYS) throw StringMessageException("anti-aliasing factor too large");
+    uint64_t seed = aa_seed_;
+    if (aa_fact_ && !aa_seed_fixed_) {
+      std::random_device rd;
+      seed = ((uint64_t)rd() << 32) ^ (uint64_t)rd();
+    }
     check_xrh(xrh_line_int_interp(ctx_, dev_vols_[vol_idx], cams.data(), (int32_t)cams.size(), xforms.data(),
                                   c4p.data(), (int64_t)num_projs_, ray_step_size_, k, store, bg,
-                                  default_bg_pixel_val_, interp, out.dev));
+                                  default_bg_pixel_val_, interp, (int32_t)aa_fact_, seed, out.dev));
   }
   out.version = DeviceDrr::next_version();  // every caster sharing the buffer now sees a stale host copy
 }

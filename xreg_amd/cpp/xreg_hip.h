@@ -257,6 +257,20 @@ class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterfac
   size_type device_pixel_capacity() const { return out_->buf ? out_->buf->capacity : 0; }
   // exact trilinear sample count of the next compute() (roofline accounting)
   uint64_t sample_count(size_type vol_idx = 0);
+  // RayCasterCPU::set_anti_alias_factor / anti_alias_factor (xregRayCastBaseCPU.h:113-130):
+  // 0 (default) casts one ray per pixel; k >= 1 casts k jittered rays per pixel and averages
+  // them (k = 1 still jitters).
+  void set_anti_alias_factor(size_type aa_fact) { aa_fact_ = aa_fact; }
+  size_type anti_alias_factor() const { return aa_fact_; }
+  // The jitter seed (no counterpart in the reference, which seeds from std::random_device on
+  // every compute()): fixed, the jitter and so the DRRs repeat exactly; by default every
+  // compute() draws a new seed from std::random_device, as the reference does.
+  void set_anti_alias_seed(uint64_t seed)
+  {
+    aa_seed_ = seed;
+    aa_seed_fixed_ = true;
+  }
+  void use_random_anti_alias_seed() { aa_seed_fixed_ = false; }
 
  protected:
   void vols_changed() override;
@@ -294,6 +308,9 @@ class RayCasterLineIntHIP : public RayCaster, public RayCastLineIntParamInterfac
   bool host_stale_ = true;
   const DeviceDrr* synced_buf_ = nullptr;  // the allocation and version the host copy reflects
   uint64_t synced_version_ = 0;
+  size_type aa_fact_ = 0;
+  uint64_t aa_seed_ = 0;
+  bool aa_seed_fixed_ = false;
 };
 
 // ------------------------------------------------------------------------------------------

@@ -452,7 +452,7 @@ template <int kGeom, int kOp, int kFmt>
 __device__ inline float cast_ray(const DrrItem& it, int col, int row)
 {
   float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
-  const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
+  const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, (float)col, (float)row) : setup_ray_f64(it, col, row);
   if (ray.n >= 0) {
     VolView vv;
     vv.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(it.vol), (short)0,
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(kThreads) void drr_count_kernel(
   lane_pixel(tr.tx, tr.ty, 0, col, row);
   double cnt = 0.0, evaluated = 0.0;
   if ((col < it.cam.cols) && (row < it.cam.rows)) {
-    const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, col, row) : setup_ray_f64(it, col, row);
+    const Ray ray = (kGeom == XRH_GEOM_FP32) ? setup_ray_f32(it, (float)col, (float)row) : setup_ray_f64(it, col, row);
     if (steps) steps[it.out_off + (int64_t)row * it.cam.cols + col] = ray.n;
     cnt = (double)(ray.n + 1);
     if (ray.n >= 0) {

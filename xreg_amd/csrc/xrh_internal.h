@@ -206,6 +206,7 @@ struct InterpGrid {
   uint32_t bytes = 0;
   uint32_t py = 0, pz = 0;
   int32_t apron = 0;
+  int32_t X = 0, Y = 0, Z = 0;  // the volume's extent (voxels)
 };
 // grids of a volume: the voxels with a zero apron (nearest neighbour, windowed sinc) and the
 // cubic B-spline coefficients with a mirrored apron
@@ -291,10 +292,13 @@ bool drr_default_reuse(int64_t workgroups);
 constexpr int32_t kMaxDrrBands = 16;
 void launch_drr_setup(const DrrLaunch& L, hipStream_t s);
 void launch_drr(const DrrLaunch& L, hipStream_t s);
-// Interpolating casts (xrh_interp.hip). interp_prepare builds the grid `interp` reads, once per
-// volume (synchronous; allocates from the volume's context); launch_drr_interp runs the cast of
-// a set-up launch with FP32 ray geometry (L.geom is ignored) on that grid.
+// Casts with RayCasterLineIntCPU's sampling options (xrh_interp.hip). interp_prepare builds the
+// grid `interp` reads, once per volume (synchronous; allocates from the volume's context; the
+// linear interpolator reads the nearest-neighbour grid); launch_drr_interp runs the cast of a
+// set-up launch with FP32 ray geometry (L.geom is ignored) on that grid, `aa_rays` jittered
+// rays per pixel (0: one ray through the pixel index, no jitter).
 const InterpGrid& interp_prepare(xrh_volume* vol, int32_t interp);
-void launch_drr_interp(const DrrLaunch& L, const InterpGrid& g, int32_t interp, hipStream_t s);
+void launch_drr_interp(const DrrLaunch& L, const InterpGrid& g, int32_t interp, int32_t aa_rays, uint64_t aa_seed,
+                       hipStream_t s);
 
 }  // namespace xrh

@@ -1,10 +1,12 @@
-// xrh_interp.hip -- line integrals with the CPU caster's other volume interpolators.
+// xrh_interp.hip -- line integrals with the CPU caster's other sampling options.
 //
 // RayCasterLineIntCPU builds one ITK interpolator per range and evaluates it at every sample
 // (lib/ray_cast/xregRayCastLineIntCPU.cpp:115-147, 270-277): nearest neighbour, windowed sinc
-// (radius 4, Lanczos window, constant-zero boundary) or an order-3 B-spline. The rays here are
-// the reference's own float set-up (setup_ray_f32) and samples advance by repeated float
-// addition, so every sample is taken at the reference's continuous index.
+// (radius 4, Lanczos window, constant-zero boundary) or an order-3 B-spline besides the linear
+// one; with an anti-aliasing factor k >= 1 it casts k rays per pixel through jittered detector
+// indices and averages them (:150-183). The rays here are the reference's own float set-up
+// (setup_ray_f32) and samples advance by repeated float addition, so every sample is taken at
+// the reference's continuous index.
 //
 // Mapping onto CDNA4: each interpolator reads a grid built once per volume from the stored
 // voxels, with an apron wide enough that no tap needs a bounds test:
@@ -141,6 +143,7 @@ __global__ void mirror_grid_kernel(const double* __restrict__ c, int X, int Y, i
 struct GridView {
   __amdgpu_buffer_rsrc_t rsrc;
   uint32_t py, pz;
+  int X, Y, Z;
 };
 
 __device__ inline float grid_load(const GridView& g, uint32_t e)
@@ -161,6 +164,41 @@ __device__ inline int round_half_up(float x) { return ((int)__builtin_rintf(x + 
 
 // itk::Math::Floor of a float index, the same way: round-to-nearest-even of 2x - 0.5, shifted.
 __device__ inline int itk_floor(float x) { return ((int)__builtin_rintf(x + x - 0.5f)) >> 1; }
+
+// LinearInterpolateImageFunction::EvaluateAtContinuousIndex (zero apron A = 4): per axis the
+// base floor(x) clamped to the volume, the float fraction x - base, and an upper neighbour that
+// falls back to the base past the last voxel (a fraction <= 0 weighs it 0); corners and lerps
+// in double, x then y then z, rounded to float once. Only the anti-aliased casts use it: plain
+// linear casts run the trilinear kernels.
+__device__ inline float sample_linear(const GridView& g, float x, float y, float z)
+{
+  const float p[3] = {x, y, z};
+  const int n[3] = {g.X, g.Y, g.Z};
+  int b[3], u[3];
+  double w[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int bk = min(max((int)floorf(p[k]), 0), n[k] - 1);
+    const float dk = p[k] - (float)bk;
+    b[k] = bk + 4;
+    u[k] = ((bk + 1 > n[k] - 1) ? bk : bk + 1) + 4;
+    w[k] = (dk > 0.0f) ? (double)dk : 0.0;
+  }
+  auto V = [&](int i, int j, int k) -> double {
+    return (double)grid_load(g, (uint32_t)i + g.py * (uint32_t)j + g.pz * (uint32_t)k);
+  };
+  const double v000 = V(b[0], b[1], b[2]), v100 = V(u[0], b[1], b[2]);
+  const double v010 = V(b[0], u[1], b[2]), v110 = V(u[0], u[1], b[2]);
+  const double v001 = V(b[0], b[1], u[2]), v101 = V(u[0], b[1], u[2]);
+  const double v011 = V(b[0], u[1], u[2]), v111 = V(u[0], u[1], u[2]);
+  const double x00 = v000 + (v100 - v000) * w[0];
+  const double x10 = v010 + (v110 - v010) * w[0];
+  const double x01 = v001 + (v101 - v001) * w[0];
+  const double x11 = v011 + (v111 - v011) * w[0];
+  const double xx0 = x00 + (x10 - x00) * w[1];
+  const double xx1 = x01 + (x11 - x01) * w[1];
+  return (float)(xx0 + (xx1 - xx0) * w[2]);
+}
 
 // NearestNeighborInterpolateImageFunction::EvaluateAtContinuousIndex: the voxel at the rounded
 // index (zero apron A = 4).
@@ -266,18 +304,48 @@ __device__ inline float sample_bspline(const GridView& g, float x, float y, floa
 template <int kInterp>
 __device__ inline float sample_interp(const GridView& g, float x, float y, float z)
 {
-  if constexpr (kInterp == XRH_INTERP_NN) return sample_nn(g, x, y, z);
+  if constexpr (kInterp == XRH_INTERP_LINEAR) return sample_linear(g, x, y, z);
+  else if constexpr (kInterp == XRH_INTERP_NN) return sample_nn(g, x, y, z);
   else if constexpr (kInterp == XRH_INTERP_SINC) return sample_sinc(g, x, y, z);
   else return sample_bspline(g, x, y, z);
 }
 
-// One lane = one ray: the reference's float set-up, n + 1 samples at positions advanced by
-// repeated float addition (:270-277), times the step, stored as store_pixel does
-// (RayCasterCPU::pre_compute semantics: ACCUM onto the buffer / background, REPLACE onto fill).
+// ---------------------------------------------------------------------------------------
+// anti-aliasing jitter
+
+// The SplitMix64 output function (Steele, Lea and Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014).
+__device__ inline uint64_t splitmix64(uint64_t state)
+{
+  uint64_t z = state;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Jitter of ray k of pixel `pixel` (the reference's range index: projection * rows * cols + row
+// * cols + col), component 0 (column) or 1 (row): uniform on [-0.5, 0.5) in steps of 2^-24, as
+// std::uniform_real_distribution<float>(-0.5, 0.5) draws it (xregRayCastLineIntCPU.cpp:162-174).
+// The reference seeds a Mersenne Twister from std::random_device, so its jitter is never
+// reproducible; here it is a pure function of (seed, pixel, ray, component): counter-based
+// SplitMix64, the same on the device and in any process.
+__device__ inline float aa_jitter(uint64_t seed, uint64_t pixel, int k, int comp)
+{
+  const uint64_t counter = (pixel << 17) | ((uint64_t)k << 1) | (uint64_t)comp;
+  const uint64_t h = splitmix64(seed + 0x9E3779B97F4A7C15ull * (counter + 1));
+  return (float)(uint32_t)(h >> 40) * 0x1.0p-24f - 0.5f;
+}
+
+// One lane = one pixel: `rays` rays (1 without anti-aliasing), each the reference's float set-up
+// through the (jittered) detector index, n + 1 samples at positions advanced by repeated float
+// addition (:270-277), times the step; the rays' values are averaged as the reference does
+// (aa_sum += sum * (1 / rays), :281-282) and stored as store_pixel does (RayCasterCPU::
+// pre_compute semantics: ACCUM onto the buffer / background, REPLACE onto fill, :285).
 template <int kOp, int kInterp>
 __global__ __launch_bounds__(kThreads) void drr_interp_kernel(
     const DrrItem* __restrict__ items, const uint32_t* __restrict__ band_start, int64_t num_items,
-    int bands, float* __restrict__ out, int accumulate, float fill, InterpGrid grid)
+    int bands, float* __restrict__ out, int accumulate, float fill, InterpGrid grid, int aa_rays,
+    uint64_t aa_seed)
 {
   const TileRef tr = decode_tile(band_start, num_items, bands, items);
   if (!tr.valid) return;
@@ -285,42 +353,66 @@ __global__ __launch_bounds__(kThreads) void drr_interp_kernel(
   int col, row;
   lane_pixel(tr.tx, tr.ty, 0, col, row);
   if (col >= it.cam.cols || row >= it.cam.rows) return;
-  float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
-  const Ray ray = setup_ray_f32(it, col, row);
-  if (ray.n >= 0) {
-    GridView g;
-    g.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(grid.data), (short)0, (int)grid.bytes,
-                                               0x00020000);
-    g.py = grid.py;
-    g.pz = grid.pz;
-    float px = ray.sx, py = ray.sy, pz = ray.sz;
-    for (int i = 0; i <= ray.n; ++i) {
-      acc = line_op<kOp>(acc, sample_interp<kInterp>(g, px, py, pz));
-      px = px + ray.dx;
-      py = py + ray.dy;
-      pz = pz + ray.dz;
-    }
-    acc = acc * it.step;
-  }
+  GridView g;
+  g.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(grid.data), (short)0, (int)grid.bytes, 0x00020000);
+  g.py = grid.py;
+  g.pz = grid.pz;
+  g.X = grid.X;
+  g.Y = grid.Y;
+  g.Z = grid.Z;
   const int64_t o = it.out_off + (int64_t)row * it.cam.cols + col;
+  const int rays = aa_rays > 0 ? aa_rays : 1;
+  const float inv = 1.0f / (float)rays;
+  float aa_sum = 0.0f;
+  for (int k = 0; k < rays; ++k) {
+    float c = (float)col, r = (float)row;
+    if (aa_rays > 0) {
+      c = c + aa_jitter(aa_seed, (uint64_t)o, k, 0);
+      r = r + aa_jitter(aa_seed, (uint64_t)o, k, 1);
+    }
+    float acc = (kOp == XRH_LINE_INT_SUM) ? 0.0f : -3.402823466e+38f;
+    const Ray ray = setup_ray_f32(it, c, r);
+    if (ray.n >= 0) {
+      float px = ray.sx, py = ray.sy, pz = ray.sz;
+      for (int i = 0; i <= ray.n; ++i) {
+        acc = line_op<kOp>(acc, sample_interp<kInterp>(g, px, py, pz));
+        px = px + ray.dx;
+        py = py + ray.dy;
+        pz = pz + ray.dz;
+      }
+      acc = acc * it.step;
+    }
+    aa_sum = aa_sum + acc * inv;
+  }
   const float prev = accumulate ? out[o] : fill;
-  const float aa = 0.0f + acc * 1.0f;
-  if constexpr (kOp == XRH_LINE_INT_SUM) out[o] = prev + aa; else out[o] = (prev < aa) ? aa : prev;
+  if constexpr (kOp == XRH_LINE_INT_SUM) out[o] = prev + aa_sum; else out[o] = (prev < aa_sum) ? aa_sum : prev;
+}
+
+template <int kOp, int kInterp>
+void launch_interp_kernel(const DrrLaunch& L, const InterpGrid& g, dim3 grid, hipStream_t s, int accumulate,
+                          int aa_rays, uint64_t aa_seed)
+{
+  hipLaunchKernelGGL((drr_interp_kernel<kOp, kInterp>), grid, dim3(kThreads), 0, s, L.items_dev, L.band_start_dev,
+                     L.num_items, L.bands, L.out, accumulate, L.fill, g, aa_rays, aa_seed);
 }
 
 template <int kOp>
 void launch_interp_op(const DrrLaunch& L, const InterpGrid& g, int32_t interp, dim3 grid, hipStream_t s,
-                      int accumulate)
+                      int accumulate, int aa_rays, uint64_t aa_seed)
 {
-  if (interp == XRH_INTERP_NN) {
-    hipLaunchKernelGGL((drr_interp_kernel<kOp, XRH_INTERP_NN>), grid, dim3(kThreads), 0, s, L.items_dev,
-                       L.band_start_dev, L.num_items, L.bands, L.out, accumulate, L.fill, g);
-  } else if (interp == XRH_INTERP_SINC) {
-    hipLaunchKernelGGL((drr_interp_kernel<kOp, XRH_INTERP_SINC>), grid, dim3(kThreads), 0, s, L.items_dev,
-                       L.band_start_dev, L.num_items, L.bands, L.out, accumulate, L.fill, g);
-  } else {
-    hipLaunchKernelGGL((drr_interp_kernel<kOp, XRH_INTERP_BSPLINE>), grid, dim3(kThreads), 0, s, L.items_dev,
-                       L.band_start_dev, L.num_items, L.bands, L.out, accumulate, L.fill, g);
+  switch (interp) {
+    case XRH_INTERP_LINEAR:
+      launch_interp_kernel<kOp, XRH_INTERP_LINEAR>(L, g, grid, s, accumulate, aa_rays, aa_seed);
+      break;
+    case XRH_INTERP_NN:
+      launch_interp_kernel<kOp, XRH_INTERP_NN>(L, g, grid, s, accumulate, aa_rays, aa_seed);
+      break;
+    case XRH_INTERP_SINC:
+      launch_interp_kernel<kOp, XRH_INTERP_SINC>(L, g, grid, s, accumulate, aa_rays, aa_seed);
+      break;
+    default:
+      launch_interp_kernel<kOp, XRH_INTERP_BSPLINE>(L, g, grid, s, accumulate, aa_rays, aa_seed);
+      break;
   }
 }
 
@@ -335,8 +427,7 @@ std::mutex& grid_mutex()
 const InterpGrid& interp_prepare(xrh_volume* v, int32_t interp)
 {
   XRH_REQUIRE(v, "null volume");
-  XRH_REQUIRE(interp == XRH_INTERP_NN || interp == XRH_INTERP_SINC || interp == XRH_INTERP_BSPLINE,
-              "bad interpolation method");
+  XRH_REQUIRE(interp >= XRH_INTERP_LINEAR && interp <= XRH_INTERP_BSPLINE, "bad interpolation method");
   const int k = (interp == XRH_INTERP_BSPLINE) ? kGridBSpline : kGridVoxels;
   std::lock_guard<std::mutex> lk(grid_mutex());
   if (v->grid_mem[k]) return v->grid[k];
@@ -397,21 +488,25 @@ const InterpGrid& interp_prepare(xrh_volume* v, int32_t interp)
   g.py = (uint32_t)GX;
   g.pz = (uint32_t)(GX * GY);
   g.apron = A;
+  g.X = (int32_t)X;
+  g.Y = (int32_t)Y;
+  g.Z = (int32_t)Z;
   return g;
 }
 
-void launch_drr_interp(const DrrLaunch& L, const InterpGrid& g, int32_t interp, hipStream_t s)
+void launch_drr_interp(const DrrLaunch& L, const InterpGrid& g, int32_t interp, int32_t aa_rays, uint64_t aa_seed,
+                       hipStream_t s)
 {
   if (L.num_items <= 0 || L.max_band_tiles <= 0) return;
-  XRH_REQUIRE(interp == XRH_INTERP_NN || interp == XRH_INTERP_SINC || interp == XRH_INTERP_BSPLINE,
-              "bad interpolation method");
+  XRH_REQUIRE(interp >= XRH_INTERP_LINEAR && interp <= XRH_INTERP_BSPLINE, "bad interpolation method");
+  XRH_REQUIRE(aa_rays >= 0 && aa_rays <= XRH_MAX_AA_RAYS, "anti-aliasing rays out of range");
   XRH_REQUIRE(g.data && g.apron == ((interp == XRH_INTERP_BSPLINE) ? 2 : 4), "interpolation grid not prepared");
   const int64_t nblocks = L.max_band_tiles * L.bands;
   XRH_REQUIRE(nblocks < (int64_t)INT32_MAX, "too many DRR tiles in one launch");
   const dim3 grid((unsigned)nblocks);
   const int accumulate = (L.store == XRH_STORE_ACCUM) ? 1 : 0;
-  if (L.kernel == XRH_LINE_INT_SUM) launch_interp_op<XRH_LINE_INT_SUM>(L, g, interp, grid, s, accumulate);
-  else launch_interp_op<XRH_LINE_INT_MAX>(L, g, interp, grid, s, accumulate);
+  if (L.kernel == XRH_LINE_INT_SUM) launch_interp_op<XRH_LINE_INT_SUM>(L, g, interp, grid, s, accumulate, aa_rays, aa_seed);
+  else launch_interp_op<XRH_LINE_INT_MAX>(L, g, interp, grid, s, accumulate, aa_rays, aa_seed);
   XRH_HIP_CHECK(hipGetLastError());
 }
 

@@ -123,13 +123,14 @@ struct Ray {
 };
 
 // Per-ray set-up in float, reference order (xregRayCastLineIntCPU.cpp:204-268).
-__device__ inline Ray setup_ray_f32(const DrrItem& it, int col, int row)
+// (col, row) is the detector index (a pixel's integer index, or a jittered one).
+__device__ inline Ray setup_ray_f32(const DrrItem& it, float col, float row)
 {
 #pragma clang fp contract(off)
   Ray ray;
   ray.n = -1;
   float det[3];
-  ind_to_det(it.cam, (float)col, (float)row, det);
+  ind_to_det(it.cam, col, row, det);
   float ph[3], dp[3];
   affine_pt(it.M, it.cam.pinhole, ph);
   affine_pt(it.M, det, dp);

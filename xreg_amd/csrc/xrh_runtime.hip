@@ -549,7 +549,8 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
                             int64_t num_projs, float step, int32_t kernel, int32_t store,
                             const float* bg_dev, float default_bg, int32_t geom, float* drr_dev,
                             uint64_t* samples_out, const FixedStaging* fixed = nullptr,
-                            int32_t interp = XRH_INTERP_LINEAR, const InterpGrid* igrid = nullptr)
+                            const InterpGrid* igrid = nullptr, int32_t interp = XRH_INTERP_LINEAR,
+                            int32_t aa_rays = 0, uint64_t aa_seed = 0)
 {
   XRH_REQUIRE(ctx && vol && cams && xforms && cam_for_proj, "null argument");
   XRH_REQUIRE(num_cams >= 1 && num_projs >= 1, "need at least one camera and one projection");
@@ -657,9 +658,8 @@ static void line_int_launch(xrh_ctx* ctx, const xrh_volume* vol, const xrh_camer
   L.reuse = var.reuse;
   L.wave_shape = var.shape;
   launch_drr_setup(L, ctx->stream);
-  if (interp != XRH_INTERP_LINEAR && !samples_out) {
-    XRH_REQUIRE(igrid, "interpolation grid not prepared");
-    launch_drr_interp(L, *igrid, interp, ctx->stream);
+  if (igrid && !samples_out) {
+    launch_drr_interp(L, *igrid, interp, aa_rays, aa_seed, ctx->stream);
   } else {
     launch_drr(L, ctx->stream);
   }
@@ -941,17 +941,18 @@ xrh_status xrh_line_int_interp(xrh_ctx* ctx, xrh_volume* vol, const xrh_camera* 
                                int32_t num_cams, const float* xforms_cam_to_phys,
                                const int64_t* cam_for_proj, int64_t num_projs, float step,
                                int32_t kernel, int32_t store, const float* bg_dev, float default_bg,
-                               int32_t interp, float* drr_dev)
+                               int32_t interp, int32_t aa_rays, uint64_t aa_seed, float* drr_dev)
 {
   return guarded([&] {
-    XRH_REQUIRE(interp == XRH_INTERP_LINEAR || interp == XRH_INTERP_NN || interp == XRH_INTERP_SINC ||
-                    interp == XRH_INTERP_BSPLINE,
-                "bad interpolation method");
+    XRH_REQUIRE(interp >= XRH_INTERP_LINEAR && interp <= XRH_INTERP_BSPLINE, "bad interpolation method");
+    XRH_REQUIRE(aa_rays >= 0 && aa_rays <= XRH_MAX_AA_RAYS, "anti-aliasing rays out of range");
     XRH_REQUIRE(ctx && vol, "null argument");
     XRH_REQUIRE(vol->ctx == ctx, "volume belongs to another context");
-    const InterpGrid* g = (interp == XRH_INTERP_LINEAR) ? nullptr : &interp_prepare(vol, interp);
+    // linear without anti-aliasing is the trilinear kernels' cast
+    const bool plain = (interp == XRH_INTERP_LINEAR && aa_rays == 0);
+    const InterpGrid* g = plain ? nullptr : &interp_prepare(vol, interp);
     line_int_launch(ctx, vol, cams, num_cams, xforms_cam_to_phys, cam_for_proj, num_projs, step, kernel, store,
-                    bg_dev, default_bg, XRH_GEOM_FP32, drr_dev, nullptr, nullptr, interp, g);
+                    bg_dev, default_bg, XRH_GEOM_FP32, drr_dev, nullptr, nullptr, g, interp, aa_rays, aa_seed);
   });
 }
 
