@@ -132,6 +132,70 @@ def test_c4_full_half_storage_bound(ctx, c4full, storage):
     assert err.max() < bound
 
 
+def test_c4_768_precision_against_fp64_oracle(ctx, oracle):
+    """C4 at its own 768^2 detector and 0.25 mm steps: 6 poses of the C4 distribution cast from
+    FP32, FP16 and BF16 storage (FP64 ray set-up, FP32 accumulation) against the FP64 oracle
+    (FP64 volume, geometry and accumulation), and the NCC of each against the FP64 NCC of a
+    fixed image made from the ground-truth pose."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    print("case: c4 768^2 precision vs FP64 oracle, 3 storages x 6 poses", flush=True)
+    w = synth.config("c4", n_poses=6)
+    cam = w.cams[0]
+    poses = np.asarray(w.poses, np.float64)
+    ref, _ = oracle.drr_f64(w.vol, w.idx_to_phys, [cam], poses, step=w.step)
+    gt, _ = oracle.drr_f64(w.vol, w.idx_to_phys, [cam], w.gt[None], step=w.step)
+    fixed = synth.noisy_fixed(gt[0].astype(np.float32))
+    ncc_ref = oracle.ncc_f64(fixed, ref)
+    scale = float(ref.max())
+    bounds = {0: (3e-5, 1e-5), 1: (2.0 ** -11, 1e-4), 2: (2.0 ** -8, 1e-3)}
+    fx = X.Fixed(ctx, fixed, X.XRH_SIM_NCC)
+    n = len(poses)
+    for storage in (0, 1, 2):
+        v = X.Volume(ctx, w.vol, w.idx_to_phys, storage=storage)
+        buf = ctx.alloc(4 * n * cam.num_det_rows * cam.num_det_cols)
+        X.line_int(ctx, v, [cam], poses_f32(poses), np.zeros(n, np.int64), w.step, buf, geom=1)
+        got = buf.download(np.float32).reshape(ref.shape)
+        ncc = fx.compute(buf, n)
+        buf.free()
+        v.close()
+        derr = np.abs(got.astype(np.float64) - ref) / scale
+        nerr = np.abs(ncc.astype(np.float64) - ncc_ref)
+        print(f"storage {('fp32', 'fp16', 'bf16')[storage]}: DRR |err|/max max {derr.max():.3e} "
+              f"median {np.median(derr):.3e}; NCC |err| max {nerr.max():.3e}", flush=True)
+        assert derr.max() < bounds[storage][0]
+        assert nerr.max() < bounds[storage][1]
+    fx.close()
+
+
+def test_c4_200_poses_half_storage_bound(ctx):
+    """All 200 poses of C4 at 768^2: FP16 and BF16 storage against FP32 storage on the same
+    device geometry, within the voxel rounding (2^-11 / 2^-8) of the image maximum, and the FP32
+    DRRs finite and non-negative (the phantom is non-negative)."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    print("case: c4 768^2, 200 poses, fp16 / bf16 vs fp32 storage", flush=True)
+    w = synth.config("c4")
+    assert len(w.poses) == 200
+    vols = [X.Volume(ctx, w.vol, w.idx_to_phys, storage=s) for s in (0, 1, 2)]
+    worst = {1: 0.0, 2: 0.0}
+    for c in range(0, 200, 50):
+        chunk = w.poses[c:c + 50]
+        a = _drr(ctx, w, vols[0], chunk, geom=1)
+        assert np.all(np.isfinite(a)) and a.min() >= 0.0
+        scale = float(a.max())
+        for s in (1, 2):
+            b = _drr(ctx, w, vols[s], chunk, geom=1)
+            worst[s] = max(worst[s], float((np.abs(b.astype(np.float64) - a) / scale).max()))
+    for v in vols:
+        v.close()
+    print(f"max |err| / max: fp16 {worst[1]:.3e}, bf16 {worst[2]:.3e}", flush=True)
+    assert worst[1] < 2.0 ** -11
+    assert worst[2] < 2.0 ** -8
+
+
 def test_c2_full_cell_reuse_is_bit_identical(ctx, c2full, monkeypatch):
     from xreg_amd import _xrh as X
 
