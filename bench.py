@@ -97,6 +97,9 @@ def parse():
     ap.add_argument("--c4-steps", type=int, default=2,
                     help="reduced-precision storage leg (512x512x400 @0.5 mm, 768^2): timed generations per "
                          "storage type (0 skips)")
+    ap.add_argument("--interp-poses", type=int, default=0,
+                    help="interpolator leg: poses of the 128x128x96 @2 mm volume at 192^2 cast with each "
+                         "interpolator and with 4-ray anti-aliasing (0 skips)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run plain device copies of this many MiB (counter calibration under --pmc)")
     return ap.parse_args()
@@ -419,6 +422,41 @@ def batched_leg(ctx, batch, lam, gens, warmup, steps, autotune):
             "S_eval": float(np.mean([c[1] for c in counts[k0:]]))}
 
 
+def interp_leg(ctx, n_poses, reps=3):
+    """The CPU caster's other sampling options (xrh_line_int_interp) on the 128x128x96 @2 mm
+    phantom, 192^2 detector, 1 mm steps: n_poses poses per call, wall clock around synchronised
+    calls (minimum of `reps` after one untimed call that also builds the volume's grids), against
+    the trilinear cast of the same call."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    w = synth.config("c1", n_poses=n_poses)
+    cam = w.cams[0]
+    poses = w.poses.astype(np.float32).reshape(-1, 16)
+    c4p = np.zeros(n_poses, np.int64)
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    buf = ctx.alloc(4 * n_poses * cam.num_det_rows * cam.num_det_cols)
+    cases = (("linear", X.XRH_INTERP_LINEAR, 0), ("nn", X.XRH_INTERP_NN, 0), ("bspline", X.XRH_INTERP_BSPLINE, 0),
+             ("sinc", X.XRH_INTERP_SINC, 0), ("linear_aa4", X.XRH_INTERP_LINEAR, 4), ("nn_aa4", X.XRH_INTERP_NN, 4))
+    out = {"volume": "128x128x96@2mm", "detector": "192x192@1.552mm", "step_mm": w.step, "poses": n_poses}
+    for name, interp, aa in cases:
+        best = None
+        for r in range(reps + 1):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            X.line_int_interp(ctx, vol, [cam], poses, c4p, w.step, buf, interp, aa_rays=aa, aa_seed=5)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            if r > 0:
+                best = dt if best is None else min(best, dt)
+        out[name] = {"ms_per_call": round(best * 1e3, 3), "poses_per_s": round(n_poses / best, 1)}
+        print(f"interp {name}: {out[name]}", file=sys.stderr, flush=True)
+    out["grid_bytes"] = vol.nbytes
+    buf.free()
+    vol.close()
+    return out
+
+
 def c4_leg(ctx, lam, steps, geom, smooth):
     """Reduced-precision volume storage: the seeded 512x512x400 phantom at 0.5 mm stored as FP32,
     FP16 and BF16 (identical contents), 768^2 detector at 0.388 mm, 0.25 mm steps, patch
@@ -574,6 +612,8 @@ def main():
     if args.c4_steps > 0 and args.config == "c2":
         c4 = c4_leg(ctx, lam, args.c4_steps, geom, args.smooth)
 
+    interp = interp_leg(ctx, args.interp_poses) if args.interp_poses > 0 else None
+
     served_mix = None
     if args.serve_mix_jobs > 0:
         served_mix = serve_mix_leg(ctx, args.serve_mix_jobs, args.serve_mix_rate, geom, args.smooth,
@@ -645,6 +685,7 @@ def main():
         "served_mix": served_mix,
         "c3_registration": c3,
         "c4": c4,
+        "interp": interp,
         "cpu_baseline": cpu,
         "sim_checksum": float(np.sum(sims)),
     }

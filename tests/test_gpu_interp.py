@@ -182,3 +182,33 @@ def test_antialiased_drr_matches_oracle(ctx, oracle, c1i, interp, kernel):
         assert np.array_equal(got, ref)
     else:
         assert err.max() <= DRR_REL_TOL
+
+
+def test_interp_rejects_bad_arguments_before_any_work(ctx, c1i):
+    """An unknown interpolator, an anti-aliasing factor outside [0, XRH_MAX_AA_RAYS] and a volume of
+    another context fail with XRH_ERR_INVALID and a message; nothing is cast and no grid is built."""
+    from xreg_amd import _xrh as X
+
+    w, cam = c1i
+    print("case: interp argument checks", flush=True)
+    poses = poses_f32(w.poses[:1])
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    b0 = vol.nbytes
+    npx = cam.num_det_rows * cam.num_det_cols
+    buf = ctx.alloc(4 * npx)
+    buf.upload(np.full(npx, 7.0, np.float32))
+    for kw, what in (({"interp": 4}, "interpolation"), ({"interp": -1}, "interpolation"),
+                     ({"interp": 1, "aa_rays": -1}, "anti-aliasing"),
+                     ({"interp": 1, "aa_rays": X.XRH_MAX_AA_RAYS + 1}, "anti-aliasing")):
+        with pytest.raises(X.XrhError, match=what):
+            X.line_int_interp(ctx, vol, [cam], poses, np.zeros(1, np.int64), w.step, buf, **kw)
+    other = X.Context(0)
+    try:
+        with pytest.raises(X.XrhError, match="another context"):
+            X.line_int_interp(other, vol, [cam], poses, np.zeros(1, np.int64), w.step, buf, 1)
+    finally:
+        other.close()
+    assert vol.nbytes == b0
+    assert np.all(buf.download(np.float32) == 7.0)
+    buf.free()
+    vol.close()
