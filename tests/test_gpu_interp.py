@@ -212,3 +212,34 @@ def test_interp_rejects_bad_arguments_before_any_work(ctx, c1i):
     assert np.all(buf.download(np.float32) == 7.0)
     buf.free()
     vol.close()
+
+
+@pytest.mark.parametrize("shape", [(5, 3, 2), (9, 7, 4)])
+def test_interp_tiny_volumes_and_ragged_detectors(ctx, oracle, shape):
+    """Volumes with sides of 2-9 voxels (the B-spline support passes both mirror boundaries of a
+    side at once; sides under 4 take the clamp) seen by ragged detectors (3 x 17, 1 x 1):
+    nearest neighbour bit for bit, B-spline and sinc within 1e-4, with and without
+    anti-aliasing."""
+    from xreg_amd import _xrh as X
+    from xreg_amd import geometry as G
+    from xreg_amd import synth
+
+    print(f"case: interp tiny volume {shape}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    vol = rng.uniform(0.5, 1.5, size=shape).astype(np.float32)
+    sp = 2.0
+    i2p = G.idx_to_phys(sp)
+    center = G.vol_center_phys(shape[::-1], sp)
+    gt = G.gt_pose(center, 510.0)
+    poses = poses_f32(np.concatenate([gt[None], synth.random_poses(gt, center, 2, 5.0, 2.0, seed=3)]))
+    v = X.Volume(ctx, vol, i2p)
+    for cam in (G.camera(1020.0, 3, 17, 2.0, 1.0), G.camera(1020.0, 1, 1, 1.0, 1.0)):
+        for interp in (1, 3, 2):
+            for aa in (0, 2):
+                got = _cast(ctx, v, cam, poses, 0.5, interp, aa_rays=aa, aa_seed=11)
+                ref, _ = oracle.drr_f32(vol, i2p, [cam], poses, step=0.5, interp=interp, aa_rays=aa, aa_seed=11)
+                if interp == 1:
+                    assert np.array_equal(got, ref), (cam.num_det_rows, interp, aa)
+                else:
+                    assert drr_rel_err(got, ref).max() <= DRR_REL_TOL, (cam.num_det_rows, interp, aa)
+    v.close()
