@@ -225,7 +225,24 @@ __device__ inline void tile_gradients_k(const float* __restrict__ img, int rows,
   gy = syp - sym;
 }
 
-// ---- kernels over SimItem tables: blockIdx.x = tile, blockIdx.y = item ----
+// ---- kernels over SimItem tables: blockIdx.x -> tile (xcd_tile), blockIdx.y = item ----
+
+// The tile of this workgroup among an image's T tiles (row-major), or -1 for a padding slot.
+// The dispatcher deals workgroups to the 8 XCDs in turn, so with gridDim.x a multiple of 8 the
+// workgroups with blockIdx.x = 8 j + k all land on XCD k; giving class k the k-th contiguous
+// run of ceil(T / 8) tiles keeps neighbouring tiles, and the window rows they share, in one
+// XCD's L2.
+__device__ inline int xcd_tile(int T)
+{
+  const int per = (T + 7) >> 3;
+  const int x = (int)blockIdx.x, k = x & 7, j = x >> 3;
+  if (j >= per) return -1;
+  const int t = k * per + j;
+  return t < T ? t : -1;
+}
+
+// gridDim.x for up to `tiles` tiles per image under xcd_tile
+inline unsigned xcd_grid_x(int tiles) { return (unsigned)(8 * ((tiles + 7) / 8)); }
 
 // NCC sums over raw moving images: per tile (sum m, sum m^2, sum m f~) [masked]
 __global__ __launch_bounds__(kThreads) void ncc_sums_kernel(const SimItem* __restrict__ items,
@@ -234,8 +251,9 @@ __global__ __launch_bounds__(kThreads) void ncc_sums_kernel(const SimItem* __res
   const SimItem it = items[blockIdx.y];
   const FixedDev& fx = *it.fixed;
   const int TX = (fx.cols + kT - 1) / kT, TY = (fx.rows + kT - 1) / kT;
-  if ((int)blockIdx.x >= TX * TY) return;
-  const int tx0 = (blockIdx.x % TX) * kT, ty0 = (blockIdx.x / TX) * kT;
+  const int tile = xcd_tile(TX * TY);
+  if (tile < 0) return;
+  const int tx0 = (tile % TX) * kT, ty0 = (tile / TX) * kT;
   const int x = tx0 + (threadIdx.x % kT), y = ty0 + (threadIdx.x / kT);
   double v[3] = {0.0, 0.0, 0.0};
   if (x < fx.cols && y < fx.rows) {
@@ -247,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void ncc_sums_kernel(const SimItem* __res
       v[2] = m * (double)fx.zf[o];
     }
   }
-  block_reduce_store<3>(v, partials + (it.out_index * ptiles + blockIdx.x) * 4);
+  block_reduce_store<3>(v, partials + (it.out_index * ptiles + tile) * 4);
 }
 
 // Grad-NCC sums: gradients in LDS, then sums for x and y directions
@@ -258,8 +276,9 @@ __global__ __launch_bounds__(kThreads) void grad_ncc_sums_kernel(const SimItem* 
   const SimItem it = items[blockIdx.y];
   const FixedDev& fx = *it.fixed;
   const int TX = (fx.cols + kT - 1) / kT, TY = (fx.rows + kT - 1) / kT;
-  if ((int)blockIdx.x >= TX * TY) return;
-  const int x0 = (blockIdx.x % TX) * kT, y0 = (blockIdx.x / TX) * kT;
+  const int tile = xcd_tile(TX * TY);
+  if (tile < 0) return;
+  const int x0 = (tile % TX) * kT, y0 = (tile / TX) * kT;
   float gx, gy;
   bool inside;
   tile_gradients_k<kRb>(it.mov, fx.rows, fx.cols, x0, y0, gx, gy, inside);
@@ -276,7 +295,7 @@ __global__ __launch_bounds__(kThreads) void grad_ncc_sums_kernel(const SimItem* 
       v[5] = b * (double)fx.zfy[o];
     }
   }
-  block_reduce_store<6>(v, partials + (it.out_index * ptiles + blockIdx.x) * 8);
+  block_reduce_store<6>(v, partials + (it.out_index * ptiles + tile) * 8);
 }
 
 // Gradient images of moving images (for the patch metric), written to it.gx / it.gy.
@@ -286,8 +305,9 @@ __global__ __launch_bounds__(kThreads) void grad_images_kernel(const SimItem* __
   const SimItem it = items[blockIdx.y];
   const FixedDev& fx = *it.fixed;
   const int TX = (fx.cols + kT - 1) / kT, TY = (fx.rows + kT - 1) / kT;
-  if ((int)blockIdx.x >= TX * TY) return;
-  const int x0 = (blockIdx.x % TX) * kT, y0 = (blockIdx.x / TX) * kT;
+  const int tile = xcd_tile(TX * TY);
+  if (tile < 0) return;
+  const int x0 = (tile % TX) * kT, y0 = (tile / TX) * kT;
   float gx, gy;
   bool inside;
   tile_gradients_k<kRb>(it.mov, fx.rows, fx.cols, x0, y0, gx, gy, inside);
@@ -324,8 +344,9 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
   const int s = fx.patch_stride, r = fx.patch_radius, d = 2 * r + 1;
   const int ppr = fx.ppr, ppc = fx.ppc;
   const int TX = (ppr + kT - 1) / kT, TY = (ppc + kT - 1) / kT;
-  if ((int)blockIdx.x >= TX * TY) return;
-  const int px0 = (blockIdx.x % TX) * kT, py0 = (blockIdx.x / TX) * kT;  // first patch index
+  const int tile = xcd_tile(TX * TY);
+  if (tile < 0) return;
+  const int px0 = (tile % TX) * kT, py0 = (tile / TX) * kT;  // first patch index
   const int X0 = px0 * s, Y0 = py0 * s;                                  // first image pixel
   const int W = patch_lds_edge(s, r);
   const bool masked = fx.pw != nullptr;
@@ -400,7 +421,7 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
       }
     }
   }
-  block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
+  block_reduce_store<2>(acc, partials + (it.out_index * ptiles + tile) * 2);
 }
 
 // patch_kernel for the common case -- no mask, stride 1, a compile-time patch diameter kD.
@@ -433,8 +454,9 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
   const ConstFixed fx = (ConstFixed)it.fixed;  // uniform: scalar loads
   const int rows = fx->rows, cols = fx->cols, ppr = fx->ppr, ppc = fx->ppc;
   const int TX = (ppr + kT - 1) / kT, TY = (ppc + kT - 1) / kT;
-  if ((int)blockIdx.x >= TX * TY) return;
-  const int X0 = (blockIdx.x % TX) * kT, Y0 = (blockIdx.x / TX) * kT;  // first patch = first pixel
+  const int tile = xcd_tile(TX * TY);
+  if (tile < 0) return;
+  const int X0 = (tile % TX) * kT, Y0 = (tile / TX) * kT;  // first patch = first pixel
   float* lm = (float*)patch_lds;
   float* lf = lm + kW * kW;
   double* h1 = (double*)(lf + kW * kW);  // 2 kW^2 floats: a multiple of 8 bytes
@@ -557,7 +579,7 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
       }
     }
   }
-  block_reduce_store<2>(acc, partials + (it.out_index * ptiles + blockIdx.x) * 2);
+  block_reduce_store<2>(acc, partials + (it.out_index * ptiles + tile) * 2);
 }
 
 // fixed-image per-patch statistics (reference float order, ComputePatchMeanStdDev)
@@ -833,7 +855,7 @@ void launch_sim(int key, const SimItem* items_dev, int64_t n_items, int max_img_
   XRH_REQUIRE(n_items < 65536, "too many moving images in one similarity launch");
   const bool patch = (metric == XRH_SIM_PATCH_NCC || metric == XRH_SIM_PATCH_GRAD_NCC);
   const int ptiles = patch ? max_patch_tiles : max_img_tiles;
-  const dim3 grid((unsigned)ptiles, (unsigned)n_items);
+  const dim3 grid(xcd_grid_x(ptiles), (unsigned)n_items);
   const int rec = sim_record_stride(metric);
   switch (metric) {
     case XRH_SIM_NCC:
@@ -850,7 +872,7 @@ void launch_sim(int key, const SimItem* items_dev, int64_t n_items, int max_img_
       }
       break;
     case XRH_SIM_PATCH_GRAD_NCC:
-      XRH_BLUR_DISPATCH(rb, grad_images_kernel, dim3((unsigned)max_img_tiles, (unsigned)n_items), dim3(kThreads), 0,
+      XRH_BLUR_DISPATCH(rb, grad_images_kernel, dim3(xcd_grid_x(max_img_tiles), (unsigned)n_items), dim3(kThreads), 0,
                         s, items_dev);
       if (dfix) {
         launch_patch_d<true>(dfix, grid, patch_lds_bytes, s, items_dev, partials, ptiles);
