@@ -57,7 +57,25 @@ def sim_case():
     np.savez_compressed(OUT / "sim_c2_small.npz", **out)
 
 
+def interp_case():
+    """The CPU caster's other sampling options on the small C1 case: nearest-neighbour,
+    windowed-sinc and B-spline DRRs, and linear / nearest-neighbour DRRs with 2 anti-aliasing
+    rays per pixel (jitter seed 7)."""
+    w = synth.config("c1", n_poses=2, small=True)
+    poses = np.concatenate([w.gt[None], w.poses]).astype(np.float32).reshape(-1, 16)
+    out = {"vol": w.vol, "idx_to_phys": w.idx_to_phys, "step": np.float32(w.step), "poses": poses}
+    meta, focal, kinv, einv, ph = cam_arrays(w.cams[0])
+    out.update(cam_meta=meta, cam_focal=focal, cam_kinv=kinv, cam_einv=einv, cam_pinhole=ph)
+    for name, interp in (("nn", 1), ("sinc", 2), ("bspline", 3)):
+        out[f"drr_{name}"], _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, poses, step=w.step, interp=interp, threads=1)
+    for name, interp in (("linear", 0), ("nn", 1)):
+        out[f"drr_{name}_aa2"], _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, poses, step=w.step, interp=interp,
+                                             aa_rays=2, aa_seed=7, threads=1)
+    np.savez_compressed(OUT / "interp_c1_small.npz", **out)
+
+
 if __name__ == "__main__":
     drr_case()
     sim_case()
+    interp_case()
     print("wrote", sorted(p.name for p in OUT.glob("*.npz")))

@@ -35,6 +35,24 @@ def gsim():
     return np.load(GOLD / "sim_c2_small.npz")
 
 
+@pytest.fixture(scope="module")
+def ginterp():
+    return np.load(GOLD / "interp_c1_small.npz")
+
+
+INTERP_CASES = (("drr_nn", 1, 0), ("drr_sinc", 2, 0), ("drr_bspline", 3, 0), ("drr_linear_aa2", 0, 2),
+                ("drr_nn_aa2", 1, 2))
+
+
+def test_oracle_reproduces_golden_interp(oracle, ginterp):
+    z = ginterp
+    cam = _cam(z)
+    for name, interp, aa in INTERP_CASES:
+        got, _ = oracle.drr_f32(z["vol"], z["idx_to_phys"], [cam], z["poses"], step=float(z["step"]), interp=interp,
+                                aa_rays=aa, aa_seed=7)
+        assert np.array_equal(got, z[name]), name
+
+
 def test_oracle_reproduces_golden_drr(oracle, gdrr):
     z = gdrr
     cam = _cam(z)
@@ -93,3 +111,27 @@ def test_gpu_matches_golden_sims(ctx, gsim):
             assert np.abs(got - z[f"sim{metric}_k{k}_f32"]).max() <= SIM_ABS_TOL, (metric, k)
             assert np.abs(got - z[f"sim{metric}_k{k}_f64"]).max() <= SIM_ABS_TOL, (metric, k)
     buf.free()
+
+
+@pytest.mark.gpu
+def test_gpu_matches_golden_interp(ctx, ginterp):
+    """Nearest neighbour and the anti-aliased linear / nearest-neighbour casts bit for bit, sinc
+    and B-spline within the DRR tolerance."""
+    from xreg_amd import _xrh as X
+
+    z = ginterp
+    cam = _cam(z)
+    n = z["poses"].shape[0]
+    rows, cols = cam.num_det_rows, cam.num_det_cols
+    vol = X.Volume(ctx, z["vol"], z["idx_to_phys"])
+    buf = ctx.alloc(4 * n * rows * cols)
+    for name, interp, aa in INTERP_CASES:
+        X.line_int_interp(ctx, vol, [cam], z["poses"], np.zeros(n, np.int64), float(z["step"]), buf, interp,
+                          aa_rays=aa, aa_seed=7)
+        got = buf.download(np.float32).reshape(n, rows, cols)
+        if interp in (0, 1):
+            assert np.array_equal(got, z[name]), name
+        else:
+            assert drr_rel_err(got, z[name]).max() <= DRR_REL_TOL, name
+    buf.free()
+    vol.close()
