@@ -288,11 +288,34 @@ xrh_status xrh_fixed_destroy(xrh_fixed* fixed);
 /* The patch list of a patch metric, as ImgSimMetric2DPatchCommon keeps it and its SimAux
  * debug record stores it per compute call (xregImgSimMetric2DPatchCommon.h:49-61,
  * xregImgSimMetric2DPatchNCCCPU.cpp:290-298, 443-485): inclusive start / stop rows and columns
- * of each patch, rows of centres outer, and its weight (1, or the normalised unmasked fraction
- * with a mask). *num_patches receives the count; with capacity 0 nothing else is written,
- * otherwise capacity must hold it and any of the arrays may be NULL. Synchronous. */
+ * of each patch, rows of centres outer, and its weight (the weights last set with
+ * xrh_fixed_set_patch_weights; before that 1, or the normalised unmasked fraction with a
+ * mask). *num_patches receives the count; with capacity 0 nothing else is written, otherwise
+ * capacity must hold it and any of the arrays may be NULL. Synchronous. */
 xrh_status xrh_fixed_patches(const xrh_fixed* fixed, int64_t capacity, int32_t* start_row, int32_t* start_col,
                              int32_t* stop_row, int32_t* stop_col, float* weight, int64_t* num_patches);
+
+/* How a patch metric combines its per-patch scores s_p into an image score, after
+ * ImgSimMetric2DPatchNCCCPU::compute (xregImgSimMetric2DPatchNCCCPU.cpp:186-287) with the
+ * state of ImgSimMetric2DPatchCommon (xregImgSimMetric2DPatchCommon.h:141-172):
+ *   weights      num_patches floats, the PatchInfo weights (a weight image's values at the
+ *                patch centres, mask fractions, normalised or not: the caller's
+ *                compute_weights, xregImgSimMetric2DPatchCommon.cpp:306-400); NULL = the
+ *                weights the image was created with (unit, or the mask's);
+ *   patch_inds   the patches used, in order, repeats allowed (patch_inds_to_use_: a random
+ *                draw, set_patches_to_use); NULL = every patch (num_inds ignored);
+ *   weight_in_combine, compute_mean   weight_patch_sims_in_combine_ and
+ *                compute_mean_of_patch_sims_.
+ * Score = sum over the used indices of (weight_in_combine ? w_p : 1) s_p, patches with
+ * |w_p| <= 1e-6 skipped when weight_in_combine, divided by num_inds if compute_mean, else by
+ * the float sum of the used weights if weight_in_combine (which must not be zero), else by 1.
+ * Both gradient directions of Patch-Grad-NCC use the same patches and weights (the
+ * reference's default enforce_same_patches_in_both_x_and_y_). Index checks and the divisor
+ * check fail with nothing changed. Synchronises the context's stream first: work queued
+ * before the call, batched and graph-replayed launches included, uses the previous
+ * combine; every later launch that reads this fixed image uses the new one. */
+xrh_status xrh_fixed_set_patch_weights(xrh_fixed* fixed, const float* weights, const int64_t* patch_inds,
+                                       int64_t num_inds, int32_t weight_in_combine, int32_t compute_mean);
 
 /* sim_host_out: num_mov floats (synchronises); sim_dev_out: optional device destination
  * (stream-ordered). At least one must be non-NULL. The moving images are not modified

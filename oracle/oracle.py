@@ -282,6 +282,72 @@ def patch_grad_ncc_f64(fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0
     return out
 
 
+def patch_weights_f32(shape, radius=5, stride=1, wgt_img=None, mask=None, use_mask_for_weighting=True,
+                      normalize=True):
+    """ImgSimMetric2DPatchCommon::compute_weights: per-patch weights, rows of centres outer."""
+    rows, cols = shape
+    ppr = (cols - 1 - 2 * radius) // stride + 1
+    ppc = (rows - 1 - 2 * radius) // stride + 1
+    out = np.zeros(ppr * ppc, dtype=np.float32)
+    wi = None if wgt_img is None else np.ascontiguousarray(wgt_img, dtype=np.float32).reshape(shape)
+    mk, mp = _mask_ptr(mask, shape)
+    lib().orc_patch_weights_f32(None if wi is None else _f(wi), mp, rows, cols, radius, stride,
+                                int(bool(use_mask_for_weighting)), int(bool(normalize)), _f(out))
+    return out
+
+
+def _inds(inds):
+    if inds is None:
+        return None, None, 0
+    a = np.ascontiguousarray(inds, dtype=np.int64).ravel()
+    return a, _i64(a), a.size
+
+
+def patch_sim_weighted_f32(metric, fixed, mov, weights=None, inds=None, weight_in_combine=True, compute_mean=False,
+                           smooth_ksize=5, radius=5, stride=1, mask=None, threads=0):
+    """Patch-NCC (metric 2) / Patch-Grad-NCC (3) with the general combine of
+    ImgSimMetric2DPatchNCCCPU::compute: patch weights, the used patch indices, and the
+    weight-in-combine / mean flags."""
+    f = np.ascontiguousarray(fixed, dtype=np.float32)
+    m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float32)
+    mk, mp = _mask_ptr(mask, f.shape)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32).ravel()
+    ia, ip, n = _inds(inds)
+    wp = None if w is None else _f(w)
+    if metric == 2:
+        lib().orc_patch_ncc_weighted_f32(_f(f), _f(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius,
+                                         stride, wp, ip, C.c_int64(n), int(bool(weight_in_combine)),
+                                         int(bool(compute_mean)), _f(out), threads)
+    else:
+        lib().orc_patch_grad_ncc_weighted_f32(_f(f), _f(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]),
+                                              smooth_ksize, radius, stride, wp, ip, C.c_int64(n),
+                                              int(bool(weight_in_combine)), int(bool(compute_mean)), _f(out),
+                                              threads)
+    return out
+
+
+def patch_sim_weighted_f64(metric, fixed, mov, weights=None, inds=None, weight_in_combine=True, compute_mean=False,
+                           smooth_ksize=5, radius=5, stride=1, mask=None, threads=0):
+    f = np.ascontiguousarray(fixed, dtype=np.float64)
+    m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, *f.shape)
+    out = np.zeros(m.shape[0], dtype=np.float64)
+    mk, mp = _mask_ptr(mask, f.shape)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64).ravel()
+    ia, ip, n = _inds(inds)
+    wp = None if w is None else _d(w)
+    if metric == 2:
+        lib().orc_patch_ncc_weighted_f64(_d(f), _d(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius,
+                                         stride, wp, ip, C.c_int64(n), int(bool(weight_in_combine)),
+                                         int(bool(compute_mean)), _d(out), threads)
+    else:
+        lib().orc_patch_grad_ncc_weighted_f64(_d(f), _d(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]),
+                                              smooth_ksize, radius, stride, wp, ip, C.c_int64(n),
+                                              int(bool(weight_in_combine)), int(bool(compute_mean)), _d(out),
+                                              threads)
+    return out
+
+
 def sim_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0, mask=None):
     if metric == 0:
         return ncc_f32(fixed, mov, mask=mask)

@@ -1338,9 +1338,11 @@ typedef struct {
   int32_t ppr; /* patches per row */
   int32_t stride;
   float* fhat; /* num_patches * diam * diam */
-  float* vals;
+  float* vals;         /* per used index */
   const uint8_t* mask; /* optional fixed-image mask (non-zero = use the pixel) */
-  const float* wgt;    /* per-patch weights when masked */
+  const float* wgt;    /* per-patch weights, or NULL for unit weights */
+  const int64_t* inds; /* patch indices used, or NULL for 0 .. num_patches - 1 */
+  int32_t wic;         /* weight_patch_sims_in_combine_ */
 } patch_f32_ctx;
 
 static void patch_stats_f32(const float* img, int32_t cols, int32_t r0, int32_t c0, int32_t diam,
@@ -1390,12 +1392,13 @@ static void patch_mov_range(void* vctx, int64_t begin, int64_t end)
 {
   patch_f32_ctx* c = (patch_f32_ctx*)vctx;
   const int32_t d = c->diam;
-  for (int64_t p = begin; p < end; ++p) {
+  for (int64_t k = begin; k < end; ++k) {
+    const int64_t p = c->inds ? c->inds[k] : k; /* local -> global patch index (:204-206) */
     const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
     const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
-    const float w = c->mask ? c->wgt[p] : 1.0f;
-    if (c->mask && !(fabsf(w) > 1.0e-6f)) {
-      c->vals[p] = 0.0f; /* low-weight patch skipped (:212) */
+    const float w = c->wgt ? c->wgt[p] : 1.0f;
+    if (c->wic && !(fabsf(w) > 1.0e-6f)) {
+      c->vals[k] = 0.0f; /* low-weight patch skipped (:212) */
       continue;
     }
     float mean, sd;
@@ -1409,8 +1412,8 @@ static void patch_mov_range(void* vctx, int64_t begin, int64_t end)
         if (!c->mask || c->mask[o]) accum += ((c->mov[o] - mean) / sd) * fh[y * d + x];
       }
     }
-    /* weighted by the patch weight (weight_patch_sims_in_combine_, :252-253) */
-    c->vals[p] = w * (1.0f - accum);
+    /* weighted by the patch weight when weight_patch_sims_in_combine_ (:252-253) */
+    c->vals[k] = (c->wic ? w : 1.0f) * (1.0f - accum);
   }
 }
 
@@ -1443,10 +1446,69 @@ void orc_patch_ncc_masked_f32(const float* fixed, const float* mov, const uint8_
                               int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                               float* sim_out, int32_t threads)
 {
+  float* wgt = NULL;
+  if (mask) {
+    const int32_t diam = 2 * patch_radius + 1;
+    const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+    const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+    const int64_t np = (int64_t)ppr * ppc;
+    wgt = (float*)malloc(sizeof(float) * (size_t)np);
+    patch_mask_weights_f32(mask, cols, ppr, np, patch_stride, diam, wgt);
+  }
+  orc_patch_ncc_weighted_f32(fixed, mov, mask, rows, cols, num_mov, patch_radius, patch_stride, wgt, NULL, 0, 1, 0,
+                             sim_out, threads);
+  free(wgt);
+}
+
+void orc_patch_weights_f32(const float* wgt_img, const uint8_t* mask, int32_t rows, int32_t cols,
+                           int32_t patch_radius, int32_t patch_stride, int32_t use_mask_for_weighting,
+                           int32_t normalize, float* w_out)
+{
   const int32_t diam = 2 * patch_radius + 1;
   const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
   const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
   const int64_t np = (int64_t)ppr * ppc;
+  const int use_mask_wgts = use_mask_for_weighting && mask;
+  (void)rows;
+  if (!wgt_img && !use_mask_wgts) {
+    for (int64_t p = 0; p < np; ++p) w_out[p] = 1.0f; /* otherwise keep weight at 1 (:384) */
+    return;
+  }
+  if (wgt_img) {
+    /* the weight image's value at the patch centre, zero where the mask is off there when
+     * the mask is used for weighting (:318-342) */
+    for (int64_t p = 0; p < np; ++p) {
+      const int64_t cr = patch_radius + (p / ppr) * patch_stride, cc = patch_radius + (p % ppr) * patch_stride;
+      w_out[p] = wgt_img[cr * cols + cc];
+      if (use_mask_wgts && !mask[cr * cols + cc]) w_out[p] = 0.0f;
+    }
+  } else {
+    /* fraction of unmasked pixels (:344-368) */
+    for (int64_t p = 0; p < np; ++p) {
+      const int32_t r0 = (int32_t)(p / ppr) * patch_stride, c0 = (int32_t)(p % ppr) * patch_stride;
+      int64_t cnt = 0;
+      for (int32_t y = 0; y < diam; ++y)
+        for (int32_t x = 0; x < diam; ++x) cnt += mask[(int64_t)(r0 + y) * cols + c0 + x] ? 1 : 0;
+      w_out[p] = (float)cnt / (float)((int64_t)diam * diam);
+    }
+  }
+  if (normalize) { /* normalize_weights_as_prob_ (:370-381), float sum in patch order */
+    float wsum = 0.0f;
+    for (int64_t p = 0; p < np; ++p) wsum += w_out[p];
+    for (int64_t p = 0; p < np; ++p) w_out[p] /= wsum;
+  }
+}
+
+void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                                int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                                const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
+                                int32_t compute_mean, float* sim_out, int32_t threads)
+{
+  const int32_t diam = 2 * patch_radius + 1;
+  const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int64_t np = (int64_t)ppr * ppc;
+  const int64_t nu = inds ? num_inds : np;
   const int64_t npix = (int64_t)rows * cols;
   patch_f32_ctx c;
   c.fixed = fixed;
@@ -1456,29 +1518,30 @@ void orc_patch_ncc_masked_f32(const float* fixed, const float* mov, const uint8_
   c.ppr = ppr;
   c.stride = patch_stride;
   c.fhat = (float*)malloc(sizeof(float) * (size_t)(np * diam * diam));
-  c.vals = (float*)malloc(sizeof(float) * (size_t)np);
+  c.vals = (float*)malloc(sizeof(float) * (size_t)(nu > 0 ? nu : 1));
   c.mask = mask;
-  float* wgt = NULL;
-  if (mask) {
-    wgt = (float*)malloc(sizeof(float) * (size_t)np);
-    patch_mask_weights_f32(mask, cols, ppr, np, patch_stride, diam, wgt);
-  }
   c.wgt = wgt;
+  c.inds = inds;
+  c.wic = weight_in_combine;
   orc_parallel_for(patch_fixed_range, &c, np, 256, threads);
   for (int64_t m = 0; m < num_mov; ++m) {
     c.mov = mov + m * npix;
-    orc_parallel_for(patch_mov_range, &c, np, 256, threads);
+    orc_parallel_for(patch_mov_range, &c, nu, 256, threads);
     float sum = 0.0f;
-    for (int64_t p = 0; p < np; ++p) sum += c.vals[p];
-    /* compute_mean_of_patch_sims_ is false by default; weighted combine divides by the
-     * sum of the weights (unit without a mask), accumulated in float (:272-285) */
-    float tot_wgt = 0.0f;
-    for (int64_t p = 0; p < np; ++p) tot_wgt += wgt ? wgt[p] : 1.0f;
-    sim_out[m] = sum / tot_wgt;
+    for (int64_t k = 0; k < nu; ++k) sum += c.vals[k];
+    /* combine (:262-287): the mean over the used patches, else the weighted sum divided by
+     * the float sum of the used patches' weights, else the plain sum */
+    if (compute_mean) {
+      sum /= (float)nu;
+    } else if (weight_in_combine) {
+      float tot_wgt = 0.0f;
+      for (int64_t k = 0; k < nu; ++k) tot_wgt += wgt ? wgt[inds ? inds[k] : k] : 1.0f;
+      sum /= tot_wgt;
+    }
+    sim_out[m] = sum;
   }
   free(c.fhat);
   free(c.vals);
-  free(wgt);
 }
 
 typedef struct {
@@ -1493,6 +1556,8 @@ typedef struct {
   double* vals;
   const uint8_t* mask;
   const double* wgt;
+  const int64_t* inds;
+  int32_t wic;
 } patch_f64_ctx;
 
 static void patch_stats_f64(const double* img, int32_t cols, int32_t r0, int32_t c0, int32_t d,
@@ -1531,12 +1596,13 @@ static void patch_mov_range_f64(void* vctx, int64_t begin, int64_t end)
   patch_f64_ctx* c = (patch_f64_ctx*)vctx;
   const int32_t d = c->diam;
   const double n = (double)d * d;
-  for (int64_t p = begin; p < end; ++p) {
+  for (int64_t k = begin; k < end; ++k) {
+    const int64_t p = c->inds ? c->inds[k] : k;
     const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
     const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
-    const double w = c->mask ? c->wgt[p] : 1.0;
-    if (c->mask && !(fabs(w) > 1.0e-6)) {
-      c->vals[p] = 0.0;
+    const double w = c->wgt ? c->wgt[p] : 1.0;
+    if (c->wic && !(fabs(w) > 1.0e-6)) {
+      c->vals[k] = 0.0;
       continue;
     }
     double mean, sd;
@@ -1548,7 +1614,7 @@ static void patch_mov_range_f64(void* vctx, int64_t begin, int64_t end)
         if (!c->mask || c->mask[o]) acc += ((c->mov[o] - mean) / sd) * ((c->fixed[o] - c->fmean[p]) / (c->fsd[p] * n));
       }
     }
-    c->vals[p] = w * (1.0 - acc);
+    c->vals[k] = (c->wic ? w : 1.0) * (1.0 - acc);
   }
 }
 
@@ -1563,24 +1629,12 @@ void orc_patch_ncc_masked_f64(const double* fixed, const double* mov, const uint
                               int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                               double* sim_out, int32_t threads)
 {
-  const int32_t diam = 2 * patch_radius + 1;
-  const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
-  const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
-  const int64_t np = (int64_t)ppr * ppc;
-  const int64_t npix = (int64_t)rows * cols;
-  patch_f64_ctx c;
-  c.fixed = fixed;
-  c.cols = cols;
-  c.diam = diam;
-  c.ppr = ppr;
-  c.stride = patch_stride;
-  c.fmean = (double*)malloc(sizeof(double) * (size_t)np);
-  c.fsd = (double*)malloc(sizeof(double) * (size_t)np);
-  c.vals = (double*)malloc(sizeof(double) * (size_t)np);
-  c.mask = mask;
   double* wgt = NULL;
-  double wtot = (double)np;
   if (mask) {
+    const int32_t diam = 2 * patch_radius + 1;
+    const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+    const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+    const int64_t np = (int64_t)ppr * ppc;
     wgt = (double*)malloc(sizeof(double) * (size_t)np);
     double wsum = 0.0;
     for (int64_t p = 0; p < np; ++p) {
@@ -1592,19 +1646,52 @@ void orc_patch_ncc_masked_f64(const double* fixed, const double* mov, const uint
       wsum += wgt[p];
     }
     for (int64_t p = 0; p < np; ++p) wgt[p] /= wsum;
-    wtot = sum_f64(wgt, np);
   }
+  orc_patch_ncc_weighted_f64(fixed, mov, mask, rows, cols, num_mov, patch_radius, patch_stride, wgt, NULL, 0, 1, 0,
+                             sim_out, threads);
+  free(wgt);
+}
+
+void orc_patch_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                                int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                                const double* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
+                                int32_t compute_mean, double* sim_out, int32_t threads)
+{
+  const int32_t diam = 2 * patch_radius + 1;
+  const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int64_t np = (int64_t)ppr * ppc;
+  const int64_t nu = inds ? num_inds : np;
+  const int64_t npix = (int64_t)rows * cols;
+  patch_f64_ctx c;
+  c.fixed = fixed;
+  c.cols = cols;
+  c.diam = diam;
+  c.ppr = ppr;
+  c.stride = patch_stride;
+  c.fmean = (double*)malloc(sizeof(double) * (size_t)np);
+  c.fsd = (double*)malloc(sizeof(double) * (size_t)np);
+  c.vals = (double*)malloc(sizeof(double) * (size_t)(nu > 0 ? nu : 1));
+  c.mask = mask;
   c.wgt = wgt;
+  c.inds = inds;
+  c.wic = weight_in_combine;
+  double norm = 1.0;
+  if (compute_mean) {
+    norm = (double)nu;
+  } else if (weight_in_combine) {
+    norm = 0.0;
+    for (int64_t k = 0; k < nu; ++k) norm += wgt ? wgt[inds ? inds[k] : k] : 1.0;
+  }
   orc_parallel_for(patch_fixed_range_f64, &c, np, 256, threads);
   for (int64_t m = 0; m < num_mov; ++m) {
     c.mov = mov + m * npix;
-    orc_parallel_for(patch_mov_range_f64, &c, np, 256, threads);
-    sim_out[m] = sum_f64(c.vals, np) / wtot;
+    orc_parallel_for(patch_mov_range_f64, &c, nu, 256, threads);
+    sim_out[m] = sum_f64(c.vals, nu) / norm;
   }
   free(c.fmean);
   free(c.fsd);
   free(c.vals);
-  free(wgt);
 }
 
 void orc_patch_grad_ncc_f32(const float* fixed, const float* mov, int32_t rows, int32_t cols,
@@ -1619,7 +1706,29 @@ void orc_patch_grad_ncc_masked_f32(const float* fixed, const float* mov, const u
                                    int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                                    int32_t patch_stride, float* sim_out, int32_t threads)
 {
-  /* ImgSimMetric2DPatchGradNCCCPU::compute (xregImgSimMetric2DPatchGradNCCCPU.cpp:104-224) */
+  float* wgt = NULL;
+  if (mask) {
+    const int32_t diam = 2 * patch_radius + 1;
+    const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+    const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+    const int64_t np = (int64_t)ppr * ppc;
+    wgt = (float*)malloc(sizeof(float) * (size_t)np);
+    patch_mask_weights_f32(mask, cols, ppr, np, patch_stride, diam, wgt);
+  }
+  orc_patch_grad_ncc_weighted_f32(fixed, mov, mask, rows, cols, num_mov, smooth_ksize, patch_radius, patch_stride,
+                                  wgt, NULL, 0, 1, 0, sim_out, threads);
+  free(wgt);
+}
+
+void orc_patch_grad_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                                     int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                     int32_t patch_stride, const float* wgt, const int64_t* inds, int64_t num_inds,
+                                     int32_t weight_in_combine, int32_t compute_mean, float* sim_out,
+                                     int32_t threads)
+{
+  /* ImgSimMetric2DPatchGradNCCCPU::compute (xregImgSimMetric2DPatchGradNCCCPU.cpp:104-224):
+   * both directions use the parent's patches and weights (enforce_same_patches_in_both_x_and_y_,
+   * :126-135) */
   const int64_t npix = (int64_t)rows * cols;
   float* fgx = (float*)malloc(sizeof(float) * (size_t)npix);
   float* fgy = (float*)malloc(sizeof(float) * (size_t)npix);
@@ -1629,8 +1738,10 @@ void orc_patch_grad_ncc_masked_f32(const float* fixed, const float* mov, const u
   for (int64_t m = 0; m < num_mov; ++m) {
     float sx, sy;
     orc_sobel_f32(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
-    orc_patch_ncc_masked_f32(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
-    orc_patch_ncc_masked_f32(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
+    orc_patch_ncc_weighted_f32(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
+                               weight_in_combine, compute_mean, &sx, threads);
+    orc_patch_ncc_weighted_f32(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
+                               weight_in_combine, compute_mean, &sy, threads);
     sim_out[m] = (float)(0.5 * (double)(sx + sy));
   }
   free(fgx);
@@ -1662,6 +1773,33 @@ void orc_patch_grad_ncc_masked_f64(const double* fixed, const double* mov, const
     orc_sobel_f64(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
     orc_patch_ncc_masked_f64(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, &sx, threads);
     orc_patch_ncc_masked_f64(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, &sy, threads);
+    sim_out[m] = 0.5 * (sx + sy);
+  }
+  free(fgx);
+  free(fgy);
+  free(mgx);
+  free(mgy);
+}
+
+void orc_patch_grad_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                                     int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                     int32_t patch_stride, const double* wgt, const int64_t* inds, int64_t num_inds,
+                                     int32_t weight_in_combine, int32_t compute_mean, double* sim_out,
+                                     int32_t threads)
+{
+  const int64_t npix = (int64_t)rows * cols;
+  double* fgx = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* fgy = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* mgx = (double*)malloc(sizeof(double) * (size_t)npix);
+  double* mgy = (double*)malloc(sizeof(double) * (size_t)npix);
+  orc_sobel_f64(fixed, rows, cols, smooth_ksize, fgx, fgy);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    double sx, sy;
+    orc_sobel_f64(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
+    orc_patch_ncc_weighted_f64(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
+                               weight_in_combine, compute_mean, &sx, threads);
+    orc_patch_ncc_weighted_f64(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
+                               weight_in_combine, compute_mean, &sy, threads);
     sim_out[m] = 0.5 * (sx + sy);
   }
   free(fgx);

@@ -165,6 +165,38 @@ void orc_patch_grad_ncc_f64(const double* fixed, const double* mov, int32_t rows
                             int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                             int32_t patch_stride, double* sim_out, int32_t threads);
 
+/* ImgSimMetric2DPatchCommon::compute_weights (xregImgSimMetric2DPatchCommon.cpp:306-400):
+ * per-patch weights from a weight image (its value at each patch centre; NULL for none) and
+ * a mask (NULL for none): with a weight image, the centre values, zeroed where the mask is off
+ * at the centre when use_mask_for_weighting; with only a mask used for weighting, the
+ * unmasked fraction of each patch; with neither, 1. normalize: divided by their float sum. */
+void orc_patch_weights_f32(const float* wgt_img, const uint8_t* mask, int32_t rows, int32_t cols,
+                           int32_t patch_radius, int32_t patch_stride, int32_t use_mask_for_weighting,
+                           int32_t normalize, float* w_out);
+
+/* Patch NCC / Patch-Grad-NCC with the reference's general combine
+ * (xregImgSimMetric2DPatchNCCCPU.cpp:186-287): per-patch weights wgt (NULL = 1), the patch
+ * indices used in order (inds, num_inds; NULL = all), weight_patch_sims_in_combine_ and
+ * compute_mean_of_patch_sims_. A mask restricts the correlation to unmasked pixels. */
+void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                                int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                                const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
+                                int32_t compute_mean, float* sim_out, int32_t threads);
+void orc_patch_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                                int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                                const double* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
+                                int32_t compute_mean, double* sim_out, int32_t threads);
+void orc_patch_grad_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                                     int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                     int32_t patch_stride, const float* wgt, const int64_t* inds, int64_t num_inds,
+                                     int32_t weight_in_combine, int32_t compute_mean, float* sim_out,
+                                     int32_t threads);
+void orc_patch_grad_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
+                                     int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                                     int32_t patch_stride, const double* wgt, const int64_t* inds, int64_t num_inds,
+                                     int32_t weight_in_combine, int32_t compute_mean, double* sim_out,
+                                     int32_t threads);
+
 /* Eigen 3.3 SSE sum restatement, exported for its own test. */
 float orc_eigen_sum_f32(const float* x, int64_t n);
 

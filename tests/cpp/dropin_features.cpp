@@ -12,7 +12,10 @@
 //   * a metric reading host moving images (set_mov_imgs_host_buf, xregImgSimMetric2D.h:122);
 //   * fewer projections and moving images than allocated, as the exhaustive driver's last batch
 //     (xregIntensity2D3DRegiExhaustive.cpp:188-196);
-//   * a metric with a mask (set_mask, xregImgSimMetric2D.h:130).
+//   * a metric with a mask (set_mask, xregImgSimMetric2D.h:130);
+//   * the patch metrics' options (ImgSimMetric2DPatchCommon, xregImgSimMetric2DPatchCommon.h:
+//     67-127): a weight image, mask weighting on and off, unnormalised weights, the mean and
+//     unweighted combines, a list of patches to use and random patches.
 //
 // usage: dropin_features <in.bin> <out.bin>
 // in.bin : int32 X Y Z rows cols n_poses metric smooth
@@ -253,6 +256,94 @@ int main(int argc, char** argv)
     rc->compute(0);
     mask_sim->compute();
     rec.put("mask_sims", mask_sim->sim_vals());
+
+    // patch options, on the same projections; the weight image is 1 + ((7 r + 3 c) mod 11)
+    {
+      auto ps = make_metric(metric, smooth);
+      auto* pc = dynamic_cast<ImgSimMetric2DPatchCommonHIP*>(ps.get());
+      if (pc) {
+        auto wimg = std::make_shared<Image2D>();
+        wimg->rows = rows;
+        wimg->cols = cols;
+        wimg->buf.resize(npx);
+        for (int r = 0; r < rows; ++r)
+          for (int c = 0; c < cols; ++c) wimg->buf[(size_t)r * cols + c] = 1.0f + (float)((7 * r + 3 * c) % 11);
+        ps->set_fixed_image(fixed);
+        ps->set_num_moving_images(np);
+        ps->set_mov_imgs_buf_from_ray_caster(rc.get(), 0);
+        ps->allocate_resources();
+        uint64_t unsupported = 0;
+        try {
+          pc->set_use_mask_for_patch_stats(true);
+        } catch (const ImgSimMetric2D::UnsupportedOperationException&) {
+          ++unsupported;
+        }
+        try {
+          pc->set_save_all_per_patch_scores(true);
+        } catch (const ImgSimMetric2D::UnsupportedOperationException&) {
+          ++unsupported;
+        }
+        rec.put_u64("popt_unsupported", unsupported);
+        pc->set_wgt_img(wimg);
+        ps->compute();
+        rec.put("popt_wimg_sims", ps->sim_vals());
+        std::vector<float> w;
+        for (const auto& p : pc->patch_infos()) w.push_back(p.weight);
+        rec.put("popt_wimg_weights", w);
+        // the mean of the unweighted patch scores
+        pc->set_wgt_img(nullptr);
+        pc->set_compute_mean_of_patch_sims(true);
+        pc->set_weight_patch_sims_in_combine(false);
+        ps->compute();
+        rec.put("popt_mean_sims", ps->sim_vals());
+        pc->set_compute_mean_of_patch_sims(false);
+        pc->set_weight_patch_sims_in_combine(true);
+        // a list of patches, with repeats, weighted by the weight image
+        pc->set_wgt_img(wimg);
+        PatchIndexList list;
+        for (size_type q = 3; q < pc->patch_infos().size(); q += 37) list.push_back(q);
+        list.push_back(list[1]);
+        pc->set_patches_to_use(list);
+        ps->compute();
+        rec.put("popt_list_sims", ps->sim_vals());
+        rec.put("popt_list_inds", std::vector<uint64_t>(list.begin(), list.end()).data(), list.size() * 8);
+        rec.put_u64("popt_list_num", pc->num_patches());
+        pc->reset_patches_to_use();
+        // random patches drawn by the weights; the draw is read back from the debug record
+        pc->set_choose_rand_patches(true);
+        pc->set_num_rand_patches(40);
+        ps->set_save_aux_info(true);
+        ps->compute();
+        rec.put("popt_rand_sims", ps->sim_vals());
+        {
+          std::shared_ptr<PatchNCCSimAux> a;
+          if (auto g = std::dynamic_pointer_cast<PatchGradNCCSimAux>(ps->aux_info())) {
+            a = g->sim_aux_x;
+          } else {
+            a = std::dynamic_pointer_cast<PatchNCCSimAux>(ps->aux_info());
+          }
+          const PatchIndexList& ix = a->patch_indices_per_compute_call.back();
+          rec.put("popt_rand_inds", std::vector<uint64_t>(ix.begin(), ix.end()).data(), ix.size() * 8);
+        }
+        ps->set_save_aux_info(false);
+        pc->set_choose_rand_patches(false);
+        // a mask and the weight image, unnormalised: zero at masked centres
+        pc->set_normalize_weights_as_prob(false);
+        ps->set_mask(mask);
+        ps->compute();
+        rec.put("popt_wimg_mask_sims", ps->sim_vals());
+        // the mask without mask weighting: unit weights, correlation still over unmasked pixels
+        pc->set_wgt_img(nullptr);
+        pc->set_normalize_weights_as_prob(true);
+        pc->set_use_mask_for_patch_weighting(false);
+        ps->compute();
+        rec.put("popt_nomaskwgt_sims", ps->sim_vals());
+        // every option back at its default: the masked metric's values
+        pc->set_use_mask_for_patch_weighting(true);
+        ps->compute();
+        rec.put("popt_default_sims", ps->sim_vals());
+      }
+    }
     rc->use_external_host_pixel_buf(nullptr);
 
     // two objects through the objective: REPLACE with A, then ACCUM with B

@@ -1,7 +1,8 @@
 """The rest of the drop-in classes' interface, driven as xReg's callers drive it
 (tests/cpp/dropin_features.cpp): MAX projections, two objects cast REPLACE then ACCUM, a static
 volume's DRR as background, an external host buffer, host-bound moving images, the exhaustive
-driver's short last batch and a masked metric, each against the oracle."""
+driver's short last batch, a masked metric and the patch metrics' options (weight image, mask
+weighting, mean and unweighted combines, listed and random patches), each against the oracle."""
 import subprocess
 
 import numpy as np
@@ -26,7 +27,7 @@ def _records(raw: bytes) -> dict:
     return out
 
 
-@pytest.mark.parametrize("metric,smooth", [(0, 0), (3, 5)])
+@pytest.mark.parametrize("metric,smooth", [(0, 0), (3, 5), (2, 0)])
 def test_dropin_features(tmp_path, oracle, metric, smooth):
     from xreg_amd import synth
 
@@ -120,3 +121,48 @@ def test_dropin_features(tmp_path, oracle, metric, smooth):
         assert u64("aux_calls") == 0
     assert u64("aux_bad") == 0
     assert u64("max_num_projs_possible") >= npose
+    if metric in (2, 3):
+        _check_patch_options(oracle, rec, metric, smooth, fixed, ref_a, mask)
+
+
+def _check_patch_options(oracle, rec, metric, smooth, fixed, drr, mask):
+    """The patch options the driver set (ImgSimMetric2DPatchCommon), each against the oracle's
+    general patch combine with the weights compute_weights gives."""
+    rows, cols = fixed.shape
+    rr, cc = np.mgrid[0:rows, 0:cols]
+    wimg = (1.0 + (7 * rr + 3 * cc) % 11).astype(np.float32)
+    assert int(np.frombuffer(rec["popt_unsupported"], np.uint64)[0]) == 2
+    n_p = (rows - 10) * (cols - 10)
+
+    def check(name, **kw):
+        got = np.frombuffer(rec[name], np.float32)
+        ref = oracle.patch_sim_weighted_f32(metric, fixed, drr, smooth_ksize=smooth, **kw)
+        ref64 = oracle.patch_sim_weighted_f64(metric, fixed.astype(np.float64), drr.astype(np.float64),
+                                              smooth_ksize=smooth, **kw)
+        print(f"  {name}: max |gpu-cpu| {np.abs(got - ref).max():.3e}", flush=True)
+        assert np.all(np.abs(got - ref) <= SIM_ABS_TOL + np.abs(ref - ref64)), name
+        return got
+
+    w_img = oracle.patch_weights_f32(fixed.shape, 5, 1, wgt_img=wimg)
+    assert np.array_equal(np.frombuffer(rec["popt_wimg_weights"], np.float32), w_img)
+    check("popt_wimg_sims", weights=w_img)
+    check("popt_mean_sims", weight_in_combine=False, compute_mean=True)
+    lst = list(range(3, n_p, 37))
+    lst.append(lst[1])
+    assert np.array_equal(np.frombuffer(rec["popt_list_inds"], np.uint64), np.array(lst, np.uint64))
+    assert int(np.frombuffer(rec["popt_list_num"], np.uint64)[0]) == len(lst)
+    check("popt_list_sims", weights=w_img, inds=lst)
+    # random patches: 40 distinct patches whose centres lie at least sqrt(2) r apart
+    ri = np.frombuffer(rec["popt_rand_inds"], np.uint64).astype(np.int64)
+    assert ri.size == 40 and np.unique(ri).size == 40 and ri.max() < n_p
+    ppr = cols - 10
+    ctr = np.stack([ri // ppr, ri % ppr], axis=1).astype(np.float64)
+    dist = np.sqrt(((ctr[:, None] - ctr[None]) ** 2).sum(-1)) + np.eye(40) * 1e9
+    assert dist.min() >= np.sqrt(2.0) * 5
+    check("popt_rand_sims", weights=w_img, inds=ri)
+    w_mask = oracle.patch_weights_f32(fixed.shape, 5, 1, wgt_img=wimg, mask=mask, normalize=False)
+    assert np.any(w_mask == 0) and np.any(w_mask > 1)
+    check("popt_wimg_mask_sims", weights=w_mask, mask=mask)
+    check("popt_nomaskwgt_sims", mask=mask)
+    assert np.array_equal(np.frombuffer(rec["popt_default_sims"], np.float32),
+                          np.frombuffer(rec["mask_sims"], np.float32))

@@ -1,0 +1,183 @@
+"""GPU parity of the general patch combine (xrh_fixed_set_patch_weights: patch weights, the
+patches used, weight-in-combine and mean flags) against the oracle, through the C-ABI, on the
+specialised stride-1 kernel and the general kernel, with and without a mask, plain and batched."""
+import numpy as np
+import pytest
+
+from tests.helpers import SIM_ABS_TOL, poses_f32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def images(oracle):
+    from xreg_amd import synth
+
+    w = synth.config("c2", n_poses=4, small=True)
+    poses = np.concatenate([w.gt[None], w.poses])
+    drr, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(poses), step=w.step)
+    fixed = synth.noisy_fixed(drr[0])
+    return fixed, drr
+
+
+def _mask(shape):
+    m = np.ones(shape, np.uint8)
+    m[:15, :25] = 0
+    m[40:52, 10:70] = 0
+    return m
+
+
+def _tol(ref):
+    # a plain sum of many patch scores grows with their count; the float reference rounds in
+    # proportion to it
+    return SIM_ABS_TOL * np.maximum(1.0, np.abs(ref))
+
+
+CASES = [
+    # (radius, stride, masked): radius 5 stride 1 unmasked runs the specialised kernel
+    (5, 1, False),
+    (4, 2, False),
+    (5, 1, True),
+    (3, 1, True),
+]
+
+
+@pytest.mark.parametrize("radius,stride,masked", CASES)
+@pytest.mark.parametrize("metric", [2, 3])
+def test_weighted_combines_match_oracle(ctx, oracle, images, metric, radius, stride, masked):
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    mask = _mask(fixed.shape) if masked else None
+    print(f"case: patch weights metric={metric} r={radius} s={stride} masked={masked}", flush=True)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, metric, smooth_ksize=5, patch_radius=radius, patch_stride=stride, mask=mask)
+    default = f.compute(buf, mov.shape[0])
+    n_p = f.patches()["weights"].size
+    rng = np.random.default_rng(radius * 10 + stride)
+    w = rng.uniform(0.05, 2.0, n_p).astype(np.float32)
+    w[rng.integers(0, n_p, 20)] = 0.0
+    inds = rng.integers(0, n_p, 150)
+    inds[:10] = inds[10]  # repeats
+    settings = [
+        dict(weights=w),
+        dict(weights=w, patch_inds=inds),
+        dict(weights=w, patch_inds=inds, compute_mean=True),
+        dict(weights=w, patch_inds=inds, weight_in_combine=False),
+        dict(weights=w, weight_in_combine=False, compute_mean=True),
+        dict(patch_inds=inds),  # the creation weights (unit, or the mask's)
+    ]
+    for st in settings:
+        f.set_patch_weights(**st)
+        got = f.compute(buf, mov.shape[0])
+        kw = dict(weights=st.get("weights"), inds=st.get("patch_inds"),
+                  weight_in_combine=st.get("weight_in_combine", True), compute_mean=st.get("compute_mean", False),
+                  smooth_ksize=5, radius=radius, stride=stride, mask=mask)
+        if kw["weights"] is None and masked:
+            kw["weights"] = oracle.patch_weights_f32(fixed.shape, radius, stride, mask=mask)
+        ref = oracle.patch_sim_weighted_f32(metric, fixed, mov, **kw)
+        ref64 = oracle.patch_sim_weighted_f64(metric, fixed.astype(np.float64), mov.astype(np.float64), **kw)
+        names = sorted(k for k in st if k != "weights")
+        print(f"  {names}: max |gpu-cpu| {np.abs(got - ref).max():.3e} |gpu-fp64| {np.abs(got - ref64).max():.3e}"
+              f" |cpu-fp64| {np.abs(ref - ref64).max():.3e}", flush=True)
+        assert np.all(np.abs(got - ref) <= _tol(ref) + np.abs(ref - ref64))
+        assert np.all(np.abs(got - ref64) <= _tol(ref64))
+    # the default restored: unmasked bit-identical, masked within rounding of the folded tables
+    f.set_patch_weights()
+    again = f.compute(buf, mov.shape[0])
+    if masked:
+        assert np.abs(again - default).max() <= 1e-6
+    else:
+        assert np.array_equal(again, default)
+    f.close()
+    buf.free()
+
+
+def test_skipped_and_single_patches(ctx, oracle, images):
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, 3, smooth_ksize=5, patch_radius=5)
+    n_p = f.patches()["weights"].size
+    w = np.ones(n_p, np.float32)
+    q = n_p // 3 + 7
+    w[q] = 5e-7
+    # a patch of weight <= 1e-6 used alone: skipped, so the score is exactly 0
+    f.set_patch_weights(weights=w, patch_inds=[q])
+    assert np.array_equal(f.compute(buf, mov.shape[0]), np.zeros(mov.shape[0], np.float32))
+    # one patch scored on its own equals the oracle's score of that patch
+    for wic in (True, False):
+        f.set_patch_weights(weights=w, patch_inds=[q + 1], weight_in_combine=wic)
+        got = f.compute(buf, mov.shape[0])
+        ref = oracle.patch_sim_weighted_f32(3, fixed, mov, inds=[q + 1], smooth_ksize=5, radius=5)
+        assert np.abs(got - ref).max() <= SIM_ABS_TOL
+    # the reported weights are the ones last set
+    assert np.array_equal(f.patches()["weights"], w)
+    f.close()
+    buf.free()
+
+
+def test_rejected_settings_change_nothing(ctx, images):
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, 2, patch_radius=5)
+    n_p = f.patches()["weights"].size
+    w = np.linspace(0.5, 1.0, n_p).astype(np.float32)
+    f.set_patch_weights(weights=w, patch_inds=np.arange(0, n_p, 7))
+    before = f.compute(buf, mov.shape[0])
+    bad = [
+        dict(patch_inds=[0, n_p]),
+        dict(patch_inds=[-1]),
+        dict(patch_inds=np.zeros(0, np.int64)),
+        dict(weights=np.zeros(n_p, np.float32)),             # weights of the used patches sum to 0
+        dict(weights=np.full(n_p, np.nan, np.float32)),
+    ]
+    for st in bad:
+        with pytest.raises(X.XrhError):
+            f.set_patch_weights(**st)
+        assert np.array_equal(f.compute(buf, mov.shape[0]), before)
+    with pytest.raises(ValueError):
+        f.set_patch_weights(weights=np.ones(n_p - 1, np.float32))
+    # zero weights are fine when they do not divide: plain sum
+    f.set_patch_weights(weights=np.zeros(n_p, np.float32), weight_in_combine=False, compute_mean=True)
+    f.close()
+    g = X.Fixed(ctx, fixed, 0)
+    with pytest.raises(X.XrhError):
+        g.set_patch_weights(patch_inds=[0])
+    g.close()
+    buf.free()
+
+
+def test_batch_and_graph_replays_follow_the_weights(ctx, oracle):
+    from xreg_amd import _xrh as X
+    from xreg_amd import synth
+
+    w = synth.config("c2", n_poses=5, small=True)
+    gt_drr, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams[:1], poses_f32(w.gt[None]), step=w.step)
+    fixed = synth.noisy_fixed(gt_drr[0])
+    vol = X.Volume(ctx, w.vol, w.idx_to_phys)
+    fx = X.Fixed(ctx, fixed, 3)
+    b = X.Batch(ctx, [X.make_view(vol, fx, w.cams[0], w.step)], max_items=8)
+    iv = np.zeros(w.poses.shape[0], np.int32)
+    drr_ref, _ = oracle.drr_f32(w.vol, w.idx_to_phys, w.cams, poses_f32(w.poses), step=w.step)
+    first = b.eval(iv, w.poses)  # builds and replays the graph
+    n_p = fx.patches()["weights"].size
+    rng = np.random.default_rng(9)
+    wt = rng.uniform(0.1, 1.0, n_p).astype(np.float32)
+    inds = rng.integers(0, n_p, 100)
+    for k in range(3):
+        fx.set_patch_weights(weights=wt, patch_inds=inds[k:], compute_mean=(k == 2))
+        got = b.eval(iv, w.poses)
+        ref = oracle.patch_sim_weighted_f32(3, fixed, drr_ref, weights=wt, inds=inds[k:], compute_mean=(k == 2))
+        print(f"case: batch weights {k}: max |gpu-cpu| {np.abs(got - ref).max():.3e}", flush=True)
+        assert np.abs(got - ref).max() <= SIM_ABS_TOL
+    fx.set_patch_weights()
+    assert np.array_equal(b.eval(iv, w.poses), first)
+    for o in (b, fx, vol):
+        o.close()

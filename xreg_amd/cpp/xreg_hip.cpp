@@ -550,6 +550,7 @@ void ImgSimMetric2DHIP::allocate_resources()
   check_xrh(xrh_fixed_create(ctx_, fixed_img_->buf.data(), (int32_t)fixed_img_->rows, (int32_t)fixed_img_->cols,
                              &params_, mk, &fixed_));
   mask_updated_ = false;
+  ++fixed_gen_;
 }
 
 void ImgSimMetric2DHIP::compute()
@@ -587,41 +588,259 @@ void ImgSimMetric2DHIP::compute()
     mov = mov_dev_;
   }
   sim_vals_.resize(num_mov_imgs_);
+  before_sim(fixed_, fixed_gen_);
   check_xrh(xrh_sim_compute(ctx_, fixed_, mov, (int64_t)num_mov_imgs_, nullptr, sim_vals_.data()));
-  if (save_aux_info_ && (params_.metric == XRH_SIM_PATCH_NCC || params_.metric == XRH_SIM_PATCH_GRAD_NCC)) {
-    // ImgSimMetric2DPatchNCCCPU::compute with save_aux_info (xregImgSimMetric2DPatchNCCCPU.cpp:
-    // 290-298): every call appends the patch list and the indices used (all: random-patch mode
-    // is off); the gradient form keeps the same for x and y (xregImgSimMetric2DPatchGradNCCCPU.cpp:
-    // 246-252)
-    int64_t n = 0;
-    check_xrh(xrh_fixed_patches(fixed_, 0, nullptr, nullptr, nullptr, nullptr, nullptr, &n));
-    std::vector<int32_t> r0((size_t)n), c0((size_t)n), r1((size_t)n), c1((size_t)n);
-    std::vector<float> w((size_t)n);
-    check_xrh(xrh_fixed_patches(fixed_, n, r0.data(), c0.data(), r1.data(), c1.data(), w.data(), &n));
-    PatchInfoList pl((size_t)n);
-    PatchIndexList inds((size_t)n);
-    for (int64_t q = 0; q < n; ++q) {
-      pl[(size_t)q] = {(size_type)r0[(size_t)q], (size_type)c0[(size_t)q], (size_type)r1[(size_t)q],
-                       (size_type)c1[(size_t)q], w[(size_t)q]};
-      inds[(size_t)q] = (size_type)q;
+  after_sim();
+}
+
+// ------------------------------------------------------------------------------------------
+// ImgSimMetric2DPatchCommon
+
+ImgSimMetric2DPatchCommonHIP::ImgSimMetric2DPatchCommonHIP(xrh_sim_params* p)
+    : pp_(p), rng_(std::make_shared<std::mt19937>())
+{
+  // SeedRNGEngWithRandDev at patch set-up (xregImgSimMetric2DPatchCommon.cpp:262-265)
+  std::random_device rd;
+  static_cast<std::mt19937*>(rng_.get())->seed(rd());
+}
+
+void ImgSimMetric2DPatchCommonHIP::set_use_mask_for_patch_weighting(bool b)
+{
+  use_mask_wgt_ = b;
+  weights_stale_ = true;
+}
+
+void ImgSimMetric2DPatchCommonHIP::set_normalize_weights_as_prob(bool b)
+{
+  normalize_ = b;
+  weights_stale_ = true;
+}
+
+void ImgSimMetric2DPatchCommonHIP::set_use_mask_for_patch_stats(bool u)
+{
+  if (u) throw ImgSimMetric2D::UnsupportedOperationException();
+}
+
+void ImgSimMetric2DPatchCommonHIP::set_save_all_per_patch_scores(bool s)
+{
+  if (s) throw ImgSimMetric2D::UnsupportedOperationException();
+}
+
+size_type ImgSimMetric2DPatchCommonHIP::num_patches() const
+{
+  // xregImgSimMetric2DPatchCommon.cpp:180-184
+  return fixed_inds_ ? patch_inds_to_use_.size() : (choose_rand_ ? num_rand_ : patch_infos_.size());
+}
+
+void ImgSimMetric2DPatchCommonHIP::set_patches_to_use(const PatchIndexList& inds)
+{
+  patch_inds_to_use_ = inds;
+  fixed_inds_ = true;
+}
+
+void ImgSimMetric2DPatchCommonHIP::reset_patches_to_use()
+{
+  patch_inds_to_use_.clear();
+  fixed_inds_ = false;
+}
+
+void ImgSimMetric2DPatchCommonHIP::set_wgt_img(WgtImgPtr w)
+{
+  wgt_img_ = w;
+  weights_stale_ = true;
+}
+
+void ImgSimMetric2DPatchCommonHIP::compute_weights(const Image2D& img, const std::vector<unsigned char>* mask)
+{
+  // ImgSimMetric2DPatchCommon::compute_weights (xregImgSimMetric2DPatchCommon.cpp:306-400)
+  const bool use_mask_wgts = use_mask_wgt_ && mask;
+  const bool use_img_wgts = (bool)wgt_img_;
+  for (PatchInfo& p : patch_infos_) p.weight = 1.0f;
+  if (!use_img_wgts && !use_mask_wgts) return;
+  const size_type cols = img.cols;
+  if (use_img_wgts) {
+    if (wgt_img_->rows != img.rows || wgt_img_->cols != img.cols) {
+      throw StringMessageException("weight image and fixed image sizes differ");
     }
-    auto append = [&](std::shared_ptr<PatchNCCSimAux>& a) {
-      if (!a) a = std::make_shared<PatchNCCSimAux>();
-      a->patch_infos_per_compute_call.push_back(pl);
-      a->patch_indices_per_compute_call.push_back(inds);
-    };
-    if (params_.metric == XRH_SIM_PATCH_NCC) {
-      auto a = std::static_pointer_cast<PatchNCCSimAux>(sim_aux_);
-      append(a);
-      sim_aux_ = a;
-    } else {
-      auto g = std::static_pointer_cast<PatchGradNCCSimAux>(sim_aux_);
-      if (!g) g = std::make_shared<PatchGradNCCSimAux>();
-      append(g->sim_aux_x);
-      append(g->sim_aux_y);
-      sim_aux_ = g;
+    for (PatchInfo& p : patch_infos_) {
+      // centre = start + (stop - start + 1) / 2 (PatchInfo::center_row_col, .cpp:33-37)
+      const size_type cr = p.start_row + (p.stop_row - p.start_row + 1) / 2;
+      const size_type cc = p.start_col + (p.stop_col - p.start_col + 1) / 2;
+      p.weight = wgt_img_->buf[cr * cols + cc];
+      if (use_mask_wgts && !(*mask)[cr * cols + cc]) p.weight = 0.0f;
+    }
+  } else {
+    for (PatchInfo& p : patch_infos_) {
+      size_type cnt = 0;
+      for (size_type r = p.start_row; r <= p.stop_row; ++r)
+        for (size_type c = p.start_col; c <= p.stop_col; ++c) cnt += (*mask)[r * cols + c] ? 1 : 0;
+      const size_type d = p.stop_row - p.start_row + 1;
+      p.weight = (float)cnt / (float)(d * d);
     }
   }
+  if (normalize_) {
+    float sum = 0.0f;
+    for (const PatchInfo& p : patch_infos_) sum += p.weight;
+    for (PatchInfo& p : patch_infos_) p.weight /= sum;
+  }
+}
+
+PatchIndexList ImgSimMetric2DPatchCommonHIP::draw_random_patches()
+{
+  // ImgSimMetric2DPatchCommon::patch_indices_to_use (xregImgSimMetric2DPatchCommon.cpp:402-493):
+  // draws proportional to the weights; a draw closer than the minimum separation to a patch
+  // already drawn is rejected and drawn again. The reference retries without bound; here a
+  // draw that cannot be completed within 1000 tries per patch throws.
+  const size_type np = patch_infos_.size();
+  if (num_rand_ >= np) throw StringMessageException("the number of random patches must be below the patch count");
+  std::vector<double> w;
+  w.reserve(np);
+  double tot = 0.0;
+  for (const PatchInfo& p : patch_infos_) {
+    if (!(p.weight >= 0.0f) || !std::isfinite(p.weight)) {
+      throw StringMessageException("random patches need finite, non-negative patch weights");
+    }
+    w.push_back(p.weight);
+    tot += p.weight;
+  }
+  if (!(tot > 0.0)) throw StringMessageException("random patches need a positive weight sum");
+  std::discrete_distribution<size_type> dist(w.begin(), w.end());
+  auto& eng = *static_cast<std::mt19937*>(rng_.get());
+  const double r = (double)pp_->patch_radius;
+  const double min_sep = (rand_sep_ < 0.0) ? std::sqrt(2.0 * r * r) : rand_sep_;
+  const bool check_sep = min_sep > 1.0e-8;
+  PatchIndexList inds;
+  inds.reserve(num_rand_);
+  std::vector<std::array<double, 2>> centres;
+  size_type tries = 0;
+  const size_type max_tries = 1000 * (num_rand_ + 1);
+  while (inds.size() < num_rand_) {
+    if (++tries > max_tries) {
+      throw StringMessageException("cannot place " + std::to_string(num_rand_) +
+                                   " random patches at the minimum separation");
+    }
+    const size_type q = dist(eng);
+    if (check_sep) {
+      const PatchInfo& p = patch_infos_[q];
+      const double cr = (double)(p.start_row + (p.stop_row - p.start_row + 1) / 2);
+      const double cc = (double)(p.start_col + (p.stop_col - p.start_col + 1) / 2);
+      bool ok = true;
+      for (const auto& e : centres) {
+        if (std::hypot(e[0] - cr, e[1] - cc) < min_sep) {
+          ok = false;
+          break;
+        }
+      }
+      if (!ok) continue;
+      centres.push_back({cr, cc});
+    }
+    inds.push_back(q);
+  }
+  return inds;
+}
+
+void ImgSimMetric2DPatchCommonHIP::prepare_patches(xrh_fixed* fixed, uint64_t gen, const Image2D& img,
+                                                  const std::vector<unsigned char>* mask)
+{
+  if (gen != grid_gen_) {
+    // the grid of a new fixed image (setup_patches, xregImgSimMetric2DPatchCommon.cpp:257-298)
+    int64_t n = 0;
+    check_xrh(xrh_fixed_patches(fixed, 0, nullptr, nullptr, nullptr, nullptr, nullptr, &n));
+    std::vector<int32_t> r0((size_t)n), c0((size_t)n), r1((size_t)n), c1((size_t)n);
+    check_xrh(xrh_fixed_patches(fixed, n, r0.data(), c0.data(), r1.data(), c1.data(), nullptr, &n));
+    patch_infos_.assign((size_t)n, PatchInfo{});
+    for (size_t q = 0; q < (size_t)n; ++q) {
+      patch_infos_[q] = {(size_type)r0[q], (size_type)c0[q], (size_type)r1[q], (size_type)c1[q], 1.0f};
+    }
+    grid_gen_ = gen;
+    weights_stale_ = true;
+    std::random_device rd;
+    static_cast<std::mt19937*>(rng_.get())->seed(rd());
+  }
+  if (weights_stale_) {
+    compute_weights(img, mask);
+    weights_stale_ = false;
+  }
+  const size_type np = patch_infos_.size();
+  if (fixed_inds_) {
+    for (size_type q : patch_inds_to_use_) {
+      if (q >= np) throw StringMessageException("patch index to use out of range");
+    }
+  } else if (choose_rand_) {
+    patch_inds_to_use_ = draw_random_patches();
+  } else {
+    patch_inds_to_use_.resize(np);
+    for (size_type q = 0; q < np; ++q) patch_inds_to_use_[q] = q;
+  }
+  // the fixed image's creation weights are the reference defaults: unit, or the normalised
+  // unmasked fractions of the mask
+  const bool default_weights = !wgt_img_ && (mask ? (use_mask_wgt_ && normalize_) : true);
+  const bool all = !fixed_inds_ && !choose_rand_;
+  const bool dflt = default_weights && all && wic_ && !compute_mean_;
+  std::vector<float> w;
+  if (!dflt) {
+    w.resize(np);
+    for (size_type q = 0; q < np; ++q) w[q] = patch_infos_[q].weight;
+  }
+  const PatchIndexList& inds = all ? PatchIndexList() : patch_inds_to_use_;
+  const bool same = applied_gen_ == gen && applied_default_ == dflt &&
+                    (dflt || (applied_w_ == w && applied_inds_ == inds && applied_wic_ == wic_ &&
+                              applied_mean_ == compute_mean_));
+  if (same) return;
+  if (dflt) {
+    check_xrh(xrh_fixed_set_patch_weights(fixed, nullptr, nullptr, 0, 1, 0));
+  } else {
+    std::vector<int64_t> i64(inds.begin(), inds.end());
+    check_xrh(xrh_fixed_set_patch_weights(fixed, w.data(), all ? nullptr : i64.data(), (int64_t)i64.size(),
+                                          wic_ ? 1 : 0, compute_mean_ ? 1 : 0));
+  }
+  applied_gen_ = gen;
+  applied_default_ = dflt;
+  applied_w_ = std::move(w);
+  applied_inds_ = inds;
+  applied_wic_ = wic_;
+  applied_mean_ = compute_mean_;
+}
+
+void ImgSimMetric2DPatchCommonHIP::record_patches(std::shared_ptr<PatchNCCSimAux>& a) const
+{
+  // ImgSimMetric2DPatchNCCCPU::compute with save_aux_info (xregImgSimMetric2DPatchNCCCPU.cpp:
+  // 290-298): every call appends the patch list and the indices used
+  if (!a) a = std::make_shared<PatchNCCSimAux>();
+  a->patch_infos_per_compute_call.push_back(patch_infos_);
+  a->patch_indices_per_compute_call.push_back(patch_inds_to_use_);
+}
+
+void ImgSimMetric2DPatchNCCHIP::before_sim(xrh_fixed* f, uint64_t gen)
+{
+  prepare_patches(f, gen, *fixed_img_, mask_.get());
+}
+
+void ImgSimMetric2DPatchNCCHIP::after_sim()
+{
+  if (!save_aux_info_) return;
+  auto a = std::static_pointer_cast<PatchNCCSimAux>(sim_aux_);
+  record_patches(a);
+  sim_aux_ = a;
+}
+
+void ImgSimMetric2DPatchGradNCCHIP::before_sim(xrh_fixed* f, uint64_t gen)
+{
+  if (!same_xy_ && (choose_rand_patches() || patches_listed())) {
+    throw ImgSimMetric2D::UnsupportedOperationException();
+  }
+  prepare_patches(f, gen, *fixed_img_, mask_.get());
+}
+
+void ImgSimMetric2DPatchGradNCCHIP::after_sim()
+{
+  if (!save_aux_info_) return;
+  // the x and y sub-metrics keep the same record (xregImgSimMetric2DPatchGradNCCCPU.cpp:246-252)
+  auto g = std::static_pointer_cast<PatchGradNCCSimAux>(sim_aux_);
+  if (!g) g = std::make_shared<PatchGradNCCSimAux>();
+  record_patches(g->sim_aux_x);
+  record_patches(g->sim_aux_y);
+  sim_aux_ = g;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -5,7 +5,7 @@
 //   RayCastLineIntParamInterface  xregRayCastInterface.h:581-591
 //   ImgSimMetric2D            lib/regi/sim_metrics_2d/xregImgSimMetric2D.h:42-156
 //   ImgSimMetric2DGradImgParamInterface  xregImgSimMetric2DGradImgParamInterface.h:31-39
-//   ImgSimMetric2DPatchCommon (patch radius / stride)  xregImgSimMetric2DPatchCommon.h:67-127
+//   ImgSimMetric2DPatchCommon xregImgSimMetric2DPatchCommon.h:38-181
 //   ImgSimMetric2DCombineMean xregImgSimMetric2DCombine.h:108
 // so that Intensity2D3DRegi and its drivers can run unchanged. xReg's ITK / Eigen / OpenCV
 // value types are replaced by small POD equivalents here (this build has none of those
@@ -368,12 +368,17 @@ class ImgSimMetric2DHIP : public ImgSimMetric2D {
   std::shared_ptr<H5ReadWriteInterface> aux_info() override { return sim_aux_; }
 
  protected:
+  // hooks of the patch metrics around each xrh_sim_compute call; fixed_gen counts the fixed
+  // images allocate_resources has created
+  virtual void before_sim(xrh_fixed*, uint64_t /*fixed_gen*/) {}
+  virtual void after_sim() {}
   xrh_sim_params params_{};
   std::shared_ptr<H5ReadWriteInterface> sim_aux_;
 
  private:
   xrh_ctx* ctx_;
   xrh_fixed* fixed_ = nullptr;
+  uint64_t fixed_gen_ = 0;
   RayCaster* rc_ = nullptr;
   size_type proj_off_ = 0;
   Scalar* host_mov_ = nullptr;
@@ -403,23 +408,93 @@ class ImgSimMetric2DGradNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2D
   void set_smooth_img_before_sobel_kernel_radius(size_type r) override { params_.smooth_ksize = (int32_t)r; }
 };
 
+// ImgSimMetric2DPatchCommon (xregImgSimMetric2DPatchCommon.h:38-181, .cpp:27-493): the patch
+// grid, the patch weights (a weight image, the mask, normalisation: compute_weights,
+// .cpp:306-400), the patches used per compute call (all; a weight-proportional random draw
+// with a minimum centre separation, .cpp:402-493; or a caller's list) and how their scores are
+// combined (PatchNCCCPU::compute, xregImgSimMetric2DPatchNCCCPU.cpp:186-287). This is host
+// bookkeeping: the device scores every patch of the grid and applies the weights and the
+// patches used through xrh_fixed_set_patch_weights. The weights are recomputed whenever an
+// input to them (weight image, mask, options, grid) changes. Not implemented, and refused with
+// UnsupportedOperationException: patch statistics over the unmasked pixels only
+// (use_mask_for_patch_stats), per-patch score records (save_all_per_patch_scores), and the
+// Patch-NCC / Patch-Grad-NCC variance-as-weight options.
 class ImgSimMetric2DPatchCommonHIP {
  public:
+  using WgtImgPtr = ImagePtr;
   virtual ~ImgSimMetric2DPatchCommonHIP() = default;
   size_type patch_radius() const { return (size_type)pp_->patch_radius; }
   void set_patch_radius(size_type r) { pp_->patch_radius = (int32_t)r; }
   size_type patch_stride() const { return (size_type)pp_->patch_stride; }
   void set_patch_stride(size_type s) { pp_->patch_stride = (int32_t)s; }
+  bool compute_mean_of_patch_sims() const { return compute_mean_; }
+  void set_compute_mean_of_patch_sims(bool b) { compute_mean_ = b; }
+  bool weight_patch_sims_in_combine() const { return wic_; }
+  void set_weight_patch_sims_in_combine(bool b) { wic_ = b; }
+  bool use_mask_for_patch_weighting() const { return use_mask_wgt_; }
+  void set_use_mask_for_patch_weighting(bool b);
+  bool use_mask_for_patch_stats() const { return false; }
+  void set_use_mask_for_patch_stats(bool u);
+  bool save_all_per_patch_scores() const { return false; }
+  void set_save_all_per_patch_scores(bool s);
+  bool normalize_weights_as_prob() const { return normalize_; }
+  void set_normalize_weights_as_prob(bool b);
+  bool choose_rand_patches() const { return choose_rand_; }
+  void set_choose_rand_patches(bool b) { choose_rand_ = b; }
+  size_type num_rand_patches() const { return num_rand_; }
+  void set_num_rand_patches(size_type n) { num_rand_ = n; }
+  double rand_patch_min_pixels_sep() const { return rand_sep_; }
+  void set_rand_patch_min_pixels_sep(double d) { rand_sep_ = d; }
+  // the grid as of the last compute call, with the weights in force
+  const PatchInfoList& patch_infos() const { return patch_infos_; }
+  // patches used per compute call: the caller's list, the random draw's size, or the grid's
+  size_type num_patches() const;
+  void set_patches_to_use(const PatchIndexList& inds);
+  void reset_patches_to_use();
+  void set_wgt_img(WgtImgPtr w);
+  WgtImgPtr wgt_img() const { return wgt_img_; }
 
  protected:
-  explicit ImgSimMetric2DPatchCommonHIP(xrh_sim_params* p) : pp_(p) {}
+  explicit ImgSimMetric2DPatchCommonHIP(xrh_sim_params* p);
+  // before each compute: the grid of a new fixed image, the weights, this call's patches, and
+  // the combine handed to the device when it differs from what the fixed image holds
+  void prepare_patches(xrh_fixed* fixed, uint64_t fixed_gen, const Image2D& fixed_img,
+                       const std::vector<unsigned char>* mask);
+  // the SimAux record of this call: the patch list and the indices used
+  void record_patches(std::shared_ptr<PatchNCCSimAux>& a) const;
+  bool patches_listed() const { return fixed_inds_; }
   xrh_sim_params* pp_;
+
+ private:
+  void compute_weights(const Image2D& fixed_img, const std::vector<unsigned char>* mask);
+  PatchIndexList draw_random_patches();
+  bool compute_mean_ = false, wic_ = true, use_mask_wgt_ = true, normalize_ = true;
+  bool choose_rand_ = false;
+  size_type num_rand_ = 100;
+  double rand_sep_ = -1.0;
+  bool fixed_inds_ = false;  // set_patches_to_use in force
+  PatchIndexList patch_inds_to_use_;
+  WgtImgPtr wgt_img_;
+  PatchInfoList patch_infos_;
+  uint64_t grid_gen_ = 0;       // fixed image the grid belongs to (0: none)
+  bool weights_stale_ = true;
+  std::shared_ptr<void> rng_;   // std::mt19937 (kept out of this header)
+  // what the fixed image's device tables hold
+  uint64_t applied_gen_ = 0;
+  bool applied_default_ = true;
+  std::vector<float> applied_w_;
+  PatchIndexList applied_inds_;
+  bool applied_wic_ = true, applied_mean_ = false;
 };
 
 class ImgSimMetric2DPatchNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2DPatchCommonHIP {
  public:
   explicit ImgSimMetric2DPatchNCCHIP(int device = 0)
       : ImgSimMetric2DHIP(XRH_SIM_PATCH_NCC, device), ImgSimMetric2DPatchCommonHIP(&params_) {}
+
+ protected:
+  void before_sim(xrh_fixed* f, uint64_t gen) override;
+  void after_sim() override;
 };
 
 class ImgSimMetric2DPatchGradNCCHIP : public ImgSimMetric2DHIP,
@@ -433,6 +508,18 @@ class ImgSimMetric2DPatchGradNCCHIP : public ImgSimMetric2DHIP,
   }
   size_type smooth_img_before_sobel_kernel_radius() const override { return (size_type)params_.smooth_ksize; }
   void set_smooth_img_before_sobel_kernel_radius(size_type r) override { params_.smooth_ksize = (int32_t)r; }
+  // ImgSimMetric2DPatchGradNCCCPU (xregImgSimMetric2DPatchGradNCCCPU.h:62-64, :100): x and y use
+  // the same patches; separate draws are not implemented and are refused at compute when
+  // patches are drawn at random or listed by the caller
+  bool enforce_same_patches_in_both_x_and_y() const { return same_xy_; }
+  void set_enforce_same_patches_in_both_x_and_y(bool b) { same_xy_ = b; }
+
+ protected:
+  void before_sim(xrh_fixed* f, uint64_t gen) override;
+  void after_sim() override;
+
+ private:
+  bool same_xy_ = true;
 };
 
 // ImgSimMetric2DCombineMean (lib/regi/sim_metrics_2d/xregImgSimMetric2DCombine.cpp:79-98)

@@ -22,12 +22,15 @@ struct FixedDev {
   const float* pf;           // patch metrics: fixed image (or fixed gx)
   const float* pfy;          // patch-gradient: fixed gy
   const float* pmean;        // per-patch fixed mean
-  const float* pden;         // per-patch fixed std * N
-  const float* pmeany;
+  const float* pden;         // per-patch fixed std * N, divided by the patch's coefficient c
+  const float* pmeany;       //   (c = 0: +inf, the patch adds 1 to the sum)
   const float* pdeny;
   const double* pg;          // per-patch sum_U(f) - |U| mean_f over the used pixels U
   const double* pgy;         //   (all pixels unmasked: the float mean's residue)
-  const float* pw;           // masked patch metrics: normalised per-patch weights, else null
+  // The patch kernels sum 1 - X_p / den_p over every patch of the grid, which with
+  // den_p = std_p N / c_p is sum_p (1 - c_p X'_p); adding patch_offset = sum_p c_p - P gives
+  // the reference's sum_p c_p s_p, then divided by patch_norm.
+  double patch_offset;
   double patch_norm;         // divisor of the patch-score sum (patch count, or weight sum)
 };
 
@@ -67,4 +70,11 @@ struct xrh_fixed {
   xrh::FixedDev host{};          // host copy with device pointers
   xrh::FixedDev* dev = nullptr;  // device copy
   std::vector<void*> buffers;    // device allocations owned by this object
+  // patch metrics: the weights the image was created with (mask fractions, normalised; or
+  // unit), the weights last set, the unscaled per-patch denominators (host copies, one per
+  // direction) and their device tables
+  std::vector<float> create_weights, weights;
+  std::vector<float> base_den[2];
+  const float* base_den_dev[2] = {nullptr, nullptr};
+  float* scaled_den_dev[2] = {nullptr, nullptr};  // allocated on the first weighted set-up
 };

@@ -328,9 +328,10 @@ __host__ __device__ inline int patch_lds_edge(int stride, int radius) { return (
 // over the pixels U the correlation uses (all, or the unmasked ones), SfU = sum m f and
 // S1U = sum m. mean = S1 / N, var = (S2 - mean S1) / (N - 1), sd = max(1e-6, sqrt(var)), and
 // sum_U (m - mean)(f - mean_f) = SfU - mean_f S1U - mean G with G = sum_U f - |U| mean_f
-// (fixed, per patch). The score is w (1 - cross / (sd * sd_f * N)); with a mask, w is the
-// patch's normalised weight and patches with |w| <= 1e-6 contribute 0 (:212), without one
-// w = 1 and S1U = S1, SfU = Sf. A 16 x 16 block of patches stages its moving and fixed
+// (fixed, per patch). The kernel adds 1 - cross / (sd * den) per patch, den = sd_f N / c with
+// c the patch's coefficient in the image score (its weight, 1, or 0 for a skipped patch,
+// :212, 252-253; den = +inf then); the finish adds sum c - P. Without a mask S1U = S1 and
+// SfU = Sf. A 16 x 16 block of patches stages its moving and fixed
 // (masked) window in LDS as doubles, forms the horizontal d-wide sums of every window row for
 // its 16 patch columns, then each lane sums d of those vertically: about 5 d + 3 d operations
 // per patch instead of 5 d^2.
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
   const int px0 = (tile % TX) * kT, py0 = (tile / TX) * kT;  // first patch index
   const int X0 = px0 * s, Y0 = py0 * s;                                  // first image pixel
   const int W = patch_lds_edge(s, r);
-  const bool masked = fx.pw != nullptr;
+  const bool masked = fx.mask != nullptr;
   double* lm = patch_lds;
   double* lf = lm + W * W;
   double* h1 = lf + W * W;
@@ -411,14 +412,11 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
       }
       if (!masked) SU = S1;
       const int64_t p = (int64_t)pj * ppr + pi;
-      const double w = masked ? (double)fx.pw[p] : 1.0;
-      if (!masked || fabs(w) > 1.0e-6) {
-        const double mean = S1 / N;
-        const double var = fmax(S2 - mean * S1, 0.0) / (N - 1.0);
-        const double sd = fmax((double)1.0e-6f, sqrt(var));
-        const double cross = (Sf - (double)pmu[p] * SU) - mean * pg[p];
-        acc[dir] = w * (1.0 - (cross / sd) / (double)pden[p]);
-      }
+      const double mean = S1 / N;
+      const double var = fmax(S2 - mean * S1, 0.0) / (N - 1.0);
+      const double sd = fmax((double)1.0e-6f, sqrt(var));
+      const double cross = (Sf - (double)pmu[p] * SU) - mean * pg[p];
+      acc[dir] = 1.0 - (cross / sd) / (double)pden[p];
     }
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + tile) * 2);
@@ -762,11 +760,11 @@ __global__ void finish_kernel(const SimItem* __restrict__ items, const double* _
       break;
     }
     case XRH_SIM_PATCH_NCC:
-      sim = acc[0] / fx.patch_norm;
+      sim = (acc[0] + fx.patch_offset) / fx.patch_norm;
       break;
     default: {
-      const float sx = (float)(acc[0] / fx.patch_norm);
-      const float sy = (float)(acc[1] / fx.patch_norm);
+      const float sx = (float)((acc[0] + fx.patch_offset) / fx.patch_norm);
+      const float sy = (float)((acc[1] + fx.patch_offset) / fx.patch_norm);
       sim = 0.5 * (double)(sx + sy);
       break;
     }
@@ -792,7 +790,8 @@ int sim_group_key(const FixedDev& f)
   int d = 0;
   // (the specialised kernel addresses images and patch tables with 32-bit byte offsets)
   const bool small = (int64_t)f.rows * f.cols * 4 < (int64_t)INT32_MAX && f.num_patches * 8 < (int64_t)INT32_MAX;
-  if ((f.metric == XRH_SIM_PATCH_NCC || f.metric == XRH_SIM_PATCH_GRAD_NCC) && !f.pw && f.patch_stride == 1 && small) {
+  if ((f.metric == XRH_SIM_PATCH_NCC || f.metric == XRH_SIM_PATCH_GRAD_NCC) && !f.mask && f.patch_stride == 1 &&
+      small) {
     const int dd = 2 * f.patch_radius + 1;
     if (dd == 7 || dd == 9 || dd == 11 || dd == 13 || dd == 15) d = dd;
   }
@@ -818,7 +817,7 @@ int sim_patch_lds_bytes(const FixedDev& f)
     // patch_kernel_d: float window pair + three double row-sum arrays
     return 2 * W * W * (int)sizeof(float) + 3 * W * kT * (int)sizeof(double);
   }
-  return (f.pw ? (3 * W * W + 4 * W * kT) : (2 * W * W + 3 * W * kT)) * (int)sizeof(double);
+  return (f.mask ? (3 * W * W + 4 * W * kT) : (2 * W * W + 3 * W * kT)) * (int)sizeof(double);
 }
 
 int sim_tiles(const FixedDev& f)
@@ -944,6 +943,74 @@ static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, floa
   *g_out = g;
 }
 
+// Coefficient of each patch's score in the image score and the divisor of their sum, after
+// ImgSimMetric2DPatchNCCCPU::compute (xregImgSimMetric2DPatchNCCCPU.cpp:186-287): every use of
+// patch p in the index list adds (weight_in_combine ? w_p : 1) to c_p, except that with
+// weight_in_combine a patch with |w_p| <= 1e-6 is skipped (:212); the sum is divided by the
+// number of uses (compute_mean), else by the float sum of the used patches' weights
+// (weight_in_combine), else by 1. The coefficients are folded into the per-patch denominators
+// (den / c_p, +inf for c_p = 0) and the offset sum c - P, so the patch kernels are the same for
+// every weighting. `upload` = false only at creation, before the device copy exists.
+static void patch_coefficients(xrh_fixed* f, const float* w, const int64_t* inds, int64_t n_inds, bool wic,
+                               bool mean, bool upload)
+{
+  FixedDev& h = f->host;
+  const int64_t np = h.num_patches;
+  std::vector<float> wt = w ? std::vector<float>(w, w + np) : f->create_weights;
+  for (float v : wt) XRH_REQUIRE(std::isfinite(v), "patch weights must be finite");
+  const int64_t nu = inds ? n_inds : np;
+  XRH_REQUIRE(nu >= 1, "need at least one patch index");
+  std::vector<double> coef((size_t)np, 0.0);
+  float wsum = 0.0f;
+  for (int64_t k = 0; k < nu; ++k) {
+    const int64_t p = inds ? inds[k] : k;
+    XRH_REQUIRE(p >= 0 && p < np, "patch index out of range");
+    const float wp = wt[(size_t)p];
+    wsum += wp;
+    if (!wic) {
+      coef[(size_t)p] += 1.0;
+    } else if (std::fabs(wp) > 1.0e-6f) {
+      coef[(size_t)p] += (double)wp;
+    }
+  }
+  double norm = 1.0;
+  if (mean) {
+    norm = (double)(float)nu;
+  } else if (wic) {
+    XRH_REQUIRE(wsum != 0.0f, "the weights of the patches used sum to zero");
+    norm = (double)wsum;
+  }
+  const bool unit = std::all_of(coef.begin(), coef.end(), [](double c) { return c == 1.0; });
+  const bool grad = h.metric == XRH_SIM_PATCH_GRAD_NCC;
+  hipStream_t s = f->ctx->stream;
+  if (upload) XRH_HIP_CHECK(hipStreamSynchronize(s));  // work queued earlier reads the old tables
+  double csum = 0.0;
+  for (double c : coef) csum += c;
+  const float* den[2] = {f->base_den_dev[0], f->base_den_dev[1]};
+  if (!unit) {
+    std::vector<float> scaled((size_t)np);
+    for (int k = 0; k < (grad ? 2 : 1); ++k) {
+      if (!f->scaled_den_dev[k]) f->scaled_den_dev[k] = fixed_buf(f, (uint64_t)np);
+      for (int64_t p = 0; p < np; ++p) {
+        const double c = coef[(size_t)p];
+        scaled[(size_t)p] = (c == 0.0) ? INFINITY : (float)((double)f->base_den[k][(size_t)p] / c);
+      }
+      XRH_HIP_CHECK(hipMemcpyAsync(f->scaled_den_dev[k], scaled.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
+      XRH_HIP_CHECK(hipStreamSynchronize(s));
+      den[k] = f->scaled_den_dev[k];
+    }
+  }
+  h.pden = den[0];
+  if (grad) h.pdeny = den[1];
+  h.patch_offset = unit ? 0.0 : csum - (double)np;
+  h.patch_norm = norm;
+  f->weights = std::move(wt);
+  if (upload) {
+    XRH_HIP_CHECK(hipMemcpyAsync(f->dev, &h, sizeof(FixedDev), hipMemcpyHostToDevice, s));
+    XRH_HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
 static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, int32_t cols,
                          const xrh_sim_params* p, const uint8_t* mask_host, xrh_fixed** out)
 {
@@ -1017,12 +1084,13 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
       h.count = n;
       h.pf = gx;
       h.patch_norm = (double)h.num_patches;
+      h.patch_offset = 0.0;
+      f->create_weights.assign((size_t)h.num_patches, 1.0f);
       if (mask_host) {
         // ImgSimMetric2DPatchCommon::compute_weights with a mask and the default options
         // (xregImgSimMetric2DPatchCommon.cpp:306-400): unmasked fraction of each patch,
-        // normalised to sum to one, in float; the score sum is divided by the weights' sum
-        // (xregImgSimMetric2DPatchNCCCPU.cpp:272-285)
-        std::vector<float> w((size_t)h.num_patches);
+        // normalised to sum to one, in float
+        std::vector<float>& w = f->create_weights;
         float wsum = 0.0f;
         for (int64_t q = 0; q < h.num_patches; ++q) {
           const int r0 = (int)(q / h.ppr) * h.patch_stride, c0 = (int)(q % h.ppr) * h.patch_stride;
@@ -1033,16 +1101,7 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
         }
         for (float v : w) wsum += v;
         XRH_REQUIRE(wsum > 0.0f, "the mask leaves no unmasked pixel in any patch");
-        float tot = 0.0f;
-        for (float& v : w) {
-          v /= wsum;
-          tot += v;
-        }
-        float* wd = fixed_buf(f, (uint64_t)h.num_patches);
-        XRH_HIP_CHECK(hipMemcpyAsync(wd, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, s));
-        XRH_HIP_CHECK(hipStreamSynchronize(s));
-        h.pw = wd;
-        h.patch_norm = (double)tot;
+        for (float& v : w) v /= wsum;
       }
       float *mu = nullptr, *den = nullptr;
       double* g = nullptr;
@@ -1050,13 +1109,25 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
       h.pmean = mu;
       h.pden = den;
       h.pg = g;
+      f->base_den_dev[0] = den;
       if (grad) {
         h.pfy = gy;
         patch_prepare(f, gy, &mu, &den, &g);
         h.pmeany = mu;
         h.pdeny = den;
         h.pgy = g;
+        f->base_den_dev[1] = den;
       }
+      for (int k = 0; k < (grad ? 2 : 1); ++k) {
+        f->base_den[k].resize((size_t)h.num_patches);
+        XRH_HIP_CHECK(hipMemcpyAsync(f->base_den[k].data(), f->base_den_dev[k], sizeof(float) * h.num_patches,
+                                     hipMemcpyDeviceToHost, s));
+      }
+      XRH_HIP_CHECK(hipStreamSynchronize(s));
+      f->weights = f->create_weights;
+      // the default combine: weighted by the patch weights, divided by their sum
+      // (xregImgSimMetric2DPatchNCCCPU.cpp:252-253, 272-285); unit weights need no scaling
+      if (mask_host) patch_coefficients(f, nullptr, nullptr, 0, true, false, false);
     }
     if (patch) {
       int max_lds = 0;
@@ -1162,15 +1233,18 @@ xrh_status xrh_fixed_patches(const xrh_fixed* fixed, int64_t capacity, int32_t* 
       if (stop_row) stop_row[q] = cr + r;
       if (stop_col) stop_col[q] = cc + r;
     }
-    if (weight) {
-      if (h.pw) {
-        XRH_HIP_CHECK(hipMemcpyAsync(weight, h.pw, sizeof(float) * h.num_patches, hipMemcpyDeviceToHost,
-                                     fixed->ctx->stream));
-        XRH_HIP_CHECK(hipStreamSynchronize(fixed->ctx->stream));
-      } else {
-        for (int64_t q = 0; q < h.num_patches; ++q) weight[q] = 1.0f;
-      }
-    }
+    if (weight) std::copy(fixed->weights.begin(), fixed->weights.end(), weight);
+  });
+}
+
+xrh_status xrh_fixed_set_patch_weights(xrh_fixed* fixed, const float* weights, const int64_t* patch_inds,
+                                       int64_t num_inds, int32_t weight_in_combine, int32_t compute_mean)
+{
+  return guarded([&] {
+    XRH_REQUIRE(fixed, "null argument");
+    const FixedDev& h = fixed->host;
+    XRH_REQUIRE(h.metric == XRH_SIM_PATCH_NCC || h.metric == XRH_SIM_PATCH_GRAD_NCC, "not a patch metric");
+    patch_coefficients(fixed, weights, patch_inds, num_inds, weight_in_combine != 0, compute_mean != 0, true);
   });
 }
 
