@@ -225,32 +225,49 @@ __device__ inline float sample(VolView& v, float x, float y, float z)
 // that lie in bricks whose one-voxel-dilated trilinear support holds only zero voxels, so
 // their interpolated value is exactly 0 and leaving them out of a SUM changes nothing. At a
 // brick at Chebyshev distance d >= 1 from the nearest occupied brick, the cube of bricks
-// within d-1 is empty: the walk jumps to the last sample inside that cube (at least one
-// sample per iteration, so at most |end - i| + 1 iterations). Positions here are start +
-// i*step in float; they differ from the accumulated ones by far less than the one-voxel
-// dilation. Returns the first sample (in walk order) that may be non-zero.
+// within d-1 is empty: every sample up to the cube's exit face is skipped and the walk
+// continues with the first sample past it (at least one sample per iteration, so at most
+// |end - i| + 1 iterations). The walk runs in brick units (positions scaled by 2^-shift,
+// exact) with branch-free exit faces: b + d moving up an axis, b + 1 - d moving down, none
+// along an axis the ray does not move on. Its positions, start + i*step in float, differ from
+// the sample loop's by far less than the one-voxel dilation. Returns the first sample (in walk
+// order) that may be non-zero.
 __device__ inline int skip_empty(const DrrItem& it, const Ray& r, int i, int end, int dir, float org)
 {
-  const float B = (float)(1 << it.skip_shift);
+  const float B = (float)(1 << it.skip_shift), invB = __builtin_ldexpf(1.0f, -it.skip_shift);
+  // ray in brick units
+  const float bx0 = (r.sx - org) * invB, by0 = (r.sy - org) * invB, bz0 = (r.sz - org) * invB;
+  const float dbx = r.dx * invB, dby = r.dy * invB, dbz = r.dz * invB;
+  // per axis: exit face b + o + s (d - 1), samples to reach it (face - p) * k (+ h: infinite
+  // when the walk does not move along the axis)
   const float vx = dir > 0 ? r.dx : -r.dx, vy = dir > 0 ? r.dy : -r.dy, vz = dir > 0 ? r.dz : -r.dz;
-  const float rx = __builtin_amdgcn_rcpf(vx), ry = __builtin_amdgcn_rcpf(vy), rz = __builtin_amdgcn_rcpf(vz);
+  const float sx = vx > 0.0f ? 1.0f : -1.0f, sy = vy > 0.0f ? 1.0f : -1.0f, sz = vz > 0.0f ? 1.0f : -1.0f;
+  const float ox = vx > 0.0f ? 1.0f : 0.0f, oy = vy > 0.0f ? 1.0f : 0.0f, oz = vz > 0.0f ? 1.0f : 0.0f;
+  const float kx = vx != 0.0f ? B * __builtin_amdgcn_rcpf(vx) : 0.0f;
+  const float ky = vy != 0.0f ? B * __builtin_amdgcn_rcpf(vy) : 0.0f;
+  const float kz = vz != 0.0f ? B * __builtin_amdgcn_rcpf(vz) : 0.0f;
+  const float hx = vx != 0.0f ? 0.0f : 3.0e38f, hy = vy != 0.0f ? 0.0f : 3.0e38f, hz = vz != 0.0f ? 0.0f : 3.0e38f;
+  const float gx1 = (float)(it.skip_gx - 1), gy1 = (float)(it.skip_gy - 1), gz1 = (float)(it.skip_gz - 1);
+  // the map is below 2^24 bytes and gy * gz below 2^24 (checked when it is built), so the
+  // 24-bit multiply-adds below are exact and a stray index reads 0 from the buffer range check
+  const auto map = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(it.skip), (short)0,
+                                                     it.skip_gx * it.skip_gy * it.skip_gz, 0x00020000);
   while (dir > 0 ? (i <= end) : (i >= end)) {
     const float fi = (float)i;
-    const float px = fmaf(fi, r.dx, r.sx) - org, py = fmaf(fi, r.dy, r.sy) - org, pz = fmaf(fi, r.dz, r.sz) - org;
-    const int bx = min(max(cvt_floor(px) >> it.skip_shift, 0), it.skip_gx - 1);
-    const int by = min(max(cvt_floor(py) >> it.skip_shift, 0), it.skip_gy - 1);
-    const int bz = min(max(cvt_floor(pz) >> it.skip_shift, 0), it.skip_gz - 1);
-    const int d = it.skip[((int64_t)bz * it.skip_gy + by) * it.skip_gx + bx];
+    const float px = fmaf(fi, dbx, bx0), py = fmaf(fi, dby, by0), pz = fmaf(fi, dbz, bz0);
+    const float fx = __builtin_amdgcn_fmed3f(floorf(px), 0.0f, gx1);
+    const float fy = __builtin_amdgcn_fmed3f(floorf(py), 0.0f, gy1);
+    const float fz = __builtin_amdgcn_fmed3f(floorf(pz), 0.0f, gz1);
+    const uint32_t e = mad_u24(mad_u24((uint32_t)fz, (uint32_t)it.skip_gy, (uint32_t)fy), (uint32_t)it.skip_gx,
+                               (uint32_t)fx);
+    const int d = __builtin_amdgcn_raw_buffer_load_b8(map, (int)e, 0, 0);
     if (d == 0) break;
     const float dm1 = (float)(d - 1);
-    float s = 3.0e38f;
-    if (vx > 0.0f) s = fminf(s, (((float)bx + dm1 + 1.0f) * B - px) * rx);
-    else if (vx < 0.0f) s = fminf(s, (((float)bx - dm1) * B - px) * rx);
-    if (vy > 0.0f) s = fminf(s, (((float)by + dm1 + 1.0f) * B - py) * ry);
-    else if (vy < 0.0f) s = fminf(s, (((float)by - dm1) * B - py) * ry);
-    if (vz > 0.0f) s = fminf(s, (((float)bz + dm1 + 1.0f) * B - pz) * rz);
-    else if (vz < 0.0f) s = fminf(s, (((float)bz - dm1) * B - pz) * rz);
-    const int jump = (s >= 1.0f) ? (int)fminf(s, 1.0e9f) : 1;
+    const float tx = fmaf((fmaf(sx, dm1, ox) + fx) - px, kx, hx);
+    const float ty = fmaf((fmaf(sy, dm1, oy) + fy) - py, ky, hy);
+    const float tz = fmaf((fmaf(sz, dm1, oz) + fz) - pz, kz, hz);
+    const float s = fminf(fminf(tx, ty), tz);
+    const int jump = (s >= 1.0f) ? (int)fminf(s, 1.0e9f) + 1 : 1;
     i += dir * jump;
   }
   return i;

@@ -445,11 +445,14 @@ static void volume_create(xrh_ctx* ctx, const float* vox, const int64_t size[3],
     hipLaunchKernelGGL(build_layout_kernel, dim3((unsigned)blocks), dim3(threads), 0, ctx->stream,
                        (const float*)raw, (int)X, (int)Y, (int)Z, v->data, storage, layout);
     XRH_HIP_CHECK(hipGetLastError());
-    if (skip_enabled()) {
-      const int B = 1 << kSkipShift;
-      skip_dims[0] = (int)((X + B - 1) / B);
-      skip_dims[1] = (int)((Y + B - 1) / B);
-      skip_dims[2] = (int)((Z + B - 1) / B);
+    const int SB = 1 << kSkipShift;
+    const int64_t sgx = (X + SB - 1) / SB, sgy = (Y + SB - 1) / SB, sgz = (Z + SB - 1) / SB;
+    // the walk indexes the map with 24-bit multiply-adds: larger maps go without skipping
+    if (skip_enabled() && sgx * sgy * sgz < (int64_t(1) << 24) && sgy * sgz < (int64_t(1) << 24)) {
+      const int B = SB;
+      skip_dims[0] = (int)sgx;
+      skip_dims[1] = (int)sgy;
+      skip_dims[2] = (int)sgz;
       const int64_t nb = (int64_t)skip_dims[0] * skip_dims[1] * skip_dims[2];
       v->skip = ctx_alloc(ctx, (uint64_t)nb);
       hipLaunchKernelGGL(brick_occupancy_kernel, dim3((unsigned)((nb + 127) / 128)), dim3(128), 0, ctx->stream,
