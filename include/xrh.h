@@ -302,20 +302,33 @@ xrh_status xrh_fixed_patches(const xrh_fixed* fixed, int64_t capacity, int32_t* 
  *                patch centres, mask fractions, normalised or not: the caller's
  *                compute_weights, xregImgSimMetric2DPatchCommon.cpp:306-400); NULL = the
  *                weights the image was created with (unit, or the mask's);
+ *   weights_y    Patch-Grad-NCC only: the y direction's weights when they differ from the x
+ *                direction's (its sub-metrics weighted by their own gradient image's patch
+ *                variances, xregImgSimMetric2DPatchGradNCCCPU.cpp:56-79); NULL = the same.
+ *                Needs non-NULL weights;
  *   patch_inds   the patches used, in order, repeats allowed (patch_inds_to_use_: a random
  *                draw, set_patches_to_use); NULL = every patch (num_inds ignored);
  *   weight_in_combine, compute_mean   weight_patch_sims_in_combine_ and
  *                compute_mean_of_patch_sims_.
  * Score = sum over the used indices of (weight_in_combine ? w_p : 1) s_p, patches with
  * |w_p| <= 1e-6 skipped when weight_in_combine, divided by num_inds if compute_mean, else by
- * the float sum of the used weights if weight_in_combine (which must not be zero), else by 1.
- * Both gradient directions of Patch-Grad-NCC use the same patches and weights (the
- * reference's default enforce_same_patches_in_both_x_and_y_). Index checks and the divisor
- * check fail with nothing changed. Synchronises the context's stream first: work queued
- * before the call, batched and graph-replayed launches included, uses the previous
- * combine; every later launch that reads this fixed image uses the new one. */
-xrh_status xrh_fixed_set_patch_weights(xrh_fixed* fixed, const float* weights, const int64_t* patch_inds,
-                                       int64_t num_inds, int32_t weight_in_combine, int32_t compute_mean);
+ * the float sum of the used weights if weight_in_combine (which must not be zero), else by 1;
+ * Patch-Grad-NCC averages its two directions' scores, each with its own weights and divisor.
+ * Both directions use the same patches (the reference's default
+ * enforce_same_patches_in_both_x_and_y_). Index checks and the divisor check fail with
+ * nothing changed. Synchronises the context's stream first: work queued before the call,
+ * batched and graph-replayed launches included, uses the previous combine; every later launch
+ * that reads this fixed image uses the new one. */
+xrh_status xrh_fixed_set_patch_weights(xrh_fixed* fixed, const float* weights, const float* weights_y,
+                                       const int64_t* patch_inds, int64_t num_inds, int32_t weight_in_combine,
+                                       int32_t compute_mean);
+
+/* A patch metric's fixed image as the device holds it (rows x cols floats into host_out):
+ * which = 0 the image the first direction correlates with (Patch-NCC: the fixed image;
+ * Patch-Grad-NCC: its x gradient after the optional blur, ImgSimMetric2DGradImgCPU's
+ * fixed_grad_img_x_, xregImgSimMetric2DGradImgCPU.cpp:54-65), which = 1 the y gradient
+ * (Patch-Grad-NCC only). For weights derived from it (patch variances). Synchronous. */
+xrh_status xrh_fixed_image(const xrh_fixed* fixed, int32_t which, float* host_out);
 
 /* sim_host_out: num_mov floats (synchronises); sim_dev_out: optional device destination
  * (stream-ordered). At least one must be non-NULL. The moving images are not modified

@@ -296,6 +296,15 @@ def patch_weights_f32(shape, radius=5, stride=1, wgt_img=None, mask=None, use_ma
     return out
 
 
+def patch_variance_image_f32(img, radius=5, stride=1, mask=None):
+    """Fixed-image patch variances at the patch centres (use_fixed_img_patch_variances_as_wgts)."""
+    im = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros_like(im)
+    mk, mp = _mask_ptr(mask, im.shape)
+    lib().orc_patch_variance_image_f32(_f(im), mp, im.shape[0], im.shape[1], radius, stride, _f(out))
+    return out
+
+
 def _inds(inds):
     if inds is None:
         return None, None, 0

@@ -1499,6 +1499,47 @@ void orc_patch_weights_f32(const float* wgt_img, const uint8_t* mask, int32_t ro
   }
 }
 
+void orc_patch_variance_image_f32(const float* img, const uint8_t* mask, int32_t rows, int32_t cols,
+                                  int32_t patch_radius, int32_t patch_stride, float* out)
+{
+  /* ImgSimMetric2DPatchNCCCPU::process_mask with use_fixed_img_patch_variances_as_wgts_
+   * (xregImgSimMetric2DPatchNCCCPU.cpp:331-440): ComputePatchMeanStdDev (:558-619) over the
+   * unmasked pixels of the patch when there is a mask, std clamped at 1e-6, squared, at the
+   * patch centre; zero elsewhere */
+  const int32_t diam = 2 * patch_radius + 1;
+  const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+  for (int64_t i = 0; i < (int64_t)rows * cols; ++i) out[i] = 0.0f;
+  for (int64_t p = 0; p < (int64_t)ppr * ppc; ++p) {
+    const int32_t r0 = (int32_t)(p / ppr) * patch_stride, c0 = (int32_t)(p % ppr) * patch_stride;
+    float mean = 0.0f;
+    int64_t n = 0;
+    for (int32_t y = 0; y < diam; ++y)
+      for (int32_t x = 0; x < diam; ++x) {
+        const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+        if (!mask || mask[o]) {
+          mean += img[o];
+          ++n;
+        }
+      }
+    float var = 0.0f;
+    if (n > 1) {
+      mean /= (float)n;
+      for (int32_t y = 0; y < diam; ++y)
+        for (int32_t x = 0; x < diam; ++x) {
+          const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+          if (!mask || mask[o]) {
+            const float t = img[o] - mean;
+            var += t * t;
+          }
+        }
+      var /= (float)n - 1.0f;
+    }
+    const float sd = sqrtf(var) > 1.0e-6f ? sqrtf(var) : 1.0e-6f;
+    out[(int64_t)(r0 + patch_radius) * cols + c0 + patch_radius] = sd * sd;
+  }
+}
+
 void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
                                 int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                                 const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,

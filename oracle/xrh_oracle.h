@@ -174,6 +174,13 @@ void orc_patch_weights_f32(const float* wgt_img, const uint8_t* mask, int32_t ro
                            int32_t patch_radius, int32_t patch_stride, int32_t use_mask_for_weighting,
                            int32_t normalize, float* w_out);
 
+/* The fixed-image patch variances the patch metrics can use as a weight image
+ * (use_fixed_img_patch_variances_as_wgts, xregImgSimMetric2DPatchNCCCPU.cpp:331-440): per patch,
+ * the variance over its pixels (its unmasked pixels with a mask; n - 1 denominator, std
+ * clamped at 1e-6), written at the patch centre; zero elsewhere. */
+void orc_patch_variance_image_f32(const float* img, const uint8_t* mask, int32_t rows, int32_t cols,
+                                  int32_t patch_radius, int32_t patch_stride, float* out);
+
 /* Patch NCC / Patch-Grad-NCC with the reference's general combine
  * (xregImgSimMetric2DPatchNCCCPU.cpp:186-287): per-patch weights wgt (NULL = 1), the patch
  * indices used in order (inds, num_inds; NULL = all), weight_patch_sims_in_combine_ and

@@ -342,6 +342,27 @@ int main(int argc, char** argv)
         pc->set_use_mask_for_patch_weighting(true);
         ps->compute();
         rec.put("popt_default_sims", ps->sim_vals());
+        // the fixed image's patch variances (over unmasked pixels) as the weight image; for
+        // Patch-Grad-NCC also each direction's own gradient-image variances
+        auto* pn = dynamic_cast<ImgSimMetric2DPatchNCCHIP*>(ps.get());
+        auto* pg = dynamic_cast<ImgSimMetric2DPatchGradNCCHIP*>(ps.get());
+        uint64_t mov_var_refused = 0;
+        try {
+          if (pn) pn->set_use_mov_img_patch_variances_as_wgts(true);
+          if (pg) pg->set_use_mov_img_patch_variances_as_wgts(true);
+        } catch (const ImgSimMetric2D::UnsupportedOperationException&) {
+          ++mov_var_refused;
+        }
+        rec.put_u64("popt_mov_var_refused", mov_var_refused);
+        if (pn) pn->set_use_fixed_img_patch_variances_as_wgts(true);
+        if (pg) pg->set_use_fixed_img_patch_variances_as_wgts(true);
+        ps->compute();
+        rec.put("popt_fvar_mask_sims", ps->sim_vals());
+        if (pg) {
+          pg->set_use_variances_in_grad_imgs_as_wgts(true);
+          ps->compute();
+          rec.put("popt_gvar_mask_sims", ps->sim_vals());
+        }
       }
     }
     rc->use_external_host_pixel_buf(nullptr);

@@ -166,3 +166,25 @@ def _check_patch_options(oracle, rec, metric, smooth, fixed, drr, mask):
     check("popt_nomaskwgt_sims", mask=mask)
     assert np.array_equal(np.frombuffer(rec["popt_default_sims"], np.float32),
                           np.frombuffer(rec["mask_sims"], np.float32))
+    # fixed-image patch variances as weights, over the unmasked pixels, zero at masked centres
+    assert int(np.frombuffer(rec["popt_mov_var_refused"], np.uint64)[0]) == 1
+    w_var = oracle.patch_weights_f32(fixed.shape, 5, 1, wgt_img=oracle.patch_variance_image_f32(fixed, 5, 1, mask),
+                                     mask=mask)
+    check("popt_fvar_mask_sims", weights=w_var, mask=mask)
+    if metric == 3:
+        # each direction weighted by its own fixed gradient image's variances
+        fx_, fy_ = oracle.sobel_f32(fixed, smooth)
+        wx = oracle.patch_weights_f32(fixed.shape, 5, 1, wgt_img=oracle.patch_variance_image_f32(fx_, 5, 1, mask),
+                                      mask=mask)
+        wy = oracle.patch_weights_f32(fixed.shape, 5, 1, wgt_img=oracle.patch_variance_image_f32(fy_, 5, 1, mask),
+                                      mask=mask)
+        got = np.frombuffer(rec["popt_gvar_mask_sims"], np.float32)
+        ref = np.empty(drr.shape[0], np.float32)
+        for m in range(drr.shape[0]):
+            mx, my = oracle.sobel_f32(drr[m], smooth)
+            sx = oracle.patch_sim_weighted_f32(2, fx_, mx[None], weights=wx, mask=mask)[0]
+            sy = oracle.patch_sim_weighted_f32(2, fy_, my[None], weights=wy, mask=mask)[0]
+            ref[m] = np.float32(0.5 * float(np.float32(sx + sy)))
+        print(f"  popt_gvar_mask_sims: max |gpu-cpu| {np.abs(got - ref).max():.3e}", flush=True)
+        assert np.abs(got - ref).max() <= 2 * SIM_ABS_TOL
+        assert np.abs(got - np.frombuffer(rec["popt_fvar_mask_sims"], np.float32)).max() > 1e-5

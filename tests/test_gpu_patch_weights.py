@@ -181,3 +181,51 @@ def test_batch_and_graph_replays_follow_the_weights(ctx, oracle):
     assert np.array_equal(b.eval(iv, w.poses), first)
     for o in (b, fx, vol):
         o.close()
+
+
+def test_per_direction_weights_and_fixed_gradients(ctx, oracle, images):
+    """Patch-Grad-NCC with separate x and y weights (each direction's own fixed gradient
+    variances, as use_variances_in_grad_imgs_as_wgts) against the oracle composed per
+    direction; the fixed gradient images the device holds equal the oracle's Sobel."""
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    print("case: per-direction patch weights", flush=True)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    for radius, stride in ((5, 1), (4, 2)):
+        f = X.Fixed(ctx, fixed, 3, smooth_ksize=5, patch_radius=radius, patch_stride=stride)
+        gx, gy = f.image(0), f.image(1)
+        rx, ry = oracle.sobel_f32(fixed, 5)
+        assert np.abs(gx - rx).max() <= 1e-5 * max(1.0, np.abs(rx).max())
+        assert np.abs(gy - ry).max() <= 1e-5 * max(1.0, np.abs(ry).max())
+        wx = oracle.patch_weights_f32(fixed.shape, radius, stride, wgt_img=oracle.patch_variance_image_f32(
+            gx, radius, stride))
+        wy = oracle.patch_weights_f32(fixed.shape, radius, stride, wgt_img=oracle.patch_variance_image_f32(
+            gy, radius, stride))
+        assert np.abs(wx - wy).max() > 1e-6
+        f.set_patch_weights(weights=wx, weights_y=wy)
+        got = f.compute(buf, mov.shape[0])
+        ref = np.empty(mov.shape[0], np.float32)
+        for m in range(mov.shape[0]):
+            mx, my = oracle.sobel_f32(mov[m], 5)
+            sx = oracle.patch_sim_weighted_f32(2, rx, mx[None], weights=wx, radius=radius, stride=stride)[0]
+            sy = oracle.patch_sim_weighted_f32(2, ry, my[None], weights=wy, radius=radius, stride=stride)[0]
+            ref[m] = np.float32(0.5 * float(np.float32(sx + sy)))
+        print(f"  r={radius} s={stride}: max |gpu-cpu| {np.abs(got - ref).max():.3e}", flush=True)
+        assert np.abs(got - ref).max() <= 2 * SIM_ABS_TOL
+        # the same weights in both directions equal the single-weight call bit for bit
+        f.set_patch_weights(weights=wx, weights_y=wx)
+        a = f.compute(buf, mov.shape[0])
+        f.set_patch_weights(weights=wx)
+        assert np.array_equal(a, f.compute(buf, mov.shape[0]))
+        f.close()
+    g = X.Fixed(ctx, fixed, 2, patch_radius=5)
+    n_p = g.patches()["weights"].size
+    with pytest.raises(X.XrhError):
+        g.set_patch_weights(weights=np.ones(n_p, np.float32), weights_y=np.ones(n_p, np.float32))
+    with pytest.raises(X.XrhError):
+        g.image(1)
+    assert np.array_equal(g.image(0), fixed.astype(np.float32))
+    g.close()
+    buf.free()

@@ -186,3 +186,27 @@ def test_patch_weights_from_weight_image_and_mask(oracle):
     # a mask not used for weighting and no image: unit weights
     assert np.array_equal(oracle.patch_weights_f32(shape, r, s, mask=mask, use_mask_for_weighting=False),
                           np.ones(ppr * ppc, np.float32))
+
+
+def test_patch_variance_image_matches_numpy(oracle):
+    """use_fixed_img_patch_variances_as_wgts: the variance of each patch (over its unmasked
+    pixels with a mask; n - 1 denominator, std clamped at 1e-6) at the patch centre."""
+    shape, r, s = (19, 23), 2, 2
+    img = _img(41, shape)
+    img[:6, :9] = 3.0                    # flat patches: std clamped at 1e-6
+    mask = (np.random.default_rng(5).random(shape) > 0.25).astype(np.uint8)
+    for mk in (None, mask):
+        got = oracle.patch_variance_image_f32(img, r, s, mask=mk)
+        want = np.zeros(shape, np.float64)
+        win = sliding_window_view(img.astype(np.float64), (5, 5))[::s, ::s]
+        kw = None if mk is None else sliding_window_view(mk.astype(bool), (5, 5))[::s, ::s]
+        for i in range(win.shape[0]):
+            for j in range(win.shape[1]):
+                v = win[i, j] if kw is None else win[i, j][kw[i, j]]
+                var = v.var(ddof=1) if v.size > 1 else 0.0
+                want[r + i * s, r + j * s] = max(1e-6, np.sqrt(var)) ** 2
+        off = np.ones(shape, bool)
+        off[r:shape[0] - r:s, r:shape[1] - r:s] = False
+        assert np.all(got[off] == 0)
+        np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-13)
+        assert got[r, r] == np.float32(1e-6) ** 2 or mk is not None

@@ -129,7 +129,8 @@ SIGNATURES = {
     "xrh_fixed_create": (_I32, [_P, _PF, _I32, _I32, C.POINTER(SimParams), _PU8, C.POINTER(_P)]),
     "xrh_fixed_destroy": (_I32, [_P]),
     "xrh_fixed_patches": (_I32, [_P, _I64, _PI32, _PI32, _PI32, _PI32, _PF, _PI64]),
-    "xrh_fixed_set_patch_weights": (_I32, [_P, _PF, _PI64, _I64, _I32, _I32]),
+    "xrh_fixed_set_patch_weights": (_I32, [_P, _PF, _PF, _PI64, _I64, _I32, _I32]),
+    "xrh_fixed_image": (_I32, [_P, _I32, _PF]),
     "xrh_sim_compute": (_I32, [_P, _P, _P, _I64, _P, _PF]),
     "xrh_batch_create": (_I32, [_P, C.POINTER(View), _I32, _I64, _I32, C.POINTER(_P)]),
     "xrh_batch_destroy": (_I32, [_P]),
@@ -395,24 +396,38 @@ class Fixed:
         return a
 
     def set_patch_weights(self, weights=None, patch_inds=None, weight_in_combine: bool = True,
-                          compute_mean: bool = False) -> None:
+                          compute_mean: bool = False, weights_y=None) -> None:
         """How a patch metric combines its patch scores (xrh_fixed_set_patch_weights): per-patch
-        weights (None: the creation weights), the patch indices used in order (None: all), and
-        ImgSimMetric2DPatchCommon's weight_patch_sims_in_combine / compute_mean_of_patch_sims."""
-        wp = None
-        if weights is not None:
-            w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
-            n_p = C.c_int64(0)
-            check(lib().xrh_fixed_patches(self.h, 0, None, None, None, None, None, C.byref(n_p)))
-            if w.size != n_p.value:
-                raise ValueError(f"{w.size} weights for {n_p.value} patches")
-            wp = _fp(w)
+        weights (None: the creation weights; weights_y: Patch-Grad-NCC's y direction when it
+        differs), the patch indices used in order (None: all), and ImgSimMetric2DPatchCommon's
+        weight_patch_sims_in_combine / compute_mean_of_patch_sims."""
+        n_p = C.c_int64(0)
+        check(lib().xrh_fixed_patches(self.h, 0, None, None, None, None, None, C.byref(n_p)))
+        keep = []
+
+        def as_ptr(a):
+            if a is None:
+                return None
+            v = np.ascontiguousarray(a, dtype=np.float32).ravel()
+            if v.size != n_p.value:
+                raise ValueError(f"{v.size} weights for {n_p.value} patches")
+            keep.append(v)
+            return _fp(v)
+
+        wp, wyp = as_ptr(weights), as_ptr(weights_y)
         ip, n = None, 0
         if patch_inds is not None:
             ind = np.ascontiguousarray(patch_inds, dtype=np.int64)
             ip, n = ind.ctypes.data_as(_PI64), ind.size
-        check(lib().xrh_fixed_set_patch_weights(self.h, wp, ip, n, int(bool(weight_in_combine)),
+        check(lib().xrh_fixed_set_patch_weights(self.h, wp, wyp, ip, n, int(bool(weight_in_combine)),
                                                 int(bool(compute_mean))))
+
+    def image(self, which: int = 0) -> np.ndarray:
+        """The fixed image as the device holds it for a patch metric (xrh_fixed_image): 0 the
+        image (Patch-NCC) or its x gradient (Patch-Grad-NCC), 1 the y gradient."""
+        out = np.empty(self.shape, np.float32)
+        check(lib().xrh_fixed_image(self.h, int(which), _fp(out)))
+        return out
 
     def close(self) -> None:
         if self.h:

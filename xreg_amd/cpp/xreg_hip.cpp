@@ -650,27 +650,31 @@ void ImgSimMetric2DPatchCommonHIP::set_wgt_img(WgtImgPtr w)
   weights_stale_ = true;
 }
 
-void ImgSimMetric2DPatchCommonHIP::compute_weights(const Image2D& img, const std::vector<unsigned char>* mask)
+void ImgSimMetric2DPatchCommonHIP::set_variance_weighting(int mode)
+{
+  var_mode_ = mode;
+  weights_stale_ = true;
+}
+
+void ImgSimMetric2DPatchCommonHIP::compute_weights(const Image2D& img, const std::vector<unsigned char>* mask,
+                                                   const float* wimg, PatchInfoList& out) const
 {
   // ImgSimMetric2DPatchCommon::compute_weights (xregImgSimMetric2DPatchCommon.cpp:306-400)
   const bool use_mask_wgts = use_mask_wgt_ && mask;
-  const bool use_img_wgts = (bool)wgt_img_;
-  for (PatchInfo& p : patch_infos_) p.weight = 1.0f;
+  const bool use_img_wgts = wimg != nullptr;
+  for (PatchInfo& p : out) p.weight = 1.0f;
   if (!use_img_wgts && !use_mask_wgts) return;
   const size_type cols = img.cols;
   if (use_img_wgts) {
-    if (wgt_img_->rows != img.rows || wgt_img_->cols != img.cols) {
-      throw StringMessageException("weight image and fixed image sizes differ");
-    }
-    for (PatchInfo& p : patch_infos_) {
+    for (PatchInfo& p : out) {
       // centre = start + (stop - start + 1) / 2 (PatchInfo::center_row_col, .cpp:33-37)
       const size_type cr = p.start_row + (p.stop_row - p.start_row + 1) / 2;
       const size_type cc = p.start_col + (p.stop_col - p.start_col + 1) / 2;
-      p.weight = wgt_img_->buf[cr * cols + cc];
+      p.weight = wimg[cr * cols + cc];
       if (use_mask_wgts && !(*mask)[cr * cols + cc]) p.weight = 0.0f;
     }
   } else {
-    for (PatchInfo& p : patch_infos_) {
+    for (PatchInfo& p : out) {
       size_type cnt = 0;
       for (size_type r = p.start_row; r <= p.stop_row; ++r)
         for (size_type c = p.start_col; c <= p.stop_col; ++c) cnt += (*mask)[r * cols + c] ? 1 : 0;
@@ -680,9 +684,44 @@ void ImgSimMetric2DPatchCommonHIP::compute_weights(const Image2D& img, const std
   }
   if (normalize_) {
     float sum = 0.0f;
-    for (const PatchInfo& p : patch_infos_) sum += p.weight;
-    for (PatchInfo& p : patch_infos_) p.weight /= sum;
+    for (const PatchInfo& p : out) sum += p.weight;
+    for (PatchInfo& p : out) p.weight /= sum;
   }
+}
+
+std::vector<float> ImgSimMetric2DPatchCommonHIP::variance_image(const float* img, size_type rows, size_type cols,
+                                                                const std::vector<unsigned char>* mask) const
+{
+  // the fixed patch variances at the patch centres, zero elsewhere: ComputePatchMeanStdDev
+  // (xregImgSimMetric2DPatchNCCCPU.cpp:558-619) over the unmasked pixels when there is a mask,
+  // in float, squared (xregImgSimMetric2DPatchNCCCPU.cpp:419-432)
+  std::vector<float> v(rows * cols, 0.0f);
+  for (const PatchInfo& p : patch_infos_) {
+    float mean = 0.0f;
+    size_type n = 0;
+    for (size_type r = p.start_row; r <= p.stop_row; ++r)
+      for (size_type c = p.start_col; c <= p.stop_col; ++c)
+        if (!mask || (*mask)[r * cols + c]) {
+          mean += img[r * cols + c];
+          ++n;
+        }
+    float var = 0.0f;
+    if (n > 1) {
+      mean /= (float)n;
+      for (size_type r = p.start_row; r <= p.stop_row; ++r)
+        for (size_type c = p.start_col; c <= p.stop_col; ++c)
+          if (!mask || (*mask)[r * cols + c]) {
+            const float t = img[r * cols + c] - mean;
+            var += t * t;
+          }
+      var /= (float)n - 1.0f;
+    }
+    const float sd = std::max(1.0e-6f, std::sqrt(var));
+    const size_type cr = p.start_row + (p.stop_row - p.start_row + 1) / 2;
+    const size_type cc = p.start_col + (p.stop_col - p.start_col + 1) / 2;
+    v[cr * cols + cc] = sd * sd;
+  }
+  return v;
 }
 
 PatchIndexList ImgSimMetric2DPatchCommonHIP::draw_random_patches()
@@ -758,7 +797,26 @@ void ImgSimMetric2DPatchCommonHIP::prepare_patches(xrh_fixed* fixed, uint64_t ge
     static_cast<std::mt19937*>(rng_.get())->seed(rd());
   }
   if (weights_stale_) {
-    compute_weights(img, mask);
+    patch_infos_y_.clear();
+    if (var_mode_ == 0) {
+      if (wgt_img_ && (wgt_img_->rows != img.rows || wgt_img_->cols != img.cols)) {
+        throw StringMessageException("weight image and fixed image sizes differ");
+      }
+      compute_weights(img, mask, wgt_img_ ? wgt_img_->buf.data() : nullptr, patch_infos_);
+    } else if (var_mode_ == 1) {
+      // the reference installs the variances as the weight image (set_wgt_img, :435)
+      const std::vector<float> v = variance_image(img.buf.data(), img.rows, img.cols, mask);
+      compute_weights(img, mask, v.data(), patch_infos_);
+    } else {
+      std::vector<float> g(img.rows * img.cols);
+      check_xrh(xrh_fixed_image(fixed, 0, g.data()));
+      const std::vector<float> vx = variance_image(g.data(), img.rows, img.cols, mask);
+      check_xrh(xrh_fixed_image(fixed, 1, g.data()));
+      const std::vector<float> vy = variance_image(g.data(), img.rows, img.cols, mask);
+      compute_weights(img, mask, vx.data(), patch_infos_);
+      patch_infos_y_ = patch_infos_;
+      compute_weights(img, mask, vy.data(), patch_infos_y_);
+    }
     weights_stale_ = false;
   }
   const size_type np = patch_infos_.size();
@@ -774,41 +832,70 @@ void ImgSimMetric2DPatchCommonHIP::prepare_patches(xrh_fixed* fixed, uint64_t ge
   }
   // the fixed image's creation weights are the reference defaults: unit, or the normalised
   // unmasked fractions of the mask
-  const bool default_weights = !wgt_img_ && (mask ? (use_mask_wgt_ && normalize_) : true);
+  const bool default_weights =
+      var_mode_ == 0 && !wgt_img_ && (mask ? (use_mask_wgt_ && normalize_) : true);
   const bool all = !fixed_inds_ && !choose_rand_;
   const bool dflt = default_weights && all && wic_ && !compute_mean_;
-  std::vector<float> w;
+  std::vector<float> w, wy;
   if (!dflt) {
     w.resize(np);
     for (size_type q = 0; q < np; ++q) w[q] = patch_infos_[q].weight;
+    if (!patch_infos_y_.empty()) {
+      wy.resize(np);
+      for (size_type q = 0; q < np; ++q) wy[q] = patch_infos_y_[q].weight;
+    }
   }
   const PatchIndexList& inds = all ? PatchIndexList() : patch_inds_to_use_;
   const bool same = applied_gen_ == gen && applied_default_ == dflt &&
-                    (dflt || (applied_w_ == w && applied_inds_ == inds && applied_wic_ == wic_ &&
+                    (dflt || (applied_w_ == w && applied_wy_ == wy && applied_inds_ == inds && applied_wic_ == wic_ &&
                               applied_mean_ == compute_mean_));
   if (same) return;
   if (dflt) {
-    check_xrh(xrh_fixed_set_patch_weights(fixed, nullptr, nullptr, 0, 1, 0));
+    check_xrh(xrh_fixed_set_patch_weights(fixed, nullptr, nullptr, nullptr, 0, 1, 0));
   } else {
     std::vector<int64_t> i64(inds.begin(), inds.end());
-    check_xrh(xrh_fixed_set_patch_weights(fixed, w.data(), all ? nullptr : i64.data(), (int64_t)i64.size(),
-                                          wic_ ? 1 : 0, compute_mean_ ? 1 : 0));
+    check_xrh(xrh_fixed_set_patch_weights(fixed, w.data(), wy.empty() ? nullptr : wy.data(),
+                                          all ? nullptr : i64.data(), (int64_t)i64.size(), wic_ ? 1 : 0,
+                                          compute_mean_ ? 1 : 0));
   }
   applied_gen_ = gen;
   applied_default_ = dflt;
   applied_w_ = std::move(w);
+  applied_wy_ = std::move(wy);
   applied_inds_ = inds;
   applied_wic_ = wic_;
   applied_mean_ = compute_mean_;
 }
 
-void ImgSimMetric2DPatchCommonHIP::record_patches(std::shared_ptr<PatchNCCSimAux>& a) const
+void ImgSimMetric2DPatchCommonHIP::record_patches(std::shared_ptr<PatchNCCSimAux>& a, bool y) const
 {
   // ImgSimMetric2DPatchNCCCPU::compute with save_aux_info (xregImgSimMetric2DPatchNCCCPU.cpp:
   // 290-298): every call appends the patch list and the indices used
   if (!a) a = std::make_shared<PatchNCCSimAux>();
-  a->patch_infos_per_compute_call.push_back(patch_infos_);
+  a->patch_infos_per_compute_call.push_back((y && !patch_infos_y_.empty()) ? patch_infos_y_ : patch_infos_);
   a->patch_indices_per_compute_call.push_back(patch_inds_to_use_);
+}
+
+void ImgSimMetric2DPatchNCCHIP::set_use_mov_img_patch_variances_as_wgts(bool b)
+{
+  if (b) throw ImgSimMetric2D::UnsupportedOperationException();
+}
+
+void ImgSimMetric2DPatchGradNCCHIP::set_use_fixed_img_patch_variances_as_wgts(bool b)
+{
+  fixed_var_ = b;
+  set_variance_weighting(fixed_var_ ? (grad_var_ ? 2 : 1) : 0);
+}
+
+void ImgSimMetric2DPatchGradNCCHIP::set_use_variances_in_grad_imgs_as_wgts(bool b)
+{
+  grad_var_ = b;
+  set_variance_weighting(fixed_var_ ? (grad_var_ ? 2 : 1) : 0);
+}
+
+void ImgSimMetric2DPatchGradNCCHIP::set_use_mov_img_patch_variances_as_wgts(bool b)
+{
+  if (b) throw ImgSimMetric2D::UnsupportedOperationException();
 }
 
 void ImgSimMetric2DPatchNCCHIP::before_sim(xrh_fixed* f, uint64_t gen)
@@ -835,11 +922,13 @@ void ImgSimMetric2DPatchGradNCCHIP::before_sim(xrh_fixed* f, uint64_t gen)
 void ImgSimMetric2DPatchGradNCCHIP::after_sim()
 {
   if (!save_aux_info_) return;
-  // the x and y sub-metrics keep the same record (xregImgSimMetric2DPatchGradNCCCPU.cpp:246-252)
+  // the x and y sub-metrics keep their records (xregImgSimMetric2DPatchGradNCCCPU.cpp:246-252):
+  // the same patches, and the same weights unless each direction is weighted by its own
+  // gradient image's variances
   auto g = std::static_pointer_cast<PatchGradNCCSimAux>(sim_aux_);
   if (!g) g = std::make_shared<PatchGradNCCSimAux>();
   record_patches(g->sim_aux_x);
-  record_patches(g->sim_aux_y);
+  record_patches(g->sim_aux_y, true);
   sim_aux_ = g;
 }
 

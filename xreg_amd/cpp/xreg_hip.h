@@ -417,8 +417,8 @@ class ImgSimMetric2DGradNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2D
 // patches used through xrh_fixed_set_patch_weights. The weights are recomputed whenever an
 // input to them (weight image, mask, options, grid) changes. Not implemented, and refused with
 // UnsupportedOperationException: patch statistics over the unmasked pixels only
-// (use_mask_for_patch_stats), per-patch score records (save_all_per_patch_scores), and the
-// Patch-NCC / Patch-Grad-NCC variance-as-weight options.
+// (use_mask_for_patch_stats), per-patch score records (save_all_per_patch_scores), and moving-
+// image patch variances as weights (use_mov_img_patch_variances_as_wgts).
 class ImgSimMetric2DPatchCommonHIP {
  public:
   using WgtImgPtr = ImagePtr;
@@ -460,13 +460,23 @@ class ImgSimMetric2DPatchCommonHIP {
   // the combine handed to the device when it differs from what the fixed image holds
   void prepare_patches(xrh_fixed* fixed, uint64_t fixed_gen, const Image2D& fixed_img,
                        const std::vector<unsigned char>* mask);
-  // the SimAux record of this call: the patch list and the indices used
-  void record_patches(std::shared_ptr<PatchNCCSimAux>& a) const;
+  // the SimAux record of this call: the patch list and the indices used (y: with the y
+  // direction's weights when they differ)
+  void record_patches(std::shared_ptr<PatchNCCSimAux>& a, bool y = false) const;
   bool patches_listed() const { return fixed_inds_; }
+  // fixed-image patch variances as weights (the reference's process_mask,
+  // xregImgSimMetric2DPatchNCCCPU.cpp:331-440, xregImgSimMetric2DPatchGradNCCCPU.cpp:313-400): 0 off,
+  // 1 the variances of the fixed image for every direction, 2 each gradient direction the
+  // variances of its own fixed gradient image
+  void set_variance_weighting(int mode);
+  int variance_weighting() const { return var_mode_; }
   xrh_sim_params* pp_;
 
  private:
-  void compute_weights(const Image2D& fixed_img, const std::vector<unsigned char>* mask);
+  void compute_weights(const Image2D& fixed_img, const std::vector<unsigned char>* mask, const float* wimg,
+                       PatchInfoList& out) const;
+  std::vector<float> variance_image(const float* img, size_type rows, size_type cols,
+                                    const std::vector<unsigned char>* mask) const;
   PatchIndexList draw_random_patches();
   bool compute_mean_ = false, wic_ = true, use_mask_wgt_ = true, normalize_ = true;
   bool choose_rand_ = false;
@@ -478,11 +488,13 @@ class ImgSimMetric2DPatchCommonHIP {
   PatchInfoList patch_infos_;
   uint64_t grid_gen_ = 0;       // fixed image the grid belongs to (0: none)
   bool weights_stale_ = true;
+  int var_mode_ = 0;
+  PatchInfoList patch_infos_y_;  // variance mode 2: the y direction's weights
   std::shared_ptr<void> rng_;   // std::mt19937 (kept out of this header)
   // what the fixed image's device tables hold
   uint64_t applied_gen_ = 0;
   bool applied_default_ = true;
-  std::vector<float> applied_w_;
+  std::vector<float> applied_w_, applied_wy_;
   PatchIndexList applied_inds_;
   bool applied_wic_ = true, applied_mean_ = false;
 };
@@ -491,6 +503,12 @@ class ImgSimMetric2DPatchNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2
  public:
   explicit ImgSimMetric2DPatchNCCHIP(int device = 0)
       : ImgSimMetric2DHIP(XRH_SIM_PATCH_NCC, device), ImgSimMetric2DPatchCommonHIP(&params_) {}
+  // ImgSimMetric2DPatchNCCCPU (xregImgSimMetric2DPatchNCCCPU.h:67-75): the fixed image's patch
+  // variances (over the unmasked pixels when there is a mask) as the weight image
+  bool use_fixed_img_patch_variances_as_wgts() const { return variance_weighting() != 0; }
+  void set_use_fixed_img_patch_variances_as_wgts(bool b) { set_variance_weighting(b ? 1 : 0); }
+  bool use_mov_img_patch_variances_as_wgts() const { return false; }
+  void set_use_mov_img_patch_variances_as_wgts(bool b);
 
  protected:
   void before_sim(xrh_fixed* f, uint64_t gen) override;
@@ -513,6 +531,15 @@ class ImgSimMetric2DPatchGradNCCHIP : public ImgSimMetric2DHIP,
   // patches are drawn at random or listed by the caller
   bool enforce_same_patches_in_both_x_and_y() const { return same_xy_; }
   void set_enforce_same_patches_in_both_x_and_y(bool b) { same_xy_ = b; }
+  // xregImgSimMetric2DPatchGradNCCCPU.h:66-76: fixed-image patch variances as weights, taken
+  // from the fixed image for both directions, or with use_variances_in_grad_imgs_as_wgts from
+  // each direction's own fixed gradient image
+  bool use_fixed_img_patch_variances_as_wgts() const { return fixed_var_; }
+  void set_use_fixed_img_patch_variances_as_wgts(bool b);
+  bool use_variances_in_grad_imgs_as_wgts() const { return grad_var_; }
+  void set_use_variances_in_grad_imgs_as_wgts(bool b);
+  bool use_mov_img_patch_variances_as_wgts() const { return false; }
+  void set_use_mov_img_patch_variances_as_wgts(bool b);
 
  protected:
   void before_sim(xrh_fixed* f, uint64_t gen) override;
@@ -520,6 +547,7 @@ class ImgSimMetric2DPatchGradNCCHIP : public ImgSimMetric2DHIP,
 
  private:
   bool same_xy_ = true;
+  bool fixed_var_ = false, grad_var_ = false;
 };
 
 // ImgSimMetric2DCombineMean (lib/regi/sim_metrics_2d/xregImgSimMetric2DCombine.cpp:79-98)

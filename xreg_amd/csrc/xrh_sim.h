@@ -29,9 +29,10 @@ struct FixedDev {
   const double* pgy;         //   (all pixels unmasked: the float mean's residue)
   // The patch kernels sum 1 - X_p / den_p over every patch of the grid, which with
   // den_p = std_p N / c_p is sum_p (1 - c_p X'_p); adding patch_offset = sum_p c_p - P gives
-  // the reference's sum_p c_p s_p, then divided by patch_norm.
-  double patch_offset;
-  double patch_norm;         // divisor of the patch-score sum (patch count, or weight sum)
+  // the reference's sum_p c_p s_p, then divided by patch_norm. Index 1: Patch-Grad-NCC's y
+  // direction (its own coefficients when the directions are weighted differently).
+  double patch_offset[2];
+  double patch_norm[2];      // divisor of the patch-score sum (patch count, or weight sum)
 };
 
 // One moving image to score.
@@ -73,7 +74,7 @@ struct xrh_fixed {
   // patch metrics: the weights the image was created with (mask fractions, normalised; or
   // unit), the weights last set, the unscaled per-patch denominators (host copies, one per
   // direction) and their device tables
-  std::vector<float> create_weights, weights;
+  std::vector<float> create_weights, weights, weights_y;
   std::vector<float> base_den[2];
   const float* base_den_dev[2] = {nullptr, nullptr};
   float* scaled_den_dev[2] = {nullptr, nullptr};  // allocated on the first weighted set-up

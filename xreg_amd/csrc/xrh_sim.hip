@@ -760,11 +760,11 @@ __global__ void finish_kernel(const SimItem* __restrict__ items, const double* _
       break;
     }
     case XRH_SIM_PATCH_NCC:
-      sim = (acc[0] + fx.patch_offset) / fx.patch_norm;
+      sim = (acc[0] + fx.patch_offset[0]) / fx.patch_norm[0];
       break;
     default: {
-      const float sx = (float)((acc[0] + fx.patch_offset) / fx.patch_norm);
-      const float sy = (float)((acc[1] + fx.patch_offset) / fx.patch_norm);
+      const float sx = (float)((acc[0] + fx.patch_offset[0]) / fx.patch_norm[0]);
+      const float sy = (float)((acc[1] + fx.patch_offset[1]) / fx.patch_norm[1]);
       sim = 0.5 * (double)(sx + sy);
       break;
     }
@@ -950,61 +950,79 @@ static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, floa
 // number of uses (compute_mean), else by the float sum of the used patches' weights
 // (weight_in_combine), else by 1. The coefficients are folded into the per-patch denominators
 // (den / c_p, +inf for c_p = 0) and the offset sum c - P, so the patch kernels are the same for
-// every weighting. `upload` = false only at creation, before the device copy exists.
-static void patch_coefficients(xrh_fixed* f, const float* w, const int64_t* inds, int64_t n_inds, bool wic,
-                               bool mean, bool upload)
+// every weighting. Patch-Grad-NCC's y direction takes its own weights when `wy` is given.
+// `upload` = false only at creation, before the device copy exists.
+static void patch_coefficients(xrh_fixed* f, const float* w, const float* wy, const int64_t* inds, int64_t n_inds,
+                               bool wic, bool mean, bool upload)
 {
   FixedDev& h = f->host;
   const int64_t np = h.num_patches;
-  std::vector<float> wt = w ? std::vector<float>(w, w + np) : f->create_weights;
-  for (float v : wt) XRH_REQUIRE(std::isfinite(v), "patch weights must be finite");
+  const bool grad = h.metric == XRH_SIM_PATCH_GRAD_NCC;
+  XRH_REQUIRE(!wy || grad, "separate y-direction weights need Patch-Grad-NCC");
+  XRH_REQUIRE(!wy || w, "y-direction weights need x-direction weights");
+  std::vector<float> wt[2];
+  wt[0] = w ? std::vector<float>(w, w + np) : f->create_weights;
+  wt[1] = wy ? std::vector<float>(wy, wy + np) : wt[0];
+  for (int k = 0; k < 2; ++k)
+    for (float v : wt[k]) XRH_REQUIRE(std::isfinite(v), "patch weights must be finite");
   const int64_t nu = inds ? n_inds : np;
   XRH_REQUIRE(nu >= 1, "need at least one patch index");
-  std::vector<double> coef((size_t)np, 0.0);
-  float wsum = 0.0f;
   for (int64_t k = 0; k < nu; ++k) {
     const int64_t p = inds ? inds[k] : k;
     XRH_REQUIRE(p >= 0 && p < np, "patch index out of range");
-    const float wp = wt[(size_t)p];
-    wsum += wp;
-    if (!wic) {
-      coef[(size_t)p] += 1.0;
-    } else if (std::fabs(wp) > 1.0e-6f) {
-      coef[(size_t)p] += (double)wp;
+  }
+  const int dirs = grad ? 2 : 1;
+  std::vector<double> coef[2];
+  double norm[2] = {1.0, 1.0};
+  bool unit[2] = {true, true};
+  for (int k = 0; k < dirs; ++k) {
+    coef[k].assign((size_t)np, 0.0);
+    float wsum = 0.0f;
+    for (int64_t q = 0; q < nu; ++q) {
+      const int64_t p = inds ? inds[q] : q;
+      const float wp = wt[k][(size_t)p];
+      wsum += wp;
+      if (!wic) {
+        coef[k][(size_t)p] += 1.0;
+      } else if (std::fabs(wp) > 1.0e-6f) {
+        coef[k][(size_t)p] += (double)wp;
+      }
     }
+    if (mean) {
+      norm[k] = (double)(float)nu;
+    } else if (wic) {
+      XRH_REQUIRE(wsum != 0.0f, "the weights of the patches used sum to zero");
+      norm[k] = (double)wsum;
+    }
+    unit[k] = std::all_of(coef[k].begin(), coef[k].end(), [](double c) { return c == 1.0; });
   }
-  double norm = 1.0;
-  if (mean) {
-    norm = (double)(float)nu;
-  } else if (wic) {
-    XRH_REQUIRE(wsum != 0.0f, "the weights of the patches used sum to zero");
-    norm = (double)wsum;
-  }
-  const bool unit = std::all_of(coef.begin(), coef.end(), [](double c) { return c == 1.0; });
-  const bool grad = h.metric == XRH_SIM_PATCH_GRAD_NCC;
   hipStream_t s = f->ctx->stream;
   if (upload) XRH_HIP_CHECK(hipStreamSynchronize(s));  // work queued earlier reads the old tables
-  double csum = 0.0;
-  for (double c : coef) csum += c;
   const float* den[2] = {f->base_den_dev[0], f->base_den_dev[1]};
-  if (!unit) {
-    std::vector<float> scaled((size_t)np);
-    for (int k = 0; k < (grad ? 2 : 1); ++k) {
-      if (!f->scaled_den_dev[k]) f->scaled_den_dev[k] = fixed_buf(f, (uint64_t)np);
-      for (int64_t p = 0; p < np; ++p) {
-        const double c = coef[(size_t)p];
-        scaled[(size_t)p] = (c == 0.0) ? INFINITY : (float)((double)f->base_den[k][(size_t)p] / c);
-      }
-      XRH_HIP_CHECK(hipMemcpyAsync(f->scaled_den_dev[k], scaled.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
-      XRH_HIP_CHECK(hipStreamSynchronize(s));
-      den[k] = f->scaled_den_dev[k];
+  std::vector<float> scaled((size_t)np);
+  for (int k = 0; k < dirs; ++k) {
+    double csum = 0.0;
+    for (double c : coef[k]) csum += c;
+    h.patch_offset[k] = unit[k] ? 0.0 : csum - (double)np;
+    h.patch_norm[k] = norm[k];
+    if (unit[k]) continue;
+    if (!f->scaled_den_dev[k]) f->scaled_den_dev[k] = fixed_buf(f, (uint64_t)np);
+    for (int64_t p = 0; p < np; ++p) {
+      const double c = coef[k][(size_t)p];
+      scaled[(size_t)p] = (c == 0.0) ? INFINITY : (float)((double)f->base_den[k][(size_t)p] / c);
     }
+    XRH_HIP_CHECK(hipMemcpyAsync(f->scaled_den_dev[k], scaled.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
+    XRH_HIP_CHECK(hipStreamSynchronize(s));
+    den[k] = f->scaled_den_dev[k];
+  }
+  if (!grad) {
+    h.patch_offset[1] = h.patch_offset[0];
+    h.patch_norm[1] = h.patch_norm[0];
   }
   h.pden = den[0];
   if (grad) h.pdeny = den[1];
-  h.patch_offset = unit ? 0.0 : csum - (double)np;
-  h.patch_norm = norm;
-  f->weights = std::move(wt);
+  f->weights = std::move(wt[0]);
+  f->weights_y = std::move(wt[1]);
   if (upload) {
     XRH_HIP_CHECK(hipMemcpyAsync(f->dev, &h, sizeof(FixedDev), hipMemcpyHostToDevice, s));
     XRH_HIP_CHECK(hipStreamSynchronize(s));
@@ -1083,8 +1101,8 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
       h.num_patches = (int64_t)h.ppr * h.ppc;
       h.count = n;
       h.pf = gx;
-      h.patch_norm = (double)h.num_patches;
-      h.patch_offset = 0.0;
+      h.patch_norm[0] = h.patch_norm[1] = (double)h.num_patches;
+      h.patch_offset[0] = h.patch_offset[1] = 0.0;
       f->create_weights.assign((size_t)h.num_patches, 1.0f);
       if (mask_host) {
         // ImgSimMetric2DPatchCommon::compute_weights with a mask and the default options
@@ -1124,10 +1142,10 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
                                      hipMemcpyDeviceToHost, s));
       }
       XRH_HIP_CHECK(hipStreamSynchronize(s));
-      f->weights = f->create_weights;
+      f->weights = f->weights_y = f->create_weights;
       // the default combine: weighted by the patch weights, divided by their sum
       // (xregImgSimMetric2DPatchNCCCPU.cpp:252-253, 272-285); unit weights need no scaling
-      if (mask_host) patch_coefficients(f, nullptr, nullptr, 0, true, false, false);
+      if (mask_host) patch_coefficients(f, nullptr, nullptr, nullptr, 0, true, false, false);
     }
     if (patch) {
       int max_lds = 0;
@@ -1237,14 +1255,32 @@ xrh_status xrh_fixed_patches(const xrh_fixed* fixed, int64_t capacity, int32_t* 
   });
 }
 
-xrh_status xrh_fixed_set_patch_weights(xrh_fixed* fixed, const float* weights, const int64_t* patch_inds,
-                                       int64_t num_inds, int32_t weight_in_combine, int32_t compute_mean)
+xrh_status xrh_fixed_set_patch_weights(xrh_fixed* fixed, const float* weights, const float* weights_y,
+                                       const int64_t* patch_inds, int64_t num_inds, int32_t weight_in_combine,
+                                       int32_t compute_mean)
 {
   return guarded([&] {
     XRH_REQUIRE(fixed, "null argument");
     const FixedDev& h = fixed->host;
     XRH_REQUIRE(h.metric == XRH_SIM_PATCH_NCC || h.metric == XRH_SIM_PATCH_GRAD_NCC, "not a patch metric");
-    patch_coefficients(fixed, weights, patch_inds, num_inds, weight_in_combine != 0, compute_mean != 0, true);
+    patch_coefficients(fixed, weights, weights_y, patch_inds, num_inds, weight_in_combine != 0, compute_mean != 0,
+                       true);
+  });
+}
+
+xrh_status xrh_fixed_image(const xrh_fixed* fixed, int32_t which, float* host_out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(fixed && host_out, "null argument");
+    const FixedDev& h = fixed->host;
+    const bool grad = h.metric == XRH_SIM_GRAD_NCC || h.metric == XRH_SIM_PATCH_GRAD_NCC;
+    const bool patch = h.metric == XRH_SIM_PATCH_NCC || h.metric == XRH_SIM_PATCH_GRAD_NCC;
+    XRH_REQUIRE(which == 0 || (which == 1 && grad), "which must be 0, or 1 for a gradient metric");
+    XRH_REQUIRE(patch, "processed images are kept for the patch metrics");
+    const float* src = (which == 0) ? h.pf : h.pfy;
+    XRH_HIP_CHECK(hipMemcpyAsync(host_out, src, sizeof(float) * (size_t)h.rows * h.cols, hipMemcpyDeviceToHost,
+                                 fixed->ctx->stream));
+    XRH_HIP_CHECK(hipStreamSynchronize(fixed->ctx->stream));
   });
 }
 
