@@ -349,7 +349,7 @@ def test_fp32_layouts_agree(ctx, c1s):
         assert np.abs(o.astype(np.float64) - outs[0]).max() <= 2e-6 * m
 
 
-@pytest.mark.parametrize("layout", [3, 4])
+@pytest.mark.parametrize("layout", [3, 4, 5])
 @pytest.mark.parametrize("name", ["c1", "c2", "c4"])
 @pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("kernel,store", [(0, 0), (1, 0), (0, 1)])
