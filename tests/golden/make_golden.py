@@ -74,8 +74,46 @@ def interp_case():
     np.savez_compressed(OUT / "interp_c1_small.npz", **out)
 
 
+PATCHW_SETTINGS = [
+    # (name, weights key or None, indices key or None, weight_in_combine, compute_mean, masked)
+    ("var_all", "w_var", None, True, False, False),
+    ("var_list_mean", "w_var", "inds", True, True, False),
+    ("unit_list_sum", None, "inds", False, False, False),
+    ("mask_var_list", "w_var_mask", "inds", True, False, True),
+]
+
+
+def patchw_case():
+    """Patch metrics with the general combine on the small C2 case: weights from the fixed
+    image's patch variances (with and without a mask), a list of patches with repeats, the
+    mean and unweighted combines."""
+    w = synth.config("c2", n_poses=3, small=True)
+    poses = np.concatenate([w.gt[None], w.poses]).astype(np.float32).reshape(-1, 16)
+    drr, _ = O.drr_f32(w.vol, w.idx_to_phys, w.cams, poses, step=w.step, threads=1)
+    fixed = synth.noisy_fixed(drr[0])
+    mask = np.ones(fixed.shape, np.uint8)
+    mask[:12, :20] = 0
+    mask[30:40, 5:60] = 0
+    r = 3
+    n_p = (fixed.shape[0] - 2 * r) * (fixed.shape[1] - 2 * r)
+    out = {"fixed": fixed, "mov": drr, "mask": mask, "radius": np.int32(r),
+           "w_var": O.patch_weights_f32(fixed.shape, r, 1, wgt_img=O.patch_variance_image_f32(fixed, r, 1)),
+           "w_var_mask": O.patch_weights_f32(fixed.shape, r, 1, wgt_img=O.patch_variance_image_f32(fixed, r, 1, mask),
+                                             mask=mask),
+           "inds": np.concatenate([np.arange(5, n_p, 53), [17, 17, 17]]).astype(np.int64)}
+    for metric in (2, 3):
+        for name, wk, ik, wic, mean, masked in PATCHW_SETTINGS:
+            kw = dict(weights=out[wk] if wk else None, inds=out[ik] if ik else None, weight_in_combine=wic,
+                      compute_mean=mean, smooth_ksize=5, radius=r, mask=mask if masked else None, threads=1)
+            out[f"m{metric}_{name}_f32"] = O.patch_sim_weighted_f32(metric, fixed, drr, **kw)
+            out[f"m{metric}_{name}_f64"] = O.patch_sim_weighted_f64(metric, fixed.astype(np.float64),
+                                                                    drr.astype(np.float64), **kw)
+    np.savez_compressed(OUT / "patchw_c2_small.npz", **out)
+
+
 if __name__ == "__main__":
     drr_case()
     sim_case()
     interp_case()
+    patchw_case()
     print("wrote", sorted(p.name for p in OUT.glob("*.npz")))

@@ -135,3 +135,53 @@ def test_gpu_matches_golden_interp(ctx, ginterp):
             assert drr_rel_err(got, z[name]).max() <= DRR_REL_TOL, name
     buf.free()
     vol.close()
+
+
+@pytest.fixture(scope="module")
+def gpatchw():
+    return np.load(GOLD / "patchw_c2_small.npz")
+
+
+# (name, weights key or None, indices key or None, weight_in_combine, compute_mean, masked); the
+# settings tests/golden/make_golden.py wrote
+PATCHW_SETTINGS = (("var_all", "w_var", None, True, False, False),
+                   ("var_list_mean", "w_var", "inds", True, True, False),
+                   ("unit_list_sum", None, "inds", False, False, False),
+                   ("mask_var_list", "w_var_mask", "inds", True, False, True))
+
+
+def test_oracle_reproduces_golden_patch_weights(oracle, gpatchw):
+    z = gpatchw
+    r = int(z["radius"])
+    assert np.array_equal(oracle.patch_weights_f32(z["fixed"].shape, r, 1, wgt_img=oracle.patch_variance_image_f32(
+        z["fixed"], r, 1)), z["w_var"])
+    for metric in (2, 3):
+        for name, wk, ik, wic, mean, masked in PATCHW_SETTINGS:
+            got = oracle.patch_sim_weighted_f32(metric, z["fixed"], z["mov"], weights=z[wk] if wk else None,
+                                                inds=z[ik] if ik else None, weight_in_combine=wic, compute_mean=mean,
+                                                smooth_ksize=5, radius=r, mask=z["mask"] if masked else None)
+            assert np.array_equal(got, z[f"m{metric}_{name}_f32"]), (metric, name)
+
+
+@pytest.mark.gpu
+def test_gpu_matches_golden_patch_weights(ctx, gpatchw):
+    from xreg_amd import _xrh as X
+
+    z = gpatchw
+    r = int(z["radius"])
+    mov = z["mov"]
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    for metric in (2, 3):
+        for name, wk, ik, wic, mean, masked in PATCHW_SETTINGS:
+            print(f"case: golden patch weights metric={metric} {name}", flush=True)
+            f = X.Fixed(ctx, z["fixed"], metric, smooth_ksize=5, patch_radius=r, mask=z["mask"] if masked else None)
+            f.set_patch_weights(weights=z[wk] if wk else None, patch_inds=z[ik] if ik else None,
+                                weight_in_combine=wic, compute_mean=mean)
+            got = f.compute(buf, mov.shape[0])
+            f.close()
+            ref32, ref64 = z[f"m{metric}_{name}_f32"], z[f"m{metric}_{name}_f64"]
+            tol = SIM_ABS_TOL * np.maximum(1.0, np.abs(ref64))
+            assert np.all(np.abs(got - ref64) <= tol), (metric, name)
+            assert np.all(np.abs(got - ref32) <= tol + np.abs(ref32 - ref64)), (metric, name)
+    buf.free()
