@@ -101,12 +101,16 @@ typedef struct xrh_camera {
  * ImgSimMetric2DGradImgParamInterface::smooth_img_before_sobel_kernel_radius (an odd
  * GaussianBlur aperture 1,3,5,7 or 0 = no smoothing; xregImgSimMetric2DGradImgCPU.cpp:48-55);
  * patch_radius / patch_stride are ImgSimMetric2DPatchCommon's (defaults 5 and 1,
- * xregImgSimMetric2DPatchCommon.h:141-143). */
+ * xregImgSimMetric2DPatchCommon.h:141-143); patch_mask_stats is its use_mask_for_patch_stats_
+ * (default 0, .h:149): with a mask, every patch's mean and standard deviation, fixed and
+ * moving, are taken over its unmasked pixels only (ComputePatchMeanStdDev,
+ * xregImgSimMetric2DPatchNCCCPU.cpp:558-619) and the fixed patch is scaled by their count. */
 typedef struct xrh_sim_params {
   int32_t metric;
   int32_t smooth_ksize;
   int32_t patch_radius;
   int32_t patch_stride;
+  int32_t patch_mask_stats;
 } xrh_sim_params;
 
 /* ---------------------------------------------------------------------------------- */

@@ -313,7 +313,7 @@ def _inds(inds):
 
 
 def patch_sim_weighted_f32(metric, fixed, mov, weights=None, inds=None, weight_in_combine=True, compute_mean=False,
-                           smooth_ksize=5, radius=5, stride=1, mask=None, threads=0):
+                           smooth_ksize=5, radius=5, stride=1, mask=None, threads=0, mask_stats=False):
     """Patch-NCC (metric 2) / Patch-Grad-NCC (3) with the general combine of
     ImgSimMetric2DPatchNCCCPU::compute: patch weights, the used patch indices, and the
     weight-in-combine / mean flags."""
@@ -327,17 +327,17 @@ def patch_sim_weighted_f32(metric, fixed, mov, weights=None, inds=None, weight_i
     if metric == 2:
         lib().orc_patch_ncc_weighted_f32(_f(f), _f(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius,
                                          stride, wp, ip, C.c_int64(n), int(bool(weight_in_combine)),
-                                         int(bool(compute_mean)), _f(out), threads)
+                                         int(bool(compute_mean)), int(bool(mask_stats)), _f(out), threads)
     else:
         lib().orc_patch_grad_ncc_weighted_f32(_f(f), _f(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]),
                                               smooth_ksize, radius, stride, wp, ip, C.c_int64(n),
-                                              int(bool(weight_in_combine)), int(bool(compute_mean)), _f(out),
-                                              threads)
+                                              int(bool(weight_in_combine)), int(bool(compute_mean)),
+                                              int(bool(mask_stats)), _f(out), threads)
     return out
 
 
 def patch_sim_weighted_f64(metric, fixed, mov, weights=None, inds=None, weight_in_combine=True, compute_mean=False,
-                           smooth_ksize=5, radius=5, stride=1, mask=None, threads=0):
+                           smooth_ksize=5, radius=5, stride=1, mask=None, threads=0, mask_stats=False):
     f = np.ascontiguousarray(fixed, dtype=np.float64)
     m = np.ascontiguousarray(mov, dtype=np.float64).reshape(-1, *f.shape)
     out = np.zeros(m.shape[0], dtype=np.float64)
@@ -348,12 +348,12 @@ def patch_sim_weighted_f64(metric, fixed, mov, weights=None, inds=None, weight_i
     if metric == 2:
         lib().orc_patch_ncc_weighted_f64(_d(f), _d(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]), radius,
                                          stride, wp, ip, C.c_int64(n), int(bool(weight_in_combine)),
-                                         int(bool(compute_mean)), _d(out), threads)
+                                         int(bool(compute_mean)), int(bool(mask_stats)), _d(out), threads)
     else:
         lib().orc_patch_grad_ncc_weighted_f64(_d(f), _d(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]),
                                               smooth_ksize, radius, stride, wp, ip, C.c_int64(n),
-                                              int(bool(weight_in_combine)), int(bool(compute_mean)), _d(out),
-                                              threads)
+                                              int(bool(weight_in_combine)), int(bool(compute_mean)),
+                                              int(bool(mask_stats)), _d(out), threads)
     return out
 
 

@@ -1343,23 +1343,35 @@ typedef struct {
   const float* wgt;    /* per-patch weights, or NULL for unit weights */
   const int64_t* inds; /* patch indices used, or NULL for 0 .. num_patches - 1 */
   int32_t wic;         /* weight_patch_sims_in_combine_ */
+  int32_t mstats;      /* use_mask_for_patch_stats_ (with a mask) */
 } patch_f32_ctx;
 
+/* ComputePatchMeanStdDev (:558-619): over the whole patch, or over the pixels `mask` keeps
+ * (mask non-NULL: use_mask_for_patch_stats_); the sum is divided only when n > 1 */
 static void patch_stats_f32(const float* img, int32_t cols, int32_t r0, int32_t c0, int32_t diam,
-                            float* mean_out, float* std_out)
+                            const uint8_t* mask, float* mean_out, float* std_out, int64_t* n_out)
 {
   float mean = 0.0f;
+  int64_t n = 0;
   for (int32_t y = 0; y < diam; ++y) {
-    for (int32_t x = 0; x < diam; ++x) mean += img[(int64_t)(r0 + y) * cols + c0 + x];
+    for (int32_t x = 0; x < diam; ++x) {
+      const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+      if (!mask || mask[o]) {
+        mean += img[o];
+        ++n;
+      }
+    }
   }
-  const int64_t n = (int64_t)diam * diam;
   float var = 0.0f;
   if (n > 1) {
     mean /= (float)n;
     for (int32_t y = 0; y < diam; ++y) {
       for (int32_t x = 0; x < diam; ++x) {
-        const float t = img[(int64_t)(r0 + y) * cols + c0 + x] - mean;
-        var += t * t;
+        const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+        if (!mask || mask[o]) {
+          const float t = img[o] - mean;
+          var += t * t;
+        }
       }
     }
     var /= (float)n - 1.0f;
@@ -1367,6 +1379,7 @@ static void patch_stats_f32(const float* img, int32_t cols, int32_t r0, int32_t 
   const float s = sqrtf(var);
   *mean_out = mean;
   *std_out = (s > 1.0e-6f) ? s : 1.0e-6f;
+  *n_out = n;
 }
 
 static void patch_fixed_range(void* vctx, int64_t begin, int64_t end)
@@ -1377,8 +1390,9 @@ static void patch_fixed_range(void* vctx, int64_t begin, int64_t end)
     const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
     const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
     float mean, sd;
-    patch_stats_f32(c->fixed, c->cols, r0, c0, d, &mean, &sd);
-    const float denom = sd * (float)((int64_t)d * d);
+    int64_t n;
+    patch_stats_f32(c->fixed, c->cols, r0, c0, d, c->mstats ? c->mask : NULL, &mean, &sd, &n);
+    const float denom = sd * (float)n;
     float* dst = c->fhat + p * d * d;
     for (int32_t y = 0; y < d; ++y) {
       for (int32_t x = 0; x < d; ++x) {
@@ -1402,8 +1416,9 @@ static void patch_mov_range(void* vctx, int64_t begin, int64_t end)
       continue;
     }
     float mean, sd;
-    /* patch statistics over the whole patch (use_mask_for_patch_stats_ is false) */
-    patch_stats_f32(c->mov, c->cols, r0, c0, d, &mean, &sd);
+    int64_t n;
+    /* patch statistics over the whole patch, or its unmasked pixels (use_mask_for_patch_stats_) */
+    patch_stats_f32(c->mov, c->cols, r0, c0, d, c->mstats ? c->mask : NULL, &mean, &sd, &n);
     const float* fh = c->fhat + p * d * d;
     float accum = 0.0f;
     for (int32_t y = 0; y < d; ++y) {
@@ -1456,7 +1471,7 @@ void orc_patch_ncc_masked_f32(const float* fixed, const float* mov, const uint8_
     patch_mask_weights_f32(mask, cols, ppr, np, patch_stride, diam, wgt);
   }
   orc_patch_ncc_weighted_f32(fixed, mov, mask, rows, cols, num_mov, patch_radius, patch_stride, wgt, NULL, 0, 1, 0,
-                             sim_out, threads);
+                             0, sim_out, threads);
   free(wgt);
 }
 
@@ -1543,7 +1558,7 @@ void orc_patch_variance_image_f32(const float* img, const uint8_t* mask, int32_t
 void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
                                 int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                                 const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
-                                int32_t compute_mean, float* sim_out, int32_t threads)
+                                int32_t compute_mean, int32_t mask_stats, float* sim_out, int32_t threads)
 {
   const int32_t diam = 2 * patch_radius + 1;
   const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
@@ -1564,6 +1579,7 @@ void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint
   c.wgt = wgt;
   c.inds = inds;
   c.wic = weight_in_combine;
+  c.mstats = mask_stats && mask;
   orc_parallel_for(patch_fixed_range, &c, np, 256, threads);
   for (int64_t m = 0; m < num_mov; ++m) {
     c.mov = mov + m * npix;
@@ -1599,27 +1615,41 @@ typedef struct {
   const double* wgt;
   const int64_t* inds;
   int32_t wic;
+  int32_t mstats;
+  double* fn; /* per patch: pixels the fixed statistics used */
 } patch_f64_ctx;
 
 static void patch_stats_f64(const double* img, int32_t cols, int32_t r0, int32_t c0, int32_t d,
-                            double* mean_out, double* sd_out)
+                            const uint8_t* mask, double* mean_out, double* sd_out, double* n_out)
 {
-  const double n = (double)d * d;
-  double mean = 0.0;
-  for (int32_t y = 0; y < d; ++y) {
-    for (int32_t x = 0; x < d; ++x) mean += img[(int64_t)(r0 + y) * cols + c0 + x];
-  }
-  mean /= n;
-  double var = 0.0;
+  double mean = 0.0, n = 0.0;
   for (int32_t y = 0; y < d; ++y) {
     for (int32_t x = 0; x < d; ++x) {
-      const double t = img[(int64_t)(r0 + y) * cols + c0 + x] - mean;
-      var += t * t;
+      const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+      if (!mask || mask[o]) {
+        mean += img[o];
+        n += 1.0;
+      }
     }
   }
-  double sd = sqrt(var / (n - 1.0));
+  double var = 0.0;
+  if (n > 1.0) {
+    mean /= n;
+    for (int32_t y = 0; y < d; ++y) {
+      for (int32_t x = 0; x < d; ++x) {
+        const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+        if (!mask || mask[o]) {
+          const double t = img[o] - mean;
+          var += t * t;
+        }
+      }
+    }
+    var /= n - 1.0;
+  }
+  const double sd = sqrt(var);
   *mean_out = mean;
   *sd_out = (sd > 1.0e-6) ? sd : 1.0e-6;
+  *n_out = n;
 }
 
 static void patch_fixed_range_f64(void* vctx, int64_t begin, int64_t end)
@@ -1628,7 +1658,8 @@ static void patch_fixed_range_f64(void* vctx, int64_t begin, int64_t end)
   for (int64_t p = begin; p < end; ++p) {
     const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
     const int32_t c0 = (int32_t)(p % c->ppr) * c->stride;
-    patch_stats_f64(c->fixed, c->cols, r0, c0, c->diam, &c->fmean[p], &c->fsd[p]);
+    patch_stats_f64(c->fixed, c->cols, r0, c0, c->diam, c->mstats ? c->mask : NULL, &c->fmean[p], &c->fsd[p],
+                    &c->fn[p]);
   }
 }
 
@@ -1636,7 +1667,6 @@ static void patch_mov_range_f64(void* vctx, int64_t begin, int64_t end)
 {
   patch_f64_ctx* c = (patch_f64_ctx*)vctx;
   const int32_t d = c->diam;
-  const double n = (double)d * d;
   for (int64_t k = begin; k < end; ++k) {
     const int64_t p = c->inds ? c->inds[k] : k;
     const int32_t r0 = (int32_t)(p / c->ppr) * c->stride;
@@ -1646,13 +1676,13 @@ static void patch_mov_range_f64(void* vctx, int64_t begin, int64_t end)
       c->vals[k] = 0.0;
       continue;
     }
-    double mean, sd;
-    patch_stats_f64(c->mov, c->cols, r0, c0, d, &mean, &sd);
+    double mean, sd, nm;
+    patch_stats_f64(c->mov, c->cols, r0, c0, d, c->mstats ? c->mask : NULL, &mean, &sd, &nm);
     double acc = 0.0;
     for (int32_t y = 0; y < d; ++y) {
       for (int32_t x = 0; x < d; ++x) {
         const int64_t o = (int64_t)(r0 + y) * c->cols + c0 + x;
-        if (!c->mask || c->mask[o]) acc += ((c->mov[o] - mean) / sd) * ((c->fixed[o] - c->fmean[p]) / (c->fsd[p] * n));
+        if (!c->mask || c->mask[o]) acc += ((c->mov[o] - mean) / sd) * ((c->fixed[o] - c->fmean[p]) / (c->fsd[p] * c->fn[p]));
       }
     }
     c->vals[k] = (c->wic ? w : 1.0) * (1.0 - acc);
@@ -1689,14 +1719,14 @@ void orc_patch_ncc_masked_f64(const double* fixed, const double* mov, const uint
     for (int64_t p = 0; p < np; ++p) wgt[p] /= wsum;
   }
   orc_patch_ncc_weighted_f64(fixed, mov, mask, rows, cols, num_mov, patch_radius, patch_stride, wgt, NULL, 0, 1, 0,
-                             sim_out, threads);
+                             0, sim_out, threads);
   free(wgt);
 }
 
 void orc_patch_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
                                 int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                                 const double* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
-                                int32_t compute_mean, double* sim_out, int32_t threads)
+                                int32_t compute_mean, int32_t mask_stats, double* sim_out, int32_t threads)
 {
   const int32_t diam = 2 * patch_radius + 1;
   const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
@@ -1712,11 +1742,13 @@ void orc_patch_ncc_weighted_f64(const double* fixed, const double* mov, const ui
   c.stride = patch_stride;
   c.fmean = (double*)malloc(sizeof(double) * (size_t)np);
   c.fsd = (double*)malloc(sizeof(double) * (size_t)np);
+  c.fn = (double*)malloc(sizeof(double) * (size_t)np);
   c.vals = (double*)malloc(sizeof(double) * (size_t)(nu > 0 ? nu : 1));
   c.mask = mask;
   c.wgt = wgt;
   c.inds = inds;
   c.wic = weight_in_combine;
+  c.mstats = mask_stats && mask;
   double norm = 1.0;
   if (compute_mean) {
     norm = (double)nu;
@@ -1732,6 +1764,7 @@ void orc_patch_ncc_weighted_f64(const double* fixed, const double* mov, const ui
   }
   free(c.fmean);
   free(c.fsd);
+  free(c.fn);
   free(c.vals);
 }
 
@@ -1757,15 +1790,15 @@ void orc_patch_grad_ncc_masked_f32(const float* fixed, const float* mov, const u
     patch_mask_weights_f32(mask, cols, ppr, np, patch_stride, diam, wgt);
   }
   orc_patch_grad_ncc_weighted_f32(fixed, mov, mask, rows, cols, num_mov, smooth_ksize, patch_radius, patch_stride,
-                                  wgt, NULL, 0, 1, 0, sim_out, threads);
+                                  wgt, NULL, 0, 1, 0, 0, sim_out, threads);
   free(wgt);
 }
 
 void orc_patch_grad_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
                                      int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                                      int32_t patch_stride, const float* wgt, const int64_t* inds, int64_t num_inds,
-                                     int32_t weight_in_combine, int32_t compute_mean, float* sim_out,
-                                     int32_t threads)
+                                     int32_t weight_in_combine, int32_t compute_mean, int32_t mask_stats,
+                                     float* sim_out, int32_t threads)
 {
   /* ImgSimMetric2DPatchGradNCCCPU::compute (xregImgSimMetric2DPatchGradNCCCPU.cpp:104-224):
    * both directions use the parent's patches and weights (enforce_same_patches_in_both_x_and_y_,
@@ -1780,9 +1813,9 @@ void orc_patch_grad_ncc_weighted_f32(const float* fixed, const float* mov, const
     float sx, sy;
     orc_sobel_f32(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
     orc_patch_ncc_weighted_f32(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
-                               weight_in_combine, compute_mean, &sx, threads);
+                               weight_in_combine, compute_mean, mask_stats, &sx, threads);
     orc_patch_ncc_weighted_f32(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
-                               weight_in_combine, compute_mean, &sy, threads);
+                               weight_in_combine, compute_mean, mask_stats, &sy, threads);
     sim_out[m] = (float)(0.5 * (double)(sx + sy));
   }
   free(fgx);
@@ -1825,8 +1858,8 @@ void orc_patch_grad_ncc_masked_f64(const double* fixed, const double* mov, const
 void orc_patch_grad_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
                                      int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                                      int32_t patch_stride, const double* wgt, const int64_t* inds, int64_t num_inds,
-                                     int32_t weight_in_combine, int32_t compute_mean, double* sim_out,
-                                     int32_t threads)
+                                     int32_t weight_in_combine, int32_t compute_mean, int32_t mask_stats,
+                                     double* sim_out, int32_t threads)
 {
   const int64_t npix = (int64_t)rows * cols;
   double* fgx = (double*)malloc(sizeof(double) * (size_t)npix);
@@ -1838,9 +1871,9 @@ void orc_patch_grad_ncc_weighted_f64(const double* fixed, const double* mov, con
     double sx, sy;
     orc_sobel_f64(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
     orc_patch_ncc_weighted_f64(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
-                               weight_in_combine, compute_mean, &sx, threads);
+                               weight_in_combine, compute_mean, mask_stats, &sx, threads);
     orc_patch_ncc_weighted_f64(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, wgt, inds, num_inds,
-                               weight_in_combine, compute_mean, &sy, threads);
+                               weight_in_combine, compute_mean, mask_stats, &sy, threads);
     sim_out[m] = 0.5 * (sx + sy);
   }
   free(fgx);

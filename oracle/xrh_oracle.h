@@ -184,25 +184,27 @@ void orc_patch_variance_image_f32(const float* img, const uint8_t* mask, int32_t
 /* Patch NCC / Patch-Grad-NCC with the reference's general combine
  * (xregImgSimMetric2DPatchNCCCPU.cpp:186-287): per-patch weights wgt (NULL = 1), the patch
  * indices used in order (inds, num_inds; NULL = all), weight_patch_sims_in_combine_ and
- * compute_mean_of_patch_sims_. A mask restricts the correlation to unmasked pixels. */
+ * compute_mean_of_patch_sims_. A mask restricts the correlation to unmasked pixels; with mask_stats
+ * (use_mask_for_patch_stats_) the patch means and deviations, fixed and moving, are taken over
+ * the unmasked pixels too and the fixed patch is scaled by their count. */
 void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
                                 int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                                 const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
-                                int32_t compute_mean, float* sim_out, int32_t threads);
+                                int32_t compute_mean, int32_t mask_stats, float* sim_out, int32_t threads);
 void orc_patch_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
                                 int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                                 const double* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
-                                int32_t compute_mean, double* sim_out, int32_t threads);
+                                int32_t compute_mean, int32_t mask_stats, double* sim_out, int32_t threads);
 void orc_patch_grad_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
                                      int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                                      int32_t patch_stride, const float* wgt, const int64_t* inds, int64_t num_inds,
-                                     int32_t weight_in_combine, int32_t compute_mean, float* sim_out,
-                                     int32_t threads);
+                                     int32_t weight_in_combine, int32_t compute_mean, int32_t mask_stats,
+                                     float* sim_out, int32_t threads);
 void orc_patch_grad_ncc_weighted_f64(const double* fixed, const double* mov, const uint8_t* mask, int32_t rows,
                                      int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
                                      int32_t patch_stride, const double* wgt, const int64_t* inds, int64_t num_inds,
-                                     int32_t weight_in_combine, int32_t compute_mean, double* sim_out,
-                                     int32_t threads);
+                                     int32_t weight_in_combine, int32_t compute_mean, int32_t mask_stats,
+                                     double* sim_out, int32_t threads);
 
 /* Eigen 3.3 SSE sum restatement, exported for its own test. */
 float orc_eigen_sum_f32(const float* x, int64_t n);

@@ -274,11 +274,6 @@ int main(int argc, char** argv)
         ps->allocate_resources();
         uint64_t unsupported = 0;
         try {
-          pc->set_use_mask_for_patch_stats(true);
-        } catch (const ImgSimMetric2D::UnsupportedOperationException&) {
-          ++unsupported;
-        }
-        try {
           pc->set_save_all_per_patch_scores(true);
         } catch (const ImgSimMetric2D::UnsupportedOperationException&) {
           ++unsupported;
@@ -362,7 +357,17 @@ int main(int argc, char** argv)
           pg->set_use_variances_in_grad_imgs_as_wgts(true);
           ps->compute();
           rec.put("popt_gvar_mask_sims", ps->sim_vals());
+          pg->set_use_variances_in_grad_imgs_as_wgts(false);
         }
+        // patch statistics over the unmasked pixels, with the mask's default weights
+        if (pn) pn->set_use_fixed_img_patch_variances_as_wgts(false);
+        if (pg) pg->set_use_fixed_img_patch_variances_as_wgts(false);
+        pc->set_use_mask_for_patch_stats(true);
+        ps->compute();
+        rec.put("popt_mstats_sims", ps->sim_vals());
+        pc->set_use_mask_for_patch_stats(false);
+        ps->compute();
+        rec.put("popt_mstats_off_sims", ps->sim_vals());
       }
     }
     rc->use_external_host_pixel_buf(nullptr);

@@ -43,7 +43,7 @@ def test_struct_layouts_match_header():
 
     # xrh_camera: 3 int32 + float + 9 + 16 + 3 + 2 floats
     assert ctypes.sizeof(X.Camera) == 4 * (4 + 9 + 16 + 3 + 2)
-    assert ctypes.sizeof(X.SimParams) == 16
+    assert ctypes.sizeof(X.SimParams) == 20
     assert X.View.cam.offset == 16
 
 

@@ -131,7 +131,7 @@ def _check_patch_options(oracle, rec, metric, smooth, fixed, drr, mask):
     rows, cols = fixed.shape
     rr, cc = np.mgrid[0:rows, 0:cols]
     wimg = (1.0 + (7 * rr + 3 * cc) % 11).astype(np.float32)
-    assert int(np.frombuffer(rec["popt_unsupported"], np.uint64)[0]) == 2
+    assert int(np.frombuffer(rec["popt_unsupported"], np.uint64)[0]) == 1
     n_p = (rows - 10) * (cols - 10)
 
     def check(name, **kw):
@@ -188,3 +188,9 @@ def _check_patch_options(oracle, rec, metric, smooth, fixed, drr, mask):
         print(f"  popt_gvar_mask_sims: max |gpu-cpu| {np.abs(got - ref).max():.3e}", flush=True)
         assert np.abs(got - ref).max() <= 2 * SIM_ABS_TOL
         assert np.abs(got - np.frombuffer(rec["popt_fvar_mask_sims"], np.float32)).max() > 1e-5
+    # patch statistics over the unmasked pixels (use_mask_for_patch_stats), then off again: the
+    # masked metric's values
+    w_mask = oracle.patch_weights_f32(fixed.shape, 5, 1, mask=mask)
+    check("popt_mstats_sims", weights=w_mask, mask=mask, mask_stats=True)
+    assert np.array_equal(np.frombuffer(rec["popt_mstats_off_sims"], np.float32),
+                          np.frombuffer(rec["mask_sims"], np.float32))

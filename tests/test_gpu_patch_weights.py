@@ -229,3 +229,48 @@ def test_per_direction_weights_and_fixed_gradients(ctx, oracle, images):
     assert np.array_equal(g.image(0), fixed.astype(np.float32))
     g.close()
     buf.free()
+
+
+@pytest.mark.parametrize("radius,stride", [(5, 1), (3, 2)])
+@pytest.mark.parametrize("metric", [2, 3])
+def test_mask_stats_match_oracle(ctx, oracle, images, metric, radius, stride):
+    """use_mask_for_patch_stats (xrh_sim_params.patch_mask_stats): patch statistics over the
+    unmasked pixels, with the mask's weights and with a weighted patch list, against the oracle;
+    patches with no unmasked pixel included."""
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    mask = _mask(fixed.shape)
+    mask[60:75, 60:75] = 0     # whole patches without an unmasked pixel
+    print(f"case: mask stats metric={metric} r={radius} s={stride}", flush=True)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, metric, smooth_ksize=5, patch_radius=radius, patch_stride=stride, mask=mask,
+                patch_mask_stats=True)
+    wm = oracle.patch_weights_f32(fixed.shape, radius, stride, mask=mask)
+    n_p = wm.size
+    inds = np.random.default_rng(3).integers(0, n_p, 200)
+    for st in (dict(), dict(patch_inds=inds, weight_in_combine=False, compute_mean=True)):
+        f.set_patch_weights(**st)
+        got = f.compute(buf, mov.shape[0])
+        kw = dict(weights=wm, inds=st.get("patch_inds"), weight_in_combine=st.get("weight_in_combine", True),
+                  compute_mean=st.get("compute_mean", False), smooth_ksize=5, radius=radius, stride=stride,
+                  mask=mask, mask_stats=True)
+        ref = oracle.patch_sim_weighted_f32(metric, fixed, mov, **kw)
+        ref64 = oracle.patch_sim_weighted_f64(metric, fixed.astype(np.float64), mov.astype(np.float64), **kw)
+        print(f"  {sorted(st)}: max |gpu-cpu| {np.abs(got - ref).max():.3e} |gpu-fp64| {np.abs(got - ref64).max():.3e}",
+              flush=True)
+        assert np.all(np.abs(got - ref64) <= _tol(ref64) + np.abs(ref - ref64))
+        assert np.all(np.abs(got - ref) <= _tol(ref) + np.abs(ref - ref64))
+    plain = oracle.patch_sim_weighted_f32(metric, fixed, mov, weights=wm, smooth_ksize=5, radius=radius,
+                                          stride=stride, mask=mask)
+    f.set_patch_weights()
+    assert np.abs(f.compute(buf, mov.shape[0]) - plain).max() > 1e-4  # the option changes the metric
+    f.close()
+    # without a mask the option changes nothing
+    g0 = X.Fixed(ctx, fixed, metric, smooth_ksize=5, patch_radius=radius, patch_stride=stride)
+    g1 = X.Fixed(ctx, fixed, metric, smooth_ksize=5, patch_radius=radius, patch_stride=stride, patch_mask_stats=True)
+    assert np.array_equal(g0.compute(buf, mov.shape[0]), g1.compute(buf, mov.shape[0]))
+    g0.close()
+    g1.close()
+    buf.free()

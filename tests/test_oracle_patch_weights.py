@@ -210,3 +210,55 @@ def test_patch_variance_image_matches_numpy(oracle):
         assert np.all(got[off] == 0)
         np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-13)
         assert got[r, r] == np.float32(1e-6) ** 2 or mk is not None
+
+
+def _patch_scores_mask_stats(fixed, mov, mask, r, s=1):
+    """use_mask_for_patch_stats: means and deviations over the unmasked pixels U of the patch
+    (the sum left undivided when |U| <= 1, deviation 0 then, clamped at 1e-6), the fixed patch
+    scaled by |U|, the correlation over U; an empty U scores 1."""
+    out = []
+    rows, cols = fixed.shape
+    for y in range(r, rows - r, s):
+        for x in range(r, cols - r, s):
+            K = mask[y - r:y + r + 1, x - r:x + r + 1].astype(bool)
+            F = fixed[y - r:y + r + 1, x - r:x + r + 1].astype(np.float64)[K]
+            M = mov[y - r:y + r + 1, x - r:x + r + 1].astype(np.float64)[K]
+            n = F.size
+            if n == 0:
+                out.append(1.0)
+                continue
+            mf, mm = (F.mean(), M.mean()) if n > 1 else (F.sum(), M.sum())
+            sf = max(1e-6, F.std(ddof=1)) if n > 1 else 1e-6
+            sm = max(1e-6, M.std(ddof=1)) if n > 1 else 1e-6
+            out.append(1.0 - (((M - mm) / sm) * ((F - mf) / (sf * n))).sum())
+    return np.array(out)
+
+
+def test_mask_stats_match_definition(oracle, pair):
+    f, m = pair
+    mask = (np.random.default_rng(8).random(f.shape) > 0.3).astype(np.uint8)
+    mask[:7, :7] = 0          # a patch with no unmasked pixel (score 1)
+    mask[9:14, 0:5] = 0
+    mask[11, 2] = 1           # a patch with a single unmasked pixel
+    for stride in (1, 2):
+        s = _patch_scores_mask_stats(f, m, mask, 2, stride)
+        assert s[0] == 1.0
+        w = oracle.patch_weights_f32(f.shape, 2, stride, mask=mask).astype(np.float64)
+        keep = np.abs(w) > 1e-6
+        want = float((w * s * keep).sum() / w.sum())
+        got = oracle.patch_sim_weighted_f64(2, f, m[None], mask=mask, radius=2, stride=stride, mask_stats=True,
+                                            weights=w.astype(np.float32))[0]
+        assert abs(got - want) < 1e-9
+        # unit weights, not weighted: the plain mean of the scores (the empty patch included)
+        got = oracle.patch_sim_weighted_f64(2, f, m[None], mask=mask, radius=2, stride=stride, mask_stats=True,
+                                            weight_in_combine=False, compute_mean=True)[0]
+        assert abs(got - float(s.mean())) < 1e-12
+        got32 = oracle.patch_sim_weighted_f32(2, f, m[None], mask=mask, radius=2, stride=stride, mask_stats=True,
+                                              weight_in_combine=False, compute_mean=True)[0]
+        assert abs(got32 - float(s.mean())) < 1e-5
+    # whole-patch statistics differ (the option matters), and without a mask it changes nothing
+    a = oracle.patch_sim_weighted_f64(2, f, m[None], mask=mask, radius=2, mask_stats=True)[0]
+    b = oracle.patch_sim_weighted_f64(2, f, m[None], mask=mask, radius=2)[0]
+    assert abs(a - b) > 1e-4
+    assert np.array_equal(oracle.patch_sim_weighted_f32(2, f, m[None], radius=2, mask_stats=True),
+                          oracle.patch_sim_weighted_f32(2, f, m[None], radius=2))

@@ -58,6 +58,7 @@ class SimParams(C.Structure):
         ("smooth_ksize", C.c_int32),
         ("patch_radius", C.c_int32),
         ("patch_stride", C.c_int32),
+        ("patch_mask_stats", C.c_int32),
     ]
 
 
@@ -361,9 +362,10 @@ def line_int_sample_count(ctx: Context, vol: Volume, cams, xforms, cam_for_proj,
 
 class Fixed:
     def __init__(self, ctx: Context, img: np.ndarray, metric: int, smooth_ksize: int = 5, patch_radius: int = 5,
-                 patch_stride: int = 1, mask: np.ndarray | None = None):
+                 patch_stride: int = 1, mask: np.ndarray | None = None, patch_mask_stats: bool = False):
         im = np.ascontiguousarray(img, dtype=np.float32)
-        prm = SimParams(int(metric), int(smooth_ksize), int(patch_radius), int(patch_stride))
+        prm = SimParams(int(metric), int(smooth_ksize), int(patch_radius), int(patch_stride),
+                        int(bool(patch_mask_stats)))
         mk = None
         if mask is not None:
             self._mask = np.ascontiguousarray(mask, dtype=np.uint8)

@@ -549,6 +549,7 @@ void ImgSimMetric2DHIP::allocate_resources()
   const unsigned char* mk = mask_ ? mask_->data() : nullptr;
   check_xrh(xrh_fixed_create(ctx_, fixed_img_->buf.data(), (int32_t)fixed_img_->rows, (int32_t)fixed_img_->cols,
                              &params_, mk, &fixed_));
+  created_params_ = params_;
   mask_updated_ = false;
   ++fixed_gen_;
 }
@@ -556,7 +557,9 @@ void ImgSimMetric2DHIP::allocate_resources()
 void ImgSimMetric2DHIP::compute()
 {
   if (!fixed_) throw StringMessageException("allocate_resources() must be called first");
-  if (mask_updated_) allocate_resources();
+  // a new mask, or parameters changed since the fixed image was processed (patch radius,
+  // stride, smoothing, use_mask_for_patch_stats), process it again
+  if (mask_updated_ || std::memcmp(&params_, &created_params_, sizeof(params_)) != 0) allocate_resources();
   const size_type npx = fixed_img_->rows * fixed_img_->cols;
   const float* mov = nullptr;
   if (rc_) {
@@ -614,11 +617,6 @@ void ImgSimMetric2DPatchCommonHIP::set_normalize_weights_as_prob(bool b)
 {
   normalize_ = b;
   weights_stale_ = true;
-}
-
-void ImgSimMetric2DPatchCommonHIP::set_use_mask_for_patch_stats(bool u)
-{
-  if (u) throw ImgSimMetric2D::UnsupportedOperationException();
 }
 
 void ImgSimMetric2DPatchCommonHIP::set_save_all_per_patch_scores(bool s)

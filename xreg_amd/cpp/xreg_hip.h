@@ -378,6 +378,7 @@ class ImgSimMetric2DHIP : public ImgSimMetric2D {
  private:
   xrh_ctx* ctx_;
   xrh_fixed* fixed_ = nullptr;
+  xrh_sim_params created_params_{};  // the parameters fixed_ was processed with
   uint64_t fixed_gen_ = 0;
   RayCaster* rc_ = nullptr;
   size_type proj_off_ = 0;
@@ -416,8 +417,7 @@ class ImgSimMetric2DGradNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2D
 // bookkeeping: the device scores every patch of the grid and applies the weights and the
 // patches used through xrh_fixed_set_patch_weights. The weights are recomputed whenever an
 // input to them (weight image, mask, options, grid) changes. Not implemented, and refused with
-// UnsupportedOperationException: patch statistics over the unmasked pixels only
-// (use_mask_for_patch_stats), per-patch score records (save_all_per_patch_scores), and moving-
+// UnsupportedOperationException: per-patch score records (save_all_per_patch_scores) and moving-
 // image patch variances as weights (use_mov_img_patch_variances_as_wgts).
 class ImgSimMetric2DPatchCommonHIP {
  public:
@@ -433,8 +433,10 @@ class ImgSimMetric2DPatchCommonHIP {
   void set_weight_patch_sims_in_combine(bool b) { wic_ = b; }
   bool use_mask_for_patch_weighting() const { return use_mask_wgt_; }
   void set_use_mask_for_patch_weighting(bool b);
-  bool use_mask_for_patch_stats() const { return false; }
-  void set_use_mask_for_patch_stats(bool u);
+  // with a mask, patch means and deviations over the unmasked pixels (the fixed image is
+  // processed again at the next compute when this changes)
+  bool use_mask_for_patch_stats() const { return pp_->patch_mask_stats != 0; }
+  void set_use_mask_for_patch_stats(bool u) { pp_->patch_mask_stats = u ? 1 : 0; }
   bool save_all_per_patch_scores() const { return false; }
   void set_save_all_per_patch_scores(bool s);
   bool normalize_weights_as_prob() const { return normalize_; }

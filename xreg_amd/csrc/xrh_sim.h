@@ -27,6 +27,9 @@ struct FixedDev {
   const float* pdeny;
   const double* pg;          // per-patch sum_U(f) - |U| mean_f over the used pixels U
   const double* pgy;         //   (all pixels unmasked: the float mean's residue)
+  // use_mask_for_patch_stats with a mask: patch means and deviations over the unmasked pixels
+  // U, whose count per patch is pcount (as float); else null
+  const float* pcount;
   // The patch kernels sum 1 - X_p / den_p over every patch of the grid, which with
   // den_p = std_p N / c_p is sum_p (1 - c_p X'_p); adding patch_offset = sum_p c_p - P gives
   // the reference's sum_p c_p s_p, then divided by patch_norm. Index 1: Patch-Grad-NCC's y

@@ -351,13 +351,15 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
   const int X0 = px0 * s, Y0 = py0 * s;                                  // first image pixel
   const int W = patch_lds_edge(s, r);
   const bool masked = fx.mask != nullptr;
+  const bool mstats = fx.pcount != nullptr;  // statistics over the unmasked pixels (needs a mask)
   double* lm = patch_lds;
   double* lf = lm + W * W;
   double* h1 = lf + W * W;
   double* h2 = h1 + W * kT;
   double* hf = h2 + W * kT;
-  double* lk = hf + W * kT;  // masked only: mask window, then sum_U m
+  double* lk = hf + W * kT;  // masked only: mask window, then sum_U m (and sum_U m^2)
   double* hu = lk + W * W;
+  double* h2u = hu + W * kT;
   const int pxl = threadIdx.x % kT, pyl = threadIdx.x / kT;
   const int pi = px0 + pxl, pj = py0 + pyl;
   const bool valid = (pi < ppr) && (pj < ppc);
@@ -395,28 +397,42 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
       hf[e] = af;
       if (masked) {
         const double* k = lk + row * W + x0;
-        double au = 0.0;
+        double au = 0.0, a2u = 0.0;
         for (int xx = 0; xx < d; ++xx) au = fma(m[xx], k[xx], au);
         hu[e] = au;
+        if (mstats) {
+          for (int xx = 0; xx < d; ++xx) a2u = fma(m[xx] * m[xx], k[xx], a2u);
+          h2u[e] = a2u;
+        }
       }
     }
     __syncthreads();
     if (valid) {
-      double S1 = 0.0, S2 = 0.0, Sf = 0.0, SU = 0.0;
+      double S1 = 0.0, S2 = 0.0, Sf = 0.0, SU = 0.0, S2U = 0.0;
       for (int yy = 0; yy < d; ++yy) {
         const int e = (pyl * s + yy) * kT + pxl;
         S1 += h1[e];
         S2 += h2[e];
         Sf += hf[e];
         if (masked) SU += hu[e];
+        if (mstats) S2U += h2u[e];
       }
       if (!masked) SU = S1;
       const int64_t p = (int64_t)pj * ppr + pi;
-      const double mean = S1 / N;
-      const double var = fmax(S2 - mean * S1, 0.0) / (N - 1.0);
+      double mean, var;
+      if (mstats) {
+        // ComputePatchMeanStdDev over the n unmasked pixels: the sum is divided only when n > 1
+        const double n = (double)fx.pcount[p];
+        mean = (n > 1.0) ? SU / n : SU;
+        var = (n > 1.0) ? fmax(S2U - mean * SU, 0.0) / (n - 1.0) : 0.0;
+      } else {
+        mean = S1 / N;
+        var = fmax(S2 - mean * S1, 0.0) / (N - 1.0);
+      }
       const double sd = fmax((double)1.0e-6f, sqrt(var));
       const double cross = (Sf - (double)pmu[p] * SU) - mean * pg[p];
-      acc[dir] = 1.0 - (cross / sd) / (double)pden[p];
+      // a patch with no unmasked pixel correlates over nothing: its score is exactly 1
+      acc[dir] = (mstats && fx.pcount[p] == 0.0f) ? 1.0 : 1.0 - (cross / sd) / (double)pden[p];
     }
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + tile) * 2);
@@ -580,32 +596,46 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + tile) * 2);
 }
 
-// fixed-image per-patch statistics (reference float order, ComputePatchMeanStdDev)
+// fixed-image per-patch statistics (reference float order, ComputePatchMeanStdDev): over the
+// whole patch, or over its unmasked pixels when count_out is given (use_mask_for_patch_stats)
 __global__ void patch_stats_kernel(const float* __restrict__ img, const uint8_t* __restrict__ mask, int cols,
                                    int ppr, int ppc, int s, int d, float* __restrict__ mean_out,
-                                   float* __restrict__ den_out, double* __restrict__ g_out)
+                                   float* __restrict__ den_out, double* __restrict__ g_out,
+                                   float* __restrict__ count_out)
 {
 #pragma clang fp contract(off)
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (int64_t)ppr * ppc) return;
   const int r0 = (int)(p / ppr) * s, c0 = (int)(p % ppr) * s;
-  const int N = d * d;
+  const bool mstats = count_out != nullptr;
   float mean = 0.0f;
-  for (int y = 0; y < d; ++y)
-    for (int x = 0; x < d; ++x) mean += img[(int64_t)(r0 + y) * cols + c0 + x];
-  mean /= (float)N;
-  float var = 0.0f;
+  int n = 0;
   for (int y = 0; y < d; ++y)
     for (int x = 0; x < d; ++x) {
-      const float t = img[(int64_t)(r0 + y) * cols + c0 + x] - mean;
-      var += t * t;
+      const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+      if (!mstats || mask[o]) {
+        mean += img[o];
+        ++n;
+      }
     }
-  var /= (float)N - 1.0f;
+  float var = 0.0f;
+  if (n > 1) {
+    mean /= (float)n;
+    for (int y = 0; y < d; ++y)
+      for (int x = 0; x < d; ++x) {
+        const int64_t o = (int64_t)(r0 + y) * cols + c0 + x;
+        if (!mstats || mask[o]) {
+          const float t = img[o] - mean;
+          var += t * t;
+        }
+      }
+    var /= (float)n - 1.0f;
+  }
   const float sd = fmaxf(1.0e-6f, sqrtf(var));
   mean_out[p] = mean;
-  den_out[p] = sd * (float)N;
-  // over the pixels the correlation uses (statistics above use the whole patch, as
-  // use_mask_for_patch_stats_ is off by default)
+  den_out[p] = sd * (float)n;
+  if (mstats) count_out[p] = (float)n;
+  // over the pixels the correlation uses
   double f1 = 0.0, nu = 0.0;
   for (int y = 0; y < d; ++y)
     for (int x = 0; x < d; ++x) {
@@ -817,7 +847,7 @@ int sim_patch_lds_bytes(const FixedDev& f)
     // patch_kernel_d: float window pair + three double row-sum arrays
     return 2 * W * W * (int)sizeof(float) + 3 * W * kT * (int)sizeof(double);
   }
-  return (f.mask ? (3 * W * W + 4 * W * kT) : (2 * W * W + 3 * W * kT)) * (int)sizeof(double);
+  return (f.mask ? (3 * W * W + (f.pcount ? 5 : 4) * W * kT) : (2 * W * W + 3 * W * kT)) * (int)sizeof(double);
 }
 
 int sim_tiles(const FixedDev& f)
@@ -928,7 +958,8 @@ static void ncc_prepare(xrh_fixed* f, const float* img, int64_t n, const uint8_t
   *count_out = (int64_t)cnt;
 }
 
-static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, float** den_out, double** g_out)
+static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, float** den_out, double** g_out,
+                          float* count)
 {
   FixedDev& h = f->host;
   float* mean = fixed_buf(f, (uint64_t)h.num_patches);
@@ -936,7 +967,7 @@ static void patch_prepare(xrh_fixed* f, const float* img, float** mean_out, floa
   double* g = (double*)fixed_buf(f, (uint64_t)h.num_patches * 2);
   const int d = 2 * h.patch_radius + 1;
   hipLaunchKernelGGL(patch_stats_kernel, dim3((unsigned)((h.num_patches + 255) / 256)), dim3(256), 0,
-                     f->ctx->stream, img, h.mask, h.cols, h.ppr, h.ppc, h.patch_stride, d, mean, den, g);
+                     f->ctx->stream, img, h.mask, h.cols, h.ppr, h.ppc, h.patch_stride, d, mean, den, g, count);
   XRH_HIP_CHECK(hipGetLastError());
   *mean_out = mean;
   *den_out = den;
@@ -1123,14 +1154,17 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
       }
       float *mu = nullptr, *den = nullptr;
       double* g = nullptr;
-      patch_prepare(f, gx, &mu, &den, &g);
+      // use_mask_for_patch_stats: only with a mask (without one it changes nothing)
+      float* count = (p->patch_mask_stats && mask) ? fixed_buf(f, (uint64_t)h.num_patches) : nullptr;
+      h.pcount = count;
+      patch_prepare(f, gx, &mu, &den, &g, count);
       h.pmean = mu;
       h.pden = den;
       h.pg = g;
       f->base_den_dev[0] = den;
       if (grad) {
         h.pfy = gy;
-        patch_prepare(f, gy, &mu, &den, &g);
+        patch_prepare(f, gy, &mu, &den, &g, count);
         h.pmeany = mu;
         h.pdeny = den;
         h.pgy = g;
