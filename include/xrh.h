@@ -340,6 +340,15 @@ xrh_status xrh_fixed_image(const xrh_fixed* fixed, int32_t which, float* host_ou
 xrh_status xrh_sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev,
                            int64_t num_mov, float* sim_dev_out, float* sim_host_out);
 
+/* The same for a patch metric, and every grid patch's unweighted score per moving image into
+ * scores_host_out (num_mov x num_patches floats, image-major): ImgSimMetric2DPatchCommon's
+ * save_all_per_patch_scores record (xregImgSimMetric2DPatchCommon.h:88-95; the values of
+ * xregImgSimMetric2DPatchNCCCPU.cpp:247-250, Patch-Grad-NCC 0.5 (x + y),
+ * xregImgSimMetric2DPatchGradNCCCPU.cpp:226-240), for patches the combine skips too. Runs the
+ * general patch kernel (a debugging aid, not the fast path); synchronous. */
+xrh_status xrh_sim_patch_scores(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev, int64_t num_mov,
+                                float* sim_host_out, float* scores_host_out);
+
 /* ---------------------------------------------------------------------------------- */
 /* batched evaluation: one optimiser generation (all poses x all views, any mix of
  * volumes, detector sizes and fixed images) as one fused DRR + similarity pass, replayed

@@ -357,6 +357,20 @@ def patch_sim_weighted_f64(metric, fixed, mov, weights=None, inds=None, weight_i
     return out
 
 
+def patch_scores_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, mask=None, mask_stats=False,
+                     threads=0):
+    """Every grid patch's unweighted score per moving image (num_mov, num_patches)."""
+    f = np.ascontiguousarray(fixed, dtype=np.float32)
+    m = np.ascontiguousarray(mov, dtype=np.float32).reshape(-1, *f.shape)
+    ppr = (f.shape[1] - 1 - 2 * radius) // stride + 1
+    ppc = (f.shape[0] - 1 - 2 * radius) // stride + 1
+    out = np.zeros((m.shape[0], ppr * ppc), dtype=np.float32)
+    mk, mp = _mask_ptr(mask, f.shape)
+    lib().orc_patch_scores_f32(int(metric), _f(f), _f(m), mp, f.shape[0], f.shape[1], C.c_int64(m.shape[0]),
+                               smooth_ksize, radius, stride, int(bool(mask_stats)), _f(out), threads)
+    return out
+
+
 def sim_f32(metric, fixed, mov, smooth_ksize=5, radius=5, stride=1, threads=0, mask=None):
     if metric == 0:
         return ncc_f32(fixed, mov, mask=mask)

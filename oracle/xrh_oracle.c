@@ -1555,10 +1555,67 @@ void orc_patch_variance_image_f32(const float* img, const uint8_t* mask, int32_t
   }
 }
 
+static void patch_ncc_core_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                               int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                               const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
+                               int32_t compute_mean, int32_t mask_stats, float* sim_out, float* vals_out,
+                               int32_t threads);
+
 void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
                                 int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
                                 const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
                                 int32_t compute_mean, int32_t mask_stats, float* sim_out, int32_t threads)
+{
+  patch_ncc_core_f32(fixed, mov, mask, rows, cols, num_mov, patch_radius, patch_stride, wgt, inds, num_inds,
+                     weight_in_combine, compute_mean, mask_stats, sim_out, NULL, threads);
+}
+
+void orc_patch_scores_f32(int32_t metric, const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                          int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                          int32_t patch_stride, int32_t mask_stats, float* scores_out, int32_t threads)
+{
+  /* sim_vals_for_each_patch_: every patch's unweighted score 1 - accum
+   * (xregImgSimMetric2DPatchNCCCPU.cpp:228-250); Patch-Grad-NCC 0.5 (x + y) in float
+   * (xregImgSimMetric2DPatchGradNCCCPU.cpp:226-240) */
+  const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int32_t ppc = (rows - 1 - 2 * patch_radius) / patch_stride + 1;
+  const int64_t np = (int64_t)ppr * ppc;
+  const int64_t npix = (int64_t)rows * cols;
+  float sim;
+  if (metric == 2) {
+    for (int64_t m = 0; m < num_mov; ++m) {
+      patch_ncc_core_f32(fixed, mov + m * npix, mask, rows, cols, 1, patch_radius, patch_stride, NULL, NULL, 0, 0, 0,
+                         mask_stats, &sim, scores_out + m * np, threads);
+    }
+    return;
+  }
+  float* fgx = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* fgy = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* mgx = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* mgy = (float*)malloc(sizeof(float) * (size_t)npix);
+  float* vy = (float*)malloc(sizeof(float) * (size_t)np);
+  orc_sobel_f32(fixed, rows, cols, smooth_ksize, fgx, fgy);
+  for (int64_t m = 0; m < num_mov; ++m) {
+    float* vx = scores_out + m * np;
+    orc_sobel_f32(mov + m * npix, rows, cols, smooth_ksize, mgx, mgy);
+    patch_ncc_core_f32(fgx, mgx, mask, rows, cols, 1, patch_radius, patch_stride, NULL, NULL, 0, 0, 0, mask_stats,
+                       &sim, vx, threads);
+    patch_ncc_core_f32(fgy, mgy, mask, rows, cols, 1, patch_radius, patch_stride, NULL, NULL, 0, 0, 0, mask_stats,
+                       &sim, vy, threads);
+    for (int64_t p = 0; p < np; ++p) vx[p] = (float)(0.5 * (double)(vx[p] + vy[p]));
+  }
+  free(fgx);
+  free(fgy);
+  free(mgx);
+  free(mgy);
+  free(vy);
+}
+
+static void patch_ncc_core_f32(const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                               int32_t cols, int64_t num_mov, int32_t patch_radius, int32_t patch_stride,
+                               const float* wgt, const int64_t* inds, int64_t num_inds, int32_t weight_in_combine,
+                               int32_t compute_mean, int32_t mask_stats, float* sim_out, float* vals_out,
+                               int32_t threads)
 {
   const int32_t diam = 2 * patch_radius + 1;
   const int32_t ppr = (cols - 1 - 2 * patch_radius) / patch_stride + 1;
@@ -1584,6 +1641,7 @@ void orc_patch_ncc_weighted_f32(const float* fixed, const float* mov, const uint
   for (int64_t m = 0; m < num_mov; ++m) {
     c.mov = mov + m * npix;
     orc_parallel_for(patch_mov_range, &c, nu, 256, threads);
+    if (vals_out) memcpy(vals_out + m * nu, c.vals, sizeof(float) * (size_t)nu);
     float sum = 0.0f;
     for (int64_t k = 0; k < nu; ++k) sum += c.vals[k];
     /* combine (:262-287): the mean over the used patches, else the weighted sum divided by

@@ -206,6 +206,12 @@ void orc_patch_grad_ncc_weighted_f64(const double* fixed, const double* mov, con
                                      int32_t weight_in_combine, int32_t compute_mean, int32_t mask_stats,
                                      double* sim_out, int32_t threads);
 
+/* Every grid patch's unweighted score per moving image (save_all_per_patch_scores: Patch-NCC
+ * 1 - accum, metric 3 Patch-Grad-NCC 0.5 (x + y) in float), num_mov x num_patches. */
+void orc_patch_scores_f32(int32_t metric, const float* fixed, const float* mov, const uint8_t* mask, int32_t rows,
+                          int32_t cols, int64_t num_mov, int32_t smooth_ksize, int32_t patch_radius,
+                          int32_t patch_stride, int32_t mask_stats, float* scores_out, int32_t threads);
+
 /* Eigen 3.3 SSE sum restatement, exported for its own test. */
 float orc_eigen_sum_f32(const float* x, int64_t n);
 

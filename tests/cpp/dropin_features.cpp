@@ -15,7 +15,8 @@
 //   * a metric with a mask (set_mask, xregImgSimMetric2D.h:130);
 //   * the patch metrics' options (ImgSimMetric2DPatchCommon, xregImgSimMetric2DPatchCommon.h:
 //     67-127): a weight image, mask weighting on and off, unnormalised weights, the mean and
-//     unweighted combines, a list of patches to use and random patches.
+//     unweighted combines, a list of patches to use with its per-patch scores, random patches,
+//     patch-variance weights and statistics over the unmasked pixels.
 //
 // usage: dropin_features <in.bin> <out.bin>
 // in.bin : int32 X Y Z rows cols n_poses metric smooth
@@ -272,13 +273,6 @@ int main(int argc, char** argv)
         ps->set_num_moving_images(np);
         ps->set_mov_imgs_buf_from_ray_caster(rc.get(), 0);
         ps->allocate_resources();
-        uint64_t unsupported = 0;
-        try {
-          pc->set_save_all_per_patch_scores(true);
-        } catch (const ImgSimMetric2D::UnsupportedOperationException&) {
-          ++unsupported;
-        }
-        rec.put_u64("popt_unsupported", unsupported);
         pc->set_wgt_img(wimg);
         ps->compute();
         rec.put("popt_wimg_sims", ps->sim_vals());
@@ -299,8 +293,16 @@ int main(int argc, char** argv)
         for (size_type q = 3; q < pc->patch_infos().size(); q += 37) list.push_back(q);
         list.push_back(list[1]);
         pc->set_patches_to_use(list);
+        pc->set_save_all_per_patch_scores(true);
         ps->compute();
         rec.put("popt_list_sims", ps->sim_vals());
+        {
+          // sim_vals_for_each_patch: per used patch, every moving image's score
+          std::vector<float> flat;
+          for (const auto& v : pc->sim_vals_for_each_patch()) flat.insert(flat.end(), v.begin(), v.end());
+          rec.put("popt_list_scores", flat);
+        }
+        pc->set_save_all_per_patch_scores(false);
         rec.put("popt_list_inds", std::vector<uint64_t>(list.begin(), list.end()).data(), list.size() * 8);
         rec.put_u64("popt_list_num", pc->num_patches());
         pc->reset_patches_to_use();

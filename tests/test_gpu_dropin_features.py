@@ -131,7 +131,6 @@ def _check_patch_options(oracle, rec, metric, smooth, fixed, drr, mask):
     rows, cols = fixed.shape
     rr, cc = np.mgrid[0:rows, 0:cols]
     wimg = (1.0 + (7 * rr + 3 * cc) % 11).astype(np.float32)
-    assert int(np.frombuffer(rec["popt_unsupported"], np.uint64)[0]) == 1
     n_p = (rows - 10) * (cols - 10)
 
     def check(name, **kw):
@@ -151,7 +150,12 @@ def _check_patch_options(oracle, rec, metric, smooth, fixed, drr, mask):
     lst.append(lst[1])
     assert np.array_equal(np.frombuffer(rec["popt_list_inds"], np.uint64), np.array(lst, np.uint64))
     assert int(np.frombuffer(rec["popt_list_num"], np.uint64)[0]) == len(lst)
-    check("popt_list_sims", weights=w_img, inds=lst)
+    list_sims = check("popt_list_sims", weights=w_img, inds=lst)
+    # save_all_per_patch_scores: per used patch (local index), every moving image's score
+    got = np.frombuffer(rec["popt_list_scores"], np.float32).reshape(len(lst), drr.shape[0])
+    ref = oracle.patch_scores_f32(metric, fixed, drr, smooth_ksize=smooth)[:, lst].T
+    assert np.abs(got - ref).max() <= 2e-5
+    assert list_sims.shape == (drr.shape[0],)
     # random patches: 40 distinct patches whose centres lie at least sqrt(2) r apart
     ri = np.frombuffer(rec["popt_rand_inds"], np.uint64).astype(np.int64)
     assert ri.size == 40 and np.unique(ri).size == 40 and ri.max() < n_p

@@ -274,3 +274,34 @@ def test_mask_stats_match_oracle(ctx, oracle, images, metric, radius, stride):
     g0.close()
     g1.close()
     buf.free()
+
+
+@pytest.mark.parametrize("radius,stride,masked", [(5, 1, False), (3, 2, True)])
+@pytest.mark.parametrize("metric", [2, 3])
+def test_per_patch_scores_match_oracle(ctx, oracle, images, metric, radius, stride, masked):
+    """xrh_sim_patch_scores (save_all_per_patch_scores): every grid patch's unweighted score
+    against the oracle, and the similarity values it returns against the plain call."""
+    from xreg_amd import _xrh as X
+
+    fixed, mov = images
+    mask = _mask(fixed.shape) if masked else None
+    print(f"case: per-patch scores metric={metric} r={radius} s={stride} masked={masked}", flush=True)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    f = X.Fixed(ctx, fixed, metric, smooth_ksize=5, patch_radius=radius, patch_stride=stride, mask=mask,
+                patch_mask_stats=masked)
+    plain = f.compute(buf, mov.shape[0])
+    sims, scores = f.patch_scores(buf, mov.shape[0])
+    ref = oracle.patch_scores_f32(metric, fixed, mov, smooth_ksize=5, radius=radius, stride=stride, mask=mask,
+                                  mask_stats=masked)
+    print(f"  max |gpu-cpu| per patch {np.abs(scores - ref).max():.3e}; sims vs plain {np.abs(sims - plain).max():.3e}",
+          flush=True)
+    assert scores.shape == ref.shape
+    assert np.abs(scores - ref).max() <= 2e-5
+    assert np.abs(sims - plain).max() <= 1e-6
+    # weights do not change the per-patch scores
+    f.set_patch_weights(weights=np.linspace(0.1, 1.0, ref.shape[1]).astype(np.float32))
+    _, again = f.patch_scores(buf, mov.shape[0])
+    assert np.array_equal(again, scores)
+    f.close()
+    buf.free()

@@ -262,3 +262,23 @@ def test_mask_stats_match_definition(oracle, pair):
     assert abs(a - b) > 1e-4
     assert np.array_equal(oracle.patch_sim_weighted_f32(2, f, m[None], radius=2, mask_stats=True),
                           oracle.patch_sim_weighted_f32(2, f, m[None], radius=2))
+
+
+def test_patch_scores_are_the_per_window_scores(oracle, pair):
+    """save_all_per_patch_scores: every patch's unweighted score; Patch-Grad-NCC the float mean
+    of its x and y scores; masked statistics included."""
+    f, m = pair
+    got = oracle.patch_scores_f32(2, f, m[None], radius=2, stride=2)[0]
+    np.testing.assert_allclose(got, _patch_scores(f, m, 2, 2), rtol=0, atol=2e-6)
+    gxf, gyf = oracle.sobel_f64(f.astype(np.float64), 0)
+    gxm, gym = oracle.sobel_f64(m.astype(np.float64), 0)
+    want = 0.5 * (_patch_scores(gxf, gxm, 2) + _patch_scores(gyf, gym, 2))
+    got = oracle.patch_scores_f32(3, f, m[None], smooth_ksize=0, radius=2)[0]
+    np.testing.assert_allclose(got, want, rtol=0, atol=5e-6)
+    mask = (np.random.default_rng(9).random(f.shape) > 0.3).astype(np.uint8)
+    got = oracle.patch_scores_f32(2, f, m[None], radius=2, mask=mask, mask_stats=True)[0]
+    np.testing.assert_allclose(got, _patch_scores_mask_stats(f, m, mask, 2), rtol=0, atol=2e-6)
+    # their plain mean is the unweighted metric
+    sc = oracle.patch_scores_f32(2, f, m[None], radius=2)[0]
+    mean = oracle.patch_sim_weighted_f64(2, f, m[None], radius=2, weight_in_combine=False, compute_mean=True)[0]
+    assert abs(float(sc.astype(np.float64).mean()) - mean) < 1e-6

@@ -132,6 +132,7 @@ SIGNATURES = {
     "xrh_fixed_patches": (_I32, [_P, _I64, _PI32, _PI32, _PI32, _PI32, _PF, _PI64]),
     "xrh_fixed_set_patch_weights": (_I32, [_P, _PF, _PF, _PI64, _I64, _I32, _I32]),
     "xrh_fixed_image": (_I32, [_P, _I32, _PF]),
+    "xrh_sim_patch_scores": (_I32, [_P, _P, C.c_void_p, _I64, _PF, _PF]),
     "xrh_sim_compute": (_I32, [_P, _P, _P, _I64, _P, _PF]),
     "xrh_batch_create": (_I32, [_P, C.POINTER(View), _I32, _I64, _I32, C.POINTER(_P)]),
     "xrh_batch_destroy": (_I32, [_P]),
@@ -383,6 +384,18 @@ class Fixed:
         check(lib().xrh_sim_compute(self.ctx.h, self.h, mov.ptr + 4 * npix * offset_imgs, int(num_mov), None,
                                     _fp(out)))
         return out
+
+    def patch_scores(self, mov: DeviceBuffer, num_mov: int, offset_imgs: int = 0):
+        """Similarity values and every grid patch's unweighted score per moving image
+        (xrh_sim_patch_scores): (num_mov,) and (num_mov, num_patches) float32."""
+        n_p = C.c_int64(0)
+        check(lib().xrh_fixed_patches(self.h, 0, None, None, None, None, None, C.byref(n_p)))
+        sims = np.empty(num_mov, np.float32)
+        scores = np.empty((num_mov, n_p.value), np.float32)
+        npix = self.shape[0] * self.shape[1]
+        check(lib().xrh_sim_patch_scores(self.ctx.h, self.h, mov.ptr + 4 * npix * offset_imgs, int(num_mov),
+                                         _fp(sims), _fp(scores)))
+        return sims, scores
 
     def patches(self) -> dict:
         """The patch list of a patch metric (xrh_fixed_patches): start / stop rows and columns

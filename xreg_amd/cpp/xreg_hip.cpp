@@ -592,7 +592,14 @@ void ImgSimMetric2DHIP::compute()
   }
   sim_vals_.resize(num_mov_imgs_);
   before_sim(fixed_, fixed_gen_);
-  check_xrh(xrh_sim_compute(ctx_, fixed_, mov, (int64_t)num_mov_imgs_, nullptr, sim_vals_.data()));
+  if (std::vector<float>* sink = patch_score_sink()) {
+    int64_t np = 0;
+    check_xrh(xrh_fixed_patches(fixed_, 0, nullptr, nullptr, nullptr, nullptr, nullptr, &np));
+    sink->resize((size_t)np * num_mov_imgs_);
+    check_xrh(xrh_sim_patch_scores(ctx_, fixed_, mov, (int64_t)num_mov_imgs_, sim_vals_.data(), sink->data()));
+  } else {
+    check_xrh(xrh_sim_compute(ctx_, fixed_, mov, (int64_t)num_mov_imgs_, nullptr, sim_vals_.data()));
+  }
   after_sim();
 }
 
@@ -617,11 +624,6 @@ void ImgSimMetric2DPatchCommonHIP::set_normalize_weights_as_prob(bool b)
 {
   normalize_ = b;
   weights_stale_ = true;
-}
-
-void ImgSimMetric2DPatchCommonHIP::set_save_all_per_patch_scores(bool s)
-{
-  if (s) throw ImgSimMetric2D::UnsupportedOperationException();
 }
 
 size_type ImgSimMetric2DPatchCommonHIP::num_patches() const
@@ -901,8 +903,21 @@ void ImgSimMetric2DPatchNCCHIP::before_sim(xrh_fixed* f, uint64_t gen)
   prepare_patches(f, gen, *fixed_img_, mask_.get());
 }
 
+void ImgSimMetric2DPatchCommonHIP::take_patch_scores(const std::vector<float>& grid, size_type num_mov)
+{
+  // per local index k of the patches used, the scores of patch_inds_to_use_[k]
+  // (xregImgSimMetric2DPatchNCCCPU.cpp:247-250); the reference leaves the entries of patches the
+  // combine skips as they were, these hold their scores as well
+  const size_type np = patch_infos_.size();
+  per_patch_.assign(patch_inds_to_use_.size(), std::vector<float>(num_mov));
+  for (size_type k = 0; k < patch_inds_to_use_.size(); ++k) {
+    for (size_type m = 0; m < num_mov; ++m) per_patch_[k][m] = grid[m * np + patch_inds_to_use_[k]];
+  }
+}
+
 void ImgSimMetric2DPatchNCCHIP::after_sim()
 {
+  if (save_all_per_patch_scores()) take_patch_scores(grid_scores_, num_mov_imgs_);
   if (!save_aux_info_) return;
   auto a = std::static_pointer_cast<PatchNCCSimAux>(sim_aux_);
   record_patches(a);
@@ -919,6 +934,7 @@ void ImgSimMetric2DPatchGradNCCHIP::before_sim(xrh_fixed* f, uint64_t gen)
 
 void ImgSimMetric2DPatchGradNCCHIP::after_sim()
 {
+  if (save_all_per_patch_scores()) take_patch_scores(grid_scores_, num_mov_imgs_);
   if (!save_aux_info_) return;
   // the x and y sub-metrics keep their records (xregImgSimMetric2DPatchGradNCCCPU.cpp:246-252):
   // the same patches, and the same weights unless each direction is weighted by its own

@@ -372,6 +372,9 @@ class ImgSimMetric2DHIP : public ImgSimMetric2D {
   // images allocate_resources has created
   virtual void before_sim(xrh_fixed*, uint64_t /*fixed_gen*/) {}
   virtual void after_sim() {}
+  // where a patch metric wants every grid patch's score of this call (num_mov x num_patches,
+  // through xrh_sim_patch_scores), or null
+  virtual std::vector<float>* patch_score_sink() { return nullptr; }
   xrh_sim_params params_{};
   std::shared_ptr<H5ReadWriteInterface> sim_aux_;
 
@@ -417,8 +420,8 @@ class ImgSimMetric2DGradNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2D
 // bookkeeping: the device scores every patch of the grid and applies the weights and the
 // patches used through xrh_fixed_set_patch_weights. The weights are recomputed whenever an
 // input to them (weight image, mask, options, grid) changes. Not implemented, and refused with
-// UnsupportedOperationException: per-patch score records (save_all_per_patch_scores) and moving-
-// image patch variances as weights (use_mov_img_patch_variances_as_wgts).
+// UnsupportedOperationException: moving-image patch variances as weights
+// (use_mov_img_patch_variances_as_wgts).
 class ImgSimMetric2DPatchCommonHIP {
  public:
   using WgtImgPtr = ImagePtr;
@@ -437,8 +440,12 @@ class ImgSimMetric2DPatchCommonHIP {
   // processed again at the next compute when this changes)
   bool use_mask_for_patch_stats() const { return pp_->patch_mask_stats != 0; }
   void set_use_mask_for_patch_stats(bool u) { pp_->patch_mask_stats = u ? 1 : 0; }
-  bool save_all_per_patch_scores() const { return false; }
-  void set_save_all_per_patch_scores(bool s);
+  // ImgSimMetric2DPatchCommon::sim_vals_for_each_patch (xregImgSimMetric2DPatchCommon.h:88-95):
+  // per used patch (local index), the unweighted score of every moving image of the last call
+  bool save_all_per_patch_scores() const { return save_scores_; }
+  void set_save_all_per_patch_scores(bool s) { save_scores_ = s; }
+  using ListOfSimScalarLists = std::vector<std::vector<float>>;
+  const ListOfSimScalarLists& sim_vals_for_each_patch() const { return per_patch_; }
   bool normalize_weights_as_prob() const { return normalize_; }
   void set_normalize_weights_as_prob(bool b);
   bool choose_rand_patches() const { return choose_rand_; }
@@ -466,6 +473,9 @@ class ImgSimMetric2DPatchCommonHIP {
   // direction's weights when they differ)
   void record_patches(std::shared_ptr<PatchNCCSimAux>& a, bool y = false) const;
   bool patches_listed() const { return fixed_inds_; }
+  // fills sim_vals_for_each_patch from a call's grid scores (num_mov x num_patches)
+  void take_patch_scores(const std::vector<float>& grid, size_type num_mov);
+  std::vector<float> grid_scores_;
   // fixed-image patch variances as weights (the reference's process_mask,
   // xregImgSimMetric2DPatchNCCCPU.cpp:331-440, xregImgSimMetric2DPatchGradNCCCPU.cpp:313-400): 0 off,
   // 1 the variances of the fixed image for every direction, 2 each gradient direction the
@@ -482,6 +492,8 @@ class ImgSimMetric2DPatchCommonHIP {
   PatchIndexList draw_random_patches();
   bool compute_mean_ = false, wic_ = true, use_mask_wgt_ = true, normalize_ = true;
   bool choose_rand_ = false;
+  bool save_scores_ = false;
+  ListOfSimScalarLists per_patch_;
   size_type num_rand_ = 100;
   double rand_sep_ = -1.0;
   bool fixed_inds_ = false;  // set_patches_to_use in force
@@ -515,6 +527,7 @@ class ImgSimMetric2DPatchNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2
  protected:
   void before_sim(xrh_fixed* f, uint64_t gen) override;
   void after_sim() override;
+  std::vector<float>* patch_score_sink() override { return save_all_per_patch_scores() ? &grid_scores_ : nullptr; }
 };
 
 class ImgSimMetric2DPatchGradNCCHIP : public ImgSimMetric2DHIP,
@@ -546,6 +559,7 @@ class ImgSimMetric2DPatchGradNCCHIP : public ImgSimMetric2DHIP,
  protected:
   void before_sim(xrh_fixed* f, uint64_t gen) override;
   void after_sim() override;
+  std::vector<float>* patch_score_sink() override { return save_all_per_patch_scores() ? &grid_scores_ : nullptr; }
 
  private:
   bool same_xy_ = true;

@@ -335,6 +335,7 @@ DrrLaunch stage(xrh_batch* b, const int32_t* item_view, const double* item_pose,
     s.gx = b->grad_dev + 2 * i * b->cap_npix;
     s.gy = b->grad_dev + (2 * i + 1) * b->cap_npix;
     s.out_index = i;
+    s.patch_scores = nullptr;
     off += (int64_t)rows[i] * cols[i];
   }
   XRH_REQUIRE(drr_out || off <= b->max_items * b->cap_npix, "DRR arena overflow");

@@ -30,6 +30,8 @@ struct FixedDev {
   // use_mask_for_patch_stats with a mask: patch means and deviations over the unmasked pixels
   // U, whose count per patch is pcount (as float); else null
   const float* pcount;
+  const float* pden_unit;    // the unscaled std_p N tables (per-patch scores of any weighting)
+  const float* pdeny_unit;
   // The patch kernels sum 1 - X_p / den_p over every patch of the grid, which with
   // den_p = std_p N / c_p is sum_p (1 - c_p X'_p); adding patch_offset = sum_p c_p - P gives
   // the reference's sum_p c_p s_p, then divided by patch_norm. Index 1: Patch-Grad-NCC's y
@@ -45,6 +47,7 @@ struct SimItem {
   float* gx;                 // scratch for patch-gradient metrics
   float* gy;
   int64_t out_index;
+  float* patch_scores;       // general patch kernel: every grid patch's unweighted score, or null
 };
 
 int sim_record_stride(int metric);
@@ -64,8 +67,9 @@ int sim_group_key(const FixedDev& f);
 void launch_sim(int key, const SimItem* items_dev, int64_t n_items, int max_img_tiles,
                 int max_patch_tiles, double* partials, float* sim_out_dev, bool fused_ncc_done,
                 int patch_lds_bytes, hipStream_t s);
-// dynamic LDS bytes the patch kernel needs for a fixed image's patch geometry
-int sim_patch_lds_bytes(const FixedDev& f);
+// dynamic LDS bytes the patch kernel needs for a fixed image's patch geometry (general: the
+// general kernel's, whatever the group key selects)
+int sim_patch_lds_bytes(const FixedDev& f, bool general = false);
 
 }  // namespace xrh
 

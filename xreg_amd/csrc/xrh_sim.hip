@@ -432,7 +432,19 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
       const double sd = fmax((double)1.0e-6f, sqrt(var));
       const double cross = (Sf - (double)pmu[p] * SU) - mean * pg[p];
       // a patch with no unmasked pixel correlates over nothing: its score is exactly 1
-      acc[dir] = (mstats && fx.pcount[p] == 0.0f) ? 1.0 : 1.0 - (cross / sd) / (double)pden[p];
+      const bool empty = mstats && fx.pcount[p] == 0.0f;
+      acc[dir] = empty ? 1.0 : 1.0 - (cross / sd) / (double)pden[p];
+      if (it.patch_scores) {
+        // the unweighted patch score (ImgSimMetric2DPatchNCCCPU's sim_vals_for_each_patch_, :247-250);
+        // Patch-Grad-NCC keeps 0.5 (x + y) of the float scores (xregImgSimMetric2DPatchGradNCCCPU.cpp:226-240)
+        const float* unit = (dir == 0) ? fx.pden_unit : fx.pdeny_unit;
+        const float sp = empty ? 1.0f : (float)(1.0 - (cross / sd) / (double)unit[p]);
+        if (dir == 0) {
+          it.patch_scores[p] = sp;
+        } else {
+          it.patch_scores[p] = (float)(0.5 * (double)(it.patch_scores[p] + sp));
+        }
+      }
     }
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + tile) * 2);
@@ -839,11 +851,11 @@ int sim_group_key(const FixedDev& f)
     default: fail(XRH_ERR_INVALID, "blur radius out of range");                       \
   }
 
-int sim_patch_lds_bytes(const FixedDev& f)
+int sim_patch_lds_bytes(const FixedDev& f, bool general)
 {
   if (f.metric != XRH_SIM_PATCH_NCC && f.metric != XRH_SIM_PATCH_GRAD_NCC) return 0;
   const int W = patch_lds_edge(f.patch_stride, f.patch_radius);
-  if ((sim_group_key(f) >> kSimKeyShift) & ((1 << (kSimKeyBlurShift - kSimKeyShift)) - 1)) {
+  if (!general && ((sim_group_key(f) >> kSimKeyShift) & ((1 << (kSimKeyBlurShift - kSimKeyShift)) - 1))) {
     // patch_kernel_d: float window pair + three double row-sum arrays
     return 2 * W * W * (int)sizeof(float) + 3 * W * kT * (int)sizeof(double);
   }
@@ -1162,6 +1174,7 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
       h.pden = den;
       h.pg = g;
       f->base_den_dev[0] = den;
+      h.pden_unit = den;
       if (grad) {
         h.pfy = gy;
         patch_prepare(f, gy, &mu, &den, &g, count);
@@ -1169,6 +1182,7 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
         h.pdeny = den;
         h.pgy = g;
         f->base_den_dev[1] = den;
+        h.pdeny_unit = den;
       }
       for (int k = 0; k < (grad ? 2 : 1); ++k) {
         f->base_den[k].resize((size_t)h.num_patches);
@@ -1203,7 +1217,7 @@ static void fixed_create(xrh_ctx* ctx, const float* fixed_host, int32_t rows, in
 }
 
 static void sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev, int64_t num_mov,
-                        float* sim_dev_out, float* sim_host_out)
+                        float* sim_dev_out, float* sim_host_out, float* scores_host_out = nullptr)
 {
   XRH_REQUIRE(ctx && fixed && mov_dev, "null argument");
   XRH_REQUIRE(fixed->ctx == ctx, "fixed image belongs to another context");
@@ -1222,7 +1236,8 @@ static void sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_d
   const uint64_t off_part = align(sizeof(SimItem) * num_mov);
   const uint64_t off_sim = align(off_part + sizeof(double) * rec * ptiles * num_mov);
   const uint64_t off_grad = align(off_sim + sizeof(float) * num_mov);
-  const uint64_t total = align(off_grad + (need_grad ? sizeof(float) * 2 * npix * num_mov : 0));
+  const uint64_t off_scores = align(off_grad + (need_grad ? sizeof(float) * 2 * npix * num_mov : 0));
+  const uint64_t total = align(off_scores + (scores_host_out ? sizeof(float) * h.num_patches * num_mov : 0));
   XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   char* dev = (char*)ctx_scratch(ctx, total);
   SimItem* hi = (SimItem*)ctx_pinned(ctx, sizeof(SimItem) * num_mov);
@@ -1232,15 +1247,26 @@ static void sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_d
     hi[i].gx = need_grad ? (float*)(dev + off_grad) + 2 * i * npix : nullptr;
     hi[i].gy = need_grad ? (float*)(dev + off_grad) + (2 * i + 1) * npix : nullptr;
     hi[i].out_index = i;
+    hi[i].patch_scores = scores_host_out ? (float*)(dev + off_scores) + i * h.num_patches : nullptr;
   }
   XRH_HIP_CHECK(hipMemcpyAsync(dev + off_items, hi, sizeof(SimItem) * num_mov, hipMemcpyHostToDevice, ctx->stream));
   float* sims = sim_dev_out ? sim_dev_out : (float*)(dev + off_sim);
-  launch_sim(sim_group_key(h), (const SimItem*)(dev + off_items), num_mov, img_tiles, ptiles, (double*)(dev + off_part),
-             sims, false, sim_patch_lds_bytes(h), ctx->stream);
+  int key = sim_group_key(h);
+  if (scores_host_out) {
+    // the general patch kernel writes the per-patch scores: clear the group key's diameter
+    const int dmask = ((1 << (kSimKeyBlurShift - kSimKeyShift)) - 1) << kSimKeyShift;
+    key &= ~dmask;
+  }
+  launch_sim(key, (const SimItem*)(dev + off_items), num_mov, img_tiles, ptiles, (double*)(dev + off_part), sims,
+             false, sim_patch_lds_bytes(h, scores_host_out != nullptr), ctx->stream);
+  if (scores_host_out) {
+    XRH_HIP_CHECK(hipMemcpyAsync(scores_host_out, dev + off_scores, sizeof(float) * h.num_patches * num_mov,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  }
   if (sim_host_out) {
     XRH_HIP_CHECK(hipMemcpyAsync(sim_host_out, sims, sizeof(float) * num_mov, hipMemcpyDeviceToHost, ctx->stream));
-    XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   }
+  if (sim_host_out || scores_host_out) XRH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace xrh
@@ -1322,6 +1348,17 @@ xrh_status xrh_sim_compute(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mo
                            float* sim_dev_out, float* sim_host_out)
 {
   return guarded([&] { sim_compute(ctx, fixed, mov_dev, num_mov, sim_dev_out, sim_host_out); });
+}
+
+xrh_status xrh_sim_patch_scores(xrh_ctx* ctx, const xrh_fixed* fixed, const float* mov_dev, int64_t num_mov,
+                                float* sim_host_out, float* scores_host_out)
+{
+  return guarded([&] {
+    XRH_REQUIRE(fixed && sim_host_out && scores_host_out, "null argument");
+    const FixedDev& h = fixed->host;
+    XRH_REQUIRE(h.metric == XRH_SIM_PATCH_NCC || h.metric == XRH_SIM_PATCH_GRAD_NCC, "not a patch metric");
+    sim_compute(ctx, fixed, mov_dev, num_mov, nullptr, sim_host_out, scores_host_out);
+  });
 }
 
 }  // extern "C"
