@@ -419,9 +419,11 @@ class ImgSimMetric2DGradNCCHIP : public ImgSimMetric2DHIP, public ImgSimMetric2D
 // combined (PatchNCCCPU::compute, xregImgSimMetric2DPatchNCCCPU.cpp:186-287). This is host
 // bookkeeping: the device scores every patch of the grid and applies the weights and the
 // patches used through xrh_fixed_set_patch_weights. The weights are recomputed whenever an
-// input to them (weight image, mask, options, grid) changes. Not implemented, and refused with
+// input to them (weight image, mask, options, grid) changes. Refused with
 // UnsupportedOperationException: moving-image patch variances as weights
-// (use_mov_img_patch_variances_as_wgts).
+// (use_mov_img_patch_variances_as_wgts), whose reference weights each patch with the variance
+// found at its offset within the parallel chunk that scores it
+// (xregImgSimMetric2DPatchNCCCPU.cpp:197, 208-210), so the result depends on the partition.
 class ImgSimMetric2DPatchCommonHIP {
  public:
   using WgtImgPtr = ImagePtr;
