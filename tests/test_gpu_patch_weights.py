@@ -305,3 +305,41 @@ def test_per_patch_scores_match_oracle(ctx, oracle, images, metric, radius, stri
     assert np.array_equal(again, scores)
     f.close()
     buf.free()
+
+
+@pytest.mark.parametrize("shape,radius,stride", [((11, 11), 5, 1), ((11, 40), 5, 1), ((9, 9), 4, 3), ((23, 12), 3, 2)])
+@pytest.mark.parametrize("metric", [2, 3])
+def test_small_images_and_single_patches(ctx, oracle, metric, shape, radius, stride):
+    """Images holding one patch, one row of patches, or a few strided patches: the specialised
+    and the general kernel, unit and explicit weights, masked statistics, per-patch scores."""
+    from xreg_amd import _xrh as X
+
+    rng = np.random.default_rng(shape[0] * 100 + shape[1])
+    fixed = rng.random(shape).astype(np.float32)
+    mov = (0.5 * fixed[None] + 0.5 * rng.random((3, *shape))).astype(np.float32)
+    mask = (rng.random(shape) > 0.25).astype(np.uint8)
+    print(f"case: small image {shape} metric={metric} r={radius} s={stride}", flush=True)
+    buf = ctx.alloc(mov.nbytes)
+    buf.upload(mov)
+    for mk, ms in ((None, False), (mask, False), (mask, True)):
+        f = X.Fixed(ctx, fixed, metric, smooth_ksize=3, patch_radius=radius, patch_stride=stride, mask=mk,
+                    patch_mask_stats=ms)
+        n_p = f.patches()["weights"].size
+        w = oracle.patch_weights_f32(shape, radius, stride, mask=mk)
+        for st in (dict(), dict(weights=np.linspace(0.5, 1.5, n_p).astype(np.float32), compute_mean=True)):
+            f.set_patch_weights(**st)
+            got = f.compute(buf, 3)
+            ref = oracle.patch_sim_weighted_f32(metric, fixed, mov, weights=st.get("weights", w),
+                                                compute_mean=st.get("compute_mean", False), smooth_ksize=3,
+                                                radius=radius, stride=stride, mask=mk, mask_stats=ms)
+            ref64 = oracle.patch_sim_weighted_f64(metric, fixed.astype(np.float64), mov.astype(np.float64),
+                                                  weights=st.get("weights", w),
+                                                  compute_mean=st.get("compute_mean", False), smooth_ksize=3,
+                                                  radius=radius, stride=stride, mask=mk, mask_stats=ms)
+            assert np.all(np.abs(got - ref) <= _tol(ref) + np.abs(ref - ref64)), (mk is not None, ms, sorted(st))
+        sims, scores = f.patch_scores(buf, 3)
+        sref = oracle.patch_scores_f32(metric, fixed, mov, smooth_ksize=3, radius=radius, stride=stride, mask=mk,
+                                       mask_stats=ms)
+        assert np.abs(scores - sref).max() <= 2e-5
+        f.close()
+    buf.free()
