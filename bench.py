@@ -185,13 +185,16 @@ def profiled(key: str, folder: Path | None = None) -> dict:
     """Per-launch counter figures of the DRR kernel measured by rocprofv3 for this exact
     configuration and kernel source, from the committed profile summaries (lines `counters: <key>
     name=value ...`): valu_insts (SQ_INSTS_VALU), vmem_insts (SQ_INSTS_VMEM_RD), traffic_bytes
-    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy). Empty when not profiled."""
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy), ta_busy_frac (TA_TA_BUSY per CU
+    over GRBM_GUI_ACTIVE per XCD: the share of the kernel's cycles in which the texture address
+    unit, which takes every active lane's fetch address, is busy). Empty when not profiled."""
     out = {}
     for p in sorted((folder or ROOT / "profiles").glob("*.md")):
         for line in p.read_text().splitlines():
             if line.startswith("counters: ") and f" {key} " in line + " ":
                 for tok in line.split():
-                    if "=" in tok and tok.split("=", 1)[0] in ("valu_insts", "vmem_insts", "traffic_bytes"):
+                    if "=" in tok and tok.split("=", 1)[0] in ("valu_insts", "vmem_insts", "traffic_bytes",
+                                                                 "ta_busy_frac"):
                         k, v = tok.split("=", 1)
                         out[k] = float(v)
     return out
@@ -647,6 +650,7 @@ def main():
                 "hbm_peak_gbs": HBM_PEAK_GBS, "profile_key": key,
                 "kernel": "drr_kernel", "kernel_ms": round(drr_ms_avg, 4), "cus": cus,
                 "valu_insts_per_launch": prof.get("valu_insts"), "vmem_insts_per_launch": prof.get("vmem_insts"),
+                "fetch_unit_busy_frac": prof.get("ta_busy_frac"),
                 "samples_per_launch": int(S), "samples_evaluated_per_launch": int(S_eval),
                 "valu_lane_ops_per_evaluated_sample": (round(64.0 * prof["valu_insts"] / S_eval, 2)
                                                        if "valu_insts" in prof else None),

@@ -25,10 +25,10 @@ def test_profile_lookup_is_keyed_by_kernel_source(tmp_path):
     key = f"workload=cmaes-lambda100-384sq-patch-grad-ncc storage=f32 geometry=fp64 skip=1 smooth=5 src={src}"
     (tmp_path / "p.md").write_text(
         "# summary\n"
-        f"counters: {key} valu_insts=2.5e9 vmem_insts=1.7e8 traffic_bytes=8.1e9\n"
+        f"counters: {key} valu_insts=2.5e9 vmem_insts=1.7e8 traffic_bytes=8.1e9 ta_busy_frac=0.86\n"
         f"counters: {key.replace(src, '0' * 12)} valu_insts=9e9\n")
     got = bench.profiled(key, tmp_path)
-    assert got == {"valu_insts": 2.5e9, "vmem_insts": 1.7e8, "traffic_bytes": 8.1e9}
+    assert got == {"valu_insts": 2.5e9, "vmem_insts": 1.7e8, "traffic_bytes": 8.1e9, "ta_busy_frac": 0.86}
     # a figure measured on other kernel code, or another configuration, is not reported
     assert bench.profiled(key.replace(src, "f" * 12), tmp_path) == {}
     assert bench.profiled(key.replace("384sq", "768sq"), tmp_path) == {}
