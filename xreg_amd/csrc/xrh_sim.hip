@@ -453,26 +453,30 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
 // patch_kernel for the common case -- no mask, stride 1, a compile-time patch diameter kD.
 // Same per-patch closed form; the box sums are formed with sliding windows so that LDS
 // traffic drops about threefold:
-//  * the first direction's window and every direction's per-patch fixed statistics are read
-//    before the first one is waited for; the second direction's window is read while the
-//    first one is being summed;
+//  * each direction's window and per-patch fixed statistics are read before the first one is
+//    waited for; the second direction's window is read while the first one is being summed;
 //  * the window is staged in LDS as float (the values are floats; products are formed in
 //    double);
 //  * horizontal pass: 4 lanes per window row, each forming 4 consecutive d-wide sums -- the
 //    first directly, the next three by adding the entering and removing the leaving column;
 //  * vertical pass: 64 lanes, one per patch column and group of 4 patch rows, slide the same
-//    way down the column and evaluate the closed form for their 4 patches.
+//    way down the column and evaluate the closed form for their 4 patches. The row sums are
+//    kept at an odd pitch (17 doubles) so that the horizontal pass's stores spread over the
+//    LDS banks (2-way instead of 8-way conflicts at a pitch of 16).
+// Occupancy: the two-direction form is held at 5 waves per SIMD (91 VGPRs; forcing 6 spills
+// registers and measured slower), the one-direction form at 8 (64 VGPRs).
 // The sums are FP64 throughout; the sliding updates differ from direct sums by a few units in
 // the last place of a double.
 template <bool kTwo, int kD>
-__global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __restrict__ items,
-                                                           double* __restrict__ partials, int ptiles)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kTwo ? 5 : 8)))
+void patch_kernel_d(const SimItem* __restrict__ items, double* __restrict__ partials, int ptiles)
 {
   constexpr int kW = (kT - 1) + kD;  // window edge at stride 1
   constexpr int kRows = (kW + 7) / 8;
   constexpr int kDirs = kTwo ? 2 : 1;
   constexpr int kSeg = 4;            // sums per lane in the sliding passes
   constexpr int kSegs = kT / kSeg;   // lanes per window row / per patch column
+  constexpr int kHP = kT + 1;        // row pitch of the double row sums (odd: spreads the banks)
   static_assert(kW <= 32, "one 32-lane column pass per window row");
   static_assert(kW * kSegs <= kThreads && kT * kSegs <= kThreads, "sliding lanes fit the block");
   extern __shared__ double patch_lds[];
@@ -486,8 +490,8 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
   float* lm = (float*)patch_lds;
   float* lf = lm + kW * kW;
   double* h1 = (double*)(lf + kW * kW);  // 2 kW^2 floats: a multiple of 8 bytes
-  double* h2 = h1 + kW * kT;
-  double* hf = h2 + kW * kT;
+  double* h2 = h1 + kW * kHP;
+  double* hf = h2 + kW * kHP;
   constexpr double kN = (double)(kD * kD);
   constexpr double kInvN = 1.0 / kN, kInvN1 = 1.0 / (kN - 1.0);
   const int lx = threadIdx.x & 31, ly0 = threadIdx.x >> 5;
@@ -515,28 +519,30 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
     }
   };
   load_window(0);
-  float mu[kDirs][kSeg], den[kDirs][kSeg];
-  double g[kDirs][kSeg];
+  float mu[kSeg], den[kSeg];
+  double g[kSeg];
   bool pv[kSeg];
 #pragma unroll
   for (int k = 0; k < kSeg; ++k) pv[k] = vlane && (X0 + vc < ppr) && (Y0 + vs * kSeg + k < ppc);
-#pragma unroll
-  for (int dir = 0; dir < kDirs; ++dir) {
+  // one direction's per-patch fixed statistics, read when that direction starts (its window
+  // staging and horizontal pass cover the reads)
+  auto load_stats = [&](int dir) {
     const auto rmu = ro_rsrc((dir == 0) ? fx->pmean : fx->pmeany, np * 4);
     const auto rden = ro_rsrc((dir == 0) ? fx->pden : fx->pdeny, np * 4);
     const auto rg = ro_rsrc((dir == 0) ? fx->pg : fx->pgy, np * 8);
 #pragma unroll
     for (int k = 0; k < kSeg; ++k) {
       const int p = pv[k] ? (Y0 + vs * kSeg + k) * ppr + X0 + vc : 0;  // invalid: patch 0, discarded
-      mu[dir][k] = ld_f32(rmu, p * 4);
-      den[dir][k] = ld_f32(rden, p * 4);
-      g[dir][k] = ld_f64(rg, p * 8);
+      mu[k] = ld_f32(rmu, p * 4);
+      den[k] = ld_f32(rden, p * 4);
+      g[k] = ld_f64(rg, p * 8);
     }
-  }
+  };
 
   double acc[2] = {0.0, 0.0};
 #pragma unroll
   for (int dir = 0; dir < kDirs; ++dir) {
+    load_stats(dir);
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
@@ -552,9 +558,9 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
       const int row = threadIdx.x / kSegs, c0 = (threadIdx.x % kSegs) * kSeg;
       const float* m = lm + row * kW + c0;
       const float* f = lf + row * kW + c0;
-      double* o1 = h1 + row * kT + c0;
-      double* o2 = h2 + row * kT + c0;
-      double* of = hf + row * kT + c0;
+      double* o1 = h1 + row * kHP + c0;
+      double* o2 = h2 + row * kHP + c0;
+      double* of = hf + row * kHP + c0;
       double a1 = 0.0, a2 = 0.0, af = 0.0;
 #pragma unroll
       for (int xx = 0; xx < kD; ++xx) {
@@ -579,18 +585,18 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
     }
     __syncthreads();
     if (vlane) {
-      const int e0 = vs * kSeg * kT + vc;
+      const int e0 = vs * kSeg * kHP + vc;
       double S1 = 0.0, S2 = 0.0, Sf = 0.0;
 #pragma unroll
       for (int yy = 0; yy < kD; ++yy) {
-        S1 += h1[e0 + yy * kT];
-        S2 += h2[e0 + yy * kT];
-        Sf += hf[e0 + yy * kT];
+        S1 += h1[e0 + yy * kHP];
+        S2 += h2[e0 + yy * kHP];
+        Sf += hf[e0 + yy * kHP];
       }
 #pragma unroll
       for (int k = 0; k < kSeg; ++k) {
         if (k > 0) {
-          const int eo = e0 + (k - 1) * kT, en = e0 + (k - 1 + kD) * kT;
+          const int eo = e0 + (k - 1) * kHP, en = e0 + (k - 1 + kD) * kHP;
           S1 = (S1 - h1[eo]) + h1[en];
           S2 = (S2 - h2[eo]) + h2[en];
           Sf = (Sf - hf[eo]) + hf[en];
@@ -599,8 +605,8 @@ __global__ __launch_bounds__(kThreads) void patch_kernel_d(const SimItem* __rest
           const double mean = S1 * kInvN;
           const double var = fmax(S2 - mean * S1, 0.0) * kInvN1;
           const double sd = fmax((double)1.0e-6f, sqrt(var));
-          const double cross = (Sf - (double)mu[dir][k] * S1) - mean * g[dir][k];
-          acc[dir] += 1.0 - cross / (sd * (double)den[dir][k]);
+          const double cross = (Sf - (double)mu[k] * S1) - mean * g[k];
+          acc[dir] += 1.0 - cross / (sd * (double)den[k]);
         }
       }
     }
@@ -857,7 +863,7 @@ int sim_patch_lds_bytes(const FixedDev& f, bool general)
   const int W = patch_lds_edge(f.patch_stride, f.patch_radius);
   if (!general && ((sim_group_key(f) >> kSimKeyShift) & ((1 << (kSimKeyBlurShift - kSimKeyShift)) - 1))) {
     // patch_kernel_d: float window pair + three double row-sum arrays
-    return 2 * W * W * (int)sizeof(float) + 3 * W * kT * (int)sizeof(double);
+    return 2 * W * W * (int)sizeof(float) + 3 * W * (kT + 1) * (int)sizeof(double);
   }
   return (f.mask ? (3 * W * W + (f.pcount ? 5 : 4) * W * kT) : (2 * W * W + 3 * W * kT)) * (int)sizeof(double);
 }
