@@ -454,7 +454,8 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
 // Same per-patch closed form; the box sums are formed with sliding windows so that LDS
 // traffic drops about threefold:
 //  * each direction's window and per-patch fixed statistics are read before the first one is
-//    waited for; the second direction's window is read while the first one is being summed;
+//    waited for; the second direction's window is read once the first direction's sums are
+//    done, so that its registers are free during them;
 //  * the window is staged in LDS as float (the values are floats; products are formed in
 //    double);
 //  * horizontal pass: 4 lanes per window row, each forming 4 consecutive d-wide sums -- the
@@ -463,12 +464,13 @@ __global__ __launch_bounds__(kThreads) void patch_kernel(const SimItem* __restri
 //    way down the column and evaluate the closed form for their 4 patches. The row sums are
 //    kept at an odd pitch (17 doubles) so that the horizontal pass's stores spread over the
 //    LDS banks (2-way instead of 8-way conflicts at a pitch of 16).
-// Occupancy: the two-direction form is held at 5 waves per SIMD (91 VGPRs; forcing 6 spills
-// registers and measured slower), the one-direction form at 8 (64 VGPRs).
+// Occupancy: the two-direction form runs at 6 waves per SIMD (78 VGPRs; with the second window
+// read during the first direction's sums it needed 91, 5 waves, and measured 3% slower), the
+// one-direction form at 8 (64 VGPRs).
 // The sums are FP64 throughout; the sliding updates differ from direct sums by a few units in
 // the last place of a double.
 template <bool kTwo, int kD>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kTwo ? 5 : 8)))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kTwo ? 6 : 8)))
 void patch_kernel_d(const SimItem* __restrict__ items, double* __restrict__ partials, int ptiles)
 {
   constexpr int kW = (kT - 1) + kD;  // window edge at stride 1
@@ -504,18 +506,17 @@ void patch_kernel_d(const SimItem* __restrict__ items, double* __restrict__ part
   const bool vlane = threadIdx.x < kT * kSegs;
   const int vc = threadIdx.x & (kT - 1), vs = (threadIdx.x >> 4) & (kSegs - 1);
 
-  // the window rows this lane stages: byte offsets are shared by both directions
-  int wo[kRows];
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) wo[r] = (min(Y0 + min(ly0 + 8 * r, kW - 1), rows - 1) * cols + xo) * 4;
+  // the byte offset of window row r staged by this lane (formed at each read)
+  auto wo = [&](int r) { return (min(Y0 + min(ly0 + 8 * r, kW - 1), rows - 1) * cols + xo) * 4; };
   float vm[kRows], vf[kRows];
   auto load_window = [&](int dir) {
     const auto rm = ro_rsrc((dir == 0) ? (kTwo ? it.gx : it.mov) : it.gy, img_bytes);
     const auto rf = ro_rsrc((dir == 0) ? fx->pf : fx->pfy, img_bytes);
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      vm[r] = ld_f32(rm, wo[r]);
-      vf[r] = ld_f32(rf, wo[r]);
+      const int o = wo(r);
+      vm[r] = ld_f32(rm, o);
+      vf[r] = ld_f32(rf, o);
     }
   };
   load_window(0);
@@ -553,7 +554,6 @@ void patch_kernel_d(const SimItem* __restrict__ items, double* __restrict__ part
       }
     }
     __syncthreads();
-    if (dir + 1 < kDirs) load_window(dir + 1);
     if (threadIdx.x < kW * kSegs) {
       const int row = threadIdx.x / kSegs, c0 = (threadIdx.x % kSegs) * kSeg;
       const float* m = lm + row * kW + c0;
@@ -610,6 +610,7 @@ void patch_kernel_d(const SimItem* __restrict__ items, double* __restrict__ part
         }
       }
     }
+    if (dir + 1 < kDirs) load_window(dir + 1);
   }
   block_reduce_store<2>(acc, partials + (it.out_index * ptiles + tile) * 2);
 }
